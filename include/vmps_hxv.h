@@ -1,0 +1,168 @@
+/*
+ * vmps_hxv.h -- C ABI of the MI355X (gfx950) effective-Hamiltonian engine.
+ *
+ * The reference (TNWorkers/VMPS) binds its Krylov solvers to the pivot
+ * operators by C++ overload resolution, not by an FFI: LanczosSolver calls
+ * the free functions HxV / dot / norm on PivotMatrix2 / PivotVector
+ * (pivot/DmrgPivotMatrix2.h:78-83, pivot/DmrgPivotVector.h:121-424).  This
+ * header is the flat boundary those calls are routed through: plain pointers,
+ * sizes and status codes, no C++ or GPU types.  Every entry point below names
+ * the reference interface it stands in for.
+ *
+ * Conventions
+ *  - Quantum numbers are int arrays of length nq (1 or 2).  A component of kind
+ *    VMPS_SYM_U1 is an additive charge, one of kind VMPS_SYM_SU2 is q = 2S+1
+ *    (symmetry/U1.h, symmetry/SU2.h).
+ *  - Dense blocks are column-major FP64, exactly Eigen's default storage, so a
+ *    caller passes block.data() without copies.
+ *  - A PivotVector is one packed FP64 array holding its blocks in the order they
+ *    were declared with vmps_pivot2_add_V_block (component s, then block q).
+ *  - Calls return VMPS_OK or an error status; vmps_last_error() gives the
+ *    message of the calling thread's last failure.  Nothing throws across the
+ *    boundary.
+ *  - Objects are independent: distinct plans may be used from distinct threads
+ *    concurrently (one HIP stream per plan); a single plan is not reentrant.
+ */
+#ifndef VMPS_HXV_H
+#define VMPS_HXV_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VMPS_ABI_VERSION 1
+
+typedef int vmps_status;
+#define VMPS_OK 0
+#define VMPS_ERR_ARG 1        /* invalid argument */
+#define VMPS_ERR_LAYOUT 2     /* block layout inconsistent with the pivot */
+#define VMPS_ERR_OOM 3        /* device allocation failed */
+#define VMPS_ERR_HIP 4        /* HIP runtime error */
+#define VMPS_ERR_NONFINITE 5  /* NaN/Inf found in an input vector */
+#define VMPS_ERR_STATE 6      /* call out of order (e.g. no block structure yet) */
+#define VMPS_ERR_INTERNAL 7
+
+#define VMPS_SYM_U1 1
+#define VMPS_SYM_SU2 2
+
+typedef enum {
+  VMPS_PREC_FP64 = 0,   /* plain FP64 drop-in (FP64 MFMA) */
+  VMPS_PREC_FP32 = 1,   /* FP32 MFMA, FP32 intermediates, FP64 vectors */
+  VMPS_PREC_BF16X3 = 2  /* split-BF16 x3 MFMA with FP32 accumulation, FP64 vectors */
+} vmps_precision;
+
+typedef struct vmps_pivot2 vmps_pivot2; /* host image of PivotMatrix2 + Vin layout */
+typedef struct vmps_plan vmps_plan;     /* device-resident staged block-triple plan */
+
+int vmps_abi_version(void);
+const char* vmps_last_error(void);
+
+/* ---- pivot description (PivotMatrix2, pivot/DmrgPivotMatrix2.h:15-76) ---- */
+
+/* PivotMatrix2(L, R, W12, W34, qloc12, qloc34, qOp12, qOp34)
+ * (DmrgPivotMatrix2.h:34-52): local bases and operator bases of both sites.
+ * kinds[nq] = VMPS_SYM_U1 / VMPS_SYM_SU2 per component. */
+vmps_status vmps_pivot2_create(int nq, const int* kinds, int nloc12, const int* qloc12, int nloc34,
+                               const int* qloc34, int nop12, const int* qop12, int nop34, const int* qop34,
+                               vmps_pivot2** out);
+void vmps_pivot2_destroy(vmps_pivot2* p);
+
+/* Combined two-site basis loc12.combine(loc34) (tensors/Qbasis.h:686-798), i.e.
+ * the index s of PivotVector::data[s] (DmrgPivotVector.h:33-48).  ns receives
+ * its size; qloc (ns*nq ints) may be NULL. */
+vmps_status vmps_pivot2_local_basis(const vmps_pivot2* p, int* ns, int* qloc);
+
+/* One sector of the environment Tripod L (which=0) or R (which=1):
+ * Tripod::push_back({in,out,mid}, block[a][0] for a < adim)
+ * (tensors/Multipede.h:77-95,152-154).  rows/cols/data describe the adim
+ * blocks packed back to back; a block with rows*cols == 0 is absent
+ * (DmrgPivotMatrix2.h:207-208 skip such blocks). */
+vmps_status vmps_pivot2_add_env(vmps_pivot2* p, int which, const int* qin, const int* qout, const int* qmid,
+                                int adim, const int64_t* rows, const int64_t* cols, const double* data);
+
+/* One sparse block of W12[s1][s2][k] (which=12) or W34[s3][s4][k] (which=34):
+ * Biped<Sym,SparseMatrix<double,ColMajor>>::push_back(qin, qout, mat)
+ * (MpoTerms.h:1175-1233).  nnz triplets (row, col, val) in any order. */
+vmps_status vmps_pivot2_add_W(vmps_pivot2* p, int which, int s1, int s2, int k, const int* qin, const int* qout,
+                              int rows, int cols, int64_t nnz, const int* row, const int* col, const double* val);
+
+/* Declare block (qin, qout) of PivotVector::data[s] (Biped::push_back,
+ * tensors/Biped.h:82-104; extend_A materialises every allowed block,
+ * DmrgContractions.h:1832-1867).  *offset receives its element offset in the
+ * packed vector. */
+vmps_status vmps_pivot2_add_V_block(vmps_pivot2* p, int s, const int* qin, const int* qout, int64_t rows,
+                                    int64_t cols, int64_t* offset);
+vmps_status vmps_pivot2_vsize(const vmps_pivot2* p, int64_t* size);
+
+/* precalc_blockStructure(L, Vin, W12, W34, Vin, R, qloc12, qloc34, qOp12,
+ * qOp34, qlhs, qrhs, factor_cgcs) (tensors/DmrgIndexGymnastics.h:698-856):
+ * fills the pivot's block structure, including the SU(2) 6j/9j factors. */
+vmps_status vmps_pivot2_precalc(vmps_pivot2* p);
+
+/* Use the caller's own PivotMatrix2::qlhs / qrhs / factor_cgcs
+ * (DmrgPivotMatrix2.h:66-70) instead of vmps_pivot2_precalc.
+ * qlhs[2*i] = {s1s3, q13}; entries rhs_begin[i] .. rhs_begin[i+1]-1 of
+ * qrhs (12 values each: s2s4,q24,qL,qR,s1,s2,k12,qW12,s3,s4,k34,qW34) and of
+ * factors belong to lhs entry i. */
+vmps_status vmps_pivot2_set_blockstructure(vmps_pivot2* p, int64_t nlhs, const int64_t* qlhs,
+                                           const int64_t* rhs_begin, const int64_t* qrhs, const double* factors);
+/* Read the block structure back.  Call with NULL arrays to get nlhs / nrhs. */
+vmps_status vmps_pivot2_get_blockstructure(const vmps_pivot2* p, int64_t* nlhs, int64_t* nrhs, int64_t* qlhs,
+                                           int64_t* rhs_begin, int64_t* qrhs, double* factors);
+
+/* Excited-state penalty: PivotMatrix2::A0proj / Epenalty
+ * (DmrgPivotMatrix2.h:72-75, applied at :249-272).  A0 holds n packed vectors. */
+vmps_status vmps_pivot2_set_projectors(vmps_pivot2* p, double Epenalty, int n, const double* A0);
+
+/* ---- device plan and HxV ---------------------------------------------- */
+
+typedef struct {
+  int64_t n_terms;  /* contributing (out, L, Vin, R) block quadruples */
+  int64_t n_T;      /* distinct (L block, Vin block) products */
+  int64_t n_U;      /* distinct (out block, R block) sums */
+  int64_t n_out;    /* output blocks */
+  int64_t n_L, n_R, n_V;
+  double flops_alg; /* sum over distinct (L,V) of 2 m'mn + over distinct (out,R) of 2 m'nn' */
+  double flops_ref; /* per-term (L.V).R work of the reference loop */
+  double bytes_alg_fp64;
+  int64_t L_elems, R_elems, V_elems, T_elems, U_elems;
+  int64_t max_block_dim;
+  double flops_A;   /* stage A share of flops_alg: T = L . Vin */
+  double flops_C;   /* stage C share of flops_alg: Vout += U . R */
+} vmps_plan_stats;
+
+/* Flatten the pivot into the staged block-triple tables and upload L, R and
+ * the tables to the current HIP device.  The pivot may be destroyed after. */
+vmps_status vmps_plan_create(const vmps_pivot2* p, vmps_precision prec, vmps_plan** out);
+void vmps_plan_destroy(vmps_plan* plan);
+vmps_status vmps_plan_stats_get(const vmps_plan* plan, vmps_plan_stats* st);
+vmps_status vmps_plan_set_precision(vmps_plan* plan, vmps_precision prec);
+
+/* HxV(H, Vin, Vout) (DmrgPivotMatrix2.h:78-83 -> OxV :85-273).
+ * Host pointers, synchronous.  Vout has Vin's layout. */
+vmps_status vmps_hxv(vmps_plan* plan, const double* vin, double* vout);
+/* Same on device-resident vectors, enqueued on `stream` (hipStream_t; NULL =
+ * the plan's own stream).  Does not synchronise. */
+vmps_status vmps_hxv_device(vmps_plan* plan, const double* d_vin, double* d_vout, void* stream);
+
+/* dot(V1, V2) = sum_s sum_q tr(V1^T V2) * coeff_dot(out[q])
+ * (DmrgPivotVector.h:296-339).  Host pointers. */
+vmps_status vmps_dot(vmps_plan* plan, const double* x, const double* y, double* result);
+
+/* Timing helper: mean device time of one HxV on the plan's stream, measured
+ * with HIP events around `iters` back-to-back calls on resident vectors. */
+vmps_status vmps_plan_time_hxv(vmps_plan* plan, int warmup, int iters, double* ms_per_hxv);
+
+/* Per-stage device time (ms per call, HIP events on the plan's stream, mean
+ * over `iters` launches of each stage alone on resident data):
+ * ms[0] stage A (T = L.V), ms[1] stage B (U = sum c T), ms[2] stage C
+ * (Vout = sum U.R), ms[3] the whole HxV. */
+vmps_status vmps_plan_time_stages(vmps_plan* plan, int warmup, int iters, double* ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VMPS_HXV_H */

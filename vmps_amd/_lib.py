@@ -1,0 +1,98 @@
+"""ctypes binding of libvmps_hxv.so (the C ABI declared in include/vmps_hxv.h).
+
+The library is loaded from the package directory only.  A missing library is
+an error: there is no CPU fallback for the device path.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libvmps_hxv.so")
+
+c_int = ctypes.c_int
+c_i64 = ctypes.c_int64
+c_dbl = ctypes.c_double
+P_int = ctypes.POINTER(ctypes.c_int)
+P_i64 = ctypes.POINTER(ctypes.c_int64)
+P_dbl = ctypes.POINTER(ctypes.c_double)
+c_void = ctypes.c_void_p
+
+VMPS_OK = 0
+STATUS_NAMES = {
+    0: "VMPS_OK", 1: "VMPS_ERR_ARG", 2: "VMPS_ERR_LAYOUT", 3: "VMPS_ERR_OOM", 4: "VMPS_ERR_HIP",
+    5: "VMPS_ERR_NONFINITE", 6: "VMPS_ERR_STATE", 7: "VMPS_ERR_INTERNAL",
+}
+PREC = {"fp64": 0, "fp32": 1, "bf16x3": 2}
+
+
+class PlanStats(ctypes.Structure):
+    _fields_ = [
+        ("n_terms", c_i64), ("n_T", c_i64), ("n_U", c_i64), ("n_out", c_i64),
+        ("n_L", c_i64), ("n_R", c_i64), ("n_V", c_i64),
+        ("flops_alg", c_dbl), ("flops_ref", c_dbl), ("bytes_alg_fp64", c_dbl),
+        ("L_elems", c_i64), ("R_elems", c_i64), ("V_elems", c_i64), ("T_elems", c_i64), ("U_elems", c_i64),
+        ("max_block_dim", c_i64), ("flops_A", c_dbl), ("flops_C", c_dbl),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class VmpsStatusError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__("%s: %s" % (STATUS_NAMES.get(status, status), msg))
+        self.status = status
+
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "vmps_abi_version": (c_int, []),
+    "vmps_last_error": (ctypes.c_char_p, []),
+    "vmps_pivot2_create": (c_int, [c_int, P_int, c_int, P_int, c_int, P_int, c_int, P_int, c_int, P_int,
+                                   ctypes.POINTER(c_void)]),
+    "vmps_pivot2_destroy": (None, [c_void]),
+    "vmps_pivot2_local_basis": (c_int, [c_void, P_int, P_int]),
+    "vmps_pivot2_add_env": (c_int, [c_void, c_int, P_int, P_int, P_int, c_int, P_i64, P_i64, P_dbl]),
+    "vmps_pivot2_add_W": (c_int, [c_void, c_int, c_int, c_int, c_int, P_int, P_int, c_int, c_int, c_i64,
+                                  P_int, P_int, P_dbl]),
+    "vmps_pivot2_add_V_block": (c_int, [c_void, c_int, P_int, P_int, c_i64, c_i64, P_i64]),
+    "vmps_pivot2_vsize": (c_int, [c_void, P_i64]),
+    "vmps_pivot2_precalc": (c_int, [c_void]),
+    "vmps_pivot2_set_blockstructure": (c_int, [c_void, c_i64, P_i64, P_i64, P_i64, P_dbl]),
+    "vmps_pivot2_get_blockstructure": (c_int, [c_void, P_i64, P_i64, P_i64, P_i64, P_i64, P_dbl]),
+    "vmps_pivot2_set_projectors": (c_int, [c_void, c_dbl, c_int, P_dbl]),
+    "vmps_plan_create": (c_int, [c_void, c_int, ctypes.POINTER(c_void)]),
+    "vmps_plan_destroy": (None, [c_void]),
+    "vmps_plan_stats_get": (c_int, [c_void, ctypes.POINTER(PlanStats)]),
+    "vmps_plan_set_precision": (c_int, [c_void, c_int]),
+    "vmps_hxv": (c_int, [c_void, P_dbl, P_dbl]),
+    "vmps_hxv_device": (c_int, [c_void, c_void, c_void, c_void]),
+    "vmps_dot": (c_int, [c_void, P_dbl, P_dbl, P_dbl]),
+    "vmps_plan_time_hxv": (c_int, [c_void, c_int, c_int, P_dbl]),
+    "vmps_plan_time_stages": (c_int, [c_void, c_int, c_int, P_dbl]),
+}
+
+_lib = None
+
+
+def load():
+    """Load the library (once).  Raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError("libvmps_hxv.so is not built (run `python -m vmps_amd.build`); "
+                           "the HIP engine has no CPU fallback")
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(status):
+    if status != VMPS_OK:
+        msg = load().vmps_last_error()
+        raise VmpsStatusError(status, msg.decode() if msg else "")

@@ -1,0 +1,410 @@
+// extern "C" boundary (include/vmps_hxv.h).  Every entry point catches all
+// exceptions and turns them into a status code plus a thread-local message.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+
+#include "engine.hpp"
+#include "errors.hpp"
+#include "pivot.hpp"
+#include "vmps_hxv.h"
+
+struct vmps_pivot2 {
+  vmps::Pivot2 P;
+  int ns = 0;
+};
+struct vmps_plan {
+  vmps::DevPlan D;
+};
+
+namespace vmps {
+namespace {
+thread_local std::string g_last_error;
+}
+void set_last_error(const std::string& msg) { g_last_error = msg; }
+const char* last_error() { return g_last_error.c_str(); }
+}  // namespace vmps
+
+using vmps::VmpsError;
+
+#define VMPS_API_BEGIN try {
+#define VMPS_API_END                                  \
+  return VMPS_OK;                                     \
+  }                                                   \
+  catch (const VmpsError& e) {                        \
+    vmps::set_last_error(e.what());                   \
+    return e.status;                                  \
+  }                                                   \
+  catch (const std::bad_alloc&) {                     \
+    vmps::set_last_error("host allocation failed");   \
+    return VMPS_ERR_OOM;                              \
+  }                                                   \
+  catch (const std::exception& e) {                   \
+    vmps::set_last_error(e.what());                   \
+    return VMPS_ERR_INTERNAL;                         \
+  }
+
+namespace {
+vmps::Qn read_qn(const int* q, int nq) {
+  vmps::Qn r;
+  for (int i = 0; i < nq; ++i) r[i] = q[i];
+  return r;
+}
+void require(bool cond, int status, const char* msg) {
+  if (!cond) throw VmpsError(status, msg);
+}
+}  // namespace
+
+extern "C" {
+
+int vmps_abi_version(void) { return VMPS_ABI_VERSION; }
+const char* vmps_last_error(void) { return vmps::last_error(); }
+
+vmps_status vmps_pivot2_create(int nq, const int* kinds, int nloc12, const int* qloc12, int nloc34, const int* qloc34,
+                               int nop12, const int* qop12, int nop34, const int* qop34, vmps_pivot2** out) {
+  VMPS_API_BEGIN
+  require(out != nullptr, VMPS_ERR_ARG, "out is NULL");
+  *out = nullptr;
+  require(nq >= 1 && nq <= vmps::kMaxNq, VMPS_ERR_ARG, "nq must be 1 or 2");
+  require(kinds != nullptr, VMPS_ERR_ARG, "kinds is NULL");
+  require(nloc12 > 0 && nloc34 > 0 && nop12 > 0 && nop34 > 0, VMPS_ERR_ARG, "empty local or operator basis");
+  require(qloc12 && qloc34 && qop12 && qop34, VMPS_ERR_ARG, "NULL basis array");
+  auto p = std::make_unique<vmps_pivot2>();
+  p->P.sym.nq = nq;
+  for (int i = 0; i < nq; ++i) {
+    require(kinds[i] == VMPS_SYM_U1 || kinds[i] == VMPS_SYM_SU2, VMPS_ERR_ARG, "unknown symmetry kind");
+    p->P.sym.kind[i] = kinds[i];
+  }
+  for (int i = 0; i < nloc12; ++i) p->P.qloc12.push_back(read_qn(qloc12 + i * nq, nq));
+  for (int i = 0; i < nloc34; ++i) p->P.qloc34.push_back(read_qn(qloc34 + i * nq, nq));
+  for (int i = 0; i < nop12; ++i) p->P.qop12.push_back(read_qn(qop12 + i * nq, nq));
+  for (int i = 0; i < nop34; ++i) p->P.qop34.push_back(read_qn(qop34 + i * nq, nq));
+  for (int i = 0; i < nq; ++i)
+    if (p->P.sym.kind[i] == vmps::kSU2) {
+      for (auto* v : {&p->P.qloc12, &p->P.qloc34, &p->P.qop12, &p->P.qop34})
+        for (auto& q : *v) require(q[i] >= 1, VMPS_ERR_ARG, "SU(2) quantum numbers are q = 2S+1 >= 1");
+    }
+  p->P.init_W();
+  vmps::TwoSiteBasis tb;
+  tb.build(p->P.sym, p->P.qloc12, p->P.qloc34);
+  p->ns = (int)tb.qloc.size();
+  p->P.V.in.resize(p->ns);
+  p->P.V.out.resize(p->ns);
+  p->P.V.block.resize(p->ns);
+  p->P.V.dict.resize(p->ns);
+  *out = p.release();
+  VMPS_API_END
+}
+
+void vmps_pivot2_destroy(vmps_pivot2* p) { delete p; }
+
+vmps_status vmps_pivot2_local_basis(const vmps_pivot2* p, int* ns, int* qloc) {
+  VMPS_API_BEGIN
+  require(p && ns, VMPS_ERR_ARG, "NULL argument");
+  *ns = p->ns;
+  if (qloc) {
+    vmps::TwoSiteBasis tb;
+    tb.build(p->P.sym, p->P.qloc12, p->P.qloc34);
+    for (int s = 0; s < p->ns; ++s)
+      for (int i = 0; i < p->P.sym.nq; ++i) qloc[s * p->P.sym.nq + i] = tb.qloc[s][i];
+  }
+  VMPS_API_END
+}
+
+vmps_status vmps_pivot2_add_env(vmps_pivot2* p, int which, const int* qin, const int* qout, const int* qmid, int adim,
+                                const int64_t* rows, const int64_t* cols, const double* data) {
+  VMPS_API_BEGIN
+  require(p && qin && qout && qmid, VMPS_ERR_ARG, "NULL argument");
+  require(which == 0 || which == 1, VMPS_ERR_ARG, "which must be 0 (L) or 1 (R)");
+  require(adim >= 0, VMPS_ERR_ARG, "adim < 0");
+  require(adim == 0 || (rows && cols), VMPS_ERR_ARG, "NULL rows/cols");
+  const int nq = p->P.sym.nq;
+  vmps::Tripod& E = which == 0 ? p->P.L : p->P.R;
+  vmps::Key3 key{read_qn(qin, nq), read_qn(qout, nq), read_qn(qmid, nq)};
+  require(!E.dict.count(key), VMPS_ERR_LAYOUT, "duplicate environment sector");
+  int64_t total = 0;
+  for (int a = 0; a < adim; ++a) {
+    require(rows[a] >= 0 && cols[a] >= 0, VMPS_ERR_ARG, "negative block dimension");
+    total += rows[a] * cols[a];
+  }
+  require(total == 0 || data, VMPS_ERR_ARG, "NULL data");
+  std::vector<vmps::DenseBlock> blocks(adim);
+  int64_t src = 0;
+  for (int a = 0; a < adim; ++a) {
+    vmps::DenseBlock b;
+    b.rows = rows[a];
+    b.cols = cols[a];
+    if (b.rows * b.cols > 0) {
+      b.off = (int64_t)E.data.size();
+      E.data.insert(E.data.end(), data + src, data + src + b.rows * b.cols);
+      src += b.rows * b.cols;
+    }
+    blocks[a] = b;
+  }
+  E.index.push_back({key.a, key.b, key.c});
+  E.block.push_back(std::move(blocks));
+  E.dict[key] = E.index.size() - 1;
+  VMPS_API_END
+}
+
+vmps_status vmps_pivot2_add_W(vmps_pivot2* p, int which, int s1, int s2, int k, const int* qin, const int* qout,
+                              int rows, int cols, int64_t nnz, const int* row, const int* col, const double* val) {
+  VMPS_API_BEGIN
+  require(p && qin && qout, VMPS_ERR_ARG, "NULL argument");
+  require(which == 12 || which == 34, VMPS_ERR_ARG, "which must be 12 or 34");
+  require(nnz >= 0 && (nnz == 0 || (row && col && val)), VMPS_ERR_ARG, "bad nonzero arrays");
+  auto& W = which == 12 ? p->P.W12 : p->P.W34;
+  const size_t nloc = which == 12 ? p->P.qloc12.size() : p->P.qloc34.size();
+  const size_t nop = which == 12 ? p->P.qop12.size() : p->P.qop34.size();
+  require(s1 >= 0 && (size_t)s1 < nloc && s2 >= 0 && (size_t)s2 < nloc && k >= 0 && (size_t)k < nop, VMPS_ERR_ARG,
+          "W index out of range");
+  require(rows >= 0 && cols >= 0, VMPS_ERR_ARG, "negative W block dimension");
+  const int nq = p->P.sym.nq;
+  vmps::WBiped& B = W[s1][s2][k];
+  vmps::Key2 key{read_qn(qin, nq), read_qn(qout, nq)};
+  require(!B.dict.count(key), VMPS_ERR_LAYOUT, "duplicate W block");
+  vmps::SparseBlock sb;
+  sb.rows = rows;
+  sb.cols = cols;
+  std::vector<int64_t> order(nnz);
+  for (int64_t i = 0; i < nnz; ++i) {
+    require(row[i] >= 0 && row[i] < rows && col[i] >= 0 && col[i] < cols, VMPS_ERR_ARG, "W nonzero out of range");
+    order[i] = i;
+  }
+  // Eigen ColMajor InnerIterator order: column outer, row inner
+  std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) {
+    if (col[a] != col[b]) return col[a] < col[b];
+    return row[a] < row[b];
+  });
+  for (int64_t i = 0; i < nnz; ++i) {
+    int64_t j = order[i];
+    if (i > 0 && col[j] == sb.col.back() && row[j] == sb.row.back())
+      throw VmpsError(VMPS_ERR_ARG, "duplicate W nonzero");
+    sb.row.push_back(row[j]);
+    sb.col.push_back(col[j]);
+    sb.val.push_back(val[j]);
+  }
+  B.in.push_back(key.a);
+  B.out.push_back(key.b);
+  B.block.push_back(std::move(sb));
+  B.dict[key] = B.in.size() - 1;
+  VMPS_API_END
+}
+
+vmps_status vmps_pivot2_add_V_block(vmps_pivot2* p, int s, const int* qin, const int* qout, int64_t rows, int64_t cols,
+                                    int64_t* offset) {
+  VMPS_API_BEGIN
+  require(p && qin && qout, VMPS_ERR_ARG, "NULL argument");
+  const int nq = p->P.sym.nq;
+  int q = p->P.add_V_block(s, read_qn(qin, nq), read_qn(qout, nq), rows, cols);
+  if (offset) *offset = p->P.V.block[s][q].off;
+  p->P.has_blockstructure = false;
+  VMPS_API_END
+}
+
+vmps_status vmps_pivot2_vsize(const vmps_pivot2* p, int64_t* size) {
+  VMPS_API_BEGIN
+  require(p && size, VMPS_ERR_ARG, "NULL argument");
+  *size = p->P.V.size;
+  VMPS_API_END
+}
+
+vmps_status vmps_pivot2_precalc(vmps_pivot2* p) {
+  VMPS_API_BEGIN
+  require(p != nullptr, VMPS_ERR_ARG, "NULL pivot");
+  vmps::precalc_blockStructure(p->P);
+  VMPS_API_END
+}
+
+vmps_status vmps_pivot2_set_blockstructure(vmps_pivot2* p, int64_t nlhs, const int64_t* qlhs, const int64_t* rhs_begin,
+                                           const int64_t* qrhs, const double* factors) {
+  VMPS_API_BEGIN
+  require(p && nlhs >= 0, VMPS_ERR_ARG, "bad argument");
+  require(nlhs == 0 || (qlhs && rhs_begin), VMPS_ERR_ARG, "NULL table");
+  const int64_t nrhs = nlhs ? rhs_begin[nlhs] : 0;
+  require(nrhs == 0 || (qrhs && factors), VMPS_ERR_ARG, "NULL table");
+  auto& P = p->P;
+  P.qlhs.assign(nlhs, {0, 0});
+  P.qrhs.assign(nlhs, {});
+  P.factor_cgcs.assign(nlhs, {});
+  for (int64_t i = 0; i < nlhs; ++i) {
+    require(qlhs[2 * i] >= 0 && qlhs[2 * i + 1] >= 0, VMPS_ERR_ARG, "negative index");
+    P.qlhs[i] = {(size_t)qlhs[2 * i], (size_t)qlhs[2 * i + 1]};
+    require(rhs_begin[i] <= rhs_begin[i + 1] && rhs_begin[i] >= 0, VMPS_ERR_ARG, "rhs_begin not monotone");
+    for (int64_t j = rhs_begin[i]; j < rhs_begin[i + 1]; ++j) {
+      std::array<size_t, 12> e;
+      for (int c = 0; c < 12; ++c) {
+        require(qrhs[12 * j + c] >= 0, VMPS_ERR_ARG, "negative index");
+        e[c] = (size_t)qrhs[12 * j + c];
+      }
+      P.qrhs[i].push_back(e);
+      P.factor_cgcs[i].push_back(factors[j]);
+    }
+  }
+  P.has_blockstructure = true;
+  VMPS_API_END
+}
+
+vmps_status vmps_pivot2_get_blockstructure(const vmps_pivot2* p, int64_t* nlhs, int64_t* nrhs, int64_t* qlhs,
+                                           int64_t* rhs_begin, int64_t* qrhs, double* factors) {
+  VMPS_API_BEGIN
+  require(p && nlhs && nrhs, VMPS_ERR_ARG, "NULL argument");
+  require(p->P.has_blockstructure, VMPS_ERR_STATE, "no block structure");
+  const auto& P = p->P;
+  *nlhs = (int64_t)P.qlhs.size();
+  int64_t n = 0;
+  for (auto& v : P.qrhs) n += (int64_t)v.size();
+  *nrhs = n;
+  if (qlhs && rhs_begin && qrhs && factors) {
+    int64_t j = 0;
+    for (size_t i = 0; i < P.qlhs.size(); ++i) {
+      qlhs[2 * i] = (int64_t)P.qlhs[i][0];
+      qlhs[2 * i + 1] = (int64_t)P.qlhs[i][1];
+      rhs_begin[i] = j;
+      for (size_t q = 0; q < P.qrhs[i].size(); ++q, ++j) {
+        for (int c = 0; c < 12; ++c) qrhs[12 * j + c] = (int64_t)P.qrhs[i][q][c];
+        factors[j] = P.factor_cgcs[i][q];
+      }
+    }
+    rhs_begin[P.qlhs.size()] = j;
+  }
+  VMPS_API_END
+}
+
+vmps_status vmps_pivot2_set_projectors(vmps_pivot2* p, double Epenalty, int n, const double* A0) {
+  VMPS_API_BEGIN
+  require(p && n >= 0 && (n == 0 || A0), VMPS_ERR_ARG, "bad argument");
+  p->P.Epenalty = Epenalty;
+  p->P.A0proj.assign(n, {});
+  for (int i = 0; i < n; ++i) p->P.A0proj[i].assign(A0 + (size_t)i * p->P.V.size, A0 + (size_t)(i + 1) * p->P.V.size);
+  VMPS_API_END
+}
+
+vmps_status vmps_plan_create(const vmps_pivot2* p, vmps_precision prec, vmps_plan** out) {
+  VMPS_API_BEGIN
+  require(p && out, VMPS_ERR_ARG, "NULL argument");
+  *out = nullptr;
+  require(prec == VMPS_PREC_FP64 || prec == VMPS_PREC_FP32 || prec == VMPS_PREC_BF16X3, VMPS_ERR_ARG,
+          "unknown precision");
+  for (auto& v : p->P.A0proj)
+    require((int64_t)v.size() == p->P.V.size, VMPS_ERR_LAYOUT, "projector size differs from the vector size");
+  auto plan = std::make_unique<vmps_plan>();
+  plan->D.create(p->P, prec);
+  *out = plan.release();
+  VMPS_API_END
+}
+
+void vmps_plan_destroy(vmps_plan* plan) { delete plan; }
+
+vmps_status vmps_plan_stats_get(const vmps_plan* plan, vmps_plan_stats* st) {
+  VMPS_API_BEGIN
+  require(plan && st, VMPS_ERR_ARG, "NULL argument");
+  const auto& s = plan->D.H.stats;
+  st->n_terms = s.n_terms;
+  st->n_T = s.n_T;
+  st->n_U = s.n_U;
+  st->n_out = s.n_out;
+  st->n_L = s.n_L;
+  st->n_R = s.n_R;
+  st->n_V = s.n_V;
+  st->flops_alg = s.flops_alg;
+  st->flops_ref = s.flops_ref;
+  st->bytes_alg_fp64 = s.bytes_alg_fp64;
+  st->L_elems = s.L_elems;
+  st->R_elems = s.R_elems;
+  st->V_elems = s.V_elems;
+  st->T_elems = s.T_elems;
+  st->U_elems = s.U_elems;
+  st->max_block_dim = s.max_block_dim;
+  st->flops_A = s.flops_A;
+  st->flops_C = s.flops_C;
+  VMPS_API_END
+}
+
+vmps_status vmps_plan_set_precision(vmps_plan* plan, vmps_precision prec) {
+  VMPS_API_BEGIN
+  require(plan != nullptr, VMPS_ERR_ARG, "NULL plan");
+  require(prec == VMPS_PREC_FP64 || prec == VMPS_PREC_FP32 || prec == VMPS_PREC_BF16X3, VMPS_ERR_ARG,
+          "unknown precision");
+  plan->D.prec = prec;
+  plan->D.ensure_precision_buffers();
+  VMPS_HIP_CHECK(hipStreamSynchronize(plan->D.stream));
+  VMPS_API_END
+}
+
+vmps_status vmps_hxv(vmps_plan* plan, const double* vin, double* vout) {
+  VMPS_API_BEGIN
+  require(plan && (plan->D.H.vsize == 0 || (vin && vout)), VMPS_ERR_ARG, "NULL argument");
+  auto& D = plan->D;
+  const size_t bytes = (size_t)D.H.vsize * sizeof(double);
+  for (int64_t i = 0; i < D.H.vsize; ++i)
+    if (!std::isfinite(vin[i])) throw VmpsError(VMPS_ERR_NONFINITE, "non-finite entry in Vin at " + std::to_string(i));
+  if (bytes) VMPS_HIP_CHECK(hipMemcpyAsync(D.vin.p, vin, bytes, hipMemcpyHostToDevice, D.stream));
+  D.hxv(D.vin.p, D.vout.p, D.stream);
+  if (bytes) VMPS_HIP_CHECK(hipMemcpyAsync(vout, D.vout.p, bytes, hipMemcpyDeviceToHost, D.stream));
+  VMPS_HIP_CHECK(hipStreamSynchronize(D.stream));
+  VMPS_API_END
+}
+
+vmps_status vmps_hxv_device(vmps_plan* plan, const double* d_vin, double* d_vout, void* stream) {
+  VMPS_API_BEGIN
+  require(plan && (plan->D.H.vsize == 0 || (d_vin && d_vout)), VMPS_ERR_ARG, "NULL argument");
+  require(d_vin != d_vout || plan->D.H.vsize == 0, VMPS_ERR_ARG, "Vin and Vout must not alias");
+  plan->D.hxv(d_vin, d_vout, (hipStream_t)stream);
+  VMPS_API_END
+}
+
+vmps_status vmps_dot(vmps_plan* plan, const double* x, const double* y, double* result) {
+  VMPS_API_BEGIN
+  require(plan && result && (plan->D.H.vsize == 0 || (x && y)), VMPS_ERR_ARG, "NULL argument");
+  auto& D = plan->D;
+  const size_t bytes = (size_t)D.H.vsize * sizeof(double);
+  if (bytes) {
+    VMPS_HIP_CHECK(hipMemcpyAsync(D.vin.p, x, bytes, hipMemcpyHostToDevice, D.stream));
+    VMPS_HIP_CHECK(hipMemcpyAsync(D.vout.p, y, bytes, hipMemcpyHostToDevice, D.stream));
+  }
+  *result = D.dot(D.vin.p, D.vout.p, D.stream);
+  VMPS_API_END
+}
+
+vmps_status vmps_plan_time_hxv(vmps_plan* plan, int warmup, int iters, double* ms_per_hxv) {
+  VMPS_API_BEGIN
+  require(plan && ms_per_hxv && iters > 0 && warmup >= 0, VMPS_ERR_ARG, "bad argument");
+  auto& D = plan->D;
+  for (int i = 0; i < warmup; ++i) D.hxv(D.vin.p, D.vout.p, D.stream);
+  VMPS_HIP_CHECK(hipEventRecord(D.ev0, D.stream));
+  for (int i = 0; i < iters; ++i) D.hxv(D.vin.p, D.vout.p, D.stream);
+  VMPS_HIP_CHECK(hipEventRecord(D.ev1, D.stream));
+  VMPS_HIP_CHECK(hipEventSynchronize(D.ev1));
+  float ms = 0.f;
+  VMPS_HIP_CHECK(hipEventElapsedTime(&ms, D.ev0, D.ev1));
+  *ms_per_hxv = (double)ms / iters;
+  VMPS_API_END
+}
+
+vmps_status vmps_plan_time_stages(vmps_plan* plan, int warmup, int iters, double* ms) {
+  VMPS_API_BEGIN
+  require(plan && ms && iters > 0 && warmup >= 0, VMPS_ERR_ARG, "bad argument");
+  auto& D = plan->D;
+  for (int i = 0; i < warmup; ++i) D.hxv(D.vin.p, D.vout.p, D.stream);
+  for (int st = 0; st < 4; ++st) {
+    VMPS_HIP_CHECK(hipEventRecord(D.ev0, D.stream));
+    for (int i = 0; i < iters; ++i) {
+      if (st < 3)
+        D.launch_stage(st, D.vin.p, D.vout.p, D.stream);
+      else
+        D.hxv(D.vin.p, D.vout.p, D.stream);
+    }
+    VMPS_HIP_CHECK(hipEventRecord(D.ev1, D.stream));
+    VMPS_HIP_CHECK(hipEventSynchronize(D.ev1));
+    float t = 0.f;
+    VMPS_HIP_CHECK(hipEventElapsedTime(&t, D.ev0, D.ev1));
+    ms[st] = (double)t / iters;
+  }
+  VMPS_API_END
+}
+
+}  // extern "C"

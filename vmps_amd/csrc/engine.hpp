@@ -1,0 +1,102 @@
+// Device plan: the staged block-triple tables resident on one GPU, plus the
+// kernels that execute one effective-Hamiltonian application on it.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "errors.hpp"
+#include "pivot.hpp"
+
+namespace vmps {
+
+#define VMPS_HIP_CHECK(x)                                                                  \
+  do {                                                                                     \
+    hipError_t e_ = (x);                                                                   \
+    if (e_ != hipSuccess)                                                                  \
+      throw ::vmps::VmpsError(e_ == hipErrorOutOfMemory ? VMPS_ERR_OOM : VMPS_ERR_HIP,     \
+                              std::string(#x) + ": " + hipGetErrorString(e_));             \
+  } while (0)
+
+// Output tile of a grouped GEMM problem: problem index and tile coordinates.
+struct Tile {
+  int32_t prob;
+  int16_t mt, nt;
+};
+// Element range of one combine list (U_u = sum c T).
+struct CombTask {
+  int32_t list;
+  int32_t start;
+};
+
+template <class T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  void alloc(size_t count) {
+    release();
+    n = count;
+    if (count) VMPS_HIP_CHECK(hipMalloc((void**)&p, count * sizeof(T)));
+  }
+  void upload(const std::vector<T>& h, hipStream_t s) {
+    alloc(h.size());
+    if (!h.empty()) VMPS_HIP_CHECK(hipMemcpyAsync(p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, s));
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  ~DevBuf() { release(); }
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+};
+
+struct DevPlan {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  vmps_precision prec = VMPS_PREC_FP64;
+  HostPlan H;
+
+  // FP64 operands and intermediates
+  DevBuf<double> L, R, T, U, vin, vout;
+  // FP32 operands and intermediates (mixed modes)
+  DevBuf<float> L32, R32, V32, T32, U32;
+  // split-BF16 (hi, lo) operands: stored as uint16 pairs
+  DevBuf<uint16_t> Lhi, Llo, Rhi, Rlo, Vhi, Vlo, Uhi, Ulo;
+
+  DevBuf<GemmProb> probA, probC;
+  DevBuf<GemmChunk> chunkA, chunkC;
+  DevBuf<Tile> tilesA, tilesC;
+  DevBuf<CombineList> comb;
+  DevBuf<int32_t> comb_t;
+  DevBuf<double> comb_c;
+  DevBuf<float> comb_c32;
+  DevBuf<int64_t> T_off;
+  DevBuf<CombTask> ctasks;
+  DevBuf<int64_t> seg_off;
+  DevBuf<int32_t> seg_len;
+  DevBuf<double> seg_w;
+  DevBuf<double> partial;  // dot-product partial sums
+  DevBuf<double> scalars;  // small device scalars (dot results, projector overlaps)
+
+  // excited-state projectors
+  int nproj = 0;
+  double Epenalty = 0.0;
+  DevBuf<double> A0;
+
+  ~DevPlan();
+  void create(const Pivot2& P, vmps_precision prec);
+  void ensure_precision_buffers();
+  void hxv(const double* d_vin, double* d_vout, hipStream_t s);
+  // one stage of hxv alone: 0 = L.V, 1 = combine, 2 = U.R
+  void launch_stage(int stage, const double* d_vin, double* d_vout, hipStream_t s);
+  double dot(const double* d_x, const double* d_y, hipStream_t s);
+  void dot_async(const double* d_x, const double* d_y, double* d_result, hipStream_t s);
+};
+
+}  // namespace vmps

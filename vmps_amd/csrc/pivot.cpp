@@ -1,0 +1,527 @@
+// Two-site pivot: combined local basis, block-structure precalculation,
+// per-term expansion and the staged plan; see pivot.hpp.
+#include "pivot.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <stdexcept>
+#include <unordered_map>
+
+#include "errors.hpp"
+
+namespace vmps {
+
+// ---------------------------------------------------------------------------
+// TwoSiteBasis: Qbasis::pullData(qloc) (Qbasis.h:580-590 with push_back
+// :228-254), Qbasis::combine (:686-798), outer_num (:378-388), inner_num
+// (:335-350) and leftOffset (:440-462).
+// ---------------------------------------------------------------------------
+namespace {
+struct PlainEntry {
+  Qn q;
+  int64_t num;
+  int64_t plain;
+};
+
+void pull_data(const std::vector<Qn>& qs, std::vector<PlainEntry>& entries, std::vector<int64_t>& inner) {
+  entries.clear();
+  int64_t curr = 0;
+  for (const Qn& q : qs) {
+    auto it = std::find_if(entries.begin(), entries.end(), [&](const PlainEntry& e) { return e.q == q; });
+    if (it == entries.end()) {
+      entries.push_back({q, curr, 1});
+    } else {
+      // the reference appends to the existing sector without shifting the
+      // offsets of later sectors (Qbasis.h:243-251)
+      it->plain += 1;
+    }
+    curr += 1;
+  }
+  inner.assign(qs.size(), 0);
+  for (size_t s = 0; s < qs.size(); ++s) {
+    bool found = false;
+    for (const auto& e : entries) {
+      for (int64_t i = 0; i < e.plain && !found; ++i) {
+        if ((int64_t)s == e.num + i) {
+          inner[s] = i;
+          found = true;
+        }
+      }
+      if (found) break;
+    }
+  }
+}
+}  // namespace
+
+void TwoSiteBasis::build(const Sym& sym, const std::vector<Qn>& qloc12, const std::vector<Qn>& qloc34) {
+  q12_ = qloc12;
+  q34_ = qloc34;
+  std::vector<PlainEntry> e12, e34;
+  pull_data(qloc12, e12, inner12_);
+  pull_data(qloc34, e34, inner34_);
+  plain12_.clear();
+  plain34_.clear();
+  for (auto& e : e12) plain12_[e.q] = e.plain;
+  for (auto& e : e34) plain34_[e.q] = e.plain;
+
+  history_.clear();
+  for (const auto& a : e12)
+    for (const auto& b : e34)
+      for (const Qn& q : sym.reduce(a.q, b.q)) {
+        auto& h = history_[q];
+        bool have = false;
+        for (const auto& src : h)
+          if (src.q1 == a.q && src.q2 == b.q) have = true;
+        if (!have) h.push_back({a.q, b.q, a.plain, b.plain});
+      }
+  for (auto& kv : history_)
+    std::stable_sort(kv.second.begin(), kv.second.end(), [](const Src& x, const Src& y) {
+      if (x.q1 != y.q1) return x.q1 < y.q1;
+      return x.q2 < y.q2;
+    });
+  outer_.clear();
+  qloc.clear();
+  int64_t num = 0;
+  for (const auto& kv : history_) {  // std::map iterates in sorted (Qbasis::sort) order
+    outer_[kv.first] = num;
+    int64_t d = 0;
+    for (const auto& src : kv.second) d += src.dim1 * src.dim2;
+    for (int64_t i = 0; i < d; ++i) qloc.push_back(kv.first);
+    num += d;
+  }
+}
+
+int64_t TwoSiteBasis::index(int s1, int s3, const Qn& qmerge) const {
+  auto ho = history_.find(qmerge);
+  if (ho == history_.end()) return -1;
+  int64_t out = outer_.at(qmerge);
+  const Qn& q1 = q12_[s1];
+  const Qn& q2 = q34_[s3];
+  for (const auto& src : ho->second) {
+    if (!(src.q1 == q1 && src.q2 == q2)) {
+      out += src.dim1 * src.dim2;
+    } else {
+      int64_t p1 = inner12_[s1], p2 = inner34_[s3];
+      for (int64_t j = 0; j < src.dim1 * src.dim2; ++j) {
+        int64_t t = j / src.dim1;
+        if (!(j - src.dim1 * t == p1 && t == p2)) {
+          out += 1;
+        } else {
+          break;
+        }
+      }
+      return out;
+    }
+  }
+  return -1;
+}
+
+// ---------------------------------------------------------------------------
+void Pivot2::init_W() {
+  W12.assign(qloc12.size(), std::vector<std::vector<WBiped>>(qloc12.size(), std::vector<WBiped>(qop12.size())));
+  W34.assign(qloc34.size(), std::vector<std::vector<WBiped>>(qloc34.size(), std::vector<WBiped>(qop34.size())));
+}
+
+int Pivot2::add_V_block(int s, const Qn& qin, const Qn& qout, int64_t rows, int64_t cols) {
+  if (s < 0 || s >= (int)V.in.size()) throw VmpsError(VMPS_ERR_ARG, "V block: s out of range");
+  if (rows < 0 || cols < 0) throw VmpsError(VMPS_ERR_ARG, "V block: negative dimension");
+  Key2 key{qin, qout};
+  if (V.dict[s].count(key)) throw VmpsError(VMPS_ERR_LAYOUT, "V block: duplicate (qin,qout) in one s");
+  DenseBlock b;
+  b.rows = rows;
+  b.cols = cols;
+  b.off = V.size;
+  V.size += rows * cols;
+  V.in[s].push_back(qin);
+  V.out[s].push_back(qout);
+  V.block[s].push_back(b);
+  V.dict[s][key] = V.in[s].size() - 1;
+  V.nblocks += 1;
+  return (int)V.in[s].size() - 1;
+}
+
+// ---------------------------------------------------------------------------
+// AAWWAA, DmrgIndexGymnastics.h:198-262
+// ---------------------------------------------------------------------------
+namespace {
+struct AawwaaHit {
+  Qn qRin, qRout, qRmid, qW;
+  size_t q13, q24, qW12, qW34;
+};
+
+bool AAWWAA(const Sym& sym, const Qn& Lin, const Qn& Lout, const Qn& Lmid, const Qn& qOp12, const Qn& qOp34,
+            const Qn& qmerge13, const Qn& qmerge24, const VLayout& V, size_t s13, size_t s24, const WBiped& W12,
+            const WBiped& W34, std::vector<AawwaaHit>& result) {
+  result.clear();
+  auto qRouts = sym.reduce(Lin, qmerge13);
+  auto qRins = sym.reduce(Lout, qmerge24);
+  auto qWs = sym.reduce(Lmid, qOp12);
+  for (const Qn& qRout : qRouts) {
+    auto q13 = V.dict[s13].find(Key2{Lin, qRout});
+    if (q13 == V.dict[s13].end()) continue;
+    for (const Qn& qRin : qRins) {
+      auto q24 = V.dict[s24].find(Key2{Lout, qRin});
+      if (q24 == V.dict[s24].end()) continue;
+      for (const Qn& qW : qWs) {
+        auto w12 = W12.dict.find(Key2{Lmid, qW});
+        if (w12 == W12.dict.end()) continue;
+        for (const Qn& qRmid : sym.reduce(qW, qOp34)) {
+          auto w34 = W34.dict.find(Key2{qW, qRmid});
+          if (w34 == W34.dict.end()) continue;
+          if (sym.validate3(qRin, qRmid, qRout))
+            result.push_back({qRin, qRout, qRmid, qW, q13->second, q24->second, w12->second, w34->second});
+        }
+      }
+    }
+  }
+  return !result.empty();
+}
+}  // namespace
+
+void precalc_blockStructure(Pivot2& P) {
+  const Sym& sym = P.sym;
+  const bool NA = sym.non_abelian();
+  TwoSiteBasis tb;
+  tb.build(sym, P.qloc12, P.qloc34);
+  if ((int64_t)tb.qloc.size() != (int64_t)P.V.in.size())
+    throw VmpsError(VMPS_ERR_LAYOUT, "PivotVector has " + std::to_string(P.V.in.size()) +
+                                         " two-site components, the combined local basis has " +
+                                         std::to_string(tb.qloc.size()));
+
+  using Key = std::array<size_t, 2>;
+  std::map<Key, std::pair<std::vector<std::array<size_t, 12>>, std::vector<double>>> lookup;
+  std::vector<AawwaaHit> ixs;
+
+  for (size_t s1 = 0; s1 < P.qloc12.size(); ++s1)
+    for (size_t s2 = 0; s2 < P.qloc12.size(); ++s2)
+      for (size_t k12 = 0; k12 < P.qop12.size(); ++k12) {
+        if (!sym.validate3(P.qloc12[s2], P.qop12[k12], P.qloc12[s1])) continue;
+        for (size_t s3 = 0; s3 < P.qloc34.size(); ++s3)
+          for (size_t s4 = 0; s4 < P.qloc34.size(); ++s4)
+            for (size_t k34 = 0; k34 < P.qop34.size(); ++k34) {
+              if (!sym.validate3(P.qloc34[s4], P.qop34[k34], P.qloc34[s3])) continue;
+              auto qOps = sym.reduce(P.qop12[k12], P.qop34[k34]);
+              for (const Qn& qOp : qOps) {
+                auto qmerges13 = sym.reduce(P.qloc12[s1], P.qloc34[s3]);
+                auto qmerges24 = sym.reduce(P.qloc12[s2], P.qloc34[s4]);
+                for (const Qn& qmerge13 : qmerges13)
+                  for (const Qn& qmerge24 : qmerges24) {
+                    int64_t s1s3 = tb.index((int)s1, (int)s3, qmerge13);
+                    int64_t s2s4 = tb.index((int)s2, (int)s4, qmerge24);
+                    if (s1s3 < 0 || s2s4 < 0) continue;
+                    double cgc9 = NA ? sym.coeff_tensorProd(P.qloc12[s2], P.qloc34[s4], qmerge24, P.qop12[k12],
+                                                            P.qop34[k34], qOp, P.qloc12[s1], P.qloc34[s3], qmerge13)
+                                     : 1.0;
+                    if (std::abs(cgc9) < kCgcEps) continue;
+                    for (size_t qL = 0; qL < P.L.dim(); ++qL) {
+                      bool found = AAWWAA(sym, P.L.in(qL), P.L.out(qL), P.L.mid(qL), P.qop12[k12], P.qop34[k34],
+                                          qmerge13, qmerge24, P.V, (size_t)s1s3, (size_t)s2s4, P.W12[s1][s2][k12],
+                                          P.W34[s3][s4][k34], ixs);
+                      if (!found) continue;
+                      for (const auto& ix : ixs) {
+                        auto qR = P.R.dict.find(Key3{ix.qRin, ix.qRout, ix.qRmid});
+                        double cgc6 = NA ? sym.coeff_Apair(P.L.mid(qL), P.qop12[k12], ix.qW, P.qop34[k34], ix.qRmid, qOp)
+                                         : 1.0;
+                        if (std::abs(cgc6) < kCgcEps) continue;
+                        if (qR == P.R.dict.end()) continue;
+                        double cgcH =
+                            NA ? sym.coeff_HPsi(P.V.in[s2s4][ix.q24], qmerge24, P.V.out[s2s4][ix.q24], P.L.mid(qL), qOp,
+                                                P.R.mid(qR->second), P.V.in[s1s3][ix.q13], qmerge13,
+                                                P.V.out[s1s3][ix.q13])
+                               : 1.0;
+                        Key key = {(size_t)s1s3, ix.q13};
+                        std::array<size_t, 12> val = {(size_t)s2s4, ix.q24, qL, qR->second, s1, s2, k12, ix.qW12,
+                                                      s3, s4, k34, ix.qW34};
+                        auto& e = lookup[key];
+                        e.first.push_back(val);
+                        e.second.push_back(cgc6 * cgc9 * cgcH);
+                      }
+                    }
+                  }
+              }
+            }
+      }
+  P.qlhs.clear();
+  P.qrhs.clear();
+  P.factor_cgcs.clear();
+  for (auto& kv : lookup) {
+    P.qlhs.push_back(kv.first);
+    P.qrhs.push_back(std::move(kv.second.first));
+    P.factor_cgcs.push_back(std::move(kv.second.second));
+  }
+  P.has_blockstructure = true;
+}
+
+// ---------------------------------------------------------------------------
+// Per-term expansion: the loop nest of OxV, DmrgPivotMatrix2.h:182-230.
+// ---------------------------------------------------------------------------
+void expand_terms(const Pivot2& P, std::vector<Term>& terms, std::vector<std::pair<size_t, size_t>>& lblocks,
+                  std::vector<std::pair<size_t, size_t>>& rblocks, std::vector<std::pair<int, int>>& vblocks) {
+  if (!P.has_blockstructure) throw VmpsError(VMPS_ERR_STATE, "pivot has no block structure (call precalc first)");
+  // global ids of V blocks in packed (s,q) order
+  std::vector<std::vector<int64_t>> vid(P.V.in.size());
+  vblocks.clear();
+  for (size_t s = 0; s < P.V.in.size(); ++s) {
+    vid[s].resize(P.V.in[s].size());
+    for (size_t q = 0; q < P.V.in[s].size(); ++q) {
+      vid[s][q] = (int64_t)vblocks.size();
+      vblocks.push_back({(int)s, (int)q});
+    }
+  }
+  std::map<std::pair<size_t, size_t>, int64_t> lid, rid;
+  lblocks.clear();
+  rblocks.clear();
+  terms.clear();
+  for (size_t q = 0; q < P.qlhs.size(); ++q) {
+    size_t s1s3 = P.qlhs[q][0], q13 = P.qlhs[q][1];
+    if (s1s3 >= P.V.in.size() || q13 >= P.V.in[s1s3].size())
+      throw VmpsError(VMPS_ERR_LAYOUT, "qlhs entry outside the vector layout");
+    for (size_t p = 0; p < P.qrhs[q].size(); ++p) {
+      const auto& e = P.qrhs[q][p];
+      size_t s2s4 = e[0], q24 = e[1], qL = e[2], qR = e[3], s1 = e[4], s2 = e[5], k12 = e[6], qW12 = e[7], s3 = e[8],
+             s4 = e[9], k34 = e[10], qW34 = e[11];
+      if (s2s4 >= P.V.in.size() || q24 >= P.V.in[s2s4].size() || qL >= P.L.dim() || qR >= P.R.dim() ||
+          s1 >= P.W12.size() || s2 >= P.W12[s1].size() || k12 >= P.W12[s1][s2].size() ||
+          qW12 >= P.W12[s1][s2][k12].block.size() || s3 >= P.W34.size() || s4 >= P.W34[s3].size() ||
+          k34 >= P.W34[s3][s4].size() || qW34 >= P.W34[s3][s4][k34].block.size())
+        throw VmpsError(VMPS_ERR_LAYOUT, "qrhs entry outside the pivot");
+      const SparseBlock& w12 = P.W12[s1][s2][k12].block[qW12];
+      const SparseBlock& w34 = P.W34[s3][s4][k34].block[qW34];
+      double f = P.factor_cgcs[q][p];
+      const DenseBlock& vin = P.V.block[s2s4][q24];
+      for (size_t i = 0; i < w12.val.size(); ++i)
+        for (size_t j = 0; j < w34.val.size(); ++j) {
+          int a = w12.row[i], w = w12.col[i], w2 = w34.row[j], b = w34.col[j];
+          if (a >= (int)P.L.block[qL].size() || b >= (int)P.R.block[qR].size())
+            throw VmpsError(VMPS_ERR_LAYOUT, "MPO index outside the environment's mid sector");
+          if (!(P.L.block[qL][a].present() && P.R.block[qR][b].present() && vin.present() && w == w2)) continue;
+          double c = f * w12.val[i] * w34.val[j];
+          if (c == 0.0) continue;  // exact zero contributes nothing
+          auto lk = std::make_pair(qL, (size_t)a);
+          auto rk = std::make_pair(qR, (size_t)b);
+          auto li = lid.find(lk);
+          if (li == lid.end()) {
+            li = lid.emplace(lk, (int64_t)lblocks.size()).first;
+            lblocks.push_back(lk);
+          }
+          auto ri = rid.find(rk);
+          if (ri == rid.end()) {
+            ri = rid.emplace(rk, (int64_t)rblocks.size()).first;
+            rblocks.push_back(rk);
+          }
+          terms.push_back({vid[s1s3][q13], li->second, vid[s2s4][q24], ri->second, c});
+        }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Every device access of the three stages stays inside its buffer: checked on
+// the host before anything is uploaded.
+namespace {
+void check_gemm(const std::vector<GemmProb>& probs, const std::vector<GemmChunk>& chunks, int64_t a_size,
+                int64_t b_size, int64_t c_size, const char* what) {
+  for (const GemmProb& p : probs) {
+    if (p.M <= 0 || p.N <= 0 || p.c_off < 0 || p.c_off + (int64_t)p.M * p.N > c_size || p.chunk_begin < 0 ||
+        p.chunk_end < p.chunk_begin || p.chunk_end > (int32_t)chunks.size())
+      throw VmpsError(VMPS_ERR_INTERNAL, std::string(what) + ": output descriptor out of range");
+    for (int32_t c = p.chunk_begin; c < p.chunk_end; ++c) {
+      const GemmChunk& ch = chunks[c];
+      if (ch.K <= 0 || ch.lda < p.M || ch.ldb < ch.K || ch.a_off < 0 || ch.b_off < 0 ||
+          ch.a_off + (int64_t)ch.lda * (ch.K - 1) + p.M > a_size ||
+          ch.b_off + (int64_t)ch.ldb * (p.N - 1) + ch.K > b_size)
+        throw VmpsError(VMPS_ERR_INTERNAL, std::string(what) + ": operand descriptor out of range");
+    }
+  }
+}
+}  // namespace
+
+void validate_host_plan(const HostPlan& H) {
+  check_gemm(H.probA, H.chunkA, (int64_t)H.Ldata.size(), H.vsize, H.T_size, "stage A");
+  check_gemm(H.probC, H.chunkC, H.U_size, (int64_t)H.Rdata.size(), H.vsize, "stage C");
+  for (const CombineList& cl : H.comb) {
+    if (cl.u_off < 0 || cl.size <= 0 || cl.u_off + cl.size > H.U_size || cl.begin < 0 || cl.end < cl.begin ||
+        cl.end > (int32_t)H.comb_t.size() || cl.size > INT32_MAX)
+      throw VmpsError(VMPS_ERR_INTERNAL, "stage B: list descriptor out of range");
+    for (int32_t j = cl.begin; j < cl.end; ++j) {
+      int32_t t = H.comb_t[j];
+      if (t < 0 || t >= (int32_t)H.T_off.size() || H.T_off[t] + cl.size > H.T_size)
+        throw VmpsError(VMPS_ERR_INTERNAL, "stage B: T reference out of range");
+    }
+  }
+  for (size_t i = 0; i < H.seg_off.size(); ++i)
+    if (H.seg_off[i] < 0 || H.seg_off[i] + H.seg_len[i] > H.vsize)
+      throw VmpsError(VMPS_ERR_INTERNAL, "dot segment out of range");
+}
+
+void build_host_plan(const Pivot2& P, HostPlan& H) {
+  std::vector<Term> terms;
+  std::vector<std::pair<size_t, size_t>> lblocks, rblocks;
+  std::vector<std::pair<int, int>> vblocks;
+  expand_terms(P, terms, lblocks, rblocks, vblocks);
+  H = HostPlan();
+  Stats& st = H.stats;
+  H.vsize = P.V.size;
+
+  auto vblk = [&](int64_t id) -> const DenseBlock& {
+    return P.V.block[vblocks[id].first][vblocks[id].second];
+  };
+
+  // pack L / R blocks that terms reference
+  std::vector<int64_t> loff(lblocks.size()), roff(rblocks.size());
+  for (size_t i = 0; i < lblocks.size(); ++i) {
+    const DenseBlock& b = P.L.block[lblocks[i].first][lblocks[i].second];
+    loff[i] = (int64_t)H.Ldata.size();
+    H.Ldata.insert(H.Ldata.end(), P.L.data.begin() + b.off, P.L.data.begin() + b.off + b.size());
+    st.max_block_dim = std::max({st.max_block_dim, b.rows, b.cols});
+  }
+  for (size_t i = 0; i < rblocks.size(); ++i) {
+    const DenseBlock& b = P.R.block[rblocks[i].first][rblocks[i].second];
+    roff[i] = (int64_t)H.Rdata.size();
+    H.Rdata.insert(H.Rdata.end(), P.R.data.begin() + b.off, P.R.data.begin() + b.off + b.size());
+    st.max_block_dim = std::max({st.max_block_dim, b.rows, b.cols});
+  }
+  auto Lb = [&](int64_t id) -> const DenseBlock& { return P.L.block[lblocks[id].first][lblocks[id].second]; };
+  auto Rb = [&](int64_t id) -> const DenseBlock& { return P.R.block[rblocks[id].first][rblocks[id].second]; };
+
+  // shapes: L (m' x m), Vin (m x n), R (n x n'), Vout (m' x n')
+  for (const Term& t : terms) {
+    const DenseBlock &L = Lb(t.l_blk), &R = Rb(t.r_blk), &vi = vblk(t.in_v), &vo = vblk(t.out_v);
+    if (L.cols != vi.rows || vi.cols != R.rows || L.rows != vo.rows || R.cols != vo.cols)
+      throw VmpsError(VMPS_ERR_LAYOUT, "block shapes of a contributing term do not chain: L " + std::to_string(L.rows) +
+                                           "x" + std::to_string(L.cols) + ", Vin " + std::to_string(vi.rows) + "x" +
+                                           std::to_string(vi.cols) + ", R " + std::to_string(R.rows) + "x" +
+                                           std::to_string(R.cols) + ", Vout " + std::to_string(vo.rows) + "x" +
+                                           std::to_string(vo.cols));
+    st.flops_ref += 2.0 * L.rows * L.cols * vi.cols + 2.0 * L.rows * vi.cols * R.cols;
+  }
+
+  // stage A: distinct (l, v)
+  std::unordered_map<int64_t, int32_t> tmap;  // key = l * nV + v
+  const int64_t nV = (int64_t)vblocks.size();
+  std::vector<int32_t> term_t(terms.size());
+  for (size_t i = 0; i < terms.size(); ++i) {
+    const Term& t = terms[i];
+    int64_t key = t.l_blk * nV + t.in_v;
+    auto it = tmap.find(key);
+    if (it == tmap.end()) {
+      const DenseBlock &L = Lb(t.l_blk), &vi = vblk(t.in_v);
+      int32_t tid = (int32_t)H.probA.size();
+      it = tmap.emplace(key, tid).first;
+      GemmProb pr;
+      pr.c_off = H.T_size;
+      pr.M = (int32_t)L.rows;
+      pr.N = (int32_t)vi.cols;
+      pr.chunk_begin = (int32_t)H.chunkA.size();
+      pr.chunk_end = pr.chunk_begin + 1;
+      GemmChunk ch;
+      ch.a_off = loff[t.l_blk];
+      ch.b_off = vi.off;
+      ch.K = (int32_t)L.cols;
+      ch.lda = (int32_t)L.rows;
+      ch.ldb = (int32_t)vi.rows;
+      ch.pad = 0;
+      H.chunkA.push_back(ch);
+      H.probA.push_back(pr);
+      H.T_off.push_back(H.T_size);
+      H.T_size += L.rows * vi.cols;
+      st.flops_alg += 2.0 * L.rows * L.cols * vi.cols;
+      st.flops_A += 2.0 * L.rows * L.cols * vi.cols;
+    }
+    term_t[i] = it->second;
+  }
+
+  // stage B: distinct (o, r), entries in term order
+  std::unordered_map<int64_t, int32_t> umap;  // key = o * nR + r
+  const int64_t nR = (int64_t)rblocks.size();
+  std::vector<std::vector<std::pair<int32_t, double>>> ulist;
+  std::vector<std::pair<int64_t, int64_t>> ukey;  // (o, r)
+  std::vector<std::vector<int32_t>> out_u(nV);
+  for (size_t i = 0; i < terms.size(); ++i) {
+    const Term& t = terms[i];
+    int64_t key = t.out_v * nR + t.r_blk;
+    auto it = umap.find(key);
+    if (it == umap.end()) {
+      int32_t uid = (int32_t)ulist.size();
+      it = umap.emplace(key, uid).first;
+      ulist.emplace_back();
+      ukey.push_back({t.out_v, t.r_blk});
+      out_u[t.out_v].push_back(uid);
+    }
+    ulist[it->second].push_back({term_t[i], t.c});
+  }
+  std::vector<int64_t> uoff(ulist.size());
+  for (size_t u = 0; u < ulist.size(); ++u) {
+    const DenseBlock& vo = vblk(ukey[u].first);
+    const DenseBlock& R = Rb(ukey[u].second);
+    CombineList cl;
+    cl.u_off = H.U_size;
+    cl.size = vo.rows * R.rows;
+    cl.begin = (int32_t)H.comb_t.size();
+    for (auto& e : ulist[u]) {
+      H.comb_t.push_back(e.first);
+      H.comb_c.push_back(e.second);
+    }
+    cl.end = (int32_t)H.comb_t.size();
+    uoff[u] = H.U_size;
+    H.U_size += cl.size;
+    H.comb.push_back(cl);
+    st.flops_alg += 2.0 * vo.rows * R.rows * R.cols;
+    st.flops_C += 2.0 * vo.rows * R.rows * R.cols;
+  }
+
+  // stage C: one problem per output block (all blocks, so untouched ones are zeroed)
+  for (int64_t o = 0; o < nV; ++o) {
+    const DenseBlock& vo = vblk(o);
+    if (vo.size() == 0) continue;
+    GemmProb pr;
+    pr.c_off = vo.off;
+    pr.M = (int32_t)vo.rows;
+    pr.N = (int32_t)vo.cols;
+    pr.chunk_begin = (int32_t)H.chunkC.size();
+    for (int32_t u : out_u[o]) {
+      const DenseBlock& R = Rb(ukey[u].second);
+      GemmChunk ch;
+      ch.a_off = uoff[u];
+      ch.b_off = roff[ukey[u].second];
+      ch.K = (int32_t)R.rows;
+      ch.lda = (int32_t)vo.rows;
+      ch.ldb = (int32_t)R.rows;
+      ch.pad = 0;
+      H.chunkC.push_back(ch);
+    }
+    pr.chunk_end = (int32_t)H.chunkC.size();
+    H.probC.push_back(pr);
+  }
+
+  // coeff_dot weights per block (DmrgPivotVector.h:323), split in segments
+  for (size_t s = 0; s < P.V.in.size(); ++s)
+    for (size_t q = 0; q < P.V.in[s].size(); ++q) {
+      const DenseBlock& b = P.V.block[s][q];
+      H.vblock_off.push_back(b.off);
+      H.vblock_size.push_back(b.size());
+      double w = P.sym.coeff_dot(P.V.out[s][q]);
+      for (int64_t e = 0; e < b.size(); e += 4096) {
+        H.seg_off.push_back(b.off + e);
+        H.seg_len.push_back((int32_t)std::min<int64_t>(4096, b.size() - e));
+        H.seg_w.push_back(w);
+      }
+    }
+
+  validate_host_plan(H);
+  st.n_terms = (int64_t)terms.size();
+  st.n_T = (int64_t)H.probA.size();
+  st.n_U = (int64_t)H.comb.size();
+  st.n_out = (int64_t)H.probC.size();
+  st.n_L = (int64_t)lblocks.size();
+  st.n_R = (int64_t)rblocks.size();
+  st.n_V = nV;
+  st.L_elems = (int64_t)H.Ldata.size();
+  st.R_elems = (int64_t)H.Rdata.size();
+  st.V_elems = P.V.size;
+  st.T_elems = H.T_size;
+  st.U_elems = H.U_size;
+  st.bytes_alg_fp64 = 8.0 * (double)(st.L_elems + st.R_elems + 2 * st.V_elems);
+}
+
+}  // namespace vmps

@@ -1,0 +1,197 @@
+// Host-side mirror of the reference's two-site pivot (PivotMatrix2 + PivotVector
+// layout) and the flattening of its block structure into the staged
+// block-triple tables that the device engine executes.
+//
+// Reference objects mirrored here:
+//   Tripod (Multipede<3>)   tensors/Multipede.h:31-160   index[q] = {in,out,mid}, block[q][a][0]
+//   Biped                   tensors/Biped.h:62-104       in[q], out[q], block[q], dict{in,out}
+//   PivotMatrix2            pivot/DmrgPivotMatrix2.h:15-76 (L, R, W12, W34, qloc, qOp, qlhs/qrhs/factor_cgcs)
+//   PivotVector             pivot/DmrgPivotVector.h:8-117  data[s] = Biped of dense col-major blocks
+#pragma once
+#include <array>
+#include <cstdint>
+#include <map>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "sym.hpp"
+
+namespace vmps {
+
+struct DenseBlock {
+  int64_t rows = 0, cols = 0;
+  int64_t off = -1;  // element offset into the owning packed array; -1 = absent
+  bool present() const { return rows > 0 && cols > 0 && off >= 0; }
+  int64_t size() const { return rows * cols; }
+};
+
+struct Key2 {
+  Qn a, b;
+  bool operator==(const Key2& o) const { return a == o.a && b == o.b; }
+};
+struct Key2Hash {
+  size_t operator()(const Key2& k) const { return QnHash()(k.a) * 1000003u ^ QnHash()(k.b); }
+};
+struct Key3 {
+  Qn a, b, c;
+  bool operator==(const Key3& o) const { return a == o.a && b == o.b && c == o.c; }
+};
+struct Key3Hash {
+  size_t operator()(const Key3& k) const {
+    return (QnHash()(k.a) * 1000003u ^ QnHash()(k.b)) * 1000033u ^ QnHash()(k.c);
+  }
+};
+
+// Environment tensor (Multipede<3,...> with block[q][a][0]).
+struct Tripod {
+  std::vector<std::array<Qn, 3>> index;          // {in, out, mid}
+  std::vector<std::vector<DenseBlock>> block;    // [q][a]
+  std::vector<double> data;                      // packed FP64 blocks
+  std::unordered_map<Key3, size_t, Key3Hash> dict;
+  size_t dim() const { return index.size(); }
+  const Qn& in(size_t q) const { return index[q][0]; }
+  const Qn& out(size_t q) const { return index[q][1]; }
+  const Qn& mid(size_t q) const { return index[q][2]; }
+};
+
+// One sparse MPO block, nonzeros stored in column-major order as Eigen's
+// InnerIterator visits them (outer = column, rows ascending).
+struct SparseBlock {
+  int rows = 0, cols = 0;
+  std::vector<int> row, col;
+  std::vector<double> val;
+};
+
+struct WBiped {
+  std::vector<Qn> in, out;
+  std::vector<SparseBlock> block;
+  std::unordered_map<Key2, size_t, Key2Hash> dict;
+};
+
+// Layout of a PivotVector: data[s] is a Biped of dense blocks; all blocks
+// packed into one array in (s, q) order.
+struct VLayout {
+  std::vector<std::vector<Qn>> in, out;                   // [s][q]
+  std::vector<std::vector<DenseBlock>> block;             // [s][q]
+  std::vector<std::unordered_map<Key2, size_t, Key2Hash>> dict;  // [s]
+  int64_t size = 0;
+  int64_t nblocks = 0;
+};
+
+// Two-site combined local basis (Qbasis::combine of loc12 and loc34,
+// tensors/Qbasis.h:686-798, indexing via outer_num/leftOffset :378-462).
+struct TwoSiteBasis {
+  std::vector<Qn> qloc;  // quantum number of each combined state s
+  // s index of (s1, s3, qmerge); -1 if qmerge not in qloc12[s1] (x) qloc34[s3]
+  int64_t index(int s1, int s3, const Qn& qmerge) const;
+  void build(const Sym& sym, const std::vector<Qn>& qloc12, const std::vector<Qn>& qloc34);
+
+ private:
+  struct Src {
+    Qn q1, q2;
+    int64_t dim1, dim2;
+  };
+  std::map<Qn, int64_t> outer_;                    // qmerge -> outer_num
+  std::map<Qn, std::vector<Src>> history_;         // sorted by (q1,q2)
+  std::vector<Qn> q12_, q34_;
+  std::vector<int64_t> inner12_, inner34_;         // inner_num of each local state
+  std::map<Qn, int64_t> plain12_, plain34_;        // plain size of each local sector
+};
+
+struct Pivot2 {
+  Sym sym;
+  std::vector<Qn> qloc12, qloc34, qop12, qop34;
+  Tripod L, R;
+  std::vector<std::vector<std::vector<WBiped>>> W12, W34;  // [s1][s2][k]
+  VLayout V;
+
+  // PivotMatrix2::qlhs / qrhs / factor_cgcs (DmrgPivotMatrix2.h:66-70)
+  std::vector<std::array<size_t, 2>> qlhs;
+  std::vector<std::vector<std::array<size_t, 12>>> qrhs;
+  std::vector<std::vector<double>> factor_cgcs;
+  bool has_blockstructure = false;
+
+  // Excited-state projection (DmrgPivotMatrix2.h:72-75, :249-272): packed
+  // vectors in the V layout.
+  std::vector<std::vector<double>> A0proj;
+  double Epenalty = 0.0;
+
+  void init_W();
+  int add_V_block(int s, const Qn& qin, const Qn& qout, int64_t rows, int64_t cols);
+};
+
+// Restatement of precalc_blockStructure (two-site, DmrgIndexGymnastics.h:698-856)
+// with the AAWWAA matcher (:198-262).  The reference returns entries in
+// unordered_map order; here the (s1s3, q13) keys are emitted sorted, entries
+// under one key keep the reference's loop order.
+void precalc_blockStructure(Pivot2& P);
+
+// ---- staged block-triple plan --------------------------------------------
+// Terms tau = (out block o, L block l, V block v, R block r, coefficient c)
+// from the OxV loop (DmrgPivotMatrix2.h:182-230), regrouped as
+//   T_{l,v} = L_l . V_v              (distinct (l,v))
+//   U_{o,r} = sum_tau c_tau T_{l,v}  (distinct (o,r))
+//   Vout_o  = sum_r U_{o,r} . R_r
+struct GemmProb {     // C(M x N, ldc=M) = sum over chunks A(M x K) . B(K x N)
+  int64_t c_off;
+  int32_t M, N;
+  int32_t chunk_begin, chunk_end;
+};
+struct GemmChunk {
+  int64_t a_off, b_off;
+  int32_t K, lda, ldb;
+  int32_t pad;
+};
+struct CombineList {  // U_u = sum_j c_j T_{t_j}
+  int64_t u_off;
+  int64_t size;
+  int32_t begin, end;
+};
+struct Stats {
+  int64_t n_terms = 0, n_T = 0, n_U = 0, n_out = 0;
+  int64_t n_L = 0, n_R = 0, n_V = 0;
+  double flops_alg = 0, flops_ref = 0;
+  double flops_A = 0, flops_C = 0;  // stage A (L.V) and stage C (U.R) shares of flops_alg
+  double bytes_alg_fp64 = 0;  // 8 * (distinct L + distinct R + Vin + Vout elements)
+  int64_t L_elems = 0, R_elems = 0, V_elems = 0, T_elems = 0, U_elems = 0;
+  int64_t max_block_dim = 0;
+};
+
+struct HostPlan {
+  // packed operands actually referenced by terms
+  std::vector<double> Ldata, Rdata;  // copied L/R blocks
+  int64_t vsize = 0;
+  std::vector<GemmProb> probA;       // stage A: one chunk each (L . V)
+  std::vector<GemmChunk> chunkA;
+  std::vector<CombineList> comb;     // stage B
+  std::vector<int32_t> comb_t;       // T index per entry
+  std::vector<double> comb_c;        // coefficient per entry
+  std::vector<int64_t> T_off;        // element offset of T_t
+  int64_t T_size = 0, U_size = 0;
+  std::vector<GemmProb> probC;       // stage C: one problem per out block
+  std::vector<GemmChunk> chunkC;     // (U_u, R_r) chunks
+  // weights for the coeff_dot-weighted inner product: (offset, length, weight)
+  std::vector<int64_t> seg_off;
+  std::vector<int32_t> seg_len;
+  std::vector<double> seg_w;
+  Stats stats;
+  // block list of the vector (offsets/sizes) for NaN checks and checks of the layout
+  std::vector<int64_t> vblock_off, vblock_size;
+};
+
+// Literal per-term list, in the order the reference's OxV visits them.
+struct Term {
+  int64_t out_v, l_blk, in_v, r_blk;  // V block ids (global), L/R block ids
+  double c;
+};
+void expand_terms(const Pivot2& P, std::vector<Term>& terms,
+                  std::vector<std::pair<size_t, size_t>>& lblocks,
+                  std::vector<std::pair<size_t, size_t>>& rblocks,
+                  std::vector<std::pair<int, int>>& vblocks);
+
+void build_host_plan(const Pivot2& P, HostPlan& H);
+// Throws VMPS_ERR_INTERNAL if any descriptor would address outside its buffer.
+void validate_host_plan(const HostPlan& H);
+
+}  // namespace vmps

@@ -1,0 +1,275 @@
+"""Symmetries, local bases and matrix-product operators of the models the
+workloads use.
+
+The reference builds its MPOs generically (MpoTerms.h:1175-1233 lays out
+W[loc][s1][s2][k] as Bipeds of sparse matrices keyed by auxiliary sectors).
+Here the nearest-neighbour Hamiltonians are written directly as lower
+triangular operator-valued matrices; the auxiliary index a of W carries the
+charge accumulated to its left, so the block (q_in(a), q_out(b)) of
+W[s1][s2][k] holds <s1|O|s2> for an operator O of charge qOp[k] = q_out - q_in
+(the flow AAWWAA matches, DmrgIndexGymnastics.h:218-246).
+
+Models:
+  HeisenbergU1   S=1/2, H = J sum Sz Sz + J/2 (S+ S- + S- S+)   (models/HeisenbergU1.h)
+  HubbardU1xU1   H = -t sum_sigma (c+ c + h.c.) + U sum n_up n_dn (models/HubbardU1xU1.h)
+"""
+from dataclasses import dataclass, field
+import numpy as np
+
+U1_KIND = 1
+SU2_KIND = 2
+
+
+class Sym:
+    """Runtime symmetry: a tuple of U(1)/SU(2) components (symmetry/U1.h, SU2.h, S1xS2.h)."""
+
+    def __init__(self, kinds):
+        self.kinds = tuple(int(k) for k in kinds)
+        self.nq = len(self.kinds)
+
+    def reduce(self, a, b):
+        comps = []
+        for i, k in enumerate(self.kinds):
+            if k == U1_KIND:
+                comps.append([a[i] + b[i]])
+            else:
+                lo, hi = abs(a[i] - b[i]) + 1, abs(a[i] + b[i]) - 1
+                comps.append(list(range(lo, hi + 1, 2)))
+        out = [()]
+        for c in comps:
+            out = [o + (x,) for o in out for x in c]
+        return out
+
+    def validate3(self, a, b, c):
+        for i, k in enumerate(self.kinds):
+            if k == U1_KIND:
+                if a[i] + b[i] != c[i]:
+                    return False
+            else:
+                if not (c[i] - 1 >= abs(a[i] - b[i]) and c[i] - 1 <= a[i] + b[i] - 2):
+                    return False
+        return True
+
+    def add(self, a, b):
+        assert all(k == U1_KIND for k in self.kinds)
+        return tuple(x + y for x, y in zip(a, b))
+
+    def coeff_dot(self, q):
+        w = 1.0
+        for i, k in enumerate(self.kinds):
+            if k == SU2_KIND:
+                w *= q[i]
+        return w
+
+    @property
+    def non_abelian(self):
+        return any(k == SU2_KIND for k in self.kinds)
+
+
+U1 = Sym([U1_KIND])
+U1xU1 = Sym([U1_KIND, U1_KIND])
+
+
+@dataclass
+class MpoSite:
+    """One site tensor W[s1][s2][k] in block-sparse form.
+
+    aux_in / aux_out: lists of (q, dim) sectors of the left / right auxiliary
+    bond; a global auxiliary index enumerates the sectors in list order.
+    blocks: list of (s1, s2, k, q_in, q_out, rows, cols, row[], col[], val[]).
+    """
+    sym: Sym
+    qloc: list
+    qop: list
+    aux_in: list
+    aux_out: list
+    blocks: list = field(default_factory=list)
+
+    @property
+    def dw_in(self):
+        return sum(d for _, d in self.aux_in)
+
+    @property
+    def dw_out(self):
+        return sum(d for _, d in self.aux_out)
+
+    def dense(self):
+        """W as a dense array [a, b, s1, s2] over global auxiliary indices (abelian only)."""
+        off_in = _offsets(self.aux_in)
+        off_out = _offsets(self.aux_out)
+        d = len(self.qloc)
+        W = np.zeros((self.dw_in, self.dw_out, d, d))
+        for (s1, s2, k, qi, qo, rows, cols, r, c, v) in self.blocks:
+            for rr, cc, vv in zip(r, c, v):
+                W[off_in[qi] + rr, off_out[qo] + cc, s1, s2] += vv
+        return W
+
+
+def _offsets(sectors):
+    off, o = {}, 0
+    for q, d in sectors:
+        off[tuple(q)] = o
+        o += d
+    return off
+
+
+def _aux_sectors(qs):
+    """Group global auxiliary indices by charge, in order of first appearance.
+    Returns (sectors, local index of each global index)."""
+    order, local, count = [], [], {}
+    for q in qs:
+        q = tuple(q)
+        if q not in count:
+            count[q] = 0
+            order.append(q)
+        local.append(count[q])
+        count[q] += 1
+    # regroup so that sector blocks are contiguous in global numbering
+    sectors = [(q, count[q]) for q in order]
+    return sectors, local
+
+
+def build_mpo_site(sym, qloc, aux_q_in, aux_q_out, entries, tol=0.0):
+    """Operator-valued matrix -> block-sparse W.
+
+    entries: list of (a, b, op, qop) with op a dense d x d matrix <s1|op|s2>
+    of charge qop, placed at global auxiliary position (a, b)."""
+    aux_in, loc_in = _aux_sectors(aux_q_in)
+    aux_out, loc_out = _aux_sectors(aux_q_out)
+    qops = []
+    for (_, _, _, qop) in entries:
+        if tuple(qop) not in qops:
+            qops.append(tuple(qop))
+    acc = {}
+    for (a, b, op, qop) in entries:
+        qa, qb = tuple(aux_q_in[a]), tuple(aux_q_out[b])
+        assert sym.validate3(qa, tuple(qop), qb), "operator charge does not match the auxiliary flow"
+        k = qops.index(tuple(qop))
+        for s1 in range(len(qloc)):
+            for s2 in range(len(qloc)):
+                v = op[s1, s2]
+                if abs(v) <= tol:
+                    continue
+                assert sym.validate3(tuple(qloc[s2]), tuple(qop), tuple(qloc[s1]))
+                key = (s1, s2, k, qa, qb)
+                acc.setdefault(key, {})
+                acc[key][(loc_in[a], loc_out[b])] = acc[key].get((loc_in[a], loc_out[b]), 0.0) + v
+    dim_in = dict(aux_in)
+    dim_out = dict(aux_out)
+    blocks = []
+    for (s1, s2, k, qa, qb), nz in sorted(acc.items()):
+        items = sorted(nz.items(), key=lambda kv: (kv[0][1], kv[0][0]))
+        r = np.array([i[0][0] for i in items], dtype=np.int32)
+        c = np.array([i[0][1] for i in items], dtype=np.int32)
+        v = np.array([i[1] for i in items], dtype=np.float64)
+        blocks.append((s1, s2, k, qa, qb, dim_in[qa], dim_out[qb], r, c, v))
+    return MpoSite(sym, [tuple(q) for q in qloc], qops, aux_in, aux_out, blocks)
+
+
+def _select(entries, rows, cols, nrow_new=None, ncol_new=None):
+    """Restrict an operator-valued matrix to rows/cols (boundary sites)."""
+    rmap = {r: i for i, r in enumerate(rows)} if rows is not None else None
+    cmap = {c: i for i, c in enumerate(cols)} if cols is not None else None
+    out = []
+    for (a, b, op, q) in entries:
+        if rmap is not None and a not in rmap:
+            continue
+        if cmap is not None and b not in cmap:
+            continue
+        out.append((rmap[a] if rmap is not None else a, cmap[b] if cmap is not None else b, op, q))
+    return out
+
+
+# ---------------------------------------------------------------------------
+# HeisenbergU1, S = 1/2.  qloc in units of 2Sz: state 0 = down (-1), 1 = up (+1)
+# ---------------------------------------------------------------------------
+def heisenberg_u1_ops():
+    Sp = np.array([[0.0, 0.0], [1.0, 0.0]])  # <s1|S+|s2>: up <- down
+    Sm = Sp.T.copy()
+    Sz = np.diag([-0.5, 0.5])
+    Id = np.eye(2)
+    qloc = [(-1,), (1,)]
+    return qloc, Id, Sp, Sm, Sz
+
+
+def heisenberg_u1_mpo(J=1.0, position="bulk"):
+    """D_W = 5 lower-triangular MPO.  Global auxiliary order
+    0 done | 1 S+ pending (charge -2) | 2 S- pending (+2) | 3 Sz pending | 4 start."""
+    qloc, Id, Sp, Sm, Sz = heisenberg_u1_ops()
+    q = [(0,), (-2,), (2,), (0,), (0,)]
+    ent = [
+        (0, 0, Id, (0,)),
+        (1, 0, Sp, (2,)),
+        (2, 0, Sm, (-2,)),
+        (3, 0, Sz, (0,)),
+        (4, 1, 0.5 * J * Sm, (-2,)),
+        (4, 2, 0.5 * J * Sp, (2,)),
+        (4, 3, J * Sz, (0,)),
+        (4, 4, Id, (0,)),
+    ]
+    return _place(U1, qloc, q, ent, position)
+
+
+# ---------------------------------------------------------------------------
+# HubbardU1xU1.  qloc (N, 2Sz): |0>, |up>, |dn>, |updn> = a+_up a+_dn |0>
+# ---------------------------------------------------------------------------
+def hubbard_ops():
+    a_up = np.zeros((4, 4))
+    a_dn = np.zeros((4, 4))
+    a_up[0, 1] = 1.0   # |up> -> |0>
+    a_up[2, 3] = 1.0   # |updn> -> |dn>
+    a_dn[0, 2] = 1.0   # |dn> -> |0>
+    a_dn[1, 3] = -1.0  # |updn> -> -|up>
+    F = np.diag([1.0, -1.0, -1.0, 1.0])
+    Id = np.eye(4)
+    n_up = a_up.T @ a_up
+    n_dn = a_dn.T @ a_dn
+    qloc = [(0, 0), (1, 1), (1, -1), (2, 0)]
+    return qloc, Id, F, a_up, a_dn, n_up, n_dn
+
+
+def hubbard_u1xu1_mpo(t=1.0, U=8.0, position="bulk"):
+    """D_W = 6 lower-triangular MPO with Jordan-Wigner strings:
+    c+_i c_{i+1} = (a+_i F_i) a_{i+1},  c+_{i+1} c_i = (F_i a_i) a+_{i+1}.
+    Global auxiliary order: 0 done | 1 (a+F)_up | 2 (F a)_up | 3 (a+F)_dn | 4 (F a)_dn | 5 start."""
+    qloc, Id, F, a_up, a_dn, n_up, n_dn = hubbard_ops()
+    q = [(0, 0), (1, 1), (-1, -1), (1, -1), (-1, 1), (0, 0)]
+    ent = [
+        (0, 0, Id, (0, 0)),
+        (1, 0, a_up, (-1, -1)),
+        (2, 0, a_up.T.copy(), (1, 1)),
+        (3, 0, a_dn, (-1, 1)),
+        (4, 0, a_dn.T.copy(), (1, -1)),
+        (5, 0, U * (n_up @ n_dn), (0, 0)),
+        (5, 1, -t * (a_up.T @ F), (1, 1)),
+        (5, 2, -t * (F @ a_up), (-1, -1)),
+        (5, 3, -t * (a_dn.T @ F), (1, -1)),
+        (5, 4, -t * (F @ a_dn), (-1, 1)),
+        (5, 5, Id, (0, 0)),
+    ]
+    return _place(U1xU1, qloc, q, ent, position)
+
+
+def _place(sym, qloc, q, ent, position):
+    n = len(q)
+    if position == "bulk":
+        return build_mpo_site(sym, qloc, q, q, ent)
+    if position == "first":   # row vector: start row only
+        e = _select(ent, [n - 1], None)
+        return build_mpo_site(sym, qloc, [q[n - 1]], q, e)
+    if position == "last":    # column vector: done column only
+        e = _select(ent, None, [0])
+        return build_mpo_site(sym, qloc, q, [q[0]], e)
+    if position == "single":
+        e = _select(ent, [n - 1], [0])
+        return build_mpo_site(sym, qloc, [q[n - 1]], [q[0]], e)
+    raise ValueError(position)
+
+
+def chain_mpo(model, L, **params):
+    """Full open-boundary MPO as a list of MpoSite."""
+    f = {"HeisenbergU1": heisenberg_u1_mpo, "HubbardU1xU1": hubbard_u1xu1_mpo}[model]
+    if L == 1:
+        return [f(position="single", **params)]
+    return [f(position="first", **params)] + [f(position="bulk", **params) for _ in range(L - 2)] + \
+           [f(position="last", **params)]

@@ -1,0 +1,119 @@
+"""Python face of the engine, mirroring the reference's pivot API.
+
+    H = PivotMatrix2(pivot_data)          # PivotMatrix2 + precalc_blockStructure
+    Vout = HxV(H, Vin)                    # HxV(PivotMatrix2, PivotVector, PivotVector&)
+    dot(H, V1, V2)                        # dot(PivotVector, PivotVector) with coeff_dot weights
+
+(pivot/DmrgPivotMatrix2.h:29-83, pivot/DmrgPivotVector.h:296-358.)  Vectors
+are packed FP64 numpy arrays in the pivot's block order.  All arithmetic runs
+in the HIP library; there is no Python or CPU fallback.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from . import synth
+
+P_dbl = ctypes.POINTER(ctypes.c_double)
+
+
+def _ptr(a):
+    return a.ctypes.data_as(P_dbl)
+
+
+class PivotMatrix2:
+    """Two-site effective Hamiltonian resident on the current HIP device."""
+
+    def __init__(self, pd, precision="fp64", blockstructure=None, projectors=None, Epenalty=0.0):
+        lib = _lib.load()
+        self.pd = pd
+        self._lib = lib
+        h = synth.feed(lib, "vmps_", pd, _lib.check, projectors=projectors, Epenalty=Epenalty)
+        try:
+            if blockstructure is None:
+                _lib.check(lib.vmps_pivot2_precalc(h))
+            else:
+                set_blockstructure(lib, h, *blockstructure)
+            self.blockstructure = get_blockstructure(lib, h)
+            vs = ctypes.c_int64()
+            _lib.check(lib.vmps_pivot2_vsize(h, ctypes.byref(vs)))
+            self.vsize = vs.value
+            plan = ctypes.c_void_p()
+            _lib.check(lib.vmps_plan_create(h, _lib.PREC[precision], ctypes.byref(plan)))
+            self._plan = plan
+        finally:
+            lib.vmps_pivot2_destroy(h)
+        self.precision = precision
+
+    def close(self):
+        if getattr(self, "_plan", None):
+            self._lib.vmps_plan_destroy(self._plan)
+            self._plan = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_precision(self, precision):
+        _lib.check(self._lib.vmps_plan_set_precision(self._plan, _lib.PREC[precision]))
+        self.precision = precision
+
+    def stats(self):
+        st = _lib.PlanStats()
+        _lib.check(self._lib.vmps_plan_stats_get(self._plan, ctypes.byref(st)))
+        return st.as_dict()
+
+    def time_stages(self, warmup=3, iters=20):
+        ms = (ctypes.c_double * 4)()
+        _lib.check(self._lib.vmps_plan_time_stages(self._plan, warmup, iters, ms))
+        return list(ms)
+
+    def time_hxv(self, warmup=3, iters=20):
+        ms = ctypes.c_double()
+        _lib.check(self._lib.vmps_plan_time_hxv(self._plan, warmup, iters, ctypes.byref(ms)))
+        return ms.value
+
+
+def HxV(H, vin):
+    vin = np.ascontiguousarray(vin, dtype=np.float64)
+    if vin.size != H.vsize:
+        raise ValueError("vector size %d does not match the pivot (%d)" % (vin.size, H.vsize))
+    vout = np.empty_like(vin)
+    _lib.check(H._lib.vmps_hxv(H._plan, _ptr(vin), _ptr(vout)))
+    return vout
+
+
+def dot(H, x, y):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    r = ctypes.c_double()
+    _lib.check(H._lib.vmps_dot(H._plan, _ptr(x), _ptr(y), ctypes.byref(r)))
+    return r.value
+
+
+def get_blockstructure(lib, h, prefix="vmps_"):
+    fn = getattr(lib, prefix + "pivot2_get_blockstructure")
+    nl, nr = ctypes.c_int64(), ctypes.c_int64()
+    _lib.check(fn(h, ctypes.byref(nl), ctypes.byref(nr), None, None, None, None))
+    qlhs = np.zeros(2 * nl.value, dtype=np.int64)
+    rb = np.zeros(nl.value + 1, dtype=np.int64)
+    qrhs = np.zeros(12 * nr.value, dtype=np.int64)
+    fac = np.zeros(nr.value, dtype=np.float64)
+    P64 = ctypes.POINTER(ctypes.c_int64)
+    _lib.check(fn(h, ctypes.byref(nl), ctypes.byref(nr), qlhs.ctypes.data_as(P64), rb.ctypes.data_as(P64),
+                  qrhs.ctypes.data_as(P64), fac.ctypes.data_as(P_dbl)))
+    return qlhs.reshape(-1, 2), rb, qrhs.reshape(-1, 12), fac
+
+
+def set_blockstructure(lib, h, qlhs, rhs_begin, qrhs, factors, prefix="vmps_"):
+    fn = getattr(lib, prefix + "pivot2_set_blockstructure")
+    P64 = ctypes.POINTER(ctypes.c_int64)
+    qlhs = np.ascontiguousarray(qlhs, dtype=np.int64).ravel()
+    rb = np.ascontiguousarray(rhs_begin, dtype=np.int64)
+    qrhs = np.ascontiguousarray(qrhs, dtype=np.int64).ravel()
+    fac = np.ascontiguousarray(factors, dtype=np.float64)
+    _lib.check(fn(h, len(rb) - 1, qlhs.ctypes.data_as(P64), rb.ctypes.data_as(P64), qrhs.ctypes.data_as(P64),
+                  fac.ctypes.data_as(P_dbl)))

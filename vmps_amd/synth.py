@@ -1,0 +1,268 @@
+"""Synthetic two-site pivots of the benchmark workloads, and the routine that
+hands a pivot to a library through the builder calls of include/vmps_hxv.h.
+
+A pivot is described by PivotData: the two MPO site tensors, the sectors of
+the left and right bonds, dense environments and the block layout of the
+wavefunction.  Environments are random N(0,1) blocks for every allowed
+(in, out, mid) sector; the wavefunction layout has every block allowed by the
+symmetry (extend_A materialises all of them, DmrgContractions.h:1832-1867).
+"""
+import ctypes
+from dataclasses import dataclass, field
+import math
+
+import numpy as np
+
+from . import models
+
+
+# ---------------------------------------------------------------------------
+def two_site_basis(sym, qloc12, qloc34):
+    """Combined basis loc12.combine(loc34) of the reference (Qbasis.h:686-798):
+    states sorted by (qmerge, q1, q3), multiplicities of repeated local quantum
+    numbers kept adjacent.  Returns list of (qmerge, s1, s3) in s order."""
+    states = []
+    for s1, q1 in enumerate(qloc12):
+        for s3, q3 in enumerate(qloc34):
+            for qm in sym.reduce(q1, q3):
+                states.append((tuple(qm), tuple(q1), tuple(q3), s1, s3))
+    # within equal (qm, q1, q3): plain index p1 + dim1*p3 (first basis fastest)
+    states.sort(key=lambda x: (x[0], x[1], x[2], x[4], x[3]))
+    return [(st[0], st[3], st[4]) for st in states]
+
+
+@dataclass
+class PivotData:
+    sym: models.Sym
+    W12: models.MpoSite
+    W34: models.MpoSite
+    left: list          # [(q, dim)] bond sectors left of site 1
+    right: list         # [(q, dim)] bond sectors right of site 2
+    L: list = field(default_factory=list)   # [(qin, qout, qmid, [F-ordered blocks])]
+    R: list = field(default_factory=list)
+    V: list = field(default_factory=list)   # per s: [(qin, qout, rows, cols)]
+    basis: list = field(default_factory=list)
+
+    @property
+    def vsize(self):
+        return sum(r * c for blocks in self.V for (_, _, r, c) in blocks)
+
+    def block_offsets(self):
+        """[(s, qin, qout, offset, rows, cols)] in packed order."""
+        out, off = [], 0
+        for s, blocks in enumerate(self.V):
+            for (qi, qo, r, c) in blocks:
+                out.append((s, qi, qo, off, r, c))
+                off += r * c
+        return out
+
+
+def _flow_ok(sym, a, b, c):
+    return sym.validate3(a, b, c)
+
+
+def make_pivot(W12, W34, left, right, rng, env_scale=1.0):
+    """Random-environment pivot for sites with MPO tensors W12, W34."""
+    sym = W12.sym
+    pd = PivotData(sym, W12, W34, [(tuple(q), d) for q, d in left], [(tuple(q), d) for q, d in right])
+    ldim = dict(pd.left)
+    rdim = dict(pd.right)
+    # L: {in (bra), out (ket), mid} with out (x) mid -> in  (DmrgContractions.h:30-33)
+    for (qmid, adim) in W12.aux_in:
+        for (qout, dout) in pd.left:
+            for qin in sym.reduce(qout, qmid):
+                if qin in ldim:
+                    blocks = [np.asfortranarray(rng.standard_normal((ldim[qin], dout)) * env_scale) for _ in range(adim)]
+                    pd.L.append((qin, qout, tuple(qmid), blocks))
+    # R: {in (ket), out (bra), mid} with in (x) mid -> out
+    for (qmid, adim) in W34.aux_out:
+        for (qin, din) in pd.right:
+            for qout in sym.reduce(qin, qmid):
+                if qout in rdim:
+                    blocks = [np.asfortranarray(rng.standard_normal((din, rdim[qout])) * env_scale) for _ in range(adim)]
+                    pd.R.append((qin, qout, tuple(qmid), blocks))
+    pd.basis = two_site_basis(sym, W12.qloc, W34.qloc)
+    pd.V = []
+    for (qm, s1, s3) in pd.basis:
+        blocks = []
+        for (qin, din) in pd.left:
+            for qout in sym.reduce(qin, qm):
+                if qout in rdim:
+                    blocks.append((qin, tuple(qout), din, rdim[qout]))
+        pd.V.append(blocks)
+    return pd
+
+
+# ---------------------------------------------------------------------------
+# bond sectors
+# ---------------------------------------------------------------------------
+def gaussian_sectors(centers, sigmas, M, parity_fn=None, span=None, max_dim=None, min_dim=1):
+    """Sectors q on a grid around `centers` with dims ~ M * gaussian weight.
+
+    centers/sigmas: per component; parity_fn(q) -> bool keeps allowed q;
+    span: per component half-width of the grid."""
+    nq = len(centers)
+    span = span or [int(4 * s) + 2 for s in sigmas]
+    grids = [range(int(c) - w, int(c) + w + 1) for c, w in zip(centers, span)]
+    qs = [()]
+    for g in grids:
+        qs = [q + (x,) for q in qs for x in g]
+    qs = [q for q in qs if (parity_fn is None or parity_fn(q))]
+    w = np.array([math.exp(-sum((q[i] - centers[i]) ** 2 / (2 * sigmas[i] ** 2) for i in range(nq))) for q in qs])
+    dims = np.floor(M * w / w.sum()).astype(int)
+    keep = dims >= min_dim
+    qs = [q for q, k in zip(qs, keep) if k]
+    dims = dims[keep]
+    w = w[keep]
+    if max_dim:
+        dims = np.minimum(dims, max_dim)
+    # distribute the remainder to the largest-weight sectors
+    order = np.argsort(-w)
+    i = 0
+    while dims.sum() < M:
+        j = order[i % len(order)]
+        if not max_dim or dims[j] < max_dim:
+            dims[j] += 1
+        i += 1
+        if i > 100 * len(order):
+            break
+    while dims.sum() > M:
+        j = order[-1 - (i % len(order))]
+        if dims[j] > 1:
+            dims[j] -= 1
+        i += 1
+    return [(q, int(d)) for q, d in zip(qs, dims) if d > 0]
+
+
+def heisenberg_bond(nsites_left, M):
+    """2Sz sectors of the bond after `nsites_left` spins 1/2, centred on 0."""
+    par = nsites_left % 2
+    sigma = max(1.0, 0.55 * math.sqrt(nsites_left) * 1.6)
+    return gaussian_sectors([0], [sigma], M, parity_fn=lambda q: (q[0] - par) % 2 == 0 and abs(q[0]) <= nsites_left,
+                            span=[min(nsites_left, int(5 * sigma) + 2)])
+
+
+def stretched_sectors(center, M, alpha, lam, allowed):
+    """Sectors with dims ~ exp(-(r/lam)^alpha), r = distance from `center`,
+    rounded to integers and scaled so the dims sum to M (heavy tail: many
+    sectors keep one or two states, a few central ones keep tens)."""
+    qs = [q for q in allowed]
+    r = np.array([math.sqrt(sum((qi - ci) ** 2 for qi, ci in zip(q, center))) for q in qs])
+    w = np.exp(-(r / lam) ** alpha)
+    lo, hi = 1e-6, 1e12
+    for _ in range(200):
+        sc = math.sqrt(lo * hi)
+        if np.floor(w * sc + 0.5).sum() > M:
+            hi = sc
+        else:
+            lo = sc
+    dims = np.floor(w * lo + 0.5).astype(int)
+    order = np.argsort(-w)
+    i = 0
+    while dims.sum() < M:  # top up the central sectors
+        dims[order[i % 4]] += 1
+        i += 1
+    return [(tuple(q), int(d)) for q, d in zip(qs, dims) if d > 0]
+
+
+def hubbard_bond(nsites_left, M, filling=1.0, alpha=0.36, lam=0.02):
+    """(N, 2Sz) sectors of the bond after `nsites_left` Hubbard sites.  With the
+    defaults and M = 512: ~110 sectors of dims 1..~55."""
+    n0 = round(filling * nsites_left)
+    allowed = [(n, s) for n in range(0, 2 * nsites_left + 1) for s in range(-nsites_left, nsites_left + 1)
+               if (n - s) % 2 == 0 and abs(s) <= min(n, 2 * nsites_left - n)]
+    return stretched_sectors((n0, 0), M, alpha, lam, allowed)
+
+
+# ---------------------------------------------------------------------------
+# workloads
+# ---------------------------------------------------------------------------
+WORKLOADS = {
+    # two-site pivot in the middle of a HeisenbergU1 chain, L=32, M=64
+    "heisenberg_u1_L32_M64": dict(model="HeisenbergU1", L=32, M=64),
+    # HubbardU1xU1 chain L=48, half filling, M=512, D_W=6
+    "hubbard_u1xu1_L48_M512": dict(model="HubbardU1xU1", L=48, M=512),
+}
+
+
+def make_workload(name, seed=20261202, **override):
+    cfg = dict(WORKLOADS[name])
+    cfg.update(override)
+    rng = np.random.default_rng(seed)
+    L = cfg["L"]
+    loc = L // 2 - 1  # pivot sites (loc, loc+1)
+    if cfg["model"] == "HeisenbergU1":
+        W = models.heisenberg_u1_mpo(J=cfg.get("J", 1.0))
+        left = heisenberg_bond(loc, cfg["M"])
+        right = heisenberg_bond(loc + 2, cfg["M"])
+    elif cfg["model"] == "HubbardU1xU1":
+        W = models.hubbard_u1xu1_mpo(t=cfg.get("t", 1.0), U=cfg.get("U", 8.0))
+        left = hubbard_bond(loc, cfg["M"])
+        right = hubbard_bond(loc + 2, cfg["M"])
+    else:
+        raise ValueError(cfg["model"])
+    return make_pivot(W, W, left, right, rng)
+
+
+# ---------------------------------------------------------------------------
+# handing a pivot to a library
+# ---------------------------------------------------------------------------
+def _iarr(seq):
+    a = np.ascontiguousarray(np.array(seq, dtype=np.int32).ravel())
+    return a, a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+
+
+def feed(lib, prefix, pd, check, projectors=None, Epenalty=0.0):
+    """Build the pivot in `lib` through the `<prefix>pivot2_*` builder calls
+    (identical signatures for the engine and the oracle).  Returns the handle."""
+    fn = lambda name: getattr(lib, prefix + name)
+    sym = pd.sym
+    kinds, pk = _iarr(sym.kinds)
+    q12, p12 = _iarr(pd.W12.qloc)
+    q34, p34 = _iarr(pd.W34.qloc)
+    o12, po12 = _iarr(pd.W12.qop)
+    o34, po34 = _iarr(pd.W34.qop)
+    h = ctypes.c_void_p()
+    check(fn("pivot2_create")(sym.nq, pk, len(pd.W12.qloc), p12, len(pd.W34.qloc), p34, len(pd.W12.qop), po12,
+                              len(pd.W34.qop), po34, ctypes.byref(h)))
+    keep = []
+    for which, env in ((0, pd.L), (1, pd.R)):
+        for (qin, qout, qmid, blocks) in env:
+            a, pa = _iarr(qin)
+            b, pb = _iarr(qout)
+            c, pc = _iarr(qmid)
+            rows = np.array([blk.shape[0] if blk is not None else 0 for blk in blocks], dtype=np.int64)
+            cols = np.array([blk.shape[1] if blk is not None else 0 for blk in blocks], dtype=np.int64)
+            data = np.concatenate([blk.ravel(order="F") for blk in blocks if blk is not None] or [np.zeros(0)])
+            data = np.ascontiguousarray(data, dtype=np.float64)
+            keep.append((a, b, c, rows, cols, data))
+            check(fn("pivot2_add_env")(h, which, pa, pb, pc, len(blocks),
+                                       rows.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                                       cols.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                                       data.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+    for which, W in ((12, pd.W12), (34, pd.W34)):
+        for (s1, s2, k, qi, qo, rows, cols, r, c, v) in W.blocks:
+            a, pa = _iarr(qi)
+            b, pb = _iarr(qo)
+            r = np.ascontiguousarray(r, dtype=np.int32)
+            c = np.ascontiguousarray(c, dtype=np.int32)
+            v = np.ascontiguousarray(v, dtype=np.float64)
+            check(fn("pivot2_add_W")(h, which, s1, s2, k, pa, pb, rows, cols, len(v),
+                                     r.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                     c.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                     v.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+    off = ctypes.c_int64()
+    for s, blocks in enumerate(pd.V):
+        for (qi, qo, rr, cc) in blocks:
+            a, pa = _iarr(qi)
+            b, pb = _iarr(qo)
+            check(fn("pivot2_add_V_block")(h, s, pa, pb, rr, cc, ctypes.byref(off)))
+    if projectors is not None and len(projectors):
+        A0 = np.ascontiguousarray(np.concatenate([np.asarray(x, dtype=np.float64) for x in projectors]))
+        check(fn("pivot2_set_projectors")(h, float(Epenalty), len(projectors),
+                                          A0.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+    return h
+
+
+def random_vector(pd, seed):
+    return np.random.default_rng(seed).standard_normal(pd.vsize)
