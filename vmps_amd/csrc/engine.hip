@@ -35,11 +35,26 @@ __device__ inline void mfma_16x16x4<float>(float a, float b, v4f& acc) {
   acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc, 0, 0, 0);
 }
 
+// Row of the 16 x 16 accumulator held in element v of `lane` (column lane%16).
+// Measured on gfx950: f32 16x16x4 and bf16 16x16x32 hold row 4*(lane/16)+v,
+// f64 16x16x4 holds row lane/16 + 4*v.  Operands: A[lane%16][k], B[k][lane%16]
+// with k = lane/16 (16x16x4) or k = 8*(lane/16)+slot (16x16x32).
+template <class T>
+__device__ inline int acc_row(int lane, int v);
+template <>
+__device__ inline int acc_row<double>(int lane, int v) {
+  return (lane >> 4) + 4 * v;
+}
+template <>
+__device__ inline int acc_row<float>(int lane, int v) {
+  return 4 * (lane >> 4) + v;
+}
+
 // Grouped GEMM over a tile list.  C (M x N, column-major, ldc = M) of problem
 // p is  sum over its chunks of  A (M x K, lda) . B (K x N, ldb).
 // 256 threads = 4 waves; wave w owns the 16 x 16 quadrant (w&1, w>>1) of the
 // 32 x 32 tile.  MFMA 16x16x4 operand layout: lane l supplies A[l%16][l/16]
-// and B[l/16][l%16]; accumulator element v of lane l is C[4*(l/16)+v][l%16].
+// and B[l/16][l%16]; accumulator element v of lane l is C[acc_row(l,v)][l%16].
 // OutT is the element type written to C (the accumulator is converted).
 template <class T, class OutT>
 __global__ __launch_bounds__(256) void k_gemm_grouped(const Tile* __restrict__ tiles, const GemmProb* __restrict__ probs,
@@ -82,7 +97,7 @@ __global__ __launch_bounds__(256) void k_gemm_grouped(const Tile* __restrict__ t
   OutT* C = Cbase + p.c_off;
 #pragma unroll
   for (int v = 0; v < 4; ++v) {
-    const int gm = m0 + wm + 4 * (lane >> 4) + v;
+    const int gm = m0 + wm + acc_row<T>(lane, v);
     const int gn = n0 + wn + (lane & 15);
     if (gm < p.M && gn < p.N) C[(int64_t)gn * p.M + gm] = (OutT)acc[v];
   }
