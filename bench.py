@@ -120,8 +120,8 @@ def main():
     t_dom = ms[0] if dom == "A" else ms[2]
     achieved = stage_flops[dom] / (t_dom * 1e-3) / 1e12
     peak = PEAK_TFLOPS[args.precision]
-    kname = {"fp64": "k_gemm_grouped<double, double>", "fp32": "k_gemm_grouped<float, float>"}.get(args.precision,
-                                                                                                   "k_gemm_grouped")
+    ktype = {"fp64": "double", "fp32": "float", "bf16x3": "bf16x3"}[args.precision]
+    kname = ("k_stage_a<%s> (T = L.V)" % ktype) if dom == "A" else ("k_stage_c<%s> ((sum c T).R)" % ktype)
 
     res = {
         "metric": "DMRG HxV GFLOP/s (F_alg / t), plain FP64 GPU",
@@ -148,10 +148,10 @@ def main():
             "flops_alg": st["flops_alg"], "flops_ref": st["flops_ref"],
             "parallelism": "single GPU",
         },
-        "stage_ms": {"A_LV": ms[0], "B_combine": ms[1], "C_UR": ms[2], "hxv": ms[3]},
+        "stage_ms": {"A_LV": ms[0], "C_combine_UR": ms[2], "hxv": ms[3]},
         "roofline": {
             "bound": "mfma",
-            "kernel": kname + " stage " + dom,
+            "kernel": kname,
             "achieved": achieved,
             "peak": peak,
             "unit": "TFLOP/s",

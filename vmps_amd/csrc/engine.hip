@@ -1,11 +1,15 @@
 // Device engine for the staged effective-Hamiltonian application on gfx950.
 //
-//   stage A  T_t   = L_l . V_v            grouped GEMM, one problem per distinct (l, v)
-//   stage B  U_u   = sum_j c_j T_{t_j}     scaled gather-sum, one list per distinct (out, r)
-//   stage C  out_o = sum_u U_u . R_{r(u)}  grouped GEMM with a K-list per output block
+//   stage A   T_t   = L_l . V_v                     one problem per distinct (l, v)
+//   stage C   out_o = sum_r (sum_j c_j T_j) . R_r    one problem per output block;
+//             the scalar combine U = sum c T is done while loading the MFMA operand
 //
-// Every output element is owned by exactly one workgroup in every stage, so
-// there are no atomics and results are bitwise reproducible run to run.
+// Work unit = one 64-lane wavefront computing one 32 x 32 output tile with four
+// 16x16 MFMA accumulators.  Operands go from global memory straight to
+// registers (every element of a wave's A and B tiles is used by exactly one
+// MFMA, so staging in LDS would buy no reuse); the tile descriptor is a single
+// load.  Every output element is owned by exactly one wave, so there are no
+// atomics and results are bitwise reproducible run to run.
 #include "engine.hpp"
 
 #include <algorithm>
@@ -19,106 +23,172 @@ typedef float v4f __attribute__((ext_vector_type(4)));
 
 namespace {
 
-constexpr int kTM = 32;  // output tile rows
-constexpr int kTN = 32;  // output tile columns
-constexpr int kKB = 16;  // K depth staged in LDS per step
-
 template <class T>
-__device__ inline void mfma_16x16x4(T a, T b, typename std::conditional<std::is_same<T, double>::value, v4d, v4f>::type& acc);
+struct Mfma;
 
+// The GEMM C = A.B is evaluated as C^T = B^T . A^T: the MFMA A operand is
+// B[k][j] (lane l: j = l%16, k = l/16) and the MFMA B operand is A[i][k]
+// (i = l%16, k = l/16).  Accumulator element v of lane l then holds
+// C[i = l%16][j = row(l, v)], so a store instruction covers 16 consecutive
+// rows of a column-major block.  row(l, v) as measured on gfx950:
+// f64 16x16x4 -> l/16 + 4v, f32 16x16x4 -> 4(l/16) + v.
 template <>
-__device__ inline void mfma_16x16x4<double>(double a, double b, v4d& acc) {
-  acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
-}
-template <>
-__device__ inline void mfma_16x16x4<float>(float a, float b, v4f& acc) {
-  acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc, 0, 0, 0);
-}
-
-// Row of the 16 x 16 accumulator held in element v of `lane` (column lane%16).
-// Measured on gfx950: f32 16x16x4 and bf16 16x16x32 hold row 4*(lane/16)+v,
-// f64 16x16x4 holds row lane/16 + 4*v.  Operands: A[lane%16][k], B[k][lane%16]
-// with k = lane/16 (16x16x4) or k = 8*(lane/16)+slot (16x16x32).
-template <class T>
-__device__ inline int acc_row(int lane, int v);
-template <>
-__device__ inline int acc_row<double>(int lane, int v) {
-  return (lane >> 4) + 4 * v;
-}
-template <>
-__device__ inline int acc_row<float>(int lane, int v) {
-  return 4 * (lane >> 4) + v;
-}
-
-// Grouped GEMM over a tile list.  C (M x N, column-major, ldc = M) of problem
-// p is  sum over its chunks of  A (M x K, lda) . B (K x N, ldb).
-// 256 threads = 4 waves; wave w owns the 16 x 16 quadrant (w&1, w>>1) of the
-// 32 x 32 tile.  MFMA 16x16x4 operand layout: lane l supplies A[l%16][l/16]
-// and B[l/16][l%16]; accumulator element v of lane l is C[acc_row(l,v)][l%16].
-// OutT is the element type written to C (the accumulator is converted).
-template <class T, class OutT>
-__global__ __launch_bounds__(256) void k_gemm_grouped(const Tile* __restrict__ tiles, const GemmProb* __restrict__ probs,
-                                                      const GemmChunk* __restrict__ chunks, const T* __restrict__ Abase,
-                                                      const T* __restrict__ Bbase, OutT* __restrict__ Cbase) {
-  typedef typename std::conditional<std::is_same<T, double>::value, v4d, v4f>::type acc_t;
-  __shared__ T As[kKB][kTM + 1];
-  __shared__ T Bs[kKB][kTN + 1];
-  const Tile t = tiles[blockIdx.x];
-  const GemmProb p = probs[t.prob];
-  const int m0 = t.mt * kTM, n0 = t.nt * kTN;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wm = (wave & 1) * 16, wn = (wave >> 1) * 16;
-  acc_t acc = {0, 0, 0, 0};
-  for (int c = p.chunk_begin; c < p.chunk_end; ++c) {
-    const GemmChunk ch = chunks[c];
-    const T* A = Abase + ch.a_off;
-    const T* B = Bbase + ch.b_off;
-    for (int k0 = 0; k0 < ch.K; k0 += kKB) {
-      for (int i = tid; i < kTM * kKB; i += 256) {
-        const int m = i % kTM, k = i / kTM;
-        const int gm = m0 + m, gk = k0 + k;
-        As[k][m] = (gm < p.M && gk < ch.K) ? A[(int64_t)gk * ch.lda + gm] : T(0);
-      }
-      for (int i = tid; i < kKB * kTN; i += 256) {
-        const int k = i % kKB, n = i / kKB;
-        const int gk = k0 + k, gn = n0 + n;
-        Bs[k][n] = (gk < ch.K && gn < p.N) ? B[(int64_t)gn * ch.ldb + gk] : T(0);
-      }
-      __syncthreads();
-#pragma unroll
-      for (int kk = 0; kk < kKB; kk += 4) {
-        const T a = As[kk + (lane >> 4)][wm + (lane & 15)];
-        const T b = Bs[kk + (lane >> 4)][wn + (lane & 15)];
-        mfma_16x16x4<T>(a, b, acc);
-      }
-      __syncthreads();
-    }
+struct Mfma<double> {
+  typedef v4d acc_t;
+  __device__ static inline void mac(double b, double a, acc_t& c) {
+    c = __builtin_amdgcn_mfma_f64_16x16x4f64(b, a, c, 0, 0, 0);
   }
-  OutT* C = Cbase + p.c_off;
+  __device__ static inline int col(int lane, int v) { return (lane >> 4) + 4 * v; }
+};
+template <>
+struct Mfma<float> {
+  typedef v4f acc_t;
+  __device__ static inline void mac(float b, float a, acc_t& c) {
+    c = __builtin_amdgcn_mfma_f32_16x16x4f32(b, a, c, 0, 0, 0);
+  }
+  __device__ static inline int col(int lane, int v) { return 4 * (lane >> 4) + v; }
+};
+
+template <class T, class OutT>
+__device__ inline void store_tile(OutT* __restrict__ C, int M, int N, int m0, int n0, int lane,
+                                  const typename Mfma<T>::acc_t& c00, const typename Mfma<T>::acc_t& c01,
+                                  const typename Mfma<T>::acc_t& c10, const typename Mfma<T>::acc_t& c11) {
+  const int i0 = m0 + (lane & 15), i1 = i0 + 16;
 #pragma unroll
   for (int v = 0; v < 4; ++v) {
-    const int gm = m0 + wm + acc_row<T>(lane, v);
-    const int gn = n0 + wn + (lane & 15);
-    if (gm < p.M && gn < p.N) C[(int64_t)gn * p.M + gm] = (OutT)acc[v];
+    const int j0 = n0 + Mfma<T>::col(lane, v), j1 = j0 + 16;
+    if (j0 < N) {
+      if (i0 < M) C[(int64_t)j0 * M + i0] = (OutT)c00[v];
+      if (i1 < M) C[(int64_t)j0 * M + i1] = (OutT)c10[v];
+    }
+    if (j1 < N) {
+      if (i0 < M) C[(int64_t)j1 * M + i0] = (OutT)c01[v];
+      if (i1 < M) C[(int64_t)j1 * M + i1] = (OutT)c11[v];
+    }
   }
 }
 
-// U_u[e] = sum_j c_j T_{t_j}[e] for 1024 consecutive elements per workgroup.
-template <class T, class CT>
-__global__ __launch_bounds__(256) void k_combine(const CombTask* __restrict__ tasks, const CombineList* __restrict__ lists,
-                                                 const int32_t* __restrict__ tix, const CT* __restrict__ cs,
-                                                 const int64_t* __restrict__ Toff, const T* __restrict__ Tbuf,
-                                                 T* __restrict__ Ubuf) {
-  const CombTask tk = tasks[blockIdx.x];
-  const CombineList L = lists[tk.list];
+// Stage A: T (M x N, ld M) = L (M x K, ld M) . V (K x N, ld K), one tile per wave.
+template <class T>
+__global__ __launch_bounds__(256) void k_stage_a(const TileA* __restrict__ tiles, int ntiles, const T* __restrict__ L,
+                                                 const T* __restrict__ V, T* __restrict__ Tout) {
+  typedef typename Mfma<T>::acc_t acc_t;
+  const int w = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (w >= ntiles) return;
+  const int lane = threadIdx.x & 63;
+  const TileA t = tiles[w];
+  const int m0 = t.mt * kTile, n0 = t.nt * kTile;
+  const int li = lane & 15, lk = lane >> 4;
+  const int i0 = m0 + li, i1 = i0 + 16, j0 = n0 + li, j1 = j0 + 16;
+  const bool has_m1 = m0 + 16 < t.M, has_n1 = n0 + 16 < t.N;  // wave-uniform
+  const T* __restrict__ A = L + t.a_off;
+  const T* __restrict__ B = V + t.b_off;
+  acc_t c00 = {0, 0, 0, 0}, c01 = {0, 0, 0, 0}, c10 = {0, 0, 0, 0}, c11 = {0, 0, 0, 0};
+  for (int kb = 0; kb < t.K; kb += 16) {
+    T a0[4], a1[4], b0[4], b1[4];
 #pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    const int64_t e = (int64_t)tk.start + threadIdx.x + r * 256;
-    if (e < L.size) {
-      T s = 0;
-      for (int j = L.begin; j < L.end; ++j) s += (T)cs[j] * Tbuf[Toff[tix[j]] + e];
-      Ubuf[L.u_off + e] = s;
+    for (int u = 0; u < 4; ++u) {  // all loads of the 16-deep block first
+      const int k = kb + 4 * u + lk;
+      const bool kin = k < t.K;
+      a0[u] = (kin && i0 < t.M) ? A[(int64_t)k * t.M + i0] : T(0);
+      a1[u] = (kin && i1 < t.M) ? A[(int64_t)k * t.M + i1] : T(0);
+      b0[u] = (kin && j0 < t.N) ? B[(int64_t)j0 * t.K + k] : T(0);
+      b1[u] = (kin && j1 < t.N) ? B[(int64_t)j1 * t.K + k] : T(0);
     }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      Mfma<T>::mac(b0[u], a0[u], c00);
+      if (has_m1) Mfma<T>::mac(b0[u], a1[u], c10);
+      if (has_n1) {
+        Mfma<T>::mac(b1[u], a0[u], c01);
+        if (has_m1) Mfma<T>::mac(b1[u], a1[u], c11);
+      }
+    }
+  }
+  store_tile<T, T>(Tout + t.c_off, t.M, t.N, m0, n0, lane, c00, c01, c10, c11);
+}
+
+// Stage C: Vout (M x N, ld M) = sum over chunks (sum_j c_j T_j)(M x K, ld M) . R (K x N, ld K).
+// One workgroup per output tile.  The iteration space (chunk, 16-deep K block)
+// is dealt round-robin to the 4 waves (split K), each K block issues all its
+// loads before its 16 MFMAs, and the 4 partial accumulators are summed through
+// LDS in wave order, so the result does not depend on timing.
+constexpr int kKU = 16;  // K depth per unrolled step (4 MFMA k-steps)
+
+template <class T, class OutT>
+__global__ __launch_bounds__(256) void k_stage_c(const TileC* __restrict__ tiles, int ntiles,
+                                                 const ChunkC* __restrict__ chunks, const TermC* __restrict__ terms,
+                                                 const T* __restrict__ Tbuf, const T* __restrict__ R,
+                                                 OutT* __restrict__ Vout) {
+  typedef typename Mfma<T>::acc_t acc_t;
+  __shared__ acc_t red[3][4][64];
+  if ((int)blockIdx.x >= ntiles) return;  // workgroup-uniform
+  const int wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const TileC t = tiles[blockIdx.x];
+  const int m0 = t.mt * kTile, n0 = t.nt * kTile;
+  const int li = lane & 15, lk = lane >> 4;
+  const int i0 = m0 + li, i1 = i0 + 16, j0 = n0 + li, j1 = j0 + 16;
+  const bool has_m1 = m0 + 16 < t.M, has_n1 = n0 + 16 < t.N;
+  const bool r0 = i0 < t.M, r1 = i1 < t.M, q0 = j0 < t.N, q1 = j1 < t.N;
+  acc_t c00 = {0, 0, 0, 0}, c01 = {0, 0, 0, 0}, c10 = {0, 0, 0, 0}, c11 = {0, 0, 0, 0};
+  int it = 0;
+  for (int c = t.chunk_begin; c < t.chunk_end; ++c) {
+    const ChunkC ch = chunks[c];
+    const T* __restrict__ B = R + ch.r_off;
+    for (int kb = 0; kb < ch.K; kb += kKU, ++it) {
+      if ((it & 3) != wave) continue;
+      T a0[4], a1[4], b0[4], b1[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int k = kb + 4 * u + lk;
+        const bool kin = k < ch.K;
+        a0[u] = T(0);
+        a1[u] = T(0);
+        b0[u] = (kin && q0) ? B[(int64_t)j0 * ch.K + k] : T(0);
+        b1[u] = (kin && q1) ? B[(int64_t)j1 * ch.K + k] : T(0);
+      }
+      for (int j = ch.term_begin; j < ch.term_end; ++j) {
+        const TermC tm = terms[j];
+        const T cj = (T)tm.c;
+        const T* __restrict__ Tt = Tbuf + tm.t_off;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int k = kb + 4 * u + lk;
+          const bool kin = k < ch.K;
+          const T x0 = (kin && r0) ? Tt[(int64_t)k * t.M + i0] : T(0);
+          const T x1 = (kin && r1) ? Tt[(int64_t)k * t.M + i1] : T(0);
+          a0[u] += cj * x0;
+          a1[u] += cj * x1;
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        Mfma<T>::mac(b0[u], a0[u], c00);
+        if (has_m1) Mfma<T>::mac(b0[u], a1[u], c10);
+        if (has_n1) {
+          Mfma<T>::mac(b1[u], a0[u], c01);
+          if (has_m1) Mfma<T>::mac(b1[u], a1[u], c11);
+        }
+      }
+    }
+  }
+  if (wave > 0) {
+    red[wave - 1][0][lane] = c00;
+    red[wave - 1][1][lane] = c01;
+    red[wave - 1][2][lane] = c10;
+    red[wave - 1][3][lane] = c11;
+  }
+  __syncthreads();
+  if (wave == 0) {
+#pragma unroll
+    for (int w = 0; w < 3; ++w) {
+      c00 += red[w][0][lane];
+      c01 += red[w][1][lane];
+      c10 += red[w][2][lane];
+      c11 += red[w][3][lane];
+    }
+    store_tile<T, OutT>(Vout + t.c_off, t.M, t.N, m0, n0, lane, c00, c01, c10, c11);
   }
 }
 
@@ -163,8 +233,14 @@ __global__ __launch_bounds__(256) void k_dot_final(const double* __restrict__ pa
 
 __global__ void k_f64_to_f32(const double* __restrict__ x, float* __restrict__ y, int64_t n) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (; i < n; i += stride) y[i] = (float)x[i];
+}
+
+__global__ void k_zero(double* __restrict__ y, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) y[i] = 0.0;
 }
 
 // y += alpha[0] * scale * x  (projector update; alpha is a device scalar)
@@ -172,22 +248,13 @@ __global__ void k_axpy_dev(const double* __restrict__ alpha, double scale, const
                            double* __restrict__ y, int64_t n) {
   const double a = alpha[0] * scale;
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (; i < n; i += stride) y[i] += a * x[i];
 }
 
-void make_tiles(const std::vector<GemmProb>& probs, std::vector<Tile>& tiles) {
-  tiles.clear();
-  for (size_t p = 0; p < probs.size(); ++p) {
-    const int mt = (probs[p].M + kTM - 1) / kTM;
-    const int nt = (probs[p].N + kTN - 1) / kTN;
-    if (mt > 32767 || nt > 32767) throw VmpsError(VMPS_ERR_ARG, "block dimension too large for the tile index");
-    for (int i = 0; i < mt; ++i)
-      for (int j = 0; j < nt; ++j) tiles.push_back({(int32_t)p, (int16_t)i, (int16_t)j});
-  }
-}
-
 constexpr int kDotSegPerBlock = 4;
+
+inline unsigned grid_for(size_t waves) { return (unsigned)((waves + 3) / 4); }
 
 }  // namespace
 
@@ -209,35 +276,21 @@ void DevPlan::create(const Pivot2& P, vmps_precision precision) {
   L.upload(H.Ldata, stream);
   R.upload(H.Rdata, stream);
   T.alloc(H.T_size);
-  U.alloc(H.U_size);
   vin.alloc(H.vsize);
   vout.alloc(H.vsize);
   if (H.vsize) {
     VMPS_HIP_CHECK(hipMemsetAsync(vin.p, 0, H.vsize * sizeof(double), stream));
     VMPS_HIP_CHECK(hipMemsetAsync(vout.p, 0, H.vsize * sizeof(double), stream));
   }
-  probA.upload(H.probA, stream);
-  chunkA.upload(H.chunkA, stream);
-  probC.upload(H.probC, stream);
-  chunkC.upload(H.chunkC, stream);
-  std::vector<Tile> tiles;
-  make_tiles(H.probA, tiles);
-  tilesA.upload(tiles, stream);
-  make_tiles(H.probC, tiles);
-  tilesC.upload(tiles, stream);
-  comb.upload(H.comb, stream);
-  comb_t.upload(H.comb_t, stream);
-  comb_c.upload(H.comb_c, stream);
-  T_off.upload(H.T_off, stream);
-  std::vector<CombTask> ct;
-  for (size_t u = 0; u < H.comb.size(); ++u)
-    for (int64_t e = 0; e < H.comb[u].size; e += 1024) ct.push_back({(int32_t)u, (int32_t)e});
-  ctasks.upload(ct, stream);
+  tilesA.upload(H.tilesA, stream);
+  tilesC.upload(H.tilesC, stream);
+  chunksC.upload(H.chunksC, stream);
+  termsC.upload(H.termsC, stream);
   seg_off.upload(H.seg_off, stream);
   seg_len.upload(H.seg_len, stream);
   seg_w.upload(H.seg_w, stream);
   const size_t nseg = H.seg_off.size();
-  partial.alloc(std::max<size_t>(1, (nseg + kDotSegPerBlock - 1) / kDotSegPerBlock + 1024));
+  partial.alloc(std::max<size_t>(1, (nseg + kDotSegPerBlock - 1) / kDotSegPerBlock));
   scalars.alloc(64);
   nproj = (int)P.A0proj.size();
   Epenalty = P.Epenalty;
@@ -252,14 +305,11 @@ void DevPlan::create(const Pivot2& P, vmps_precision precision) {
 }
 
 void DevPlan::ensure_precision_buffers() {
-  if (prec == VMPS_PREC_FP32 && !L32.p && (H.Ldata.size() || H.Rdata.size())) {
+  if (prec == VMPS_PREC_FP32 && !T32.p && (H.T_size || H.vsize)) {
     L32.alloc(H.Ldata.size());
     R32.alloc(H.Rdata.size());
     V32.alloc(H.vsize);
     T32.alloc(H.T_size);
-    U32.alloc(H.U_size);
-    std::vector<float> c32(H.comb_c.begin(), H.comb_c.end());
-    comb_c32.upload(c32, stream);
     if (L.n) hipLaunchKernelGGL(k_f64_to_f32, dim3(1024), dim3(256), 0, stream, L.p, L32.p, (int64_t)L.n);
     if (R.n) hipLaunchKernelGGL(k_f64_to_f32, dim3(1024), dim3(256), 0, stream, R.p, R32.p, (int64_t)R.n);
     VMPS_HIP_CHECK(hipGetLastError());
@@ -268,30 +318,24 @@ void DevPlan::ensure_precision_buffers() {
 }
 
 void DevPlan::launch_stage(int stage, const double* d_vin, double* d_vout, hipStream_t s) {
-  const unsigned nA = (unsigned)tilesA.n, nB = (unsigned)ctasks.n, nC = (unsigned)tilesC.n;
+  const int nA = (int)tilesA.n, nC = (int)tilesC.n;
   if (prec == VMPS_PREC_FP64) {
     if (stage == 0 && nA)
-      hipLaunchKernelGGL((k_gemm_grouped<double, double>), dim3(nA), dim3(256), 0, s, tilesA.p, probA.p, chunkA.p, L.p,
-                         d_vin, T.p);
-    if (stage == 1 && nB)
-      hipLaunchKernelGGL((k_combine<double, double>), dim3(nB), dim3(256), 0, s, ctasks.p, comb.p, comb_t.p, comb_c.p,
-                         T_off.p, T.p, U.p);
-    if (stage == 2 && nC)
-      hipLaunchKernelGGL((k_gemm_grouped<double, double>), dim3(nC), dim3(256), 0, s, tilesC.p, probC.p, chunkC.p, U.p,
-                         R.p, d_vout);
+      hipLaunchKernelGGL(k_stage_a<double>, dim3(grid_for(nA)), dim3(256), 0, s, tilesA.p, nA, L.p, d_vin, T.p);
+    if (stage == 2) {
+      if (nC)
+        hipLaunchKernelGGL((k_stage_c<double, double>), dim3(nC), dim3(256), 0, s, tilesC.p, nC, chunksC.p,
+                         termsC.p, T.p, R.p, d_vout);
+    }
   } else if (prec == VMPS_PREC_FP32) {
     if (stage == 0) {
       if (H.vsize) hipLaunchKernelGGL(k_f64_to_f32, dim3(1024), dim3(256), 0, s, d_vin, V32.p, (int64_t)H.vsize);
       if (nA)
-        hipLaunchKernelGGL((k_gemm_grouped<float, float>), dim3(nA), dim3(256), 0, s, tilesA.p, probA.p, chunkA.p,
-                           L32.p, V32.p, T32.p);
+        hipLaunchKernelGGL(k_stage_a<float>, dim3(grid_for(nA)), dim3(256), 0, s, tilesA.p, nA, L32.p, V32.p, T32.p);
     }
-    if (stage == 1 && nB)
-      hipLaunchKernelGGL((k_combine<float, float>), dim3(nB), dim3(256), 0, s, ctasks.p, comb.p, comb_t.p, comb_c32.p,
-                         T_off.p, T32.p, U32.p);
     if (stage == 2 && nC)
-      hipLaunchKernelGGL((k_gemm_grouped<float, double>), dim3(nC), dim3(256), 0, s, tilesC.p, probC.p, chunkC.p,
-                         U32.p, R32.p, d_vout);
+      hipLaunchKernelGGL((k_stage_c<float, double>), dim3(nC), dim3(256), 0, s, tilesC.p, nC, chunksC.p, termsC.p,
+                         T32.p, R32.p, d_vout);
   } else {
     throw VmpsError(VMPS_ERR_ARG, "unsupported precision");
   }
@@ -300,7 +344,8 @@ void DevPlan::launch_stage(int stage, const double* d_vin, double* d_vout, hipSt
 
 void DevPlan::hxv(const double* d_vin, double* d_vout, hipStream_t s) {
   if (!s) s = stream;
-  for (int st = 0; st < 3; ++st) launch_stage(st, d_vin, d_vout, s);
+  launch_stage(0, d_vin, d_vout, s);
+  launch_stage(2, d_vin, d_vout, s);
   // excited-state penalty: Vout += Epenalty * <A0_n|Vin> * A0_n, the overlap
   // being A0proj[n][s].adjoint().contract(Vin[s]).trace() = coeff_dot-weighted
   // dot (DmrgPivotMatrix2.h:250-271, tensors/Biped.h:936-944, :976-1010)
@@ -316,7 +361,8 @@ void DevPlan::dot_async(const double* d_x, const double* d_y, double* d_result, 
   if (!s) s = stream;
   const int nseg = (int)H.seg_off.size();
   if (nseg == 0) {
-    VMPS_HIP_CHECK(hipMemsetAsync(d_result, 0, sizeof(double), s));
+    hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, s, d_result, (int64_t)1);
+    VMPS_HIP_CHECK(hipGetLastError());
     return;
   }
   const int nb = (nseg + kDotSegPerBlock - 1) / kDotSegPerBlock;

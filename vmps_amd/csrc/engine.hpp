@@ -20,17 +20,6 @@ namespace vmps {
                               std::string(#x) + ": " + hipGetErrorString(e_));             \
   } while (0)
 
-// Output tile of a grouped GEMM problem: problem index and tile coordinates.
-struct Tile {
-  int32_t prob;
-  int16_t mt, nt;
-};
-// Element range of one combine list (U_u = sum c T).
-struct CombTask {
-  int32_t list;
-  int32_t start;
-};
-
 template <class T>
 struct DevBuf {
   T* p = nullptr;
@@ -63,21 +52,15 @@ struct DevPlan {
   HostPlan H;
 
   // FP64 operands and intermediates
-  DevBuf<double> L, R, T, U, vin, vout;
+  DevBuf<double> L, R, T, vin, vout;
   // FP32 operands and intermediates (mixed modes)
-  DevBuf<float> L32, R32, V32, T32, U32;
-  // split-BF16 (hi, lo) operands: stored as uint16 pairs
-  DevBuf<uint16_t> Lhi, Llo, Rhi, Rlo, Vhi, Vlo, Uhi, Ulo;
+  DevBuf<float> L32, R32, V32, T32;
 
-  DevBuf<GemmProb> probA, probC;
-  DevBuf<GemmChunk> chunkA, chunkC;
-  DevBuf<Tile> tilesA, tilesC;
-  DevBuf<CombineList> comb;
-  DevBuf<int32_t> comb_t;
-  DevBuf<double> comb_c;
-  DevBuf<float> comb_c32;
-  DevBuf<int64_t> T_off;
-  DevBuf<CombTask> ctasks;
+  // work lists
+  DevBuf<TileA> tilesA;
+  DevBuf<TileC> tilesC;
+  DevBuf<ChunkC> chunksC;
+  DevBuf<TermC> termsC;
   DevBuf<int64_t> seg_off;
   DevBuf<int32_t> seg_len;
   DevBuf<double> seg_w;
@@ -93,7 +76,7 @@ struct DevPlan {
   void create(const Pivot2& P, vmps_precision prec);
   void ensure_precision_buffers();
   void hxv(const double* d_vin, double* d_vout, hipStream_t s);
-  // one stage of hxv alone: 0 = L.V, 1 = combine, 2 = U.R
+  // one stage of hxv alone: 0 = L.V, 2 = (sum c T).R; stage 1 (the combine) is fused into 2
   void launch_stage(int stage, const double* d_vin, double* d_vout, hipStream_t s);
   double dot(const double* d_x, const double* d_y, hipStream_t s);
   void dot_async(const double* d_x, const double* d_y, double* d_result, hipStream_t s);

@@ -354,6 +354,94 @@ void validate_host_plan(const HostPlan& H) {
       throw VmpsError(VMPS_ERR_INTERNAL, "dot segment out of range");
 }
 
+// Flatten the stage tables into per-tile work lists.  Stage B (U = sum c T)
+// is folded into stage C: each stage C chunk carries the list of (T, c) that
+// forms its U operand.  Tiles are ordered by decreasing work so the largest
+// start first.
+void build_tiles(HostPlan& H) {
+  H.tilesA.clear();
+  H.tilesC.clear();
+  H.chunksC.clear();
+  H.termsC.clear();
+  std::vector<std::pair<int64_t, size_t>> order;
+  for (size_t p = 0; p < H.probA.size(); ++p) {
+    const GemmProb& pr = H.probA[p];
+    const GemmChunk& ch = H.chunkA[pr.chunk_begin];
+    for (int i = 0; i * kTile < pr.M; ++i)
+      for (int j = 0; j * kTile < pr.N; ++j) {
+        if (i > 32767 || j > 32767) throw VmpsError(VMPS_ERR_ARG, "block dimension too large for the tile index");
+        TileA t;
+        t.a_off = ch.a_off;
+        t.b_off = ch.b_off;
+        t.c_off = pr.c_off;
+        t.M = pr.M;
+        t.N = pr.N;
+        t.K = ch.K;
+        t.mt = (int16_t)i;
+        t.nt = (int16_t)j;
+        H.tilesA.push_back(t);
+      }
+  }
+  order.clear();
+  for (size_t i = 0; i < H.tilesA.size(); ++i) order.push_back({-(int64_t)H.tilesA[i].K, i});
+  std::stable_sort(order.begin(), order.end());
+  {
+    std::vector<TileA> sorted;
+    for (auto& o : order) sorted.push_back(H.tilesA[o.second]);
+    H.tilesA.swap(sorted);
+  }
+  for (size_t j = 0; j < H.comb_t.size(); ++j) H.termsC.push_back({H.T_off[H.comb_t[j]], H.comb_c[j]});
+  for (const GemmChunk& ch : H.chunkC) {
+    const CombineList& cl = H.comb[ch.pad];
+    ChunkC c;
+    c.r_off = ch.b_off;
+    c.K = ch.K;
+    c.term_begin = cl.begin;
+    c.term_end = cl.end;
+    c.pad = 0;
+    H.chunksC.push_back(c);
+  }
+  std::vector<int64_t> cost;
+  for (const GemmProb& pr : H.probC) {
+    int64_t w = 0;
+    for (int32_t c = pr.chunk_begin; c < pr.chunk_end; ++c)
+      w += (int64_t)H.chunksC[c].K * (1 + H.chunksC[c].term_end - H.chunksC[c].term_begin);
+    for (int i = 0; i * kTile < pr.M; ++i)
+      for (int j = 0; j * kTile < pr.N; ++j) {
+        if (i > 32767 || j > 32767) throw VmpsError(VMPS_ERR_ARG, "block dimension too large for the tile index");
+        TileC t;
+        t.c_off = pr.c_off;
+        t.M = pr.M;
+        t.N = pr.N;
+        t.chunk_begin = pr.chunk_begin;
+        t.chunk_end = pr.chunk_end;
+        t.mt = (int16_t)i;
+        t.nt = (int16_t)j;
+        t.pad = 0;
+        H.tilesC.push_back(t);
+        cost.push_back(w);
+      }
+  }
+  order.clear();
+  for (size_t i = 0; i < H.tilesC.size(); ++i) order.push_back({-cost[i], i});
+  std::stable_sort(order.begin(), order.end());
+  {
+    std::vector<TileC> sorted;
+    for (auto& o : order) sorted.push_back(H.tilesC[o.second]);
+    H.tilesC.swap(sorted);
+  }
+  // bounds of the fused stage C: every T read stays inside T
+  for (const ChunkC& c : H.chunksC)
+    for (int32_t j = c.term_begin; j < c.term_end; ++j)
+      if (j < 0 || j >= (int32_t)H.termsC.size() || H.termsC[j].t_off < 0)
+        throw VmpsError(VMPS_ERR_INTERNAL, "stage C: term reference out of range");
+  for (const TileC& t : H.tilesC)
+    for (int32_t c = t.chunk_begin; c < t.chunk_end; ++c)
+      for (int32_t j = H.chunksC[c].term_begin; j < H.chunksC[c].term_end; ++j)
+        if (H.termsC[j].t_off + (int64_t)t.M * H.chunksC[c].K > H.T_size)
+          throw VmpsError(VMPS_ERR_INTERNAL, "stage C: T block out of range");
+}
+
 void build_host_plan(const Pivot2& P, HostPlan& H) {
   std::vector<Term> terms;
   std::vector<std::pair<size_t, size_t>> lblocks, rblocks;
@@ -487,7 +575,7 @@ void build_host_plan(const Pivot2& P, HostPlan& H) {
       ch.K = (int32_t)R.rows;
       ch.lda = (int32_t)vo.rows;
       ch.ldb = (int32_t)R.rows;
-      ch.pad = 0;
+      ch.pad = u;  // combine list that forms this U
       H.chunkC.push_back(ch);
     }
     pr.chunk_end = (int32_t)H.chunkC.size();
@@ -509,6 +597,7 @@ void build_host_plan(const Pivot2& P, HostPlan& H) {
     }
 
   validate_host_plan(H);
+  build_tiles(H);
   st.n_terms = (int64_t)terms.size();
   st.n_T = (int64_t)H.probA.size();
   st.n_U = (int64_t)H.comb.size();

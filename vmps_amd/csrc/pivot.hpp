@@ -158,10 +158,40 @@ struct Stats {
   int64_t max_block_dim = 0;
 };
 
+// Flattened per-tile descriptors of the device kernels (one load per tile).
+constexpr int kTile = 32;  // output tile edge of one wavefront
+struct TileA {             // T(M x N, ld M) tile = L(M x K, ld M) . V(K x N, ld K)
+  int64_t a_off, b_off, c_off;
+  int32_t M, N, K;
+  int16_t mt, nt;          // tile coordinates (units of kTile)
+};
+struct TileC {             // Vout(M x N, ld M) tile = sum over chunks U . R
+  int64_t c_off;
+  int32_t M, N;
+  int32_t chunk_begin, chunk_end;
+  int16_t mt, nt;
+  int32_t pad;
+};
+struct ChunkC {            // one (U_u, R_r) pair: U_u = sum_j c_j T_j (M x K), R_r (K x N, ld K)
+  int64_t r_off;
+  int32_t K;
+  int32_t term_begin, term_end;
+  int32_t pad;
+};
+struct TermC {             // c_j T_j with T_j an M x K block at t_off
+  int64_t t_off;
+  double c;
+};
+
 struct HostPlan {
   // packed operands actually referenced by terms
   std::vector<double> Ldata, Rdata;  // copied L/R blocks
   int64_t vsize = 0;
+  // device work lists (built from the tables below by build_tiles)
+  std::vector<TileA> tilesA;
+  std::vector<TileC> tilesC;
+  std::vector<ChunkC> chunksC;
+  std::vector<TermC> termsC;
   std::vector<GemmProb> probA;       // stage A: one chunk each (L . V)
   std::vector<GemmChunk> chunkA;
   std::vector<CombineList> comb;     // stage B
@@ -191,6 +221,7 @@ void expand_terms(const Pivot2& P, std::vector<Term>& terms,
                   std::vector<std::pair<int, int>>& vblocks);
 
 void build_host_plan(const Pivot2& P, HostPlan& H);
+void build_tiles(HostPlan& H);
 // Throws VMPS_ERR_INTERNAL if any descriptor would address outside its buffer.
 void validate_host_plan(const HostPlan& H);
 
