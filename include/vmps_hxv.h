@@ -155,6 +155,62 @@ vmps_status vmps_dot(vmps_plan* plan, const double* x, const double* y, double* 
  * with HIP events around `iters` back-to-back calls on resident vectors. */
 vmps_status vmps_plan_time_hxv(vmps_plan* plan, int warmup, int iters, double* ms_per_hxv);
 
+/* ---- PivotVector algebra on device vectors (DmrgPivotVector.h:121-424) ----
+ * All vectors are packed device arrays of the plan's layout; work is enqueued
+ * on the plan's stream and scalar results are returned after a synchronise. */
+
+/* Device vector of the plan's layout (PivotVector storage), zero-filled;
+ * upload / download copy whole vectors between host and device. */
+vmps_status vmps_vec_alloc(vmps_plan* plan, double** d_v);
+vmps_status vmps_vec_free(vmps_plan* plan, double* d_v);
+vmps_status vmps_vec_upload(vmps_plan* plan, double* d_v, const double* h_v);
+vmps_status vmps_vec_download(vmps_plan* plan, const double* d_v, double* h_v);
+
+/* dot(V1, V2) with coeff_dot weights (DmrgPivotVector.h:296-339) */
+vmps_status vmps_vec_dot(vmps_plan* plan, const double* d_x, const double* d_y, double* result);
+/* norm(V) = sqrt(dot(V, V)) (DmrgPivotVector.h:341-352) */
+vmps_status vmps_vec_norm(vmps_plan* plan, const double* d_x, double* result);
+/* y = a*x + b*y: addScale(a, x, y) is (a, 1) (DmrgPivotVector.h:289-293),
+ * y *= a is (0, a), y /= a is (0, 1/a) (:223-253), y += x / y -= x are (+-1, 1) */
+vmps_status vmps_vec_axpby(vmps_plan* plan, double a, const double* d_x, double b, double* d_y);
+/* GaussianRandomVector<PivotVector>::fill: entries uniform in (-1, 1), then
+ * normalised (DmrgPivotVector.h:410-424); counter-based, reproducible per seed */
+vmps_status vmps_vec_fill_random(vmps_plan* plan, double* d_v, uint64_t seed);
+
+/* ---- Krylov eigensolver (stands in for the external LanczosSolver that
+ * DmrgSolver::iteration_two calls, solvers/DmrgSolver.h:1466-1471) ---- */
+typedef struct {
+  int max_iter;             /* Krylov dimension limit (dimK, DmrgTypedefs.h:373) */
+  int min_iter;
+  double tol_eigval;        /* |E_k - E_{k-1}| (DmrgTypedefs.h:371) */
+  double tol_state;         /* beta_k |y_k|  (DmrgTypedefs.h:372) */
+  int precision;            /* vmps_precision of the HxV inside the iterations */
+  int refine_fp64;          /* mixed solves: restart in FP64 from the mixed eigenvector */
+  double refine_tol_eigval;
+  double refine_tol_state;
+  uint64_t seed;            /* start vector when none is given */
+} vmps_lanczos_opts;
+
+typedef struct {
+  double energy;            /* <psi|H|psi> with the FP64 operator when refined / FP64 */
+  double residual;          /* ||H psi - E psi||, same operator */
+  int iterations;           /* iterations of the main solve */
+  int iterations_refine;    /* iterations of the FP64 refinement */
+  int hxv_calls;
+  int converged;
+  double energy_mixed;      /* Rayleigh quotient of the mixed solve with its own operator */
+  double residual_mixed;
+  double ms;                /* wall time of the call */
+} vmps_lanczos_result;
+
+void vmps_lanczos_opts_default(vmps_lanczos_opts* o);
+/* Lowest eigenpair of the pivot.  v: host vector, on entry the start vector
+ * (NULL: random start from opts->seed), on exit the normalised eigenvector. */
+vmps_status vmps_lanczos(vmps_plan* plan, double* v, const vmps_lanczos_opts* opts, vmps_lanczos_result* res);
+/* Same with a device-resident start / result vector. */
+vmps_status vmps_lanczos_device(vmps_plan* plan, double* d_v, const vmps_lanczos_opts* opts,
+                                vmps_lanczos_result* res);
+
 /* Per-stage device time (ms per call, HIP events on the plan's stream, mean
  * over `iters` launches of each stage alone on resident data):
  * ms[0] stage A (T = L.V), ms[1] stage B (U = sum c T), ms[2] stage C

@@ -1,0 +1,97 @@
+"""Device Lanczos (stands in for the external LanczosSolver) and the device
+PivotVector algebra, on symmetric pivots."""
+import ctypes
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from vmps_amd import models, synth
+from vmps_amd.pivot import PivotMatrix2, lanczos
+
+sys.path.insert(0, __file__.rsplit("/", 1)[0])
+import dense  # noqa: E402
+import pivots  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+def _herm_small(kind, seed=3):
+    if kind == "heis":
+        W = models.heisenberg_u1_mpo()
+        lb, rb = synth.heisenberg_bond(5, 10), synth.heisenberg_bond(7, 10)
+    else:
+        W = models.hubbard_u1xu1_mpo()
+        lb = synth.hubbard_bond(3, 12, alpha=1.0, lam=1.2)
+        rb = synth.hubbard_bond(5, 12, alpha=1.0, lam=1.2)
+    return synth.make_pivot(W, W, lb, rb, np.random.default_rng(seed), hermitian=True)
+
+
+@pytest.mark.parametrize("kind", ["heis", "hub"])
+def test_lanczos_fp64_vs_dense_eigensolver(kind):
+    pd = _herm_small(kind)
+    n = pd.vsize
+    Hm = np.stack([dense.dense_hxv(pd, np.eye(n)[i]) for i in range(n)], axis=1)
+    evals, evecs = np.linalg.eigh(0.5 * (Hm + Hm.T))
+    H = PivotMatrix2(pd)
+    v, r = lanczos(H, max_iter=min(n, 120), tol_eigval=1e-13, tol_state=1e-9)
+    print(kind, r, evals[:2])
+    assert abs(r["energy"] - evals[0]) <= 1e-10 * max(1.0, abs(evals[0]))
+    assert r["residual"] <= 1e-6
+    assert abs(abs(v @ evecs[:, 0]) - 1.0) <= 1e-8  # same eigenvector (non-degenerate ground state)
+
+
+def test_lanczos_mixed_fp32_refined_to_fp64():
+    pd = _herm_small("hub", seed=5)
+    n = pd.vsize
+    Hm = np.stack([dense.dense_hxv(pd, np.eye(n)[i]) for i in range(n)], axis=1)
+    e0 = np.linalg.eigvalsh(0.5 * (Hm + Hm.T))[0]
+    H = PivotMatrix2(pd, precision="fp32")
+    v, r = lanczos(H, precision="fp32", max_iter=min(n, 120), tol_eigval=1e-6, tol_state=1e-4)
+    print(r, e0)
+    assert abs(r["energy_mixed"] - e0) <= 1e-4 * abs(e0)   # FP32 operator
+    assert abs(r["energy"] - e0) <= 1e-9 * max(1.0, abs(e0))  # after FP64 refinement
+
+
+def test_lanczos_workload_vs_scipy_on_oracle():
+    """Hubbard M=512 symmetric pivot: device FP64 Lanczos vs ARPACK (scipy) driving the oracle's HxV."""
+    from scipy.sparse.linalg import LinearOperator, eigsh
+    pd = synth.make_workload("hubbard_u1xu1_L48_M512", hermitian=True)
+    lib = oracle.load()
+    h = pivots.build_oracle(lib, pd)
+    op = LinearOperator((pd.vsize, pd.vsize), matvec=lambda x: pivots.oracle_hxv(lib, h, np.asarray(x).ravel(), 16),
+                        dtype=np.float64)
+    e_ref = eigsh(op, k=1, which="SA", tol=1e-12, v0=np.ones(pd.vsize))[0][0]
+    lib.oracle_pivot2_destroy(h)
+    H = PivotMatrix2(pd)
+    v, r = lanczos(H, max_iter=300, tol_eigval=1e-12, tol_state=1e-6)
+    print(r, e_ref)
+    assert abs(r["energy"] - e_ref) <= 1e-9 * abs(e_ref)
+    # mixed precision + FP64 refinement reaches |dE| <= 1e-9 relative as well
+    Hm = PivotMatrix2(pd, precision="fp32")
+    v2, r2 = lanczos(Hm, precision="fp32", max_iter=300, tol_eigval=1e-7, tol_state=1e-4)
+    print(r2)
+    assert abs(r2["energy"] - e_ref) <= 1e-9 * abs(e_ref)
+
+
+def test_vector_algebra_device():
+    from vmps_amd.pivot import DeviceVector
+    pd = _herm_small("hub")
+    H = PivotMatrix2(pd)
+    lib, plan = H._lib, H._plan
+    xh, yh = synth.random_vector(pd, 1), synth.random_vector(pd, 2)
+    x, y = DeviceVector(H, xh), DeviceVector(H, yh)
+    r = ctypes.c_double()
+    assert lib.vmps_vec_dot(plan, x.ptr, y.ptr, ctypes.byref(r)) == 0
+    assert abs(r.value - xh @ yh) <= 1e-12 * np.abs(xh).sum() * np.abs(yh).max()
+    assert lib.vmps_vec_axpby(plan, 0.5, x.ptr, -2.0, y.ptr) == 0
+    assert np.abs(y.download() - (0.5 * xh - 2.0 * yh)).max() <= 1e-13
+    assert lib.vmps_vec_norm(plan, x.ptr, ctypes.byref(r)) == 0
+    assert abs(r.value - np.linalg.norm(xh)) <= 1e-12 * np.linalg.norm(xh)
+    assert lib.vmps_vec_fill_random(plan, x.ptr, 42) == 0
+    xr = x.download()
+    assert abs(np.linalg.norm(xr) - 1.0) < 1e-12 and np.abs(xr).max() < 1.0
+    # the same seed gives the same vector
+    assert lib.vmps_vec_fill_random(plan, y.ptr, 42) == 0
+    assert np.array_equal(xr, y.download())

@@ -38,6 +38,25 @@ class PlanStats(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class LanczosOpts(ctypes.Structure):
+    _fields_ = [
+        ("max_iter", c_int), ("min_iter", c_int), ("tol_eigval", c_dbl), ("tol_state", c_dbl),
+        ("precision", c_int), ("refine_fp64", c_int), ("refine_tol_eigval", c_dbl), ("refine_tol_state", c_dbl),
+        ("seed", ctypes.c_uint64),
+    ]
+
+
+class LanczosResult(ctypes.Structure):
+    _fields_ = [
+        ("energy", c_dbl), ("residual", c_dbl), ("iterations", c_int), ("iterations_refine", c_int),
+        ("hxv_calls", c_int), ("converged", c_int), ("energy_mixed", c_dbl), ("residual_mixed", c_dbl),
+        ("ms", c_dbl),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class VmpsStatusError(RuntimeError):
     def __init__(self, status, msg):
         super().__init__("%s: %s" % (STATUS_NAMES.get(status, status), msg))
@@ -70,6 +89,17 @@ SIGNATURES = {
     "vmps_dot": (c_int, [c_void, P_dbl, P_dbl, P_dbl]),
     "vmps_plan_time_hxv": (c_int, [c_void, c_int, c_int, P_dbl]),
     "vmps_plan_time_stages": (c_int, [c_void, c_int, c_int, P_dbl]),
+    "vmps_vec_alloc": (c_int, [c_void, ctypes.POINTER(c_void)]),
+    "vmps_vec_free": (c_int, [c_void, c_void]),
+    "vmps_vec_upload": (c_int, [c_void, c_void, P_dbl]),
+    "vmps_vec_download": (c_int, [c_void, c_void, P_dbl]),
+    "vmps_vec_dot": (c_int, [c_void, c_void, c_void, P_dbl]),
+    "vmps_vec_norm": (c_int, [c_void, c_void, P_dbl]),
+    "vmps_vec_axpby": (c_int, [c_void, c_dbl, c_void, c_dbl, c_void]),
+    "vmps_vec_fill_random": (c_int, [c_void, c_void, ctypes.c_uint64]),
+    "vmps_lanczos_opts_default": (None, [ctypes.POINTER(LanczosOpts)]),
+    "vmps_lanczos": (c_int, [c_void, P_dbl, ctypes.POINTER(LanczosOpts), ctypes.POINTER(LanczosResult)]),
+    "vmps_lanczos_device": (c_int, [c_void, c_void, ctypes.POINTER(LanczosOpts), ctypes.POINTER(LanczosResult)]),
 }
 
 _lib = None

@@ -2,12 +2,15 @@
 // exceptions and turns them into a status code plus a thread-local message.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <memory>
 #include <string>
 
 #include "engine.hpp"
+#include "lanczos.hpp"
 #include "errors.hpp"
 #include "pivot.hpp"
 #include "vmps_hxv.h"
@@ -382,6 +385,165 @@ vmps_status vmps_plan_time_hxv(vmps_plan* plan, int warmup, int iters, double* m
   float ms = 0.f;
   VMPS_HIP_CHECK(hipEventElapsedTime(&ms, D.ev0, D.ev1));
   *ms_per_hxv = (double)ms / iters;
+  VMPS_API_END
+}
+
+vmps_status vmps_vec_alloc(vmps_plan* plan, double** d_v) {
+  VMPS_API_BEGIN
+  require(plan && d_v, VMPS_ERR_ARG, "NULL argument");
+  *d_v = nullptr;
+  const size_t bytes = std::max<size_t>(1, (size_t)plan->D.H.vsize) * sizeof(double);
+  VMPS_HIP_CHECK(hipMalloc((void**)d_v, bytes));
+  VMPS_HIP_CHECK(hipMemsetAsync(*d_v, 0, bytes, plan->D.stream));
+  VMPS_HIP_CHECK(hipStreamSynchronize(plan->D.stream));
+  VMPS_API_END
+}
+
+vmps_status vmps_vec_free(vmps_plan* plan, double* d_v) {
+  VMPS_API_BEGIN
+  require(plan != nullptr, VMPS_ERR_ARG, "NULL plan");
+  if (d_v) {
+    VMPS_HIP_CHECK(hipStreamSynchronize(plan->D.stream));
+    VMPS_HIP_CHECK(hipFree(d_v));
+  }
+  VMPS_API_END
+}
+
+vmps_status vmps_vec_upload(vmps_plan* plan, double* d_v, const double* h_v) {
+  VMPS_API_BEGIN
+  require(plan && (plan->D.H.vsize == 0 || (d_v && h_v)), VMPS_ERR_ARG, "NULL argument");
+  const size_t bytes = (size_t)plan->D.H.vsize * sizeof(double);
+  if (bytes) VMPS_HIP_CHECK(hipMemcpyAsync(d_v, h_v, bytes, hipMemcpyHostToDevice, plan->D.stream));
+  VMPS_HIP_CHECK(hipStreamSynchronize(plan->D.stream));
+  VMPS_API_END
+}
+
+vmps_status vmps_vec_download(vmps_plan* plan, const double* d_v, double* h_v) {
+  VMPS_API_BEGIN
+  require(plan && (plan->D.H.vsize == 0 || (d_v && h_v)), VMPS_ERR_ARG, "NULL argument");
+  const size_t bytes = (size_t)plan->D.H.vsize * sizeof(double);
+  if (bytes) VMPS_HIP_CHECK(hipMemcpyAsync(h_v, d_v, bytes, hipMemcpyDeviceToHost, plan->D.stream));
+  VMPS_HIP_CHECK(hipStreamSynchronize(plan->D.stream));
+  VMPS_API_END
+}
+
+vmps_status vmps_vec_dot(vmps_plan* plan, const double* d_x, const double* d_y, double* result) {
+  VMPS_API_BEGIN
+  require(plan && result && (plan->D.H.vsize == 0 || (d_x && d_y)), VMPS_ERR_ARG, "NULL argument");
+  *result = plan->D.dot(d_x, d_y, plan->D.stream);
+  VMPS_API_END
+}
+
+vmps_status vmps_vec_norm(vmps_plan* plan, const double* d_x, double* result) {
+  VMPS_API_BEGIN
+  require(plan && result && (plan->D.H.vsize == 0 || d_x), VMPS_ERR_ARG, "NULL argument");
+  *result = std::sqrt(std::max(0.0, plan->D.dot(d_x, d_x, plan->D.stream)));
+  VMPS_API_END
+}
+
+vmps_status vmps_vec_axpby(vmps_plan* plan, double a, const double* d_x, double b, double* d_y) {
+  VMPS_API_BEGIN
+  require(plan && (plan->D.H.vsize == 0 || (d_x && d_y)), VMPS_ERR_ARG, "NULL argument");
+  if (plan->D.H.vsize) vmps::dev_axpby(a, d_x, b, d_y, plan->D.H.vsize, plan->D.stream);
+  VMPS_HIP_CHECK(hipStreamSynchronize(plan->D.stream));
+  VMPS_API_END
+}
+
+vmps_status vmps_vec_fill_random(vmps_plan* plan, double* d_v, uint64_t seed) {
+  VMPS_API_BEGIN
+  require(plan && (plan->D.H.vsize == 0 || d_v), VMPS_ERR_ARG, "NULL argument");
+  auto& D = plan->D;
+  const int64_t n = D.H.vsize;
+  if (n) {
+    vmps::dev_fill_uniform(d_v, n, seed, D.stream);
+    const double nrm = std::sqrt(D.dot(d_v, d_v, D.stream));
+    require(nrm > 0.0, VMPS_ERR_INTERNAL, "random vector has zero norm");
+    vmps::dev_axpby(0.0, d_v, 1.0 / nrm, d_v, n, D.stream);
+    VMPS_HIP_CHECK(hipStreamSynchronize(D.stream));
+  }
+  VMPS_API_END
+}
+
+void vmps_lanczos_opts_default(vmps_lanczos_opts* o) {
+  if (!o) return;
+  o->max_iter = 100;
+  o->min_iter = 2;
+  o->tol_eigval = 1e-8;
+  o->tol_state = 1e-5;
+  o->precision = VMPS_PREC_FP64;
+  o->refine_fp64 = 1;
+  o->refine_tol_eigval = 1e-12;
+  o->refine_tol_state = 1e-7;
+  o->seed = 1;
+}
+
+static vmps_status lanczos_impl(vmps_plan* plan, double* d_v, bool random_start, const vmps_lanczos_opts* opts,
+                                vmps_lanczos_result* res) {
+  VMPS_API_BEGIN
+  require(plan && opts && res, VMPS_ERR_ARG, "NULL argument");
+  require(opts->max_iter >= 1 && opts->tol_eigval >= 0 && opts->tol_state >= 0, VMPS_ERR_ARG, "bad options");
+  require(opts->precision >= VMPS_PREC_FP64 && opts->precision <= VMPS_PREC_BF16X3, VMPS_ERR_ARG, "bad precision");
+  auto& D = plan->D;
+  const auto t0 = std::chrono::steady_clock::now();
+  std::memset(res, 0, sizeof(*res));
+  if (random_start) {
+    vmps::dev_fill_uniform(d_v, D.H.vsize, opts->seed, D.stream);
+  }
+  vmps::LanczosOpts lo;
+  lo.max_iter = opts->max_iter;
+  lo.min_iter = opts->min_iter;
+  lo.tol_eigval = opts->tol_eigval;
+  lo.tol_state = opts->tol_state;
+  const auto prec = (vmps_precision)opts->precision;
+  vmps::LanczosResult r = vmps::lanczos_solve(D, d_v, lo, prec);
+  res->iterations = r.iterations;
+  res->hxv_calls = r.hxv_calls;
+  res->energy_mixed = r.energy;
+  res->residual_mixed = r.residual;
+  res->energy = r.energy;
+  res->residual = r.residual;
+  res->converged = r.converged;
+  if (prec != VMPS_PREC_FP64 && opts->refine_fp64) {
+    vmps::LanczosOpts ro = lo;
+    ro.tol_eigval = opts->refine_tol_eigval;
+    ro.tol_state = opts->refine_tol_state;
+    ro.min_iter = 2;
+    vmps::LanczosResult f = vmps::lanczos_solve(D, d_v, ro, VMPS_PREC_FP64);
+    res->iterations_refine = f.iterations;
+    res->hxv_calls += f.hxv_calls;
+    res->energy = f.energy;
+    res->residual = f.residual;
+    res->converged = f.converged;
+  }
+  res->ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  VMPS_API_END
+}
+
+vmps_status vmps_lanczos_device(vmps_plan* plan, double* d_v, const vmps_lanczos_opts* opts,
+                                vmps_lanczos_result* res) {
+  if (!plan || !d_v) {
+    vmps::set_last_error("NULL argument");
+    return VMPS_ERR_ARG;
+  }
+  return lanczos_impl(plan, d_v, false, opts, res);
+}
+
+vmps_status vmps_lanczos(vmps_plan* plan, double* v, const vmps_lanczos_opts* opts, vmps_lanczos_result* res) {
+  VMPS_API_BEGIN
+  require(plan != nullptr, VMPS_ERR_ARG, "NULL plan");
+  auto& D = plan->D;
+  const size_t bytes = (size_t)D.H.vsize * sizeof(double);
+  if (v && bytes) {
+    for (int64_t i = 0; i < D.H.vsize; ++i)
+      if (!std::isfinite(v[i])) throw VmpsError(VMPS_ERR_NONFINITE, "non-finite entry in the start vector");
+    VMPS_HIP_CHECK(hipMemcpyAsync(D.vin.p, v, bytes, hipMemcpyHostToDevice, D.stream));
+  }
+  vmps_status st = lanczos_impl(plan, D.vin.p, v == nullptr, opts, res);
+  if (st != VMPS_OK) return st;
+  if (v && bytes) {
+    VMPS_HIP_CHECK(hipMemcpyAsync(v, D.vin.p, bytes, hipMemcpyDeviceToHost, D.stream));
+    VMPS_HIP_CHECK(hipStreamSynchronize(D.stream));
+  }
   VMPS_API_END
 }
 

@@ -44,6 +44,16 @@ struct DevBuf {
   DevBuf& operator=(const DevBuf&) = delete;
 };
 
+struct DevPlan;
+
+// Krylov workspace: basis Q ((kmax+1) x n), work vector, coefficients.
+struct Krylov {
+  DevBuf<double> Q, w, c, scal, part;
+  int cap = 0;
+  int64_t nvec = -1;
+  void ensure(DevPlan& D, int kmax);
+};
+
 struct DevPlan {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -66,6 +76,8 @@ struct DevPlan {
   DevBuf<double> seg_w;
   DevBuf<double> partial;  // dot-product partial sums
   DevBuf<double> scalars;  // small device scalars (dot results, projector overlaps)
+
+  Krylov krylov;
 
   // excited-state projectors
   int nproj = 0;

@@ -84,6 +84,10 @@ class MpoSite:
     aux_in: list
     aux_out: list
     blocks: list = field(default_factory=list)
+    # global auxiliary index -> index of the transposed operator string
+    # (W[a, b]^T = W[conj a, conj b]); None when not provided
+    conj_in: list = None
+    conj_out: list = None
 
     @property
     def dw_in(self):
@@ -207,7 +211,7 @@ def heisenberg_u1_mpo(J=1.0, position="bulk"):
         (4, 3, J * Sz, (0,)),
         (4, 4, Id, (0,)),
     ]
-    return _place(U1, qloc, q, ent, position)
+    return _place(U1, qloc, q, ent, position, conj=[0, 2, 1, 3, 4])
 
 
 # ---------------------------------------------------------------------------
@@ -247,13 +251,31 @@ def hubbard_u1xu1_mpo(t=1.0, U=8.0, position="bulk"):
         (5, 4, -t * (F @ a_dn), (-1, 1)),
         (5, 5, Id, (0, 0)),
     ]
-    return _place(U1xU1, qloc, q, ent, position)
+    return _place(U1xU1, qloc, q, ent, position, conj=[0, 2, 1, 4, 3, 5])
 
 
-def _place(sym, qloc, q, ent, position):
+def _sector_order(qs):
+    """Global index -> position in the sector-contiguous numbering of _aux_sectors."""
+    sectors, local = _aux_sectors(qs)
+    off, o = {}, 0
+    for q, d in sectors:
+        off[q] = o
+        o += d
+    return [off[tuple(q)] + l for q, l in zip(qs, local)]
+
+
+def _place(sym, qloc, q, ent, position, conj=None):
     n = len(q)
     if position == "bulk":
-        return build_mpo_site(sym, qloc, q, q, ent)
+        site = build_mpo_site(sym, qloc, q, q, ent)
+        if conj is not None:
+            pos = _sector_order(q)
+            m = [0] * n
+            for g in range(n):
+                m[pos[g]] = pos[conj[g]]
+            site.conj_in = m
+            site.conj_out = m
+        return site
     if position == "first":   # row vector: start row only
         e = _select(ent, [n - 1], None)
         return build_mpo_site(sym, qloc, [q[n - 1]], q, e)

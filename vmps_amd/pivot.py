@@ -77,6 +77,39 @@ class PivotMatrix2:
         return ms.value
 
 
+class DeviceVector:
+    """A PivotVector resident in device memory (the plan's layout)."""
+
+    def __init__(self, H, host=None):
+        self.H = H
+        p = ctypes.c_void_p()
+        _lib.check(H._lib.vmps_vec_alloc(H._plan, ctypes.byref(p)))
+        self.ptr = p.value
+        if host is not None:
+            self.upload(host)
+
+    def upload(self, host):
+        host = np.ascontiguousarray(host, dtype=np.float64)
+        assert host.size == self.H.vsize
+        _lib.check(self.H._lib.vmps_vec_upload(self.H._plan, self.ptr, _ptr(host)))
+
+    def download(self):
+        out = np.empty(self.H.vsize)
+        _lib.check(self.H._lib.vmps_vec_download(self.H._plan, self.ptr, _ptr(out)))
+        return out
+
+    def free(self):
+        if self.ptr:
+            _lib.check(self.H._lib.vmps_vec_free(self.H._plan, self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
 def HxV(H, vin):
     vin = np.ascontiguousarray(vin, dtype=np.float64)
     if vin.size != H.vsize:
@@ -92,6 +125,25 @@ def dot(H, x, y):
     r = ctypes.c_double()
     _lib.check(H._lib.vmps_dot(H._plan, _ptr(x), _ptr(y), ctypes.byref(r)))
     return r.value
+
+
+def lanczos(H, v0=None, precision="fp64", refine_fp64=True, max_iter=100, tol_eigval=1e-8, tol_state=1e-5,
+            refine_tol_eigval=1e-12, refine_tol_state=1e-7, seed=1):
+    """Lowest eigenpair of the pivot (the LanczosSolver::edgeState call of
+    DmrgSolver::iteration_two).  Returns (eigenvector, result dict)."""
+    o = _lib.LanczosOpts()
+    H._lib.vmps_lanczos_opts_default(ctypes.byref(o))
+    o.max_iter, o.tol_eigval, o.tol_state = max_iter, tol_eigval, tol_state
+    o.precision = _lib.PREC[precision]
+    o.refine_fp64 = 1 if refine_fp64 else 0
+    o.refine_tol_eigval, o.refine_tol_state = refine_tol_eigval, refine_tol_state
+    o.seed = seed
+    r = _lib.LanczosResult()
+    if v0 is None:  # GaussianRandomVector::fill: uniform(-1, 1) entries
+        v0 = np.random.default_rng(seed).uniform(-1.0, 1.0, H.vsize)
+    v = np.ascontiguousarray(v0, dtype=np.float64).copy()
+    _lib.check(H._lib.vmps_lanczos(H._plan, _ptr(v), ctypes.byref(o), ctypes.byref(r)))
+    return v, r.as_dict()
 
 
 def get_blockstructure(lib, h, prefix="vmps_"):

@@ -61,8 +61,41 @@ def _flow_ok(sym, a, b, c):
     return sym.validate3(a, b, c)
 
 
-def make_pivot(W12, W34, left, right, rng, env_scale=1.0):
-    """Random-environment pivot for sites with MPO tensors W12, W34."""
+def _symmetrize_env(env, aux, conj, flip):
+    """Impose E[(qin, qout, qmid)][a] = E[(qout, qin, flip(qmid))][a']^T with a'
+    the conjugate auxiliary index, so the effective Hamiltonian is symmetric."""
+    off, o = {}, 0
+    for q, d in aux:
+        off[tuple(q)] = o
+        o += d
+    glob2sec = {}
+    for q, d in aux:
+        for a in range(d):
+            glob2sec[off[tuple(q)] + a] = (tuple(q), a)
+    idx = {(qin, qout, qmid): i for i, (qin, qout, qmid, _) in enumerate(env)}
+    done = set()
+    for i, (qin, qout, qmid, blocks) in enumerate(env):
+        for a in range(len(blocks)):
+            if (i, a) in done:
+                continue
+            qc, ac = glob2sec[conj[off[qmid] + a]]
+            j = idx.get((qout, qin, qc))
+            if j is None:
+                raise ValueError("environment has no conjugate sector")
+            X = blocks[a]
+            Y = env[j][3][ac]
+            S = 0.5 * (X + Y.T)
+            blocks[a] = np.asfortranarray(S)
+            env[j][3][ac] = np.asfortranarray(S.T.copy())
+            done.add((i, a))
+            done.add((j, ac))
+
+
+def make_pivot(W12, W34, left, right, rng, env_scale=1.0, hermitian=False):
+    """Random-environment pivot for sites with MPO tensors W12, W34.  With
+    hermitian=True the environments are symmetrised through the MPO's
+    auxiliary conjugation map, so H_eff is a symmetric matrix (needed by the
+    eigensolver); this requires left == right-compatible sector sets."""
     sym = W12.sym
     pd = PivotData(sym, W12, W34, [(tuple(q), d) for q, d in left], [(tuple(q), d) for q, d in right])
     ldim = dict(pd.left)
@@ -81,6 +114,10 @@ def make_pivot(W12, W34, left, right, rng, env_scale=1.0):
                 if qout in rdim:
                     blocks = [np.asfortranarray(rng.standard_normal((din, rdim[qout])) * env_scale) for _ in range(adim)]
                     pd.R.append((qin, qout, tuple(qmid), blocks))
+    if hermitian:
+        flip = lambda q: tuple(-x for x in q)
+        _symmetrize_env(pd.L, W12.aux_in, W12.conj_in, flip)
+        _symmetrize_env(pd.R, W34.aux_out, W34.conj_out, flip)
     pd.basis = two_site_basis(sym, W12.qloc, W34.qloc)
     pd.V = []
     for (qm, s1, s3) in pd.basis:
@@ -185,7 +222,7 @@ WORKLOADS = {
 }
 
 
-def make_workload(name, seed=20261202, **override):
+def make_workload(name, seed=20261202, hermitian=False, **override):
     cfg = dict(WORKLOADS[name])
     cfg.update(override)
     rng = np.random.default_rng(seed)
@@ -201,7 +238,7 @@ def make_workload(name, seed=20261202, **override):
         right = hubbard_bond(loc + 2, cfg["M"])
     else:
         raise ValueError(cfg["model"])
-    return make_pivot(W, W, left, right, rng)
+    return make_pivot(W, W, left, right, rng, hermitian=hermitian)
 
 
 # ---------------------------------------------------------------------------
