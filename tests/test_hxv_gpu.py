@@ -17,6 +17,7 @@ pytestmark = pytest.mark.gpu
 
 FP64_RTOL = 1e-12   # ||Vout_gpu - Vout_oracle||_2 / ||Vout_oracle||_2
 FP32_RTOL = 1e-5
+BF16X3_RTOL = 5e-5  # hi/lo split keeps ~16 mantissa bits per operand
 
 
 def _rel(a, b):
@@ -121,6 +122,21 @@ def test_fp32_mixed_error(hub):
     rel = _rel(HxV(H, v), _oracle_out(hub, v))
     print("fp32 rel err", rel)
     assert rel <= FP32_RTOL
+
+
+@pytest.mark.parametrize("name", [None, "hubbard_u1xu1_L48_M512"])
+def test_bf16x3_mixed_error(hub, name):
+    pd = hub if name is None else synth.make_workload(name)
+    H = PivotMatrix2(pd, precision="bf16x3")
+    v = synth.random_vector(pd, 12)
+    ref = _oracle_out(pd, v, nthreads=16)
+    out = HxV(H, v)
+    rel = _rel(out, ref)
+    print("bf16x3 rel err", rel)
+    assert rel <= BF16X3_RTOL
+    # switching precision on the same plan gives the FP64 drop-in again
+    H.set_precision("fp64")
+    assert _rel(HxV(H, v), ref) <= FP64_RTOL
 
 
 def test_nonfinite_input_rejected(hub):

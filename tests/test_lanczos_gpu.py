@@ -42,13 +42,14 @@ def test_lanczos_fp64_vs_dense_eigensolver(kind):
     assert abs(abs(v @ evecs[:, 0]) - 1.0) <= 1e-8  # same eigenvector (non-degenerate ground state)
 
 
-def test_lanczos_mixed_fp32_refined_to_fp64():
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
+def test_lanczos_mixed_refined_to_fp64(prec):
     pd = _herm_small("hub", seed=5)
     n = pd.vsize
     Hm = np.stack([dense.dense_hxv(pd, np.eye(n)[i]) for i in range(n)], axis=1)
     e0 = np.linalg.eigvalsh(0.5 * (Hm + Hm.T))[0]
-    H = PivotMatrix2(pd, precision="fp32")
-    v, r = lanczos(H, precision="fp32", max_iter=min(n, 120), tol_eigval=1e-6, tol_state=1e-4)
+    H = PivotMatrix2(pd, precision=prec)
+    v, r = lanczos(H, precision=prec, max_iter=min(n, 120), tol_eigval=1e-6, tol_state=1e-4)
     print(r, e0)
     assert abs(r["energy_mixed"] - e0) <= 1e-4 * abs(e0)   # FP32 operator
     assert abs(r["energy"] - e0) <= 1e-9 * max(1.0, abs(e0))  # after FP64 refinement
@@ -69,10 +70,11 @@ def test_lanczos_workload_vs_scipy_on_oracle():
     print(r, e_ref)
     assert abs(r["energy"] - e_ref) <= 1e-9 * abs(e_ref)
     # mixed precision + FP64 refinement reaches |dE| <= 1e-9 relative as well
-    Hm = PivotMatrix2(pd, precision="fp32")
-    v2, r2 = lanczos(Hm, precision="fp32", max_iter=300, tol_eigval=1e-7, tol_state=1e-4)
-    print(r2)
-    assert abs(r2["energy"] - e_ref) <= 1e-9 * abs(e_ref)
+    for prec in ("fp32", "bf16x3"):
+        Hm = PivotMatrix2(pd, precision=prec)
+        v2, r2 = lanczos(Hm, precision=prec, max_iter=300, tol_eigval=1e-7, tol_state=1e-4)
+        print(prec, r2)
+        assert abs(r2["energy"] - e_ref) <= 1e-9 * abs(e_ref)
 
 
 def test_vector_algebra_device():

@@ -15,7 +15,7 @@ OUT = os.path.join(HERE, "libvmps_hxv.so")
 BUILD = os.path.join(HERE, "_build")
 ARCH = os.environ.get("VMPS_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["sym.cpp", "pivot.cpp", "capi.cpp", "engine.hip", "lanczos.hip"]
+SOURCES = ["sym.cpp", "pivot.cpp", "capi.cpp", "engine.hip", "engine_bf16.hip", "lanczos.hip"]
 # C++ units that call the HIP runtime API are compiled in HIP mode
 HIP_HOST_UNITS = {"capi.cpp"}
 

@@ -314,7 +314,7 @@ void DevPlan::ensure_precision_buffers() {
     if (R.n) hipLaunchKernelGGL(k_f64_to_f32, dim3(1024), dim3(256), 0, stream, R.p, R32.p, (int64_t)R.n);
     VMPS_HIP_CHECK(hipGetLastError());
   }
-  if (prec == VMPS_PREC_BF16X3) throw VmpsError(VMPS_ERR_ARG, "BF16x3 precision is not available in this build");
+  if (prec == VMPS_PREC_BF16X3) ensure_bf16_buffers();
 }
 
 void DevPlan::launch_stage(int stage, const double* d_vin, double* d_vout, hipStream_t s) {
@@ -336,6 +336,8 @@ void DevPlan::launch_stage(int stage, const double* d_vin, double* d_vout, hipSt
     if (stage == 2 && nC)
       hipLaunchKernelGGL((k_stage_c<float, double>), dim3(nC), dim3(256), 0, s, tilesC.p, nC, chunksC.p, termsC.p,
                          T32.p, R32.p, d_vout);
+  } else if (prec == VMPS_PREC_BF16X3) {
+    launch_stage_bf16x3(stage, d_vin, d_vout, s);
   } else {
     throw VmpsError(VMPS_ERR_ARG, "unsupported precision");
   }

@@ -66,6 +66,14 @@ struct DevPlan {
   // FP32 operands and intermediates (mixed modes)
   DevBuf<float> L32, R32, V32, T32;
 
+  // split-BF16 x3 operands, intermediates and work lists
+  DevBuf<uint16_t> Lhi, Llo, Rhi, Rlo, Vhi, Vlo;
+  DevBuf<float> Tp;
+  DevBuf<TileA16> bfA;
+  DevBuf<ChunkC16> bfC;
+  DevBuf<TermC16> bfT;
+  DevBuf<VConv> vconv;
+
   // work lists
   DevBuf<TileA> tilesA;
   DevBuf<TileC> tilesC;
@@ -87,6 +95,8 @@ struct DevPlan {
   ~DevPlan();
   void create(const Pivot2& P, vmps_precision prec);
   void ensure_precision_buffers();
+  void ensure_bf16_buffers();
+  void launch_stage_bf16x3(int stage, const double* d_vin, double* d_vout, hipStream_t s);
   void hxv(const double* d_vin, double* d_vout, hipStream_t s);
   // one stage of hxv alone: 0 = L.V, 2 = (sum c T).R; stage 1 (the combine) is fused into 2
   void launch_stage(int stage, const double* d_vin, double* d_vout, hipStream_t s);

@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <stdexcept>
 #include <unordered_map>
 
@@ -460,12 +461,14 @@ void build_host_plan(const Pivot2& P, HostPlan& H) {
   for (size_t i = 0; i < lblocks.size(); ++i) {
     const DenseBlock& b = P.L.block[lblocks[i].first][lblocks[i].second];
     loff[i] = (int64_t)H.Ldata.size();
+    H.Lblk.push_back({loff[i], b.rows, b.cols});
     H.Ldata.insert(H.Ldata.end(), P.L.data.begin() + b.off, P.L.data.begin() + b.off + b.size());
     st.max_block_dim = std::max({st.max_block_dim, b.rows, b.cols});
   }
   for (size_t i = 0; i < rblocks.size(); ++i) {
     const DenseBlock& b = P.R.block[rblocks[i].first][rblocks[i].second];
     roff[i] = (int64_t)H.Rdata.size();
+    H.Rblk.push_back({roff[i], b.rows, b.cols});
     H.Rdata.insert(H.Rdata.end(), P.R.data.begin() + b.off, P.R.data.begin() + b.off + b.size());
     st.max_block_dim = std::max({st.max_block_dim, b.rows, b.cols});
   }
@@ -588,6 +591,7 @@ void build_host_plan(const Pivot2& P, HostPlan& H) {
       const DenseBlock& b = P.V.block[s][q];
       H.vblock_off.push_back(b.off);
       H.vblock_size.push_back(b.size());
+      H.Vblk.push_back({b.off, b.rows, b.cols});
       double w = P.sym.coeff_dot(P.V.out[s][q]);
       for (int64_t e = 0; e < b.size(); e += 4096) {
         H.seg_off.push_back(b.off + e);
@@ -611,6 +615,123 @@ void build_host_plan(const Pivot2& P, HostPlan& H) {
   st.T_elems = H.T_size;
   st.U_elems = H.U_size;
   st.bytes_alg_fp64 = 8.0 * (double)(st.L_elems + st.R_elems + 2 * st.V_elems);
+}
+
+}  // namespace vmps
+
+namespace vmps {
+
+// float -> bfloat16, round to nearest even (NaN kept quiet)
+uint16_t bf16_rne(float f) {
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
+  const uint32_t lsb = (u >> 16) & 1u;
+  u += 0x7fffu + lsb;
+  return (uint16_t)(u >> 16);
+}
+
+namespace {
+float bf16_to_f32(uint16_t h) {
+  uint32_t u = (uint32_t)h << 16;
+  float f;
+  std::memcpy(&f, &u, 4);
+  return f;
+}
+void split(double x, uint16_t& hi, uint16_t& lo) {
+  hi = bf16_rne((float)x);
+  lo = bf16_rne((float)(x - (double)bf16_to_f32(hi)));
+}
+}  // namespace
+
+void build_bf16_layout(const HostPlan& H, Bf16Layout& B) {
+  B = Bf16Layout();
+  // L^T, row-major M x Kp at 8-aligned offsets
+  std::unordered_map<int64_t, int64_t> Lmap, Rmap, Vmap, Tmap;
+  int64_t o = 0;
+  for (const BlockDim& b : H.Lblk) {
+    Lmap[b.off] = o;
+    o += b.rows * pad8(b.cols);
+  }
+  B.Lhi.assign(o, 0);
+  B.Llo.assign(o, 0);
+  for (const BlockDim& b : H.Lblk) {
+    const int64_t Kp = pad8(b.cols), base = Lmap[b.off];
+    for (int64_t i = 0; i < b.rows; ++i)
+      for (int64_t k = 0; k < b.cols; ++k)
+        split(H.Ldata[b.off + k * b.rows + i], B.Lhi[base + i * Kp + k], B.Llo[base + i * Kp + k]);
+  }
+  // R, column-major Kp x N
+  o = 0;
+  for (const BlockDim& b : H.Rblk) {
+    Rmap[b.off] = o;
+    o += pad8(b.rows) * b.cols;
+  }
+  B.Rhi.assign(o, 0);
+  B.Rlo.assign(o, 0);
+  for (const BlockDim& b : H.Rblk) {
+    const int64_t Kp = pad8(b.rows), base = Rmap[b.off];
+    for (int64_t j = 0; j < b.cols; ++j)
+      for (int64_t k = 0; k < b.rows; ++k)
+        split(H.Rdata[b.off + j * b.rows + k], B.Rhi[base + j * Kp + k], B.Rlo[base + j * Kp + k]);
+  }
+  // V, column-major Kp x N (filled per call)
+  o = 0;
+  for (const BlockDim& b : H.Vblk) {
+    if (b.rows * b.cols == 0) continue;
+    Vmap[b.off] = o;
+    B.vconv.push_back({b.off, o, (int32_t)b.rows, (int32_t)b.cols, (int32_t)pad8(b.rows), 0});
+    o += pad8(b.rows) * b.cols;
+  }
+  B.V_size = o;
+  // T, row-major M x Np
+  o = 0;
+  for (size_t t = 0; t < H.probA.size(); ++t) {
+    Tmap[H.probA[t].c_off] = o;
+    o += (int64_t)H.probA[t].M * pad8(H.probA[t].N);
+  }
+  B.T_size = o;
+  for (const TileA& t : H.tilesA) {
+    TileA16 a;
+    a.a_off = Lmap.at(t.a_off);
+    a.b_off = Vmap.at(t.b_off);
+    a.c_off = Tmap.at(t.c_off);
+    a.M = t.M;
+    a.N = t.N;
+    a.Kp = (int32_t)pad8(t.K);
+    a.Np = (int32_t)pad8(t.N);
+    a.mt = t.mt;
+    a.nt = t.nt;
+    a.pad = 0;
+    B.tilesA.push_back(a);
+  }
+  for (const ChunkC& c : H.chunksC) {
+    ChunkC16 d;
+    d.r_off = Rmap.at(c.r_off);
+    d.Kp = (int32_t)pad8(c.K);
+    d.term_begin = c.term_begin;
+    d.term_end = c.term_end;
+    d.pad = 0;
+    B.chunksC.push_back(d);
+  }
+  for (const TermC& t : H.termsC) B.termsC.push_back({Tmap.at(t.t_off), (float)t.c, 0});
+  // bounds: every padded access stays inside its array
+  for (const TileA16& a : B.tilesA)
+    if (a.a_off + (int64_t)a.M * a.Kp > (int64_t)B.Lhi.size() || a.b_off + (int64_t)a.Kp * a.N > B.V_size ||
+        a.c_off + (int64_t)a.M * a.Np > B.T_size || (a.a_off | a.b_off | a.c_off) % 8)
+      throw VmpsError(VMPS_ERR_INTERNAL, "bf16 stage A descriptor out of range");
+  for (const TileC& t : H.tilesC)
+    for (int32_t c = t.chunk_begin; c < t.chunk_end; ++c) {
+      const ChunkC16& ch = B.chunksC[c];
+      if (ch.r_off % 8 || ch.r_off + (int64_t)ch.Kp * t.N > (int64_t)B.Rhi.size())
+        throw VmpsError(VMPS_ERR_INTERNAL, "bf16 stage C operand out of range");
+      for (int32_t j = ch.term_begin; j < ch.term_end; ++j)
+        if (B.termsC[j].t_off % 8 || B.termsC[j].t_off + (int64_t)t.M * ch.Kp > B.T_size)
+          throw VmpsError(VMPS_ERR_INTERNAL, "bf16 stage C term out of range");
+    }
+  for (const VConv& v : B.vconv)
+    if (v.dst_off + (int64_t)v.Kp * v.N > B.V_size || v.src_off + (int64_t)v.K * v.N > H.vsize)
+      throw VmpsError(VMPS_ERR_INTERNAL, "bf16 conversion task out of range");
 }
 
 }  // namespace vmps

@@ -183,9 +183,48 @@ struct TermC {             // c_j T_j with T_j an M x K block at t_off
   double c;
 };
 
+// Split-BF16 layouts: k-contiguous operands with the k extent padded to a
+// multiple of 8 (so an MFMA lane's 8 consecutive k are one aligned 16-byte
+// load and the padding supplies the zeros past K).
+inline int64_t pad8(int64_t x) { return (x + 7) & ~int64_t(7); }
+struct TileA16 {           // T(M x Np, row-major) tile = Lt(M x Kp, row-major) . V(Kp x N, col-major)
+  int64_t a_off, b_off, c_off;
+  int32_t M, N, Kp, Np;
+  int16_t mt, nt;
+  int32_t pad;
+};
+struct ChunkC16 {          // (U_u, R_r): R padded col-major (Kp x N), T terms row-major (M x Kp)
+  int64_t r_off;
+  int32_t Kp;
+  int32_t term_begin, term_end;
+  int32_t pad;
+};
+struct TermC16 {
+  int64_t t_off;
+  float c;
+  int32_t pad;
+};
+struct VConv {             // V block: packed FP64 (K x N, ld K) -> padded split BF16 (Kp x N)
+  int64_t src_off, dst_off;
+  int32_t K, N, Kp, pad;
+};
+struct Bf16Layout {
+  std::vector<uint16_t> Lhi, Llo, Rhi, Rlo;  // split operands, built once
+  int64_t V_size = 0, T_size = 0;            // padded sizes of the per-call arrays
+  std::vector<TileA16> tilesA;
+  std::vector<ChunkC16> chunksC;
+  std::vector<TermC16> termsC;
+  std::vector<VConv> vconv;
+};
+
+struct BlockDim {
+  int64_t off, rows, cols;
+};
+
 struct HostPlan {
   // packed operands actually referenced by terms
   std::vector<double> Ldata, Rdata;  // copied L/R blocks
+  std::vector<BlockDim> Lblk, Rblk, Vblk;  // blocks of Ldata, Rdata and the vector
   int64_t vsize = 0;
   // device work lists (built from the tables below by build_tiles)
   std::vector<TileA> tilesA;
@@ -222,6 +261,9 @@ void expand_terms(const Pivot2& P, std::vector<Term>& terms,
 
 void build_host_plan(const Pivot2& P, HostPlan& H);
 void build_tiles(HostPlan& H);
+// Split-BF16 operands and work lists of the BF16x3 path (built on demand).
+void build_bf16_layout(const HostPlan& H, Bf16Layout& B);
+uint16_t bf16_rne(float f);
 // Throws VMPS_ERR_INTERNAL if any descriptor would address outside its buffer.
 void validate_host_plan(const HostPlan& H);
 
