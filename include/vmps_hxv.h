@@ -133,6 +133,10 @@ typedef struct {
   double flops_C;   /* stage C share of flops_alg: Vout += U . R */
 } vmps_plan_stats;
 
+/* Work counts of the staged plan of a pivot, computed on the host only (no
+ * device needed): the numbers vmps_plan_stats_get reports for its plan. */
+vmps_status vmps_pivot2_stats(const vmps_pivot2* p, vmps_plan_stats* st);
+
 /* Flatten the pivot into the staged block-triple tables and upload L, R and
  * the tables to the current HIP device.  The pivot may be destroyed after. */
 vmps_status vmps_plan_create(const vmps_pivot2* p, vmps_precision prec, vmps_plan** out);

@@ -124,7 +124,7 @@ def test_fp32_mixed_error(hub):
     assert rel <= FP32_RTOL
 
 
-@pytest.mark.parametrize("name", [None, "hubbard_u1xu1_L48_M512"])
+@pytest.mark.parametrize("name", [None, "hubbard_u1xu1_L48_M512", "heisenberg_u1_L64_M2048"])
 def test_bf16x3_mixed_error(hub, name):
     pd = hub if name is None else synth.make_workload(name)
     H = PivotMatrix2(pd, precision="bf16x3")

@@ -80,6 +80,7 @@ SIGNATURES = {
     "vmps_pivot2_set_blockstructure": (c_int, [c_void, c_i64, P_i64, P_i64, P_i64, P_dbl]),
     "vmps_pivot2_get_blockstructure": (c_int, [c_void, P_i64, P_i64, P_i64, P_i64, P_i64, P_dbl]),
     "vmps_pivot2_set_projectors": (c_int, [c_void, c_dbl, c_int, P_dbl]),
+    "vmps_pivot2_stats": (c_int, [c_void, ctypes.POINTER(PlanStats)]),
     "vmps_plan_create": (c_int, [c_void, c_int, ctypes.POINTER(c_void)]),
     "vmps_plan_destroy": (None, [c_void]),
     "vmps_plan_stats_get": (c_int, [c_void, ctypes.POINTER(PlanStats)]),

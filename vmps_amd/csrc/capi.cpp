@@ -302,10 +302,7 @@ vmps_status vmps_plan_create(const vmps_pivot2* p, vmps_precision prec, vmps_pla
 
 void vmps_plan_destroy(vmps_plan* plan) { delete plan; }
 
-vmps_status vmps_plan_stats_get(const vmps_plan* plan, vmps_plan_stats* st) {
-  VMPS_API_BEGIN
-  require(plan && st, VMPS_ERR_ARG, "NULL argument");
-  const auto& s = plan->D.H.stats;
+static void copy_stats(const vmps::Stats& s, vmps_plan_stats* st) {
   st->n_terms = s.n_terms;
   st->n_T = s.n_T;
   st->n_U = s.n_U;
@@ -324,6 +321,21 @@ vmps_status vmps_plan_stats_get(const vmps_plan* plan, vmps_plan_stats* st) {
   st->max_block_dim = s.max_block_dim;
   st->flops_A = s.flops_A;
   st->flops_C = s.flops_C;
+}
+
+vmps_status vmps_plan_stats_get(const vmps_plan* plan, vmps_plan_stats* st) {
+  VMPS_API_BEGIN
+  require(plan && st, VMPS_ERR_ARG, "NULL argument");
+  copy_stats(plan->D.H.stats, st);
+  VMPS_API_END
+}
+
+vmps_status vmps_pivot2_stats(const vmps_pivot2* p, vmps_plan_stats* st) {
+  VMPS_API_BEGIN
+  require(p && st, VMPS_ERR_ARG, "NULL argument");
+  vmps::HostPlan H;
+  vmps::build_host_plan(p->P, H);
+  copy_stats(H.stats, st);
   VMPS_API_END
 }
 

@@ -69,7 +69,11 @@ struct DevPlan {
   // split-BF16 x3 operands, intermediates and work lists
   DevBuf<uint16_t> Lhi, Llo, Rhi, Rlo, Vhi, Vlo;
   DevBuf<float> Tp;
-  DevBuf<TileA16> bfA;
+  DevBuf<TileA16> bfA, bfAbig;
+  DevBuf<TileC> bfCsmall, bfCbig;
+  DevBuf<int32_t> wTile, wBegin, wEnd, rTile, rNs;  // split-K work items / reductions
+  DevBuf<int64_t> wSlot, rSlot;
+  DevBuf<float> Cpart;                              // FP32 partial tiles
   DevBuf<ChunkC16> bfC;
   DevBuf<TermC16> bfT;
   DevBuf<VConv> vconv;

@@ -39,13 +39,14 @@ __device__ inline void split8(const float (&x)[8], v8bf& hi, v8bf& lo) {
   }
 }
 
-// Packed FP64 V blocks -> padded split-BF16 copies (one workgroup per block).
+// Packed FP64 V columns -> padded split-BF16 copies (one workgroup per task of
+// at most 4096 padded elements).
 __global__ __launch_bounds__(256) void k_split_v(const VConv* __restrict__ tasks, const double* __restrict__ vin,
                                                  __bf16* __restrict__ Vhi, __bf16* __restrict__ Vlo) {
   const VConv t = tasks[blockIdx.x];
-  const int64_t n = (int64_t)t.Kp * t.N;
-  for (int64_t e = threadIdx.x; e < n; e += 256) {
-    const int j = (int)(e / t.Kp), k = (int)(e % t.Kp);
+  const int n = t.Kp * t.N;
+  for (int e = threadIdx.x; e < n; e += 256) {
+    const int j = e / t.Kp, k = e - j * t.Kp;
     const double x = (k < t.K) ? vin[t.src_off + (int64_t)j * t.K + k] : 0.0;
     const __bf16 h = (__bf16)(float)x;
     Vhi[t.dst_off + e] = h;
@@ -200,6 +201,211 @@ __global__ __launch_bounds__(256) void k_stage_c_bf16x3(const TileC* __restrict_
   }
 }
 
+// ---- LDS-staged 128 x 128 workgroup tiles for large blocks -------------------
+// 4 waves in a 2 x 2 grid, each owning 64 x 64 = 4 x 4 MFMA blocks.  Per 32-deep
+// K step the workgroup stages the split operand tiles A[128 x 32] and
+// B[32 x 128] (hi and lo) in LDS, k-contiguous with rows padded to 40
+// elements, and each wave then issues 4 x 4 x 3 MFMAs from LDS reads.
+constexpr int kBT = kBigTile;
+constexpr int kBK = 32;
+constexpr int kLS = 40;  // LDS row stride in bf16 elements (80 bytes)
+
+struct BigSmem {
+  __bf16 Ah[kBT * kLS], Al[kBT * kLS], Bh[kBT * kLS], Bl[kBT * kLS];
+};
+
+__device__ inline void big_compute(const BigSmem& s, int wm, int wn, int li, int lg, v4f (&acc)[4][4]) {
+  v8bf ah[4], al[4], bh[4], bl[4];
+#pragma unroll
+  for (int x = 0; x < 4; ++x) {
+    ah[x] = *reinterpret_cast<const v8bf*>(&s.Ah[(wm + 16 * x + li) * kLS + 8 * lg]);
+    al[x] = *reinterpret_cast<const v8bf*>(&s.Al[(wm + 16 * x + li) * kLS + 8 * lg]);
+    bh[x] = *reinterpret_cast<const v8bf*>(&s.Bh[(wn + 16 * x + li) * kLS + 8 * lg]);
+    bl[x] = *reinterpret_cast<const v8bf*>(&s.Bl[(wn + 16 * x + li) * kLS + 8 * lg]);
+  }
+#pragma unroll
+  for (int mi = 0; mi < 4; ++mi)
+#pragma unroll
+    for (int ni = 0; ni < 4; ++ni) acc[mi][ni] = mac3(bh[ni], bl[ni], ah[mi], al[mi], acc[mi][ni]);
+}
+
+// Stage A, large problems: Tp (row-major M x Np) = Lt (row-major M x Kp) . V (col-major Kp x N)
+__global__ __launch_bounds__(256) void k_stage_a_bf16x3_big(const TileA16* __restrict__ tiles, int ntiles,
+                                                            const __bf16* __restrict__ Lhi,
+                                                            const __bf16* __restrict__ Llo,
+                                                            const __bf16* __restrict__ Vhi,
+                                                            const __bf16* __restrict__ Vlo, float* __restrict__ Tp) {
+  __shared__ BigSmem s;
+  if ((int)blockIdx.x >= ntiles) return;  // workgroup-uniform
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int li = lane & 15, lg = lane >> 4;
+  const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
+  const TileA16 t = tiles[blockIdx.x];
+  const int m0 = t.mt * kBT, n0 = t.nt * kBT;
+  const v8bf z = {};
+  v4f acc[4][4];
+#pragma unroll
+  for (int mi = 0; mi < 4; ++mi)
+#pragma unroll
+    for (int ni = 0; ni < 4; ++ni) acc[mi][ni] = v4f{0, 0, 0, 0};
+  for (int kb = 0; kb < t.Kp; kb += kBK) {
+    v8bf ah[2], al[2], bh[2], bl[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {  // 512 16-byte chunks per operand tile, 2 per thread
+      const int c = tid + 256 * h, row = c >> 2, kc = (c & 3) * 8;
+      const int k = kb + kc;
+      const bool ka = (m0 + row < t.M) && (k < t.Kp), kbv = (n0 + row < t.N) && (k < t.Kp);
+      ah[h] = ka ? ld8(Lhi + t.a_off + (int64_t)(m0 + row) * t.Kp + k) : z;
+      al[h] = ka ? ld8(Llo + t.a_off + (int64_t)(m0 + row) * t.Kp + k) : z;
+      bh[h] = kbv ? ld8(Vhi + t.b_off + (int64_t)(n0 + row) * t.Kp + k) : z;
+      bl[h] = kbv ? ld8(Vlo + t.b_off + (int64_t)(n0 + row) * t.Kp + k) : z;
+    }
+    __syncthreads();  // the previous step's LDS reads are done
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int c = tid + 256 * h, row = c >> 2, kc = (c & 3) * 8;
+      *reinterpret_cast<v8bf*>(&s.Ah[row * kLS + kc]) = ah[h];
+      *reinterpret_cast<v8bf*>(&s.Al[row * kLS + kc]) = al[h];
+      *reinterpret_cast<v8bf*>(&s.Bh[row * kLS + kc]) = bh[h];
+      *reinterpret_cast<v8bf*>(&s.Bl[row * kLS + kc]) = bl[h];
+    }
+    __syncthreads();
+    big_compute(s, wm, wn, li, lg, acc);
+  }
+  float* __restrict__ C = Tp + t.c_off;
+#pragma unroll
+  for (int mi = 0; mi < 4; ++mi) {
+    const int i = m0 + wm + 16 * mi + li;
+    if (i >= t.M) continue;
+#pragma unroll
+    for (int ni = 0; ni < 4; ++ni)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int j = n0 + wn + 16 * ni + 4 * lg + v;
+        if (j < t.N) C[(int64_t)i * t.Np + j] = acc[mi][ni][v];
+      }
+  }
+}
+
+// Stage C, large output blocks: Vout (FP64 col-major M x N) = sum_chunks (sum_j c_j T_j) . R.
+// Work item w covers iterations [it_begin, it_end) of its tile's flattened
+// (chunk, 32-deep k block) loop; with slot >= 0 it writes an FP32 partial tile
+// (column-major 128 x 128) that k_reduce_partials sums in slot order.
+__global__ __launch_bounds__(256) void k_stage_c_bf16x3_big(const TileC* __restrict__ tiles,
+                                                            const int32_t* __restrict__ w_tile,
+                                                            const int32_t* __restrict__ w_begin,
+                                                            const int32_t* __restrict__ w_end,
+                                                            const int64_t* __restrict__ w_slot, int nwork,
+                                                            const ChunkC16* __restrict__ chunks,
+                                                            const TermC16* __restrict__ terms,
+                                                            const float* __restrict__ Tp,
+                                                            const __bf16* __restrict__ Rhi,
+                                                            const __bf16* __restrict__ Rlo,
+                                                            double* __restrict__ Vout, float* __restrict__ partial) {
+  __shared__ BigSmem s;
+  if ((int)blockIdx.x >= nwork) return;  // workgroup-uniform
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int li = lane & 15, lg = lane >> 4;
+  const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
+  const TileC t = tiles[w_tile[blockIdx.x]];
+  const int it0 = w_begin[blockIdx.x], it1 = w_end[blockIdx.x];
+  const int64_t slot = w_slot[blockIdx.x];
+  const int m0 = t.mt * kBT, n0 = t.nt * kBT;
+  const v8bf z = {};
+  v4f acc[4][4];
+#pragma unroll
+  for (int mi = 0; mi < 4; ++mi)
+#pragma unroll
+    for (int ni = 0; ni < 4; ++ni) acc[mi][ni] = v4f{0, 0, 0, 0};
+  int it = 0;
+  for (int c = t.chunk_begin; c < t.chunk_end && it < it1; ++c) {
+    const ChunkC16 ch = chunks[c];
+    const int nsteps = (ch.Kp + kBK - 1) / kBK;
+    if (it + nsteps <= it0) {
+      it += nsteps;
+      continue;
+    }
+    for (int kb = 0; kb < ch.Kp; kb += kBK, ++it) {
+      if (it < it0 || it >= it1) continue;  // workgroup-uniform
+      v8bf ah[2], al[2], bh[2], bl[2];
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int cc = tid + 256 * h, row = cc >> 2, kc = (cc & 3) * 8;
+        const int k = kb + kc;
+        const bool ka = (m0 + row < t.M) && (k < ch.Kp), kbv = (n0 + row < t.N) && (k < ch.Kp);
+        float u[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) u[e] = 0.f;
+        if (ka) {
+          for (int j = ch.term_begin; j < ch.term_end; ++j) {
+            const TermC16 tm = terms[j];
+            const float* __restrict__ Tt = Tp + tm.t_off + (int64_t)(m0 + row) * ch.Kp + k;
+            const v4f x = *reinterpret_cast<const v4f*>(Tt);
+            const v4f y = *reinterpret_cast<const v4f*>(Tt + 4);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+              u[e] += tm.c * x[e];
+              u[e + 4] += tm.c * y[e];
+            }
+          }
+        }
+        split8(u, ah[h], al[h]);
+        bh[h] = kbv ? ld8(Rhi + ch.r_off + (int64_t)(n0 + row) * ch.Kp + k) : z;
+        bl[h] = kbv ? ld8(Rlo + ch.r_off + (int64_t)(n0 + row) * ch.Kp + k) : z;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int cc = tid + 256 * h, row = cc >> 2, kc = (cc & 3) * 8;
+        *reinterpret_cast<v8bf*>(&s.Ah[row * kLS + kc]) = ah[h];
+        *reinterpret_cast<v8bf*>(&s.Al[row * kLS + kc]) = al[h];
+        *reinterpret_cast<v8bf*>(&s.Bh[row * kLS + kc]) = bh[h];
+        *reinterpret_cast<v8bf*>(&s.Bl[row * kLS + kc]) = bl[h];
+      }
+      __syncthreads();
+      big_compute(s, wm, wn, li, lg, acc);
+    }
+  }
+  double* __restrict__ C = Vout + t.c_off;
+  float* __restrict__ P = partial + (slot < 0 ? 0 : slot * (int64_t)(kBT * kBT));
+#pragma unroll
+  for (int mi = 0; mi < 4; ++mi) {
+    const int r = wm + 16 * mi + li, i = m0 + r;
+    if (i >= t.M) continue;
+#pragma unroll
+    for (int ni = 0; ni < 4; ++ni)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int cl = wn + 16 * ni + 4 * lg + v, j = n0 + cl;
+        if (j >= t.N) continue;
+        if (slot < 0)
+          C[(int64_t)j * t.M + i] = (double)acc[mi][ni][v];
+        else
+          P[cl * kBT + r] = acc[mi][ni][v];
+      }
+  }
+}
+
+// Vout tile = sum over its split partials, in slot order.
+__global__ __launch_bounds__(256) void k_reduce_partials(const TileC* __restrict__ tiles,
+                                                         const int32_t* __restrict__ r_tile,
+                                                         const int64_t* __restrict__ r_slot,
+                                                         const int32_t* __restrict__ r_nsplit,
+                                                         const float* __restrict__ partial, double* __restrict__ Vout) {
+  const TileC t = tiles[r_tile[blockIdx.x]];
+  const int64_t s0 = r_slot[blockIdx.x];
+  const int ns = r_nsplit[blockIdx.x];
+  const int m0 = t.mt * kBT, n0 = t.nt * kBT;
+  for (int e = threadIdx.x; e < kBT * kBT; e += 256) {
+    const int cl = e / kBT, r = e - cl * kBT;
+    const int i = m0 + r, j = n0 + cl;
+    if (i >= t.M || j >= t.N) continue;
+    float acc = 0.f;
+    for (int s = 0; s < ns; ++s) acc += partial[(s0 + s) * (int64_t)(kBT * kBT) + e];
+    Vout[t.c_off + (int64_t)j * t.M + i] = (double)acc;
+  }
+}
+
 }  // namespace
 
 void DevPlan::ensure_bf16_buffers() {
@@ -215,6 +421,17 @@ void DevPlan::ensure_bf16_buffers() {
   Tp.alloc(std::max<int64_t>(1, B.T_size));
   VMPS_HIP_CHECK(hipMemsetAsync(Tp.p, 0, Tp.n * sizeof(float), stream));  // padding columns stay zero
   bfA.upload(B.tilesA, stream);
+  bfAbig.upload(B.tilesA_big, stream);
+  bfCsmall.upload(B.tilesC_small, stream);
+  bfCbig.upload(B.tilesC_big, stream);
+  wTile.upload(B.splitC_tile, stream);
+  wBegin.upload(B.splitC_begin, stream);
+  wEnd.upload(B.splitC_end, stream);
+  wSlot.upload(B.splitC_slot, stream);
+  rTile.upload(B.redC_tile, stream);
+  rSlot.upload(B.redC_slot, stream);
+  rNs.upload(B.redC_nsplit, stream);
+  Cpart.alloc(std::max<int64_t>(1, B.partial_slots) * (int64_t)kBigTile * kBigTile);
   bfC.upload(B.chunksC, stream);
   bfT.upload(B.termsC, stream);
   vconv.upload(B.vconv, stream);
@@ -222,17 +439,29 @@ void DevPlan::ensure_bf16_buffers() {
 }
 
 void DevPlan::launch_stage_bf16x3(int stage, const double* d_vin, double* d_vout, hipStream_t s) {
-  const int nA = (int)bfA.n, nC = (int)tilesC.n;
+  const int nA = (int)bfA.n, nAb = (int)bfAbig.n, nC = (int)bfCsmall.n, nCb = (int)bfCbig.n;
   if (stage == 0) {
     if (vconv.n)
       hipLaunchKernelGGL(k_split_v, dim3((unsigned)vconv.n), dim3(256), 0, s, vconv.p, d_vin, (__bf16*)Vhi.p,
                          (__bf16*)Vlo.p);
+    if (nAb)
+      hipLaunchKernelGGL(k_stage_a_bf16x3_big, dim3((unsigned)nAb), dim3(256), 0, s, bfAbig.p, nAb,
+                         (const __bf16*)Lhi.p, (const __bf16*)Llo.p, (const __bf16*)Vhi.p, (const __bf16*)Vlo.p, Tp.p);
     if (nA)
       hipLaunchKernelGGL(k_stage_a_bf16x3, dim3((unsigned)((nA + 3) / 4)), dim3(256), 0, s, bfA.p, nA,
                          (const __bf16*)Lhi.p, (const __bf16*)Llo.p, (const __bf16*)Vhi.p, (const __bf16*)Vlo.p, Tp.p);
-  } else if (stage == 2 && nC) {
-    hipLaunchKernelGGL(k_stage_c_bf16x3, dim3((unsigned)nC), dim3(256), 0, s, tilesC.p, nC, bfC.p, bfT.p, Tp.p,
-                       (const __bf16*)Rhi.p, (const __bf16*)Rlo.p, d_vout);
+  } else if (stage == 2) {
+    const int nW = (int)wTile.n, nR = (int)rTile.n;
+    if (nCb && nW)
+      hipLaunchKernelGGL(k_stage_c_bf16x3_big, dim3((unsigned)nW), dim3(256), 0, s, bfCbig.p, wTile.p, wBegin.p,
+                         wEnd.p, wSlot.p, nW, bfC.p, bfT.p, Tp.p, (const __bf16*)Rhi.p, (const __bf16*)Rlo.p, d_vout,
+                         Cpart.p);
+    if (nR)
+      hipLaunchKernelGGL(k_reduce_partials, dim3((unsigned)nR), dim3(256), 0, s, bfCbig.p, rTile.p, rSlot.p, rNs.p,
+                         Cpart.p, d_vout);
+    if (nC)
+      hipLaunchKernelGGL(k_stage_c_bf16x3, dim3((unsigned)nC), dim3(256), 0, s, bfCsmall.p, nC, bfC.p, bfT.p, Tp.p,
+                         (const __bf16*)Rhi.p, (const __bf16*)Rlo.p, d_vout);
   }
   VMPS_HIP_CHECK(hipGetLastError());
 }

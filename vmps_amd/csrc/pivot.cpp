@@ -680,8 +680,14 @@ void build_bf16_layout(const HostPlan& H, Bf16Layout& B) {
   for (const BlockDim& b : H.Vblk) {
     if (b.rows * b.cols == 0) continue;
     Vmap[b.off] = o;
-    B.vconv.push_back({b.off, o, (int32_t)b.rows, (int32_t)b.cols, (int32_t)pad8(b.rows), 0});
-    o += pad8(b.rows) * b.cols;
+    // one conversion task per group of columns holding at most 4096 padded elements
+    const int64_t Kp = pad8(b.rows);
+    const int64_t cols_per = std::max<int64_t>(1, 4096 / Kp);
+    for (int64_t j0 = 0; j0 < b.cols; j0 += cols_per) {
+      const int64_t nc = std::min(cols_per, b.cols - j0);
+      B.vconv.push_back({b.off + j0 * b.rows, o + j0 * Kp, (int32_t)b.rows, (int32_t)nc, (int32_t)Kp, 0});
+    }
+    o += Kp * b.cols;
   }
   B.V_size = o;
   // T, row-major M x Np
@@ -691,6 +697,7 @@ void build_bf16_layout(const HostPlan& H, Bf16Layout& B) {
     o += (int64_t)H.probA[t].M * pad8(H.probA[t].N);
   }
   B.T_size = o;
+  auto is_big = [](int64_t M, int64_t N) { return M >= kBigMin && N >= kBigMin; };
   for (const TileA& t : H.tilesA) {
     TileA16 a;
     a.a_off = Lmap.at(t.a_off);
@@ -703,7 +710,47 @@ void build_bf16_layout(const HostPlan& H, Bf16Layout& B) {
     a.mt = t.mt;
     a.nt = t.nt;
     a.pad = 0;
-    B.tilesA.push_back(a);
+    if (!is_big(t.M, t.N)) {
+      B.tilesA.push_back(a);
+    } else if (t.mt % (kBigTile / kTile) == 0 && t.nt % (kBigTile / kTile) == 0) {
+      a.mt = (int16_t)(t.mt / (kBigTile / kTile));  // one 128 x 128 tile per 4 x 4 wave tiles
+      a.nt = (int16_t)(t.nt / (kBigTile / kTile));
+      B.tilesA_big.push_back(a);
+    }
+  }
+  for (const TileC& t : H.tilesC) {
+    if (!is_big(t.M, t.N)) {
+      B.tilesC_small.push_back(t);
+    } else if (t.mt % (kBigTile / kTile) == 0 && t.nt % (kBigTile / kTile) == 0) {
+      TileC b = t;
+      b.mt = (int16_t)(t.mt / (kBigTile / kTile));
+      b.nt = (int16_t)(t.nt / (kBigTile / kTile));
+      B.tilesC_big.push_back(b);
+    }
+  }
+  // Split the K iterations of a big stage C tile over several workgroups only
+  // when the big tiles alone cannot fill the GPU (the FP32 partials cost a
+  // round trip through HBM): aim at ~512 workgroups, at least 16 iterations of
+  // 32 k per split, at most 4 splits.
+  const int64_t nbig = (int64_t)B.tilesC_big.size();
+  const int64_t want = nbig > 0 ? std::max<int64_t>(1, std::min<int64_t>(4, 512 / nbig)) : 1;
+  for (size_t ti = 0; ti < B.tilesC_big.size(); ++ti) {
+    const TileC& t = B.tilesC_big[ti];
+    int64_t steps = 0;
+    for (int32_t c = t.chunk_begin; c < t.chunk_end; ++c) steps += (pad8(H.chunksC[c].K) + 31) / 32;
+    const int64_t S = std::max<int64_t>(1, std::min<int64_t>(want, steps / 16));
+    for (int64_t s = 0; s < S; ++s) {
+      B.splitC_tile.push_back((int32_t)ti);
+      B.splitC_begin.push_back((int32_t)(steps * s / S));
+      B.splitC_end.push_back((int32_t)(steps * (s + 1) / S));
+      B.splitC_slot.push_back(S == 1 ? -1 : B.partial_slots + s);
+    }
+    if (S > 1) {
+      B.redC_tile.push_back((int32_t)ti);
+      B.redC_slot.push_back(B.partial_slots);
+      B.redC_nsplit.push_back((int32_t)S);
+      B.partial_slots += S;
+    }
   }
   for (const ChunkC& c : H.chunksC) {
     ChunkC16 d;
@@ -716,10 +763,11 @@ void build_bf16_layout(const HostPlan& H, Bf16Layout& B) {
   }
   for (const TermC& t : H.termsC) B.termsC.push_back({Tmap.at(t.t_off), (float)t.c, 0});
   // bounds: every padded access stays inside its array
-  for (const TileA16& a : B.tilesA)
-    if (a.a_off + (int64_t)a.M * a.Kp > (int64_t)B.Lhi.size() || a.b_off + (int64_t)a.Kp * a.N > B.V_size ||
-        a.c_off + (int64_t)a.M * a.Np > B.T_size || (a.a_off | a.b_off | a.c_off) % 8)
-      throw VmpsError(VMPS_ERR_INTERNAL, "bf16 stage A descriptor out of range");
+  for (const auto* list : {&B.tilesA, &B.tilesA_big})
+    for (const TileA16& a : *list)
+      if (a.a_off + (int64_t)a.M * a.Kp > (int64_t)B.Lhi.size() || a.b_off + (int64_t)a.Kp * a.N > B.V_size ||
+          a.c_off + (int64_t)a.M * a.Np > B.T_size || (a.a_off | a.b_off | a.c_off) % 8)
+        throw VmpsError(VMPS_ERR_INTERNAL, "bf16 stage A descriptor out of range");
   for (const TileC& t : H.tilesC)
     for (int32_t c = t.chunk_begin; c < t.chunk_end; ++c) {
       const ChunkC16& ch = B.chunksC[c];

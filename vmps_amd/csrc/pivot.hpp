@@ -208,9 +208,24 @@ struct VConv {             // V block: packed FP64 (K x N, ld K) -> padded split
   int64_t src_off, dst_off;
   int32_t K, N, Kp, pad;
 };
+constexpr int kBigTile = 128;  // workgroup tile edge of the LDS-staged kernels
+// A block is handled by the LDS-staged big-tile kernels when both its
+// dimensions are at least this (below, the wave-tile kernels waste less).
+constexpr int kBigMin = 96;
 struct Bf16Layout {
   std::vector<uint16_t> Lhi, Llo, Rhi, Rlo;  // split operands, built once
   int64_t V_size = 0, T_size = 0;            // padded sizes of the per-call arrays
+  std::vector<TileA16> tilesA_big;           // 128 x 128 tiles of the large stage A problems
+  std::vector<TileC> tilesC_big, tilesC_small;  // stage C tiles: 128-unit and 32-unit
+  // split-K work items of the big stage C tiles: tile index, iteration range
+  // [it_begin, it_end) over the flattened (chunk, 32-deep k block) loop, and
+  // the partial-sum slot (-1 = write the output directly)
+  std::vector<int32_t> splitC_tile, splitC_begin, splitC_end;
+  std::vector<int64_t> splitC_slot;
+  std::vector<int32_t> redC_tile;     // big tiles reduced from partials
+  std::vector<int64_t> redC_slot;     // first partial slot of each reduced tile
+  std::vector<int32_t> redC_nsplit;
+  int64_t partial_slots = 0;          // number of 128 x 128 FP32 partial tiles
   std::vector<TileA16> tilesA;
   std::vector<ChunkC16> chunksC;
   std::vector<TermC16> termsC;

@@ -219,6 +219,9 @@ WORKLOADS = {
     "heisenberg_u1_L32_M64": dict(model="HeisenbergU1", L=32, M=64),
     # HubbardU1xU1 chain L=48, half filling, M=512, D_W=6
     "hubbard_u1xu1_L48_M512": dict(model="HubbardU1xU1", L=48, M=512),
+    # large-block pivots (MFMA-bound regime)
+    "heisenberg_u1_L64_M2048": dict(model="HeisenbergU1", L=64, M=2048),
+    "hubbard_u1xu1_L48_M2048": dict(model="HubbardU1xU1", L=48, M=2048, alpha=1.0, lam=2.2),
 }
 
 
@@ -234,8 +237,9 @@ def make_workload(name, seed=20261202, hermitian=False, **override):
         right = heisenberg_bond(loc + 2, cfg["M"])
     elif cfg["model"] == "HubbardU1xU1":
         W = models.hubbard_u1xu1_mpo(t=cfg.get("t", 1.0), U=cfg.get("U", 8.0))
-        left = hubbard_bond(loc, cfg["M"])
-        right = hubbard_bond(loc + 2, cfg["M"])
+        kw = {k: cfg[k] for k in ("alpha", "lam") if k in cfg}
+        left = hubbard_bond(loc, cfg["M"], **kw)
+        right = hubbard_bond(loc + 2, cfg["M"], **kw)
     else:
         raise ValueError(cfg["model"])
     return make_pivot(W, W, left, right, rng, hermitian=hermitian)
