@@ -159,6 +159,37 @@ vmps_status vmps_dot(vmps_plan* plan, const double* x, const double* y, double* 
  * with HIP events around `iters` back-to-back calls on resident vectors. */
 vmps_status vmps_plan_time_hxv(vmps_plan* plan, int warmup, int iters, double* ms_per_hxv);
 
+/* ---- environment updates ---------------------------------------------------
+ * contract_L(Lold, Abra, W, Aket, qloc, qOp, Lnew)  (tensors/DmrgContractions.h:34-158)
+ * contract_R(Rold, Abra, W, Aket, qloc, qOp, Rnew)  (tensors/DmrgContractions.h:179-311)
+ * (MODE = FULL, RANDOMIZE = false).  The new environment is returned with its
+ * sectors in creation order and, within a sector, blocks a = 0 .. adim-1. */
+typedef struct vmps_env vmps_env;
+
+/* which = 0: contract_L, which = 1: contract_R */
+vmps_status vmps_env_create(int which, int nq, const int* kinds, int nloc, const int* qloc, int nop, const int* qop,
+                            vmps_env** out);
+void vmps_env_destroy(vmps_env* e);
+/* one sector {in, out, mid} of Lold / Rold, packed as in vmps_pivot2_add_env */
+vmps_status vmps_env_add_old(vmps_env* e, const int* qin, const int* qout, const int* qmid, int adim,
+                             const int64_t* rows, const int64_t* cols, const double* data);
+/* block (qin, qout) of A[s] (bra = 1: Abra, bra = 0: Aket), column-major rows x cols */
+vmps_status vmps_env_add_A(vmps_env* e, int bra, int s, const int* qin, const int* qout, int64_t rows, int64_t cols,
+                           const double* data);
+/* sparse block of W[s1][s2][k] */
+vmps_status vmps_env_add_W(vmps_env* e, int s1, int s2, int k, const int* qin, const int* qout, int rows, int cols,
+                           int64_t nnz, const int* row, const int* col, const double* val);
+/* plan and run the contraction on the current HIP device (synchronous) */
+vmps_status vmps_env_contract(vmps_env* e, vmps_precision prec);
+/* result: nsec sectors, nslots = sum of adim, size = total elements */
+vmps_status vmps_env_result_size(const vmps_env* e, int64_t* nsec, int64_t* nslots, int64_t* size);
+/* qin/qout/qmid: nsec*nq ints each, adim: nsec, rows/cols: nslots (0 = absent
+ * block), data: the blocks packed in slot order */
+vmps_status vmps_env_result_get(const vmps_env* e, int* qin, int* qout, int* qmid, int* adim, int64_t* rows,
+                                int64_t* cols, double* data);
+/* work counts of the last contraction's plan */
+vmps_status vmps_env_stats(const vmps_env* e, vmps_plan_stats* st);
+
 /* ---- PivotVector algebra on device vectors (DmrgPivotVector.h:121-424) ----
  * All vectors are packed device arrays of the plan's layout; work is enqueued
  * on the plan's stream and scalar results are returned after a synchronise. */

@@ -90,6 +90,15 @@ SIGNATURES = {
     "vmps_dot": (c_int, [c_void, P_dbl, P_dbl, P_dbl]),
     "vmps_plan_time_hxv": (c_int, [c_void, c_int, c_int, P_dbl]),
     "vmps_plan_time_stages": (c_int, [c_void, c_int, c_int, P_dbl]),
+    "vmps_env_create": (c_int, [c_int, c_int, P_int, c_int, P_int, c_int, P_int, ctypes.POINTER(c_void)]),
+    "vmps_env_destroy": (None, [c_void]),
+    "vmps_env_add_old": (c_int, [c_void, P_int, P_int, P_int, c_int, P_i64, P_i64, P_dbl]),
+    "vmps_env_add_A": (c_int, [c_void, c_int, c_int, P_int, P_int, c_i64, c_i64, P_dbl]),
+    "vmps_env_add_W": (c_int, [c_void, c_int, c_int, c_int, P_int, P_int, c_int, c_int, c_i64, P_int, P_int, P_dbl]),
+    "vmps_env_contract": (c_int, [c_void, c_int]),
+    "vmps_env_result_size": (c_int, [c_void, P_i64, P_i64, P_i64]),
+    "vmps_env_result_get": (c_int, [c_void, P_int, P_int, P_int, P_int, P_i64, P_i64, P_dbl]),
+    "vmps_env_stats": (c_int, [c_void, ctypes.POINTER(PlanStats)]),
     "vmps_vec_alloc": (c_int, [c_void, ctypes.POINTER(c_void)]),
     "vmps_vec_free": (c_int, [c_void, c_void]),
     "vmps_vec_upload": (c_int, [c_void, c_void, P_dbl]),

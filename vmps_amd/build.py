@@ -15,9 +15,10 @@ OUT = os.path.join(HERE, "libvmps_hxv.so")
 BUILD = os.path.join(HERE, "_build")
 ARCH = os.environ.get("VMPS_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["sym.cpp", "pivot.cpp", "capi.cpp", "engine.hip", "engine_bf16.hip", "lanczos.hip"]
+SOURCES = ["sym.cpp", "pivot.cpp", "env.cpp", "capi.cpp", "capi_env.cpp", "engine.hip", "engine_bf16.hip",
+           "lanczos.hip"]
 # C++ units that call the HIP runtime API are compiled in HIP mode
-HIP_HOST_UNITS = {"capi.cpp"}
+HIP_HOST_UNITS = {"capi.cpp", "capi_env.cpp"}
 
 COMMON = [
     "-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function",

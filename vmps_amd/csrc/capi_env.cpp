@@ -1,0 +1,242 @@
+// extern "C" entry points of the environment updates (include/vmps_hxv.h).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <memory>
+
+#include "engine.hpp"
+#include "env.hpp"
+#include "errors.hpp"
+#include "vmps_hxv.h"
+
+struct vmps_env {
+  vmps::EnvProblem P;
+  vmps::TripleSet S;
+  vmps::Stats stats;
+  std::vector<double> result;  // output vector in S.Ob order
+  bool done = false;
+};
+
+using vmps::VmpsError;
+
+#define VMPS_API_BEGIN try {
+#define VMPS_API_END                                \
+  return VMPS_OK;                                   \
+  }                                                 \
+  catch (const VmpsError& e) {                      \
+    vmps::set_last_error(e.what());                 \
+    return e.status;                                \
+  }                                                 \
+  catch (const std::bad_alloc&) {                   \
+    vmps::set_last_error("host allocation failed"); \
+    return VMPS_ERR_OOM;                            \
+  }                                                 \
+  catch (const std::exception& e) {                 \
+    vmps::set_last_error(e.what());                 \
+    return VMPS_ERR_INTERNAL;                       \
+  }
+
+namespace {
+vmps::Qn qn(const int* q, int nq) {
+  vmps::Qn r;
+  for (int i = 0; i < nq; ++i) r[i] = q[i];
+  return r;
+}
+void require(bool cond, int status, const char* msg) {
+  if (!cond) throw VmpsError(status, msg);
+}
+}  // namespace
+
+extern "C" {
+
+vmps_status vmps_env_create(int which, int nq, const int* kinds, int nloc, const int* qloc, int nop, const int* qop,
+                            vmps_env** out) {
+  VMPS_API_BEGIN
+  require(out != nullptr, VMPS_ERR_ARG, "out is NULL");
+  *out = nullptr;
+  require(which == 0 || which == 1, VMPS_ERR_ARG, "which must be 0 (contract_L) or 1 (contract_R)");
+  require(nq >= 1 && nq <= vmps::kMaxNq && kinds, VMPS_ERR_ARG, "bad symmetry");
+  require(nloc > 0 && nop > 0 && qloc && qop, VMPS_ERR_ARG, "empty local or operator basis");
+  auto e = std::make_unique<vmps_env>();
+  e->P.which = which;
+  e->P.sym.nq = nq;
+  for (int i = 0; i < nq; ++i) {
+    require(kinds[i] == VMPS_SYM_U1 || kinds[i] == VMPS_SYM_SU2, VMPS_ERR_ARG, "unknown symmetry kind");
+    e->P.sym.kind[i] = kinds[i];
+  }
+  for (int i = 0; i < nloc; ++i) e->P.qloc.push_back(qn(qloc + i * nq, nq));
+  for (int i = 0; i < nop; ++i) e->P.qop.push_back(qn(qop + i * nq, nq));
+  e->P.Abra.resize(nloc);
+  e->P.Aket.resize(nloc);
+  e->P.W.assign(nloc, std::vector<std::vector<vmps::WBiped>>(nloc, std::vector<vmps::WBiped>(nop)));
+  *out = e.release();
+  VMPS_API_END
+}
+
+void vmps_env_destroy(vmps_env* e) { delete e; }
+
+vmps_status vmps_env_add_old(vmps_env* e, const int* qin, const int* qout, const int* qmid, int adim,
+                             const int64_t* rows, const int64_t* cols, const double* data) {
+  VMPS_API_BEGIN
+  require(e && qin && qout && qmid && adim >= 0 && (adim == 0 || (rows && cols)), VMPS_ERR_ARG, "bad argument");
+  const int nq = e->P.sym.nq;
+  vmps::Tripod& E = e->P.old;
+  vmps::Key3 key{qn(qin, nq), qn(qout, nq), qn(qmid, nq)};
+  require(!E.dict.count(key), VMPS_ERR_LAYOUT, "duplicate environment sector");
+  std::vector<vmps::DenseBlock> blocks(adim);
+  int64_t src = 0;
+  for (int a = 0; a < adim; ++a) {
+    require(rows[a] >= 0 && cols[a] >= 0, VMPS_ERR_ARG, "negative block dimension");
+    blocks[a].rows = rows[a];
+    blocks[a].cols = cols[a];
+    if (rows[a] * cols[a] > 0) {
+      require(data != nullptr, VMPS_ERR_ARG, "NULL data");
+      blocks[a].off = (int64_t)E.data.size();
+      E.data.insert(E.data.end(), data + src, data + src + rows[a] * cols[a]);
+      src += rows[a] * cols[a];
+    }
+  }
+  E.index.push_back({key.a, key.b, key.c});
+  E.block.push_back(std::move(blocks));
+  E.dict[key] = E.index.size() - 1;
+  e->done = false;
+  VMPS_API_END
+}
+
+vmps_status vmps_env_add_A(vmps_env* e, int bra, int s, const int* qin, const int* qout, int64_t rows, int64_t cols,
+                           const double* data) {
+  VMPS_API_BEGIN
+  require(e && qin && qout && rows >= 0 && cols >= 0, VMPS_ERR_ARG, "bad argument");
+  require(s >= 0 && s < (int)e->P.qloc.size(), VMPS_ERR_ARG, "s out of range");
+  require(rows * cols == 0 || data, VMPS_ERR_ARG, "NULL data");
+  const int nq = e->P.sym.nq;
+  vmps::DBiped& A = bra ? e->P.Abra[s] : e->P.Aket[s];
+  vmps::Key2 key{qn(qin, nq), qn(qout, nq)};
+  require(!A.dict.count(key), VMPS_ERR_LAYOUT, "duplicate A block");
+  vmps::DenseBlock b;
+  b.rows = rows;
+  b.cols = cols;
+  b.off = (int64_t)A.data.size();
+  A.data.insert(A.data.end(), data, data + rows * cols);
+  A.in.push_back(key.a);
+  A.out.push_back(key.b);
+  A.block.push_back(b);
+  A.dict[key] = A.in.size() - 1;
+  e->done = false;
+  VMPS_API_END
+}
+
+vmps_status vmps_env_add_W(vmps_env* e, int s1, int s2, int k, const int* qin, const int* qout, int rows, int cols,
+                           int64_t nnz, const int* row, const int* col, const double* val) {
+  VMPS_API_BEGIN
+  require(e && qin && qout && nnz >= 0 && (nnz == 0 || (row && col && val)), VMPS_ERR_ARG, "bad argument");
+  const int nloc = (int)e->P.qloc.size(), nop = (int)e->P.qop.size();
+  require(s1 >= 0 && s1 < nloc && s2 >= 0 && s2 < nloc && k >= 0 && k < nop, VMPS_ERR_ARG, "W index out of range");
+  const int nq = e->P.sym.nq;
+  vmps::WBiped& B = e->P.W[s1][s2][k];
+  vmps::Key2 key{qn(qin, nq), qn(qout, nq)};
+  require(!B.dict.count(key), VMPS_ERR_LAYOUT, "duplicate W block");
+  std::vector<int64_t> order(nnz);
+  for (int64_t i = 0; i < nnz; ++i) {
+    require(row[i] >= 0 && row[i] < rows && col[i] >= 0 && col[i] < cols, VMPS_ERR_ARG, "W nonzero out of range");
+    order[i] = i;
+  }
+  std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) {
+    return col[a] != col[b] ? col[a] < col[b] : row[a] < row[b];
+  });
+  vmps::SparseBlock sb;
+  sb.rows = rows;
+  sb.cols = cols;
+  for (int64_t i = 0; i < nnz; ++i) {
+    sb.row.push_back(row[order[i]]);
+    sb.col.push_back(col[order[i]]);
+    sb.val.push_back(val[order[i]]);
+  }
+  B.in.push_back(key.a);
+  B.out.push_back(key.b);
+  B.block.push_back(std::move(sb));
+  B.dict[key] = B.in.size() - 1;
+  e->done = false;
+  VMPS_API_END
+}
+
+vmps_status vmps_env_contract(vmps_env* e, vmps_precision prec) {
+  VMPS_API_BEGIN
+  require(e != nullptr, VMPS_ERR_ARG, "NULL env");
+  require(prec == VMPS_PREC_FP64 || prec == VMPS_PREC_FP32 || prec == VMPS_PREC_BF16X3, VMPS_ERR_ARG,
+          "unknown precision");
+  vmps::plan_env(e->P, e->S);
+  e->result.assign(e->S.osize, 0.0);
+  if (!e->S.terms.empty() && e->S.osize > 0) {
+    vmps::DevPlan D;
+    D.create_triples(e->S, prec);
+    e->stats = D.H.stats;
+    const size_t ib = (size_t)D.H.ysize * sizeof(double), ob = (size_t)D.H.osize * sizeof(double);
+    if (ib) VMPS_HIP_CHECK(hipMemcpyAsync(D.vin.p, e->P.old.data.data(), ib, hipMemcpyHostToDevice, D.stream));
+    D.hxv(D.vin.p, D.vout.p, D.stream);
+    VMPS_HIP_CHECK(hipMemcpyAsync(e->result.data(), D.vout.p, ob, hipMemcpyDeviceToHost, D.stream));
+    VMPS_HIP_CHECK(hipStreamSynchronize(D.stream));
+  }
+  e->done = true;
+  VMPS_API_END
+}
+
+vmps_status vmps_env_result_size(const vmps_env* e, int64_t* nsec, int64_t* nslots, int64_t* size) {
+  VMPS_API_BEGIN
+  require(e && nsec && nslots && size, VMPS_ERR_ARG, "NULL argument");
+  require(e->done, VMPS_ERR_STATE, "vmps_env_contract has not run");
+  *nsec = (int64_t)e->P.new_index.size();
+  int64_t ns = 0;
+  for (int a : e->P.new_adim) ns += a;
+  *nslots = ns;
+  *size = e->S.osize;
+  VMPS_API_END
+}
+
+vmps_status vmps_env_result_get(const vmps_env* e, int* qin, int* qout, int* qmid, int* adim, int64_t* rows,
+                                int64_t* cols, double* data) {
+  VMPS_API_BEGIN
+  require(e && qin && qout && qmid && adim && rows && cols, VMPS_ERR_ARG, "NULL argument");
+  require(e->done, VMPS_ERR_STATE, "vmps_env_contract has not run");
+  const int nq = e->P.sym.nq;
+  int64_t slot = 0, pos = 0;
+  for (size_t q = 0; q < e->P.new_index.size(); ++q) {
+    for (int i = 0; i < nq; ++i) {
+      qin[q * nq + i] = e->P.new_index[q][0][i];
+      qout[q * nq + i] = e->P.new_index[q][1][i];
+      qmid[q * nq + i] = e->P.new_index[q][2][i];
+    }
+    adim[q] = e->P.new_adim[q];
+    for (int a = 0; a < e->P.new_adim[q]; ++a, ++slot) {
+      const int64_t id = e->P.new_block[q][a];
+      if (id < 0) {
+        rows[slot] = cols[slot] = 0;
+        continue;
+      }
+      const vmps::BlockDim& b = e->S.Ob[id];
+      rows[slot] = b.rows;
+      cols[slot] = b.cols;
+      if (data) std::memcpy(data + pos, e->result.data() + b.off, sizeof(double) * b.rows * b.cols);
+      pos += b.rows * b.cols;
+    }
+  }
+  VMPS_API_END
+}
+
+vmps_status vmps_env_stats(const vmps_env* e, vmps_plan_stats* st) {
+  VMPS_API_BEGIN
+  require(e && st, VMPS_ERR_ARG, "NULL argument");
+  std::memset(st, 0, sizeof(*st));
+  st->n_terms = e->stats.n_terms;
+  st->n_T = e->stats.n_T;
+  st->n_U = e->stats.n_U;
+  st->n_out = e->stats.n_out;
+  st->flops_alg = e->stats.flops_alg;
+  st->flops_ref = e->stats.flops_ref;
+  st->flops_A = e->stats.flops_A;
+  st->flops_C = e->stats.flops_C;
+  VMPS_API_END
+}
+
+}  // extern "C"

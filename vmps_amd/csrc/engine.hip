@@ -269,6 +269,23 @@ DevPlan::~DevPlan() {
 
 void DevPlan::create(const Pivot2& P, vmps_precision precision) {
   build_host_plan(P, H);
+  upload(precision);
+  nproj = (int)P.A0proj.size();
+  Epenalty = P.Epenalty;
+  if (nproj) {
+    std::vector<double> a0;
+    for (auto& v : P.A0proj) a0.insert(a0.end(), v.begin(), v.end());
+    A0.upload(a0, stream);
+    VMPS_HIP_CHECK(hipStreamSynchronize(stream));
+  }
+}
+
+void DevPlan::create_triples(const TripleSet& S, vmps_precision precision) {
+  build_plan(S, H);
+  upload(precision);
+}
+
+void DevPlan::upload(vmps_precision precision) {
   VMPS_HIP_CHECK(hipGetDevice(&device));
   VMPS_HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
   VMPS_HIP_CHECK(hipEventCreate(&ev0));
@@ -276,11 +293,11 @@ void DevPlan::create(const Pivot2& P, vmps_precision precision) {
   L.upload(H.Ldata, stream);
   R.upload(H.Rdata, stream);
   T.alloc(H.T_size);
-  vin.alloc(H.vsize);
-  vout.alloc(H.vsize);
-  if (H.vsize) {
-    VMPS_HIP_CHECK(hipMemsetAsync(vin.p, 0, H.vsize * sizeof(double), stream));
-    VMPS_HIP_CHECK(hipMemsetAsync(vout.p, 0, H.vsize * sizeof(double), stream));
+  vin.alloc(H.ysize);
+  vout.alloc(H.osize);
+  if (H.ysize || H.osize) {
+    VMPS_HIP_CHECK(hipMemsetAsync(vin.p, 0, std::max<int64_t>(1, H.ysize) * sizeof(double), stream));
+    VMPS_HIP_CHECK(hipMemsetAsync(vout.p, 0, std::max<int64_t>(1, H.osize) * sizeof(double), stream));
   }
   tilesA.upload(H.tilesA, stream);
   tilesC.upload(H.tilesC, stream);
@@ -292,13 +309,6 @@ void DevPlan::create(const Pivot2& P, vmps_precision precision) {
   const size_t nseg = H.seg_off.size();
   partial.alloc(std::max<size_t>(1, (nseg + kDotSegPerBlock - 1) / kDotSegPerBlock));
   scalars.alloc(64);
-  nproj = (int)P.A0proj.size();
-  Epenalty = P.Epenalty;
-  if (nproj) {
-    std::vector<double> a0;
-    for (auto& v : P.A0proj) a0.insert(a0.end(), v.begin(), v.end());
-    A0.upload(a0, stream);
-  }
   prec = precision;
   ensure_precision_buffers();
   VMPS_HIP_CHECK(hipStreamSynchronize(stream));
@@ -308,7 +318,7 @@ void DevPlan::ensure_precision_buffers() {
   if (prec == VMPS_PREC_FP32 && !T32.p && (H.T_size || H.vsize)) {
     L32.alloc(H.Ldata.size());
     R32.alloc(H.Rdata.size());
-    V32.alloc(H.vsize);
+    V32.alloc(H.ysize);
     T32.alloc(H.T_size);
     if (L.n) hipLaunchKernelGGL(k_f64_to_f32, dim3(1024), dim3(256), 0, stream, L.p, L32.p, (int64_t)L.n);
     if (R.n) hipLaunchKernelGGL(k_f64_to_f32, dim3(1024), dim3(256), 0, stream, R.p, R32.p, (int64_t)R.n);
@@ -329,7 +339,7 @@ void DevPlan::launch_stage(int stage, const double* d_vin, double* d_vout, hipSt
     }
   } else if (prec == VMPS_PREC_FP32) {
     if (stage == 0) {
-      if (H.vsize) hipLaunchKernelGGL(k_f64_to_f32, dim3(1024), dim3(256), 0, s, d_vin, V32.p, (int64_t)H.vsize);
+      if (H.ysize) hipLaunchKernelGGL(k_f64_to_f32, dim3(1024), dim3(256), 0, s, d_vin, V32.p, (int64_t)H.ysize);
       if (nA)
         hipLaunchKernelGGL(k_stage_a<float>, dim3(grid_for(nA)), dim3(256), 0, s, tilesA.p, nA, L32.p, V32.p, T32.p);
     }

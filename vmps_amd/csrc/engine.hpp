@@ -98,6 +98,8 @@ struct DevPlan {
 
   ~DevPlan();
   void create(const Pivot2& P, vmps_precision prec);
+  void create_triples(const TripleSet& S, vmps_precision prec);
+  void upload(vmps_precision prec);  // device copies of H
   void ensure_precision_buffers();
   void ensure_bf16_buffers();
   void launch_stage_bf16x3(int stage, const double* d_vin, double* d_vout, hipStream_t s);

@@ -338,8 +338,8 @@ void check_gemm(const std::vector<GemmProb>& probs, const std::vector<GemmChunk>
 }  // namespace
 
 void validate_host_plan(const HostPlan& H) {
-  check_gemm(H.probA, H.chunkA, (int64_t)H.Ldata.size(), H.vsize, H.T_size, "stage A");
-  check_gemm(H.probC, H.chunkC, H.U_size, (int64_t)H.Rdata.size(), H.vsize, "stage C");
+  check_gemm(H.probA, H.chunkA, (int64_t)H.Ldata.size(), H.ysize, H.T_size, "stage A");
+  check_gemm(H.probC, H.chunkC, H.U_size, (int64_t)H.Rdata.size(), H.osize, "stage C");
   for (const CombineList& cl : H.comb) {
     if (cl.u_off < 0 || cl.size <= 0 || cl.u_off + cl.size > H.U_size || cl.begin < 0 || cl.end < cl.begin ||
         cl.end > (int32_t)H.comb_t.size() || cl.size > INT32_MAX)
@@ -351,7 +351,7 @@ void validate_host_plan(const HostPlan& H) {
     }
   }
   for (size_t i = 0; i < H.seg_off.size(); ++i)
-    if (H.seg_off[i] < 0 || H.seg_off[i] + H.seg_len[i] > H.vsize)
+    if (H.seg_off[i] < 0 || H.seg_off[i] + H.seg_len[i] > H.osize)
       throw VmpsError(VMPS_ERR_INTERNAL, "dot segment out of range");
 }
 
@@ -443,41 +443,58 @@ void build_tiles(HostPlan& H) {
           throw VmpsError(VMPS_ERR_INTERNAL, "stage C: T block out of range");
 }
 
-void build_host_plan(const Pivot2& P, HostPlan& H) {
-  std::vector<Term> terms;
+void triples_from_pivot(const Pivot2& P, TripleSet& S) {
   std::vector<std::pair<size_t, size_t>> lblocks, rblocks;
   std::vector<std::pair<int, int>> vblocks;
-  expand_terms(P, terms, lblocks, rblocks, vblocks);
+  S = TripleSet();
+  expand_terms(P, S.terms, lblocks, rblocks, vblocks);
+  for (const auto& lb : lblocks) {
+    const DenseBlock& b = P.L.block[lb.first][lb.second];
+    S.Xb.push_back({(int64_t)S.Xdata.size(), b.rows, b.cols});
+    S.Xdata.insert(S.Xdata.end(), P.L.data.begin() + b.off, P.L.data.begin() + b.off + b.size());
+  }
+  for (const auto& rb : rblocks) {
+    const DenseBlock& b = P.R.block[rb.first][rb.second];
+    S.Zb.push_back({(int64_t)S.Zdata.size(), b.rows, b.cols});
+    S.Zdata.insert(S.Zdata.end(), P.R.data.begin() + b.off, P.R.data.begin() + b.off + b.size());
+  }
+  for (const auto& vb : vblocks) {
+    const DenseBlock& b = P.V.block[vb.first][vb.second];
+    S.Yb.push_back({b.off, b.rows, b.cols});
+    S.Ow.push_back(P.sym.coeff_dot(P.V.out[vb.first][vb.second]));
+  }
+  S.Ob = S.Yb;
+  S.ysize = S.osize = P.V.size;
+}
+
+void build_host_plan(const Pivot2& P, HostPlan& H) {
+  TripleSet S;
+  triples_from_pivot(P, S);
+  build_plan(S, H);
+}
+
+void build_plan(const TripleSet& S, HostPlan& H) {
   H = HostPlan();
   Stats& st = H.stats;
-  H.vsize = P.V.size;
+  H.ysize = S.ysize;
+  H.osize = S.osize;
+  H.vsize = S.osize;
+  H.Ldata = S.Xdata;
+  H.Rdata = S.Zdata;
+  H.Lblk = S.Xb;
+  H.Rblk = S.Zb;
+  H.Vblk = S.Yb;
+  for (const BlockDim& b : S.Xb) st.max_block_dim = std::max({st.max_block_dim, b.rows, b.cols});
+  for (const BlockDim& b : S.Zb) st.max_block_dim = std::max({st.max_block_dim, b.rows, b.cols});
+  const std::vector<Term>& terms = S.terms;
+  const int64_t nY = (int64_t)S.Yb.size(), nO = (int64_t)S.Ob.size(), nZ = (int64_t)S.Zb.size();
 
-  auto vblk = [&](int64_t id) -> const DenseBlock& {
-    return P.V.block[vblocks[id].first][vblocks[id].second];
-  };
-
-  // pack L / R blocks that terms reference
-  std::vector<int64_t> loff(lblocks.size()), roff(rblocks.size());
-  for (size_t i = 0; i < lblocks.size(); ++i) {
-    const DenseBlock& b = P.L.block[lblocks[i].first][lblocks[i].second];
-    loff[i] = (int64_t)H.Ldata.size();
-    H.Lblk.push_back({loff[i], b.rows, b.cols});
-    H.Ldata.insert(H.Ldata.end(), P.L.data.begin() + b.off, P.L.data.begin() + b.off + b.size());
-    st.max_block_dim = std::max({st.max_block_dim, b.rows, b.cols});
-  }
-  for (size_t i = 0; i < rblocks.size(); ++i) {
-    const DenseBlock& b = P.R.block[rblocks[i].first][rblocks[i].second];
-    roff[i] = (int64_t)H.Rdata.size();
-    H.Rblk.push_back({roff[i], b.rows, b.cols});
-    H.Rdata.insert(H.Rdata.end(), P.R.data.begin() + b.off, P.R.data.begin() + b.off + b.size());
-    st.max_block_dim = std::max({st.max_block_dim, b.rows, b.cols});
-  }
-  auto Lb = [&](int64_t id) -> const DenseBlock& { return P.L.block[lblocks[id].first][lblocks[id].second]; };
-  auto Rb = [&](int64_t id) -> const DenseBlock& { return P.R.block[rblocks[id].first][rblocks[id].second]; };
-
-  // shapes: L (m' x m), Vin (m x n), R (n x n'), Vout (m' x n')
+  // shapes: X (m' x m), Y (m x n), Z (n x n'), O (m' x n')
   for (const Term& t : terms) {
-    const DenseBlock &L = Lb(t.l_blk), &R = Rb(t.r_blk), &vi = vblk(t.in_v), &vo = vblk(t.out_v);
+    if (t.l_blk < 0 || t.l_blk >= (int64_t)S.Xb.size() || t.r_blk < 0 || t.r_blk >= nZ || t.in_v < 0 ||
+        t.in_v >= nY || t.out_v < 0 || t.out_v >= nO)
+      throw VmpsError(VMPS_ERR_LAYOUT, "term references a block that does not exist");
+    const BlockDim &L = S.Xb[t.l_blk], &R = S.Zb[t.r_blk], &vi = S.Yb[t.in_v], &vo = S.Ob[t.out_v];
     if (L.cols != vi.rows || vi.cols != R.rows || L.rows != vo.rows || R.cols != vo.cols)
       throw VmpsError(VMPS_ERR_LAYOUT, "block shapes of a contributing term do not chain: L " + std::to_string(L.rows) +
                                            "x" + std::to_string(L.cols) + ", Vin " + std::to_string(vi.rows) + "x" +
@@ -487,16 +504,15 @@ void build_host_plan(const Pivot2& P, HostPlan& H) {
     st.flops_ref += 2.0 * L.rows * L.cols * vi.cols + 2.0 * L.rows * vi.cols * R.cols;
   }
 
-  // stage A: distinct (l, v)
-  std::unordered_map<int64_t, int32_t> tmap;  // key = l * nV + v
-  const int64_t nV = (int64_t)vblocks.size();
+  // stage A: distinct (x, y)
+  std::unordered_map<int64_t, int32_t> tmap;  // key = x * nY + y
   std::vector<int32_t> term_t(terms.size());
   for (size_t i = 0; i < terms.size(); ++i) {
     const Term& t = terms[i];
-    int64_t key = t.l_blk * nV + t.in_v;
+    int64_t key = t.l_blk * nY + t.in_v;
     auto it = tmap.find(key);
     if (it == tmap.end()) {
-      const DenseBlock &L = Lb(t.l_blk), &vi = vblk(t.in_v);
+      const BlockDim &L = S.Xb[t.l_blk], &vi = S.Yb[t.in_v];
       int32_t tid = (int32_t)H.probA.size();
       it = tmap.emplace(key, tid).first;
       GemmProb pr;
@@ -506,7 +522,7 @@ void build_host_plan(const Pivot2& P, HostPlan& H) {
       pr.chunk_begin = (int32_t)H.chunkA.size();
       pr.chunk_end = pr.chunk_begin + 1;
       GemmChunk ch;
-      ch.a_off = loff[t.l_blk];
+      ch.a_off = L.off;
       ch.b_off = vi.off;
       ch.K = (int32_t)L.cols;
       ch.lda = (int32_t)L.rows;
@@ -522,15 +538,14 @@ void build_host_plan(const Pivot2& P, HostPlan& H) {
     term_t[i] = it->second;
   }
 
-  // stage B: distinct (o, r), entries in term order
-  std::unordered_map<int64_t, int32_t> umap;  // key = o * nR + r
-  const int64_t nR = (int64_t)rblocks.size();
+  // stage B: distinct (o, z), entries in term order
+  std::unordered_map<int64_t, int32_t> umap;  // key = o * nZ + z
   std::vector<std::vector<std::pair<int32_t, double>>> ulist;
-  std::vector<std::pair<int64_t, int64_t>> ukey;  // (o, r)
-  std::vector<std::vector<int32_t>> out_u(nV);
+  std::vector<std::pair<int64_t, int64_t>> ukey;  // (o, z)
+  std::vector<std::vector<int32_t>> out_u(nO);
   for (size_t i = 0; i < terms.size(); ++i) {
     const Term& t = terms[i];
-    int64_t key = t.out_v * nR + t.r_blk;
+    int64_t key = t.out_v * nZ + t.r_blk;
     auto it = umap.find(key);
     if (it == umap.end()) {
       int32_t uid = (int32_t)ulist.size();
@@ -543,8 +558,8 @@ void build_host_plan(const Pivot2& P, HostPlan& H) {
   }
   std::vector<int64_t> uoff(ulist.size());
   for (size_t u = 0; u < ulist.size(); ++u) {
-    const DenseBlock& vo = vblk(ukey[u].first);
-    const DenseBlock& R = Rb(ukey[u].second);
+    const BlockDim& vo = S.Ob[ukey[u].first];
+    const BlockDim& R = S.Zb[ukey[u].second];
     CombineList cl;
     cl.u_off = H.U_size;
     cl.size = vo.rows * R.rows;
@@ -562,19 +577,19 @@ void build_host_plan(const Pivot2& P, HostPlan& H) {
   }
 
   // stage C: one problem per output block (all blocks, so untouched ones are zeroed)
-  for (int64_t o = 0; o < nV; ++o) {
-    const DenseBlock& vo = vblk(o);
-    if (vo.size() == 0) continue;
+  for (int64_t o = 0; o < nO; ++o) {
+    const BlockDim& vo = S.Ob[o];
+    if (vo.rows * vo.cols == 0) continue;
     GemmProb pr;
     pr.c_off = vo.off;
     pr.M = (int32_t)vo.rows;
     pr.N = (int32_t)vo.cols;
     pr.chunk_begin = (int32_t)H.chunkC.size();
     for (int32_t u : out_u[o]) {
-      const DenseBlock& R = Rb(ukey[u].second);
+      const BlockDim& R = S.Zb[ukey[u].second];
       GemmChunk ch;
       ch.a_off = uoff[u];
-      ch.b_off = roff[ukey[u].second];
+      ch.b_off = R.off;
       ch.K = (int32_t)R.rows;
       ch.lda = (int32_t)vo.rows;
       ch.ldb = (int32_t)R.rows;
@@ -585,20 +600,19 @@ void build_host_plan(const Pivot2& P, HostPlan& H) {
     H.probC.push_back(pr);
   }
 
-  // coeff_dot weights per block (DmrgPivotVector.h:323), split in segments
-  for (size_t s = 0; s < P.V.in.size(); ++s)
-    for (size_t q = 0; q < P.V.in[s].size(); ++q) {
-      const DenseBlock& b = P.V.block[s][q];
-      H.vblock_off.push_back(b.off);
-      H.vblock_size.push_back(b.size());
-      H.Vblk.push_back({b.off, b.rows, b.cols});
-      double w = P.sym.coeff_dot(P.V.out[s][q]);
-      for (int64_t e = 0; e < b.size(); e += 4096) {
-        H.seg_off.push_back(b.off + e);
-        H.seg_len.push_back((int32_t)std::min<int64_t>(4096, b.size() - e));
-        H.seg_w.push_back(w);
-      }
+  // coeff_dot weights per output block (DmrgPivotVector.h:323), split in segments
+  for (int64_t o = 0; o < nO; ++o) {
+    const BlockDim& b = S.Ob[o];
+    const int64_t size = b.rows * b.cols;
+    H.vblock_off.push_back(b.off);
+    H.vblock_size.push_back(size);
+    const double w = S.Ow.empty() ? 1.0 : S.Ow[o];
+    for (int64_t e = 0; e < size; e += 4096) {
+      H.seg_off.push_back(b.off + e);
+      H.seg_len.push_back((int32_t)std::min<int64_t>(4096, size - e));
+      H.seg_w.push_back(w);
     }
+  }
 
   validate_host_plan(H);
   build_tiles(H);
@@ -606,15 +620,15 @@ void build_host_plan(const Pivot2& P, HostPlan& H) {
   st.n_T = (int64_t)H.probA.size();
   st.n_U = (int64_t)H.comb.size();
   st.n_out = (int64_t)H.probC.size();
-  st.n_L = (int64_t)lblocks.size();
-  st.n_R = (int64_t)rblocks.size();
-  st.n_V = nV;
+  st.n_L = (int64_t)S.Xb.size();
+  st.n_R = (int64_t)S.Zb.size();
+  st.n_V = nO;
   st.L_elems = (int64_t)H.Ldata.size();
   st.R_elems = (int64_t)H.Rdata.size();
-  st.V_elems = P.V.size;
+  st.V_elems = S.osize;
   st.T_elems = H.T_size;
   st.U_elems = H.U_size;
-  st.bytes_alg_fp64 = 8.0 * (double)(st.L_elems + st.R_elems + 2 * st.V_elems);
+  st.bytes_alg_fp64 = 8.0 * (double)(st.L_elems + st.R_elems + S.ysize + S.osize);
 }
 
 }  // namespace vmps
@@ -778,7 +792,7 @@ void build_bf16_layout(const HostPlan& H, Bf16Layout& B) {
           throw VmpsError(VMPS_ERR_INTERNAL, "bf16 stage C term out of range");
     }
   for (const VConv& v : B.vconv)
-    if (v.dst_off + (int64_t)v.Kp * v.N > B.V_size || v.src_off + (int64_t)v.K * v.N > H.vsize)
+    if (v.dst_off + (int64_t)v.Kp * v.N > B.V_size || v.src_off + (int64_t)v.K * v.N > H.ysize)
       throw VmpsError(VMPS_ERR_INTERNAL, "bf16 conversion task out of range");
 }
 

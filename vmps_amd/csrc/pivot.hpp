@@ -236,11 +236,32 @@ struct BlockDim {
   int64_t off, rows, cols;
 };
 
+// Literal per-term list: out block o, X block l, Y block v, Z block r, coefficient.
+struct Term {
+  int64_t out_v, l_blk, in_v, r_blk;
+  double c;
+};
+
+// A sum of scalar-weighted block triple products  O_o += c X_l Y_v Z_r: the
+// common form of the two-site / one-site / zero-site HxV (X = L, Y = Vin,
+// Z = R) and of the environment updates (X = A_bra^T, Y = L, Z = A_ket for
+// contract_L; X = A_ket, Y = R, Z = A_bra^T for contract_R).  X and Z are
+// packed column-major in product orientation (transposes already applied).
+struct TripleSet {
+  std::vector<double> Xdata, Zdata;
+  std::vector<BlockDim> Xb, Zb;
+  std::vector<BlockDim> Yb, Ob;  // blocks of the input and output vectors
+  int64_t ysize = 0, osize = 0;
+  std::vector<double> Ow;        // coeff_dot weight of each output block (empty: all 1)
+  std::vector<Term> terms;
+};
+
 struct HostPlan {
   // packed operands actually referenced by terms
-  std::vector<double> Ldata, Rdata;  // copied L/R blocks
-  std::vector<BlockDim> Lblk, Rblk, Vblk;  // blocks of Ldata, Rdata and the vector
-  int64_t vsize = 0;
+  std::vector<double> Ldata, Rdata;  // X and Z blocks
+  std::vector<BlockDim> Lblk, Rblk, Vblk;  // blocks of Ldata, Rdata and the input vector
+  int64_t vsize = 0;                 // output vector size (== input size for HxV plans)
+  int64_t ysize = 0, osize = 0;      // input / output vector sizes
   // device work lists (built from the tables below by build_tiles)
   std::vector<TileA> tilesA;
   std::vector<TileC> tilesC;
@@ -264,17 +285,15 @@ struct HostPlan {
   std::vector<int64_t> vblock_off, vblock_size;
 };
 
-// Literal per-term list, in the order the reference's OxV visits them.
-struct Term {
-  int64_t out_v, l_blk, in_v, r_blk;  // V block ids (global), L/R block ids
-  double c;
-};
+// Per-term list of a two-site pivot, in the order the reference's OxV visits them.
 void expand_terms(const Pivot2& P, std::vector<Term>& terms,
                   std::vector<std::pair<size_t, size_t>>& lblocks,
                   std::vector<std::pair<size_t, size_t>>& rblocks,
                   std::vector<std::pair<int, int>>& vblocks);
 
 void build_host_plan(const Pivot2& P, HostPlan& H);
+void triples_from_pivot(const Pivot2& P, TripleSet& S);
+void build_plan(const TripleSet& S, HostPlan& H);
 void build_tiles(HostPlan& H);
 // Split-BF16 operands and work lists of the BF16x3 path (built on demand).
 void build_bf16_layout(const HostPlan& H, Bf16Layout& B);
