@@ -1,0 +1,67 @@
+"""Exact diagonalisation of small open chains, built from Kronecker products
+of local operators (independent of the MPO construction)."""
+import numpy as np
+from scipy.sparse import csr_matrix, identity, kron
+from scipy.sparse.linalg import eigsh
+
+
+def _op(local, i, L, d, string=None):
+    """local operator on site i; `string` is applied on every site < i."""
+    mats = []
+    for j in range(L):
+        if j == i:
+            mats.append(csr_matrix(local))
+        elif string is not None and j < i:
+            mats.append(csr_matrix(string))
+        else:
+            mats.append(identity(d, format="csr"))
+    out = mats[0]
+    for m in mats[1:]:
+        out = kron(out, m, format="csr")
+    return out
+
+
+def heisenberg_ground(L, J=1.0, two_sz=0):
+    Sp = np.array([[0.0, 0.0], [1.0, 0.0]])
+    Sm = Sp.T
+    Sz = np.diag([-0.5, 0.5])
+    H = 0
+    for i in range(L - 1):
+        H = H + J * (_op(Sz, i, L, 2) @ _op(Sz, i + 1, L, 2)
+                     + 0.5 * (_op(Sp, i, L, 2) @ _op(Sm, i + 1, L, 2) + _op(Sm, i, L, 2) @ _op(Sp, i + 1, L, 2)))
+    # sector 2Sz: basis index bits, state 1 = up (+1), 0 = down (-1); site 0 is the most significant
+    idx = [n for n in range(2 ** L) if sum(1 if (n >> (L - 1 - i)) & 1 else -1 for i in range(L)) == two_sz]
+    Hs = H[idx][:, idx]
+    return float(eigsh(Hs, k=1, which="SA", tol=1e-13)[0][0]) if Hs.shape[0] > 50 else \
+        float(np.linalg.eigvalsh(Hs.toarray())[0])
+
+
+def hubbard_ground(L, t=1.0, U=8.0, N=None, two_sz=0):
+    """Fermions with an explicit Jordan-Wigner string; local basis |0>, |up>, |dn>, |updn>."""
+    N = L if N is None else N
+    c_up = np.zeros((4, 4))
+    c_dn = np.zeros((4, 4))
+    c_up[0, 1] = 1.0
+    c_up[2, 3] = 1.0
+    c_dn[0, 2] = 1.0
+    c_dn[1, 3] = -1.0
+    F = np.diag([1.0, -1.0, -1.0, 1.0])
+    n_up = c_up.T @ c_up
+    n_dn = c_dn.T @ c_dn
+    H = 0
+    for i in range(L):
+        H = H + U * _op(n_up @ n_dn, i, L, 4)
+    for i in range(L - 1):
+        for c in (c_up, c_dn):
+            ci = _op(c, i, L, 4, string=F)
+            cj = _op(c, i + 1, L, 4, string=F)
+            H = H - t * (ci.T @ cj + cj.T @ ci)
+    nloc = [0, 1, 1, 2]
+    szloc = [0, 1, -1, 0]
+    idx = []
+    for n in range(4 ** L):
+        digits = [(n // 4 ** (L - 1 - i)) % 4 for i in range(L)]
+        if sum(nloc[x] for x in digits) == N and sum(szloc[x] for x in digits) == two_sz:
+            idx.append(n)
+    Hs = H[idx][:, idx]
+    return float(eigsh(Hs, k=1, which="SA", tol=1e-13)[0][0])

@@ -1,0 +1,43 @@
+"""Two-site DMRG with the device backend (Lanczos + HxV + contract_L/R on the
+GPU): exact diagonalisation, the CPU oracle backend, and mixed precision."""
+import sys
+
+import pytest
+
+from vmps_amd import dmrg
+
+sys.path.insert(0, __file__.rsplit("/", 1)[0])
+import dmrg_oracle  # noqa: E402
+import ed  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+def test_gpu_dmrg_matches_ed():
+    e_ed = ed.heisenberg_ground(10)
+    r = dmrg.run_dmrg("HeisenbergU1", 10, 32, 2, backend=dmrg.GpuBackend())
+    assert abs(r["energy"] - e_ed) <= 1e-10 * abs(e_ed)
+    e_ed = ed.hubbard_ground(6, U=8.0)
+    r = dmrg.run_dmrg("HubbardU1xU1", 6, 64, 2, backend=dmrg.GpuBackend(), params=dict(t=1.0, U=8.0))
+    assert abs(r["energy"] - e_ed) <= 1e-10 * abs(e_ed)
+
+
+def test_gpu_dmrg_matches_oracle_dmrg_L32_M64():
+    """HeisenbergU1 L=32, M=64, 4 sweeps from the same random MPS: device and
+    CPU-oracle backends give the same energy."""
+    r_gpu = dmrg.run_dmrg("HeisenbergU1", 32, 64, 4, backend=dmrg.GpuBackend(tol_eigval=1e-13, tol_state=1e-8))
+    r_cpu = dmrg.run_dmrg("HeisenbergU1", 32, 64, 4, backend=dmrg_oracle.OracleBackend())
+    print(r_gpu["energy"], r_cpu["energy"], r_gpu["sweep_seconds"], r_cpu["sweep_seconds"])
+    assert abs(r_gpu["energy"] - r_cpu["energy"]) <= 1e-9 * abs(r_cpu["energy"])
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
+def test_mixed_precision_dmrg_energy(prec):
+    """Mixed-precision Lanczos with FP64 refinement: |dE| <= 1e-9 vs the FP64 run."""
+    r64 = dmrg.run_dmrg("HubbardU1xU1", 12, 128, 3, backend=dmrg.GpuBackend(tol_eigval=1e-12, tol_state=1e-7),
+                        params=dict(t=1.0, U=8.0))
+    rmx = dmrg.run_dmrg("HubbardU1xU1", 12, 128, 3,
+                        backend=dmrg.GpuBackend(precision=prec, tol_eigval=1e-12, tol_state=1e-7),
+                        params=dict(t=1.0, U=8.0))
+    print(prec, r64["energy"], rmx["energy"], rmx["energy"] - r64["energy"])
+    assert abs(rmx["energy"] - r64["energy"]) <= 1e-9
