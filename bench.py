@@ -76,6 +76,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=20261202)
+    ap.add_argument("--sweeps", type=int, default=0, help="also run this many DMRG sweeps of the chain")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -164,7 +165,27 @@ def main():
         ref = cb.pop("out")
         res["cpu_baseline"] = cb
         res["parity_rel_err_vs_oracle"] = float(np.linalg.norm(out_gpu - ref) / np.linalg.norm(ref))
+    if args.sweeps > 0:
+        res["sweeps"] = sweep_bench(args)
     print(json.dumps(res))
+
+
+def sweep_bench(args):
+    """Full two-site DMRG sweeps of the workload's chain from the same random
+    MPS, FP64 and mixed (BF16x3 Lanczos refined in FP64): energies, the
+    energy difference and the wall time per sweep."""
+    from vmps_amd import dmrg, synth
+    cfg = synth.WORKLOADS[args.workload]
+    params = {"t": 1.0, "U": 8.0} if cfg["model"] == "HubbardU1xU1" else {"J": 1.0}
+    out = {"model": cfg["model"], "L": cfg["L"], "M": cfg["M"], "sweeps": args.sweeps}
+    for prec in ("fp64", "bf16x3"):
+        be = dmrg.GpuBackend(precision=prec, tol_eigval=1e-10, tol_state=1e-6)
+        r = dmrg.run_dmrg(cfg["model"], cfg["L"], cfg["M"], args.sweeps, seed=args.seed, backend=be, params=params)
+        out[prec] = {"energy": r["energy"], "energies": r["energies"], "sweep_seconds": r["sweep_seconds"],
+                     "hxv_calls": be.hxv_calls, "max_truncation": max(r["truncation"][-(cfg["L"] - 1):])}
+    out["dE_mixed_vs_fp64"] = out["bf16x3"]["energy"] - out["fp64"]["energy"]
+    out["sweep_speedup_mixed_vs_fp64"] = out["fp64"]["sweep_seconds"][-1] / out["bf16x3"]["sweep_seconds"][-1]
+    return out
 
 
 if __name__ == "__main__":

@@ -48,10 +48,11 @@ def reachable_counts(qloc, nsites, vac):
     return counts
 
 
-def random_mps(sites, Q, M, rng):
+def random_mps(sites, Q, M, rng, spread=2):
     """Random N(0,1) MPS with charge Q, bond dims at most M: each bond keeps the
-    sectors reachable from both ends, dims split evenly and capped by the
-    number of product states on either side."""
+    sectors reachable from both ends whose charge lies within `spread` of the
+    uniform profile l*Q/L, dims split evenly and capped by the number of
+    product states on either side."""
     L = len(sites)
     qloc = sites[0].qloc
     vac = tuple(0 for _ in Q)
@@ -59,7 +60,9 @@ def random_mps(sites, Q, M, rng):
     for l in range(L + 1):
         cl = reachable_counts(qloc, l, vac)
         cr = reachable_counts(qloc, L - l, vac)
-        allowed = {q: min(c, cr[_sub(Q, q)]) for q, c in cl.items() if _sub(Q, q) in cr}
+        centre = tuple(l * x / L for x in Q)
+        allowed = {q: min(c, cr[_sub(Q, q)]) for q, c in cl.items()
+                   if _sub(Q, q) in cr and all(abs(a - b) <= spread for a, b in zip(q, centre))}
         share = max(1, M // max(1, len(allowed)))
         bonds.append({q: max(1, min(c, share)) for q, c in sorted(allowed.items())})
     A = []
