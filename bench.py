@@ -77,7 +77,19 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=20261202)
     ap.add_argument("--sweeps", type=int, default=0, help="also run this many DMRG sweeps of the chain")
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="only run 5 device copies of this many MiB (reference kernel for the HBM byte counters)")
     args = ap.parse_args()
+    if args.calibrate_copy_mib:
+        import torch
+        n = args.calibrate_copy_mib * (1 << 20) // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibration_copy_bytes": n * 8, "copies": 5}))
+        return
 
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported "
@@ -123,6 +135,15 @@ def main():
     peak = PEAK_TFLOPS[args.precision]
     ktype = {"fp64": "double", "fp32": "float", "bf16x3": "bf16x3"}[args.precision]
     kname = ("k_stage_a<%s> (T = L.V)" % ktype) if dom == "A" else ("k_stage_c<%s> ((sum c T).R)" % ktype)
+    # HBM bytes per launch of that kernel, from the committed copy-calibrated
+    # FETCH_SIZE / WRITE_SIZE profile of this workload (null when not profiled)
+    traffic = None
+    try:
+        with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
+            tr = json.load(f).get("%s/%s" % (args.workload, args.precision), {})
+        traffic = tr.get("k_stage_a" if dom == "A" else "k_stage_c")
+    except (OSError, ValueError):
+        pass
 
     res = {
         "metric": "DMRG HxV GFLOP/s (F_alg / t), plain FP64 GPU",
@@ -157,7 +178,9 @@ def main():
             "peak": peak,
             "unit": "TFLOP/s",
             "frac": achieved / peak,
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
+            "algorithmic_bytes_hxv": st["bytes_alg_fp64"],
         },
     }
     if not args.no_cpu:
