@@ -53,7 +53,10 @@ typedef enum {
   VMPS_PREC_BF16X3 = 2  /* split-BF16 x3 MFMA with FP32 accumulation, FP64 vectors */
 } vmps_precision;
 
-typedef struct vmps_pivot2 vmps_pivot2; /* host image of PivotMatrix2 + Vin layout */
+/* A pivot: the host image of PivotMatrix2, PivotMatrix1 or PivotMatrix0 plus
+ * the PivotVector layout.  The vmps_pivot2_* builder calls below apply to all
+ * three kinds (the name is kept from the two-site API). */
+typedef struct vmps_pivot2 vmps_pivot2;
 typedef struct vmps_plan vmps_plan;     /* device-resident staged block-triple plan */
 
 int vmps_abi_version(void);
@@ -67,11 +70,31 @@ const char* vmps_last_error(void);
 vmps_status vmps_pivot2_create(int nq, const int* kinds, int nloc12, const int* qloc12, int nloc34,
                                const int* qloc34, int nop12, const int* qop12, int nop34, const int* qop34,
                                vmps_pivot2** out);
+/* PivotMatrix1 (pivot/DmrgPivotMatrix1.h:11-66; Terms[0].qloc / .qOp): a
+ * one-site pivot.  Its W goes into slot 12 of vmps_pivot2_add_W; PivotVector
+ * components are the local states s (DmrgPivotVector.h:28-31). */
+vmps_status vmps_pivot1_create(int nq, const int* kinds, int nloc, const int* qloc, int nop, const int* qop,
+                               vmps_pivot2** out);
+/* PivotMatrix0(L, R) (pivot/DmrgPivotMatrix0.h:10-27): a zero-site pivot (the
+ * bond matrix C); the vector has one component, s = 0, with blocks (q, q). */
+vmps_status vmps_pivot0_create(int nq, const int* kinds, vmps_pivot2** out);
+/* PivotMatrix2::Terms / PivotMatrix1::Terms (DmrgPivotMatrix2.h:54,
+ * DmrgPivotMatrix1.h:56): HxV sums over terms (DmrgPivotMatrix2.h:233-245).
+ * A pivot starts with one term.  add_term appends a term with its own
+ * operator bases (qop34 unused for one-site, both unused for zero-site) and
+ * makes it current; add_env, add_W and the block-structure calls act on the
+ * current term.  All terms share the local bases and the vector layout. */
+vmps_status vmps_pivot_add_term(vmps_pivot2* p, int nop12, const int* qop12, int nop34, const int* qop34,
+                                int* term);
+vmps_status vmps_pivot_select_term(vmps_pivot2* p, int term);
+/* nsite (0, 1, 2) and number of terms; either pointer may be NULL. */
+vmps_status vmps_pivot_info(const vmps_pivot2* p, int* nsite, int* nterms);
 void vmps_pivot2_destroy(vmps_pivot2* p);
 
-/* Combined two-site basis loc12.combine(loc34) (tensors/Qbasis.h:686-798), i.e.
- * the index s of PivotVector::data[s] (DmrgPivotVector.h:33-48).  ns receives
- * its size; qloc (ns*nq ints) may be NULL. */
+/* Index s of PivotVector::data[s]: for two sites the combined basis
+ * loc12.combine(loc34) (tensors/Qbasis.h:686-798; DmrgPivotVector.h:33-48),
+ * for one site qloc itself, for zero sites the single vacuum component.
+ * ns receives its size; qloc (ns*nq ints) may be NULL. */
 vmps_status vmps_pivot2_local_basis(const vmps_pivot2* p, int* ns, int* qloc);
 
 /* One sector of the environment Tripod L (which=0) or R (which=1):
@@ -84,7 +107,8 @@ vmps_status vmps_pivot2_add_env(vmps_pivot2* p, int which, const int* qin, const
 
 /* One sparse block of W12[s1][s2][k] (which=12) or W34[s3][s4][k] (which=34):
  * Biped<Sym,SparseMatrix<double,ColMajor>>::push_back(qin, qout, mat)
- * (MpoTerms.h:1175-1233).  nnz triplets (row, col, val) in any order. */
+ * (MpoTerms.h:1175-1233).  nnz triplets (row, col, val) in any order.
+ * One-site pivots take their W[s1][s2][k] in slot 12. */
 vmps_status vmps_pivot2_add_W(vmps_pivot2* p, int which, int s1, int s2, int k, const int* qin, const int* qout,
                               int rows, int cols, int64_t nnz, const int* row, const int* col, const double* val);
 
@@ -97,15 +121,18 @@ vmps_status vmps_pivot2_add_V_block(vmps_pivot2* p, int s, const int* qin, const
 vmps_status vmps_pivot2_vsize(const vmps_pivot2* p, int64_t* size);
 
 /* precalc_blockStructure(L, Vin, W12, W34, Vin, R, qloc12, qloc34, qOp12,
- * qOp34, qlhs, qrhs, factor_cgcs) (tensors/DmrgIndexGymnastics.h:698-856):
- * fills the pivot's block structure, including the SU(2) 6j/9j factors. */
+ * qOp34, qlhs, qrhs, factor_cgcs) (tensors/DmrgIndexGymnastics.h:698-856),
+ * or the one-site form (L, Vin, W, Vin, R, qloc, qOp, ...) (:485-590), for
+ * every term: fills the pivot's block structure, including the SU(2) 6j/9j
+ * factors.  Zero-site pivots need none (the call is accepted). */
 vmps_status vmps_pivot2_precalc(vmps_pivot2* p);
 
-/* Use the caller's own PivotMatrix2::qlhs / qrhs / factor_cgcs
- * (DmrgPivotMatrix2.h:66-70) instead of vmps_pivot2_precalc.
- * qlhs[2*i] = {s1s3, q13}; entries rhs_begin[i] .. rhs_begin[i+1]-1 of
- * qrhs (12 values each: s2s4,q24,qL,qR,s1,s2,k12,qW12,s3,s4,k34,qW34) and of
- * factors belong to lhs entry i. */
+/* Use the caller's own qlhs / qrhs / factor_cgcs of the current term
+ * (DmrgPivotMatrix2.h:66-70, DmrgPivotMatrix1.h:58-62) instead of
+ * vmps_pivot2_precalc.  qlhs[2*i] = {s1s3, q13}; entries
+ * rhs_begin[i] .. rhs_begin[i+1]-1 of qrhs (12 values each:
+ * s2s4,q24,qL,qR,s1,s2,k12,qW12,s3,s4,k34,qW34; one-site: s2,q2,qL,qR,k,qW
+ * followed by six zeros) and of factors belong to lhs entry i. */
 vmps_status vmps_pivot2_set_blockstructure(vmps_pivot2* p, int64_t nlhs, const int64_t* qlhs,
                                            const int64_t* rhs_begin, const int64_t* qrhs, const double* factors);
 /* Read the block structure back.  Call with NULL arrays to get nlhs / nrhs. */

@@ -1,6 +1,6 @@
 """ORACLE -- test infrastructure, not part of the product.
 
-Loader of the CPU FP64 restatement of the reference's two-site HxV
+Loader of the CPU FP64 restatement of the reference's zero-, one- and two-site HxV
 (hxv_oracle.c).  Only tests/, __graft_entry__.smoke() and bench.py's
 cpu_baseline leg use it, always as the checker or the timed CPU baseline.
 """
@@ -19,6 +19,9 @@ SIGNATURES = {
     "oracle_last_error": (ctypes.c_char_p, []),
     "oracle_pivot2_create": (ctypes.c_int, [ctypes.c_int, P_int, ctypes.c_int, P_int, ctypes.c_int, P_int,
                                             ctypes.c_int, P_int, ctypes.c_int, P_int, ctypes.POINTER(c_void)]),
+    "oracle_pivot1_create": (ctypes.c_int, [ctypes.c_int, P_int, ctypes.c_int, P_int, ctypes.c_int, P_int,
+                                            ctypes.POINTER(c_void)]),
+    "oracle_pivot0_create": (ctypes.c_int, [ctypes.c_int, P_int, ctypes.POINTER(c_void)]),
     "oracle_pivot2_destroy": (None, [c_void]),
     "oracle_pivot2_local_basis": (ctypes.c_int, [c_void, P_int, P_int]),
     "oracle_pivot2_add_env": (ctypes.c_int, [c_void, ctypes.c_int, P_int, P_int, P_int, ctypes.c_int, P_i64, P_i64,

@@ -1,10 +1,10 @@
 /*
  * ORACLE -- test infrastructure, not part of the product.
  *
- * A CPU FP64 restatement of the reference's two-site effective-Hamiltonian
- * application, written to check the HIP engine and to serve as the timed CPU
- * baseline.  Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
- * leg may load it.  It follows the reference literally:
+ * A CPU FP64 restatement of the reference's zero-, one- and two-site
+ * effective-Hamiltonian application, written to check the HIP engine and to
+ * serve as the timed CPU baseline.  Only tests/, __graft_entry__.smoke() and
+ * bench.py's cpu_baseline leg may load it.  It follows the reference literally:
  *
  *   two-site basis      Qbasis::pullData / combine / outer_num / inner_num /
  *                       leftOffset            tensors/Qbasis.h:228-254,335-350,
@@ -12,6 +12,10 @@
  *   block structure     precalc_blockStructure (two-site)
  *                                             tensors/DmrgIndexGymnastics.h:698-856
  *                       AAWWAA                tensors/DmrgIndexGymnastics.h:198-262
+ *                       precalc_blockStructure (one-site) + LAWA
+ *                                             tensors/DmrgIndexGymnastics.h:485-590, 32-77
+ *   one-site OxV        pivot/DmrgPivotMatrix1.h:131-332
+ *   zero-site HxV       pivot/DmrgPivotMatrix0.h:31-96
  *   coefficients        coeff_tensorProd / coeff_Apair / coeff_HPsi / coeff_dot
  *                                             symmetry/SU2.h:284-290,388-397,455-466,508-519
  *                       phase                 symmetry/functions.h:50-55
@@ -274,6 +278,7 @@ typedef struct {
 
 typedef struct {
   sym_t S;
+  int nsite; /* 2: PivotMatrix2, 1: PivotMatrix1 (W, qloc, qOp in the 12 slots), 0: PivotMatrix0 */
   int nloc12, nloc34, nop12, nop34;
   qn_t *qloc12, *qloc34, *qop12, *qop34;
   envlist_t L, R;
@@ -473,8 +478,50 @@ int oracle_pivot2_create(int nq, const int* kinds, int nloc12, const int* qloc12
   for (int i = 0; i < nop34; ++i) P->qop34[i] = qread(qop34 + i * nq, nq);
   P->W12 = (wbiped_t*)calloc((size_t)nloc12 * nloc12 * nop12, sizeof(wbiped_t));
   P->W34 = (wbiped_t*)calloc((size_t)nloc34 * nloc34 * nop34, sizeof(wbiped_t));
+  P->nsite = 2;
   build_basis(P);
   P->V = (vbiped_t*)calloc(P->ns ? P->ns : 1, sizeof(vbiped_t));
+  *out = P;
+  return OK;
+}
+
+/* PivotMatrix1 (pivot/DmrgPivotMatrix1.h:11-66): one site, PivotVector
+ * components = local states */
+int oracle_pivot1_create(int nq, const int* kinds, int nloc, const int* qloc, int nop, const int* qop, void** out) {
+  if (!out || nq < 1 || nq > 2 || nloc <= 0 || nop <= 0) return fail(ERR_ARG, "bad arguments");
+  opivot_t* P = (opivot_t*)calloc(1, sizeof(opivot_t));
+  P->S.nq = nq;
+  P->S.kind[0] = kinds[0];
+  P->S.kind[1] = nq > 1 ? kinds[1] : 1;
+  P->nsite = 1;
+  P->nloc12 = nloc;
+  P->nop12 = nop;
+  P->qloc12 = (qn_t*)xrealloc(NULL, sizeof(qn_t) * nloc);
+  P->qop12 = (qn_t*)xrealloc(NULL, sizeof(qn_t) * nop);
+  for (int i = 0; i < nloc; ++i) P->qloc12[i] = qread(qloc + i * nq, nq);
+  for (int i = 0; i < nop; ++i) P->qop12[i] = qread(qop + i * nq, nq);
+  P->W12 = (wbiped_t*)calloc((size_t)nloc * nloc * nop, sizeof(wbiped_t));
+  P->ns = nloc;
+  P->qs = (qn_t*)xrealloc(NULL, sizeof(qn_t) * nloc);
+  memcpy(P->qs, P->qloc12, sizeof(qn_t) * nloc);
+  P->V = (vbiped_t*)calloc(nloc, sizeof(vbiped_t));
+  *out = P;
+  return OK;
+}
+
+/* PivotMatrix0 (pivot/DmrgPivotMatrix0.h:10-27): L and R only, one vector
+ * component */
+int oracle_pivot0_create(int nq, const int* kinds, void** out) {
+  if (!out || nq < 1 || nq > 2) return fail(ERR_ARG, "bad arguments");
+  opivot_t* P = (opivot_t*)calloc(1, sizeof(opivot_t));
+  P->S.nq = nq;
+  P->S.kind[0] = kinds[0];
+  P->S.kind[1] = nq > 1 ? kinds[1] : 1;
+  P->nsite = 0;
+  P->ns = 1;
+  P->qs = (qn_t*)calloc(1, sizeof(qn_t));
+  for (int i = 0; i < nq; ++i) P->qs[0].v[i] = P->S.kind[i] == 2 ? 1 : 0;
+  P->V = (vbiped_t*)calloc(1, sizeof(vbiped_t));
   *out = P;
   return OK;
 }
@@ -678,8 +725,118 @@ static int entry_cmp(const void* a, const void* b) {
 
 #define CGC_EPS 1e-14
 
+static void push_entry(entry_t** ent, int64_t* nent, int64_t* cap, int64_t k0, int64_t k1, const int64_t* v, double f) {
+  if (*nent == *cap) {
+    *cap = *cap ? 2 * *cap : 1024;
+    *ent = (entry_t*)xrealloc(*ent, sizeof(entry_t) * *cap);
+  }
+  entry_t* e = &(*ent)[*nent];
+  e->key0 = k0;
+  e->key1 = k1;
+  memcpy(e->val, v, sizeof(int64_t) * 12);
+  e->f = f;
+  e->seq = *nent;
+  ++*nent;
+}
+
+/* sort the entries by key (stable in visiting order) into qlhs/qrhs/fac */
+static void store_entries(opivot_t* P, entry_t* ent, int64_t nent) {
+  qsort(ent, (size_t)nent, sizeof(entry_t), entry_cmp);
+  free(P->qlhs);
+  free(P->rhs_begin);
+  free(P->qrhs);
+  free(P->fac);
+  int64_t nl = 0;
+  for (int64_t i = 0; i < nent; ++i)
+    if (i == 0 || ent[i].key0 != ent[i - 1].key0 || ent[i].key1 != ent[i - 1].key1) ++nl;
+  P->nlhs = nl;
+  P->nrhs = nent;
+  P->qlhs = (int64_t*)xrealloc(NULL, sizeof(int64_t) * (2 * nl + 1));
+  P->rhs_begin = (int64_t*)xrealloc(NULL, sizeof(int64_t) * (nl + 1));
+  P->qrhs = (int64_t*)xrealloc(NULL, sizeof(int64_t) * (12 * nent + 1));
+  P->fac = (double*)xrealloc(NULL, sizeof(double) * (nent + 1));
+  int64_t l = -1;
+  for (int64_t i = 0; i < nent; ++i) {
+    if (i == 0 || ent[i].key0 != ent[i - 1].key0 || ent[i].key1 != ent[i - 1].key1) {
+      ++l;
+      P->qlhs[2 * l] = ent[i].key0;
+      P->qlhs[2 * l + 1] = ent[i].key1;
+      P->rhs_begin[l] = i;
+    }
+    memcpy(&P->qrhs[12 * i], ent[i].val, sizeof(int64_t) * 12);
+    P->fac[i] = ent[i].f;
+  }
+  P->rhs_begin[nl] = nent;
+  P->has_bs = 1;
+}
+
+/* One-site precalc_blockStructure (tensors/DmrgIndexGymnastics.h:485-590)
+ * with the LAWA matcher (:32-77) on Abra = Aket = Vin. */
+static int precalc_one_site(opivot_t* P) {
+  const sym_t* S = &P->S;
+  const int NA = non_abelian(S);
+  entry_t* ent = NULL;
+  int64_t nent = 0, cap = 0;
+  qn_t Routs[64], Rins[64], Rmids[64];
+  for (int s1 = 0; s1 < P->nloc12; ++s1)
+    for (int s2 = 0; s2 < P->nloc12; ++s2)
+      for (int k = 0; k < P->nop12; ++k) {
+        if (!validate3(S, P->qloc12[s2], P->qop12[k], P->qloc12[s1])) continue;
+        const wbiped_t* W = W12AT(P, s1, s2, k);
+        for (int qL = 0; qL < P->L.n; ++qL) {
+          const env_t* Le = &P->L.s[qL];
+          int nRo = reduce(S, Le->in, P->qloc12[s1], Routs);
+          for (int a = 0; a < nRo; ++a) {
+            int q1 = vfind(&P->V[s1], Le->in, Routs[a]);
+            if (q1 < 0) continue;
+            int nRi = reduce(S, Le->out, P->qloc12[s2], Rins);
+            for (int b = 0; b < nRi; ++b) {
+              int q2 = vfind(&P->V[s2], Le->out, Rins[b]);
+              if (q2 < 0) continue;
+              int nRm = reduce(S, Le->mid, P->qop12[k], Rmids);
+              for (int c = 0; c < nRm; ++c) {
+                int qW = wfind(W, Le->mid, Rmids[c]);
+                if (qW < 0) continue;
+                if (!validate3(S, Rins[b], Rmids[c], Routs[a])) continue;
+                int qR = efind(&P->R, Rins[b], Routs[a], Rmids[c]);
+                if (qR < 0) continue;
+                const wblk_t* w = &W->b[qW];
+                int empty = 1;
+                for (int64_t i = 0; i < w->nnz; ++i) {
+                  if (w->row[i] >= Le->adim || w->col[i] >= P->R.s[qR].adim) return fail(ERR_LAYOUT, "W index");
+                  const mat_t* La = &Le->blk[w->row[i]];
+                  const mat_t* Rb = &P->R.s[qR].blk[w->col[i]];
+                  if (La->rows * La->cols > 0 && Rb->rows * Rb->cols > 0) empty = 0;
+                }
+                if (empty) continue;
+                double f = 1.0;
+                if (NA) {
+                  qn_t a9[9] = {P->V[s2].b[q2].in, P->qloc12[s2], P->V[s2].b[q2].out,
+                                Le->mid,           P->qop12[k],   Rmids[c],
+                                P->V[s1].b[q1].in, P->qloc12[s1], P->V[s1].b[q1].out};
+                  f = c9(S, a9, 0);
+                }
+                if (fabs(f) < CGC_EPS) continue;
+                int64_t v[12] = {s2, q2, qL, qR, k, qW, 0, 0, 0, 0, 0, 0};
+                push_entry(&ent, &nent, &cap, s1, q1, v, f);
+              }
+            }
+          }
+        }
+      }
+  store_entries(P, ent, nent);
+  free(ent);
+  return OK;
+}
+
 int oracle_pivot2_precalc(void* h) {
   opivot_t* P = (opivot_t*)h;
+  if (P->nsite == 1) return precalc_one_site(P);
+  if (P->nsite == 0) {
+    P->nlhs = P->nrhs = 0;
+    P->has_bs = 1;
+    return OK;
+  }
   const sym_t* S = &P->S;
   const int NA = non_abelian(S);
   entry_t* ent = NULL;
@@ -749,18 +906,8 @@ int oracle_pivot2_precalc(void* h) {
                                               A13->b[q13].in, qmerge13, A13->b[q13].out};
                                 cgcH = c9(S, a9, 0);
                               }
-                              if (nent == cap) {
-                                cap = cap ? 2 * cap : 1024;
-                                ent = (entry_t*)xrealloc(ent, sizeof(entry_t) * cap);
-                              }
-                              entry_t* e = &ent[nent];
-                              e->key0 = s1s3;
-                              e->key1 = q13;
                               int64_t v[12] = {s2s4, q24, qL, qR, s1, s2, k12, qW12, s3, s4, k34, qW34};
-                              memcpy(e->val, v, sizeof v);
-                              e->f = cgc6 * cgc9 * cgcH;
-                              e->seq = nent;
-                              ++nent;
+                              push_entry(&ent, &nent, &cap, s1s3, q13, v, cgc6 * cgc9 * cgcH);
                             }
                           }
                         }
@@ -770,34 +917,8 @@ int oracle_pivot2_precalc(void* h) {
               }
             }
       }
-  qsort(ent, (size_t)nent, sizeof(entry_t), entry_cmp);
-  free(P->qlhs);
-  free(P->rhs_begin);
-  free(P->qrhs);
-  free(P->fac);
-  int64_t nl = 0;
-  for (int64_t i = 0; i < nent; ++i)
-    if (i == 0 || ent[i].key0 != ent[i - 1].key0 || ent[i].key1 != ent[i - 1].key1) ++nl;
-  P->nlhs = nl;
-  P->nrhs = nent;
-  P->qlhs = (int64_t*)xrealloc(NULL, sizeof(int64_t) * (2 * nl + 1));
-  P->rhs_begin = (int64_t*)xrealloc(NULL, sizeof(int64_t) * (nl + 1));
-  P->qrhs = (int64_t*)xrealloc(NULL, sizeof(int64_t) * (12 * nent + 1));
-  P->fac = (double*)xrealloc(NULL, sizeof(double) * (nent + 1));
-  int64_t l = -1;
-  for (int64_t i = 0; i < nent; ++i) {
-    if (i == 0 || ent[i].key0 != ent[i - 1].key0 || ent[i].key1 != ent[i - 1].key1) {
-      ++l;
-      P->qlhs[2 * l] = ent[i].key0;
-      P->qlhs[2 * l + 1] = ent[i].key1;
-      P->rhs_begin[l] = i;
-    }
-    memcpy(&P->qrhs[12 * i], ent[i].val, sizeof(int64_t) * 12);
-    P->fac[i] = ent[i].f;
-  }
-  P->rhs_begin[nl] = nent;
+  store_entries(P, ent, nent);
   free(ent);
-  P->has_bs = 1;
   return OK;
 }
 
@@ -963,6 +1084,84 @@ static int oxv_range(const opivot_t* P, const double* vin, double* vout, int64_t
   return OK;
 }
 
+/* one literal term  o += f * (L . V) . R  with heap temporaries, as Eigen's
+ * left-to-right evaluation */
+static int term_lvr(const mat_t* La, const vblk_t* Vb, const mat_t* Rb, const vblk_t* O, double f, const double* vin,
+                    double* vout) {
+  if (La->cols != Vb->rows || Vb->cols != Rb->rows || La->rows != O->rows || Rb->cols != O->cols) return ERR_LAYOUT;
+  int64_t m = La->rows, k = La->cols, n = Vb->cols, n2 = Rb->cols;
+  double* t1 = (double*)xrealloc(NULL, sizeof(double) * (m * n + 1));
+  double* t2 = (double*)xrealloc(NULL, sizeof(double) * (m * n2 + 1));
+  dgemm_nn(m, n, k, La->data, m, vin + Vb->off, k, t1, m);
+  dgemm_nn(m, n2, n, t1, m, Rb->data, n, t2, m);
+  double* o = vout + O->off;
+  for (int64_t x = 0; x < m * n2; ++x) o[x] += f * t2[x];
+  free(t1);
+  free(t2);
+  return OK;
+}
+
+/* One-site OxV (pivot/DmrgPivotMatrix1.h:218-262) for lhs entries [l0, l1):
+ * Vout[s1][q1] += f * W(a,b) * L[qL][a] . Vin[s2][q2] . R[qR][b] */
+static int oxv1_range(const opivot_t* P, const double* vin, double* vout, int64_t l0, int64_t l1) {
+  for (int64_t q = l0; q < l1; ++q) {
+    int64_t s1 = P->qlhs[2 * q], q1 = P->qlhs[2 * q + 1];
+    if (s1 < 0 || s1 >= P->ns || q1 < 0 || q1 >= P->V[s1].n) return ERR_LAYOUT;
+    const vblk_t* O = &P->V[s1].b[q1];
+    for (int64_t p = P->rhs_begin[q]; p < P->rhs_begin[q + 1]; ++p) {
+      const int64_t* e = &P->qrhs[12 * p];
+      int64_t s2 = e[0], q2 = e[1], qL = e[2], qR = e[3], k = e[4], qW = e[5];
+      if (s2 < 0 || s2 >= P->ns || q2 < 0 || q2 >= P->V[s2].n || qL < 0 || qL >= P->L.n || qR < 0 || qR >= P->R.n ||
+          k < 0 || k >= P->nop12 || qW < 0 || qW >= W12AT(P, s1, s2, k)->n)
+        return ERR_LAYOUT;
+      const wblk_t* w = &W12AT(P, s1, s2, k)->b[qW];
+      const vblk_t* Vb = &P->V[s2].b[q2];
+      const env_t* Le = &P->L.s[qL];
+      const env_t* Re = &P->R.s[qR];
+      for (int64_t i = 0; i < w->nnz; ++i) {
+        if (w->row[i] >= Le->adim || w->col[i] >= Re->adim) return ERR_LAYOUT;
+        const mat_t* La = &Le->blk[w->row[i]];
+        const mat_t* Rb = &Re->blk[w->col[i]];
+        if (!(La->rows * La->cols > 0 && Rb->rows * Rb->cols > 0 && Vb->rows * Vb->cols > 0)) continue;
+        int st = term_lvr(La, Vb, Rb, O, P->fac[p] * w->val[i], vin, vout);
+        if (st) return st;
+      }
+    }
+  }
+  return OK;
+}
+
+/* Zero-site HxV (pivot/DmrgPivotMatrix0.h:31-96):
+ * Vout[0](L.in, L.in) += sum_a L[qL][a] . Vin[0](L.out, L.out) . R[{L.out, L.in, L.mid}][a] */
+static int oxv0(const opivot_t* P, const double* vin, double* vout) {
+  for (int qL = 0; qL < P->L.n; ++qL) {
+    const env_t* Le = &P->L.s[qL];
+    int qR = efind(&P->R, Le->out, Le->in, Le->mid);
+    if (qR < 0) continue;
+    int qi = vfind(&P->V[0], Le->out, Le->out);
+    if (qi < 0) continue;
+    const env_t* Re = &P->R.s[qR];
+    int qo = vfind(&P->V[0], Re->out, Re->out);
+    if (qo < 0) continue;
+    if (Le->adim != Re->adim) return ERR_LAYOUT;
+    const vblk_t* Vb = &P->V[0].b[qi];
+    if (Vb->rows * Vb->cols == 0) continue;
+    for (int a = 0; a < Le->adim; ++a) {
+      const mat_t* La = &Le->blk[a];
+      const mat_t* Rb = &Re->blk[a];
+      if (!(La->rows * La->cols > 0 && Rb->rows * Rb->cols > 0)) continue;
+      int st = term_lvr(La, Vb, Rb, &P->V[0].b[qo], 1.0, vin, vout);
+      if (st) return st;
+    }
+  }
+  return OK;
+}
+
+static int oxv_any(const opivot_t* P, const double* vin, double* vout, int64_t l0, int64_t l1) {
+  if (P->nsite == 1) return oxv1_range(P, vin, vout, l0, l1);
+  return oxv_range(P, vin, vout, l0, l1);
+}
+
 static int finish(const opivot_t* P, const double* vin, double* vout) {
   for (int n = 0; n < P->nproj; ++n) {
     const double* a0 = P->A0 + (size_t)n * P->vsize;
@@ -977,7 +1176,7 @@ int oracle_hxv(const void* h, const double* vin, double* vout) {
   const opivot_t* P = (const opivot_t*)h;
   if (!P->has_bs) return fail(ERR_STATE, "no block structure");
   memset(vout, 0, sizeof(double) * P->vsize);
-  int st = oxv_range(P, vin, vout, 0, P->nlhs);
+  int st = P->nsite == 0 ? oxv0(P, vin, vout) : oxv_any(P, vin, vout, 0, P->nlhs);
   if (st) return fail(st, "layout mismatch in OxV");
   return finish(P, vin, vout);
 }
@@ -987,12 +1186,13 @@ int oracle_hxv(const void* h, const double* vin, double* vout) {
 int oracle_hxv_omp(const void* h, const double* vin, double* vout, int nthreads) {
   const opivot_t* P = (const opivot_t*)h;
   if (!P->has_bs) return fail(ERR_STATE, "no block structure");
+  if (P->nsite == 0) return oracle_hxv(h, vin, vout);
   memset(vout, 0, sizeof(double) * P->vsize);
   int bad = 0;
 #ifdef _OPENMP
 #pragma omp parallel for schedule(dynamic) num_threads(nthreads) reduction(| : bad)
 #endif
-  for (int64_t q = 0; q < P->nlhs; ++q) bad |= oxv_range(P, vin, vout, q, q + 1);
+  for (int64_t q = 0; q < P->nlhs; ++q) bad |= oxv_any(P, vin, vout, q, q + 1);
   (void)nthreads;
   if (bad) return fail(ERR_LAYOUT, "layout mismatch in OxV");
   return finish(P, vin, vout);
@@ -1002,6 +1202,7 @@ int oracle_hxv_omp(const void* h, const double* vin, double* vout, int nthreads)
 int oracle_flops_ref(const void* h, double* flops) {
   const opivot_t* P = (const opivot_t*)h;
   double f = 0.0;
+  if (P->nsite != 2) return fail(ERR_STATE, "flop count implemented for two-site pivots");
   for (int64_t q = 0; q < P->nlhs; ++q)
     for (int64_t p = P->rhs_begin[q]; p < P->rhs_begin[q + 1]; ++p) {
       const int64_t* e = &P->qrhs[12 * p];

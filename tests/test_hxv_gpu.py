@@ -159,3 +159,36 @@ def test_stats_consistent(hub):
     oracle.check(lib.oracle_flops_ref(h, ctypes.byref(f)))
     lib.oracle_pivot2_destroy(h)
     assert st["flops_ref"] == pytest.approx(f.value, rel=1e-12)
+
+
+_CLASS = {0: "PivotMatrix0", 1: "PivotMatrix1", 2: "PivotMatrix2"}
+
+
+def _matrix(pd, **kw):
+    from vmps_amd import pivot
+    return getattr(pivot, _CLASS[pd.nsite])(pd, **kw)
+
+
+@pytest.mark.parametrize("kind", ["heis1", "hub1", "heis0", "hub0"])
+@pytest.mark.parametrize("precision,tol", [("fp64", FP64_RTOL), ("bf16x3", BF16X3_RTOL)])
+def test_one_and_zero_site_match_oracle(kind, precision, tol):
+    """PivotMatrix1 / PivotMatrix0 HxV on the device vs the oracle's literal
+    one-site OxV and zero-site HxV."""
+    pd = pivots.ALL_KINDS[kind]()
+    print("case", kind, precision, "vsize", pd.vsize, flush=True)
+    H = _matrix(pd, precision=precision)
+    for seed in (1, 2):
+        v = synth.random_vector(pd, seed)
+        assert _rel(HxV(H, v), _oracle_out(pd, v)) <= tol
+
+
+@pytest.mark.parametrize("kind", ["heis2", "hub2", "hub1"])
+def test_sum_of_terms_matches_oracle(kind):
+    """Pivots with three terms (PivotMatrix2::Terms / PivotMatrix1::Terms):
+    device result == sum of the oracle's per-term results."""
+    pd = pivots.with_terms(pivots.ALL_KINDS[kind]())
+    print("case", kind, "terms", 1 + len(pd.extra), flush=True)
+    H = _matrix(pd)
+    v = synth.random_vector(pd, 3)
+    ref = pivots.oracle_hxv_terms(oracle.load(), pd, v)
+    assert _rel(HxV(H, v), ref) <= FP64_RTOL

@@ -86,3 +86,64 @@ def test_local_basis_agrees():
         assert all(tuple(py[i]) <= tuple(py[i + 1]) for i in range(len(py) - 1))
         olib.oracle_pivot2_destroy(ho)
         vlib.vmps_pivot2_destroy(hv)
+
+
+@pytest.mark.parametrize("kind", ["heis1", "hub1", "heis0", "hub0"])
+@pytest.mark.parametrize("threads", [0, 3])
+def test_oracle_one_and_zero_site_match_dense(kind, threads):
+    """One-site OxV (DmrgPivotMatrix1.h) and zero-site HxV (DmrgPivotMatrix0.h)
+    of the oracle == dense contraction."""
+    pd = pivots.ALL_KINDS[kind]()
+    lib = oracle.load()
+    h = pivots.build_oracle(lib, pd)
+    for seed in (1, 2):
+        v = synth.random_vector(pd, seed)
+        out = pivots.oracle_hxv(lib, h, v, nthreads=threads)
+        ref = dense.dense_hxv(pd, v)
+        assert np.linalg.norm(ref) > 0
+        assert np.linalg.norm(out - ref) <= 1e-13 * np.linalg.norm(ref)
+    lib.oracle_pivot2_destroy(h)
+
+
+@pytest.mark.parametrize("kind", ["heis2", "hub1"])
+def test_oracle_sum_of_terms_matches_dense(kind):
+    """A pivot with three terms: the sum of the oracle's per-term results ==
+    the dense sum over terms."""
+    pd = pivots.with_terms(pivots.ALL_KINDS[kind]())
+    v = synth.random_vector(pd, 4)
+    out = pivots.oracle_hxv_terms(oracle.load(), pd, v)
+    ref = dense.dense_hxv(pd, v)
+    assert np.linalg.norm(out - ref) <= 1e-13 * np.linalg.norm(ref)
+
+
+@pytest.mark.parametrize("kind", ["heis1", "hub1"])
+def test_engine_one_site_blockstructure_equals_oracle(kind):
+    pd = pivots.ALL_KINDS[kind]()
+    olib = oracle.load()
+    vlib = _lib.load()
+    ho = pivots.build_oracle(olib, pd)
+    hv = synth.feed(vlib, "vmps_", pd, _lib.check)
+    _lib.check(vlib.vmps_pivot2_precalc(hv))
+    bo = get_blockstructure(olib, ho, prefix="oracle_")
+    bv = get_blockstructure(vlib, hv)
+    assert len(bo[0]) > 0
+    for a, b in zip(bo, bv):
+        assert np.array_equal(a, b)
+    olib.oracle_pivot2_destroy(ho)
+    vlib.vmps_pivot2_destroy(hv)
+
+
+def test_engine_pivot_terms_and_kinds():
+    """Builder bookkeeping of the engine: pivot kind and term count."""
+    vlib = _lib.load()
+    for kind, ns in (("heis2", 2), ("hub1", 1), ("hub0", 0)):
+        pd = pivots.ALL_KINDS[kind]()
+        if ns:
+            pivots.with_terms(pd, 2)
+        hv = synth.feed(vlib, "vmps_", pd, _lib.check)
+        nsite, nterms = ctypes.c_int(), ctypes.c_int()
+        _lib.check(vlib.vmps_pivot_info(hv, ctypes.byref(nsite), ctypes.byref(nterms)))
+        assert (nsite.value, nterms.value) == (ns, 3 if ns else 1)
+        assert vlib.vmps_pivot_select_term(hv, nterms.value) == 1  # VMPS_ERR_ARG
+        _lib.check(vlib.vmps_pivot2_precalc(hv))
+        vlib.vmps_pivot2_destroy(hv)

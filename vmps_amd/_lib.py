@@ -67,6 +67,11 @@ class VmpsStatusError(RuntimeError):
 SIGNATURES = {
     "vmps_abi_version": (c_int, []),
     "vmps_last_error": (ctypes.c_char_p, []),
+    "vmps_pivot1_create": (c_int, [c_int, P_int, c_int, P_int, c_int, P_int, ctypes.POINTER(c_void)]),
+    "vmps_pivot0_create": (c_int, [c_int, P_int, ctypes.POINTER(c_void)]),
+    "vmps_pivot_add_term": (c_int, [c_void, c_int, P_int, c_int, P_int, P_int]),
+    "vmps_pivot_select_term": (c_int, [c_void, c_int]),
+    "vmps_pivot_info": (c_int, [c_void, P_int, P_int]),
     "vmps_pivot2_create": (c_int, [c_int, P_int, c_int, P_int, c_int, P_int, c_int, P_int, c_int, P_int,
                                    ctypes.POINTER(c_void)]),
     "vmps_pivot2_destroy": (None, [c_void]),

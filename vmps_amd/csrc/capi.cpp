@@ -16,7 +16,7 @@
 #include "vmps_hxv.h"
 
 struct vmps_pivot2 {
-  vmps::Pivot2 P;
+  vmps::Pivot P;
   int ns = 0;
 };
 struct vmps_plan {
@@ -59,6 +59,36 @@ vmps::Qn read_qn(const int* q, int nq) {
 void require(bool cond, int status, const char* msg) {
   if (!cond) throw VmpsError(status, msg);
 }
+
+std::vector<vmps::Qn> read_qns(const int* q, int n, int nq) {
+  std::vector<vmps::Qn> v;
+  for (int i = 0; i < n; ++i) v.push_back(read_qn(q + i * nq, nq));
+  return v;
+}
+
+void set_sym(vmps::Sym& sym, int nq, const int* kinds) {
+  require(nq >= 1 && nq <= vmps::kMaxNq, VMPS_ERR_ARG, "nq must be 1 or 2");
+  require(kinds != nullptr, VMPS_ERR_ARG, "kinds is NULL");
+  sym.nq = nq;
+  for (int i = 0; i < nq; ++i) {
+    require(kinds[i] == VMPS_SYM_U1 || kinds[i] == VMPS_SYM_SU2, VMPS_ERR_ARG, "unknown symmetry kind");
+    sym.kind[i] = kinds[i];
+  }
+}
+
+void require_su2_positive(const vmps::Sym& sym, const std::vector<vmps::Qn>& qs) {
+  for (int i = 0; i < sym.nq; ++i)
+    if (sym.kind[i] == vmps::kSU2)
+      for (auto& q : qs) require(q[i] >= 1, VMPS_ERR_ARG, "SU(2) quantum numbers are q = 2S+1 >= 1");
+}
+
+void init_vector_layout(vmps_pivot2* p, int ns) {
+  p->ns = ns;
+  p->P.V.in.resize(ns);
+  p->P.V.out.resize(ns);
+  p->P.V.block.resize(ns);
+  p->P.V.dict.resize(ns);
+}
 }  // namespace
 
 extern "C" {
@@ -71,34 +101,86 @@ vmps_status vmps_pivot2_create(int nq, const int* kinds, int nloc12, const int* 
   VMPS_API_BEGIN
   require(out != nullptr, VMPS_ERR_ARG, "out is NULL");
   *out = nullptr;
-  require(nq >= 1 && nq <= vmps::kMaxNq, VMPS_ERR_ARG, "nq must be 1 or 2");
-  require(kinds != nullptr, VMPS_ERR_ARG, "kinds is NULL");
   require(nloc12 > 0 && nloc34 > 0 && nop12 > 0 && nop34 > 0, VMPS_ERR_ARG, "empty local or operator basis");
   require(qloc12 && qloc34 && qop12 && qop34, VMPS_ERR_ARG, "NULL basis array");
   auto p = std::make_unique<vmps_pivot2>();
-  p->P.sym.nq = nq;
-  for (int i = 0; i < nq; ++i) {
-    require(kinds[i] == VMPS_SYM_U1 || kinds[i] == VMPS_SYM_SU2, VMPS_ERR_ARG, "unknown symmetry kind");
-    p->P.sym.kind[i] = kinds[i];
-  }
-  for (int i = 0; i < nloc12; ++i) p->P.qloc12.push_back(read_qn(qloc12 + i * nq, nq));
-  for (int i = 0; i < nloc34; ++i) p->P.qloc34.push_back(read_qn(qloc34 + i * nq, nq));
-  for (int i = 0; i < nop12; ++i) p->P.qop12.push_back(read_qn(qop12 + i * nq, nq));
-  for (int i = 0; i < nop34; ++i) p->P.qop34.push_back(read_qn(qop34 + i * nq, nq));
-  for (int i = 0; i < nq; ++i)
-    if (p->P.sym.kind[i] == vmps::kSU2) {
-      for (auto* v : {&p->P.qloc12, &p->P.qloc34, &p->P.qop12, &p->P.qop34})
-        for (auto& q : *v) require(q[i] >= 1, VMPS_ERR_ARG, "SU(2) quantum numbers are q = 2S+1 >= 1");
-    }
-  p->P.init_W();
+  set_sym(p->P.sym, nq, kinds);
+  p->P.nsite = 2;
+  p->P.qloc12 = read_qns(qloc12, nloc12, nq);
+  p->P.qloc34 = read_qns(qloc34, nloc34, nq);
+  auto o12 = read_qns(qop12, nop12, nq), o34 = read_qns(qop34, nop34, nq);
+  for (auto* v : {&p->P.qloc12, &p->P.qloc34, &o12, &o34}) require_su2_positive(p->P.sym, *v);
+  p->P.add_term(o12, o34);
   vmps::TwoSiteBasis tb;
   tb.build(p->P.sym, p->P.qloc12, p->P.qloc34);
-  p->ns = (int)tb.qloc.size();
-  p->P.V.in.resize(p->ns);
-  p->P.V.out.resize(p->ns);
-  p->P.V.block.resize(p->ns);
-  p->P.V.dict.resize(p->ns);
+  init_vector_layout(p.get(), (int)tb.qloc.size());
   *out = p.release();
+  VMPS_API_END
+}
+
+vmps_status vmps_pivot1_create(int nq, const int* kinds, int nloc, const int* qloc, int nop, const int* qop,
+                               vmps_pivot2** out) {
+  VMPS_API_BEGIN
+  require(out != nullptr, VMPS_ERR_ARG, "out is NULL");
+  *out = nullptr;
+  require(nloc > 0 && nop > 0, VMPS_ERR_ARG, "empty local or operator basis");
+  require(qloc && qop, VMPS_ERR_ARG, "NULL basis array");
+  auto p = std::make_unique<vmps_pivot2>();
+  set_sym(p->P.sym, nq, kinds);
+  p->P.nsite = 1;
+  p->P.qloc12 = read_qns(qloc, nloc, nq);
+  auto o = read_qns(qop, nop, nq);
+  require_su2_positive(p->P.sym, p->P.qloc12);
+  require_su2_positive(p->P.sym, o);
+  p->P.add_term(o, {});
+  init_vector_layout(p.get(), nloc);
+  *out = p.release();
+  VMPS_API_END
+}
+
+vmps_status vmps_pivot0_create(int nq, const int* kinds, vmps_pivot2** out) {
+  VMPS_API_BEGIN
+  require(out != nullptr, VMPS_ERR_ARG, "out is NULL");
+  *out = nullptr;
+  auto p = std::make_unique<vmps_pivot2>();
+  set_sym(p->P.sym, nq, kinds);
+  p->P.nsite = 0;
+  p->P.add_term({}, {});
+  init_vector_layout(p.get(), 1);
+  *out = p.release();
+  VMPS_API_END
+}
+
+vmps_status vmps_pivot_add_term(vmps_pivot2* p, int nop12, const int* qop12, int nop34, const int* qop34,
+                                int* term) {
+  VMPS_API_BEGIN
+  require(p != nullptr, VMPS_ERR_ARG, "NULL pivot");
+  const int nq = p->P.sym.nq;
+  const int ns = p->P.nsite;
+  require(ns < 1 || (nop12 > 0 && qop12), VMPS_ERR_ARG, "empty operator basis");
+  require(ns < 2 || (nop34 > 0 && qop34), VMPS_ERR_ARG, "empty operator basis");
+  auto o12 = ns >= 1 ? read_qns(qop12, nop12, nq) : std::vector<vmps::Qn>{};
+  auto o34 = ns == 2 ? read_qns(qop34, nop34, nq) : std::vector<vmps::Qn>{};
+  require_su2_positive(p->P.sym, o12);
+  require_su2_positive(p->P.sym, o34);
+  size_t t = p->P.add_term(o12, o34);
+  if (term) *term = (int)t;
+  VMPS_API_END
+}
+
+vmps_status vmps_pivot_select_term(vmps_pivot2* p, int term) {
+  VMPS_API_BEGIN
+  require(p != nullptr, VMPS_ERR_ARG, "NULL pivot");
+  require(term >= 0 && (size_t)term < p->P.terms.size(), VMPS_ERR_ARG, "term index out of range");
+  p->P.cur = (size_t)term;
+  VMPS_API_END
+}
+
+vmps_status vmps_pivot_info(const vmps_pivot2* p, int* nsite, int* nterms) {
+  VMPS_API_BEGIN
+  require(p != nullptr, VMPS_ERR_ARG, "NULL pivot");
+  if (nsite) *nsite = p->P.nsite;
+  if (nterms) *nterms = (int)p->P.terms.size();
   VMPS_API_END
 }
 
@@ -109,10 +191,20 @@ vmps_status vmps_pivot2_local_basis(const vmps_pivot2* p, int* ns, int* qloc) {
   require(p && ns, VMPS_ERR_ARG, "NULL argument");
   *ns = p->ns;
   if (qloc) {
-    vmps::TwoSiteBasis tb;
-    tb.build(p->P.sym, p->P.qloc12, p->P.qloc34);
+    std::vector<vmps::Qn> q;
+    if (p->P.nsite == 2) {
+      vmps::TwoSiteBasis tb;
+      tb.build(p->P.sym, p->P.qloc12, p->P.qloc34);
+      q = tb.qloc;
+    } else if (p->P.nsite == 1) {
+      q = p->P.qloc12;
+    } else {
+      vmps::Qn vac{};
+      for (int i = 0; i < p->P.sym.nq; ++i) vac[i] = p->P.sym.kind[i] == vmps::kSU2 ? 1 : 0;
+      q.assign(1, vac);
+    }
     for (int s = 0; s < p->ns; ++s)
-      for (int i = 0; i < p->P.sym.nq; ++i) qloc[s * p->P.sym.nq + i] = tb.qloc[s][i];
+      for (int i = 0; i < p->P.sym.nq; ++i) qloc[s * p->P.sym.nq + i] = q[s][i];
   }
   VMPS_API_END
 }
@@ -125,7 +217,7 @@ vmps_status vmps_pivot2_add_env(vmps_pivot2* p, int which, const int* qin, const
   require(adim >= 0, VMPS_ERR_ARG, "adim < 0");
   require(adim == 0 || (rows && cols), VMPS_ERR_ARG, "NULL rows/cols");
   const int nq = p->P.sym.nq;
-  vmps::Tripod& E = which == 0 ? p->P.L : p->P.R;
+  vmps::Tripod& E = which == 0 ? p->P.term().L : p->P.term().R;
   vmps::Key3 key{read_qn(qin, nq), read_qn(qout, nq), read_qn(qmid, nq)};
   require(!E.dict.count(key), VMPS_ERR_LAYOUT, "duplicate environment sector");
   int64_t total = 0;
@@ -159,9 +251,11 @@ vmps_status vmps_pivot2_add_W(vmps_pivot2* p, int which, int s1, int s2, int k, 
   require(p && qin && qout, VMPS_ERR_ARG, "NULL argument");
   require(which == 12 || which == 34, VMPS_ERR_ARG, "which must be 12 or 34");
   require(nnz >= 0 && (nnz == 0 || (row && col && val)), VMPS_ERR_ARG, "bad nonzero arrays");
-  auto& W = which == 12 ? p->P.W12 : p->P.W34;
+  require(p->P.nsite == 2 || (p->P.nsite == 1 && which == 12), VMPS_ERR_ARG,
+          "W slot 34 exists only for two-site pivots, zero-site pivots have no W");
+  auto& W = which == 12 ? p->P.term().W12 : p->P.term().W34;
   const size_t nloc = which == 12 ? p->P.qloc12.size() : p->P.qloc34.size();
-  const size_t nop = which == 12 ? p->P.qop12.size() : p->P.qop34.size();
+  const size_t nop = which == 12 ? p->P.term().qop12.size() : p->P.term().qop34.size();
   require(s1 >= 0 && (size_t)s1 < nloc && s2 >= 0 && (size_t)s2 < nloc && k >= 0 && (size_t)k < nop, VMPS_ERR_ARG,
           "W index out of range");
   require(rows >= 0 && cols >= 0, VMPS_ERR_ARG, "negative W block dimension");
@@ -204,7 +298,7 @@ vmps_status vmps_pivot2_add_V_block(vmps_pivot2* p, int s, const int* qin, const
   const int nq = p->P.sym.nq;
   int q = p->P.add_V_block(s, read_qn(qin, nq), read_qn(qout, nq), rows, cols);
   if (offset) *offset = p->P.V.block[s][q].off;
-  p->P.has_blockstructure = false;
+  p->P.invalidate_blockstructure();
   VMPS_API_END
 }
 
@@ -229,7 +323,7 @@ vmps_status vmps_pivot2_set_blockstructure(vmps_pivot2* p, int64_t nlhs, const i
   require(nlhs == 0 || (qlhs && rhs_begin), VMPS_ERR_ARG, "NULL table");
   const int64_t nrhs = nlhs ? rhs_begin[nlhs] : 0;
   require(nrhs == 0 || (qrhs && factors), VMPS_ERR_ARG, "NULL table");
-  auto& P = p->P;
+  auto& P = p->P.term();
   P.qlhs.assign(nlhs, {0, 0});
   P.qrhs.assign(nlhs, {});
   P.factor_cgcs.assign(nlhs, {});
@@ -255,8 +349,8 @@ vmps_status vmps_pivot2_get_blockstructure(const vmps_pivot2* p, int64_t* nlhs, 
                                            int64_t* rhs_begin, int64_t* qrhs, double* factors) {
   VMPS_API_BEGIN
   require(p && nlhs && nrhs, VMPS_ERR_ARG, "NULL argument");
-  require(p->P.has_blockstructure, VMPS_ERR_STATE, "no block structure");
-  const auto& P = p->P;
+  require(p->P.term().has_blockstructure, VMPS_ERR_STATE, "no block structure");
+  const auto& P = p->P.term();
   *nlhs = (int64_t)P.qlhs.size();
   int64_t n = 0;
   for (auto& v : P.qrhs) n += (int64_t)v.size();

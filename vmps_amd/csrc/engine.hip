@@ -267,7 +267,7 @@ DevPlan::~DevPlan() {
   if (ev1) (void)hipEventDestroy(ev1);
 }
 
-void DevPlan::create(const Pivot2& P, vmps_precision precision) {
+void DevPlan::create(const Pivot& P, vmps_precision precision) {
   build_host_plan(P, H);
   upload(precision);
   nproj = (int)P.A0proj.size();

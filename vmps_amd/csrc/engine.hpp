@@ -97,7 +97,7 @@ struct DevPlan {
   DevBuf<double> A0;
 
   ~DevPlan();
-  void create(const Pivot2& P, vmps_precision prec);
+  void create(const Pivot& P, vmps_precision prec);
   void create_triples(const TripleSet& S, vmps_precision prec);
   void upload(vmps_precision prec);  // device copies of H
   void ensure_precision_buffers();

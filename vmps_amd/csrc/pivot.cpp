@@ -1,5 +1,5 @@
-// Two-site pivot: combined local basis, block-structure precalculation,
-// per-term expansion and the staged plan; see pivot.hpp.
+// Pivots (zero-, one- and two-site): combined local basis, block-structure
+// precalculation, per-term expansion and the staged plan; see pivot.hpp.
 #include "pivot.hpp"
 
 #include <algorithm>
@@ -118,12 +118,20 @@ int64_t TwoSiteBasis::index(int s1, int s3, const Qn& qmerge) const {
 }
 
 // ---------------------------------------------------------------------------
-void Pivot2::init_W() {
-  W12.assign(qloc12.size(), std::vector<std::vector<WBiped>>(qloc12.size(), std::vector<WBiped>(qop12.size())));
-  W34.assign(qloc34.size(), std::vector<std::vector<WBiped>>(qloc34.size(), std::vector<WBiped>(qop34.size())));
+size_t Pivot::add_term(const std::vector<Qn>& qop12, const std::vector<Qn>& qop34) {
+  PivotTerm t;
+  t.qop12 = qop12;
+  t.qop34 = qop34;
+  if (nsite >= 1)
+    t.W12.assign(qloc12.size(), std::vector<std::vector<WBiped>>(qloc12.size(), std::vector<WBiped>(qop12.size())));
+  if (nsite == 2)
+    t.W34.assign(qloc34.size(), std::vector<std::vector<WBiped>>(qloc34.size(), std::vector<WBiped>(qop34.size())));
+  terms.push_back(std::move(t));
+  cur = terms.size() - 1;
+  return cur;
 }
 
-int Pivot2::add_V_block(int s, const Qn& qin, const Qn& qout, int64_t rows, int64_t cols) {
+int Pivot::add_V_block(int s, const Qn& qin, const Qn& qout, int64_t rows, int64_t cols) {
   if (s < 0 || s >= (int)V.in.size()) throw VmpsError(VMPS_ERR_ARG, "V block: s out of range");
   if (rows < 0 || cols < 0) throw VmpsError(VMPS_ERR_ARG, "V block: negative dimension");
   Key2 key{qin, qout};
@@ -179,7 +187,100 @@ bool AAWWAA(const Sym& sym, const Qn& Lin, const Qn& Lout, const Qn& Lmid, const
 }
 }  // namespace
 
-void precalc_blockStructure(Pivot2& P) {
+namespace {
+// LAWA, DmrgIndexGymnastics.h:32-77, on the wavefunction layout (Abra = Aket = V):
+// right sectors (Rin, Rout, Rmid) reachable from the left sector (Lin, Lout, Lmid)
+// through Vout[s1] (Lin, Rout), Vin[s2] (Lout, Rin) and W[s1][s2][k] (Lmid, Rmid).
+struct LawaHit {
+  Qn qRin, qRout, qRmid;
+  size_t q1, q2, qW;
+};
+
+void LAWA_V(const Sym& sym, const Qn& Lin, const Qn& Lout, const Qn& Lmid, const Qn& qloc1, const Qn& qloc2,
+            const Qn& qOp, const VLayout& V, size_t s1, size_t s2, const WBiped& W, std::vector<LawaHit>& result) {
+  result.clear();
+  for (const Qn& Rout : sym.reduce(Lin, qloc1)) {
+    auto q1 = V.dict[s1].find(Key2{Lin, Rout});
+    if (q1 == V.dict[s1].end()) continue;
+    for (const Qn& Rin : sym.reduce(Lout, qloc2)) {
+      auto q2 = V.dict[s2].find(Key2{Lout, Rin});
+      if (q2 == V.dict[s2].end()) continue;
+      for (const Qn& Rmid : sym.reduce(Lmid, qOp)) {
+        auto qW = W.dict.find(Key2{Lmid, Rmid});
+        if (qW == W.dict.end()) continue;
+        if (sym.validate3(Rin, Rmid, Rout)) result.push_back({Rin, Rout, Rmid, q1->second, q2->second, qW->second});
+      }
+    }
+  }
+}
+
+// One-site precalc_blockStructure, DmrgIndexGymnastics.h:485-590.
+void precalc_one_site(Pivot& P, PivotTerm& T) {
+  const Sym& sym = P.sym;
+  const bool NA = sym.non_abelian();
+  if (P.V.in.size() != P.qloc12.size())
+    throw VmpsError(VMPS_ERR_LAYOUT, "PivotVector has " + std::to_string(P.V.in.size()) +
+                                         " components, the local basis has " + std::to_string(P.qloc12.size()));
+  using Key = std::array<size_t, 2>;
+  std::map<Key, std::pair<std::vector<std::array<size_t, 12>>, std::vector<double>>> lookup;
+  std::vector<LawaHit> ix;
+  for (size_t s1 = 0; s1 < P.qloc12.size(); ++s1)
+    for (size_t s2 = 0; s2 < P.qloc12.size(); ++s2)
+      for (size_t k = 0; k < T.qop12.size(); ++k) {
+        if (!sym.validate3(P.qloc12[s2], T.qop12[k], P.qloc12[s1])) continue;
+        for (size_t qL = 0; qL < T.L.dim(); ++qL) {
+          LAWA_V(sym, T.L.in(qL), T.L.out(qL), T.L.mid(qL), P.qloc12[s1], P.qloc12[s2], T.qop12[k], P.V, s1, s2,
+                 T.W12[s1][s2][k], ix);
+          for (const LawaHit& h : ix) {
+            auto qR = T.R.dict.find(Key3{h.qRin, h.qRout, h.qRmid});
+            if (qR == T.R.dict.end()) continue;
+            // skip if no nonzero of W connects a present L block with a present R block
+            const SparseBlock& w = T.W12[s1][s2][k].block[h.qW];
+            bool any = false;
+            for (size_t i = 0; i < w.val.size() && !any; ++i) {
+              if (w.row[i] >= (int)T.L.block[qL].size() || w.col[i] >= (int)T.R.block[qR->second].size())
+                throw VmpsError(VMPS_ERR_LAYOUT, "MPO index outside the environment's mid sector");
+              any = T.L.block[qL][w.row[i]].present() && T.R.block[qR->second][w.col[i]].present();
+            }
+            if (!any) continue;
+            double f = NA ? sym.coeff_HPsi(P.V.in[s2][h.q2], P.qloc12[s2], P.V.out[s2][h.q2], T.L.mid(qL), T.qop12[k],
+                                           h.qRmid, P.V.in[s1][h.q1], P.qloc12[s1], P.V.out[s1][h.q1])
+                          : 1.0;
+            if (std::abs(f) < kCgcEps) continue;
+            auto& e = lookup[Key{s1, h.q1}];
+            e.first.push_back({s2, h.q2, qL, qR->second, k, h.qW, 0, 0, 0, 0, 0, 0});
+            e.second.push_back(f);
+          }
+        }
+      }
+  T.qlhs.clear();
+  T.qrhs.clear();
+  T.factor_cgcs.clear();
+  for (auto& kv : lookup) {
+    T.qlhs.push_back(kv.first);
+    T.qrhs.push_back(std::move(kv.second.first));
+    T.factor_cgcs.push_back(std::move(kv.second.second));
+  }
+  T.has_blockstructure = true;
+}
+}  // namespace
+
+void precalc_blockStructure(Pivot& P) {
+  for (auto& t : P.terms) precalc_blockStructure(P, t);
+}
+
+void precalc_blockStructure(Pivot& P, PivotTerm& T) {
+  if (P.nsite == 0) {
+    T.qlhs.clear();
+    T.qrhs.clear();
+    T.factor_cgcs.clear();
+    T.has_blockstructure = true;
+    return;
+  }
+  if (P.nsite == 1) {
+    precalc_one_site(P, T);
+    return;
+  }
   const Sym& sym = P.sym;
   const bool NA = sym.non_abelian();
   TwoSiteBasis tb;
@@ -193,15 +294,17 @@ void precalc_blockStructure(Pivot2& P) {
   std::map<Key, std::pair<std::vector<std::array<size_t, 12>>, std::vector<double>>> lookup;
   std::vector<AawwaaHit> ixs;
 
+  const auto& qop12 = T.qop12;
+  const auto& qop34 = T.qop34;
   for (size_t s1 = 0; s1 < P.qloc12.size(); ++s1)
     for (size_t s2 = 0; s2 < P.qloc12.size(); ++s2)
-      for (size_t k12 = 0; k12 < P.qop12.size(); ++k12) {
-        if (!sym.validate3(P.qloc12[s2], P.qop12[k12], P.qloc12[s1])) continue;
+      for (size_t k12 = 0; k12 < qop12.size(); ++k12) {
+        if (!sym.validate3(P.qloc12[s2], qop12[k12], P.qloc12[s1])) continue;
         for (size_t s3 = 0; s3 < P.qloc34.size(); ++s3)
           for (size_t s4 = 0; s4 < P.qloc34.size(); ++s4)
-            for (size_t k34 = 0; k34 < P.qop34.size(); ++k34) {
-              if (!sym.validate3(P.qloc34[s4], P.qop34[k34], P.qloc34[s3])) continue;
-              auto qOps = sym.reduce(P.qop12[k12], P.qop34[k34]);
+            for (size_t k34 = 0; k34 < qop34.size(); ++k34) {
+              if (!sym.validate3(P.qloc34[s4], qop34[k34], P.qloc34[s3])) continue;
+              auto qOps = sym.reduce(qop12[k12], qop34[k34]);
               for (const Qn& qOp : qOps) {
                 auto qmerges13 = sym.reduce(P.qloc12[s1], P.qloc34[s3]);
                 auto qmerges24 = sym.reduce(P.qloc12[s2], P.qloc34[s4]);
@@ -210,24 +313,24 @@ void precalc_blockStructure(Pivot2& P) {
                     int64_t s1s3 = tb.index((int)s1, (int)s3, qmerge13);
                     int64_t s2s4 = tb.index((int)s2, (int)s4, qmerge24);
                     if (s1s3 < 0 || s2s4 < 0) continue;
-                    double cgc9 = NA ? sym.coeff_tensorProd(P.qloc12[s2], P.qloc34[s4], qmerge24, P.qop12[k12],
-                                                            P.qop34[k34], qOp, P.qloc12[s1], P.qloc34[s3], qmerge13)
+                    double cgc9 = NA ? sym.coeff_tensorProd(P.qloc12[s2], P.qloc34[s4], qmerge24, qop12[k12],
+                                                            qop34[k34], qOp, P.qloc12[s1], P.qloc34[s3], qmerge13)
                                      : 1.0;
                     if (std::abs(cgc9) < kCgcEps) continue;
-                    for (size_t qL = 0; qL < P.L.dim(); ++qL) {
-                      bool found = AAWWAA(sym, P.L.in(qL), P.L.out(qL), P.L.mid(qL), P.qop12[k12], P.qop34[k34],
-                                          qmerge13, qmerge24, P.V, (size_t)s1s3, (size_t)s2s4, P.W12[s1][s2][k12],
-                                          P.W34[s3][s4][k34], ixs);
+                    for (size_t qL = 0; qL < T.L.dim(); ++qL) {
+                      bool found = AAWWAA(sym, T.L.in(qL), T.L.out(qL), T.L.mid(qL), qop12[k12], qop34[k34],
+                                          qmerge13, qmerge24, P.V, (size_t)s1s3, (size_t)s2s4, T.W12[s1][s2][k12],
+                                          T.W34[s3][s4][k34], ixs);
                       if (!found) continue;
                       for (const auto& ix : ixs) {
-                        auto qR = P.R.dict.find(Key3{ix.qRin, ix.qRout, ix.qRmid});
-                        double cgc6 = NA ? sym.coeff_Apair(P.L.mid(qL), P.qop12[k12], ix.qW, P.qop34[k34], ix.qRmid, qOp)
+                        auto qR = T.R.dict.find(Key3{ix.qRin, ix.qRout, ix.qRmid});
+                        double cgc6 = NA ? sym.coeff_Apair(T.L.mid(qL), qop12[k12], ix.qW, qop34[k34], ix.qRmid, qOp)
                                          : 1.0;
                         if (std::abs(cgc6) < kCgcEps) continue;
-                        if (qR == P.R.dict.end()) continue;
+                        if (qR == T.R.dict.end()) continue;
                         double cgcH =
-                            NA ? sym.coeff_HPsi(P.V.in[s2s4][ix.q24], qmerge24, P.V.out[s2s4][ix.q24], P.L.mid(qL), qOp,
-                                                P.R.mid(qR->second), P.V.in[s1s3][ix.q13], qmerge13,
+                            NA ? sym.coeff_HPsi(P.V.in[s2s4][ix.q24], qmerge24, P.V.out[s2s4][ix.q24], T.L.mid(qL), qOp,
+                                                T.R.mid(qR->second), P.V.in[s1s3][ix.q13], qmerge13,
                                                 P.V.out[s1s3][ix.q13])
                                : 1.0;
                         Key key = {(size_t)s1s3, ix.q13};
@@ -242,23 +345,52 @@ void precalc_blockStructure(Pivot2& P) {
               }
             }
       }
-  P.qlhs.clear();
-  P.qrhs.clear();
-  P.factor_cgcs.clear();
+  T.qlhs.clear();
+  T.qrhs.clear();
+  T.factor_cgcs.clear();
   for (auto& kv : lookup) {
-    P.qlhs.push_back(kv.first);
-    P.qrhs.push_back(std::move(kv.second.first));
-    P.factor_cgcs.push_back(std::move(kv.second.second));
+    T.qlhs.push_back(kv.first);
+    T.qrhs.push_back(std::move(kv.second.first));
+    T.factor_cgcs.push_back(std::move(kv.second.second));
   }
-  P.has_blockstructure = true;
+  T.has_blockstructure = true;
 }
 
 // ---------------------------------------------------------------------------
-// Per-term expansion: the loop nest of OxV, DmrgPivotMatrix2.h:182-230.
+// Per-term expansion: the loop nests of OxV for two sites
+// (DmrgPivotMatrix2.h:182-230), one site (DmrgPivotMatrix1.h:218-262) and
+// zero sites (HxV, DmrgPivotMatrix0.h:31-96), over every term of the sum.
 // ---------------------------------------------------------------------------
-void expand_terms(const Pivot2& P, std::vector<Term>& terms, std::vector<std::pair<size_t, size_t>>& lblocks,
-                  std::vector<std::pair<size_t, size_t>>& rblocks, std::vector<std::pair<int, int>>& vblocks) {
-  if (!P.has_blockstructure) throw VmpsError(VMPS_ERR_STATE, "pivot has no block structure (call precalc first)");
+namespace {
+struct TermSink {
+  std::vector<Term>& terms;
+  std::vector<EnvRef>& lblocks;
+  std::vector<EnvRef>& rblocks;
+  std::map<EnvRef, int64_t> lid, rid;
+
+  static int64_t id(std::map<EnvRef, int64_t>& m, std::vector<EnvRef>& list, const EnvRef& k) {
+    auto it = m.find(k);
+    if (it == m.end()) {
+      it = m.emplace(k, (int64_t)list.size()).first;
+      list.push_back(k);
+    }
+    return it->second;
+  }
+  void add(int64_t out_v, const EnvRef& l, int64_t in_v, const EnvRef& r, double c) {
+    if (c == 0.0) return;  // exact zero contributes nothing
+    terms.push_back({out_v, id(lid, lblocks, l), in_v, id(rid, rblocks, r), c});
+  }
+};
+
+const DenseBlock& env_block(const Tripod& E, size_t q, int a) {
+  if (a < 0 || a >= (int)E.block[q].size())
+    throw VmpsError(VMPS_ERR_LAYOUT, "MPO index outside the environment's mid sector");
+  return E.block[q][a];
+}
+}  // namespace
+
+void expand_terms(const Pivot& P, std::vector<Term>& terms, std::vector<EnvRef>& lblocks,
+                  std::vector<EnvRef>& rblocks, std::vector<std::pair<int, int>>& vblocks) {
   // global ids of V blocks in packed (s,q) order
   std::vector<std::vector<int64_t>> vid(P.V.in.size());
   vblocks.clear();
@@ -269,49 +401,71 @@ void expand_terms(const Pivot2& P, std::vector<Term>& terms, std::vector<std::pa
       vblocks.push_back({(int)s, (int)q});
     }
   }
-  std::map<std::pair<size_t, size_t>, int64_t> lid, rid;
   lblocks.clear();
   rblocks.clear();
   terms.clear();
-  for (size_t q = 0; q < P.qlhs.size(); ++q) {
-    size_t s1s3 = P.qlhs[q][0], q13 = P.qlhs[q][1];
-    if (s1s3 >= P.V.in.size() || q13 >= P.V.in[s1s3].size())
-      throw VmpsError(VMPS_ERR_LAYOUT, "qlhs entry outside the vector layout");
-    for (size_t p = 0; p < P.qrhs[q].size(); ++p) {
-      const auto& e = P.qrhs[q][p];
-      size_t s2s4 = e[0], q24 = e[1], qL = e[2], qR = e[3], s1 = e[4], s2 = e[5], k12 = e[6], qW12 = e[7], s3 = e[8],
-             s4 = e[9], k34 = e[10], qW34 = e[11];
-      if (s2s4 >= P.V.in.size() || q24 >= P.V.in[s2s4].size() || qL >= P.L.dim() || qR >= P.R.dim() ||
-          s1 >= P.W12.size() || s2 >= P.W12[s1].size() || k12 >= P.W12[s1][s2].size() ||
-          qW12 >= P.W12[s1][s2][k12].block.size() || s3 >= P.W34.size() || s4 >= P.W34[s3].size() ||
-          k34 >= P.W34[s3][s4].size() || qW34 >= P.W34[s3][s4][k34].block.size())
-        throw VmpsError(VMPS_ERR_LAYOUT, "qrhs entry outside the pivot");
-      const SparseBlock& w12 = P.W12[s1][s2][k12].block[qW12];
-      const SparseBlock& w34 = P.W34[s3][s4][k34].block[qW34];
-      double f = P.factor_cgcs[q][p];
-      const DenseBlock& vin = P.V.block[s2s4][q24];
-      for (size_t i = 0; i < w12.val.size(); ++i)
-        for (size_t j = 0; j < w34.val.size(); ++j) {
-          int a = w12.row[i], w = w12.col[i], w2 = w34.row[j], b = w34.col[j];
-          if (a >= (int)P.L.block[qL].size() || b >= (int)P.R.block[qR].size())
-            throw VmpsError(VMPS_ERR_LAYOUT, "MPO index outside the environment's mid sector");
-          if (!(P.L.block[qL][a].present() && P.R.block[qR][b].present() && vin.present() && w == w2)) continue;
-          double c = f * w12.val[i] * w34.val[j];
-          if (c == 0.0) continue;  // exact zero contributes nothing
-          auto lk = std::make_pair(qL, (size_t)a);
-          auto rk = std::make_pair(qR, (size_t)b);
-          auto li = lid.find(lk);
-          if (li == lid.end()) {
-            li = lid.emplace(lk, (int64_t)lblocks.size()).first;
-            lblocks.push_back(lk);
+  TermSink sink{terms, lblocks, rblocks, {}, {}};
+  for (size_t t = 0; t < P.terms.size(); ++t) {
+    const PivotTerm& T = P.terms[t];
+    if (P.nsite == 0) {
+      // Vout[0](qL.in, qL.in) += sum_a L[qL][a] . Vin[0](qL.out, qL.out) . R[{out,in,mid}][a]
+      if (P.V.in.size() != 1) throw VmpsError(VMPS_ERR_LAYOUT, "zero-site pivot vectors have one component");
+      for (size_t qL = 0; qL < T.L.dim(); ++qL) {
+        auto qR = T.R.dict.find(Key3{T.L.out(qL), T.L.in(qL), T.L.mid(qL)});
+        if (qR == T.R.dict.end()) continue;
+        auto qAin = P.V.dict[0].find(Key2{T.L.out(qL), T.L.out(qL)});
+        if (qAin == P.V.dict[0].end()) continue;
+        const Qn& qo = T.R.out(qR->second);
+        auto qAout = P.V.dict[0].find(Key2{qo, qo});
+        if (qAout == P.V.dict[0].end()) continue;
+        if (T.L.block[qL].size() != T.R.block[qR->second].size())
+          throw VmpsError(VMPS_ERR_LAYOUT, "zero-site pivot: L and R sectors differ in their mid dimension");
+        if (!P.V.block[0][qAin->second].present()) continue;
+        for (size_t a = 0; a < T.L.block[qL].size(); ++a)
+          if (T.L.block[qL][a].present() && T.R.block[qR->second][a].present())
+            sink.add(vid[0][qAout->second], {t, qL, a}, vid[0][qAin->second], {t, qR->second, a}, 1.0);
+      }
+      continue;
+    }
+    if (!T.has_blockstructure) throw VmpsError(VMPS_ERR_STATE, "pivot has no block structure (call precalc first)");
+    for (size_t q = 0; q < T.qlhs.size(); ++q) {
+      size_t so = T.qlhs[q][0], qo = T.qlhs[q][1];
+      if (so >= P.V.in.size() || qo >= P.V.in[so].size())
+        throw VmpsError(VMPS_ERR_LAYOUT, "qlhs entry outside the vector layout");
+      for (size_t p = 0; p < T.qrhs[q].size(); ++p) {
+        const auto& e = T.qrhs[q][p];
+        const double f = T.factor_cgcs[q][p];
+        size_t si = e[0], qi = e[1], qL = e[2], qR = e[3];
+        if (si >= P.V.in.size() || qi >= P.V.in[si].size() || qL >= T.L.dim() || qR >= T.R.dim())
+          throw VmpsError(VMPS_ERR_LAYOUT, "qrhs entry outside the pivot");
+        const bool vin_present = P.V.block[si][qi].present();
+        if (P.nsite == 1) {
+          size_t k = e[4], qW = e[5];
+          if (k >= T.W12[so][si].size() || qW >= T.W12[so][si][k].block.size())
+            throw VmpsError(VMPS_ERR_LAYOUT, "qrhs entry outside the pivot");
+          const SparseBlock& w = T.W12[so][si][k].block[qW];
+          for (size_t i = 0; i < w.val.size(); ++i) {
+            int a = w.row[i], b = w.col[i];
+            if (!(env_block(T.L, qL, a).present() && env_block(T.R, qR, b).present() && vin_present)) continue;
+            sink.add(vid[so][qo], {t, qL, (size_t)a}, vid[si][qi], {t, qR, (size_t)b}, f * w.val[i]);
           }
-          auto ri = rid.find(rk);
-          if (ri == rid.end()) {
-            ri = rid.emplace(rk, (int64_t)rblocks.size()).first;
-            rblocks.push_back(rk);
-          }
-          terms.push_back({vid[s1s3][q13], li->second, vid[s2s4][q24], ri->second, c});
+          continue;
         }
+        size_t s1 = e[4], s2 = e[5], k12 = e[6], qW12 = e[7], s3 = e[8], s4 = e[9], k34 = e[10], qW34 = e[11];
+        if (s1 >= T.W12.size() || s2 >= T.W12[s1].size() || k12 >= T.W12[s1][s2].size() ||
+            qW12 >= T.W12[s1][s2][k12].block.size() || s3 >= T.W34.size() || s4 >= T.W34[s3].size() ||
+            k34 >= T.W34[s3][s4].size() || qW34 >= T.W34[s3][s4][k34].block.size())
+          throw VmpsError(VMPS_ERR_LAYOUT, "qrhs entry outside the pivot");
+        const SparseBlock& w12 = T.W12[s1][s2][k12].block[qW12];
+        const SparseBlock& w34 = T.W34[s3][s4][k34].block[qW34];
+        for (size_t i = 0; i < w12.val.size(); ++i)
+          for (size_t j = 0; j < w34.val.size(); ++j) {
+            int a = w12.row[i], w = w12.col[i], w2 = w34.row[j], b = w34.col[j];
+            if (!(env_block(T.L, qL, a).present() && env_block(T.R, qR, b).present() && vin_present && w == w2))
+              continue;
+            sink.add(vid[so][qo], {t, qL, (size_t)a}, vid[si][qi], {t, qR, (size_t)b}, f * w12.val[i] * w34.val[j]);
+          }
+      }
     }
   }
 }
@@ -443,20 +597,22 @@ void build_tiles(HostPlan& H) {
           throw VmpsError(VMPS_ERR_INTERNAL, "stage C: T block out of range");
 }
 
-void triples_from_pivot(const Pivot2& P, TripleSet& S) {
-  std::vector<std::pair<size_t, size_t>> lblocks, rblocks;
+void triples_from_pivot(const Pivot& P, TripleSet& S) {
+  std::vector<EnvRef> lblocks, rblocks;
   std::vector<std::pair<int, int>> vblocks;
   S = TripleSet();
   expand_terms(P, S.terms, lblocks, rblocks, vblocks);
   for (const auto& lb : lblocks) {
-    const DenseBlock& b = P.L.block[lb.first][lb.second];
+    const Tripod& E = P.terms[lb.t].L;
+    const DenseBlock& b = E.block[lb.q][lb.a];
     S.Xb.push_back({(int64_t)S.Xdata.size(), b.rows, b.cols});
-    S.Xdata.insert(S.Xdata.end(), P.L.data.begin() + b.off, P.L.data.begin() + b.off + b.size());
+    S.Xdata.insert(S.Xdata.end(), E.data.begin() + b.off, E.data.begin() + b.off + b.size());
   }
   for (const auto& rb : rblocks) {
-    const DenseBlock& b = P.R.block[rb.first][rb.second];
+    const Tripod& E = P.terms[rb.t].R;
+    const DenseBlock& b = E.block[rb.q][rb.a];
     S.Zb.push_back({(int64_t)S.Zdata.size(), b.rows, b.cols});
-    S.Zdata.insert(S.Zdata.end(), P.R.data.begin() + b.off, P.R.data.begin() + b.off + b.size());
+    S.Zdata.insert(S.Zdata.end(), E.data.begin() + b.off, E.data.begin() + b.off + b.size());
   }
   for (const auto& vb : vblocks) {
     const DenseBlock& b = P.V.block[vb.first][vb.second];
@@ -467,7 +623,7 @@ void triples_from_pivot(const Pivot2& P, TripleSet& S) {
   S.ysize = S.osize = P.V.size;
 }
 
-void build_host_plan(const Pivot2& P, HostPlan& H) {
+void build_host_plan(const Pivot& P, HostPlan& H) {
   TripleSet S;
   triples_from_pivot(P, S);
   build_plan(S, H);

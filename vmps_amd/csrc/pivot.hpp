@@ -6,6 +6,8 @@
 //   Tripod (Multipede<3>)   tensors/Multipede.h:31-160   index[q] = {in,out,mid}, block[q][a][0]
 //   Biped                   tensors/Biped.h:62-104       in[q], out[q], block[q], dict{in,out}
 //   PivotMatrix2            pivot/DmrgPivotMatrix2.h:15-76 (L, R, W12, W34, qloc, qOp, qlhs/qrhs/factor_cgcs)
+//   PivotMatrix1            pivot/DmrgPivotMatrix1.h:11-66 (one-site: L, R, W, qloc, qOp)
+//   PivotMatrix0            pivot/DmrgPivotMatrix0.h:10-27 (zero-site: L, R)
 //   PivotVector             pivot/DmrgPivotVector.h:8-117  data[s] = Biped of dense col-major blocks
 #pragma once
 #include <array>
@@ -99,33 +101,56 @@ struct TwoSiteBasis {
   std::map<Qn, int64_t> plain12_, plain34_;        // plain size of each local sector
 };
 
-struct Pivot2 {
-  Sym sym;
-  std::vector<Qn> qloc12, qloc34, qop12, qop34;
+// One summand of a pivot: PivotMatrix2Terms (DmrgPivotMatrix2.h:15-26) or
+// PivotMatrix1Terms (DmrgPivotMatrix1.h:12-22; W, qOp in the "12" slots).
+struct PivotTerm {
   Tripod L, R;
+  std::vector<Qn> qop12, qop34;
   std::vector<std::vector<std::vector<WBiped>>> W12, W34;  // [s1][s2][k]
-  VLayout V;
-
-  // PivotMatrix2::qlhs / qrhs / factor_cgcs (DmrgPivotMatrix2.h:66-70)
+  // qlhs / qrhs / factor_cgcs (DmrgPivotMatrix2.h:66-70, DmrgPivotMatrix1.h:58-62).
+  // Two-site rows: {s2s4,q24,qL,qR,s1,s2,k12,qW12,s3,s4,k34,qW34}; one-site
+  // rows use the first six entries {s2,q2,qL,qR,k,qW}, the rest are 0.
   std::vector<std::array<size_t, 2>> qlhs;
   std::vector<std::vector<std::array<size_t, 12>>> qrhs;
   std::vector<std::vector<double>> factor_cgcs;
   bool has_blockstructure = false;
+};
+
+// A pivot of the zero-, one- or two-site effective Hamiltonian
+// (PivotMatrix0 DmrgPivotMatrix0.h:10-27, PivotMatrix1 DmrgPivotMatrix1.h:24-66,
+// PivotMatrix2 DmrgPivotMatrix2.h:28-76): a sum over terms sharing one
+// wavefunction layout.
+struct Pivot {
+  int nsite = 2;
+  Sym sym;
+  std::vector<Qn> qloc12, qloc34;  // one-site: qloc12 is the local basis
+  std::vector<PivotTerm> terms;    // at least one
+  size_t cur = 0;                  // term the builder calls fill
+  VLayout V;
 
   // Excited-state projection (DmrgPivotMatrix2.h:72-75, :249-272): packed
   // vectors in the V layout.
   std::vector<std::vector<double>> A0proj;
   double Epenalty = 0.0;
 
-  void init_W();
+  PivotTerm& term() { return terms[cur]; }
+  const PivotTerm& term() const { return terms[cur]; }
+  // new empty term with the given operator bases; becomes the current one
+  size_t add_term(const std::vector<Qn>& qop12, const std::vector<Qn>& qop34);
   int add_V_block(int s, const Qn& qin, const Qn& qout, int64_t rows, int64_t cols);
+  void invalidate_blockstructure() {
+    for (auto& t : terms) t.has_blockstructure = false;
+  }
 };
 
-// Restatement of precalc_blockStructure (two-site, DmrgIndexGymnastics.h:698-856)
-// with the AAWWAA matcher (:198-262).  The reference returns entries in
-// unordered_map order; here the (s1s3, q13) keys are emitted sorted, entries
-// under one key keep the reference's loop order.
-void precalc_blockStructure(Pivot2& P);
+// Restatement of precalc_blockStructure for one term:
+//   two-site: DmrgIndexGymnastics.h:698-856 with the AAWWAA matcher (:198-262);
+//   one-site: DmrgIndexGymnastics.h:485-590 with the LAWA matcher (:32-77);
+//   zero-site: no table (PivotMatrix0's HxV matches sectors directly).
+// The reference returns entries in unordered_map order; here the keys are
+// emitted sorted, entries under one key keep the reference's loop order.
+void precalc_blockStructure(Pivot& P, PivotTerm& T);
+void precalc_blockStructure(Pivot& P);  // every term
 
 // ---- staged block-triple plan --------------------------------------------
 // Terms tau = (out block o, L block l, V block v, R block r, coefficient c)
@@ -285,14 +310,23 @@ struct HostPlan {
   std::vector<int64_t> vblock_off, vblock_size;
 };
 
-// Per-term list of a two-site pivot, in the order the reference's OxV visits them.
-void expand_terms(const Pivot2& P, std::vector<Term>& terms,
-                  std::vector<std::pair<size_t, size_t>>& lblocks,
-                  std::vector<std::pair<size_t, size_t>>& rblocks,
-                  std::vector<std::pair<int, int>>& vblocks);
+// (term, environment sector, mid index) of an environment block
+struct EnvRef {
+  size_t t, q, a;
+  bool operator<(const EnvRef& o) const {
+    if (t != o.t) return t < o.t;
+    if (q != o.q) return q < o.q;
+    return a < o.a;
+  }
+};
 
-void build_host_plan(const Pivot2& P, HostPlan& H);
-void triples_from_pivot(const Pivot2& P, TripleSet& S);
+// Per-term list of a pivot, in the order the reference's OxV visits them
+// (terms of the sum one after the other).
+void expand_terms(const Pivot& P, std::vector<Term>& terms, std::vector<EnvRef>& lblocks,
+                  std::vector<EnvRef>& rblocks, std::vector<std::pair<int, int>>& vblocks);
+
+void build_host_plan(const Pivot& P, HostPlan& H);
+void triples_from_pivot(const Pivot& P, TripleSet& S);
 void build_plan(const TripleSet& S, HostPlan& H);
 void build_tiles(HostPlan& H);
 // Split-BF16 operands and work lists of the BF16x3 path (built on demand).

@@ -23,9 +23,13 @@ def _ptr(a):
 
 
 class PivotMatrix2:
-    """Two-site effective Hamiltonian resident on the current HIP device."""
+    """Two-site effective Hamiltonian resident on the current HIP device (a
+    sum over the terms of `pd`)."""
+    NSITE = 2
 
     def __init__(self, pd, precision="fp64", blockstructure=None, projectors=None, Epenalty=0.0):
+        if pd.nsite != self.NSITE:
+            raise ValueError("%s needs a %d-site pivot, got %d sites" % (type(self).__name__, self.NSITE, pd.nsite))
         lib = _lib.load()
         self.pd = pd
         self._lib = lib
@@ -75,6 +79,16 @@ class PivotMatrix2:
         ms = ctypes.c_double()
         _lib.check(self._lib.vmps_plan_time_hxv(self._plan, warmup, iters, ctypes.byref(ms)))
         return ms.value
+
+
+class PivotMatrix1(PivotMatrix2):
+    """One-site effective Hamiltonian (PivotMatrix1, pivot/DmrgPivotMatrix1.h:24-66)."""
+    NSITE = 1
+
+
+class PivotMatrix0(PivotMatrix2):
+    """Zero-site effective Hamiltonian (PivotMatrix0, pivot/DmrgPivotMatrix0.h:10-27)."""
+    NSITE = 0
 
 
 class DeviceVector:

@@ -1,5 +1,6 @@
-"""Synthetic two-site pivots of the benchmark workloads, and the routine that
-hands a pivot to a library through the builder calls of include/vmps_hxv.h.
+"""Synthetic pivots (two-site for the benchmark workloads; one- and zero-site,
+and sums of terms, for the tests), and the routine that hands a pivot to a
+library through the builder calls of include/vmps_hxv.h.
 
 A pivot is described by PivotData: the two MPO site tensors, the sectors of
 the left and right bonds, dense environments and the block layout of the
@@ -32,16 +33,28 @@ def two_site_basis(sym, qloc12, qloc34):
 
 
 @dataclass
-class PivotData:
-    sym: models.Sym
+class PivotTerm:
+    """An additional summand of a pivot (PivotMatrix2Terms / PivotMatrix1Terms):
+    its own MPO tensors and environments on the same bonds."""
     W12: models.MpoSite
     W34: models.MpoSite
+    L: list
+    R: list
+
+
+@dataclass
+class PivotData:
+    sym: models.Sym
+    W12: models.MpoSite   # one-site pivots: the site's W; zero-site: None
+    W34: models.MpoSite   # two-site pivots only
     left: list          # [(q, dim)] bond sectors left of site 1
     right: list         # [(q, dim)] bond sectors right of site 2
     L: list = field(default_factory=list)   # [(qin, qout, qmid, [F-ordered blocks])]
     R: list = field(default_factory=list)
     V: list = field(default_factory=list)   # per s: [(qin, qout, rows, cols)]
     basis: list = field(default_factory=list)
+    nsite: int = 2
+    extra: list = field(default_factory=list)  # [PivotTerm] beyond the first
 
     @property
     def vsize(self):
@@ -91,6 +104,90 @@ def _symmetrize_env(env, aux, conj, flip):
             done.add((j, ac))
 
 
+def random_envs(sym, Win, Wout, left, right, rng, env_scale=1.0, hermitian=False):
+    """Random L over the left bond with Win's incoming auxiliary sectors and R
+    over the right bond with Wout's outgoing ones, every allowed sector."""
+    ldim = dict(left)
+    rdim = dict(right)
+    L, R = [], []
+    # L: {in (bra), out (ket), mid} with out (x) mid -> in  (DmrgContractions.h:30-33)
+    for (qmid, adim) in Win.aux_in:
+        for (qout, dout) in left:
+            for qin in sym.reduce(qout, qmid):
+                if qin in ldim:
+                    blocks = [np.asfortranarray(rng.standard_normal((ldim[qin], dout)) * env_scale) for _ in range(adim)]
+                    L.append((qin, qout, tuple(qmid), blocks))
+    # R: {in (ket), out (bra), mid} with in (x) mid -> out
+    for (qmid, adim) in Wout.aux_out:
+        for (qin, din) in right:
+            for qout in sym.reduce(qin, qmid):
+                if qout in rdim:
+                    blocks = [np.asfortranarray(rng.standard_normal((din, rdim[qout])) * env_scale) for _ in range(adim)]
+                    R.append((qin, qout, tuple(qmid), blocks))
+    if hermitian:
+        flip = lambda q: tuple(-x for x in q)
+        _symmetrize_env(L, Win.aux_in, Win.conj_in, flip)
+        _symmetrize_env(R, Wout.aux_out, Wout.conj_out, flip)
+    return L, R
+
+
+def make_pivot1(W, left, right, rng, env_scale=1.0, hermitian=False):
+    """One-site pivot (PivotMatrix1) on a site with MPO tensor W: PivotVector
+    components are the local states s, with every block (qin, qin + qloc[s])."""
+    sym = W.sym
+    pd = PivotData(sym, W, None, [(tuple(q), d) for q, d in left], [(tuple(q), d) for q, d in right], nsite=1)
+    pd.L, pd.R = random_envs(sym, W, W, pd.left, pd.right, rng, env_scale, hermitian)
+    rdim = dict(pd.right)
+    pd.basis = [(tuple(q), s) for s, q in enumerate(W.qloc)]
+    for q in W.qloc:
+        blocks = []
+        for (qin, din) in pd.left:
+            for qout in sym.reduce(qin, q):
+                if qout in rdim:
+                    blocks.append((qin, tuple(qout), din, rdim[qout]))
+        pd.V.append(blocks)
+    return pd
+
+
+def make_pivot0(W, bond, rng, env_scale=1.0):
+    """Zero-site pivot (PivotMatrix0) on a bond whose MPO auxiliary sectors are
+    W.aux_out: L and R share the mid sectors, the vector is one component of
+    blocks (q, q) for every bond sector q."""
+    sym = W.sym
+    bond = [(tuple(q), d) for q, d in bond]
+    pd = PivotData(sym, None, None, bond, bond, nsite=0)
+    bdim = dict(bond)
+    for (qmid, adim) in W.aux_out:
+        for (qout, dout) in bond:
+            for qin in sym.reduce(qout, qmid):
+                if qin in bdim:
+                    pd.L.append((qin, qout, tuple(qmid),
+                                 [np.asfortranarray(rng.standard_normal((bdim[qin], dout)) * env_scale)
+                                  for _ in range(adim)]))
+                    # R sector {in = L.out, out = L.in, mid}: ket x bra
+                    pd.R.append((qout, qin, tuple(qmid),
+                                 [np.asfortranarray(rng.standard_normal((dout, bdim[qin])) * env_scale)
+                                  for _ in range(adim)]))
+    pd.basis = [(None, 0)]
+    pd.V = [[(q, q, d, d) for (q, d) in bond]]
+    return pd
+
+
+def add_random_term(pd, rng, W12=None, W34=None, env_scale=1.0):
+    """Append a summand with fresh random environments (and optionally other
+    MPO tensors) on the same bonds."""
+    W12 = W12 or pd.W12
+    W34 = W34 or pd.W34
+    if pd.nsite == 2:
+        L, R = random_envs(pd.sym, W12, W34, pd.left, pd.right, rng, env_scale)
+    elif pd.nsite == 1:
+        L, R = random_envs(pd.sym, W12, W12, pd.left, pd.right, rng, env_scale)
+    else:
+        raise ValueError("zero-site pivots have a single term")
+    pd.extra.append(PivotTerm(W12, W34, L, R))
+    return pd
+
+
 def make_pivot(W12, W34, left, right, rng, env_scale=1.0, hermitian=False):
     """Random-environment pivot for sites with MPO tensors W12, W34.  With
     hermitian=True the environments are symmetrised through the MPO's
@@ -98,26 +195,8 @@ def make_pivot(W12, W34, left, right, rng, env_scale=1.0, hermitian=False):
     eigensolver); this requires left == right-compatible sector sets."""
     sym = W12.sym
     pd = PivotData(sym, W12, W34, [(tuple(q), d) for q, d in left], [(tuple(q), d) for q, d in right])
-    ldim = dict(pd.left)
     rdim = dict(pd.right)
-    # L: {in (bra), out (ket), mid} with out (x) mid -> in  (DmrgContractions.h:30-33)
-    for (qmid, adim) in W12.aux_in:
-        for (qout, dout) in pd.left:
-            for qin in sym.reduce(qout, qmid):
-                if qin in ldim:
-                    blocks = [np.asfortranarray(rng.standard_normal((ldim[qin], dout)) * env_scale) for _ in range(adim)]
-                    pd.L.append((qin, qout, tuple(qmid), blocks))
-    # R: {in (ket), out (bra), mid} with in (x) mid -> out
-    for (qmid, adim) in W34.aux_out:
-        for (qin, din) in pd.right:
-            for qout in sym.reduce(qin, qmid):
-                if qout in rdim:
-                    blocks = [np.asfortranarray(rng.standard_normal((din, rdim[qout])) * env_scale) for _ in range(adim)]
-                    pd.R.append((qin, qout, tuple(qmid), blocks))
-    if hermitian:
-        flip = lambda q: tuple(-x for x in q)
-        _symmetrize_env(pd.L, W12.aux_in, W12.conj_in, flip)
-        _symmetrize_env(pd.R, W34.aux_out, W34.conj_out, flip)
+    pd.L, pd.R = random_envs(sym, W12, W34, pd.left, pd.right, rng, env_scale, hermitian)
     pd.basis = two_site_basis(sym, W12.qloc, W34.qloc)
     pd.V = []
     for (qm, s1, s3) in pd.basis:
@@ -253,21 +332,8 @@ def _iarr(seq):
     return a, a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
 
 
-def feed(lib, prefix, pd, check, projectors=None, Epenalty=0.0):
-    """Build the pivot in `lib` through the `<prefix>pivot2_*` builder calls
-    (identical signatures for the engine and the oracle).  Returns the handle."""
-    fn = lambda name: getattr(lib, prefix + name)
-    sym = pd.sym
-    kinds, pk = _iarr(sym.kinds)
-    q12, p12 = _iarr(pd.W12.qloc)
-    q34, p34 = _iarr(pd.W34.qloc)
-    o12, po12 = _iarr(pd.W12.qop)
-    o34, po34 = _iarr(pd.W34.qop)
-    h = ctypes.c_void_p()
-    check(fn("pivot2_create")(sym.nq, pk, len(pd.W12.qloc), p12, len(pd.W34.qloc), p34, len(pd.W12.qop), po12,
-                              len(pd.W34.qop), po34, ctypes.byref(h)))
-    keep = []
-    for which, env in ((0, pd.L), (1, pd.R)):
+def _feed_envs(fn, h, L, R, check, keep):
+    for which, env in ((0, L), (1, R)):
         for (qin, qout, qmid, blocks) in env:
             a, pa = _iarr(qin)
             b, pb = _iarr(qout)
@@ -281,7 +347,10 @@ def feed(lib, prefix, pd, check, projectors=None, Epenalty=0.0):
                                        rows.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
                                        cols.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
                                        data.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
-    for which, W in ((12, pd.W12), (34, pd.W34)):
+
+
+def _feed_W(fn, h, slots, check):
+    for which, W in slots:
         for (s1, s2, k, qi, qo, rows, cols, r, c, v) in W.blocks:
             a, pa = _iarr(qi)
             b, pb = _iarr(qo)
@@ -292,6 +361,53 @@ def feed(lib, prefix, pd, check, projectors=None, Epenalty=0.0):
                                      r.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
                                      c.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
                                      v.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+
+
+def _w_slots(pd, W12, W34):
+    if pd.nsite == 2:
+        return ((12, W12), (34, W34))
+    if pd.nsite == 1:
+        return ((12, W12),)
+    return ()
+
+
+def feed(lib, prefix, pd, check, projectors=None, Epenalty=0.0, terms=None):
+    """Build the pivot in `lib` through the `<prefix>pivot*` builder calls
+    (identical signatures for the engine and the oracle).  `terms` selects
+    which summands to feed (0 = the first, i >= 1 = pd.extra[i-1]); the
+    default is all of them.  Returns the handle."""
+    fn = lambda name: getattr(lib, prefix + name)
+    sym = pd.sym
+    kinds, pk = _iarr(sym.kinds)
+    h = ctypes.c_void_p()
+    if pd.nsite == 2:
+        q12, p12 = _iarr(pd.W12.qloc)
+        q34, p34 = _iarr(pd.W34.qloc)
+        o12, po12 = _iarr(pd.W12.qop)
+        o34, po34 = _iarr(pd.W34.qop)
+        check(fn("pivot2_create")(sym.nq, pk, len(pd.W12.qloc), p12, len(pd.W34.qloc), p34, len(pd.W12.qop), po12,
+                                  len(pd.W34.qop), po34, ctypes.byref(h)))
+    elif pd.nsite == 1:
+        q12, p12 = _iarr(pd.W12.qloc)
+        o12, po12 = _iarr(pd.W12.qop)
+        check(fn("pivot1_create")(sym.nq, pk, len(pd.W12.qloc), p12, len(pd.W12.qop), po12, ctypes.byref(h)))
+    else:
+        check(fn("pivot0_create")(sym.nq, pk, ctypes.byref(h)))
+    keep = []
+    all_terms = [PivotTerm(pd.W12, pd.W34, pd.L, pd.R)] + list(pd.extra)
+    sel = range(len(all_terms)) if terms is None else list(terms)
+    for n, i in enumerate(sel):
+        t = all_terms[i]
+        if n > 0:
+            if prefix != "vmps_":
+                raise ValueError("the oracle takes one term per pivot")
+            o12, po12 = _iarr(t.W12.qop if t.W12 is not None else [0] * sym.nq)
+            o34, po34 = _iarr(t.W34.qop if t.W34 is not None else [0] * sym.nq)
+            n12 = len(t.W12.qop) if t.W12 is not None else 0
+            n34 = len(t.W34.qop) if t.W34 is not None else 0
+            check(fn("pivot_add_term")(h, n12, po12, n34, po34, None))
+        _feed_envs(fn, h, t.L, t.R, check, keep)
+        _feed_W(fn, h, _w_slots(pd, t.W12, t.W34), check)
     off = ctypes.c_int64()
     for s, blocks in enumerate(pd.V):
         for (qi, qo, rr, cc) in blocks:
