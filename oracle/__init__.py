@@ -44,6 +44,7 @@ SIGNATURES = {
     "oracle_wigner9j": (ctypes.c_double, [ctypes.c_int] * 9),
     "oracle_coeff_HPsi": (ctypes.c_double, [ctypes.c_int, P_int, P_int]),
     "oracle_coeff_buildR": (ctypes.c_double, [ctypes.c_int, P_int, P_int]),
+    "oracle_coeff_buildL": (ctypes.c_double, [ctypes.c_int, P_int, P_int]),
     "oracle_coeff_Apair": (ctypes.c_double, [ctypes.c_int, P_int, P_int]),
 }
 

@@ -208,6 +208,8 @@ double oracle_coeff_buildR(int nq, const int* kinds, const int* q9) {
   for (int j = 0; j < 9; ++j) q[j] = qread(q9 + j * nq, nq);
   return c9(&S, q, 1);
 }
+/* coeff_buildL (symmetry/SU2.h:481-492) has the same form as coeff_HPsi */
+double oracle_coeff_buildL(int nq, const int* kinds, const int* q9) { return oracle_coeff_HPsi(nq, kinds, q9); }
 static double c_apair(const sym_t* S, qn_t q1, qn_t q2, qn_t q3, qn_t q4, qn_t q5, qn_t q6) {
   double out = 1.0;
   for (int k = 0; k < S->nq; ++k) {

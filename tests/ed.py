@@ -65,3 +65,41 @@ def hubbard_ground(L, t=1.0, U=8.0, N=None, two_sz=0):
             idx.append(n)
     Hs = H[idx][:, idx]
     return float(eigsh(Hs, k=1, which="SA", tol=1e-13)[0][0])
+
+
+def heisenberg_dense(L, J=1.0, S=0.5):
+    """Dense spin-S Heisenberg chain and total Sz, S^2 (basis: site 0 most significant)."""
+    d = int(round(2 * S)) + 1
+    m = np.arange(S, -S - 1, -1)[::-1]  # local Sz values, index 0 = lowest
+    Sz = np.diag(m)
+    Sp = np.zeros((d, d))
+    for i in range(d - 1):
+        Sp[i + 1, i] = np.sqrt(S * (S + 1) - m[i] * (m[i] + 1))
+    Sm = Sp.T
+    ops = lambda A, i: _op(A, i, L, d).toarray()
+    H = np.zeros((d ** L, d ** L))
+    for i in range(L - 1):
+        H += J * (ops(Sz, i) @ ops(Sz, i + 1) + 0.5 * (ops(Sp, i) @ ops(Sm, i + 1) + ops(Sm, i) @ ops(Sp, i + 1)))
+    Szt = sum(ops(Sz, i) for i in range(L))
+    Spt = sum(ops(Sp, i) for i in range(L))
+    S2 = Spt.T @ Spt + Szt @ Szt + Szt  # S^2 = S-S+ + Sz^2 + Sz
+    return H, np.diag(Szt), S2
+
+
+def heisenberg_sector_spectrum(L, two_sz, J=1.0):
+    """All eigenvalues of the S=1/2 chain with 2 Sz_total = two_sz."""
+    H, sz, _ = heisenberg_dense(L, J)
+    idx = np.where(np.isclose(2 * sz, two_sz))[0]
+    return np.sort(np.linalg.eigvalsh(H[np.ix_(idx, idx)]))
+
+
+def heisenberg_multiplet_spectrum(L, S_tot, J=1.0):
+    """One eigenvalue per spin-S_tot multiplet: H restricted to the highest-weight
+    states (Sz = S_tot and S^2 = S_tot(S_tot+1))."""
+    H, sz, S2 = heisenberg_dense(L, J)
+    idx = np.where(np.isclose(sz, S_tot))[0]
+    w, U = np.linalg.eigh(S2[np.ix_(idx, idx)])
+    P = U[:, np.isclose(w, S_tot * (S_tot + 1))]
+    if P.shape[1] == 0:
+        return np.zeros(0)
+    return np.sort(np.linalg.eigvalsh(P.T @ H[np.ix_(idx, idx)] @ P))

@@ -12,6 +12,7 @@ W[s1][s2][k] holds <s1|O|s2> for an operator O of charge qOp[k] = q_out - q_in
 Models:
   HeisenbergU1   S=1/2, H = J sum Sz Sz + J/2 (S+ S- + S- S+)   (models/HeisenbergU1.h)
   HubbardU1xU1   H = -t sum_sigma (c+ c + h.c.) + U sum n_up n_dn (models/HubbardU1xU1.h)
+  HeisenbergSU2  S=1/2, H = J sum S.S in SU(2)-reduced form          (models/HeisenbergSU2.h)
 """
 from dataclasses import dataclass, field
 import numpy as np
@@ -254,6 +255,40 @@ def hubbard_u1xu1_mpo(t=1.0, U=8.0, position="bulk"):
     return _place(U1xU1, qloc, q, ent, position, conj=[0, 2, 1, 4, 3, 5])
 
 
+# ---------------------------------------------------------------------------
+# HeisenbergSU2, S = 1/2, in reduced (Wigner-Eckart) form: one local
+# multiplet q = 2S+1 = 2.  Reduced elements as sites/SpinSiteSU2.h:57-61:
+# <S||S||S> = sqrt(S(S+1)); S^dagger = S^T * coeff_adjoint(2, 2, 3) = -S
+# (tensors/SiteOperatorQ.h:292-305, symmetry/SU2.h coeff_adjoint).  The
+# nearest-neighbour term is sqrt(3) J (S^dagger_i S_{i+1}) coupled to a
+# singlet (models/HeisenbergSU2.h:259-262).
+# ---------------------------------------------------------------------------
+SU2 = Sym([SU2_KIND])
+SU2xU1 = Sym([SU2_KIND, U1_KIND])
+
+
+def heisenberg_su2_mpo(J=1.0, position="bulk", S=0.5):
+    q2 = int(round(2 * S)) + 1
+    red = np.array([[np.sqrt(S * (S + 1.0))]])
+    Id = np.eye(1)
+    qloc = [(q2,)]
+    # global auxiliary order: 0 done | 1 S pending (q=3) | 2 start
+    q = [(1,), (3,), (1,)]
+    ent = [
+        (0, 0, Id, (1,)),
+        (1, 0, red, (3,)),
+        (2, 1, np.sqrt(3.0) * J * (-red), (3,)),
+        (2, 2, Id, (1,)),
+    ]
+    return _place(SU2, qloc, q, ent, position)
+
+
+def identity_mpo(sym, qloc):
+    """W = identity with one vacuum auxiliary state (the overlap operator)."""
+    vac = tuple(1 if k == SU2_KIND else 0 for k in sym.kinds)
+    return build_mpo_site(sym, qloc, [vac], [vac], [(0, 0, np.eye(len(qloc)), vac)])
+
+
 def _sector_order(qs):
     """Global index -> position in the sector-contiguous numbering of _aux_sectors."""
     sectors, local = _aux_sectors(qs)
@@ -290,7 +325,8 @@ def _place(sym, qloc, q, ent, position, conj=None):
 
 def chain_mpo(model, L, **params):
     """Full open-boundary MPO as a list of MpoSite."""
-    f = {"HeisenbergU1": heisenberg_u1_mpo, "HubbardU1xU1": hubbard_u1xu1_mpo}[model]
+    f = {"HeisenbergU1": heisenberg_u1_mpo, "HubbardU1xU1": hubbard_u1xu1_mpo,
+         "HeisenbergSU2": heisenberg_su2_mpo}[model]
     if L == 1:
         return [f(position="single", **params)]
     return [f(position="first", **params)] + [f(position="bulk", **params) for _ in range(L - 2)] + \

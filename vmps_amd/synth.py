@@ -281,6 +281,19 @@ def stretched_sectors(center, M, alpha, lam, allowed):
     return [(tuple(q), int(d)) for q, d in zip(qs, dims) if d > 0]
 
 
+def heisenberg_su2_bond(nsites_left, M, sigma=None):
+    """SU(2) multiplets q = 2S+1 of the bond after `nsites_left` spins 1/2:
+    S runs over the values the spins allow, the multiplet counts follow a
+    Gaussian in S (sum = M multiplets)."""
+    par = nsites_left % 2
+    qs = [(q,) for q in range(1 + par, nsites_left + 2, 2)]
+    sigma = sigma or max(1.0, 0.5 * math.sqrt(nsites_left))
+    w = np.array([math.exp(-((q[0] - 1) / 2.0) ** 2 / (2 * sigma ** 2)) for q in qs])
+    dims = np.maximum(1, np.floor(M * w / w.sum() + 0.5)).astype(int)
+    keep = dims > 0
+    return [(q, int(d)) for q, d, k in zip(qs, dims, keep) if k and (M * w[qs.index(q)] / w.sum()) >= 0.5]
+
+
 def hubbard_bond(nsites_left, M, filling=1.0, alpha=0.36, lam=0.02):
     """(N, 2Sz) sectors of the bond after `nsites_left` Hubbard sites.  With the
     defaults and M = 512: ~110 sectors of dims 1..~55."""
