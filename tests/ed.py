@@ -103,3 +103,42 @@ def heisenberg_multiplet_spectrum(L, S_tot, J=1.0):
     if P.shape[1] == 0:
         return np.zeros(0)
     return np.sort(np.linalg.eigvalsh(P.T @ H[np.ix_(idx, idx)] @ P))
+
+
+def hubbard_dense(L, t=1.0, U=8.0):
+    """Dense Hubbard chain with an explicit Jordan-Wigner string; returns H,
+    the diagonals of N and 2 Sz, and S^2."""
+    c_up = np.zeros((4, 4))
+    c_dn = np.zeros((4, 4))
+    c_up[0, 1] = 1.0
+    c_up[2, 3] = 1.0
+    c_dn[0, 2] = 1.0
+    c_dn[1, 3] = -1.0
+    F = np.diag([1.0, -1.0, -1.0, 1.0])
+    ops = lambda A, i, s=None: _op(A, i, L, 4, string=s).toarray()
+    H = np.zeros((4 ** L, 4 ** L))
+    for i in range(L):
+        H += U * ops(c_up.T @ c_up @ c_dn.T @ c_dn, i)
+    for i in range(L - 1):
+        for c in (c_up, c_dn):
+            ci, cj = ops(c, i, F), ops(c, i + 1, F)
+            H -= t * (ci.T @ cj + cj.T @ ci)
+    n_up, n_dn = c_up.T @ c_up, c_dn.T @ c_dn
+    N = sum(np.diag(ops(n_up + n_dn, i)) for i in range(L))
+    two_sz = sum(np.diag(ops(n_up - n_dn, i)) for i in range(L))
+    Sp = c_up.T @ c_dn  # on-site S+ (bosonic: no string)
+    Spt = sum(ops(Sp, i) for i in range(L))
+    Szt = np.diag(0.5 * two_sz)
+    S2 = Spt.T @ Spt + Szt @ Szt + Szt
+    return H, N, two_sz, S2
+
+
+def hubbard_multiplet_spectrum(L, N_tot, S_tot, t=1.0, U=8.0):
+    """One eigenvalue per spin-S_tot multiplet at particle number N_tot."""
+    H, N, two_sz, S2 = hubbard_dense(L, t, U)
+    idx = np.where(np.isclose(N, N_tot) & np.isclose(two_sz, 2 * S_tot))[0]
+    w, Uv = np.linalg.eigh(S2[np.ix_(idx, idx)])
+    P = Uv[:, np.isclose(w, S_tot * (S_tot + 1))]
+    if P.shape[1] == 0:
+        return np.zeros(0)
+    return np.sort(np.linalg.eigvalsh(P.T @ H[np.ix_(idx, idx)] @ P))

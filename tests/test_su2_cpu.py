@@ -101,3 +101,17 @@ def test_engine_su2_blockstructure_equals_oracle(make):
     assert np.allclose(bo[3], bv[3], rtol=1e-13, atol=1e-15)
     # non-trivial factors occur
     assert np.any(np.abs(np.abs(bo[3]) - 1.0) > 1e-3)
+
+
+@pytest.mark.parametrize("nsite,p", [(2, 1), (1, 2)])
+@pytest.mark.parametrize("S2,N", [(1, 4), (3, 4), (2, 3), (2, 5), (1, 2)])
+def test_fullbasis_hubbard_su2xu1_spectrum_equals_ed(nsite, p, S2, N):
+    """HubbardSU2xU1 (reduced c, c^dagger with the fermionic sign, U d):
+    exact-pivot spectrum at (2S+1, N) == Hubbard ED multiplets, L = 4."""
+    L = 4
+    mpo = models.chain_mpo("HubbardSU2xU1", L, t=1.0, U=8.0)
+    ev = fb.exact_spectrum(models.SU2xU1, mpo, (S2, N), p, nsite, fb.oracle_contracts(models.SU2xU1),
+                           fb.oracle_factory)
+    ref = ed.hubbard_multiplet_spectrum(L, N, (S2 - 1) / 2)
+    assert ev.shape == ref.shape
+    assert np.allclose(ev, ref, atol=1e-10)

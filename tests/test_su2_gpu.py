@@ -74,3 +74,15 @@ def test_su2_exact_pivot_spectrum_device(nsite, p, S):
     ref = ed.heisenberg_multiplet_spectrum(L, S)
     assert ev.shape == ref.shape
     assert np.allclose(ev, ref, atol=1e-10)
+
+
+@pytest.mark.parametrize("S2,N", [(1, 4), (2, 3)])
+def test_hubbard_su2xu1_exact_pivot_spectrum_device(S2, N):
+    print("hubbard su2xu1 exact pivot", S2, N, flush=True)
+    L = 4
+    mpo = models.chain_mpo("HubbardSU2xU1", L, t=1.0, U=8.0)
+    ev = fb.exact_spectrum(models.SU2xU1, mpo, (S2, N), 1, 2, fb.device_contracts(models.SU2xU1),
+                           fb.device_factory)
+    ref = ed.hubbard_multiplet_spectrum(L, N, (S2 - 1) / 2)
+    assert ev.shape == ref.shape
+    assert np.allclose(ev, ref, atol=1e-10)

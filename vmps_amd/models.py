@@ -283,6 +283,46 @@ def heisenberg_su2_mpo(J=1.0, position="bulk", S=0.5):
     return _place(SU2, qloc, q, ent, position)
 
 
+# ---------------------------------------------------------------------------
+# HubbardSU2xU1 (spin SU(2) x charge U(1)) in reduced form.  Local multiplets
+# (q = 2S+1, N): empty (1,0), single (2,1), double (1,2)
+# (sites/FermionSiteSU2xU1.h:57-79).  Reduced elements: c(empty, single) =
+# sqrt(2), c(single, double) = 1; c^dagger = c.adjoint() (coeff_adjoint:
+# cdag(single, empty) = 1, cdag(double, single) = -sqrt(2)); F = diag(1,-1,1);
+# d = double occupancy.  Products with the scalar F carry coeff_prod = 1.
+# Hopping -sqrt(2) t [(c^dagger F)_i c_{i+1} + (c F)_i c^dagger_{i+1}]
+# (models/HubbardSU2xU1.h:404-438), on-site U d.
+# ---------------------------------------------------------------------------
+def hubbard_su2xu1_ops():
+    qloc = [(1, 0), (2, 1), (1, 2)]
+    c = np.zeros((3, 3))
+    c[0, 1] = np.sqrt(2.0)
+    c[1, 2] = 1.0
+    cdag = np.zeros((3, 3))
+    cdag[1, 0] = 1.0
+    cdag[2, 1] = -np.sqrt(2.0)
+    F = np.diag([1.0, -1.0, 1.0])
+    d = np.diag([0.0, 0.0, 1.0])
+    return qloc, np.eye(3), F, c, cdag, d
+
+
+def hubbard_su2xu1_mpo(t=1.0, U=8.0, position="bulk"):
+    """D_W = 4 (in multiplets): 0 done | 1 c pending (2,+1) | 2 c^dagger pending (2,-1) | 3 start."""
+    qloc, Id, F, c, cdag, d = hubbard_su2xu1_ops()
+    q = [(1, 0), (2, 1), (2, -1), (1, 0)]
+    s2 = np.sqrt(2.0)
+    ent = [
+        (0, 0, Id, (1, 0)),
+        (1, 0, c, (2, -1)),
+        (2, 0, cdag, (2, 1)),
+        (3, 0, U * d, (1, 0)),
+        (3, 1, -s2 * t * (cdag @ F), (2, 1)),
+        (3, 2, -s2 * t * (c @ F), (2, -1)),
+        (3, 3, Id, (1, 0)),
+    ]
+    return _place(SU2xU1, qloc, q, ent, position)
+
+
 def identity_mpo(sym, qloc):
     """W = identity with one vacuum auxiliary state (the overlap operator)."""
     vac = tuple(1 if k == SU2_KIND else 0 for k in sym.kinds)
@@ -326,7 +366,7 @@ def _place(sym, qloc, q, ent, position, conj=None):
 def chain_mpo(model, L, **params):
     """Full open-boundary MPO as a list of MpoSite."""
     f = {"HeisenbergU1": heisenberg_u1_mpo, "HubbardU1xU1": hubbard_u1xu1_mpo,
-         "HeisenbergSU2": heisenberg_su2_mpo}[model]
+         "HeisenbergSU2": heisenberg_su2_mpo, "HubbardSU2xU1": hubbard_su2xu1_mpo}[model]
     if L == 1:
         return [f(position="single", **params)]
     return [f(position="first", **params)] + [f(position="bulk", **params) for _ in range(L - 2)] + \
