@@ -182,6 +182,14 @@ vmps_status vmps_hxv_device(vmps_plan* plan, const double* d_vin, double* d_vout
  * (DmrgPivotVector.h:296-339).  Host pointers. */
 vmps_status vmps_dot(vmps_plan* plan, const double* x, const double* y, double* result);
 
+/* SU(2) recoupling symbols with the reference's argument convention q = 2S+1:
+ * coupling_6j / coupling_9j (symmetry/SU2Wrappers.h:71-94, which pass q-1 =
+ * 2j to GSL).  Host-only, evaluated by the same Racah routine the block
+ * structure uses; for callers that recouple MPS tensors (contract_AA2,
+ * split_AA2). */
+double vmps_coupling_6j(int q1, int q2, int q3, int q4, int q5, int q6);
+double vmps_coupling_9j(int q1, int q2, int q3, int q4, int q5, int q6, int q7, int q8, int q9);
+
 /* Timing helper: mean device time of one HxV on the plan's stream, measured
  * with HIP events around `iters` back-to-back calls on resident vectors. */
 vmps_status vmps_plan_time_hxv(vmps_plan* plan, int warmup, int iters, double* ms_per_hxv);

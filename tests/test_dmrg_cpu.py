@@ -31,3 +31,71 @@ def test_truncated_run_is_variational():
     assert r["energy"] > e_ed
     assert max(r["truncation"]) > 0
     assert r["energy"] - e_ed < 1e-2
+
+
+def test_heisenberg_su2_L10_matches_ed_and_u1():
+    """SU(2)-reduced DMRG (coeff_twoSiteGate / coeff_splitAA recoupling, weighted
+    right-normalisation, multiplet truncation) reaches the exact ground state,
+    equal to the U(1) result (the reference's own check, test_Heisenberg.cc)."""
+    e_ed = ed.heisenberg_ground(10)
+    r = dmrg.run_dmrg("HeisenbergSU2", 10, 16, 2, backend=dmrg_oracle.OracleBackend())
+    assert abs(r["energy"] - e_ed) <= 1e-10 * abs(e_ed)
+
+
+def test_hubbard_su2xu1_L6_matches_ed():
+    e_ed = ed.hubbard_ground(6, U=8.0)
+    r = dmrg.run_dmrg("HubbardSU2xU1", 6, 40, 2, backend=dmrg_oracle.OracleBackend(), params=dict(t=1.0, U=8.0))
+    assert abs(r["energy"] - e_ed) <= 1e-10 * abs(e_ed)
+
+
+def test_su2_truncation_weight_counts_multiplets():
+    """Truncated SU(2) run: variational, and the truncation weight is
+    sum deg(q) sigma^2 over discarded multiplets (tensors/Biped.h:800-837)."""
+    e_ed = ed.heisenberg_ground(12)
+    r = dmrg.run_dmrg("HeisenbergSU2", 12, 4, 2, backend=dmrg_oracle.OracleBackend())
+    assert r["energy"] > e_ed
+    assert max(r["truncation"]) > 0
+
+
+def _split_roundtrip_case(model):
+    import numpy as np
+    from vmps_amd import models, synth
+    rng = np.random.default_rng(11)
+    if model == "HeisenbergSU2":
+        W = models.heisenberg_su2_mpo()
+        left, right = synth.heisenberg_su2_bond(9, 14), synth.heisenberg_su2_bond(11, 14)
+    elif model == "HubbardSU2xU1":
+        W = models.hubbard_su2xu1_mpo()
+        left = [((q, n), 2) for q in (1, 2, 3) for n in (3, 4, 5)]
+        right = [((q, n), 2) for q in (1, 2, 3) for n in (5, 6, 7)]
+    else:
+        W = models.hubbard_u1xu1_mpo()
+        left = synth.hubbard_bond(4, 16, alpha=1.0, lam=1.2)
+        right = synth.hubbard_bond(6, 16, alpha=1.0, lam=1.2)
+    return synth.make_pivot(W, W, left, right, rng), W
+
+
+@pytest.mark.parametrize("model", ["HeisenbergSU2", "HubbardSU2xU1", "HubbardU1xU1"])
+@pytest.mark.parametrize("left_to_right", [True, False])
+def test_split_then_contract_roundtrip(model, left_to_right):
+    """split_AA2 (coeff_splitAA) followed by contract_AA2 (coeff_twoSiteGate)
+    reproduces the two-site vector; the kept weight sum deg(Qm) sigma^2 equals
+    the coeff_dot-weighted norm; with truncation, kept + discarded = norm."""
+    import numpy as np
+    import dmrg_oracle as do
+    from vmps_amd import synth
+    pd, W = _split_roundtrip_case(model)
+    v = synth.random_vector(pd, 5)
+    w = do.element_weights(pd)
+    v /= np.sqrt(np.sum(w * v * v))
+    Al, Ar, tw, ent, kept = dmrg.split_psi(pd, v, W.qloc, 10 ** 9, left_to_right=left_to_right)
+    assert tw == 0.0
+    back = dmrg.two_site_psi(pd, Al, Ar, W.qloc)
+    # blocks of v that no middle sector can carry are dropped by the recoupling
+    assert np.linalg.norm(back - v) <= 1e-12 * np.linalg.norm(v)
+    # truncated: kept weight + discarded weight = 1
+    Al, Ar, tw, ent, kept = dmrg.split_psi(pd, v, W.qloc, 5, left_to_right=left_to_right)
+    back = dmrg.two_site_psi(pd, Al, Ar, W.qloc)
+    kept_w = float(np.sum(w * back * v))
+    assert tw > 0
+    assert abs(kept_w + tw - 1.0) <= 1e-12

@@ -41,3 +41,25 @@ def test_mixed_precision_dmrg_energy(prec):
                         params=dict(t=1.0, U=8.0))
     print(prec, r64["energy"], rmx["energy"], rmx["energy"] - r64["energy"])
     assert abs(rmx["energy"] - r64["energy"]) <= 1e-9
+
+
+def test_gpu_dmrg_su2_matches_ed_and_u1():
+    """SU(2)-reduced device DMRG: HeisenbergSU2 L=10 == exact; L=16 with the
+    untruncated M = 70 multiplets == U(1) DMRG with the untruncated M = 256
+    states (U(1)/SU(2) equality, the reference's own check,
+    test_Heisenberg.cc:393)."""
+    e_ed = ed.heisenberg_ground(10)
+    r = dmrg.run_dmrg("HeisenbergSU2", 10, 16, 2, backend=dmrg.GpuBackend())
+    assert abs(r["energy"] - e_ed) <= 1e-10 * abs(e_ed)
+    r_su2 = dmrg.run_dmrg("HeisenbergSU2", 16, 70, 3, backend=dmrg.GpuBackend(tol_eigval=1e-13, tol_state=1e-8))
+    r_u1 = dmrg.run_dmrg("HeisenbergU1", 16, 256, 3, backend=dmrg.GpuBackend(tol_eigval=1e-13, tol_state=1e-8))
+    print("SU2", r_su2["energy"], "U1", r_u1["energy"], "trunc", max(r_su2["truncation"][-15:]),
+          max(r_u1["truncation"][-15:]))
+    assert abs(r_su2["energy"] - r_u1["energy"]) <= 1e-10 * abs(r_u1["energy"])
+    assert abs(r_su2["energy"] - ed.heisenberg_ground(16)) <= 1e-10 * abs(r_u1["energy"])
+
+
+def test_gpu_dmrg_hubbard_su2xu1_matches_ed():
+    e_ed = ed.hubbard_ground(6, U=8.0)
+    r = dmrg.run_dmrg("HubbardSU2xU1", 6, 40, 2, backend=dmrg.GpuBackend(), params=dict(t=1.0, U=8.0))
+    assert abs(r["energy"] - e_ed) <= 1e-10 * abs(e_ed)

@@ -466,6 +466,14 @@ vmps_status vmps_hxv_device(vmps_plan* plan, const double* d_vin, double* d_vout
   VMPS_API_END
 }
 
+double vmps_coupling_6j(int q1, int q2, int q3, int q4, int q5, int q6) {
+  return vmps::wigner6j_q(q1, q2, q3, q4, q5, q6);
+}
+
+double vmps_coupling_9j(int q1, int q2, int q3, int q4, int q5, int q6, int q7, int q8, int q9) {
+  return vmps::wigner9j_q(q1, q2, q3, q4, q5, q6, q7, q8, q9);
+}
+
 vmps_status vmps_dot(vmps_plan* plan, const double* x, const double* y, double* result) {
   VMPS_API_BEGIN
   require(plan && result && (plan->D.H.vsize == 0 || (x && y)), VMPS_ERR_ARG, "NULL argument");

@@ -1,68 +1,133 @@
-"""Two-site DMRG driver for abelian (U(1), U(1)xU(1)) chains: the caller of the
-effective-Hamiltonian engine, following DmrgSolver::edgeState / halfsweep /
-iteration_two (solvers/DmrgSolver.h:749-874, :1398-1523, :1669-1792):
+"""Two-site DMRG driver for U(1), U(1)xU(1), SU(2) and SU(2)xU(1) chains: the
+caller of the effective-Hamiltonian engine, following DmrgSolver::edgeState /
+halfsweep / iteration_two (solvers/DmrgSolver.h:749-874, :1398-1523,
+:1669-1792):
 
   per two-site pivot (l, l+1):
-    Psi   = contract_AA2(A[l], A[l+1]), all allowed blocks   (DmrgContractions.h:1832-1927)
+    Psi   = contract_AA2(A[l], A[l+1]), all allowed blocks, SU(2) recoupled with
+            coeff_twoSiteGate                    (DmrgContractions.h:1832-1927)
     E, Psi <- lowest eigenpair of H_eff (L[l], W[l], W[l+1], R[l+2])
-    A[l], A[l+1] = split_AA2(Psi): per-sector SVD, keep the largest M singular
-                   values, truncation weight = sum of discarded sigma^2
-                   (DmrgContractions.h:2460-2630, tensors/Biped.h:741-897)
+    A[l], A[l+1] = split_AA2(Psi): recoupling with coeff_splitAA, SVD per
+                   middle sector, keep the largest M singular values
+                   (multiplets), truncation weight = sum of discarded
+                   deg(q) sigma^2   (DmrgContractions.h:2460-2630, tensors/Biped.h:741-897)
     L[l+1] = contract_L(L[l], A[l], W[l], A[l])   (left-to-right half sweep)
     R[l+1] = contract_R(R[l+2], A[l+1], W[l+1], A[l+1]) (right-to-left)
 
 The eigensolver and the environment updates are pluggable (a backend); the
 default backend runs both on the GPU through the C ABI.  Bond sectors carry
-the total charge of the sites to their left; the right boundary carries the
-target charge Q.
+the total quantum numbers of the sites to their left; the right boundary
+carries the target Q.  SU(2) bond sectors are multiplets q = 2S+1 and M counts
+multiplets.
 """
 import math
 import time
 
 import numpy as np
 
-from . import models, synth
+from . import _lib, models, synth
 
 
 # ---------------------------------------------------------------------------
-# MPS helpers (abelian: every combination of charges is a single irrep)
+# SU(2) recoupling factors of contract_AA2 / split_AA2 (symmetry/SU2.h; the
+# U(1) components contribute 1).  phase() follows symmetry/functions.h:50-55
+# with C++ integer division.
 # ---------------------------------------------------------------------------
-def _add(a, b):
-    return tuple(x + y for x, y in zip(a, b))
+def _cdiv2(x):
+    return int(x / 2)
 
 
-def _sub(a, b):
-    return tuple(x - y for x, y in zip(a, b))
+def _phase(x):
+    return -1.0 if x % 2 else 1.0
 
 
-def reachable_counts(qloc, nsites, vac):
-    """Number of product states with each total charge after nsites sites."""
-    counts = {tuple(vac): 1}
+def _6j(q1, q2, q3, q4, q5, q6):
+    return _lib.load().vmps_coupling_6j(q1, q2, q3, q4, q5, q6)
+
+
+def coeff_two_site_gate(sym, q1, q2, q3, q4, q5, q6):
+    """coeff_twoSiteGate: 6j{q1 q2 q3; q4 q5 q6} sqrt(q3 q6) (-1)^((q1+q2+q4+q5-4)/2)."""
+    out = 1.0
+    for i, k in enumerate(sym.kinds):
+        if k == models.SU2_KIND:
+            out *= _6j(q1[i], q2[i], q3[i], q4[i], q5[i], q6[i]) * math.sqrt(q3[i] * q6[i]) * \
+                _phase(_cdiv2(q1[i] + q2[i] + q4[i] + q5[i] - 4))
+    return out
+
+
+def coeff_split6(sym, q1, q2, q3, q4, q5, q6):
+    """coeff_splitAA (six arguments): 6j{q1 q2 q3; q4 q5 q6} sqrt(q2 q3) (-1)^((q1+q5+q6-3)/2)."""
+    out = 1.0
+    for i, k in enumerate(sym.kinds):
+        if k == models.SU2_KIND:
+            out *= _6j(q1[i], q2[i], q3[i], q4[i], q5[i], q6[i]) * math.sqrt(q2[i] * q3[i]) * \
+                _phase(_cdiv2(q1[i] + q5[i] + q6[i] - 3))
+    return out
+
+
+def coeff_split3(sym, q1, q2, q3):
+    """coeff_splitAA (three arguments): (-1)^((q1-q2-q3-3)/2) sqrt(q1/q2)."""
+    out = 1.0
+    for i, k in enumerate(sym.kinds):
+        if k == models.SU2_KIND:
+            out *= _phase(_cdiv2(q1[i] - q2[i] - q3[i] - 3)) * math.sqrt(q1[i] / q2[i])
+    return out
+
+
+# ---------------------------------------------------------------------------
+# MPS helpers
+# ---------------------------------------------------------------------------
+def left_counts(sym, qloc, nsites):
+    """Number of multiplets (SU(2)) or states (U(1)) of each quantum number
+    after nsites sites, coupled from the vacuum."""
+    counts = {sym.vacuum(): 1}
     for _ in range(nsites):
         new = {}
         for q, c in counts.items():
             for s in qloc:
-                k = _add(q, s)
-                new[k] = new.get(k, 0) + c
+                for k in sym.reduce(q, s):
+                    new[k] = new.get(k, 0) + c
         counts = new
     return counts
 
 
-def random_mps(sites, Q, M, rng, spread=2):
-    """Random N(0,1) MPS with charge Q, bond dims at most M: each bond keeps the
-    sectors reachable from both ends whose charge lies within `spread` of the
-    uniform profile l*Q/L, dims split evenly and capped by the number of
-    product states on either side."""
+def right_counts(sym, qloc, nsites, Q):
+    """Same for the bond left of the last nsites sites, coupled to Q at the end."""
+    counts = {tuple(Q): 1}
+    for _ in range(nsites):
+        new = {}
+        for q, c in counts.items():
+            for s in qloc:
+                for k in sym.preimages(q, s):
+                    new[k] = new.get(k, 0) + c
+        counts = new
+    return counts
+
+
+def random_mps(sites, Q, M, rng, spread=2, max_spin2=6):
+    """Random N(0,1) MPS with total quantum number Q, bond dims at most M:
+    each bond keeps the sectors reachable from both ends whose U(1) charges
+    lie within `spread` of the uniform profile l*Q/L (SU(2) components up to
+    2S = max_spin2), dims split evenly and capped by the counts on either side."""
     L = len(sites)
+    sym = sites[0].sym
     qloc = sites[0].qloc
-    vac = tuple(0 for _ in Q)
     bonds = []
     for l in range(L + 1):
-        cl = reachable_counts(qloc, l, vac)
-        cr = reachable_counts(qloc, L - l, vac)
-        centre = tuple(l * x / L for x in Q)
-        allowed = {q: min(c, cr[_sub(Q, q)]) for q, c in cl.items()
-                   if _sub(Q, q) in cr and all(abs(a - b) <= spread for a, b in zip(q, centre))}
+        cl = left_counts(sym, qloc, l)
+        cr = right_counts(sym, qloc, L - l, Q)
+        allowed = {}
+        for q, c in cl.items():
+            if q not in cr:
+                continue
+            ok = True
+            for i, k in enumerate(sym.kinds):
+                if k == models.U1_KIND and abs(q[i] - l * Q[i] / L) > spread:
+                    ok = False
+                if k == models.SU2_KIND and q[i] - 1 > max_spin2:
+                    ok = False
+            if ok:
+                allowed[q] = min(c, cr[q])
         share = max(1, M // max(1, len(allowed)))
         bonds.append({q: max(1, min(c, share)) for q, c in sorted(allowed.items())})
     A = []
@@ -71,42 +136,43 @@ def random_mps(sites, Q, M, rng, spread=2):
         for s in qloc:
             blocks = {}
             for qin, din in bonds[l].items():
-                qout = _add(qin, s)
-                if qout in bonds[l + 1]:
-                    blocks[(qin, qout)] = rng.standard_normal((din, bonds[l + 1][qout]))
+                for qout in sym.reduce(qin, s):
+                    if qout in bonds[l + 1]:
+                        blocks[(qin, qout)] = rng.standard_normal((din, bonds[l + 1][qout]))
             site.append(blocks)
         A.append(site)
     return A, bonds
 
 
-def right_normalize(A, bonds, qloc):
+def right_normalize(A, bonds, qloc, sym=None):
     """Right-to-left sweep of QR decompositions (Mps::rightSweep): every site
-    but the first becomes right-normalized."""
+    but the first becomes right-normalized,
+    sum_{s, qout} (deg qout / deg qin) A[s](qin,qout) A[s](qin,qout)^T = 1
+    (deg = coeff_dot; the weight is 1 for U(1))."""
     L = len(A)
+    sym = sym or models.Sym([models.U1_KIND] * len(qloc[0]))
     for l in range(L - 1, 0, -1):
-        # group by the left charge: matrix [qin] x [(s, qout)]
         newA = [dict() for _ in qloc]
         carry = {}
         for qin, din in bonds[l].items():
-            cols, parts = [], []
+            parts = []
             for s, qs in enumerate(qloc):
-                qout = _add(qin, qs)
-                if (qin, qout) in A[l][s]:
-                    parts.append((s, qout, A[l][s][(qin, qout)]))
+                for qout in sym.reduce(qin, qs):
+                    if (qin, qout) in A[l][s]:
+                        w = math.sqrt(sym.coeff_dot(qout) / sym.coeff_dot(qin))
+                        parts.append((s, qout, w, A[l][s][(qin, qout)]))
             if not parts:
                 continue
-            Mm = np.concatenate([p[2] for p in parts], axis=1)
+            Mm = np.concatenate([p[2] * p[3] for p in parts], axis=1)
             q, r = np.linalg.qr(Mm.T)  # Mm = r^T q^T
-            Bt = q.T  # rows: min(din, cols) orthonormal
-            k = Bt.shape[0]
+            Bt = q.T  # orthonormal rows
             c0 = 0
-            for (s, qout, blk) in parts:
-                newA[s][(qin, qout)] = Bt[:, c0:c0 + blk.shape[1]].copy()
+            for (s, qout, w, blk) in parts:
+                newA[s][(qin, qout)] = Bt[:, c0:c0 + blk.shape[1]] / w
                 c0 += blk.shape[1]
             carry[qin] = r.T  # din x k
         A[l] = newA
         bonds[l] = {q: carry[q].shape[1] for q in carry}
-        # absorb into the left neighbour
         for s in range(len(qloc)):
             for key in list(A[l - 1][s].keys()):
                 qin, qout = key
@@ -118,80 +184,103 @@ def right_normalize(A, bonds, qloc):
 
 
 def two_site_psi(pd, Al, Ar, qloc):
-    """contract_AA2: Psi[s][(qin, qout)] = sum_qm A_l[s1][(qin, qm)] A_r[s3][(qm, qout)]
-    laid out in the pivot's vector (all allowed blocks, zero where nothing contributes)."""
+    """contract_AA2: Psi[s1s3](qin, qout) = sum_qm coeff_twoSiteGate(qin, q1, qm,
+    q3, qout, qmerge) A_l[s1](qin, qm) A_r[s3](qm, qout), laid out in the
+    pivot's vector (all allowed blocks, zero where nothing contributes)."""
+    sym = pd.sym
     v = np.zeros(pd.vsize)
     off = 0
     for s, blocks in enumerate(pd.V):
-        _, s1, s3 = pd.basis[s]
+        qmerge, s1, s3 = pd.basis[s]
         for (qin, qout, r, c) in blocks:
-            qm = _add(qin, qloc[s1])
-            a = Al[s1].get((qin, qm))
-            b = Ar[s3].get((qm, qout))
-            if a is not None and b is not None:
-                v[off:off + r * c] = (a @ b).ravel(order="F")
+            acc = None
+            for qm in sym.reduce(qin, qloc[s1]):
+                a = Al[s1].get((qin, qm))
+                b = Ar[s3].get((qm, qout))
+                if a is None or b is None:
+                    continue
+                f = coeff_two_site_gate(sym, qin, qloc[s1], qm, qloc[s3], qout, qmerge)
+                if abs(f) < 1e-14:
+                    continue
+                t = f * (a @ b)
+                acc = t if acc is None else acc + t
+            if acc is not None:
+                v[off:off + r * c] = acc.ravel(order="F")
             off += r * c
     return v
 
 
 def split_psi(pd, v, qloc, M, eps_svd=1e-14, left_to_right=True):
-    """split_AA2 + truncateSVD for abelian symmetries: SVD per middle charge,
-    global selection of the M largest singular values above eps_svd.
-    Returns (A_l, A_r, truncation weight, entropy, bond)."""
-    # gather the two-site blocks by middle charge qm = qin + q(s1)
-    blocks = {}
+    """split_AA2 + truncateSVD: recouple the two-site blocks to the middle
+    sector Qm (coeff_splitAA), SVD per Qm, keep the M largest singular values
+    (multiplets) above eps_svd.  Returns (A_l, A_r, truncation weight =
+    sum of discarded deg(Qm) sigma^2, entropy, kept dims per Qm)."""
+    sym = pd.sym
+    clump = {}  # Qm -> {(row key, col key): block}
+    rdims, cdims = {}, {}
     off = 0
     for s, bl in enumerate(pd.V):
-        _, s1, s3 = pd.basis[s]
+        q12, sl, sr = pd.basis[s]
         for (qin, qout, r, c) in bl:
-            qm = _add(qin, qloc[s1])
-            blocks.setdefault(qm, []).append((s1, s3, qin, qout, v[off:off + r * c].reshape((r, c), order="F")))
+            blk = v[off:off + r * c].reshape((r, c), order="F")
             off += r * c
+            for Qm in sym.reduce(qin, qloc[sl]):
+                if qout not in sym.reduce(Qm, qloc[sr]):
+                    continue
+                f = coeff_split6(sym, qin, qout, q12, qloc[sr], qloc[sl], Qm)
+                if abs(f) < 1e-14:
+                    continue
+                key = ((sl, qin), (sr, qout))
+                d = clump.setdefault(Qm, {})
+                d[key] = f * blk if key not in d else d[key] + f * blk
+                rdims.setdefault(Qm, {})[(sl, qin)] = r
+                cdims.setdefault(Qm, {})[(sr, qout)] = c
     svd = {}
     allsv = []
-    for qm, parts in blocks.items():
-        rows = sorted({(p[0], p[2]) for p in parts})
-        cols = sorted({(p[1], p[3]) for p in parts})
-        rdim = {k: next(p[4].shape[0] for p in parts if (p[0], p[2]) == k) for k in rows}
-        cdim = {k: next(p[4].shape[1] for p in parts if (p[1], p[3]) == k) for k in cols}
+    for Qm, parts in clump.items():
         ro, o = {}, 0
-        for k in rows:
+        for k in sorted(rdims[Qm]):
             ro[k] = o
-            o += rdim[k]
+            o += rdims[Qm][k]
         nr = o
         co, o = {}, 0
-        for k in cols:
+        for k in sorted(cdims[Qm]):
             co[k] = o
-            o += cdim[k]
+            o += cdims[Qm][k]
         nc = o
         Mm = np.zeros((nr, nc))
-        for (s1, s3, qin, qout, blk) in parts:
-            Mm[ro[(s1, qin)]:ro[(s1, qin)] + blk.shape[0], co[(s3, qout)]:co[(s3, qout)] + blk.shape[1]] = blk
+        for (rk, ck), blk in parts.items():
+            Mm[ro[rk]:ro[rk] + blk.shape[0], co[ck]:co[ck] + blk.shape[1]] = blk
         U, S, Vt = np.linalg.svd(Mm, full_matrices=False)
-        svd[qm] = (U, S, Vt, ro, co, rdim, cdim)
-        allsv.extend((sv, qm, i) for i, sv in enumerate(S))
+        svd[Qm] = (U, S, Vt, ro, co)
+        allsv.extend((sv, Qm, i) for i, sv in enumerate(S))
     allsv.sort(key=lambda x: -x[0])
     keep = [x for x in allsv[:M] if x[0] > eps_svd]
     kept = {}
-    for (sv, qm, i) in keep:
-        kept[qm] = max(kept.get(qm, 0), i + 1)
-    trunc = float(sum(x[0] ** 2 for x in allsv[len(keep):]))
+    for (sv, Qm, i) in keep:
+        kept[Qm] = max(kept.get(Qm, 0), i + 1)
+    deg = lambda q: sym.coeff_dot(q)
+    trunc = float(sum(deg(x[1]) * x[0] ** 2 for x in allsv[len(keep):]))
     p = np.array([x[0] ** 2 for x in keep])
-    p = p / p.sum() if p.sum() > 0 else p
-    entropy = float(-(p[p > 0] * np.log(p[p > 0])).sum())
+    w = np.array([deg(x[1]) for x in keep])
+    tot = float((w * p).sum())
+    entropy = 0.0
+    if tot > 0:
+        p = p / tot
+        entropy = float(-(w[p > 0] * p[p > 0] * np.log(p[p > 0])).sum())
     Al = [dict() for _ in qloc]
     Ar = [dict() for _ in qloc]
-    for qm, k in kept.items():
-        U, S, Vt, ro, co, rdim, cdim = svd[qm]
+    for Qm, k in kept.items():
+        U, S, Vt, ro, co = svd[Qm]
         U, S, Vt = U[:, :k], S[:k], Vt[:k, :]
         if left_to_right:
             X, Y = U, S[:, None] * Vt
         else:
             X, Y = U * S[None, :], Vt
-        for (s1, qin), o in ro.items():
-            Al[s1][(qin, qm)] = X[o:o + rdim[(s1, qin)], :].copy()
-        for (s3, qout), o in co.items():
-            Ar[s3][(qm, qout)] = Y[:, o:o + cdim[(s3, qout)]].copy()
+        for (sl, qin), o in ro.items():
+            Al[sl][(qin, Qm)] = X[o:o + rdims[Qm][(sl, qin)], :].copy()
+        for (sr, qout), o in co.items():
+            Ar[sr][(Qm, qout)] = coeff_split3(sym, Qm, qout, qloc[sr]) * Y[:, o:o + cdims[Qm][(sr, qout)]]
     return Al, Ar, trunc, entropy, kept
 
 
@@ -236,14 +325,14 @@ def run_dmrg(model, L, M, sweeps, Q=None, seed=20261202, backend=None, params=No
     backend = backend or GpuBackend()
     sites = models.chain_mpo(model, L, **params)
     qloc = sites[0].qloc
-    nq = len(qloc[0])
+    sym = sites[0].sym
     if Q is None:
-        Q = (0,) if model == "HeisenbergU1" else (L, 0)
+        Q = {"HeisenbergU1": (0,), "HubbardU1xU1": (L, 0), "HeisenbergSU2": (1,), "HubbardSU2xU1": (1, L)}[model]
     Q = tuple(Q)
     rng = np.random.default_rng(seed)
     A, bonds = random_mps(sites, Q, M, rng)
-    A, bonds = right_normalize(A, bonds, qloc)
-    vac = tuple(0 for _ in range(nq))
+    A, bonds = right_normalize(A, bonds, qloc, sym)
+    vac = sym.vacuum()
     # environments: L[l] sits on bond l, R[l] on bond l
     Lenv = [None] * (L + 1)
     Renv = [None] * (L + 1)
@@ -269,9 +358,9 @@ def run_dmrg(model, L, M, sweeps, Q=None, seed=20261202, backend=None, params=No
                 for (qm, s1, s3) in pd.basis:
                     bl = []
                     for qin, din in sorted(bonds[l].items()):
-                        qout = _add(qin, qm)
-                        if qout in rdim:
-                            bl.append((qin, qout, din, rdim[qout]))
+                        for qout in sym.reduce(qin, qm):
+                            if qout in rdim:
+                                bl.append((qin, qout, din, rdim[qout]))
                     pd.V.append(bl)
                 v0 = two_site_psi(pd, A[l], A[l + 1], qloc)
                 if not np.any(v0):

@@ -55,6 +55,22 @@ class Sym:
         assert all(k == U1_KIND for k in self.kinds)
         return tuple(x + y for x, y in zip(a, b))
 
+    def preimages(self, qout, b):
+        """Every q with qout in q (x) b."""
+        comps = []
+        for i, k in enumerate(self.kinds):
+            if k == U1_KIND:
+                comps.append([qout[i] - b[i]])
+            else:
+                comps.append([x for x in range(abs(qout[i] - b[i]) + 1, qout[i] + b[i], 2) if x >= 1])
+        out = [()]
+        for c in comps:
+            out = [o + (x,) for o in out for x in c]
+        return out
+
+    def vacuum(self):
+        return tuple(1 if k == SU2_KIND else 0 for k in self.kinds)
+
     def coeff_dot(self, q):
         w = 1.0
         for i, k in enumerate(self.kinds):
