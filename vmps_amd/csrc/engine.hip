@@ -4,8 +4,9 @@
 //   stage C   out_o = sum_r (sum_j c_j T_j) . R_r    one problem per output block;
 //             the scalar combine U = sum c T is done while loading the MFMA operand
 //
-// Work unit = one 64-lane wavefront computing one 32 x 32 output tile with four
-// 16x16 MFMA accumulators.  Operands go from global memory straight to
+// Work unit = one 64-lane wavefront computing one E x E output tile (E = 16
+// for pivots whose blocks are all small, else 32) with (E/16)^2 16x16 MFMA
+// accumulators.  Operands go from global memory straight to
 // registers (every element of a wave's A and B tiles is used by exactly one
 // MFMA, so staging in LDS would buy no reuse); the tile descriptor is a single
 // load.  Every output element is owned by exactly one wave, so there are no
@@ -49,146 +50,233 @@ struct Mfma<float> {
   __device__ static inline int col(int lane, int v) { return 4 * (lane >> 4) + v; }
 };
 
-template <class T, class OutT>
-__device__ inline void store_tile(OutT* __restrict__ C, int M, int N, int m0, int n0, int lane,
-                                  const typename Mfma<T>::acc_t& c00, const typename Mfma<T>::acc_t& c01,
-                                  const typename Mfma<T>::acc_t& c10, const typename Mfma<T>::acc_t& c11) {
-  const int i0 = m0 + (lane & 15), i1 = i0 + 16;
+// 64-bit wave-uniform broadcast of lane `src`'s value.
+__device__ inline int64_t readlane64(int64_t x, int src) {
+  const int lo = __builtin_amdgcn_readlane((int)(uint32_t)(uint64_t)x, src);
+  const int hi = __builtin_amdgcn_readlane((int)(uint32_t)((uint64_t)x >> 32), src);
+  return (int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint64_t)(uint32_t)lo);
+}
+__device__ inline double readlane_d(double x, int src) { return __longlong_as_double(readlane64(__double_as_longlong(x), src)); }
+__device__ inline float readlane_f(float x, int src) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), src)); }
+template <class T>
+__device__ inline T readlane_t(T x, int src);
+template <>
+__device__ inline double readlane_t<double>(double x, int src) { return readlane_d(x, src); }
+template <>
+__device__ inline float readlane_t<float>(float x, int src) { return readlane_f(x, src); }
+
+// A wave computes an E x E output tile (E = 16 or 32) as S x S MFMA
+// accumulators of 16 x 16 (S = E / 16); sub-tiles past the block edge are
+// skipped with wave-uniform tests.
+template <class T, int S>
+struct WaveTile {
+  typedef typename Mfma<T>::acc_t acc_t;
+  acc_t c[S][S];
+  __device__ void zero() {
 #pragma unroll
-  for (int v = 0; v < 4; ++v) {
-    const int j0 = n0 + Mfma<T>::col(lane, v), j1 = j0 + 16;
-    if (j0 < N) {
-      if (i0 < M) C[(int64_t)j0 * M + i0] = (OutT)c00[v];
-      if (i1 < M) C[(int64_t)j0 * M + i1] = (OutT)c10[v];
-    }
-    if (j1 < N) {
-      if (i0 < M) C[(int64_t)j1 * M + i0] = (OutT)c01[v];
-      if (i1 < M) C[(int64_t)j1 * M + i1] = (OutT)c11[v];
+    for (int x = 0; x < S; ++x)
+#pragma unroll
+      for (int y = 0; y < S; ++y) c[x][y] = acc_t{0, 0, 0, 0};
+  }
+  // a[x][u]: A[i = m0 + 16x + lane%16][k = 4u + lane/16], b[y][u]: B[k][j = n0 + 16y + lane%16]
+  __device__ void mac(const T (&a)[S][4], const T (&b)[S][4], bool has_m1, bool has_n1) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int y = 0; y < S; ++y)
+#pragma unroll
+        for (int x = 0; x < S; ++x)
+          if ((x == 0 || has_m1) && (y == 0 || has_n1)) Mfma<T>::mac(b[y][u], a[x][u], c[x][y]);
+  }
+  template <class OutT>
+  __device__ void store(OutT* __restrict__ C, int M, int N, int m0, int n0, int lane) const {
+#pragma unroll
+    for (int x = 0; x < S; ++x) {
+      const int i = m0 + 16 * x + (lane & 15);
+      if (i >= M) continue;
+#pragma unroll
+      for (int y = 0; y < S; ++y)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int j = n0 + 16 * y + Mfma<T>::col(lane, v);
+          if (j < N) C[(int64_t)j * M + i] = (OutT)c[x][y][v];
+        }
     }
   }
-}
+};
 
-// Stage A: T (M x N, ld M) = L (M x K, ld M) . V (K x N, ld K), one tile per wave.
-template <class T>
+// Stage A: T (M x N, ld M) = L (M x K, ld M) . V (K x N, ld K), one E x E tile
+// per wave.  The loads of the next 16-deep K block are issued before the
+// MFMAs of the current one (two register buffers), so a wave waits for memory
+// once per tile rather than once per K block.
+template <class T, int E>
 __global__ __launch_bounds__(256) void k_stage_a(const TileA* __restrict__ tiles, int ntiles, const T* __restrict__ L,
                                                  const T* __restrict__ V, T* __restrict__ Tout) {
-  typedef typename Mfma<T>::acc_t acc_t;
+  constexpr int S = E / 16;
   const int w = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (w >= ntiles) return;
   const int lane = threadIdx.x & 63;
   const TileA t = tiles[w];
-  const int m0 = t.mt * kTile, n0 = t.nt * kTile;
+  const int m0 = t.mt * E, n0 = t.nt * E;
   const int li = lane & 15, lk = lane >> 4;
-  const int i0 = m0 + li, i1 = i0 + 16, j0 = n0 + li, j1 = j0 + 16;
-  const bool has_m1 = m0 + 16 < t.M, has_n1 = n0 + 16 < t.N;  // wave-uniform
+  const bool has_m1 = S > 1 && m0 + 16 < t.M, has_n1 = S > 1 && n0 + 16 < t.N;  // wave-uniform
   const T* __restrict__ A = L + t.a_off;
   const T* __restrict__ B = V + t.b_off;
-  acc_t c00 = {0, 0, 0, 0}, c01 = {0, 0, 0, 0}, c10 = {0, 0, 0, 0}, c11 = {0, 0, 0, 0};
-  for (int kb = 0; kb < t.K; kb += 16) {
-    T a0[4], a1[4], b0[4], b1[4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {  // all loads of the 16-deep block first
-      const int k = kb + 4 * u + lk;
-      const bool kin = k < t.K;
-      a0[u] = (kin && i0 < t.M) ? A[(int64_t)k * t.M + i0] : T(0);
-      a1[u] = (kin && i1 < t.M) ? A[(int64_t)k * t.M + i1] : T(0);
-      b0[u] = (kin && j0 < t.N) ? B[(int64_t)j0 * t.K + k] : T(0);
-      b1[u] = (kin && j1 < t.N) ? B[(int64_t)j1 * t.K + k] : T(0);
-    }
+  WaveTile<T, S> acc;
+  acc.zero();
+  auto load = [&](int kb, T (&a)[S][4], T (&b)[S][4]) {
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-      Mfma<T>::mac(b0[u], a0[u], c00);
-      if (has_m1) Mfma<T>::mac(b0[u], a1[u], c10);
-      if (has_n1) {
-        Mfma<T>::mac(b1[u], a0[u], c01);
-        if (has_m1) Mfma<T>::mac(b1[u], a1[u], c11);
+      const int k = kb + 4 * u + lk;
+      const bool kin = k < t.K;
+#pragma unroll
+      for (int x = 0; x < S; ++x) {
+        const int i = m0 + 16 * x + li, j = n0 + 16 * x + li;
+        a[x][u] = (kin && i < t.M) ? A[(int64_t)k * t.M + i] : T(0);
+        b[x][u] = (kin && j < t.N) ? B[(int64_t)j * t.K + k] : T(0);
       }
     }
+  };
+  T a0[S][4], b0[S][4], a1[S][4], b1[S][4];
+  load(0, a0, b0);
+  for (int kb = 0; kb < t.K; kb += 32) {
+    if (kb + 16 < t.K) load(kb + 16, a1, b1);
+    acc.mac(a0, b0, has_m1, has_n1);
+    if (kb + 16 >= t.K) break;
+    if (kb + 32 < t.K) load(kb + 32, a0, b0);
+    acc.mac(a1, b1, has_m1, has_n1);
   }
-  store_tile<T, T>(Tout + t.c_off, t.M, t.N, m0, n0, lane, c00, c01, c10, c11);
+  acc.store(Tout + t.c_off, t.M, t.N, m0, n0, lane);
 }
 
 // Stage C: Vout (M x N, ld M) = sum over chunks (sum_j c_j T_j)(M x K, ld M) . R (K x N, ld K).
-// One workgroup per output tile.  The iteration space (chunk, 16-deep K block)
-// is dealt round-robin to the 4 waves (split K), each K block issues all its
-// loads before its 16 MFMAs, and the 4 partial accumulators are summed through
-// LDS in wave order, so the result does not depend on timing.
-constexpr int kKU = 16;  // K depth per unrolled step (4 MFMA k-steps)
-
-template <class T, class OutT>
+// One workgroup per E x E output tile.  The iteration space (chunk, 16-deep K
+// block) is dealt round-robin to the 4 waves (split K).  Chunk descriptors and
+// each chunk's term list (at most kMaxTermsPerChunk) are fetched with one
+// load per lane and broadcast with readlane, so within a K block all loads
+// (R and every T_j) are independent and in flight together.  The 4 partial
+// accumulators are summed through LDS in wave order, so the result does not
+// depend on timing.
+template <class T, class OutT, int E>
 __global__ __launch_bounds__(256) void k_stage_c(const TileC* __restrict__ tiles, int ntiles,
                                                  const ChunkC* __restrict__ chunks, const TermC* __restrict__ terms,
                                                  const T* __restrict__ Tbuf, const T* __restrict__ R,
                                                  OutT* __restrict__ Vout) {
+  constexpr int S = E / 16;
   typedef typename Mfma<T>::acc_t acc_t;
-  __shared__ acc_t red[3][4][64];
+  __shared__ acc_t red[3][S * S][64];
   if ((int)blockIdx.x >= ntiles) return;  // workgroup-uniform
   const int wave = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
   const TileC t = tiles[blockIdx.x];
-  const int m0 = t.mt * kTile, n0 = t.nt * kTile;
+  const int m0 = t.mt * E, n0 = t.nt * E;
   const int li = lane & 15, lk = lane >> 4;
-  const int i0 = m0 + li, i1 = i0 + 16, j0 = n0 + li, j1 = j0 + 16;
-  const bool has_m1 = m0 + 16 < t.M, has_n1 = n0 + 16 < t.N;
-  const bool r0 = i0 < t.M, r1 = i1 < t.M, q0 = j0 < t.N, q1 = j1 < t.N;
-  acc_t c00 = {0, 0, 0, 0}, c01 = {0, 0, 0, 0}, c10 = {0, 0, 0, 0}, c11 = {0, 0, 0, 0};
+  const bool has_m1 = S > 1 && m0 + 16 < t.M, has_n1 = S > 1 && n0 + 16 < t.N;
+  WaveTile<T, S> acc;
+  acc.zero();
   int it = 0;
-  for (int c = t.chunk_begin; c < t.chunk_end; ++c) {
-    const ChunkC ch = chunks[c];
-    const T* __restrict__ B = R + ch.r_off;
-    for (int kb = 0; kb < ch.K; kb += kKU, ++it) {
-      if ((it & 3) != wave) continue;
-      T a0[4], a1[4], b0[4], b1[4];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const int k = kb + 4 * u + lk;
-        const bool kin = k < ch.K;
-        a0[u] = T(0);
-        a1[u] = T(0);
-        b0[u] = (kin && q0) ? B[(int64_t)j0 * ch.K + k] : T(0);
-        b1[u] = (kin && q1) ? B[(int64_t)j1 * ch.K + k] : T(0);
+  for (int g = t.chunk_begin; g < t.chunk_end; g += 64) {
+    const int nch = min(64, t.chunk_end - g);
+    int64_t my_r = 0;
+    int my_K = 0, my_tb = 0, my_te = 0;
+    if (lane < nch) {
+      const ChunkC cd = chunks[g + lane];
+      my_r = cd.r_off;
+      my_K = cd.K;
+      my_tb = cd.term_begin;
+      my_te = cd.term_end;
+    }
+    for (int ci = 0; ci < nch; ++ci) {
+      const int K = __builtin_amdgcn_readlane(my_K, ci);
+      const int nkb = (K + 15) >> 4;
+      // K blocks of this chunk handled by this wave: it + q with (it + q) % 4 == wave
+      const int first = (wave - (it & 3)) & 3;
+      if (first >= nkb) {
+        it += nkb;
+        continue;
       }
-      for (int j = ch.term_begin; j < ch.term_end; ++j) {
-        const TermC tm = terms[j];
-        const T cj = (T)tm.c;
-        const T* __restrict__ Tt = Tbuf + tm.t_off;
+      const T* __restrict__ B = R + readlane64(my_r, ci);
+      const int tb = __builtin_amdgcn_readlane(my_tb, ci);
+      const int nt = __builtin_amdgcn_readlane(my_te, ci) - tb;
+      int64_t my_toff = 0;
+      T my_c = T(0);
+      if (lane < nt) {
+        const TermC tm = terms[tb + lane];
+        my_toff = tm.t_off;
+        my_c = (T)tm.c;
+      }
+      for (int q = first; q < nkb; q += 4) {
+        const int kb = q * 16;
+        T a[S][4], b[S][4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
           const int k = kb + 4 * u + lk;
-          const bool kin = k < ch.K;
-          const T x0 = (kin && r0) ? Tt[(int64_t)k * t.M + i0] : T(0);
-          const T x1 = (kin && r1) ? Tt[(int64_t)k * t.M + i1] : T(0);
-          a0[u] += cj * x0;
-          a1[u] += cj * x1;
-        }
-      }
+          const bool kin = k < K;
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        Mfma<T>::mac(b0[u], a0[u], c00);
-        if (has_m1) Mfma<T>::mac(b0[u], a1[u], c10);
-        if (has_n1) {
-          Mfma<T>::mac(b1[u], a0[u], c01);
-          if (has_m1) Mfma<T>::mac(b1[u], a1[u], c11);
+          for (int y = 0; y < S; ++y) {
+            const int j = n0 + 16 * y + li;
+            b[y][u] = (kin && j < t.N) ? B[(int64_t)j * K + k] : T(0);
+          }
+#pragma unroll
+          for (int x = 0; x < S; ++x) a[x][u] = T(0);
         }
+        int jt = 0;
+        for (; jt + 1 < nt; jt += 2) {  // two terms per round, all loads first
+          const T* __restrict__ T0 = Tbuf + readlane64(my_toff, jt);
+          const T* __restrict__ T1 = Tbuf + readlane64(my_toff, jt + 1);
+          const T c0 = readlane_t<T>(my_c, jt), c1 = readlane_t<T>(my_c, jt + 1);
+          T x0[S][4], x1[S][4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const int k = kb + 4 * u + lk;
+            const bool kin = k < K;
+#pragma unroll
+            for (int x = 0; x < S; ++x) {
+              const int i = m0 + 16 * x + li;
+              const bool ok = kin && i < t.M;
+              x0[x][u] = ok ? T0[(int64_t)k * t.M + i] : T(0);
+              x1[x][u] = ok ? T1[(int64_t)k * t.M + i] : T(0);
+            }
+          }
+#pragma unroll
+          for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int x = 0; x < S; ++x) a[x][u] += c0 * x0[x][u] + c1 * x1[x][u];
+        }
+        if (jt < nt) {
+          const T* __restrict__ T0 = Tbuf + readlane64(my_toff, jt);
+          const T c0 = readlane_t<T>(my_c, jt);
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const int k = kb + 4 * u + lk;
+            const bool kin = k < K;
+#pragma unroll
+            for (int x = 0; x < S; ++x) {
+              const int i = m0 + 16 * x + li;
+              a[x][u] += c0 * ((kin && i < t.M) ? T0[(int64_t)k * t.M + i] : T(0));
+            }
+          }
+        }
+        acc.mac(a, b, has_m1, has_n1);
       }
+      it += nkb;
     }
   }
   if (wave > 0) {
-    red[wave - 1][0][lane] = c00;
-    red[wave - 1][1][lane] = c01;
-    red[wave - 1][2][lane] = c10;
-    red[wave - 1][3][lane] = c11;
+#pragma unroll
+    for (int x = 0; x < S; ++x)
+#pragma unroll
+      for (int y = 0; y < S; ++y) red[wave - 1][x * S + y][lane] = acc.c[x][y];
   }
   __syncthreads();
   if (wave == 0) {
 #pragma unroll
-    for (int w = 0; w < 3; ++w) {
-      c00 += red[w][0][lane];
-      c01 += red[w][1][lane];
-      c10 += red[w][2][lane];
-      c11 += red[w][3][lane];
-    }
-    store_tile<T, OutT>(Vout + t.c_off, t.M, t.N, m0, n0, lane, c00, c01, c10, c11);
+    for (int w = 0; w < 3; ++w)
+#pragma unroll
+      for (int x = 0; x < S; ++x)
+#pragma unroll
+        for (int y = 0; y < S; ++y) acc.c[x][y] += red[w][x * S + y][lane];
+    acc.store(Vout + t.c_off, t.M, t.N, m0, n0, lane);
   }
 }
 
@@ -299,8 +387,8 @@ void DevPlan::upload(vmps_precision precision) {
     VMPS_HIP_CHECK(hipMemsetAsync(vin.p, 0, std::max<int64_t>(1, H.ysize) * sizeof(double), stream));
     VMPS_HIP_CHECK(hipMemsetAsync(vout.p, 0, std::max<int64_t>(1, H.osize) * sizeof(double), stream));
   }
-  tilesA.upload(H.tilesA, stream);
-  tilesC.upload(H.tilesC, stream);
+  tilesA.upload(H.tilesA_w, stream);  // FP64 / FP32 wave kernels (edge H.wave_edge)
+  tilesC.upload(H.tilesC_w, stream);
   chunksC.upload(H.chunksC, stream);
   termsC.upload(H.termsC, stream);
   seg_off.upload(H.seg_off, stream);
@@ -330,22 +418,42 @@ void DevPlan::ensure_precision_buffers() {
 void DevPlan::launch_stage(int stage, const double* d_vin, double* d_vout, hipStream_t s) {
   const int nA = (int)tilesA.n, nC = (int)tilesC.n;
   if (prec == VMPS_PREC_FP64) {
-    if (stage == 0 && nA)
-      hipLaunchKernelGGL(k_stage_a<double>, dim3(grid_for(nA)), dim3(256), 0, s, tilesA.p, nA, L.p, d_vin, T.p);
-    if (stage == 2) {
-      if (nC)
-        hipLaunchKernelGGL((k_stage_c<double, double>), dim3(nC), dim3(256), 0, s, tilesC.p, nC, chunksC.p,
-                         termsC.p, T.p, R.p, d_vout);
+    if (stage == 0 && nA) {
+      if (H.wave_edge == 16)
+        hipLaunchKernelGGL((k_stage_a<double, 16>), dim3(grid_for(nA)), dim3(256), 0, s, tilesA.p, nA, L.p, d_vin,
+                           T.p);
+      else
+        hipLaunchKernelGGL((k_stage_a<double, 32>), dim3(grid_for(nA)), dim3(256), 0, s, tilesA.p, nA, L.p, d_vin,
+                           T.p);
+    }
+    if (stage == 2 && nC) {
+      if (H.wave_edge == 16)
+        hipLaunchKernelGGL((k_stage_c<double, double, 16>), dim3(nC), dim3(256), 0, s, tilesC.p, nC, chunksC.p,
+                           termsC.p, T.p, R.p, d_vout);
+      else
+        hipLaunchKernelGGL((k_stage_c<double, double, 32>), dim3(nC), dim3(256), 0, s, tilesC.p, nC, chunksC.p,
+                           termsC.p, T.p, R.p, d_vout);
     }
   } else if (prec == VMPS_PREC_FP32) {
     if (stage == 0) {
       if (H.ysize) hipLaunchKernelGGL(k_f64_to_f32, dim3(1024), dim3(256), 0, s, d_vin, V32.p, (int64_t)H.ysize);
-      if (nA)
-        hipLaunchKernelGGL(k_stage_a<float>, dim3(grid_for(nA)), dim3(256), 0, s, tilesA.p, nA, L32.p, V32.p, T32.p);
+      if (nA) {
+        if (H.wave_edge == 16)
+          hipLaunchKernelGGL((k_stage_a<float, 16>), dim3(grid_for(nA)), dim3(256), 0, s, tilesA.p, nA, L32.p, V32.p,
+                             T32.p);
+        else
+          hipLaunchKernelGGL((k_stage_a<float, 32>), dim3(grid_for(nA)), dim3(256), 0, s, tilesA.p, nA, L32.p, V32.p,
+                             T32.p);
+      }
     }
-    if (stage == 2 && nC)
-      hipLaunchKernelGGL((k_stage_c<float, double>), dim3(nC), dim3(256), 0, s, tilesC.p, nC, chunksC.p, termsC.p,
-                         T32.p, R32.p, d_vout);
+    if (stage == 2 && nC) {
+      if (H.wave_edge == 16)
+        hipLaunchKernelGGL((k_stage_c<float, double, 16>), dim3(nC), dim3(256), 0, s, tilesC.p, nC, chunksC.p,
+                           termsC.p, T32.p, R32.p, d_vout);
+      else
+        hipLaunchKernelGGL((k_stage_c<float, double, 32>), dim3(nC), dim3(256), 0, s, tilesC.p, nC, chunksC.p,
+                           termsC.p, T32.p, R32.p, d_vout);
+    }
   } else if (prec == VMPS_PREC_BF16X3) {
     launch_stage_bf16x3(stage, d_vin, d_vout, s);
   } else {

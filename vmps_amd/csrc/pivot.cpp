@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <stdexcept>
 #include <unordered_map>
@@ -511,19 +512,20 @@ void validate_host_plan(const HostPlan& H) {
 
 // Flatten the stage tables into per-tile work lists.  Stage B (U = sum c T)
 // is folded into stage C: each stage C chunk carries the list of (T, c) that
-// forms its U operand.  Tiles are ordered by decreasing work so the largest
-// start first.
-void build_tiles(HostPlan& H) {
-  H.tilesA.clear();
-  H.tilesC.clear();
-  H.chunksC.clear();
-  H.termsC.clear();
+// forms its U operand; a chunk with more than kMaxTermsPerChunk terms is split
+// into several chunks on the same R block (U . R is linear in U).  Tiles are
+// ordered by decreasing work so the largest start first.
+namespace {
+void make_tiles(const HostPlan& H, int edge, const std::vector<std::pair<int32_t, int32_t>>& crange,
+                std::vector<TileA>& outA, std::vector<TileC>& outC) {
+  outA.clear();
+  outC.clear();
   std::vector<std::pair<int64_t, size_t>> order;
   for (size_t p = 0; p < H.probA.size(); ++p) {
     const GemmProb& pr = H.probA[p];
     const GemmChunk& ch = H.chunkA[pr.chunk_begin];
-    for (int i = 0; i * kTile < pr.M; ++i)
-      for (int j = 0; j * kTile < pr.N; ++j) {
+    for (int i = 0; i * edge < pr.M; ++i)
+      for (int j = 0; j * edge < pr.N; ++j) {
         if (i > 32767 || j > 32767) throw VmpsError(VMPS_ERR_ARG, "block dimension too large for the tile index");
         TileA t;
         t.a_off = ch.a_off;
@@ -534,67 +536,110 @@ void build_tiles(HostPlan& H) {
         t.K = ch.K;
         t.mt = (int16_t)i;
         t.nt = (int16_t)j;
-        H.tilesA.push_back(t);
+        outA.push_back(t);
       }
   }
-  order.clear();
-  for (size_t i = 0; i < H.tilesA.size(); ++i) order.push_back({-(int64_t)H.tilesA[i].K, i});
+  for (size_t i = 0; i < outA.size(); ++i) order.push_back({-(int64_t)outA[i].K, i});
   std::stable_sort(order.begin(), order.end());
   {
     std::vector<TileA> sorted;
-    for (auto& o : order) sorted.push_back(H.tilesA[o.second]);
-    H.tilesA.swap(sorted);
-  }
-  for (size_t j = 0; j < H.comb_t.size(); ++j) H.termsC.push_back({H.T_off[H.comb_t[j]], H.comb_c[j]});
-  for (const GemmChunk& ch : H.chunkC) {
-    const CombineList& cl = H.comb[ch.pad];
-    ChunkC c;
-    c.r_off = ch.b_off;
-    c.K = ch.K;
-    c.term_begin = cl.begin;
-    c.term_end = cl.end;
-    c.pad = 0;
-    H.chunksC.push_back(c);
+    for (auto& o : order) sorted.push_back(outA[o.second]);
+    outA.swap(sorted);
   }
   std::vector<int64_t> cost;
-  for (const GemmProb& pr : H.probC) {
+  for (size_t p = 0; p < H.probC.size(); ++p) {
+    const GemmProb& pr = H.probC[p];
     int64_t w = 0;
-    for (int32_t c = pr.chunk_begin; c < pr.chunk_end; ++c)
+    for (int32_t c = crange[p].first; c < crange[p].second; ++c)
       w += (int64_t)H.chunksC[c].K * (1 + H.chunksC[c].term_end - H.chunksC[c].term_begin);
-    for (int i = 0; i * kTile < pr.M; ++i)
-      for (int j = 0; j * kTile < pr.N; ++j) {
+    for (int i = 0; i * edge < pr.M; ++i)
+      for (int j = 0; j * edge < pr.N; ++j) {
         if (i > 32767 || j > 32767) throw VmpsError(VMPS_ERR_ARG, "block dimension too large for the tile index");
         TileC t;
         t.c_off = pr.c_off;
         t.M = pr.M;
         t.N = pr.N;
-        t.chunk_begin = pr.chunk_begin;
-        t.chunk_end = pr.chunk_end;
+        t.chunk_begin = crange[p].first;
+        t.chunk_end = crange[p].second;
         t.mt = (int16_t)i;
         t.nt = (int16_t)j;
         t.pad = 0;
-        H.tilesC.push_back(t);
+        outC.push_back(t);
         cost.push_back(w);
       }
   }
   order.clear();
-  for (size_t i = 0; i < H.tilesC.size(); ++i) order.push_back({-cost[i], i});
+  for (size_t i = 0; i < outC.size(); ++i) order.push_back({-cost[i], i});
   std::stable_sort(order.begin(), order.end());
   {
     std::vector<TileC> sorted;
-    for (auto& o : order) sorted.push_back(H.tilesC[o.second]);
-    H.tilesC.swap(sorted);
+    for (auto& o : order) sorted.push_back(outC[o.second]);
+    outC.swap(sorted);
+  }
+}
+
+int choose_wave_edge(const HostPlan& H) {
+  if (const char* e = std::getenv("VMPS_WAVE_TILE")) {
+    const int v = std::atoi(e);
+    if (v == 16 || v == 32) return v;
+  }
+  return H.stats.max_block_dim <= kSmallTileMaxDim ? 16 : 32;
+}
+}  // namespace
+
+void build_tiles(HostPlan& H) {
+  H.chunksC.clear();
+  H.termsC.clear();
+  for (size_t j = 0; j < H.comb_t.size(); ++j) H.termsC.push_back({H.T_off[H.comb_t[j]], H.comb_c[j]});
+  std::vector<std::pair<int32_t, int32_t>> crange(H.probC.size());
+  for (size_t p = 0; p < H.probC.size(); ++p) {
+    const GemmProb& pr = H.probC[p];
+    crange[p].first = (int32_t)H.chunksC.size();
+    for (int32_t k = pr.chunk_begin; k < pr.chunk_end; ++k) {
+      const GemmChunk& ch = H.chunkC[k];
+      const CombineList& cl = H.comb[ch.pad];
+      for (int32_t j0 = cl.begin; j0 < cl.end; j0 += kMaxTermsPerChunk) {
+        ChunkC c;
+        c.r_off = ch.b_off;
+        c.K = ch.K;
+        c.term_begin = j0;
+        c.term_end = std::min(cl.end, j0 + kMaxTermsPerChunk);
+        c.pad = 0;
+        H.chunksC.push_back(c);
+      }
+    }
+    crange[p].second = (int32_t)H.chunksC.size();
+  }
+  make_tiles(H, kTile, crange, H.tilesA, H.tilesC);
+  H.wave_edge = choose_wave_edge(H);
+  if (H.wave_edge == kTile) {
+    H.tilesA_w = H.tilesA;
+    H.tilesC_w = H.tilesC;
+  } else {
+    make_tiles(H, H.wave_edge, crange, H.tilesA_w, H.tilesC_w);
   }
   // bounds of the fused stage C: every T read stays inside T
-  for (const ChunkC& c : H.chunksC)
+  for (const ChunkC& c : H.chunksC) {
+    if (c.term_end - c.term_begin > kMaxTermsPerChunk || c.term_end < c.term_begin)
+      throw VmpsError(VMPS_ERR_INTERNAL, "stage C: chunk term list too long");
     for (int32_t j = c.term_begin; j < c.term_end; ++j)
       if (j < 0 || j >= (int32_t)H.termsC.size() || H.termsC[j].t_off < 0)
         throw VmpsError(VMPS_ERR_INTERNAL, "stage C: term reference out of range");
-  for (const TileC& t : H.tilesC)
-    for (int32_t c = t.chunk_begin; c < t.chunk_end; ++c)
-      for (int32_t j = H.chunksC[c].term_begin; j < H.chunksC[c].term_end; ++j)
-        if (H.termsC[j].t_off + (int64_t)t.M * H.chunksC[c].K > H.T_size)
-          throw VmpsError(VMPS_ERR_INTERNAL, "stage C: T block out of range");
+  }
+  for (const auto* list : {&H.tilesC, &H.tilesC_w})
+    for (const TileC& t : *list)
+      for (int32_t c = t.chunk_begin; c < t.chunk_end; ++c) {
+        if (H.chunksC[c].r_off < 0 || H.chunksC[c].r_off + (int64_t)H.chunksC[c].K * t.N > (int64_t)H.Rdata.size())
+          throw VmpsError(VMPS_ERR_INTERNAL, "stage C: R block out of range");
+        for (int32_t j = H.chunksC[c].term_begin; j < H.chunksC[c].term_end; ++j)
+          if (H.termsC[j].t_off + (int64_t)t.M * H.chunksC[c].K > H.T_size)
+            throw VmpsError(VMPS_ERR_INTERNAL, "stage C: T block out of range");
+      }
+  for (const auto* list : {&H.tilesA, &H.tilesA_w})
+    for (const TileA& t : *list)
+      if (t.a_off < 0 || t.a_off + (int64_t)t.M * t.K > (int64_t)H.Ldata.size() || t.b_off < 0 ||
+          t.b_off + (int64_t)t.K * t.N > H.ysize || t.c_off < 0 || t.c_off + (int64_t)t.M * t.N > H.T_size)
+        throw VmpsError(VMPS_ERR_INTERNAL, "stage A: tile operand out of range");
 }
 
 void triples_from_pivot(const Pivot& P, TripleSet& S) {

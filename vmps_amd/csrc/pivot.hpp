@@ -184,7 +184,12 @@ struct Stats {
 };
 
 // Flattened per-tile descriptors of the device kernels (one load per tile).
-constexpr int kTile = 32;  // output tile edge of one wavefront
+constexpr int kTile = 32;  // output tile edge of one wavefront (split-BF16 wave kernels)
+// The FP64 / FP32 wave kernels use 16 x 16 tiles (one MFMA accumulator) when
+// every block is at most this large, so the MFMA work of the few big blocks
+// of a small-block pivot spreads over 4x more SIMDs; 32 x 32 tiles otherwise.
+constexpr int kSmallTileMaxDim = 64;
+constexpr int kMaxTermsPerChunk = 64;  // stage C loads a chunk's term list with one load per lane
 struct TileA {             // T(M x N, ld M) tile = L(M x K, ld M) . V(K x N, ld K)
   int64_t a_off, b_off, c_off;
   int32_t M, N, K;
@@ -287,9 +292,14 @@ struct HostPlan {
   std::vector<BlockDim> Lblk, Rblk, Vblk;  // blocks of Ldata, Rdata and the input vector
   int64_t vsize = 0;                 // output vector size (== input size for HxV plans)
   int64_t ysize = 0, osize = 0;      // input / output vector sizes
-  // device work lists (built from the tables below by build_tiles)
+  // device work lists (built from the tables below by build_tiles): tiles of
+  // edge kTile (from which the split-BF16 lists derive) and tiles of edge
+  // wave_edge for the FP64 / FP32 kernels (the same lists when wave_edge == kTile)
   std::vector<TileA> tilesA;
   std::vector<TileC> tilesC;
+  int wave_edge = kTile;
+  std::vector<TileA> tilesA_w;
+  std::vector<TileC> tilesC_w;
   std::vector<ChunkC> chunksC;
   std::vector<TermC> termsC;
   std::vector<GemmProb> probA;       // stage A: one chunk each (L . V)
