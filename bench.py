@@ -1,15 +1,21 @@
 """Benchmark: effective-Hamiltonian application (HxV) on one MI355X.
 
-One step = one HxV of the two-site pivot of the workload on device-resident
-vectors.  Default workload: HubbardU1xU1 chain, L=48, half filling, U=8, t=1,
-M=512, D_W=6, pivot in the middle of the chain with random N(0,1) environment
-and wavefunction blocks (synthetic data).  The metric value is the algorithmic
+One step = one HxV of the workload's two-site pivot (or of every pivot of a
+batched workload, in one launch per stage) on device-resident vectors.
+Default workload: HubbardU1xU1 chain, L=48, half filling, U=8, t=1, M=512,
+D_W=6, pivot in the middle of the chain with random N(0,1) environment and
+wavefunction blocks (synthetic data).  The metric value is the algorithmic
 rate F_alg / t in GFLOP/s, where F_alg counts the staged contraction
 (distinct L.V products plus distinct U.R products).
+
+Beside the HxV line the default run also times full two-site DMRG sweeps of
+the HeisenbergU1 L=32 M=64 chain, FP64 and mixed (split-BF16x3 Lanczos refined
+in FP64), and reports the energy difference (`sweeps` key).
 
 Prints one JSON line.  --gpus N with N > 1 is not supported (one GPU).
 """
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -31,39 +37,68 @@ def host_cores():
         return os.cpu_count() or 1
 
 
-def cpu_baseline(pd, vin, flops_alg, budget_s, threads):
+def cpu_baseline(pds, vins, falgs, budget_s, threads):
     """Time the oracle (CPU FP64 restatement of the reference loop) on the
-    same pivot: repeated HxV calls until about budget_s seconds have passed."""
-    import ctypes
+    same pivots: HxV calls over the pivots in turn until about budget_s
+    seconds have passed.  Returns the rate over the work actually done and
+    the oracle's outputs for the pivots it reached."""
     import oracle
-    lib = oracle.load()
     from vmps_amd import synth
-    h = synth.feed(lib, "oracle_", pd, oracle.check)
-    oracle.check(lib.oracle_pivot2_precalc(h))
-    out = np.zeros_like(vin)
+    lib = oracle.load()
     P = ctypes.POINTER(ctypes.c_double)
-    calls = 0
-    t0 = time.perf_counter()
+    outs, flops_done, flops_ref_done, calls = {}, 0.0, 0.0, 0
+    handles, frefs = {}, {}
+    t_used = 0.0
+    i = 0
     while True:
-        oracle.check(lib.oracle_hxv_omp(h, vin.ctypes.data_as(P), out.ctypes.data_as(P), threads))
+        n = i % len(pds)
+        pd, vin, falg = pds[n], vins[n], falgs[n]
+        if n not in handles:  # each pivot is built once (the build is not timed)
+            h = synth.feed(lib, "oracle_", pd, oracle.check)
+            oracle.check(lib.oracle_pivot2_precalc(h))
+            fref = ctypes.c_double()
+            oracle.check(lib.oracle_flops_ref(h, ctypes.byref(fref)))
+            handles[n], frefs[n] = h, fref.value
+        out = np.zeros_like(vin)
+        t0 = time.perf_counter()
+        oracle.check(lib.oracle_hxv_omp(handles[n], vin.ctypes.data_as(P), out.ctypes.data_as(P), threads))
+        t_used += time.perf_counter() - t0
+        outs.setdefault(n, out)
+        flops_done += falg
+        flops_ref_done += frefs[n]
         calls += 1
-        if time.perf_counter() - t0 >= budget_s and calls >= 2:
+        i += 1
+        if t_used >= budget_s and calls >= 2:
             break
-    dt = (time.perf_counter() - t0) / calls
-    fref = ctypes.c_double()
-    oracle.check(lib.oracle_flops_ref(h, ctypes.byref(fref)))
-    lib.oracle_pivot2_destroy(h)
+    for h in handles.values():
+        lib.oracle_pivot2_destroy(h)
     return {
-        "value": flops_alg / dt / 1e9,
+        "value": flops_done / t_used / 1e9,
         "unit": "GFLOP/s",
         "cores": threads,
         "kind": "port",
         "sample": "%d HxV calls of the oracle (literal per-term (L.V).R loop, OpenMP over output blocks, %s) "
-                  "on the same pivot; %.3f s per call; literal loop rate %.2f GFLOP/s"
-                  % (calls, lib.variant, dt, fref.value / dt / 1e9),
-        "ms_per_hxv": dt * 1e3,
-        "out": out,
-    }
+                  "over %d of the workload's %d pivot(s); %.3f s of CPU time; literal loop rate %.2f GFLOP/s"
+                  % (calls, lib.variant, min(calls, len(pds)), len(pds), t_used, flops_ref_done / t_used / 1e9),
+        "ms_per_hxv": t_used / calls * 1e3,
+    }, outs
+
+
+def sweep_bench(args, model="HeisenbergU1", L=32, M=64, sweeps=4):
+    """Full two-site DMRG sweeps of a chain from the same random MPS, FP64 and
+    mixed (BF16x3 Lanczos refined in FP64): energies, the energy difference
+    and the wall time per sweep."""
+    from vmps_amd import dmrg
+    params = {"t": 1.0, "U": 8.0} if "Hubbard" in model else {"J": 1.0}
+    out = {"model": model, "L": L, "M": M, "sweeps": sweeps}
+    for prec in ("fp64", "bf16x3"):
+        be = dmrg.GpuBackend(precision=prec, tol_eigval=1e-10, tol_state=1e-6)
+        r = dmrg.run_dmrg(model, L, M, sweeps, seed=args.seed, backend=be, params=params)
+        out[prec] = {"energy": r["energy"], "sweep_seconds": r["sweep_seconds"], "hxv_calls": be.hxv_calls,
+                     "mixed_solves": be.mixed_solves, "max_truncation": max(r["truncation"][-(L - 1):])}
+    out["dE_mixed_vs_fp64"] = out["bf16x3"]["energy"] - out["fp64"]["energy"]
+    out["sweep_speedup_mixed_vs_fp64"] = out["fp64"]["sweep_seconds"][-1] / out["bf16x3"]["sweep_seconds"][-1]
+    return out
 
 
 def main():
@@ -76,7 +111,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=20261202)
-    ap.add_argument("--sweeps", type=int, default=0, help="also run this many DMRG sweeps of the chain")
+    ap.add_argument("--sweeps", type=int, default=4, help="DMRG sweeps of the HeisenbergU1 L=32 M=64 chain (0: none)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="only run 5 device copies of this many MiB (reference kernel for the HBM byte counters)")
     args = ap.parse_args()
@@ -99,13 +134,19 @@ def main():
 
     import torch
     from vmps_amd import _lib, synth
-    from vmps_amd.pivot import PivotMatrix2
+    from vmps_amd.pivot import PivotBatch, PivotMatrix2
 
     assert torch.cuda.is_available(), "no HIP device"
-    pd = synth.make_workload(args.workload, seed=args.seed)
-    H = PivotMatrix2(pd, precision=args.precision)
+    batched = args.workload in synth.BATCHES
+    if batched:
+        pds = synth.make_batch(args.workload, seed=args.seed)
+        H = PivotBatch(pds, precision=args.precision)
+    else:
+        pds = [synth.make_workload(args.workload, seed=args.seed)]
+        H = PivotMatrix2(pds[0], precision=args.precision)
     st = H.stats()
-    vin_h = synth.random_vector(pd, args.seed + 1)
+    vins = [synth.random_vector(pd, args.seed + 1 + i) for i, pd in enumerate(pds)]
+    vin_h = np.concatenate(vins)
     vin = torch.from_numpy(vin_h).to("cuda")
     vout = torch.empty_like(vin)
     lib = H._lib
@@ -145,8 +186,12 @@ def main():
     except (OSError, ValueError):
         pass
 
+    desc = (("%d independent two-site pivots (HeisenbergU1 / HubbardU1xU1 chains at random bonds, "
+             "M log-uniform 128..2048), one plan, one launch per stage" % len(pds)) if batched else
+            "two-site, middle of chain, random N(0,1) environments and wavefunction")
     res = {
-        "metric": "DMRG HxV GFLOP/s (F_alg / t), plain FP64 GPU",
+        "metric": "DMRG HxV GFLOP/s (F_alg / t), %s GPU" % ("plain FP64" if args.precision == "fp64" else
+                                                           "mixed " + args.precision),
         "value": round(gflops, 3),
         "unit": "GFLOP/s",
         "n_gpus": 1,
@@ -160,12 +205,10 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": args.workload,
-            "model": "HubbardU1xU1" if "hubbard" in args.workload else "HeisenbergU1",
-            "pivot": "two-site, middle of chain, random N(0,1) environments and wavefunction",
-            "vsize": int(pd.vsize),
-            "bond_sectors": [len(pd.left), len(pd.right)],
-            "max_sector_dim": max(d for _, d in pd.left + pd.right),
-            "D_W": pd.W12.dw_in,
+            "pivot": desc,
+            "n_pivots": len(pds),
+            "vsize": int(H.vsize),
+            "max_block_dim": st["max_block_dim"],
             "n_terms": st["n_terms"], "n_T": st["n_T"], "n_U": st["n_U"],
             "flops_alg": st["flops_alg"], "flops_ref": st["flops_ref"],
             "parallelism": "single GPU",
@@ -183,32 +226,36 @@ def main():
             "algorithmic_bytes_hxv": st["bytes_alg_fp64"],
         },
     }
+    if not batched:
+        pd = pds[0]
+        res["config"].update({
+            "model": "HubbardU1xU1" if "hubbard" in args.workload else "HeisenbergU1",
+            "bond_sectors": [len(pd.left), len(pd.right)],
+            "D_W": pd.W12.dw_in,
+        })
     if not args.no_cpu:
-        cb = cpu_baseline(pd, vin_h, st["flops_alg"], args.cpu_seconds, host_cores())
-        ref = cb.pop("out")
+        # per-pivot algorithmic flops for the CPU rate (each pivot's own plan statistics)
+        if batched:
+            falgs = []
+            for pd in pds:
+                h = synth.feed(lib, "vmps_", pd, _lib.check)
+                _lib.check(lib.vmps_pivot2_precalc(h))
+                s2 = _lib.PlanStats()
+                _lib.check(lib.vmps_pivot2_stats(h, ctypes.byref(s2)))
+                lib.vmps_pivot2_destroy(h)
+                falgs.append(s2.flops_alg)
+        else:
+            falgs = [st["flops_alg"]]
+        cb, outs = cpu_baseline(pds, vins, falgs, args.cpu_seconds, host_cores())
         res["cpu_baseline"] = cb
-        res["parity_rel_err_vs_oracle"] = float(np.linalg.norm(out_gpu - ref) / np.linalg.norm(ref))
+        off = np.cumsum([0] + [pd.vsize for pd in pds])
+        errs = [float(np.linalg.norm(out_gpu[off[i]:off[i + 1]] - ref) / np.linalg.norm(ref))
+                for i, ref in sorted(outs.items())]
+        res["parity_rel_err_vs_oracle"] = max(errs)
+        res["parity_pivots_checked"] = len(errs)
     if args.sweeps > 0:
-        res["sweeps"] = sweep_bench(args)
+        res["sweeps"] = sweep_bench(args, sweeps=args.sweeps)
     print(json.dumps(res))
-
-
-def sweep_bench(args):
-    """Full two-site DMRG sweeps of the workload's chain from the same random
-    MPS, FP64 and mixed (BF16x3 Lanczos refined in FP64): energies, the
-    energy difference and the wall time per sweep."""
-    from vmps_amd import dmrg, synth
-    cfg = synth.WORKLOADS[args.workload]
-    params = {"t": 1.0, "U": 8.0} if cfg["model"] == "HubbardU1xU1" else {"J": 1.0}
-    out = {"model": cfg["model"], "L": cfg["L"], "M": cfg["M"], "sweeps": args.sweeps}
-    for prec in ("fp64", "bf16x3"):
-        be = dmrg.GpuBackend(precision=prec, tol_eigval=1e-10, tol_state=1e-6)
-        r = dmrg.run_dmrg(cfg["model"], cfg["L"], cfg["M"], args.sweeps, seed=args.seed, backend=be, params=params)
-        out[prec] = {"energy": r["energy"], "energies": r["energies"], "sweep_seconds": r["sweep_seconds"],
-                     "hxv_calls": be.hxv_calls, "max_truncation": max(r["truncation"][-(cfg["L"] - 1):])}
-    out["dE_mixed_vs_fp64"] = out["bf16x3"]["energy"] - out["fp64"]["energy"]
-    out["sweep_speedup_mixed_vs_fp64"] = out["fp64"]["sweep_seconds"][-1] / out["bf16x3"]["sweep_seconds"][-1]
-    return out
 
 
 if __name__ == "__main__":

@@ -167,6 +167,15 @@ vmps_status vmps_pivot2_stats(const vmps_pivot2* p, vmps_plan_stats* st);
 /* Flatten the pivot into the staged block-triple tables and upload L, R and
  * the tables to the current HIP device.  The pivot may be destroyed after. */
 vmps_status vmps_plan_create(const vmps_pivot2* p, vmps_precision prec, vmps_plan** out);
+/* Several independent pivots in one plan, one launch per stage for all of
+ * them (the batched HxV of VUMPS cell sites, VumpsSolver.h:1168-1211, run
+ * there one pivot per OpenMP thread).  The plan's vectors are the pivots'
+ * vectors concatenated: pivot i owns elements [offsets[i], offsets[i+1]);
+ * offsets (n+1 entries) may be NULL.  vmps_hxv / vmps_hxv_device apply every
+ * pivot at once; vmps_dot sums over all of them.  Projectors are not
+ * supported here. */
+vmps_status vmps_plan_create_batch(const vmps_pivot2* const* pivots, int n, vmps_precision prec, vmps_plan** out,
+                                   int64_t* offsets);
 void vmps_plan_destroy(vmps_plan* plan);
 vmps_status vmps_plan_stats_get(const vmps_plan* plan, vmps_plan_stats* st);
 vmps_status vmps_plan_set_precision(vmps_plan* plan, vmps_precision prec);

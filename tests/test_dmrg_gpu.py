@@ -37,7 +37,7 @@ def test_mixed_precision_dmrg_energy(prec):
     r64 = dmrg.run_dmrg("HubbardU1xU1", 12, 128, 3, backend=dmrg.GpuBackend(tol_eigval=1e-12, tol_state=1e-7),
                         params=dict(t=1.0, U=8.0))
     rmx = dmrg.run_dmrg("HubbardU1xU1", 12, 128, 3,
-                        backend=dmrg.GpuBackend(precision=prec, tol_eigval=1e-12, tol_state=1e-7),
+                        backend=dmrg.GpuBackend(precision=prec, tol_eigval=1e-12, tol_state=1e-7, mixed_min_flops=0),
                         params=dict(t=1.0, U=8.0))
     print(prec, r64["energy"], rmx["energy"], rmx["energy"] - r64["energy"])
     assert abs(rmx["energy"] - r64["energy"]) <= 1e-9

@@ -97,3 +97,18 @@ def test_vector_algebra_device():
     # the same seed gives the same vector
     assert lib.vmps_vec_fill_random(plan, y.ptr, 42) == 0
     assert np.array_equal(xr, y.download())
+
+
+@pytest.mark.parametrize("precision", ["fp64", "bf16x3"])
+def test_lanczos_invariant_start_vector(precision):
+    """A start vector that spans an invariant subspace (here the whole
+    one-dimensional space of the two-spin pivot with both spins up) ends the
+    iteration by breakdown, also before min_iter, with a finite exact energy."""
+    import fullbasis as fb
+    pd = fb.chain_pivot(models.U1, models.chain_mpo("HeisenbergU1", 2), (2,), 0, 2, None, None)
+    assert pd.vsize == 1
+    H = PivotMatrix2(pd)
+    v, r = lanczos(H, v0=np.ones(1), precision=precision)
+    print(precision, r)
+    assert np.isfinite(r["energy"]) and abs(r["energy"] - 0.25) <= 1e-12
+    assert np.all(np.isfinite(v))

@@ -67,6 +67,7 @@ class VmpsStatusError(RuntimeError):
 SIGNATURES = {
     "vmps_abi_version": (c_int, []),
     "vmps_last_error": (ctypes.c_char_p, []),
+    "vmps_plan_create_batch": (c_int, [ctypes.POINTER(c_void), c_int, c_int, ctypes.POINTER(c_void), P_i64]),
     "vmps_coupling_6j": (c_dbl, [c_int] * 6),
     "vmps_coupling_9j": (c_dbl, [c_int] * 9),
     "vmps_pivot1_create": (c_int, [c_int, P_int, c_int, P_int, c_int, P_int, ctypes.POINTER(c_void)]),

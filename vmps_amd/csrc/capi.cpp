@@ -394,6 +394,41 @@ vmps_status vmps_plan_create(const vmps_pivot2* p, vmps_precision prec, vmps_pla
   VMPS_API_END
 }
 
+vmps_status vmps_plan_create_batch(const vmps_pivot2* const* pivots, int n, vmps_precision prec, vmps_plan** out,
+                                   int64_t* offsets) {
+  VMPS_API_BEGIN
+  require(pivots && out && n > 0, VMPS_ERR_ARG, "bad argument");
+  *out = nullptr;
+  require(prec == VMPS_PREC_FP64 || prec == VMPS_PREC_FP32 || prec == VMPS_PREC_BF16X3, VMPS_ERR_ARG,
+          "unknown precision");
+  // one triple set over the concatenated vectors: pivot i owns [off_i, off_i + vsize_i)
+  vmps::TripleSet S;
+  for (int i = 0; i < n; ++i) {
+    require(pivots[i] != nullptr, VMPS_ERR_ARG, "NULL pivot in batch");
+    require(pivots[i]->P.A0proj.empty(), VMPS_ERR_ARG, "projectors are not supported in a batch");
+    vmps::TripleSet Si;
+    vmps::triples_from_pivot(pivots[i]->P, Si);
+    const int64_t nx = (int64_t)S.Xb.size(), nz = (int64_t)S.Zb.size(), ny = (int64_t)S.Yb.size();
+    const int64_t xo = (int64_t)S.Xdata.size(), zo = (int64_t)S.Zdata.size(), vo = S.ysize;
+    if (offsets) offsets[i] = vo;
+    for (auto b : Si.Xb) S.Xb.push_back({b.off + xo, b.rows, b.cols});
+    for (auto b : Si.Zb) S.Zb.push_back({b.off + zo, b.rows, b.cols});
+    for (auto b : Si.Yb) S.Yb.push_back({b.off + vo, b.rows, b.cols});
+    for (auto b : Si.Ob) S.Ob.push_back({b.off + vo, b.rows, b.cols});
+    S.Xdata.insert(S.Xdata.end(), Si.Xdata.begin(), Si.Xdata.end());
+    S.Zdata.insert(S.Zdata.end(), Si.Zdata.begin(), Si.Zdata.end());
+    S.Ow.insert(S.Ow.end(), Si.Ow.begin(), Si.Ow.end());
+    for (const auto& t : Si.terms) S.terms.push_back({t.out_v + ny, t.l_blk + nx, t.in_v + ny, t.r_blk + nz, t.c});
+    S.ysize += Si.ysize;
+    S.osize += Si.osize;
+  }
+  if (offsets) offsets[n] = S.ysize;
+  auto plan = std::make_unique<vmps_plan>();
+  plan->D.create_triples(S, prec);
+  *out = plan.release();
+  VMPS_API_END
+}
+
 void vmps_plan_destroy(vmps_plan* plan) { delete plan; }
 
 static void copy_stats(const vmps::Stats& s, vmps_plan_stats* st) {
@@ -609,6 +644,15 @@ static vmps_status lanczos_impl(vmps_plan* plan, double* d_v, bool random_start,
   lo.tol_eigval = opts->tol_eigval;
   lo.tol_state = opts->tol_state;
   const auto prec = (vmps_precision)opts->precision;
+  if (prec != VMPS_PREC_FP64 && opts->refine_fp64) {
+    // The mixed phase cannot resolve the state below the rounding level of its
+    // products (~2^-24 FP32, ~2^-16 split-BF16): stop it at a few times that
+    // level (the energy error is quadratic in the state error) and leave the
+    // rest to the FP64 refinement.
+    const double state_floor = prec == VMPS_PREC_FP32 ? 1e-6 : 2e-4;
+    lo.tol_state = std::max(lo.tol_state, state_floor);
+    lo.tol_eigval = std::max(lo.tol_eigval, state_floor * state_floor);
+  }
   vmps::LanczosResult r = vmps::lanczos_solve(D, d_v, lo, prec);
   res->iterations = r.iterations;
   res->hxv_calls = r.hxv_calls;

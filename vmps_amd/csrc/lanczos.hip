@@ -99,7 +99,7 @@ __global__ void k_axpby(double a, const double* __restrict__ x, double b, double
 // y = x / sqrt(nrm2[0])
 __global__ void k_normalize_into(const double* __restrict__ nrm2, const double* __restrict__ x, double* __restrict__ y,
                                  int64_t n) {
-  const double s = 1.0 / sqrt(nrm2[0]);
+  const double s = nrm2[0] > 0.0 ? 1.0 / sqrt(nrm2[0]) : 0.0;  // a zero vector stays zero
   int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (; e < n; e += stride) y[e] = x[e] * s;
@@ -263,8 +263,12 @@ LanczosResult lanczos_solve(DevPlan& D, double* d_v, const LanczosOpts& o, vmps_
     // residual estimate of the Ritz pair: beta_k |y_k|
     const double rest = b_k * std::fabs(y[k]);
     res.iterations = k + 1;
-    if (k + 1 >= o.min_iter && ((k > 0 && std::fabs(E - Eprev) < o.tol_eigval && rest < o.tol_state) ||
-                                b_k < 1e-14 * std::max(1.0, std::fabs(a_k)))) {
+    // convergence after min_iter iterations; a breakdown (the Krylov space is
+    // invariant, the Ritz pair exact) ends the iteration at any point, since
+    // the next basis vector would be the normalised zero vector
+    const bool breakdown = b_k < 1e-14 * std::max(1.0, std::fabs(a_k));
+    if (breakdown ||
+        (k + 1 >= o.min_iter && k > 0 && std::fabs(E - Eprev) < o.tol_eigval && rest < o.tol_state)) {
       converged = true;
       ++k;
       break;

@@ -290,19 +290,35 @@ def _mps_to_blocks(Asite):
 
 # ---------------------------------------------------------------------------
 class GpuBackend:
-    """Pivot eigensolver and environment updates on the device."""
+    """Pivot eigensolver and environment updates on the device.
 
-    def __init__(self, precision="fp64", refine_fp64=True, tol_eigval=1e-10, tol_state=1e-6, max_iter=100):
+    With a mixed precision the Lanczos solve runs in that precision to its
+    rounding level and is refined in FP64 to the same tolerances as an FP64
+    solve.  Pivots whose algorithmic work is below `mixed_min_flops` per HxV
+    are latency-bound (the mixed products are no faster there) and are solved
+    in FP64 directly."""
+
+    def __init__(self, precision="fp64", refine_fp64=True, tol_eigval=1e-10, tol_state=1e-6, max_iter=100,
+                 mixed_min_flops=5e8):
         self.precision = precision
         self.refine = refine_fp64
         self.tol_eigval, self.tol_state, self.max_iter = tol_eigval, tol_state, max_iter
+        self.mixed_min_flops = mixed_min_flops
         self.hxv_calls = 0
+        self.mixed_solves = 0
 
     def solve(self, pd, v0):
         from .pivot import PivotMatrix2, lanczos
-        H = PivotMatrix2(pd, precision=self.precision)
-        v, r = lanczos(H, v0=v0, precision=self.precision, refine_fp64=self.refine, max_iter=self.max_iter,
-                       tol_eigval=self.tol_eigval, tol_state=self.tol_state)
+        H = PivotMatrix2(pd, precision="fp64")
+        prec = self.precision
+        if prec != "fp64" and H.stats()["flops_alg"] < self.mixed_min_flops:
+            prec = "fp64"
+        if prec != "fp64":
+            H.set_precision(prec)
+            self.mixed_solves += 1
+        v, r = lanczos(H, v0=v0, precision=prec, refine_fp64=self.refine, max_iter=self.max_iter,
+                       tol_eigval=self.tol_eigval, tol_state=self.tol_state,
+                       refine_tol_eigval=self.tol_eigval, refine_tol_state=self.tol_state)
         H.close()
         self.hxv_calls += r["hxv_calls"]
         return r["energy"], v, r

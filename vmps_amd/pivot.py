@@ -91,6 +91,38 @@ class PivotMatrix0(PivotMatrix2):
     NSITE = 0
 
 
+class PivotBatch(PivotMatrix2):
+    """Independent pivots applied together: one plan over the concatenated
+    vectors, one launch per stage for the whole batch.  `offsets[i]` is the
+    start of pivot i's vector."""
+
+    def __init__(self, pds, precision="fp64"):
+        lib = _lib.load()
+        self._lib = lib
+        self.pds = list(pds)
+        handles = []
+        try:
+            for pd in self.pds:
+                h = synth.feed(lib, "vmps_", pd, _lib.check)
+                handles.append(h)
+                _lib.check(lib.vmps_pivot2_precalc(h))
+            arr = (ctypes.c_void_p * len(handles))(*[h.value for h in handles])
+            off = np.zeros(len(handles) + 1, dtype=np.int64)
+            plan = ctypes.c_void_p()
+            _lib.check(lib.vmps_plan_create_batch(arr, len(handles), _lib.PREC[precision], ctypes.byref(plan),
+                                                  off.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))))
+            self._plan = plan
+        finally:
+            for h in handles:
+                lib.vmps_pivot2_destroy(h)
+        self.offsets = off
+        self.vsize = int(off[-1])
+        self.precision = precision
+
+    def split(self, v):
+        return [v[self.offsets[i]:self.offsets[i + 1]] for i in range(len(self.pds))]
+
+
 class DeviceVector:
     """A PivotVector resident in device memory (the plan's layout)."""
 

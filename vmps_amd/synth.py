@@ -317,6 +317,35 @@ WORKLOADS = {
 }
 
 
+BATCHES = {
+    # 64 independent pivots of HeisenbergU1 (D_W = 5) and HubbardU1xU1 (D_W = 6)
+    # chains at random bonds, bond dimension log-uniform in [128, 2048]
+    "batch64_u1_M128_2048": dict(n=64, Mmin=128, Mmax=2048),
+}
+
+
+def make_batch(name, seed=20261202):
+    """List of PivotData for a batched workload (deterministic in the seed)."""
+    cfg = BATCHES[name]
+    rng = np.random.default_rng(seed)
+    out = []
+    heis = models.heisenberg_u1_mpo()
+    hub = models.hubbard_u1xu1_mpo()
+    for i in range(cfg["n"]):
+        M = int(round(math.exp(rng.uniform(math.log(cfg["Mmin"]), math.log(cfg["Mmax"])))))
+        if i % 2 == 0:
+            nleft = int(rng.integers(12, 52))
+            left, right = heisenberg_bond(nleft, M), heisenberg_bond(nleft + 2, M)
+            W = heis
+        else:
+            nleft = int(rng.integers(10, 38))
+            left = hubbard_bond(nleft, M, alpha=1.0, lam=2.2)
+            right = hubbard_bond(nleft + 2, M, alpha=1.0, lam=2.2)
+            W = hub
+        out.append(make_pivot(W, W, left, right, rng))
+    return out
+
+
 def make_workload(name, seed=20261202, hermitian=False, **override):
     cfg = dict(WORKLOADS[name])
     cfg.update(override)
