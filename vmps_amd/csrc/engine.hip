@@ -12,6 +12,7 @@
 // load.  Every output element is owned by exactly one wave, so there are no
 // atomics and results are bitwise reproducible run to run.
 #include "engine.hpp"
+#include "wave.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -49,21 +50,6 @@ struct Mfma<float> {
   }
   __device__ static inline int col(int lane, int v) { return 4 * (lane >> 4) + v; }
 };
-
-// 64-bit wave-uniform broadcast of lane `src`'s value.
-__device__ inline int64_t readlane64(int64_t x, int src) {
-  const int lo = __builtin_amdgcn_readlane((int)(uint32_t)(uint64_t)x, src);
-  const int hi = __builtin_amdgcn_readlane((int)(uint32_t)((uint64_t)x >> 32), src);
-  return (int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint64_t)(uint32_t)lo);
-}
-__device__ inline double readlane_d(double x, int src) { return __longlong_as_double(readlane64(__double_as_longlong(x), src)); }
-__device__ inline float readlane_f(float x, int src) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), src)); }
-template <class T>
-__device__ inline T readlane_t(T x, int src);
-template <>
-__device__ inline double readlane_t<double>(double x, int src) { return readlane_d(x, src); }
-template <>
-__device__ inline float readlane_t<float>(float x, int src) { return readlane_f(x, src); }
 
 // A wave computes an E x E output tile (E = 16 or 32) as S x S MFMA
 // accumulators of 16 x 16 (S = E / 16); sub-tiles past the block edge are
@@ -375,6 +361,7 @@ void DevPlan::create_triples(const TripleSet& S, vmps_precision precision) {
 
 void DevPlan::upload(vmps_precision precision) {
   VMPS_HIP_CHECK(hipGetDevice(&device));
+  VMPS_HIP_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device));
   VMPS_HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
   VMPS_HIP_CHECK(hipEventCreate(&ev0));
   VMPS_HIP_CHECK(hipEventCreate(&ev1));

@@ -3,6 +3,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -56,6 +57,7 @@ struct Krylov {
 
 struct DevPlan {
   int device = 0;
+  int ncu = 0;  // compute units of the device
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   vmps_precision prec = VMPS_PREC_FP64;
@@ -106,6 +108,14 @@ struct DevPlan {
   void hxv(const double* d_vin, double* d_vout, hipStream_t s);
   // one stage of hxv alone: 0 = L.V, 2 = (sum c T).R; stage 1 (the combine) is fused into 2
   void launch_stage(int stage, const double* d_vin, double* d_vout, hipStream_t s);
+  // workgroups (4 waves each) of a kernel whose waves loop over `items` work
+  // items: one per wave when there are few, else as many as are resident at
+  // once (`resident` workgroups per compute unit)
+  int persistent_blocks(int64_t items, int resident) const {
+    const int64_t cap = (int64_t)std::max(resident, 1) * std::max(ncu, 1);
+    return (int)std::max<int64_t>(1, std::min<int64_t>((items + 3) / 4, cap));
+  }
+  int occ_a16 = 0;  // resident workgroups per compute unit of the split-BF16 small-tile stage A kernel
   double dot(const double* d_x, const double* d_y, hipStream_t s);
   void dot_async(const double* d_x, const double* d_y, double* d_result, hipStream_t s);
 };

@@ -10,6 +10,7 @@
 // tile, stage C one workgroup per tile with the combine sum_j c_j T_j done on
 // the operand load and a 4-way split of the K iterations reduced through LDS.
 #include "engine.hpp"
+#include "wave.hpp"
 
 namespace vmps {
 
@@ -54,50 +55,131 @@ __global__ __launch_bounds__(256) void k_split_v(const VConv* __restrict__ tasks
   }
 }
 
-// Stage A: Tp (row-major M x Np) = L (Lt row-major M x Kp) . V (col-major Kp x N)
+// The split operands of one 32-deep K block of a 32 x 32 wave tile:
+// A rows i0 = m0 + lane%16 and i0 + 16, B columns j0 = n0 + lane%16 and j0 + 16,
+// k = kb + 8 (lane/16) .. +7.
+struct WaveOps {
+  v8bf a0h, a0l, a1h, a1l, b0h, b0l, b1h, b1l;
+};
+
+__device__ inline void wave_mac(const WaveOps& o, bool has_m1, bool has_n1, v4f& c00, v4f& c01, v4f& c10,
+                                v4f& c11) {
+  c00 = mac3(o.b0h, o.b0l, o.a0h, o.a0l, c00);
+  if (has_m1) c10 = mac3(o.b0h, o.b0l, o.a1h, o.a1l, c10);
+  if (has_n1) {
+    c01 = mac3(o.b1h, o.b1l, o.a0h, o.a0l, c01);
+    if (has_m1) c11 = mac3(o.b1h, o.b1l, o.a1h, o.a1l, c11);
+  }
+}
+
+// Stage A, small problems: Tp (row-major M x Np) = L (Lt row-major M x Kp) . V (col-major Kp x N).
+// Most of these tiles need one or two 32-deep K blocks, so a wave that took a
+// single tile would spend its life waiting: for the descriptor, then for the
+// operands.  Instead each wave walks the tiles w, w + W, w + 2W, ... (W = waves
+// in the grid): their descriptors arrive 64 at a time, one per lane, and are
+// broadcast with readlane, and the operand loads of the next K block -- of the
+// same tile or of the wave's next tile -- are issued before the MFMAs of the
+// current one.
+struct TileCursor {
+  int64_t a_off, b_off, c_off;
+  int M, N, Kp, Np, m0, n0;
+};
+
+__device__ inline TileCursor tile_at(const TileA16& d, int src) {
+  TileCursor c;
+  c.a_off = readlane64(d.a_off, src);
+  c.b_off = readlane64(d.b_off, src);
+  c.c_off = readlane64(d.c_off, src);
+  c.M = readlane32(d.M, src);
+  c.N = readlane32(d.N, src);
+  c.Kp = readlane32(d.Kp, src);
+  c.Np = readlane32(d.Np, src);
+  c.m0 = readlane32((int)d.mt, src) * kTile;
+  c.n0 = readlane32((int)d.nt, src) * kTile;
+  return c;
+}
+
 __global__ __launch_bounds__(256) void k_stage_a_bf16x3(const TileA16* __restrict__ tiles, int ntiles,
                                                         const __bf16* __restrict__ Lhi, const __bf16* __restrict__ Llo,
                                                         const __bf16* __restrict__ Vhi, const __bf16* __restrict__ Vlo,
                                                         float* __restrict__ Tp) {
-  const int w = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (w >= ntiles) return;
+  const int nwaves = gridDim.x * 4;
+  const int w0 = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (w0 >= ntiles) return;
+  const int nmine = (ntiles - 1 - w0) / nwaves + 1;  // tiles w0 + j * nwaves, j < nmine
   const int lane = threadIdx.x & 63;
-  const TileA16 t = tiles[w];
-  const int m0 = t.mt * kTile, n0 = t.nt * kTile;
   const int li = lane & 15, lg = lane >> 4;
-  const int i0 = m0 + li, i1 = i0 + 16, j0 = n0 + li, j1 = j0 + 16;
-  const bool has_m1 = m0 + 16 < t.M, has_n1 = n0 + 16 < t.N;
   const v8bf z = {};
-  v4f c00 = {0, 0, 0, 0}, c01 = c00, c10 = c00, c11 = c00;
-  for (int kb = 0; kb < t.Kp; kb += 32) {
+  auto load = [&](const TileCursor& t, int kb, WaveOps& o) {
     const int k = kb + 8 * lg;
     const bool kin = k < t.Kp;
-    const v8bf a0h = (kin && i0 < t.M) ? ld8(Lhi + t.a_off + (int64_t)i0 * t.Kp + k) : z;
-    const v8bf a0l = (kin && i0 < t.M) ? ld8(Llo + t.a_off + (int64_t)i0 * t.Kp + k) : z;
-    const v8bf a1h = (kin && i1 < t.M) ? ld8(Lhi + t.a_off + (int64_t)i1 * t.Kp + k) : z;
-    const v8bf a1l = (kin && i1 < t.M) ? ld8(Llo + t.a_off + (int64_t)i1 * t.Kp + k) : z;
-    const v8bf b0h = (kin && j0 < t.N) ? ld8(Vhi + t.b_off + (int64_t)j0 * t.Kp + k) : z;
-    const v8bf b0l = (kin && j0 < t.N) ? ld8(Vlo + t.b_off + (int64_t)j0 * t.Kp + k) : z;
-    const v8bf b1h = (kin && j1 < t.N) ? ld8(Vhi + t.b_off + (int64_t)j1 * t.Kp + k) : z;
-    const v8bf b1l = (kin && j1 < t.N) ? ld8(Vlo + t.b_off + (int64_t)j1 * t.Kp + k) : z;
-    c00 = mac3(b0h, b0l, a0h, a0l, c00);
-    if (has_m1) c10 = mac3(b0h, b0l, a1h, a1l, c10);
-    if (has_n1) {
-      c01 = mac3(b1h, b1l, a0h, a0l, c01);
-      if (has_m1) c11 = mac3(b1h, b1l, a1h, a1l, c11);
-    }
-  }
-  float* __restrict__ C = Tp + t.c_off;
+    const int i0 = t.m0 + li, i1 = i0 + 16, j0 = t.n0 + li, j1 = j0 + 16;
+    const bool ra = kin && i0 < t.M, rb = kin && i1 < t.M, qa = kin && j0 < t.N, qb = kin && j1 < t.N;
+    o.a0h = ra ? ld8(Lhi + t.a_off + (int64_t)i0 * t.Kp + k) : z;
+    o.a0l = ra ? ld8(Llo + t.a_off + (int64_t)i0 * t.Kp + k) : z;
+    o.a1h = rb ? ld8(Lhi + t.a_off + (int64_t)i1 * t.Kp + k) : z;
+    o.a1l = rb ? ld8(Llo + t.a_off + (int64_t)i1 * t.Kp + k) : z;
+    o.b0h = qa ? ld8(Vhi + t.b_off + (int64_t)j0 * t.Kp + k) : z;
+    o.b0l = qa ? ld8(Vlo + t.b_off + (int64_t)j0 * t.Kp + k) : z;
+    o.b1h = qb ? ld8(Vhi + t.b_off + (int64_t)j1 * t.Kp + k) : z;
+    o.b1l = qb ? ld8(Vlo + t.b_off + (int64_t)j1 * t.Kp + k) : z;
+  };
+  auto store = [&](const TileCursor& t, const v4f& c00, const v4f& c01, const v4f& c10, const v4f& c11) {
+    const int i0 = t.m0 + li, i1 = i0 + 16;
+    float* __restrict__ C = Tp + t.c_off;
 #pragma unroll
-  for (int v = 0; v < 4; ++v) {
-    const int ja = n0 + 4 * lg + v, jb = ja + 16;
-    if (i0 < t.M) {
-      if (ja < t.N) C[(int64_t)i0 * t.Np + ja] = c00[v];
-      if (jb < t.N) C[(int64_t)i0 * t.Np + jb] = c01[v];
+    for (int v = 0; v < 4; ++v) {
+      const int ja = t.n0 + 4 * lg + v, jb = ja + 16;
+      if (i0 < t.M) {
+        if (ja < t.N) C[(int64_t)i0 * t.Np + ja] = c00[v];
+        if (jb < t.N) C[(int64_t)i0 * t.Np + jb] = c01[v];
+      }
+      if (i1 < t.M) {
+        if (ja < t.N) C[(int64_t)i1 * t.Np + ja] = c10[v];
+        if (jb < t.N) C[(int64_t)i1 * t.Np + jb] = c11[v];
+      }
     }
-    if (i1 < t.M) {
-      if (ja < t.N) C[(int64_t)i1 * t.Np + ja] = c10[v];
-      if (jb < t.N) C[(int64_t)i1 * t.Np + jb] = c11[v];
+  };
+  for (int g = 0; g < nmine; g += 64) {
+    const int n = min(64, nmine - g);
+    TileA16 d = {};
+    if (lane < n) d = tiles[w0 + (int64_t)(g + lane) * nwaves];
+    // load cursor (lt, lj, lk) runs one K block ahead of the compute cursor (ct, cj, ck);
+    // a tile has max(1, Kp / 32) K blocks (Kp is a multiple of 8)
+    TileCursor lt = tile_at(d, 0), ct = lt;
+    int lj = 0, lk = 0, cj = 0, ck = 0;
+    auto advance_load = [&]() {
+      lk += 32;
+      if (lk >= max(lt.Kp, 1)) {
+        lk = 0;
+        if (++lj < n) lt = tile_at(d, lj);
+      }
+    };
+    WaveOps o0, o1;
+    load(lt, 0, o0);
+    advance_load();
+    v4f c00 = {0, 0, 0, 0}, c01 = c00, c10 = c00, c11 = c00;
+    // one pipeline step: prefetch into `nx`, multiply `cur`; false when the group is done
+    auto step = [&](const WaveOps& cur, WaveOps& nx) -> bool {
+      if (lj < n) {
+        load(lt, lk, nx);
+        advance_load();
+      }
+      wave_mac(cur, ct.m0 + 16 < ct.M, ct.n0 + 16 < ct.N, c00, c01, c10, c11);
+      ck += 32;
+      if (ck >= max(ct.Kp, 1)) {
+        store(ct, c00, c01, c10, c11);
+        c00 = v4f{0, 0, 0, 0};
+        c01 = c00;
+        c10 = c00;
+        c11 = c00;
+        ck = 0;
+        if (++cj == n) return false;
+        ct = tile_at(d, cj);
+      }
+      return true;
+    };
+    while (step(o0, o1) && step(o1, o0)) {
     }
   }
 }
@@ -121,53 +203,101 @@ __global__ __launch_bounds__(256) void k_stage_c_bf16x3(const TileC* __restrict_
   const v8bf z = {};
   v4f c00 = {0, 0, 0, 0}, c01 = c00, c10 = c00, c11 = c00;
   int it = 0;
-  for (int c = t.chunk_begin; c < t.chunk_end; ++c) {
-    const ChunkC16 ch = chunks[c];
-    for (int kb = 0; kb < ch.Kp; kb += 32, ++it) {
-      if ((it & 3) != wave) continue;
-      const int k = kb + 8 * lg;
-      const bool kin = k < ch.Kp;
-      float u0[8], u1[8];
-#pragma unroll
-      for (int e = 0; e < 8; ++e) {
-        u0[e] = 0.f;
-        u1[e] = 0.f;
+  for (int g = t.chunk_begin; g < t.chunk_end; g += 64) {
+    // chunk descriptors, one per lane, broadcast with readlane
+    const int nch = min(64, t.chunk_end - g);
+    int64_t my_r = 0;
+    int my_K = 0, my_tb = 0, my_te = 0;
+    if (lane < nch) {
+      const ChunkC16 cd = chunks[g + lane];
+      my_r = cd.r_off;
+      my_K = cd.Kp;
+      my_tb = cd.term_begin;
+      my_te = cd.term_end;
+    }
+    for (int ci = 0; ci < nch; ++ci) {
+      const int Kp = readlane32(my_K, ci);
+      const int nkb = (Kp + 31) >> 5;
+      // K blocks of this chunk handled by this wave: it + q with (it + q) % 4 == wave
+      const int first = (wave - (it & 3)) & 3;
+      if (first >= nkb) {
+        it += nkb;
+        continue;
       }
-      for (int j = ch.term_begin; j < ch.term_end; ++j) {
-        const TermC16 tm = terms[j];
-        const float* __restrict__ Tt = Tp + tm.t_off + k;
-        if (kin && r0) {
-          const v4f x = *reinterpret_cast<const v4f*>(Tt + (int64_t)i0 * ch.Kp);
-          const v4f y = *reinterpret_cast<const v4f*>(Tt + (int64_t)i0 * ch.Kp + 4);
+      const int64_t roff = readlane64(my_r, ci);
+      const int tb = readlane32(my_tb, ci);
+      const int nt = readlane32(my_te, ci) - tb;
+      // the chunk's terms (at most 64), one per lane
+      int64_t my_toff = 0;
+      float my_c = 0.f;
+      if (lane < nt) {
+        const TermC16 tm = terms[tb + lane];
+        my_toff = tm.t_off;
+        my_c = tm.c;
+      }
+      for (int q = first; q < nkb; q += 4) {
+        const int k = q * 32 + 8 * lg;
+        const bool kin = k < Kp;
+        const bool a0 = kin && r0, a1 = kin && r1;
+        const v8bf b0h = (kin && q0) ? ld8(Rhi + roff + (int64_t)j0 * Kp + k) : z;
+        const v8bf b0l = (kin && q0) ? ld8(Rlo + roff + (int64_t)j0 * Kp + k) : z;
+        const v8bf b1h = (kin && q1) ? ld8(Rhi + roff + (int64_t)j1 * Kp + k) : z;
+        const v8bf b1l = (kin && q1) ? ld8(Rlo + roff + (int64_t)j1 * Kp + k) : z;
+        float u0[8], u1[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+          u0[e] = 0.f;
+          u1[e] = 0.f;
+        }
+        // U rows i0, i1 at k..k+7 = sum_j c_j T_j, two terms per round with all loads first
+        int jt = 0;
+        for (; jt + 1 < nt; jt += 2) {
+          const float* __restrict__ T0 = Tp + readlane64(my_toff, jt) + k;
+          const float* __restrict__ T1 = Tp + readlane64(my_toff, jt + 1) + k;
+          const float c0 = readlane_f(my_c, jt), c1 = readlane_f(my_c, jt + 1);
+          const v4f zf = {0, 0, 0, 0};
+          const v4f x00 = a0 ? *reinterpret_cast<const v4f*>(T0 + (int64_t)i0 * Kp) : zf;
+          const v4f x01 = a0 ? *reinterpret_cast<const v4f*>(T0 + (int64_t)i0 * Kp + 4) : zf;
+          const v4f x10 = a1 ? *reinterpret_cast<const v4f*>(T0 + (int64_t)i1 * Kp) : zf;
+          const v4f x11 = a1 ? *reinterpret_cast<const v4f*>(T0 + (int64_t)i1 * Kp + 4) : zf;
+          const v4f y00 = a0 ? *reinterpret_cast<const v4f*>(T1 + (int64_t)i0 * Kp) : zf;
+          const v4f y01 = a0 ? *reinterpret_cast<const v4f*>(T1 + (int64_t)i0 * Kp + 4) : zf;
+          const v4f y10 = a1 ? *reinterpret_cast<const v4f*>(T1 + (int64_t)i1 * Kp) : zf;
+          const v4f y11 = a1 ? *reinterpret_cast<const v4f*>(T1 + (int64_t)i1 * Kp + 4) : zf;
 #pragma unroll
           for (int e = 0; e < 4; ++e) {
-            u0[e] += tm.c * x[e];
-            u0[e + 4] += tm.c * y[e];
+            u0[e] += c0 * x00[e] + c1 * y00[e];
+            u0[e + 4] += c0 * x01[e] + c1 * y01[e];
+            u1[e] += c0 * x10[e] + c1 * y10[e];
+            u1[e + 4] += c0 * x11[e] + c1 * y11[e];
           }
         }
-        if (kin && r1) {
-          const v4f x = *reinterpret_cast<const v4f*>(Tt + (int64_t)i1 * ch.Kp);
-          const v4f y = *reinterpret_cast<const v4f*>(Tt + (int64_t)i1 * ch.Kp + 4);
+        if (jt < nt) {
+          const float* __restrict__ T0 = Tp + readlane64(my_toff, jt) + k;
+          const float c0 = readlane_f(my_c, jt);
+          const v4f zf = {0, 0, 0, 0};
+          const v4f x00 = a0 ? *reinterpret_cast<const v4f*>(T0 + (int64_t)i0 * Kp) : zf;
+          const v4f x01 = a0 ? *reinterpret_cast<const v4f*>(T0 + (int64_t)i0 * Kp + 4) : zf;
+          const v4f x10 = a1 ? *reinterpret_cast<const v4f*>(T0 + (int64_t)i1 * Kp) : zf;
+          const v4f x11 = a1 ? *reinterpret_cast<const v4f*>(T0 + (int64_t)i1 * Kp + 4) : zf;
 #pragma unroll
           for (int e = 0; e < 4; ++e) {
-            u1[e] += tm.c * x[e];
-            u1[e + 4] += tm.c * y[e];
+            u0[e] += c0 * x00[e];
+            u0[e + 4] += c0 * x01[e];
+            u1[e] += c0 * x10[e];
+            u1[e + 4] += c0 * x11[e];
           }
         }
+        WaveOps o;
+        split8(u0, o.a0h, o.a0l);
+        split8(u1, o.a1h, o.a1l);
+        o.b0h = b0h;
+        o.b0l = b0l;
+        o.b1h = b1h;
+        o.b1l = b1l;
+        wave_mac(o, has_m1, has_n1, c00, c01, c10, c11);
       }
-      v8bf a0h, a0l, a1h, a1l;
-      split8(u0, a0h, a0l);
-      split8(u1, a1h, a1l);
-      const v8bf b0h = (kin && q0) ? ld8(Rhi + ch.r_off + (int64_t)j0 * ch.Kp + k) : z;
-      const v8bf b0l = (kin && q0) ? ld8(Rlo + ch.r_off + (int64_t)j0 * ch.Kp + k) : z;
-      const v8bf b1h = (kin && q1) ? ld8(Rhi + ch.r_off + (int64_t)j1 * ch.Kp + k) : z;
-      const v8bf b1l = (kin && q1) ? ld8(Rlo + ch.r_off + (int64_t)j1 * ch.Kp + k) : z;
-      c00 = mac3(b0h, b0l, a0h, a0l, c00);
-      if (has_m1) c10 = mac3(b0h, b0l, a1h, a1l, c10);
-      if (has_n1) {
-        c01 = mac3(b1h, b1l, a0h, a0l, c01);
-        if (has_m1) c11 = mac3(b1h, b1l, a1h, a1l, c11);
-      }
+      it += nkb;
     }
   }
   if (wave > 0) {
@@ -202,31 +332,53 @@ __global__ __launch_bounds__(256) void k_stage_c_bf16x3(const TileC* __restrict_
 }
 
 // ---- LDS-staged 128 x 128 workgroup tiles for large blocks -------------------
-// 4 waves in a 2 x 2 grid, each owning 64 x 64 = 4 x 4 MFMA blocks.  Per 32-deep
-// K step the workgroup stages the split operand tiles A[128 x 32] and
-// B[32 x 128] (hi and lo) in LDS, k-contiguous with rows padded to 40
-// elements, and each wave then issues 4 x 4 x 3 MFMAs from LDS reads.
+// 4 waves in a 2 x 2 grid, each owning 64 x 64 = 4 x 4 MFMA blocks.  Per
+// 64-deep K step the workgroup stages the split operand tiles A[128 x 64] and
+// B[64 x 128] (hi and lo) in LDS, k-contiguous with rows padded to 72
+// elements (conflict-free 16-byte reads), and each wave then issues
+// 2 x 4 x 4 x 3 MFMAs from LDS reads.  The global loads of the next step are
+// in flight during those MFMAs.
 constexpr int kBT = kBigTile;
-constexpr int kBK = 32;
-constexpr int kLS = 40;  // LDS row stride in bf16 elements (80 bytes)
+constexpr int kBK = kBigK;
+constexpr int kLS = kBK + 8;   // LDS row stride in bf16 elements
+constexpr int kPieces = kBT * kBK / 8 / 256;  // 16-byte pieces per operand tile and thread
 
 struct BigSmem {
   __bf16 Ah[kBT * kLS], Al[kBT * kLS], Bh[kBT * kLS], Bl[kBT * kLS];
 };
 
 __device__ inline void big_compute(const BigSmem& s, int wm, int wn, int li, int lg, v4f (&acc)[4][4]) {
-  v8bf ah[4], al[4], bh[4], bl[4];
 #pragma unroll
-  for (int x = 0; x < 4; ++x) {
-    ah[x] = *reinterpret_cast<const v8bf*>(&s.Ah[(wm + 16 * x + li) * kLS + 8 * lg]);
-    al[x] = *reinterpret_cast<const v8bf*>(&s.Al[(wm + 16 * x + li) * kLS + 8 * lg]);
-    bh[x] = *reinterpret_cast<const v8bf*>(&s.Bh[(wn + 16 * x + li) * kLS + 8 * lg]);
-    bl[x] = *reinterpret_cast<const v8bf*>(&s.Bl[(wn + 16 * x + li) * kLS + 8 * lg]);
+  for (int ks = 0; ks < kBK; ks += 32) {
+    v8bf ah[4], al[4], bh[4], bl[4];
+#pragma unroll
+    for (int x = 0; x < 4; ++x) {
+      ah[x] = *reinterpret_cast<const v8bf*>(&s.Ah[(wm + 16 * x + li) * kLS + ks + 8 * lg]);
+      al[x] = *reinterpret_cast<const v8bf*>(&s.Al[(wm + 16 * x + li) * kLS + ks + 8 * lg]);
+      bh[x] = *reinterpret_cast<const v8bf*>(&s.Bh[(wn + 16 * x + li) * kLS + ks + 8 * lg]);
+      bl[x] = *reinterpret_cast<const v8bf*>(&s.Bl[(wn + 16 * x + li) * kLS + ks + 8 * lg]);
+    }
+#pragma unroll
+    for (int mi = 0; mi < 4; ++mi)
+#pragma unroll
+      for (int ni = 0; ni < 4; ++ni) acc[mi][ni] = mac3(bh[ni], bl[ni], ah[mi], al[mi], acc[mi][ni]);
   }
+}
+
+// thread tid's piece h of a staged operand tile: row (of A) / column (of B), k offset
+__device__ inline int piece_row(int tid, int h) { return (tid + 256 * h) / (kBK / 8); }
+__device__ inline int piece_k(int tid, int h) { return ((tid + 256 * h) % (kBK / 8)) * 8; }
+
+__device__ inline void big_stage(BigSmem& s, int tid, const v8bf (&ah)[kPieces], const v8bf (&al)[kPieces],
+                                 const v8bf (&bh)[kPieces], const v8bf (&bl)[kPieces]) {
 #pragma unroll
-  for (int mi = 0; mi < 4; ++mi)
-#pragma unroll
-    for (int ni = 0; ni < 4; ++ni) acc[mi][ni] = mac3(bh[ni], bl[ni], ah[mi], al[mi], acc[mi][ni]);
+  for (int h = 0; h < kPieces; ++h) {
+    const int o = piece_row(tid, h) * kLS + piece_k(tid, h);
+    *reinterpret_cast<v8bf*>(&s.Ah[o]) = ah[h];
+    *reinterpret_cast<v8bf*>(&s.Al[o]) = al[h];
+    *reinterpret_cast<v8bf*>(&s.Bh[o]) = bh[h];
+    *reinterpret_cast<v8bf*>(&s.Bl[o]) = bl[h];
+  }
 }
 
 // Stage A, large problems: Tp (row-major M x Np) = Lt (row-major M x Kp) . V (col-major Kp x N)
@@ -248,28 +400,24 @@ __global__ __launch_bounds__(256) void k_stage_a_bf16x3_big(const TileA16* __res
   for (int mi = 0; mi < 4; ++mi)
 #pragma unroll
     for (int ni = 0; ni < 4; ++ni) acc[mi][ni] = v4f{0, 0, 0, 0};
-  for (int kb = 0; kb < t.Kp; kb += kBK) {
-    v8bf ah[2], al[2], bh[2], bl[2];
+  v8bf ah[kPieces], al[kPieces], bh[kPieces], bl[kPieces];
+  auto gload = [&](int kb) {
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {  // 512 16-byte chunks per operand tile, 2 per thread
-      const int c = tid + 256 * h, row = c >> 2, kc = (c & 3) * 8;
-      const int k = kb + kc;
+    for (int h = 0; h < kPieces; ++h) {
+      const int row = piece_row(tid, h), k = kb + piece_k(tid, h);
       const bool ka = (m0 + row < t.M) && (k < t.Kp), kbv = (n0 + row < t.N) && (k < t.Kp);
       ah[h] = ka ? ld8(Lhi + t.a_off + (int64_t)(m0 + row) * t.Kp + k) : z;
       al[h] = ka ? ld8(Llo + t.a_off + (int64_t)(m0 + row) * t.Kp + k) : z;
       bh[h] = kbv ? ld8(Vhi + t.b_off + (int64_t)(n0 + row) * t.Kp + k) : z;
       bl[h] = kbv ? ld8(Vlo + t.b_off + (int64_t)(n0 + row) * t.Kp + k) : z;
     }
+  };
+  gload(0);
+  for (int kb = 0; kb < t.Kp; kb += kBK) {
     __syncthreads();  // the previous step's LDS reads are done
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const int c = tid + 256 * h, row = c >> 2, kc = (c & 3) * 8;
-      *reinterpret_cast<v8bf*>(&s.Ah[row * kLS + kc]) = ah[h];
-      *reinterpret_cast<v8bf*>(&s.Al[row * kLS + kc]) = al[h];
-      *reinterpret_cast<v8bf*>(&s.Bh[row * kLS + kc]) = bh[h];
-      *reinterpret_cast<v8bf*>(&s.Bl[row * kLS + kc]) = bl[h];
-    }
+    big_stage(s, tid, ah, al, bh, bl);
     __syncthreads();
+    if (kb + kBK < t.Kp) gload(kb + kBK);  // in flight during the MFMAs below
     big_compute(s, wm, wn, li, lg, acc);
   }
   float* __restrict__ C = Tp + t.c_off;
@@ -289,7 +437,7 @@ __global__ __launch_bounds__(256) void k_stage_a_bf16x3_big(const TileA16* __res
 
 // Stage C, large output blocks: Vout (FP64 col-major M x N) = sum_chunks (sum_j c_j T_j) . R.
 // Work item w covers iterations [it_begin, it_end) of its tile's flattened
-// (chunk, 32-deep k block) loop; with slot >= 0 it writes an FP32 partial tile
+// (chunk, kBK-deep k block) loop; with slot >= 0 it writes an FP32 partial tile
 // (column-major 128 x 128) that k_reduce_partials sums in slot order.
 __global__ __launch_bounds__(256) void k_stage_c_bf16x3_big(const TileC* __restrict__ tiles,
                                                             const int32_t* __restrict__ w_tile,
@@ -312,11 +460,12 @@ __global__ __launch_bounds__(256) void k_stage_c_bf16x3_big(const TileC* __restr
   const int64_t slot = w_slot[blockIdx.x];
   const int m0 = t.mt * kBT, n0 = t.nt * kBT;
   const v8bf z = {};
+  const v4f zf = {0, 0, 0, 0};
   v4f acc[4][4];
 #pragma unroll
   for (int mi = 0; mi < 4; ++mi)
 #pragma unroll
-    for (int ni = 0; ni < 4; ++ni) acc[mi][ni] = v4f{0, 0, 0, 0};
+    for (int ni = 0; ni < 4; ++ni) acc[mi][ni] = zf;
   int it = 0;
   for (int c = t.chunk_begin; c < t.chunk_end && it < it1; ++c) {
     const ChunkC16 ch = chunks[c];
@@ -325,43 +474,54 @@ __global__ __launch_bounds__(256) void k_stage_c_bf16x3_big(const TileC* __restr
       it += nsteps;
       continue;
     }
+    // the chunk's terms (at most 64), one per lane, broadcast with readlane
+    const int nt = ch.term_end - ch.term_begin;
+    int64_t my_toff = 0;
+    float my_c = 0.f;
+    if (lane < nt) {
+      const TermC16 tm = terms[ch.term_begin + lane];
+      my_toff = tm.t_off;
+      my_c = tm.c;
+    }
     for (int kb = 0; kb < ch.Kp; kb += kBK, ++it) {
       if (it < it0 || it >= it1) continue;  // workgroup-uniform
-      v8bf ah[2], al[2], bh[2], bl[2];
+      v8bf ah[kPieces], al[kPieces], bh[kPieces], bl[kPieces];
+      bool ka[kPieces];
+      int64_t arow[kPieces];
+      float u[kPieces][8];
 #pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        const int cc = tid + 256 * h, row = cc >> 2, kc = (cc & 3) * 8;
-        const int k = kb + kc;
-        const bool ka = (m0 + row < t.M) && (k < ch.Kp), kbv = (n0 + row < t.N) && (k < ch.Kp);
-        float u[8];
-#pragma unroll
-        for (int e = 0; e < 8; ++e) u[e] = 0.f;
-        if (ka) {
-          for (int j = ch.term_begin; j < ch.term_end; ++j) {
-            const TermC16 tm = terms[j];
-            const float* __restrict__ Tt = Tp + tm.t_off + (int64_t)(m0 + row) * ch.Kp + k;
-            const v4f x = *reinterpret_cast<const v4f*>(Tt);
-            const v4f y = *reinterpret_cast<const v4f*>(Tt + 4);
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-              u[e] += tm.c * x[e];
-              u[e + 4] += tm.c * y[e];
-            }
-          }
-        }
-        split8(u, ah[h], al[h]);
+      for (int h = 0; h < kPieces; ++h) {
+        const int row = piece_row(tid, h), k = kb + piece_k(tid, h);
+        ka[h] = (m0 + row < t.M) && (k < ch.Kp);
+        arow[h] = (int64_t)(m0 + row) * ch.Kp + k;
+        const bool kbv = (n0 + row < t.N) && (k < ch.Kp);
         bh[h] = kbv ? ld8(Rhi + ch.r_off + (int64_t)(n0 + row) * ch.Kp + k) : z;
         bl[h] = kbv ? ld8(Rlo + ch.r_off + (int64_t)(n0 + row) * ch.Kp + k) : z;
-      }
-      __syncthreads();
 #pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        const int cc = tid + 256 * h, row = cc >> 2, kc = (cc & 3) * 8;
-        *reinterpret_cast<v8bf*>(&s.Ah[row * kLS + kc]) = ah[h];
-        *reinterpret_cast<v8bf*>(&s.Al[row * kLS + kc]) = al[h];
-        *reinterpret_cast<v8bf*>(&s.Bh[row * kLS + kc]) = bh[h];
-        *reinterpret_cast<v8bf*>(&s.Bl[row * kLS + kc]) = bl[h];
+        for (int e = 0; e < 8; ++e) u[h][e] = 0.f;
       }
+      // U = sum_j c_j T_j on this thread's 8-wide row pieces
+      for (int jt = 0; jt < nt; ++jt) {
+        const float* __restrict__ T0 = Tp + readlane64(my_toff, jt);
+        const float c0 = readlane_f(my_c, jt);
+        v4f x[kPieces][2];
+#pragma unroll
+        for (int h = 0; h < kPieces; ++h) {
+          x[h][0] = ka[h] ? *reinterpret_cast<const v4f*>(T0 + arow[h]) : zf;
+          x[h][1] = ka[h] ? *reinterpret_cast<const v4f*>(T0 + arow[h] + 4) : zf;
+        }
+#pragma unroll
+        for (int h = 0; h < kPieces; ++h)
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            u[h][e] += c0 * x[h][0][e];
+            u[h][e + 4] += c0 * x[h][1][e];
+          }
+      }
+#pragma unroll
+      for (int h = 0; h < kPieces; ++h) split8(u[h], ah[h], al[h]);
+      __syncthreads();
+      big_stage(s, tid, ah, al, bh, bl);
       __syncthreads();
       big_compute(s, wm, wn, li, lg, acc);
     }
@@ -435,6 +595,7 @@ void DevPlan::ensure_bf16_buffers() {
   bfC.upload(B.chunksC, stream);
   bfT.upload(B.termsC, stream);
   vconv.upload(B.vconv, stream);
+  VMPS_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_a16, k_stage_a_bf16x3, 256, 0));
   VMPS_HIP_CHECK(hipStreamSynchronize(stream));
 }
 
@@ -448,7 +609,7 @@ void DevPlan::launch_stage_bf16x3(int stage, const double* d_vin, double* d_vout
       hipLaunchKernelGGL(k_stage_a_bf16x3_big, dim3((unsigned)nAb), dim3(256), 0, s, bfAbig.p, nAb,
                          (const __bf16*)Lhi.p, (const __bf16*)Llo.p, (const __bf16*)Vhi.p, (const __bf16*)Vlo.p, Tp.p);
     if (nA)
-      hipLaunchKernelGGL(k_stage_a_bf16x3, dim3((unsigned)((nA + 3) / 4)), dim3(256), 0, s, bfA.p, nA,
+      hipLaunchKernelGGL(k_stage_a_bf16x3, dim3((unsigned)persistent_blocks(nA, occ_a16)), dim3(256), 0, s, bfA.p, nA,
                          (const __bf16*)Lhi.p, (const __bf16*)Llo.p, (const __bf16*)Vhi.p, (const __bf16*)Vlo.p, Tp.p);
   } else if (stage == 2) {
     const int nW = (int)wTile.n, nR = (int)rTile.n;

@@ -945,15 +945,15 @@ void build_bf16_layout(const HostPlan& H, Bf16Layout& B) {
   }
   // Split the K iterations of a big stage C tile over several workgroups only
   // when the big tiles alone cannot fill the GPU (the FP32 partials cost a
-  // round trip through HBM): aim at ~512 workgroups, at least 16 iterations of
-  // 32 k per split, at most 4 splits.
+  // round trip through HBM): aim at ~512 workgroups, at least 8 iterations of
+  // kBigK k per split, at most 4 splits.
   const int64_t nbig = (int64_t)B.tilesC_big.size();
   const int64_t want = nbig > 0 ? std::max<int64_t>(1, std::min<int64_t>(4, 512 / nbig)) : 1;
   for (size_t ti = 0; ti < B.tilesC_big.size(); ++ti) {
     const TileC& t = B.tilesC_big[ti];
     int64_t steps = 0;
-    for (int32_t c = t.chunk_begin; c < t.chunk_end; ++c) steps += (pad8(H.chunksC[c].K) + 31) / 32;
-    const int64_t S = std::max<int64_t>(1, std::min<int64_t>(want, steps / 16));
+    for (int32_t c = t.chunk_begin; c < t.chunk_end; ++c) steps += (pad8(H.chunksC[c].K) + kBigK - 1) / kBigK;
+    const int64_t S = std::max<int64_t>(1, std::min<int64_t>(want, steps / 8));
     for (int64_t s = 0; s < S; ++s) {
       B.splitC_tile.push_back((int32_t)ti);
       B.splitC_begin.push_back((int32_t)(steps * s / S));
@@ -988,6 +988,9 @@ void build_bf16_layout(const HostPlan& H, Bf16Layout& B) {
       const ChunkC16& ch = B.chunksC[c];
       if (ch.r_off % 8 || ch.r_off + (int64_t)ch.Kp * t.N > (int64_t)B.Rhi.size())
         throw VmpsError(VMPS_ERR_INTERNAL, "bf16 stage C operand out of range");
+      if (ch.term_end - ch.term_begin > kMaxTermsPerChunk || ch.term_end < ch.term_begin ||
+          ch.term_end > (int32_t)B.termsC.size())
+        throw VmpsError(VMPS_ERR_INTERNAL, "bf16 stage C chunk term list out of range");
       for (int32_t j = ch.term_begin; j < ch.term_end; ++j)
         if (B.termsC[j].t_off % 8 || B.termsC[j].t_off + (int64_t)t.M * ch.Kp > B.T_size)
           throw VmpsError(VMPS_ERR_INTERNAL, "bf16 stage C term out of range");

@@ -239,6 +239,7 @@ struct VConv {             // V block: packed FP64 (K x N, ld K) -> padded split
   int32_t K, N, Kp, pad;
 };
 constexpr int kBigTile = 128;  // workgroup tile edge of the LDS-staged kernels
+constexpr int kBigK = 64;      // k depth of one LDS stage of those kernels
 // A block is handled by the LDS-staged big-tile kernels when both its
 // dimensions are at least this (below, the wave-tile kernels waste less).
 constexpr int kBigMin = 96;
