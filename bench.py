@@ -8,6 +8,12 @@ wavefunction blocks (synthetic data).  The metric value is the algorithmic
 rate F_alg / t in GFLOP/s, where F_alg counts the staged contraction
 (distinct L.V products plus distinct U.R products).
 
+Each plan is autotuned first (its tile configuration is timed over the
+candidates and the fastest kept; `tiles` key).  The same workload is also
+timed in the mixed modes, FP32 MFMA and split-BF16x3 with FP32 accumulation
+(`mixed_hxv`: rate, speedup over FP64 and the relative difference of the
+result from the FP64 one).
+
 Beside the HxV line the default run also times full two-site DMRG sweeps of
 the HeisenbergU1 L=32 M=64 chain, FP64 and mixed (split-BF16x3 Lanczos refined
 in FP64), and reports the energy difference (`sweeps` key).
@@ -110,6 +116,8 @@ def main():
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32", "bf16x3"])
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default tile configuration")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the FP32 / split-BF16 x3 HxV lines")
     ap.add_argument("--seed", type=int, default=20261202)
     ap.add_argument("--sweeps", type=int, default=4, help="DMRG sweeps of the HeisenbergU1 L=32 M=64 chain (0: none)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
@@ -145,27 +153,45 @@ def main():
         pds = [synth.make_workload(args.workload, seed=args.seed)]
         H = PivotMatrix2(pds[0], precision=args.precision)
     st = H.stats()
+    tiles = H.autotune(iters=10) if not args.no_tune else H.tiles()
     vins = [synth.random_vector(pd, args.seed + 1 + i) for i, pd in enumerate(pds)]
     vin_h = np.concatenate(vins)
     vin = torch.from_numpy(vin_h).to("cuda")
     vout = torch.empty_like(vin)
     lib = H._lib
-    plan = H._plan
 
-    def step():
-        _lib.check(lib.vmps_hxv_device(plan, vin.data_ptr(), vout.data_ptr(), None))
+    def timed(plan, out):
+        """Seconds per HxV over args.steps back-to-back calls on resident vectors."""
+        def step():
+            _lib.check(lib.vmps_hxv_device(plan, vin.data_ptr(), out.data_ptr(), None))
+        for _ in range(args.warmup):
+            step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            step()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / args.steps
 
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    sec = (t1 - t0) / args.steps
+    sec = timed(H._plan, vout)
     gflops = st["flops_alg"] / sec / 1e9
     out_gpu = vout.cpu().numpy()
+
+    # the same workload in the mixed modes (split-BF16 x3 and FP32 MFMA with
+    # FP32 accumulation), each autotuned, against the FP64 result
+    mixed = {}
+    if args.precision == "fp64" and not args.no_mixed:
+        for prec in ("fp32", "bf16x3"):
+            Hm = PivotBatch(pds, precision=prec) if batched else PivotMatrix2(pds[0], precision=prec)
+            tm = Hm.autotune(iters=10) if not args.no_tune else Hm.tiles()
+            om = torch.empty_like(vin)
+            sm = timed(Hm._plan, om)
+            o = om.cpu().numpy()
+            mixed[prec] = {"gflops": st["flops_alg"] / sm / 1e9, "ms_per_hxv": sm * 1e3,
+                           "speedup_vs_fp64": sec / sm,
+                           "rel_err_vs_fp64": float(np.linalg.norm(o - out_gpu) / np.linalg.norm(out_gpu)),
+                           "tiles": {"wave_edge": tm["wave_edge"], "big_min": tm["big_min"]}}
+            Hm.close()
 
     # per-kernel device times (HIP events on the plan's stream)
     ms = H.time_stages(warmup=5, iters=200)
@@ -214,6 +240,9 @@ def main():
             "parallelism": "single GPU",
         },
         "stage_ms": {"A_LV": ms[0], "C_combine_UR": ms[2], "hxv": ms[3]},
+        "tiles": {"wave_edge": tiles["wave_edge"], "big_min": tiles["big_min"],
+                  "autotuned": not args.no_tune},
+        "mixed_hxv": mixed,
         "roofline": {
             "bound": "mfma",
             "kernel": kname,

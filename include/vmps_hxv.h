@@ -203,6 +203,31 @@ double vmps_coupling_9j(int q1, int q2, int q3, int q4, int q5, int q6, int q7, 
  * with HIP events around `iters` back-to-back calls on resident vectors. */
 vmps_status vmps_plan_time_hxv(vmps_plan* plan, int warmup, int iters, double* ms_per_hxv);
 
+/* Tile configuration of a plan.  No counterpart in the reference (its Eigen
+ * GEMMs need none); it serves the GPU path's tuning.
+ *   wave_edge: output tile edge of the FP64 / FP32 wave kernels, 16 or 32;
+ *   big_min:   split-BF16 x3 only: problems with both dimensions >= big_min
+ *              use the LDS-staged 128 x 128 workgroup tiles, the rest
+ *              32 x 32 wave tiles (0 = none use the big tiles). */
+typedef struct {
+  int wave_edge;
+  int big_min;
+  double ms;       /* HxV time of the chosen configuration (autotune; 0 if taken from the cache) */
+  int from_cache;  /* 1: a plan with the same block-size histogram was tuned before in this process */
+  int n_tried;     /* configurations timed */
+} vmps_tile_config;
+
+vmps_status vmps_plan_get_tiles(const vmps_plan* plan, vmps_tile_config* cfg);
+/* Results do not depend on the configuration beyond FP rounding of the
+ * split-BF16 path (FP64: bit-identical sums in the same order per tile). */
+vmps_status vmps_plan_set_tiles(vmps_plan* plan, int wave_edge, int big_min);
+/* Autotuner: times every candidate configuration of the plan's precision
+ * (`iters` HxV each on the plan's resident vectors) and keeps the fastest.
+ * The choice is cached per block-size histogram (the share of the work in
+ * classes of block rows x columns) and per precision, so later plans of the
+ * same shape take it without timing.  Thread-safe. */
+vmps_status vmps_plan_autotune(vmps_plan* plan, int iters, vmps_tile_config* out);
+
 /* ---- environment updates ---------------------------------------------------
  * contract_L(Lold, Abra, W, Aket, qloc, qOp, Lnew)  (tensors/DmrgContractions.h:34-158)
  * contract_R(Rold, Abra, W, Aket, qloc, qOp, Rnew)  (tensors/DmrgContractions.h:179-311)

@@ -1,0 +1,85 @@
+"""Tile configurations and the autotuner: every configuration gives the
+oracle's HxV (FP64: 1e-12 relative; split-BF16 x3: 5e-5), the autotuner picks
+one of its candidates and caches it per block-size histogram."""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from vmps_amd import synth
+from vmps_amd.pivot import HxV, PivotMatrix2
+
+sys.path.insert(0, __file__.rsplit("/", 1)[0])
+import pivots  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+FP64_RTOL = 1e-12
+BF16X3_RTOL = 5e-5
+
+
+def _rel(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+@pytest.fixture(scope="module")
+def case():
+    # blocks up to ~150, so that the 128 x 128 split-BF16 tiles are used for some
+    pd = pivots.small_heisenberg(seed=3, M=400, nleft=6)
+    v = synth.random_vector(pd, 4)
+    lib = oracle.load()
+    h = pivots.build_oracle(lib, pd)
+    try:
+        ref = pivots.oracle_hxv(lib, h, v, nthreads=8)
+    finally:
+        lib.oracle_pivot2_destroy(h)
+    return pd, v, ref
+
+
+@pytest.mark.parametrize("edge", [16, 32])
+def test_fp64_wave_edges(case, edge):
+    pd, v, ref = case
+    print("fp64 wave edge", edge, "vsize", pd.vsize, flush=True)
+    H = PivotMatrix2(pd)
+    H.set_tiles(edge, H.tiles()["big_min"])
+    assert H.tiles()["wave_edge"] == edge
+    assert _rel(HxV(H, v), ref) <= FP64_RTOL
+
+
+@pytest.mark.parametrize("big_min", [96, 64, 0])
+def test_bf16x3_big_thresholds(case, big_min):
+    pd, v, ref = case
+    print("bf16x3 big_min", big_min, flush=True)
+    H = PivotMatrix2(pd, precision="bf16x3")
+    H.set_tiles(H.tiles()["wave_edge"], big_min)
+    assert H.tiles()["big_min"] == big_min
+    assert _rel(HxV(H, v), ref) <= BF16X3_RTOL
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", FP64_RTOL), ("fp32", 1e-5), ("bf16x3", BF16X3_RTOL)])
+def test_autotune_picks_candidate_and_caches(case, precision, tol):
+    pd, v, ref = case
+    print("autotune", precision, flush=True)
+    H = PivotMatrix2(pd, precision=precision)
+    cfg = H.autotune(iters=3)
+    if cfg["from_cache"] == 0:
+        assert cfg["n_tried"] >= 2 and cfg["ms"] > 0
+    if precision == "bf16x3":
+        assert cfg["big_min"] in (96, 192, 0)
+    else:
+        assert cfg["wave_edge"] in (16, 32)
+    assert _rel(HxV(H, v), ref) <= tol
+    # a second plan of the same shape takes the cached choice without timing
+    H2 = PivotMatrix2(pd, precision=precision)
+    cfg2 = H2.autotune(iters=3)
+    assert cfg2["from_cache"] == 1
+    assert (cfg2["wave_edge"], cfg2["big_min"]) == (cfg["wave_edge"], cfg["big_min"])
+    assert _rel(HxV(H2, v), ref) <= tol
+
+
+def test_set_tiles_rejects_bad_edge(case):
+    pd, _, _ = case
+    H = PivotMatrix2(pd)
+    with pytest.raises(Exception):
+        H.set_tiles(24, 96)

@@ -104,6 +104,11 @@ struct DevPlan {
   void upload(vmps_precision prec);  // device copies of H
   void ensure_precision_buffers();
   void ensure_bf16_buffers();
+  bool bf16_ready = false;
+  // switch the tile configuration: FP64 / FP32 wave tile edge (16 or 32) and
+  // the split-BF16 big-tile threshold (H.big_min); rebuilds and uploads the
+  // affected work lists only
+  void retile(int wave_edge, int big_min);
   void launch_stage_bf16x3(int stage, const double* d_vin, double* d_vout, hipStream_t s);
   void hxv(const double* d_vin, double* d_vout, hipStream_t s);
   // one stage of hxv alone: 0 = L.V, 2 = (sum c T).R; stage 1 (the combine) is fused into 2

@@ -569,7 +569,7 @@ __global__ __launch_bounds__(256) void k_reduce_partials(const TileC* __restrict
 }  // namespace
 
 void DevPlan::ensure_bf16_buffers() {
-  if (bfA.p || H.tilesA.empty() && H.tilesC.empty()) return;
+  if (bf16_ready || (H.tilesA.empty() && H.tilesC.empty())) return;
   Bf16Layout B;
   build_bf16_layout(H, B);
   Lhi.upload(B.Lhi, stream);
@@ -596,6 +596,24 @@ void DevPlan::ensure_bf16_buffers() {
   bfT.upload(B.termsC, stream);
   vconv.upload(B.vconv, stream);
   VMPS_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_a16, k_stage_a_bf16x3, 256, 0));
+  VMPS_HIP_CHECK(hipStreamSynchronize(stream));
+  bf16_ready = true;
+}
+
+void DevPlan::retile(int wave_edge, int big_min) {
+  if (wave_edge != H.wave_edge) {
+    set_wave_edge(H, wave_edge);
+    tilesA.upload(H.tilesA_w, stream);
+    tilesC.upload(H.tilesC_w, stream);
+  }
+  if (big_min != H.big_min) {
+    H.big_min = big_min;
+    if (bf16_ready) {  // the split operands are rebuilt with the tile lists
+      VMPS_HIP_CHECK(hipStreamSynchronize(stream));
+      bf16_ready = false;
+      ensure_bf16_buffers();
+    }
+  }
   VMPS_HIP_CHECK(hipStreamSynchronize(stream));
 }
 

@@ -587,11 +587,51 @@ int choose_wave_edge(const HostPlan& H) {
 }
 }  // namespace
 
+void set_wave_edge(HostPlan& H, int edge) {
+  if (edge != 16 && edge != 32) throw VmpsError(VMPS_ERR_ARG, "wave tile edge must be 16 or 32");
+  if (H.crange.size() != H.probC.size()) throw VmpsError(VMPS_ERR_INTERNAL, "wave tiles: chunk ranges missing");
+  H.wave_edge = edge;
+  if (edge == kTile) {
+    H.tilesA_w = H.tilesA;
+    H.tilesC_w = H.tilesC;
+  } else {
+    make_tiles(H, edge, H.crange, H.tilesA_w, H.tilesC_w);
+  }
+}
+
+std::string tile_signature(const HostPlan& H) {
+  auto cls = [](int64_t d) { return d <= 16 ? 0 : d <= 32 ? 1 : d <= 64 ? 2 : d <= 128 ? 3 : d <= 256 ? 4 : 5; };
+  double share[2][6][6] = {};
+  double total = 0;
+  for (const GemmProb& p : H.probA) {
+    const double f = 2.0 * p.M * p.N * H.chunkA[p.chunk_begin].K;
+    share[0][cls(p.M)][cls(p.N)] += f;
+    total += f;
+  }
+  for (const GemmProb& p : H.probC)
+    for (int32_t c = p.chunk_begin; c < p.chunk_end; ++c) {
+      const double f = 2.0 * p.M * p.N * H.chunkC[c].K;
+      share[1][cls(p.M)][cls(p.N)] += f;
+      total += f;
+    }
+  std::string key;
+  for (int st = 0; st < 2; ++st)
+    for (int a = 0; a < 6; ++a)
+      for (int b = 0; b < 6; ++b) {
+        const int q = total > 0 ? (int)std::lround(8.0 * share[st][a][b] / total) : 0;
+        if (q) key += std::to_string(st) + std::to_string(a) + std::to_string(b) + ":" + std::to_string(q) + ";";
+      }
+  // the total work decides whether launches or MFMA time dominate
+  key += "w" + std::to_string(total > 0 ? (int)std::floor(std::log2(total)) : 0);
+  return key;
+}
+
 void build_tiles(HostPlan& H) {
   H.chunksC.clear();
   H.termsC.clear();
   for (size_t j = 0; j < H.comb_t.size(); ++j) H.termsC.push_back({H.T_off[H.comb_t[j]], H.comb_c[j]});
-  std::vector<std::pair<int32_t, int32_t>> crange(H.probC.size());
+  std::vector<std::pair<int32_t, int32_t>>& crange = H.crange;
+  crange.assign(H.probC.size(), {0, 0});
   for (size_t p = 0; p < H.probC.size(); ++p) {
     const GemmProb& pr = H.probC[p];
     crange[p].first = (int32_t)H.chunksC.size();
@@ -611,13 +651,7 @@ void build_tiles(HostPlan& H) {
     crange[p].second = (int32_t)H.chunksC.size();
   }
   make_tiles(H, kTile, crange, H.tilesA, H.tilesC);
-  H.wave_edge = choose_wave_edge(H);
-  if (H.wave_edge == kTile) {
-    H.tilesA_w = H.tilesA;
-    H.tilesC_w = H.tilesC;
-  } else {
-    make_tiles(H, H.wave_edge, crange, H.tilesA_w, H.tilesC_w);
-  }
+  set_wave_edge(H, choose_wave_edge(H));
   // bounds of the fused stage C: every T read stays inside T
   for (const ChunkC& c : H.chunksC) {
     if (c.term_end - c.term_begin > kMaxTermsPerChunk || c.term_end < c.term_begin)
@@ -912,7 +946,8 @@ void build_bf16_layout(const HostPlan& H, Bf16Layout& B) {
     o += (int64_t)H.probA[t].M * pad8(H.probA[t].N);
   }
   B.T_size = o;
-  auto is_big = [](int64_t M, int64_t N) { return M >= kBigMin && N >= kBigMin; };
+  const int64_t big_min = H.big_min;
+  auto is_big = [big_min](int64_t M, int64_t N) { return big_min > 0 && M >= big_min && N >= big_min; };
   for (const TileA& t : H.tilesA) {
     TileA16 a;
     a.a_off = Lmap.at(t.a_off);

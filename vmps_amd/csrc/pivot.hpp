@@ -301,6 +301,10 @@ struct HostPlan {
   int wave_edge = kTile;
   std::vector<TileA> tilesA_w;
   std::vector<TileC> tilesC_w;
+  std::vector<std::pair<int32_t, int32_t>> crange;  // chunksC range of each stage C problem
+  // split-BF16: problems with both dimensions >= big_min take the LDS-staged
+  // kBigTile tiles (big_min <= 0: none do)
+  int big_min = kBigMin;
   std::vector<ChunkC> chunksC;
   std::vector<TermC> termsC;
   std::vector<GemmProb> probA;       // stage A: one chunk each (L . V)
@@ -340,6 +344,12 @@ void build_host_plan(const Pivot& P, HostPlan& H);
 void triples_from_pivot(const Pivot& P, TripleSet& S);
 void build_plan(const TripleSet& S, HostPlan& H);
 void build_tiles(HostPlan& H);
+// rebuild the FP64 / FP32 wave tile lists tilesA_w / tilesC_w for edge 16 or 32
+void set_wave_edge(HostPlan& H, int edge);
+// key of the plan's block-size histogram: the share of stage A and stage C
+// flops in classes of (rows, columns), quantised, so that pivots of similar
+// shape share one tuned tile configuration
+std::string tile_signature(const HostPlan& H);
 // Split-BF16 operands and work lists of the BF16x3 path (built on demand).
 void build_bf16_layout(const HostPlan& H, Bf16Layout& B);
 uint16_t bf16_rne(float f);

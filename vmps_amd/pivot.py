@@ -70,6 +70,22 @@ class PivotMatrix2:
         _lib.check(self._lib.vmps_plan_stats_get(self._plan, ctypes.byref(st)))
         return st.as_dict()
 
+    def tiles(self):
+        """Current tile configuration: {'wave_edge', 'big_min', ...}."""
+        cfg = _lib.TileConfig()
+        _lib.check(self._lib.vmps_plan_get_tiles(self._plan, ctypes.byref(cfg)))
+        return cfg.as_dict()
+
+    def set_tiles(self, wave_edge, big_min):
+        _lib.check(self._lib.vmps_plan_set_tiles(self._plan, int(wave_edge), int(big_min)))
+
+    def autotune(self, iters=10):
+        """Time the candidate tile configurations of this plan's precision and
+        keep the fastest (cached per block-size histogram)."""
+        cfg = _lib.TileConfig()
+        _lib.check(self._lib.vmps_plan_autotune(self._plan, int(iters), ctypes.byref(cfg)))
+        return cfg.as_dict()
+
     def time_stages(self, warmup=3, iters=20):
         ms = (ctypes.c_double * 4)()
         _lib.check(self._lib.vmps_plan_time_stages(self._plan, warmup, iters, ms))
