@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define VMPS_ABI_VERSION 1
+#define VMPS_ABI_VERSION 2
 
 typedef int vmps_status;
 #define VMPS_OK 0
@@ -293,6 +293,8 @@ typedef struct {
   double refine_tol_eigval;
   double refine_tol_state;
   uint64_t seed;            /* start vector when none is given */
+  int graph;                /* 1: each iteration is one replay of a captured HIP graph (default);
+                               0: the same kernels launched one by one */
 } vmps_lanczos_opts;
 
 typedef struct {

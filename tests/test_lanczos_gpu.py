@@ -112,3 +112,24 @@ def test_lanczos_invariant_start_vector(precision):
     print(precision, r)
     assert np.isfinite(r["energy"]) and abs(r["energy"] - 0.25) <= 1e-12
     assert np.all(np.isfinite(v))
+
+
+@pytest.mark.parametrize("kind", ["heis", "hub"])
+def test_graph_iteration_matches_eager(kind):
+    """Each iteration replayed from one captured HIP graph gives exactly the
+    result of the same kernels launched one by one, over repeated solves on one
+    plan (graph reuse), a mixed solve with FP64 refinement (one graph per
+    precision) and a change of tiles (the graph is captured again)."""
+    pd = _herm_small(kind)
+    H = PivotMatrix2(pd)
+    runs = [dict(precision="fp64"), dict(precision="fp64", seed=7), dict(precision="bf16x3"),
+            dict(precision="fp64")]
+    for i, kw in enumerate(runs):
+        if i == 3:
+            H.set_tiles(32 if H.tiles()["wave_edge"] == 16 else 16, H.tiles()["big_min"])
+        print(kind, "run", i, kw, flush=True)
+        vg, rg = lanczos(H, max_iter=80, tol_eigval=1e-12, tol_state=1e-8, graph=True, **kw)
+        ve, re_ = lanczos(H, max_iter=80, tol_eigval=1e-12, tol_state=1e-8, graph=False, **kw)
+        assert rg["iterations"] == re_["iterations"] and rg["iterations_refine"] == re_["iterations_refine"]
+        assert rg["energy"] == re_["energy"]
+        assert np.array_equal(vg, ve)

@@ -42,7 +42,7 @@ class LanczosOpts(ctypes.Structure):
     _fields_ = [
         ("max_iter", c_int), ("min_iter", c_int), ("tol_eigval", c_dbl), ("tol_state", c_dbl),
         ("precision", c_int), ("refine_fp64", c_int), ("refine_tol_eigval", c_dbl), ("refine_tol_state", c_dbl),
-        ("seed", ctypes.c_uint64),
+        ("seed", ctypes.c_uint64), ("graph", c_int),
     ]
 
 

@@ -700,6 +700,7 @@ void vmps_lanczos_opts_default(vmps_lanczos_opts* o) {
   o->refine_tol_eigval = 1e-12;
   o->refine_tol_state = 1e-7;
   o->seed = 1;
+  o->graph = 1;
 }
 
 static vmps_status lanczos_impl(vmps_plan* plan, double* d_v, bool random_start, const vmps_lanczos_opts* opts,
@@ -719,6 +720,7 @@ static vmps_status lanczos_impl(vmps_plan* plan, double* d_v, bool random_start,
   lo.min_iter = opts->min_iter;
   lo.tol_eigval = opts->tol_eigval;
   lo.tol_state = opts->tol_state;
+  lo.graph = opts->graph != 0;
   const auto prec = (vmps_precision)opts->precision;
   if (prec != VMPS_PREC_FP64 && opts->refine_fp64) {
     // The mixed phase cannot resolve the state below the rounding level of its

@@ -398,6 +398,7 @@ void DevPlan::ensure_precision_buffers() {
     if (L.n) hipLaunchKernelGGL(k_f64_to_f32, dim3(1024), dim3(256), 0, stream, L.p, L32.p, (int64_t)L.n);
     if (R.n) hipLaunchKernelGGL(k_f64_to_f32, dim3(1024), dim3(256), 0, stream, R.p, R32.p, (int64_t)R.n);
     VMPS_HIP_CHECK(hipGetLastError());
+    ++gen;
   }
   if (prec == VMPS_PREC_BF16X3) ensure_bf16_buffers();
 }

@@ -47,17 +47,39 @@ struct DevBuf {
 
 struct DevPlan;
 
-// Krylov workspace: basis Q ((kmax+1) x n), work vector, coefficients.
+// Krylov workspace: basis Q ((kmax+1) x n), work vector w, fixed HxV input v,
+// Gram-Schmidt coefficients c (two passes of kmax + 1), the iteration index
+// in device memory, and the HIP graph of one iteration.
 struct Krylov {
-  DevBuf<double> Q, w, c, scal, part;
+  DevBuf<double> Q, w, v, c, scal, out3;
+  DevBuf<int> kdev;
+  DevBuf<double> ew;      // coeff_dot weight of every vector element
+  DevBuf<double> gpart;   // partial sums of the basis dot products, (kmax + 1) x ngrp
+  DevBuf<double> npart;   // partial sums of ||w||^2
+  double* host3 = nullptr;  // pinned readback of out3
+  int ngrp = 0;           // element chunks of the basis dot products
+  int64_t chunk = 0;
   int cap = 0;
   int64_t nvec = -1;
+  // one captured iteration per precision (a mixed solve and its FP64
+  // refinement each keep theirs), valid for the plan generation it was made at
+  hipGraph_t graph[3] = {nullptr, nullptr, nullptr};
+  hipGraphExec_t exec[3] = {nullptr, nullptr, nullptr};
+  int graph_gen[3] = {-1, -1, -1};
   void ensure(DevPlan& D, int kmax);
+  void enqueue_iteration(DevPlan& D, hipStream_t s);
+  hipGraphExec_t ensure_graph(DevPlan& D);  // the graph of D.prec
+  void release_graph();
+  Krylov() = default;
+  Krylov(const Krylov&) = delete;
+  Krylov& operator=(const Krylov&) = delete;
+  ~Krylov();
 };
 
 struct DevPlan {
   int device = 0;
   int ncu = 0;  // compute units of the device
+  int gen = 0;  // bumped whenever device work lists or operand buffers change (invalidates captured graphs)
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   vmps_precision prec = VMPS_PREC_FP64;

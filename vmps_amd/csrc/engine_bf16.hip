@@ -598,9 +598,11 @@ void DevPlan::ensure_bf16_buffers() {
   VMPS_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_a16, k_stage_a_bf16x3, 256, 0));
   VMPS_HIP_CHECK(hipStreamSynchronize(stream));
   bf16_ready = true;
+  ++gen;
 }
 
 void DevPlan::retile(int wave_edge, int big_min) {
+  ++gen;
   if (wave_edge != H.wave_edge) {
     set_wave_edge(H, wave_edge);
     tilesA.upload(H.tilesA_w, stream);

@@ -20,63 +20,6 @@ namespace vmps {
 
 namespace {
 
-// c[i] = <Q_i, w>_weighted for i < k; one workgroup per (segment group, i).
-// Partial sums go to partial[i * nb + b]; reduced by k_multidot_final.
-__global__ __launch_bounds__(256) void k_multidot_partial(const int64_t* __restrict__ seg_off,
-                                                          const int32_t* __restrict__ seg_len,
-                                                          const double* __restrict__ seg_w, int nseg, int spb,
-                                                          const double* __restrict__ Q, int64_t n,
-                                                          const double* __restrict__ w, double* __restrict__ partial,
-                                                          int nb) {
-  __shared__ double red[256];
-  const int i = blockIdx.y;
-  const double* __restrict__ q = Q + (int64_t)i * n;
-  double acc = 0.0;
-  const int s0 = blockIdx.x * spb;
-  const int s1 = min(nseg, s0 + spb);
-  for (int s = s0; s < s1; ++s) {
-    const int64_t off = seg_off[s];
-    const int len = seg_len[s];
-    double part = 0.0;
-    for (int e = threadIdx.x; e < len; e += 256) part += q[off + e] * w[off + e];
-    acc += seg_w[s] * part;
-  }
-  red[threadIdx.x] = acc;
-  __syncthreads();
-  for (int h = 128; h > 0; h >>= 1) {
-    if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) partial[(int64_t)i * nb + blockIdx.x] = red[0];
-}
-
-__global__ __launch_bounds__(256) void k_multidot_final(const double* __restrict__ partial, int nb,
-                                                        double* __restrict__ c) {
-  __shared__ double red[256];
-  const int i = blockIdx.x;
-  double acc = 0.0;
-  for (int b = threadIdx.x; b < nb; b += 256) acc += partial[(int64_t)i * nb + b];
-  red[threadIdx.x] = acc;
-  __syncthreads();
-  for (int h = 128; h > 0; h >>= 1) {
-    if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) c[i] = red[0];
-}
-
-// w -= sum_{i<k} c[i] Q_i
-__global__ void k_sub_combination(const double* __restrict__ Q, int64_t n, const double* __restrict__ c, int k,
-                                  double* __restrict__ w) {
-  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (; e < n; e += stride) {
-    double s = 0.0;
-    for (int i = 0; i < k; ++i) s += c[i] * Q[(int64_t)i * n + e];
-    w[e] -= s;
-  }
-}
-
 // y = sum_{i<k} c[i] Q_i   (Ritz vector)
 __global__ void k_combination(const double* __restrict__ Q, int64_t n, const double* __restrict__ c, int k,
                               double* __restrict__ y) {
@@ -119,6 +62,147 @@ __global__ void k_fill_uniform(double* __restrict__ v, int64_t n, uint64_t seed)
 }
 
 inline unsigned vgrid(int64_t n) { return (unsigned)std::max<int64_t>(1, std::min<int64_t>(2048, (n + 255) / 256)); }
+
+// ---- kernels of the graph-captured iteration ----------------------------------
+// The iteration index k lives in device memory (*kdev), so one captured graph
+// serves every iteration: the kernels read k instead of taking it as an argument.
+
+// partial[i * nb + b] = sum over elements e of chunk b of ew[e] Q_i[e] w[e]
+// (ew: the coeff_dot weight of e's block), for the rows i = 4 y .. 4 y + 3
+// that are <= k; workgroup (b, y).  Groups past k exit at once, so the grid is
+// sized for the largest k and stays the same in every iteration.
+__global__ __launch_bounds__(256) void k_multidot_rows(const int* __restrict__ kdev, const double* __restrict__ Q,
+                                                       int64_t n, const double* __restrict__ w,
+                                                       const double* __restrict__ ew, int64_t chunk,
+                                                       double* __restrict__ partial, int nb) {
+  const int k = *kdev;
+  const int i0 = 4 * blockIdx.y;
+  if (i0 > k) return;  // workgroup-uniform
+  __shared__ double red[4][256];
+  const int nr = min(4, k + 1 - i0);
+  const int b = blockIdx.x;
+  const int64_t e0 = (int64_t)b * chunk, e1 = min(n, e0 + chunk);
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int64_t e = e0 + threadIdx.x; e < e1; e += 256) {
+    const double we = w[e] * ew[e];
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+      if (r < nr) acc[r] += Q[(int64_t)(i0 + r) * n + e] * we;
+  }
+#pragma unroll
+  for (int r = 0; r < 4; ++r) red[r][threadIdx.x] = acc[r];
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) red[r][threadIdx.x] += red[r][threadIdx.x + h];
+    __syncthreads();
+  }
+  if ((int)threadIdx.x < nr) partial[(int64_t)(i0 + threadIdx.x) * nb + b] = red[threadIdx.x][0];
+}
+
+// c[i] = sum_b partial[i * nb + b] for i <= k; one workgroup per possible row.
+__global__ __launch_bounds__(256) void k_multidot_rows_final(const int* __restrict__ kdev,
+                                                             const double* __restrict__ partial, int nb,
+                                                             double* __restrict__ c) {
+  const int i = blockIdx.x;
+  if (i > *kdev) return;  // workgroup-uniform
+  __shared__ double red[256];
+  double acc = 0.0;
+  for (int b = threadIdx.x; b < nb; b += 256) acc += partial[(int64_t)i * nb + b];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) c[i] = red[0];
+}
+
+// w -= sum_{i<=k} c[i] Q_i; with norm_part, also the workgroup's share of the
+// weighted ||w||^2 of the result (reduced by k_sum_final)
+__global__ __launch_bounds__(256) void k_sub_combination_dk(const double* __restrict__ Q, int64_t n,
+                                                            const double* __restrict__ c, const int* __restrict__ kdev,
+                                                            double* __restrict__ w, const double* __restrict__ ew,
+                                                            double* __restrict__ norm_part) {
+  const int k = *kdev;
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  double nrm = 0.0;
+  for (; e < n; e += stride) {
+    double s = 0.0;
+    for (int i = 0; i <= k; ++i) s += c[i] * Q[(int64_t)i * n + e];
+    const double y = w[e] - s;
+    w[e] = y;
+    nrm += ew[e] * y * y;
+  }
+  if (!norm_part) return;
+  __shared__ double red[256];
+  red[threadIdx.x] = nrm;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) norm_part[blockIdx.x] = red[0];
+}
+
+// out[0] = sum of part[0 .. np-1] (one workgroup, fixed order)
+__global__ __launch_bounds__(256) void k_sum_final(const double* __restrict__ part, int np, double* __restrict__ out) {
+  __shared__ double red[256];
+  double acc = 0.0;
+  for (int b = threadIdx.x; b < np; b += 256) acc += part[b];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = red[0];
+}
+
+// the three numbers the host needs after iteration k (<q_k, H q_k> in two
+// Gram-Schmidt parts and ||w||^2), then k += 1.  Runs after every other
+// kernel of the iteration has read k.
+__global__ void k_lanczos_scalars_next(int* __restrict__ kdev, const double* __restrict__ c1,
+                                       const double* __restrict__ c2, const double* __restrict__ nrm2,
+                                       double* __restrict__ out) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    const int k = *kdev;
+    out[0] = c1[k];
+    out[1] = c2[k];
+    out[2] = nrm2[0];
+    *kdev = k + 1;
+  }
+}
+
+// q_{k+1} = w / ||w|| into the basis and into the fixed HxV input vector v
+// (zero for a zero w: the iteration then ends on the breakdown test)
+__global__ void k_normalize_next(const int* __restrict__ kdev, const double* __restrict__ nrm2,
+                                 const double* __restrict__ w, double* __restrict__ Q, int64_t n,
+                                 double* __restrict__ v) {
+  const int k = *kdev;
+  const double nv = nrm2[0];
+  const double f = nv > 0.0 ? 1.0 / sqrt(nv) : 0.0;
+  double* __restrict__ q = Q + (int64_t)(k + 1) * n;
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (; e < n; e += stride) {
+    const double y = f * w[e];
+    q[e] = y;
+    v[e] = y;
+  }
+}
+
+__global__ void k_set_int(int* __restrict__ p, int value) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *p = value;
+}
+
+__global__ void k_copy(const double* __restrict__ x, double* __restrict__ y, int64_t n) {
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (; e < n; e += stride) y[e] = x[e];
+}
 
 }  // namespace
 
@@ -206,6 +290,65 @@ void dev_fill_uniform(double* v, int64_t n, uint64_t seed, hipStream_t s) {
 }
 
 // ---- Lanczos ---------------------------------------------------------------
+// One iteration (HxV of q_k, two classical Gram-Schmidt passes against
+// q_0 .. q_k, ||w||^2, q_{k+1} = w / ||w||) is a fixed sequence of kernels whose
+// only varying input, k, is read from device memory.  With `graph` the sequence
+// is captured once per plan configuration as a HIP graph and replayed each
+// iteration; without it the same kernels are launched one by one.  Either way
+// the host waits once per iteration, for three numbers.
+void Krylov::enqueue_iteration(DevPlan& D, hipStream_t s) {
+  const int64_t n = D.H.vsize;
+  const unsigned gy = (unsigned)((cap + 3) / 4);
+  const unsigned gs = vgrid(n);
+  auto pass = [&](double* c_out, double* norm_part) {
+    hipLaunchKernelGGL(k_multidot_rows, dim3(ngrp, gy), dim3(256), 0, s, kdev.p, Q.p, n, w.p, ew.p, chunk, gpart.p,
+                       ngrp);
+    hipLaunchKernelGGL(k_multidot_rows_final, dim3(cap), dim3(256), 0, s, kdev.p, gpart.p, ngrp, c_out);
+    hipLaunchKernelGGL(k_sub_combination_dk, dim3(gs), dim3(256), 0, s, Q.p, n, c_out, kdev.p, w.p, ew.p, norm_part);
+  };
+  D.hxv(v.p, w.p, s);
+  pass(c.p, nullptr);
+  pass(c.p + cap, npart.p);
+  hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(256), 0, s, npart.p, (int)gs, scal.p);
+  hipLaunchKernelGGL(k_normalize_next, dim3(gs), dim3(256), 0, s, kdev.p, scal.p, w.p, Q.p, n, v.p);
+  hipLaunchKernelGGL(k_lanczos_scalars_next, dim3(1), dim3(64), 0, s, kdev.p, c.p, c.p + cap, scal.p, out3.p);
+  VMPS_HIP_CHECK(hipGetLastError());
+}
+
+void Krylov::release_graph() {
+  for (int i = 0; i < 3; ++i) {
+    if (exec[i]) (void)hipGraphExecDestroy(exec[i]);
+    if (graph[i]) (void)hipGraphDestroy(graph[i]);
+    exec[i] = nullptr;
+    graph[i] = nullptr;
+    graph_gen[i] = -1;
+  }
+}
+
+hipGraphExec_t Krylov::ensure_graph(DevPlan& D) {
+  const int p = (int)D.prec;
+  if (p < 0 || p > 2) throw VmpsError(VMPS_ERR_ARG, "unsupported precision");
+  if (exec[p] && graph_gen[p] == D.gen) return exec[p];
+  if (exec[p]) (void)hipGraphExecDestroy(exec[p]);
+  if (graph[p]) (void)hipGraphDestroy(graph[p]);
+  exec[p] = nullptr;
+  graph[p] = nullptr;
+  hipStream_t s = D.stream;
+  VMPS_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  try {
+    enqueue_iteration(D, s);
+  } catch (...) {
+    hipGraph_t g = nullptr;
+    (void)hipStreamEndCapture(s, &g);
+    if (g) (void)hipGraphDestroy(g);
+    throw;
+  }
+  VMPS_HIP_CHECK(hipStreamEndCapture(s, &graph[p]));
+  VMPS_HIP_CHECK(hipGraphInstantiate(&exec[p], graph[p], nullptr, nullptr, 0));
+  graph_gen[p] = D.gen;
+  return exec[p];
+}
+
 LanczosResult lanczos_solve(DevPlan& D, double* d_v, const LanczosOpts& o, vmps_precision prec) {
   hipStream_t s = D.stream;
   const int64_t n = D.H.vsize;
@@ -213,49 +356,33 @@ LanczosResult lanczos_solve(DevPlan& D, double* d_v, const LanczosOpts& o, vmps_
   if (n == 0) return res;
   const int kmax = std::max(2, (int)std::min<int64_t>(o.max_iter, n));
   Krylov& K = D.krylov;
-  K.ensure(D, kmax);
   const vmps_precision saved = D.prec;
   D.prec = prec;
   D.ensure_precision_buffers();
+  K.ensure(D, kmax);
+  hipGraphExec_t iteration = o.graph ? K.ensure_graph(D) : nullptr;
 
-  const int nseg = (int)D.H.seg_off.size();
-  const int spb = 16;
-  const int nb = (nseg + spb - 1) / spb;
-
-  auto multidot = [&](int k, const double* w, double* c_dev) {
-    hipLaunchKernelGGL(k_multidot_partial, dim3(nb, k), dim3(256), 0, s, D.seg_off.p, D.seg_len.p, D.seg_w.p, nseg,
-                       spb, K.Q.p, n, w, K.part.p, nb);
-    hipLaunchKernelGGL(k_multidot_final, dim3(k), dim3(256), 0, s, K.part.p, nb, c_dev);
-    VMPS_HIP_CHECK(hipGetLastError());
-  };
-
-  // q0 = v / ||v||
+  // q0 = v / ||v|| into the basis and into the HxV input; k = 0
   D.dot_async(d_v, d_v, K.scal.p, s);
   hipLaunchKernelGGL(k_normalize_into, dim3(vgrid(n)), dim3(256), 0, s, K.scal.p, d_v, K.Q.p, n);
+  hipLaunchKernelGGL(k_copy, dim3(vgrid(n)), dim3(256), 0, s, K.Q.p, K.v.p, n);
+  hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, s, K.kdev.p, 0);
   VMPS_HIP_CHECK(hipGetLastError());
 
-  std::vector<double> alpha, beta, y, c_host(kmax + 1);
+  std::vector<double> alpha, beta, y;
   double E = 0.0, Eprev = 0.0;
   int k = 0;
   bool converged = false;
   for (k = 0; k < kmax; ++k) {
-    double* qk = K.Q.p + (int64_t)k * n;
-    D.hxv(qk, K.w.p, s);
+    if (iteration)
+      VMPS_HIP_CHECK(hipGraphLaunch(iteration, s));
+    else
+      K.enqueue_iteration(D, s);
     ++res.hxv_calls;
-    // two passes of classical Gram-Schmidt against q_0 .. q_k
-    multidot(k + 1, K.w.p, K.c.p);
-    hipLaunchKernelGGL(k_sub_combination, dim3(vgrid(n)), dim3(256), 0, s, K.Q.p, n, K.c.p, k + 1, K.w.p);
-    multidot(k + 1, K.w.p, K.c.p + kmax);
-    hipLaunchKernelGGL(k_sub_combination, dim3(vgrid(n)), dim3(256), 0, s, K.Q.p, n, K.c.p + kmax, k + 1, K.w.p);
-    D.dot_async(K.w.p, K.w.p, K.scal.p, s);
-    VMPS_HIP_CHECK(hipGetLastError());
-    double hc[2];
-    VMPS_HIP_CHECK(hipMemcpyAsync(c_host.data(), K.c.p, sizeof(double) * (k + 1), hipMemcpyDeviceToHost, s));
-    VMPS_HIP_CHECK(hipMemcpyAsync(hc, K.c.p + kmax + k, sizeof(double), hipMemcpyDeviceToHost, s));
-    VMPS_HIP_CHECK(hipMemcpyAsync(hc + 1, K.scal.p, sizeof(double), hipMemcpyDeviceToHost, s));
+    VMPS_HIP_CHECK(hipMemcpyAsync(K.host3, K.out3.p, 3 * sizeof(double), hipMemcpyDeviceToHost, s));
     VMPS_HIP_CHECK(hipStreamSynchronize(s));
-    const double a_k = c_host[k] + hc[0];
-    const double b_k = std::sqrt(std::max(0.0, hc[1]));
+    const double a_k = K.host3[0] + K.host3[1];
+    const double b_k = std::sqrt(std::max(0.0, K.host3[2]));
     alpha.push_back(a_k);
     beta.push_back(b_k);
     Eprev = E;
@@ -273,11 +400,6 @@ LanczosResult lanczos_solve(DevPlan& D, double* d_v, const LanczosOpts& o, vmps_
       ++k;
       break;
     }
-    if (k + 1 < kmax) {
-      hipLaunchKernelGGL(k_normalize_into, dim3(vgrid(n)), dim3(256), 0, s, K.scal.p, K.w.p, K.Q.p + (int64_t)(k + 1) * n,
-                         n);
-      VMPS_HIP_CHECK(hipGetLastError());
-    }
   }
   if (!converged) k = res.iterations;
   // Ritz vector psi = Q y (k basis vectors), normalised, then the true residual
@@ -290,14 +412,14 @@ LanczosResult lanczos_solve(DevPlan& D, double* d_v, const LanczosOpts& o, vmps_
   D.hxv(d_v, K.w.p, s);
   ++res.hxv_calls;
   D.dot_async(d_v, K.w.p, K.scal.p, s);
-  double rq = 0.0;
-  VMPS_HIP_CHECK(hipMemcpyAsync(&rq, K.scal.p, sizeof(double), hipMemcpyDeviceToHost, s));
+  VMPS_HIP_CHECK(hipMemcpyAsync(K.host3, K.scal.p, sizeof(double), hipMemcpyDeviceToHost, s));
   VMPS_HIP_CHECK(hipStreamSynchronize(s));
+  const double rq = K.host3[0];
   dev_axpby(-rq, d_v, 1.0, K.w.p, n, s);
   D.dot_async(K.w.p, K.w.p, K.scal.p, s);
-  double r2 = 0.0;
-  VMPS_HIP_CHECK(hipMemcpyAsync(&r2, K.scal.p, sizeof(double), hipMemcpyDeviceToHost, s));
+  VMPS_HIP_CHECK(hipMemcpyAsync(K.host3, K.scal.p, sizeof(double), hipMemcpyDeviceToHost, s));
   VMPS_HIP_CHECK(hipStreamSynchronize(s));
+  const double r2 = K.host3[0];
   res.energy = rq;
   res.ritz = E;
   res.residual = std::sqrt(std::max(0.0, r2));
@@ -309,15 +431,33 @@ LanczosResult lanczos_solve(DevPlan& D, double* d_v, const LanczosOpts& o, vmps_
 void Krylov::ensure(DevPlan& D, int kmax) {
   const int64_t n = D.H.vsize;
   if (cap >= kmax && nvec == n) return;
+  release_graph();
   Q.alloc((size_t)(kmax + 1) * n);
   w.alloc(n);
+  v.alloc(n);
   c.alloc(2 * (size_t)kmax + 2);
   scal.alloc(8);
-  const int nseg = (int)D.H.seg_off.size();
-  const int nb = (nseg + 15) / 16;
-  part.alloc((size_t)std::max(1, nb) * (kmax + 1));
+  out3.alloc(4);
+  kdev.alloc(1);
+  if (!host3) VMPS_HIP_CHECK(hipHostMalloc((void**)&host3, 4 * sizeof(double), hipHostMallocDefault));
+  // per-element coeff_dot weights and element chunks of the row dot products
+  std::vector<double> hw((size_t)n, 1.0);
+  for (size_t sg = 0; sg < D.H.seg_off.size(); ++sg)
+    for (int32_t e = 0; e < D.H.seg_len[sg]; ++e) hw[(size_t)(D.H.seg_off[sg] + e)] = D.H.seg_w[sg];
+  ew.upload(hw, D.stream);
+  ngrp = (int)std::max<int64_t>(1, std::min<int64_t>(1024, (n + 4095) / 4096));
+  chunk = (n + ngrp - 1) / ngrp;
+  npart.alloc(vgrid(n));
+  gpart.alloc((size_t)std::max(1, ngrp) * (kmax + 1));
+  VMPS_HIP_CHECK(hipStreamSynchronize(D.stream));
   cap = kmax;
   nvec = n;
+}
+
+Krylov::~Krylov() {
+  release_graph();
+  if (host3) (void)hipHostFree(host3);
+  host3 = nullptr;
 }
 
 }  // namespace vmps

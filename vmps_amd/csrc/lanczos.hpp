@@ -11,6 +11,7 @@ struct LanczosOpts {
   int min_iter = 2;
   double tol_eigval = 1e-8;  // DmrgTypedefs.h:371 default
   double tol_state = 1e-5;   // DmrgTypedefs.h:372 default
+  bool graph = true;         // replay one captured HIP graph per iteration
 };
 
 struct LanczosResult {

@@ -190,7 +190,7 @@ def dot(H, x, y):
 
 
 def lanczos(H, v0=None, precision="fp64", refine_fp64=True, max_iter=100, tol_eigval=1e-8, tol_state=1e-5,
-            refine_tol_eigval=1e-12, refine_tol_state=1e-7, seed=1):
+            refine_tol_eigval=1e-12, refine_tol_state=1e-7, seed=1, graph=True):
     """Lowest eigenpair of the pivot (the LanczosSolver::edgeState call of
     DmrgSolver::iteration_two).  Returns (eigenvector, result dict)."""
     o = _lib.LanczosOpts()
@@ -200,6 +200,7 @@ def lanczos(H, v0=None, precision="fp64", refine_fp64=True, max_iter=100, tol_ei
     o.refine_fp64 = 1 if refine_fp64 else 0
     o.refine_tol_eigval, o.refine_tol_state = refine_tol_eigval, refine_tol_state
     o.seed = seed
+    o.graph = 1 if graph else 0
     r = _lib.LanczosResult()
     if v0 is None:  # GaussianRandomVector::fill: uniform(-1, 1) entries
         v0 = np.random.default_rng(seed).uniform(-1.0, 1.0, H.vsize)
