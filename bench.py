@@ -190,7 +190,7 @@ def main():
             mixed[prec] = {"gflops": st["flops_alg"] / sm / 1e9, "ms_per_hxv": sm * 1e3,
                            "speedup_vs_fp64": sec / sm,
                            "rel_err_vs_fp64": float(np.linalg.norm(o - out_gpu) / np.linalg.norm(out_gpu)),
-                           "tiles": {"wave_edge": tm["wave_edge"], "big_min": tm["big_min"]}}
+                           "tiles": {k: tm[k] for k in ("wave_edge", "big_min", "explicit_u")}}
             Hm.close()
 
     # per-kernel device times (HIP events on the plan's stream)
@@ -240,7 +240,7 @@ def main():
             "parallelism": "single GPU",
         },
         "stage_ms": {"A_LV": ms[0], "C_combine_UR": ms[2], "hxv": ms[3]},
-        "tiles": {"wave_edge": tiles["wave_edge"], "big_min": tiles["big_min"],
+        "tiles": {"wave_edge": tiles["wave_edge"], "big_min": tiles["big_min"], "explicit_u": tiles["explicit_u"],
                   "autotuned": not args.no_tune},
         "mixed_hxv": mixed,
         "roofline": {

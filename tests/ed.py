@@ -67,8 +67,10 @@ def hubbard_ground(L, t=1.0, U=8.0, N=None, two_sz=0):
     return float(eigsh(Hs, k=1, which="SA", tol=1e-13)[0][0])
 
 
-def heisenberg_dense(L, J=1.0, S=0.5):
-    """Dense spin-S Heisenberg chain and total Sz, S^2 (basis: site 0 most significant)."""
+def heisenberg_dense(L, J=1.0, S=0.5, Js=None):
+    """Dense spin-S Heisenberg chain and total Sz, S^2 (basis: site 0 most
+    significant).  Js = [J_1, J_2, ...] couples sites at distance d with J_d
+    (default: J at distance 1 only)."""
     d = int(round(2 * S)) + 1
     m = np.arange(S, -S - 1, -1)[::-1]  # local Sz values, index 0 = lowest
     Sz = np.diag(m)
@@ -78,8 +80,11 @@ def heisenberg_dense(L, J=1.0, S=0.5):
     Sm = Sp.T
     ops = lambda A, i: _op(A, i, L, d).toarray()
     H = np.zeros((d ** L, d ** L))
-    for i in range(L - 1):
-        H += J * (ops(Sz, i) @ ops(Sz, i + 1) + 0.5 * (ops(Sp, i) @ ops(Sm, i + 1) + ops(Sm, i) @ ops(Sp, i + 1)))
+    Js = [J] if Js is None else list(Js)
+    for dist, Jd in enumerate(Js, start=1):
+        for i in range(L - dist):
+            j = i + dist
+            H += Jd * (ops(Sz, i) @ ops(Sz, j) + 0.5 * (ops(Sp, i) @ ops(Sm, j) + ops(Sm, i) @ ops(Sp, j)))
     Szt = sum(ops(Sz, i) for i in range(L))
     Spt = sum(ops(Sp, i) for i in range(L))
     S2 = Spt.T @ Spt + Szt @ Szt + Szt  # S^2 = S-S+ + Sz^2 + Sz
@@ -93,10 +98,10 @@ def heisenberg_sector_spectrum(L, two_sz, J=1.0):
     return np.sort(np.linalg.eigvalsh(H[np.ix_(idx, idx)]))
 
 
-def heisenberg_multiplet_spectrum(L, S_tot, J=1.0):
+def heisenberg_multiplet_spectrum(L, S_tot, J=1.0, Js=None):
     """One eigenvalue per spin-S_tot multiplet: H restricted to the highest-weight
     states (Sz = S_tot and S^2 = S_tot(S_tot+1))."""
-    H, sz, S2 = heisenberg_dense(L, J)
+    H, sz, S2 = heisenberg_dense(L, J, Js=Js)
     idx = np.where(np.isclose(sz, S_tot))[0]
     w, U = np.linalg.eigh(S2[np.ix_(idx, idx)])
     P = U[:, np.isclose(w, S_tot * (S_tot + 1))]
@@ -105,9 +110,15 @@ def heisenberg_multiplet_spectrum(L, S_tot, J=1.0):
     return np.sort(np.linalg.eigvalsh(P.T @ H[np.ix_(idx, idx)] @ P))
 
 
-def hubbard_dense(L, t=1.0, U=8.0):
-    """Dense Hubbard chain with an explicit Jordan-Wigner string; returns H,
-    the diagonals of N and 2 Sz, and S^2."""
+def ladder_bonds(L):
+    """Bonds of the two-leg ladder in snake order: rungs (2x, 2x+1), legs (i, i+2)."""
+    return [(i, i + 1) for i in range(0, L - 1, 2)] + [(i, i + 2) for i in range(L - 2)]
+
+
+def hubbard_dense(L, t=1.0, U=8.0, bonds=None):
+    """Dense Hubbard model with an explicit Jordan-Wigner string; returns H,
+    the diagonals of N and 2 Sz, and S^2.  bonds: hopping pairs (i, j), i < j
+    (default: the open chain)."""
     c_up = np.zeros((4, 4))
     c_dn = np.zeros((4, 4))
     c_up[0, 1] = 1.0
@@ -119,9 +130,9 @@ def hubbard_dense(L, t=1.0, U=8.0):
     H = np.zeros((4 ** L, 4 ** L))
     for i in range(L):
         H += U * ops(c_up.T @ c_up @ c_dn.T @ c_dn, i)
-    for i in range(L - 1):
+    for (i, j) in (bonds if bonds is not None else [(i, i + 1) for i in range(L - 1)]):
         for c in (c_up, c_dn):
-            ci, cj = ops(c, i, F), ops(c, i + 1, F)
+            ci, cj = ops(c, i, F), ops(c, j, F)
             H -= t * (ci.T @ cj + cj.T @ ci)
     n_up, n_dn = c_up.T @ c_up, c_dn.T @ c_dn
     N = sum(np.diag(ops(n_up + n_dn, i)) for i in range(L))
@@ -133,9 +144,9 @@ def hubbard_dense(L, t=1.0, U=8.0):
     return H, N, two_sz, S2
 
 
-def hubbard_multiplet_spectrum(L, N_tot, S_tot, t=1.0, U=8.0):
+def hubbard_multiplet_spectrum(L, N_tot, S_tot, t=1.0, U=8.0, bonds=None):
     """One eigenvalue per spin-S_tot multiplet at particle number N_tot."""
-    H, N, two_sz, S2 = hubbard_dense(L, t, U)
+    H, N, two_sz, S2 = hubbard_dense(L, t, U, bonds)
     idx = np.where(np.isclose(N, N_tot) & np.isclose(two_sz, 2 * S_tot))[0]
     w, Uv = np.linalg.eigh(S2[np.ix_(idx, idx)])
     P = Uv[:, np.isclose(w, S_tot * (S_tot + 1))]

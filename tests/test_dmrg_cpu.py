@@ -99,3 +99,24 @@ def test_split_then_contract_roundtrip(model, left_to_right):
     kept_w = float(np.sum(w * back * v))
     assert tw > 0
     assert abs(kept_w + tw - 1.0) <= 1e-12
+
+
+def test_heisenberg_su2_long_range_matches_ed():
+    """Long-range SU(2) MPO (spin-1 channels passed on with the identity,
+    D_W = R + 2): DMRG equals the lowest singlet of the J1-J2-J3 chain."""
+    Js = [1.0, 0.45, -0.2]
+    e_ed = ed.heisenberg_multiplet_spectrum(8, 0, Js=Js)[0]
+    r = dmrg.run_dmrg("HeisenbergSU2LR", 8, 16, 3, backend=dmrg_oracle.OracleBackend(), params=dict(Js=Js))
+    assert abs(r["energy"] - e_ed) <= 1e-9 * abs(e_ed)
+
+
+@pytest.mark.parametrize("N,S", [(6, 0), (5, 0.5)])
+def test_hubbard_su2xu1_ladder_matches_ed(N, S):
+    """Snake-ordered two-leg Hubbard ladder 2x3 (rungs at distance 1, legs at
+    distance 2 crossing one site with the fermionic string): DMRG in the
+    (2S+1, N) sector equals the lowest multiplet of exact diagonalisation."""
+    L = 6
+    e_ed = ed.hubbard_multiplet_spectrum(L, N, S, t=1.0, U=6.0, bonds=ed.ladder_bonds(L))[0]
+    r = dmrg.run_dmrg("HubbardSU2xU1Ladder", L, 48, 3, Q=(int(round(2 * S)) + 1, N),
+                      backend=dmrg_oracle.OracleBackend(), params=dict(t=1.0, U=6.0))
+    assert abs(r["energy"] - e_ed) <= 1e-9 * abs(e_ed)

@@ -192,3 +192,17 @@ def test_sum_of_terms_matches_oracle(kind):
     v = synth.random_vector(pd, 3)
     ref = pivots.oracle_hxv_terms(oracle.load(), pd, v)
     assert _rel(HxV(H, v), ref) <= FP64_RTOL
+
+
+@pytest.mark.parametrize("name,M", [("hubbard_su2xu1_ladder_L64_M2048", 256), ("heisenberg_su2_lr40_L60_M1024", 96)])
+@pytest.mark.parametrize("precision,tol", [("fp64", FP64_RTOL), ("bf16x3", BF16X3_RTOL)])
+def test_su2_workloads_match_oracle(name, M, precision, tol):
+    """The SU(2) workload shapes (snake ladder SU(2) x U(1), D_W = 6; long-range
+    SU(2) Heisenberg, D_W = 40) at reduced bond dimension against the oracle."""
+    pd = synth.make_workload(name, M=M)
+    print("case", name, "M", M, precision, "vsize", pd.vsize, flush=True)
+    H = PivotMatrix2(pd, precision=precision)
+    v = synth.random_vector(pd, 21)
+    rel = _rel(HxV(H, v), _oracle_out(pd, v, nthreads=16))
+    print("rel err", rel)
+    assert rel <= tol

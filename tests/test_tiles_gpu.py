@@ -58,6 +58,35 @@ def test_bf16x3_big_thresholds(case, big_min):
 
 
 @pytest.mark.parametrize("precision,tol", [("fp64", FP64_RTOL), ("fp32", 1e-5), ("bf16x3", BF16X3_RTOL)])
+@pytest.mark.parametrize("which", ["heis", "lr40"])
+def test_explicit_stage_b(case, precision, tol, which):
+    """U = sum c T formed by its own kernel, stage C reading U: same result as
+    the fused combine (and the oracle), on the D_W = 5 chain and on the
+    long-range D_W = 40 SU(2) pivot with many terms per output block."""
+    if which == "heis":
+        pd, v, ref = case
+    else:
+        pd = synth.make_workload("heisenberg_su2_lr40_L60_M1024", M=96)
+        v = synth.random_vector(pd, 8)
+        lib = oracle.load()
+        h = pivots.build_oracle(lib, pd)
+        try:
+            ref = pivots.oracle_hxv(lib, h, v, nthreads=8)
+        finally:
+            lib.oracle_pivot2_destroy(h)
+    print("explicit stage B", which, precision, flush=True)
+    H = PivotMatrix2(pd, precision=precision)
+    fused = HxV(H, v)
+    H.set_tiles(explicit_u=1)
+    assert H.tiles()["explicit_u"] == 1
+    out = HxV(H, v)
+    assert _rel(out, ref) <= tol
+    assert _rel(out, fused) <= (1e-15 if precision == "fp64" else 2 * tol)
+    H.set_tiles(explicit_u=0)  # and back
+    assert _rel(HxV(H, v), fused) == 0.0
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", FP64_RTOL), ("fp32", 1e-5), ("bf16x3", BF16X3_RTOL)])
 def test_autotune_picks_candidate_and_caches(case, precision, tol):
     pd, v, ref = case
     print("autotune", precision, flush=True)
@@ -74,7 +103,8 @@ def test_autotune_picks_candidate_and_caches(case, precision, tol):
     H2 = PivotMatrix2(pd, precision=precision)
     cfg2 = H2.autotune(iters=3)
     assert cfg2["from_cache"] == 1
-    assert (cfg2["wave_edge"], cfg2["big_min"]) == (cfg["wave_edge"], cfg["big_min"])
+    assert (cfg2["wave_edge"], cfg2["big_min"], cfg2["explicit_u"]) == (cfg["wave_edge"], cfg["big_min"],
+                                                                      cfg["explicit_u"])
     assert _rel(HxV(H2, v), ref) <= tol
 
 

@@ -542,22 +542,28 @@ vmps_status vmps_plan_time_hxv(vmps_plan* plan, int warmup, int iters, double* m
 vmps_status vmps_plan_get_tiles(const vmps_plan* plan, vmps_tile_config* cfg) {
   VMPS_API_BEGIN
   require(plan && cfg, VMPS_ERR_ARG, "NULL argument");
-  *cfg = vmps_tile_config{plan->D.H.wave_edge, plan->D.H.big_min, 0.0, 0, 0};
+  const auto& H = plan->D.H;
+  *cfg = vmps_tile_config{H.wave_edge, H.big_min, H.explicit_u ? 1 : 0, 0.0, 0, 0};
   VMPS_API_END
 }
 
-vmps_status vmps_plan_set_tiles(vmps_plan* plan, int wave_edge, int big_min) {
+vmps_status vmps_plan_set_tiles(vmps_plan* plan, int wave_edge, int big_min, int explicit_u) {
   VMPS_API_BEGIN
   require(plan != nullptr, VMPS_ERR_ARG, "NULL plan");
   require(wave_edge == 16 || wave_edge == 32, VMPS_ERR_ARG, "wave_edge must be 16 or 32");
   require(big_min >= 0, VMPS_ERR_ARG, "big_min must be >= 0");
-  plan->D.retile(wave_edge, big_min);
+  require(explicit_u == 0 || explicit_u == 1, VMPS_ERR_ARG, "explicit_u must be 0 or 1");
+  plan->D.retile(wave_edge, big_min, explicit_u != 0);
   VMPS_API_END
 }
 
 namespace {
+struct TileChoice {
+  int wave_edge, big_min;
+  bool explicit_u;
+};
 std::mutex g_tune_mutex;
-std::map<std::string, std::pair<int, int>> g_tune_cache;  // (precision, histogram) -> (wave_edge, big_min)
+std::map<std::string, TileChoice> g_tune_cache;  // (precision, block-size histogram) -> choice
 }  // namespace
 
 vmps_status vmps_plan_autotune(vmps_plan* plan, int iters, vmps_tile_config* out) {
@@ -565,28 +571,32 @@ vmps_status vmps_plan_autotune(vmps_plan* plan, int iters, vmps_tile_config* out
   require(plan && iters > 0, VMPS_ERR_ARG, "bad argument");
   auto& D = plan->D;
   const std::string key = std::to_string((int)D.prec) + "/" + vmps::tile_signature(D.H);
-  vmps_tile_config res{D.H.wave_edge, D.H.big_min, 0.0, 0, 0};
+  auto result = [&](double ms, int from_cache, int tried) {
+    return vmps_tile_config{D.H.wave_edge, D.H.big_min, D.H.explicit_u ? 1 : 0, ms, from_cache, tried};
+  };
   {
     std::lock_guard<std::mutex> lock(g_tune_mutex);
     auto it = g_tune_cache.find(key);
     if (it != g_tune_cache.end()) {
-      D.retile(it->second.first, it->second.second);
-      res = vmps_tile_config{D.H.wave_edge, D.H.big_min, 0.0, 1, 0};
-      if (out) *out = res;
+      D.retile(it->second.wave_edge, it->second.big_min, it->second.explicit_u);
+      if (out) *out = result(0.0, 1, 0);
       return VMPS_OK;
     }
   }
-  // candidates of the plan's precision (the other parameter stays as it is)
-  std::vector<std::pair<int, int>> cand;
-  if (D.prec == VMPS_PREC_BF16X3) {
-    for (int b : {vmps::kBigMin, 2 * vmps::kBigMin, 0}) cand.push_back({D.H.wave_edge, b});
-  } else {
-    for (int e : {16, 32}) cand.push_back({e, D.H.big_min});
+  // candidates of the plan's precision (parameters it does not use stay as they are)
+  std::vector<TileChoice> cand;
+  for (bool xu : {false, true}) {
+    if (D.prec == VMPS_PREC_BF16X3) {
+      for (int b : {vmps::kBigMin, 2 * vmps::kBigMin, 0}) cand.push_back({D.H.wave_edge, b, xu});
+    } else {
+      for (int e : {16, 32}) cand.push_back({e, D.H.big_min, xu});
+    }
   }
   double best = -1.0;
-  std::pair<int, int> pick = {D.H.wave_edge, D.H.big_min};
+  TileChoice pick{D.H.wave_edge, D.H.big_min, D.H.explicit_u};
+  int tried = 0;
   for (const auto& c : cand) {
-    D.retile(c.first, c.second);
+    D.retile(c.wave_edge, c.big_min, c.explicit_u);
     for (int i = 0; i < 2; ++i) D.hxv(D.vin.p, D.vout.p, D.stream);
     VMPS_HIP_CHECK(hipEventRecord(D.ev0, D.stream));
     for (int i = 0; i < iters; ++i) D.hxv(D.vin.p, D.vout.p, D.stream);
@@ -595,21 +605,18 @@ vmps_status vmps_plan_autotune(vmps_plan* plan, int iters, vmps_tile_config* out
     float ms = 0.f;
     VMPS_HIP_CHECK(hipEventElapsedTime(&ms, D.ev0, D.ev1));
     const double t = (double)ms / iters;
-    ++res.n_tried;
+    ++tried;
     if (best < 0 || t < best) {
       best = t;
       pick = c;
     }
   }
-  D.retile(pick.first, pick.second);
+  D.retile(pick.wave_edge, pick.big_min, pick.explicit_u);
   {
     std::lock_guard<std::mutex> lock(g_tune_mutex);
     g_tune_cache[key] = pick;
   }
-  res.wave_edge = D.H.wave_edge;
-  res.big_min = D.H.big_min;
-  res.ms = best;
-  if (out) *out = res;
+  if (out) *out = result(best, 0, tried);
   VMPS_API_END
 }
 

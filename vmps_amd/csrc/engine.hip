@@ -12,6 +12,7 @@
 // load.  Every output element is owned by exactly one wave, so there are no
 // atomics and results are bitwise reproducible run to run.
 #include "engine.hpp"
+#include "combine.hpp"
 #include "wave.hpp"
 
 #include <algorithm>
@@ -374,10 +375,7 @@ void DevPlan::upload(vmps_precision precision) {
     VMPS_HIP_CHECK(hipMemsetAsync(vin.p, 0, std::max<int64_t>(1, H.ysize) * sizeof(double), stream));
     VMPS_HIP_CHECK(hipMemsetAsync(vout.p, 0, std::max<int64_t>(1, H.osize) * sizeof(double), stream));
   }
-  tilesA.upload(H.tilesA_w, stream);  // FP64 / FP32 wave kernels (edge H.wave_edge)
-  tilesC.upload(H.tilesC_w, stream);
-  chunksC.upload(H.chunksC, stream);
-  termsC.upload(H.termsC, stream);
+  upload_lists();
   seg_off.upload(H.seg_off, stream);
   seg_len.upload(H.seg_len, stream);
   seg_w.upload(H.seg_w, stream);
@@ -389,12 +387,34 @@ void DevPlan::upload(vmps_precision precision) {
   VMPS_HIP_CHECK(hipStreamSynchronize(stream));
 }
 
+void DevPlan::upload_lists() {
+  tilesA.upload(H.tilesA_w, stream);  // FP64 / FP32 wave kernels (edge H.wave_edge)
+  tilesC.upload(H.tilesC_w, stream);
+  chunksC.upload(H.chunksC, stream);
+  termsC.upload(H.termsC, stream);
+  combTasks.upload(H.comb_tasks, stream);
+  combTerms.upload(H.comb_terms, stream);
+  if (H.explicit_u) {
+    if (Ub.n != (size_t)H.U_size) Ub.alloc(H.U_size);
+    if (T32.p && U32.n != (size_t)H.U_size) U32.alloc(H.U_size);
+  } else {
+    Ub.release();
+    U32.release();
+  }
+  ++gen;
+}
+
 void DevPlan::ensure_precision_buffers() {
+  if (prec == VMPS_PREC_FP32 && H.explicit_u && T32.p && U32.n != (size_t)H.U_size) {
+    U32.alloc(H.U_size);
+    ++gen;
+  }
   if (prec == VMPS_PREC_FP32 && !T32.p && (H.T_size || H.vsize)) {
     L32.alloc(H.Ldata.size());
     R32.alloc(H.Rdata.size());
     V32.alloc(H.ysize);
     T32.alloc(H.T_size);
+    if (H.explicit_u) U32.alloc(H.U_size);
     if (L.n) hipLaunchKernelGGL(k_f64_to_f32, dim3(1024), dim3(256), 0, stream, L.p, L32.p, (int64_t)L.n);
     if (R.n) hipLaunchKernelGGL(k_f64_to_f32, dim3(1024), dim3(256), 0, stream, R.p, R32.p, (int64_t)R.n);
     VMPS_HIP_CHECK(hipGetLastError());
@@ -415,12 +435,19 @@ void DevPlan::launch_stage(int stage, const double* d_vin, double* d_vout, hipSt
                            T.p);
     }
     if (stage == 2 && nC) {
+      const double* src = T.p;
+      if (H.explicit_u) {  // stage B: U = sum c T, then stage C reads U
+        if (combTasks.n)
+          hipLaunchKernelGGL((k_combine<double, TermC>), dim3((unsigned)combTasks.n), dim3(256), 0, s, combTasks.p,
+                             combTerms.p, T.p, Ub.p);
+        src = Ub.p;
+      }
       if (H.wave_edge == 16)
         hipLaunchKernelGGL((k_stage_c<double, double, 16>), dim3(nC), dim3(256), 0, s, tilesC.p, nC, chunksC.p,
-                           termsC.p, T.p, R.p, d_vout);
+                           termsC.p, src, R.p, d_vout);
       else
         hipLaunchKernelGGL((k_stage_c<double, double, 32>), dim3(nC), dim3(256), 0, s, tilesC.p, nC, chunksC.p,
-                           termsC.p, T.p, R.p, d_vout);
+                           termsC.p, src, R.p, d_vout);
     }
   } else if (prec == VMPS_PREC_FP32) {
     if (stage == 0) {
@@ -435,12 +462,19 @@ void DevPlan::launch_stage(int stage, const double* d_vin, double* d_vout, hipSt
       }
     }
     if (stage == 2 && nC) {
+      const float* src = T32.p;
+      if (H.explicit_u) {
+        if (combTasks.n)
+          hipLaunchKernelGGL((k_combine<float, TermC>), dim3((unsigned)combTasks.n), dim3(256), 0, s, combTasks.p,
+                             combTerms.p, T32.p, U32.p);
+        src = U32.p;
+      }
       if (H.wave_edge == 16)
         hipLaunchKernelGGL((k_stage_c<float, double, 16>), dim3(nC), dim3(256), 0, s, tilesC.p, nC, chunksC.p,
-                           termsC.p, T32.p, R32.p, d_vout);
+                           termsC.p, src, R32.p, d_vout);
       else
         hipLaunchKernelGGL((k_stage_c<float, double, 32>), dim3(nC), dim3(256), 0, s, tilesC.p, nC, chunksC.p,
-                           termsC.p, T32.p, R32.p, d_vout);
+                           termsC.p, src, R32.p, d_vout);
     }
   } else if (prec == VMPS_PREC_BF16X3) {
     launch_stage_bf16x3(stage, d_vin, d_vout, s);

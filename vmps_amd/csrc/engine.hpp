@@ -89,6 +89,11 @@ struct DevPlan {
   DevBuf<double> L, R, T, vin, vout;
   // FP32 operands and intermediates (mixed modes)
   DevBuf<float> L32, R32, V32, T32;
+  // explicit stage B (H.explicit_u): U in FP64 / FP32 and its task tables
+  DevBuf<double> Ub;
+  DevBuf<float> U32;
+  DevBuf<CombTask> combTasks;
+  DevBuf<TermC> combTerms;
 
   // split-BF16 x3 operands, intermediates and work lists
   DevBuf<uint16_t> Lhi, Llo, Rhi, Rlo, Vhi, Vlo;
@@ -101,6 +106,9 @@ struct DevPlan {
   DevBuf<ChunkC16> bfC;
   DevBuf<TermC16> bfT;
   DevBuf<VConv> vconv;
+  DevBuf<float> Up;                     // explicit stage B: padded FP32 U
+  DevBuf<CombTask> bfCombTasks;
+  DevBuf<TermC16> bfCombTerms;
 
   // work lists
   DevBuf<TileA> tilesA;
@@ -126,11 +134,13 @@ struct DevPlan {
   void upload(vmps_precision prec);  // device copies of H
   void ensure_precision_buffers();
   void ensure_bf16_buffers();
+  void upload_bf16_lists(const Bf16Layout& B);
   bool bf16_ready = false;
-  // switch the tile configuration: FP64 / FP32 wave tile edge (16 or 32) and
-  // the split-BF16 big-tile threshold (H.big_min); rebuilds and uploads the
-  // affected work lists only
-  void retile(int wave_edge, int big_min);
+  // switch the tile configuration: FP64 / FP32 wave tile edge (16 or 32), the
+  // split-BF16 big-tile threshold (H.big_min) and fused / explicit stage B
+  // (H.explicit_u); rebuilds and uploads the work lists only
+  void retile(int wave_edge, int big_min, bool explicit_u);
+  void upload_lists();  // FP64 / FP32 work lists and the explicit stage B buffers
   void launch_stage_bf16x3(int stage, const double* d_vin, double* d_vout, hipStream_t s);
   void hxv(const double* d_vin, double* d_vout, hipStream_t s);
   // one stage of hxv alone: 0 = L.V, 2 = (sum c T).R; stage 1 (the combine) is fused into 2

@@ -10,6 +10,7 @@
 // tile, stage C one workgroup per tile with the combine sum_j c_j T_j done on
 // the operand load and a 4-way split of the K iterations reduced through LDS.
 #include "engine.hpp"
+#include "combine.hpp"
 #include "wave.hpp"
 
 namespace vmps {
@@ -580,6 +581,14 @@ void DevPlan::ensure_bf16_buffers() {
   Vlo.alloc(std::max<int64_t>(1, B.V_size));
   Tp.alloc(std::max<int64_t>(1, B.T_size));
   VMPS_HIP_CHECK(hipMemsetAsync(Tp.p, 0, Tp.n * sizeof(float), stream));  // padding columns stay zero
+  upload_bf16_lists(B);
+  VMPS_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_a16, k_stage_a_bf16x3, 256, 0));
+  VMPS_HIP_CHECK(hipStreamSynchronize(stream));
+  bf16_ready = true;
+  ++gen;
+}
+
+void DevPlan::upload_bf16_lists(const Bf16Layout& B) {
   bfA.upload(B.tilesA, stream);
   bfAbig.upload(B.tilesA_big, stream);
   bfCsmall.upload(B.tilesC_small, stream);
@@ -595,26 +604,29 @@ void DevPlan::ensure_bf16_buffers() {
   bfC.upload(B.chunksC, stream);
   bfT.upload(B.termsC, stream);
   vconv.upload(B.vconv, stream);
-  VMPS_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_a16, k_stage_a_bf16x3, 256, 0));
-  VMPS_HIP_CHECK(hipStreamSynchronize(stream));
-  bf16_ready = true;
-  ++gen;
+  bfCombTasks.upload(B.comb_tasks, stream);
+  bfCombTerms.upload(B.comb_terms, stream);
+  if (H.explicit_u)
+    Up.alloc(std::max<int64_t>(1, B.U_size));
+  else
+    Up.release();
 }
 
-void DevPlan::retile(int wave_edge, int big_min) {
+void DevPlan::retile(int wave_edge, int big_min, bool explicit_u) {
   ++gen;
-  if (wave_edge != H.wave_edge) {
-    set_wave_edge(H, wave_edge);
-    tilesA.upload(H.tilesA_w, stream);
-    tilesC.upload(H.tilesC_w, stream);
+  const bool bf16_lists = bf16_ready && (big_min != H.big_min || explicit_u != H.explicit_u);
+  if (explicit_u != H.explicit_u) {
+    H.explicit_u = explicit_u;
+    build_tiles(H);  // chunk / term tables and every tile list
   }
-  if (big_min != H.big_min) {
-    H.big_min = big_min;
-    if (bf16_ready) {  // the split operands are rebuilt with the tile lists
-      VMPS_HIP_CHECK(hipStreamSynchronize(stream));
-      bf16_ready = false;
-      ensure_bf16_buffers();
-    }
+  set_wave_edge(H, wave_edge);
+  H.big_min = big_min;
+  upload_lists();
+  if (bf16_lists) {  // split-BF16 work lists; the split operands stay
+    VMPS_HIP_CHECK(hipStreamSynchronize(stream));
+    Bf16Layout B;
+    build_bf16_layout(H, B, false);
+    upload_bf16_lists(B);
   }
   VMPS_HIP_CHECK(hipStreamSynchronize(stream));
 }
@@ -633,15 +645,22 @@ void DevPlan::launch_stage_bf16x3(int stage, const double* d_vin, double* d_vout
                          (const __bf16*)Lhi.p, (const __bf16*)Llo.p, (const __bf16*)Vhi.p, (const __bf16*)Vlo.p, Tp.p);
   } else if (stage == 2) {
     const int nW = (int)wTile.n, nR = (int)rTile.n;
+    const float* src = Tp.p;
+    if (H.explicit_u) {  // stage B: U = sum c T, then stage C reads U
+      if (bfCombTasks.n)
+        hipLaunchKernelGGL((k_combine<float, TermC16>), dim3((unsigned)bfCombTasks.n), dim3(256), 0, s,
+                           bfCombTasks.p, bfCombTerms.p, Tp.p, Up.p);
+      src = Up.p;
+    }
     if (nCb && nW)
       hipLaunchKernelGGL(k_stage_c_bf16x3_big, dim3((unsigned)nW), dim3(256), 0, s, bfCbig.p, wTile.p, wBegin.p,
-                         wEnd.p, wSlot.p, nW, bfC.p, bfT.p, Tp.p, (const __bf16*)Rhi.p, (const __bf16*)Rlo.p, d_vout,
+                         wEnd.p, wSlot.p, nW, bfC.p, bfT.p, src, (const __bf16*)Rhi.p, (const __bf16*)Rlo.p, d_vout,
                          Cpart.p);
     if (nR)
       hipLaunchKernelGGL(k_reduce_partials, dim3((unsigned)nR), dim3(256), 0, s, bfCbig.p, rTile.p, rSlot.p, rNs.p,
                          Cpart.p, d_vout);
     if (nC)
-      hipLaunchKernelGGL(k_stage_c_bf16x3, dim3((unsigned)nC), dim3(256), 0, s, bfCsmall.p, nC, bfC.p, bfT.p, Tp.p,
+      hipLaunchKernelGGL(k_stage_c_bf16x3, dim3((unsigned)nC), dim3(256), 0, s, bfCsmall.p, nC, bfC.p, bfT.p, src,
                          (const __bf16*)Rhi.p, (const __bf16*)Rlo.p, d_vout);
   }
   VMPS_HIP_CHECK(hipGetLastError());

@@ -629,7 +629,19 @@ std::string tile_signature(const HostPlan& H) {
 void build_tiles(HostPlan& H) {
   H.chunksC.clear();
   H.termsC.clear();
-  for (size_t j = 0; j < H.comb_t.size(); ++j) H.termsC.push_back({H.T_off[H.comb_t[j]], H.comb_c[j]});
+  H.comb_tasks.clear();
+  H.comb_terms.clear();
+  if (H.explicit_u) {
+    // stage C reads U_u directly: term u = (U offset, 1)
+    for (const CombineList& cl : H.comb) H.termsC.push_back({cl.u_off, 1.0});
+    for (size_t j = 0; j < H.comb_t.size(); ++j) H.comb_terms.push_back({H.T_off[H.comb_t[j]], H.comb_c[j]});
+    for (const CombineList& cl : H.comb)
+      for (int64_t e0 = 0; e0 < cl.size; e0 += kCombTaskElems)
+        H.comb_tasks.push_back({cl.u_off, e0, (int32_t)std::min<int64_t>(kCombTaskElems, cl.size - e0), cl.begin,
+                                cl.end, 0});
+  } else {
+    for (size_t j = 0; j < H.comb_t.size(); ++j) H.termsC.push_back({H.T_off[H.comb_t[j]], H.comb_c[j]});
+  }
   std::vector<std::pair<int32_t, int32_t>>& crange = H.crange;
   crange.assign(H.probC.size(), {0, 0});
   for (size_t p = 0; p < H.probC.size(); ++p) {
@@ -638,6 +650,10 @@ void build_tiles(HostPlan& H) {
     for (int32_t k = pr.chunk_begin; k < pr.chunk_end; ++k) {
       const GemmChunk& ch = H.chunkC[k];
       const CombineList& cl = H.comb[ch.pad];
+      if (H.explicit_u) {
+        H.chunksC.push_back({ch.b_off, ch.K, ch.pad, ch.pad + 1, 0});
+        continue;
+      }
       for (int32_t j0 = cl.begin; j0 < cl.end; j0 += kMaxTermsPerChunk) {
         ChunkC c;
         c.r_off = ch.b_off;
@@ -652,7 +668,16 @@ void build_tiles(HostPlan& H) {
   }
   make_tiles(H, kTile, crange, H.tilesA, H.tilesC);
   set_wave_edge(H, choose_wave_edge(H));
-  // bounds of the fused stage C: every T read stays inside T
+  // bounds of stage C: every T (or U) read stays inside T (or U)
+  const int64_t src_size = H.explicit_u ? H.U_size : H.T_size;
+  for (const CombTask& t : H.comb_tasks) {
+    if (t.term_begin < 0 || t.term_end > (int32_t)H.comb_terms.size() || t.term_end < t.term_begin || t.n < 0 ||
+        t.u_off + t.e0 + t.n > H.U_size)
+      throw VmpsError(VMPS_ERR_INTERNAL, "stage B: task out of range");
+    for (int32_t j = t.term_begin; j < t.term_end; ++j)
+      if (H.comb_terms[j].t_off < 0 || H.comb_terms[j].t_off + t.e0 + t.n > H.T_size)
+        throw VmpsError(VMPS_ERR_INTERNAL, "stage B: T block out of range");
+  }
   for (const ChunkC& c : H.chunksC) {
     if (c.term_end - c.term_begin > kMaxTermsPerChunk || c.term_end < c.term_begin)
       throw VmpsError(VMPS_ERR_INTERNAL, "stage C: chunk term list too long");
@@ -666,7 +691,7 @@ void build_tiles(HostPlan& H) {
         if (H.chunksC[c].r_off < 0 || H.chunksC[c].r_off + (int64_t)H.chunksC[c].K * t.N > (int64_t)H.Rdata.size())
           throw VmpsError(VMPS_ERR_INTERNAL, "stage C: R block out of range");
         for (int32_t j = H.chunksC[c].term_begin; j < H.chunksC[c].term_end; ++j)
-          if (H.termsC[j].t_off + (int64_t)t.M * H.chunksC[c].K > H.T_size)
+          if (H.termsC[j].t_off + (int64_t)t.M * H.chunksC[c].K > src_size)
             throw VmpsError(VMPS_ERR_INTERNAL, "stage C: T block out of range");
       }
   for (const auto* list : {&H.tilesA, &H.tilesA_w})
@@ -893,22 +918,25 @@ void split(double x, uint16_t& hi, uint16_t& lo) {
 }
 }  // namespace
 
-void build_bf16_layout(const HostPlan& H, Bf16Layout& B) {
+void build_bf16_layout(const HostPlan& H, Bf16Layout& B, bool operands) {
   B = Bf16Layout();
   // L^T, row-major M x Kp at 8-aligned offsets
-  std::unordered_map<int64_t, int64_t> Lmap, Rmap, Vmap, Tmap;
+  std::unordered_map<int64_t, int64_t> Lmap, Rmap, Vmap, Tmap, Umap;
   int64_t o = 0;
   for (const BlockDim& b : H.Lblk) {
     Lmap[b.off] = o;
     o += b.rows * pad8(b.cols);
   }
-  B.Lhi.assign(o, 0);
-  B.Llo.assign(o, 0);
-  for (const BlockDim& b : H.Lblk) {
-    const int64_t Kp = pad8(b.cols), base = Lmap[b.off];
-    for (int64_t i = 0; i < b.rows; ++i)
-      for (int64_t k = 0; k < b.cols; ++k)
-        split(H.Ldata[b.off + k * b.rows + i], B.Lhi[base + i * Kp + k], B.Llo[base + i * Kp + k]);
+  B.L_size = o;
+  if (operands) {
+    B.Lhi.assign(o, 0);
+    B.Llo.assign(o, 0);
+    for (const BlockDim& b : H.Lblk) {
+      const int64_t Kp = pad8(b.cols), base = Lmap[b.off];
+      for (int64_t i = 0; i < b.rows; ++i)
+        for (int64_t k = 0; k < b.cols; ++k)
+          split(H.Ldata[b.off + k * b.rows + i], B.Lhi[base + i * Kp + k], B.Llo[base + i * Kp + k]);
+    }
   }
   // R, column-major Kp x N
   o = 0;
@@ -916,13 +944,16 @@ void build_bf16_layout(const HostPlan& H, Bf16Layout& B) {
     Rmap[b.off] = o;
     o += pad8(b.rows) * b.cols;
   }
-  B.Rhi.assign(o, 0);
-  B.Rlo.assign(o, 0);
-  for (const BlockDim& b : H.Rblk) {
-    const int64_t Kp = pad8(b.rows), base = Rmap[b.off];
-    for (int64_t j = 0; j < b.cols; ++j)
-      for (int64_t k = 0; k < b.rows; ++k)
-        split(H.Rdata[b.off + j * b.rows + k], B.Rhi[base + j * Kp + k], B.Rlo[base + j * Kp + k]);
+  B.R_size = o;
+  if (operands) {
+    B.Rhi.assign(o, 0);
+    B.Rlo.assign(o, 0);
+    for (const BlockDim& b : H.Rblk) {
+      const int64_t Kp = pad8(b.rows), base = Rmap[b.off];
+      for (int64_t j = 0; j < b.cols; ++j)
+        for (int64_t k = 0; k < b.rows; ++k)
+          split(H.Rdata[b.off + j * b.rows + k], B.Rhi[base + j * Kp + k], B.Rlo[base + j * Kp + k]);
+    }
   }
   // V, column-major Kp x N (filled per call)
   o = 0;
@@ -946,6 +977,23 @@ void build_bf16_layout(const HostPlan& H, Bf16Layout& B) {
     o += (int64_t)H.probA[t].M * pad8(H.probA[t].N);
   }
   B.T_size = o;
+  // explicit stage B: U blocks in the layout of their T blocks (row-major
+  // M x Kp, Kp = K of the stage C chunk padded to 8)
+  if (H.explicit_u) {
+    o = 0;
+    for (const CombineList& cl : H.comb) {
+      Umap[cl.u_off] = o;
+      const GemmProb& pa = H.probA[H.comb_t[cl.begin]];  // a T block of this U: same M x K
+      const int64_t Np = pad8(pa.N);
+      for (int64_t e0 = 0; e0 < (int64_t)pa.M * Np; e0 += kCombTaskElems)
+        B.comb_tasks.push_back({o, e0, (int32_t)std::min<int64_t>(kCombTaskElems, (int64_t)pa.M * Np - e0),
+                                cl.begin, cl.end, 0});
+      o += (int64_t)pa.M * Np;
+    }
+    B.U_size = o;
+    for (size_t j = 0; j < H.comb_t.size(); ++j)
+      B.comb_terms.push_back({Tmap.at(H.T_off[H.comb_t[j]]), (float)H.comb_c[j], 0});
+  }
   const int64_t big_min = H.big_min;
   auto is_big = [big_min](int64_t M, int64_t N) { return big_min > 0 && M >= big_min && N >= big_min; };
   for (const TileA& t : H.tilesA) {
@@ -1011,23 +1059,33 @@ void build_bf16_layout(const HostPlan& H, Bf16Layout& B) {
     d.pad = 0;
     B.chunksC.push_back(d);
   }
-  for (const TermC& t : H.termsC) B.termsC.push_back({Tmap.at(t.t_off), (float)t.c, 0});
+  for (const TermC& t : H.termsC)
+    B.termsC.push_back({H.explicit_u ? Umap.at(t.t_off) : Tmap.at(t.t_off), (float)t.c, 0});
+  const int64_t src_size = H.explicit_u ? B.U_size : B.T_size;
   // bounds: every padded access stays inside its array
+  for (const CombTask& t : B.comb_tasks) {
+    if (t.term_begin < 0 || t.term_end > (int32_t)B.comb_terms.size() || t.term_end < t.term_begin ||
+        t.u_off + t.e0 + t.n > B.U_size)
+      throw VmpsError(VMPS_ERR_INTERNAL, "bf16 stage B task out of range");
+    for (int32_t j = t.term_begin; j < t.term_end; ++j)
+      if (B.comb_terms[j].t_off + t.e0 + t.n > B.T_size)
+        throw VmpsError(VMPS_ERR_INTERNAL, "bf16 stage B term out of range");
+  }
   for (const auto* list : {&B.tilesA, &B.tilesA_big})
     for (const TileA16& a : *list)
-      if (a.a_off + (int64_t)a.M * a.Kp > (int64_t)B.Lhi.size() || a.b_off + (int64_t)a.Kp * a.N > B.V_size ||
+      if (a.a_off + (int64_t)a.M * a.Kp > B.L_size || a.b_off + (int64_t)a.Kp * a.N > B.V_size ||
           a.c_off + (int64_t)a.M * a.Np > B.T_size || (a.a_off | a.b_off | a.c_off) % 8)
         throw VmpsError(VMPS_ERR_INTERNAL, "bf16 stage A descriptor out of range");
   for (const TileC& t : H.tilesC)
     for (int32_t c = t.chunk_begin; c < t.chunk_end; ++c) {
       const ChunkC16& ch = B.chunksC[c];
-      if (ch.r_off % 8 || ch.r_off + (int64_t)ch.Kp * t.N > (int64_t)B.Rhi.size())
+      if (ch.r_off % 8 || ch.r_off + (int64_t)ch.Kp * t.N > B.R_size)
         throw VmpsError(VMPS_ERR_INTERNAL, "bf16 stage C operand out of range");
       if (ch.term_end - ch.term_begin > kMaxTermsPerChunk || ch.term_end < ch.term_begin ||
           ch.term_end > (int32_t)B.termsC.size())
         throw VmpsError(VMPS_ERR_INTERNAL, "bf16 stage C chunk term list out of range");
       for (int32_t j = ch.term_begin; j < ch.term_end; ++j)
-        if (B.termsC[j].t_off % 8 || B.termsC[j].t_off + (int64_t)t.M * ch.Kp > B.T_size)
+        if (B.termsC[j].t_off % 8 || B.termsC[j].t_off + (int64_t)t.M * ch.Kp > src_size)
           throw VmpsError(VMPS_ERR_INTERNAL, "bf16 stage C term out of range");
     }
   for (const VConv& v : B.vconv)

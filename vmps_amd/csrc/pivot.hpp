@@ -234,6 +234,17 @@ struct TermC16 {
   float c;
   int32_t pad;
 };
+// Explicit stage B: U_u = sum_j c_j T_j formed by its own kernel.  A task
+// covers elements [e0, e0 + n) of one U block (same shape and layout as its
+// T blocks), reading terms [term_begin, term_end) of the term table.
+struct CombTask {
+  int64_t u_off;   // first element of the U block
+  int64_t e0;      // first element of the task within the block
+  int32_t n;       // elements of the task
+  int32_t term_begin, term_end;
+  int32_t pad;
+};
+constexpr int kCombTaskElems = 4096;
 struct VConv {             // V block: packed FP64 (K x N, ld K) -> padded split BF16 (Kp x N)
   int64_t src_off, dst_off;
   int32_t K, N, Kp, pad;
@@ -245,7 +256,11 @@ constexpr int kBigK = 64;      // k depth of one LDS stage of those kernels
 constexpr int kBigMin = 96;
 struct Bf16Layout {
   std::vector<uint16_t> Lhi, Llo, Rhi, Rlo;  // split operands, built once
+  int64_t L_size = 0, R_size = 0;            // their sizes (also when not filled)
   int64_t V_size = 0, T_size = 0;            // padded sizes of the per-call arrays
+  int64_t U_size = 0;                        // padded U (explicit stage B only)
+  std::vector<CombTask> comb_tasks;          // explicit stage B on the padded layouts
+  std::vector<TermC16> comb_terms;
   std::vector<TileA16> tilesA_big;           // 128 x 128 tiles of the large stage A problems
   std::vector<TileC> tilesC_big, tilesC_small;  // stage C tiles: 128-unit and 32-unit
   // split-K work items of the big stage C tiles: tile index, iteration range
@@ -305,6 +320,12 @@ struct HostPlan {
   // split-BF16: problems with both dimensions >= big_min take the LDS-staged
   // kBigTile tiles (big_min <= 0: none do)
   int big_min = kBigMin;
+  // stage B: false = fused into stage C's operand load (U never stored);
+  // true = U formed by its own kernel (comb_tasks / comb_terms) and stage C's
+  // chunks read U with one unit-weight term each (termsC t_off = U offsets)
+  bool explicit_u = false;
+  std::vector<CombTask> comb_tasks;
+  std::vector<TermC> comb_terms;
   std::vector<ChunkC> chunksC;
   std::vector<TermC> termsC;
   std::vector<GemmProb> probA;       // stage A: one chunk each (L . V)
@@ -351,7 +372,9 @@ void set_wave_edge(HostPlan& H, int edge);
 // shape share one tuned tile configuration
 std::string tile_signature(const HostPlan& H);
 // Split-BF16 operands and work lists of the BF16x3 path (built on demand).
-void build_bf16_layout(const HostPlan& H, Bf16Layout& B);
+// split-BF16 layouts and work lists; with operands = false the split L / R
+// arrays are left empty (their offsets and sizes are still computed)
+void build_bf16_layout(const HostPlan& H, Bf16Layout& B, bool operands = true);
 uint16_t bf16_rne(float f);
 // Throws VMPS_ERR_INTERNAL if any descriptor would address outside its buffer.
 void validate_host_plan(const HostPlan& H);

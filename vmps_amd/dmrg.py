@@ -343,7 +343,8 @@ def run_dmrg(model, L, M, sweeps, Q=None, seed=20261202, backend=None, params=No
     qloc = sites[0].qloc
     sym = sites[0].sym
     if Q is None:
-        Q = {"HeisenbergU1": (0,), "HubbardU1xU1": (L, 0), "HeisenbergSU2": (1,), "HubbardSU2xU1": (1, L)}[model]
+        Q = {"HeisenbergU1": (0,), "HubbardU1xU1": (L, 0), "HeisenbergSU2": (1,), "HubbardSU2xU1": (1, L),
+             "HeisenbergSU2LR": (1,), "HubbardSU2xU1Ladder": (1, L)}[model]
     Q = tuple(Q)
     rng = np.random.default_rng(seed)
     A, bonds = random_mps(sites, Q, M, rng)

@@ -13,6 +13,8 @@ Models:
   HeisenbergU1   S=1/2, H = J sum Sz Sz + J/2 (S+ S- + S- S+)   (models/HeisenbergU1.h)
   HubbardU1xU1   H = -t sum_sigma (c+ c + h.c.) + U sum n_up n_dn (models/HubbardU1xU1.h)
   HeisenbergSU2  S=1/2, H = J sum S.S in SU(2)-reduced form          (models/HeisenbergSU2.h)
+  HeisenbergSU2LR  H = sum_d J_d sum_i S_i.S_{i+d}, range R, D_W = R + 2
+  HubbardSU2xU1 / HubbardSU2xU1Ladder  chain / snake-ordered two-leg ladder
 """
 from dataclasses import dataclass, field
 import numpy as np
@@ -339,6 +341,54 @@ def hubbard_su2xu1_mpo(t=1.0, U=8.0, position="bulk"):
     return _place(SU2xU1, qloc, q, ent, position)
 
 
+def heisenberg_su2_lr_mpo(Js, position="bulk", S=0.5):
+    """H = sum_i sum_{d=1..R} Js[d-1] S_i . S_{i+d} in SU(2)-reduced form
+    (the long-range Heisenberg MPO; HeisenbergSU2 with Jfull,
+    models/HeisenbergSU2.h:223-229, has this channel structure).  Auxiliary
+    order: 0 done | d = 1..R: an S placed d sites ago (q = 3) | R+1 start;
+    D_W = R + 2.  The spin-1 channel passes sites with the identity."""
+    R = len(Js)
+    q2 = int(round(2 * S)) + 1
+    red = np.array([[np.sqrt(S * (S + 1.0))]])
+    Id = np.eye(1)
+    qloc = [(q2,)]
+    q = [(1,)] + [(3,)] * R + [(1,)]
+    ent = [(0, 0, Id, (1,)), (R + 1, 1, np.sqrt(3.0) * (-red), (3,)), (R + 1, R + 1, Id, (1,))]
+    for d in range(1, R + 1):
+        if Js[d - 1] != 0.0:
+            ent.append((d, 0, Js[d - 1] * red, (3,)))
+        if d < R:
+            ent.append((d, d + 1, Id, (1,)))
+    return _place(SU2, qloc, q, ent, position)
+
+
+def hubbard_su2xu1_ladder_mpo(t=1.0, U=6.0, rung=True, position="bulk"):
+    """Two-leg Hubbard ladder in snake order (site 2x: leg 0, site 2x+1: leg 1
+    of column x), spin SU(2) x charge U(1), reduced form (HubbardSU2xU1 with
+    tFull, models/HubbardSU2xU1.h:224-233).  Bonds: rungs (2x, 2x+1), i.e.
+    distance 1 from even sites (`rung` = this site is even), legs (i, i+2).
+    Auxiliary order: 0 done | 1 c pending, distance 1 (2,+1) | 2 c^dagger
+    pending, distance 1 (2,-1) | 3, 4 the same at distance 2 | 5 start;
+    D_W = 6.  A distance-2 channel crosses the middle site with the string F."""
+    qloc, Id, F, c, cdag, d = hubbard_su2xu1_ops()
+    q = [(1, 0), (2, 1), (2, -1), (2, 1), (2, -1), (1, 0)]
+    s2 = np.sqrt(2.0)
+    ent = [
+        (0, 0, Id, (1, 0)),
+        (1, 0, c, (2, -1)),
+        (2, 0, cdag, (2, 1)),
+        (3, 1, F, (1, 0)),
+        (4, 2, F, (1, 0)),
+        (5, 0, U * d, (1, 0)),
+        (5, 3, -s2 * t * (cdag @ F), (2, 1)),
+        (5, 4, -s2 * t * (c @ F), (2, -1)),
+        (5, 5, Id, (1, 0)),
+    ]
+    if rung:
+        ent += [(5, 1, -s2 * t * (cdag @ F), (2, 1)), (5, 2, -s2 * t * (c @ F), (2, -1))]
+    return _place(SU2xU1, qloc, q, ent, position)
+
+
 def identity_mpo(sym, qloc):
     """W = identity with one vacuum auxiliary state (the overlap operator)."""
     vac = tuple(1 if k == SU2_KIND else 0 for k in sym.kinds)
@@ -381,6 +431,14 @@ def _place(sym, qloc, q, ent, position, conj=None):
 
 def chain_mpo(model, L, **params):
     """Full open-boundary MPO as a list of MpoSite."""
+    if model in ("HubbardSU2xU1Ladder", "HeisenbergSU2LR"):
+        def f(i, position):
+            if model == "HubbardSU2xU1Ladder":
+                return hubbard_su2xu1_ladder_mpo(rung=(i % 2 == 0), position=position, **params)
+            return heisenberg_su2_lr_mpo(position=position, **params)
+        if L == 1:
+            return [f(0, "single")]
+        return [f(0, "first")] + [f(i, "bulk") for i in range(1, L - 1)] + [f(L - 1, "last")]
     f = {"HeisenbergU1": heisenberg_u1_mpo, "HubbardU1xU1": hubbard_u1xu1_mpo,
          "HeisenbergSU2": heisenberg_su2_mpo, "HubbardSU2xU1": hubbard_su2xu1_mpo}[model]
     if L == 1:

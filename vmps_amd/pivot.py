@@ -76,8 +76,12 @@ class PivotMatrix2:
         _lib.check(self._lib.vmps_plan_get_tiles(self._plan, ctypes.byref(cfg)))
         return cfg.as_dict()
 
-    def set_tiles(self, wave_edge, big_min):
-        _lib.check(self._lib.vmps_plan_set_tiles(self._plan, int(wave_edge), int(big_min)))
+    def set_tiles(self, wave_edge=None, big_min=None, explicit_u=None):
+        cur = self.tiles()
+        _lib.check(self._lib.vmps_plan_set_tiles(
+            self._plan, int(cur["wave_edge"] if wave_edge is None else wave_edge),
+            int(cur["big_min"] if big_min is None else big_min),
+            int(cur["explicit_u"] if explicit_u is None else explicit_u)))
 
     def autotune(self, iters=10):
         """Time the candidate tile configurations of this plan's precision and

@@ -294,6 +294,24 @@ def heisenberg_su2_bond(nsites_left, M, sigma=None):
     return [(q, int(d)) for q, d, k in zip(qs, dims, keep) if k and (M * w[qs.index(q)] / w.sum()) >= 0.5]
 
 
+def hubbard_su2xu1_bond(nsites_left, M, filling=1.0, sigma_n=None, sigma_s=None):
+    """(q = 2S+1, N) multiplet sectors of the bond after `nsites_left` Hubbard
+    sites (spin SU(2) x charge U(1)): N around filling * nsites_left, S small,
+    2S = N mod 2; multiplet counts Gaussian in (N, S), summing to M."""
+    n0 = filling * nsites_left
+    sigma_n = sigma_n or max(1.0, 0.45 * math.sqrt(nsites_left))
+    sigma_s = sigma_s or max(1.0, 0.35 * math.sqrt(nsites_left))
+    qs, w = [], []
+    for N in range(max(0, int(n0 - 5 * sigma_n)), min(2 * nsites_left, int(n0 + 5 * sigma_n)) + 1):
+        smax = min(N, 2 * nsites_left - N)  # 2 S_max
+        for two_s in range(N % 2, smax + 1, 2):
+            qs.append((two_s + 1, N))
+            w.append(math.exp(-((N - n0) ** 2) / (2 * sigma_n ** 2) - (two_s / 2.0) ** 2 / (2 * sigma_s ** 2)))
+    w = np.array(w)
+    dims = np.floor(M * w / w.sum() + 0.5).astype(int)
+    return [(q, int(d)) for q, d in zip(qs, dims) if d > 0]
+
+
 def hubbard_bond(nsites_left, M, filling=1.0, alpha=0.36, lam=0.02):
     """(N, 2Sz) sectors of the bond after `nsites_left` Hubbard sites.  With the
     defaults and M = 512: ~110 sectors of dims 1..~55."""
@@ -314,7 +332,21 @@ WORKLOADS = {
     # large-block pivots (MFMA-bound regime)
     "heisenberg_u1_L64_M2048": dict(model="HeisenbergU1", L=64, M=2048),
     "hubbard_u1xu1_L48_M2048": dict(model="HubbardU1xU1", L=48, M=2048, alpha=1.0, lam=2.2),
+    # two-leg Hubbard ladder 2 x 32 in snake order, spin SU(2) x charge U(1),
+    # t=1, U=6, filling n = 0.875, M = 2048 multiplets, D_W = 6 (rung + leg hopping)
+    "hubbard_su2xu1_ladder_L64_M2048": dict(model="HubbardSU2xU1Ladder", L=64, M=2048, filling=0.875),
+    # long-range SU(2) Heisenberg, 60 sites, couplings up to distance 38
+    # (D_W = 40, the bandwidth of the snake-ordered C60 graph's Jfull MPO),
+    # M = 4096 multiplets; smaller variants for parity tests
+    "heisenberg_su2_lr40_L60_M4096": dict(model="HeisenbergSU2LR", L=60, M=4096, R=38),
+    "heisenberg_su2_lr40_L60_M1024": dict(model="HeisenbergSU2LR", L=60, M=1024, R=38),
 }
+
+
+def lr_couplings(R, rng):
+    """Distance couplings J_1 .. J_R of the long-range workloads: J_1 = 1, the
+    rest random in [-0.5, 0.5] with a 1/d envelope."""
+    return [1.0] + [float(rng.uniform(-0.5, 0.5)) / d for d in range(2, R + 1)]
 
 
 BATCHES = {
@@ -361,6 +393,18 @@ def make_workload(name, seed=20261202, hermitian=False, **override):
         kw = {k: cfg[k] for k in ("alpha", "lam") if k in cfg}
         left = hubbard_bond(loc, cfg["M"], **kw)
         right = hubbard_bond(loc + 2, cfg["M"], **kw)
+    elif cfg["model"] == "HubbardSU2xU1Ladder":
+        # sites loc (odd: no rung to its right) and loc + 1 (even: rung)
+        t, U = cfg.get("t", 1.0), cfg.get("U", 6.0)
+        W12 = models.hubbard_su2xu1_ladder_mpo(t=t, U=U, rung=(loc % 2 == 0))
+        W34 = models.hubbard_su2xu1_ladder_mpo(t=t, U=U, rung=((loc + 1) % 2 == 0))
+        left = hubbard_su2xu1_bond(loc, cfg["M"], filling=cfg["filling"])
+        right = hubbard_su2xu1_bond(loc + 2, cfg["M"], filling=cfg["filling"])
+        return make_pivot(W12, W34, left, right, rng, hermitian=hermitian)
+    elif cfg["model"] == "HeisenbergSU2LR":
+        W = models.heisenberg_su2_lr_mpo(lr_couplings(cfg["R"], rng))
+        left = heisenberg_su2_bond(loc, cfg["M"])
+        right = heisenberg_su2_bond(loc + 2, cfg["M"])
     else:
         raise ValueError(cfg["model"])
     return make_pivot(W, W, left, right, rng, hermitian=hermitian)
