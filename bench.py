@@ -117,6 +117,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the default tile configuration")
+    ap.add_argument("--tiles", default=None,
+                    help="fixed tile configuration wave_edge,big_min,big_min64,explicit_u (instead of autotuning)")
     ap.add_argument("--no-mixed", action="store_true", help="skip the FP32 / split-BF16 x3 HxV lines")
     ap.add_argument("--seed", type=int, default=20261202)
     ap.add_argument("--sweeps", type=int, default=4, help="DMRG sweeps of the HeisenbergU1 L=32 M=64 chain (0: none)")
@@ -153,6 +155,10 @@ def main():
         pds = [synth.make_workload(args.workload, seed=args.seed)]
         H = PivotMatrix2(pds[0], precision=args.precision)
     st = H.stats()
+    if args.tiles:
+        we, bm, b64, xu = (int(x) for x in args.tiles.split(","))
+        H.set_tiles(wave_edge=we, big_min=bm, big_min64=b64, explicit_u=xu)
+        args.no_tune = True
     tiles = H.autotune(iters=10) if not args.no_tune else H.tiles()
     vins = [synth.random_vector(pd, args.seed + 1 + i) for i, pd in enumerate(pds)]
     vin_h = np.concatenate(vins)
@@ -190,11 +196,12 @@ def main():
             mixed[prec] = {"gflops": st["flops_alg"] / sm / 1e9, "ms_per_hxv": sm * 1e3,
                            "speedup_vs_fp64": sec / sm,
                            "rel_err_vs_fp64": float(np.linalg.norm(o - out_gpu) / np.linalg.norm(out_gpu)),
-                           "tiles": {k: tm[k] for k in ("wave_edge", "big_min", "explicit_u")}}
+                           "tiles": {k: tm[k] for k in ("wave_edge", "big_min", "big_min64", "explicit_u")}}
             Hm.close()
 
     # per-kernel device times (HIP events on the plan's stream)
-    ms = H.time_stages(warmup=5, iters=200)
+    stage_iters = int(max(3, min(200, 0.5 / max(sec, 1e-9))))  # about half a second per stage
+    ms = H.time_stages(warmup=2, iters=stage_iters)
     stage_flops = {"A": st["flops_A"], "C": st["flops_C"]}
     dom = "A" if ms[0] >= ms[2] else "C"
     t_dom = ms[0] if dom == "A" else ms[2]
@@ -240,7 +247,7 @@ def main():
             "parallelism": "single GPU",
         },
         "stage_ms": {"A_LV": ms[0], "C_combine_UR": ms[2], "hxv": ms[3]},
-        "tiles": {"wave_edge": tiles["wave_edge"], "big_min": tiles["big_min"], "explicit_u": tiles["explicit_u"],
+        "tiles": {k: tiles[k] for k in ("wave_edge", "big_min", "big_min64", "explicit_u")} | {
                   "autotuned": not args.no_tune},
         "mixed_hxv": mixed,
         "roofline": {

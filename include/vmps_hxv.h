@@ -209,6 +209,8 @@ vmps_status vmps_plan_time_hxv(vmps_plan* plan, int warmup, int iters, double* m
  *   big_min:   split-BF16 x3 only: problems with both dimensions >= big_min
  *              use the LDS-staged 128 x 128 workgroup tiles, the rest
  *              32 x 32 wave tiles (0 = none use the big tiles);
+ *   big_min64: FP64 / FP32 only: problems with both dimensions >= big_min64
+ *              use LDS-staged 128 x 128 workgroup tiles (0 = none, the default);
  *   explicit_u: 0 = the combine U = sum c T is done on stage C's operand
  *              load (U never stored); 1 = a separate kernel forms U once per
  *              (output block, R block) and stage C reads it (pays off when
@@ -216,6 +218,7 @@ vmps_status vmps_plan_time_hxv(vmps_plan* plan, int warmup, int iters, double* m
 typedef struct {
   int wave_edge;
   int big_min;
+  int big_min64;
   int explicit_u;
   double ms;       /* HxV time of the chosen configuration (autotune; 0 if taken from the cache) */
   int from_cache;  /* 1: a plan with the same block-size histogram was tuned before in this process */
@@ -225,7 +228,7 @@ typedef struct {
 vmps_status vmps_plan_get_tiles(const vmps_plan* plan, vmps_tile_config* cfg);
 /* Results do not depend on the configuration beyond FP rounding of the
  * split-BF16 path (FP64: bit-identical sums in the same order per tile). */
-vmps_status vmps_plan_set_tiles(vmps_plan* plan, int wave_edge, int big_min, int explicit_u);
+vmps_status vmps_plan_set_tiles(vmps_plan* plan, const vmps_tile_config* cfg);  /* ms .. n_tried ignored */
 /* Autotuner: times every candidate configuration of the plan's precision
  * (`iters` HxV each on the plan's resident vectors) and keeps the fastest.
  * The choice is cached per block-size histogram (the share of the work in

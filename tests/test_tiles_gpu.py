@@ -113,3 +113,18 @@ def test_set_tiles_rejects_bad_edge(case):
     H = PivotMatrix2(pd)
     with pytest.raises(Exception):
         H.set_tiles(24, 96)
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", FP64_RTOL), ("fp32", 1e-5)])
+@pytest.mark.parametrize("big_min64,xu", [(96, 0), (64, 1), (128, 1)])
+def test_fp64_fp32_big_tiles(case, precision, tol, big_min64, xu):
+    """LDS-staged 128 x 128 FP64 / FP32 tiles for the problems >= big_min64
+    (with the fused and with the explicit stage B) against the oracle; FP64 is
+    bit-identical to the wave tiles only up to summation order."""
+    pd, v, ref = case
+    print("big tiles", precision, big_min64, xu, flush=True)
+    H = PivotMatrix2(pd, precision=precision)
+    H.set_tiles(big_min64=big_min64, explicit_u=xu)
+    cfg = H.tiles()
+    assert cfg["big_min64"] == big_min64 and cfg["explicit_u"] == xu
+    assert _rel(HxV(H, v), ref) <= tol

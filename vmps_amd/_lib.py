@@ -58,8 +58,8 @@ class LanczosResult(ctypes.Structure):
 
 
 class TileConfig(ctypes.Structure):
-    _fields_ = [("wave_edge", c_int), ("big_min", c_int), ("explicit_u", c_int), ("ms", c_dbl), ("from_cache", c_int),
-                ("n_tried", c_int)]
+    _fields_ = [("wave_edge", c_int), ("big_min", c_int), ("big_min64", c_int), ("explicit_u", c_int), ("ms", c_dbl),
+                ("from_cache", c_int), ("n_tried", c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -107,7 +107,7 @@ SIGNATURES = {
     "vmps_plan_time_hxv": (c_int, [c_void, c_int, c_int, P_dbl]),
     "vmps_plan_time_stages": (c_int, [c_void, c_int, c_int, P_dbl]),
     "vmps_plan_get_tiles": (c_int, [c_void, ctypes.POINTER(TileConfig)]),
-    "vmps_plan_set_tiles": (c_int, [c_void, c_int, c_int, c_int]),
+    "vmps_plan_set_tiles": (c_int, [c_void, ctypes.POINTER(TileConfig)]),
     "vmps_plan_autotune": (c_int, [c_void, c_int, ctypes.POINTER(TileConfig)]),
     "vmps_env_create": (c_int, [c_int, c_int, P_int, c_int, P_int, c_int, P_int, ctypes.POINTER(c_void)]),
     "vmps_env_destroy": (None, [c_void]),

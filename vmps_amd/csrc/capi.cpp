@@ -543,23 +543,23 @@ vmps_status vmps_plan_get_tiles(const vmps_plan* plan, vmps_tile_config* cfg) {
   VMPS_API_BEGIN
   require(plan && cfg, VMPS_ERR_ARG, "NULL argument");
   const auto& H = plan->D.H;
-  *cfg = vmps_tile_config{H.wave_edge, H.big_min, H.explicit_u ? 1 : 0, 0.0, 0, 0};
+  *cfg = vmps_tile_config{H.wave_edge, H.big_min, H.big_min64, H.explicit_u ? 1 : 0, 0.0, 0, 0};
   VMPS_API_END
 }
 
-vmps_status vmps_plan_set_tiles(vmps_plan* plan, int wave_edge, int big_min, int explicit_u) {
+vmps_status vmps_plan_set_tiles(vmps_plan* plan, const vmps_tile_config* cfg) {
   VMPS_API_BEGIN
-  require(plan != nullptr, VMPS_ERR_ARG, "NULL plan");
-  require(wave_edge == 16 || wave_edge == 32, VMPS_ERR_ARG, "wave_edge must be 16 or 32");
-  require(big_min >= 0, VMPS_ERR_ARG, "big_min must be >= 0");
-  require(explicit_u == 0 || explicit_u == 1, VMPS_ERR_ARG, "explicit_u must be 0 or 1");
-  plan->D.retile(wave_edge, big_min, explicit_u != 0);
+  require(plan && cfg, VMPS_ERR_ARG, "NULL argument");
+  require(cfg->wave_edge == 16 || cfg->wave_edge == 32, VMPS_ERR_ARG, "wave_edge must be 16 or 32");
+  require(cfg->big_min >= 0 && cfg->big_min64 >= 0, VMPS_ERR_ARG, "big-tile thresholds must be >= 0");
+  require(cfg->explicit_u == 0 || cfg->explicit_u == 1, VMPS_ERR_ARG, "explicit_u must be 0 or 1");
+  plan->D.retile(cfg->wave_edge, cfg->big_min, cfg->big_min64, cfg->explicit_u != 0);
   VMPS_API_END
 }
 
 namespace {
 struct TileChoice {
-  int wave_edge, big_min;
+  int wave_edge, big_min, big_min64;
   bool explicit_u;
 };
 std::mutex g_tune_mutex;
@@ -572,13 +572,13 @@ vmps_status vmps_plan_autotune(vmps_plan* plan, int iters, vmps_tile_config* out
   auto& D = plan->D;
   const std::string key = std::to_string((int)D.prec) + "/" + vmps::tile_signature(D.H);
   auto result = [&](double ms, int from_cache, int tried) {
-    return vmps_tile_config{D.H.wave_edge, D.H.big_min, D.H.explicit_u ? 1 : 0, ms, from_cache, tried};
+    return vmps_tile_config{D.H.wave_edge, D.H.big_min, D.H.big_min64, D.H.explicit_u ? 1 : 0, ms, from_cache, tried};
   };
   {
     std::lock_guard<std::mutex> lock(g_tune_mutex);
     auto it = g_tune_cache.find(key);
     if (it != g_tune_cache.end()) {
-      D.retile(it->second.wave_edge, it->second.big_min, it->second.explicit_u);
+      D.retile(it->second.wave_edge, it->second.big_min, it->second.big_min64, it->second.explicit_u);
       if (out) *out = result(0.0, 1, 0);
       return VMPS_OK;
     }
@@ -587,16 +587,17 @@ vmps_status vmps_plan_autotune(vmps_plan* plan, int iters, vmps_tile_config* out
   std::vector<TileChoice> cand;
   for (bool xu : {false, true}) {
     if (D.prec == VMPS_PREC_BF16X3) {
-      for (int b : {vmps::kBigMin, 2 * vmps::kBigMin, 0}) cand.push_back({D.H.wave_edge, b, xu});
+      for (int b : {vmps::kBigMin, 2 * vmps::kBigMin, 0}) cand.push_back({D.H.wave_edge, b, D.H.big_min64, xu});
     } else {
-      for (int e : {16, 32}) cand.push_back({e, D.H.big_min, xu});
+      for (int e : {16, 32})
+        for (int b64 : {0, vmps::kBigTile}) cand.push_back({e, D.H.big_min, b64, xu});
     }
   }
   double best = -1.0;
-  TileChoice pick{D.H.wave_edge, D.H.big_min, D.H.explicit_u};
+  TileChoice pick{D.H.wave_edge, D.H.big_min, D.H.big_min64, D.H.explicit_u};
   int tried = 0;
   for (const auto& c : cand) {
-    D.retile(c.wave_edge, c.big_min, c.explicit_u);
+    D.retile(c.wave_edge, c.big_min, c.big_min64, c.explicit_u);
     for (int i = 0; i < 2; ++i) D.hxv(D.vin.p, D.vout.p, D.stream);
     VMPS_HIP_CHECK(hipEventRecord(D.ev0, D.stream));
     for (int i = 0; i < iters; ++i) D.hxv(D.vin.p, D.vout.p, D.stream);
@@ -611,7 +612,7 @@ vmps_status vmps_plan_autotune(vmps_plan* plan, int iters, vmps_tile_config* out
       pick = c;
     }
   }
-  D.retile(pick.wave_edge, pick.big_min, pick.explicit_u);
+  D.retile(pick.wave_edge, pick.big_min, pick.big_min64, pick.explicit_u);
   {
     std::lock_guard<std::mutex> lock(g_tune_mutex);
     g_tune_cache[key] = pick;

@@ -267,6 +267,186 @@ __global__ __launch_bounds__(256) void k_stage_c(const TileC* __restrict__ tiles
   }
 }
 
+// ---- LDS-staged 128 x 128 workgroup tiles for large blocks ------------------
+// Problems with both dimensions >= H.big_min64 (a tile configuration choice,
+// off by default).  4 waves in a 2 x 2 grid, each owning 64 x 64 = 4 x 4 MFMA
+// accumulators.  Per 16-deep K step the workgroup
+// stages A[128 x 16] and B[16 x 128] k-major in LDS (the layout the MFMA
+// lanes read: 16 consecutive rows / columns per k) with coalesced global
+// loads, so every operand element is fetched from HBM once per tile instead
+// of once per 16- or 32-wide wave tile.  The next K step's global loads are
+// issued before the MFMAs of the current one.
+constexpr int kBT = kBigTile;
+constexpr int kBKd = 16;
+
+template <class T>
+struct BigTileSmem {
+  T A[kBKd][kBT];
+  T B[kBKd][kBT];
+};
+
+template <class T>
+__device__ inline void big_tile_mac(const BigTileSmem<T>& s, int wm, int wn, int li, int lk,
+                                    typename Mfma<T>::acc_t (&acc)[4][4]) {
+#pragma unroll
+  for (int ks = 0; ks < kBKd; ks += 4) {
+    T a[4], b[4];
+#pragma unroll
+    for (int x = 0; x < 4; ++x) {
+      a[x] = s.A[ks + lk][wm + 16 * x + li];
+      b[x] = s.B[ks + lk][wn + 16 * x + li];
+    }
+#pragma unroll
+    for (int y = 0; y < 4; ++y)
+#pragma unroll
+      for (int x = 0; x < 4; ++x) Mfma<T>::mac(b[y], a[x], acc[x][y]);
+  }
+}
+
+template <class T, class OutT>
+__device__ inline void big_tile_store(OutT* __restrict__ C, int M, int N, int m0, int n0, int lane,
+                                      const typename Mfma<T>::acc_t (&acc)[4][4]) {
+#pragma unroll
+  for (int x = 0; x < 4; ++x) {
+    const int i = m0 + 16 * x + (lane & 15);
+    if (i >= M) continue;
+#pragma unroll
+    for (int y = 0; y < 4; ++y)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int j = n0 + 16 * y + Mfma<T>::col(lane, v);
+        if (j < N) C[(int64_t)j * M + i] = (OutT)acc[x][y][v];
+      }
+  }
+}
+
+// Thread roles in the staging loads (256 threads, 2048 elements per operand):
+//   A (column-major, ld M): row i = tid % 128, k = kb + tid/128 + 2q, q < 8
+//   B (column-major K x N, ld K): column j = tid / 2, k = kb + 8 (tid % 2) + q
+template <class T>
+__global__ __launch_bounds__(256) void k_stage_a_big(const TileA* __restrict__ tiles, int ntiles,
+                                                     const T* __restrict__ L, const T* __restrict__ V,
+                                                     T* __restrict__ Tout) {
+  __shared__ BigTileSmem<T> s;
+  if ((int)blockIdx.x >= ntiles) return;  // workgroup-uniform
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int li = lane & 15, lk = lane >> 4;
+  const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
+  const TileA t = tiles[blockIdx.x];
+  const int m0 = t.mt * kBT, n0 = t.nt * kBT;
+  const T* __restrict__ A = L + t.a_off;
+  const T* __restrict__ B = V + t.b_off;
+  const int ai = tid & 127, ak = tid >> 7, bj = tid >> 1, bk = (tid & 1) * 8;
+  const bool arow = m0 + ai < t.M, bcol = n0 + bj < t.N;
+  typename Mfma<T>::acc_t acc[4][4];
+#pragma unroll
+  for (int x = 0; x < 4; ++x)
+#pragma unroll
+    for (int y = 0; y < 4; ++y) acc[x][y] = typename Mfma<T>::acc_t{0, 0, 0, 0};
+  T ra[8], rb[8];
+  auto gload = [&](int kb) {
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int ka = kb + ak + 2 * q, kbb = kb + bk + q;
+      ra[q] = (arow && ka < t.K) ? A[(int64_t)ka * t.M + m0 + ai] : T(0);
+      rb[q] = (bcol && kbb < t.K) ? B[(int64_t)(n0 + bj) * t.K + kbb] : T(0);
+    }
+  };
+  gload(0);
+  for (int kb = 0; kb < t.K; kb += kBKd) {
+    __syncthreads();  // the previous step's LDS reads are done
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      s.A[ak + 2 * q][ai] = ra[q];
+      s.B[bk + q][bj] = rb[q];
+    }
+    __syncthreads();
+    if (kb + kBKd < t.K) gload(kb + kBKd);
+    big_tile_mac<T>(s, wm, wn, li, lk, acc);
+  }
+  big_tile_store<T, T>(Tout + t.c_off, t.M, t.N, m0 + wm, n0 + wn, lane, acc);
+}
+
+// Stage C, large output blocks: Vout = sum over chunks (sum_j c_j T_j) . R_r.
+// The combine U = sum c_j T_j is formed while staging the A operand: the
+// chunk's term list is loaded once per lane and broadcast with readlane.
+template <class T, class OutT>
+__global__ __launch_bounds__(256) void k_stage_c_big(const TileC* __restrict__ tiles, int ntiles,
+                                                     const ChunkC* __restrict__ chunks,
+                                                     const TermC* __restrict__ terms, const T* __restrict__ Tbuf,
+                                                     const T* __restrict__ R, OutT* __restrict__ Vout) {
+  __shared__ BigTileSmem<T> s;
+  if ((int)blockIdx.x >= ntiles) return;  // workgroup-uniform
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int li = lane & 15, lk = lane >> 4;
+  const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
+  const TileC t = tiles[blockIdx.x];
+  const int m0 = t.mt * kBT, n0 = t.nt * kBT;
+  const int ai = tid & 127, ak = tid >> 7, bj = tid >> 1, bk = (tid & 1) * 8;
+  const bool arow = m0 + ai < t.M, bcol = n0 + bj < t.N;
+  typename Mfma<T>::acc_t acc[4][4];
+#pragma unroll
+  for (int x = 0; x < 4; ++x)
+#pragma unroll
+    for (int y = 0; y < 4; ++y) acc[x][y] = typename Mfma<T>::acc_t{0, 0, 0, 0};
+  for (int c = t.chunk_begin; c < t.chunk_end; ++c) {
+    const ChunkC ch = chunks[c];
+    const int K = ch.K;
+    const T* __restrict__ B = R + ch.r_off;
+    const int nt = ch.term_end - ch.term_begin;
+    int64_t my_toff = 0;
+    T my_c = T(0);
+    if (lane < nt) {
+      const TermC tm = terms[ch.term_begin + lane];
+      my_toff = tm.t_off;
+      my_c = (T)tm.c;
+    }
+    for (int kb = 0; kb < K; kb += kBKd) {
+      T ra[8], rb[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const int kbb = kb + bk + q;
+        rb[q] = (bcol && kbb < K) ? B[(int64_t)(n0 + bj) * K + kbb] : T(0);
+        ra[q] = T(0);
+      }
+      int jt = 0;
+      for (; jt + 1 < nt; jt += 2) {
+        const T* __restrict__ T0 = Tbuf + readlane64(my_toff, jt);
+        const T* __restrict__ T1 = Tbuf + readlane64(my_toff, jt + 1);
+        const T c0 = readlane_t<T>(my_c, jt), c1 = readlane_t<T>(my_c, jt + 1);
+        T x0[8], x1[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          const int ka = kb + ak + 2 * q;
+          const bool ok = arow && ka < K;
+          x0[q] = ok ? T0[(int64_t)ka * t.M + m0 + ai] : T(0);
+          x1[q] = ok ? T1[(int64_t)ka * t.M + m0 + ai] : T(0);
+        }
+#pragma unroll
+        for (int q = 0; q < 8; ++q) ra[q] += c0 * x0[q] + c1 * x1[q];
+      }
+      if (jt < nt) {
+        const T* __restrict__ T0 = Tbuf + readlane64(my_toff, jt);
+        const T c0 = readlane_t<T>(my_c, jt);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          const int ka = kb + ak + 2 * q;
+          ra[q] += c0 * ((arow && ka < K) ? T0[(int64_t)ka * t.M + m0 + ai] : T(0));
+        }
+      }
+      __syncthreads();  // the previous step's LDS reads are done
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        s.A[ak + 2 * q][ai] = ra[q];
+        s.B[bk + q][bj] = rb[q];
+      }
+      __syncthreads();
+      big_tile_mac<T>(s, wm, wn, li, lk, acc);
+    }
+  }
+  big_tile_store<T, OutT>(Vout + t.c_off, t.M, t.N, m0 + wm, n0 + wn, lane, acc);
+}
+
 // Weighted dot product, pass 1: workgroup b reduces segments [b*spb, (b+1)*spb).
 __global__ __launch_bounds__(256) void k_dot_partial(const int64_t* __restrict__ seg_off, const int32_t* __restrict__ seg_len,
                                                      const double* __restrict__ seg_w, int nseg, int spb,
@@ -390,6 +570,8 @@ void DevPlan::upload(vmps_precision precision) {
 void DevPlan::upload_lists() {
   tilesA.upload(H.tilesA_w, stream);  // FP64 / FP32 wave kernels (edge H.wave_edge)
   tilesC.upload(H.tilesC_w, stream);
+  tilesAb.upload(H.tilesA_b, stream);  // FP64 / FP32 big-tile kernels
+  tilesCb.upload(H.tilesC_b, stream);
   chunksC.upload(H.chunksC, stream);
   termsC.upload(H.termsC, stream);
   combTasks.upload(H.comb_tasks, stream);
@@ -424,8 +606,10 @@ void DevPlan::ensure_precision_buffers() {
 }
 
 void DevPlan::launch_stage(int stage, const double* d_vin, double* d_vout, hipStream_t s) {
-  const int nA = (int)tilesA.n, nC = (int)tilesC.n;
+  const int nA = (int)tilesA.n, nC = (int)tilesC.n, nAb = (int)tilesAb.n, nCb = (int)tilesCb.n;
   if (prec == VMPS_PREC_FP64) {
+    if (stage == 0 && nAb)
+      hipLaunchKernelGGL(k_stage_a_big<double>, dim3(nAb), dim3(256), 0, s, tilesAb.p, nAb, L.p, d_vin, T.p);
     if (stage == 0 && nA) {
       if (H.wave_edge == 16)
         hipLaunchKernelGGL((k_stage_a<double, 16>), dim3(grid_for(nA)), dim3(256), 0, s, tilesA.p, nA, L.p, d_vin,
@@ -434,7 +618,7 @@ void DevPlan::launch_stage(int stage, const double* d_vin, double* d_vout, hipSt
         hipLaunchKernelGGL((k_stage_a<double, 32>), dim3(grid_for(nA)), dim3(256), 0, s, tilesA.p, nA, L.p, d_vin,
                            T.p);
     }
-    if (stage == 2 && nC) {
+    if (stage == 2 && (nC || nCb)) {
       const double* src = T.p;
       if (H.explicit_u) {  // stage B: U = sum c T, then stage C reads U
         if (combTasks.n)
@@ -442,16 +626,21 @@ void DevPlan::launch_stage(int stage, const double* d_vin, double* d_vout, hipSt
                              combTerms.p, T.p, Ub.p);
         src = Ub.p;
       }
-      if (H.wave_edge == 16)
+      if (nCb)
+        hipLaunchKernelGGL((k_stage_c_big<double, double>), dim3(nCb), dim3(256), 0, s, tilesCb.p, nCb, chunksC.p,
+                           termsC.p, src, R.p, d_vout);
+      if (nC && H.wave_edge == 16)
         hipLaunchKernelGGL((k_stage_c<double, double, 16>), dim3(nC), dim3(256), 0, s, tilesC.p, nC, chunksC.p,
                            termsC.p, src, R.p, d_vout);
-      else
+      else if (nC)
         hipLaunchKernelGGL((k_stage_c<double, double, 32>), dim3(nC), dim3(256), 0, s, tilesC.p, nC, chunksC.p,
                            termsC.p, src, R.p, d_vout);
     }
   } else if (prec == VMPS_PREC_FP32) {
     if (stage == 0) {
       if (H.ysize) hipLaunchKernelGGL(k_f64_to_f32, dim3(1024), dim3(256), 0, s, d_vin, V32.p, (int64_t)H.ysize);
+      if (nAb)
+        hipLaunchKernelGGL(k_stage_a_big<float>, dim3(nAb), dim3(256), 0, s, tilesAb.p, nAb, L32.p, V32.p, T32.p);
       if (nA) {
         if (H.wave_edge == 16)
           hipLaunchKernelGGL((k_stage_a<float, 16>), dim3(grid_for(nA)), dim3(256), 0, s, tilesA.p, nA, L32.p, V32.p,
@@ -461,7 +650,7 @@ void DevPlan::launch_stage(int stage, const double* d_vin, double* d_vout, hipSt
                              T32.p);
       }
     }
-    if (stage == 2 && nC) {
+    if (stage == 2 && (nC || nCb)) {
       const float* src = T32.p;
       if (H.explicit_u) {
         if (combTasks.n)
@@ -469,10 +658,13 @@ void DevPlan::launch_stage(int stage, const double* d_vin, double* d_vout, hipSt
                              combTerms.p, T32.p, U32.p);
         src = U32.p;
       }
-      if (H.wave_edge == 16)
+      if (nCb)
+        hipLaunchKernelGGL((k_stage_c_big<float, double>), dim3(nCb), dim3(256), 0, s, tilesCb.p, nCb, chunksC.p,
+                           termsC.p, src, R32.p, d_vout);
+      if (nC && H.wave_edge == 16)
         hipLaunchKernelGGL((k_stage_c<float, double, 16>), dim3(nC), dim3(256), 0, s, tilesC.p, nC, chunksC.p,
                            termsC.p, src, R32.p, d_vout);
-      else
+      else if (nC)
         hipLaunchKernelGGL((k_stage_c<float, double, 32>), dim3(nC), dim3(256), 0, s, tilesC.p, nC, chunksC.p,
                            termsC.p, src, R32.p, d_vout);
     }

@@ -113,6 +113,8 @@ struct DevPlan {
   // work lists
   DevBuf<TileA> tilesA;
   DevBuf<TileC> tilesC;
+  DevBuf<TileA> tilesAb;  // LDS-staged 128 x 128 tiles (H.big_min64)
+  DevBuf<TileC> tilesCb;
   DevBuf<ChunkC> chunksC;
   DevBuf<TermC> termsC;
   DevBuf<int64_t> seg_off;
@@ -136,10 +138,11 @@ struct DevPlan {
   void ensure_bf16_buffers();
   void upload_bf16_lists(const Bf16Layout& B);
   bool bf16_ready = false;
-  // switch the tile configuration: FP64 / FP32 wave tile edge (16 or 32), the
-  // split-BF16 big-tile threshold (H.big_min) and fused / explicit stage B
+  // switch the tile configuration: FP64 / FP32 wave tile edge (16 or 32) and
+  // big-tile threshold (H.big_min64), the split-BF16 big-tile threshold
+  // (H.big_min) and fused / explicit stage B
   // (H.explicit_u); rebuilds and uploads the work lists only
-  void retile(int wave_edge, int big_min, bool explicit_u);
+  void retile(int wave_edge, int big_min, int big_min64, bool explicit_u);
   void upload_lists();  // FP64 / FP32 work lists and the explicit stage B buffers
   void launch_stage_bf16x3(int stage, const double* d_vin, double* d_vout, hipStream_t s);
   void hxv(const double* d_vin, double* d_vout, hipStream_t s);

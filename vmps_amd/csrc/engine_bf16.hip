@@ -612,14 +612,14 @@ void DevPlan::upload_bf16_lists(const Bf16Layout& B) {
     Up.release();
 }
 
-void DevPlan::retile(int wave_edge, int big_min, bool explicit_u) {
+void DevPlan::retile(int wave_edge, int big_min, int big_min64, bool explicit_u) {
   ++gen;
   const bool bf16_lists = bf16_ready && (big_min != H.big_min || explicit_u != H.explicit_u);
   if (explicit_u != H.explicit_u) {
     H.explicit_u = explicit_u;
     build_tiles(H);  // chunk / term tables and every tile list
   }
-  set_wave_edge(H, wave_edge);
+  set_wave_edge(H, wave_edge, big_min64);
   H.big_min = big_min;
   upload_lists();
   if (bf16_lists) {  // split-BF16 work lists; the split operands stay

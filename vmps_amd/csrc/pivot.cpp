@@ -516,13 +516,22 @@ void validate_host_plan(const HostPlan& H) {
 // into several chunks on the same R block (U . R is linear in U).  Tiles are
 // ordered by decreasing work so the largest start first.
 namespace {
+// select: 0 = every problem, > 0 = problems with both dimensions >= select,
+// < 0 = problems with a dimension below -select
+bool selected(int select, int64_t M, int64_t N) {
+  if (select == 0) return true;
+  const bool big = M >= std::abs(select) && N >= std::abs(select);
+  return select > 0 ? big : !big;
+}
+
 void make_tiles(const HostPlan& H, int edge, const std::vector<std::pair<int32_t, int32_t>>& crange,
-                std::vector<TileA>& outA, std::vector<TileC>& outC) {
+                std::vector<TileA>& outA, std::vector<TileC>& outC, int select = 0) {
   outA.clear();
   outC.clear();
   std::vector<std::pair<int64_t, size_t>> order;
   for (size_t p = 0; p < H.probA.size(); ++p) {
     const GemmProb& pr = H.probA[p];
+    if (!selected(select, pr.M, pr.N)) continue;
     const GemmChunk& ch = H.chunkA[pr.chunk_begin];
     for (int i = 0; i * edge < pr.M; ++i)
       for (int j = 0; j * edge < pr.N; ++j) {
@@ -549,6 +558,7 @@ void make_tiles(const HostPlan& H, int edge, const std::vector<std::pair<int32_t
   std::vector<int64_t> cost;
   for (size_t p = 0; p < H.probC.size(); ++p) {
     const GemmProb& pr = H.probC[p];
+    if (!selected(select, pr.M, pr.N)) continue;
     int64_t w = 0;
     for (int32_t c = crange[p].first; c < crange[p].second; ++c)
       w += (int64_t)H.chunksC[c].K * (1 + H.chunksC[c].term_end - H.chunksC[c].term_begin);
@@ -587,16 +597,42 @@ int choose_wave_edge(const HostPlan& H) {
 }
 }  // namespace
 
-void set_wave_edge(HostPlan& H, int edge) {
+void set_wave_edge(HostPlan& H, int edge, int big_min64) {
   if (edge != 16 && edge != 32) throw VmpsError(VMPS_ERR_ARG, "wave tile edge must be 16 or 32");
+  if (big_min64 < 0) throw VmpsError(VMPS_ERR_ARG, "big_min64 must be >= 0");
   if (H.crange.size() != H.probC.size()) throw VmpsError(VMPS_ERR_INTERNAL, "wave tiles: chunk ranges missing");
   H.wave_edge = edge;
-  if (edge == kTile) {
-    H.tilesA_w = H.tilesA;
-    H.tilesC_w = H.tilesC;
+  H.big_min64 = big_min64;
+  if (big_min64 > 0) {
+    make_tiles(H, edge, H.crange, H.tilesA_w, H.tilesC_w, -big_min64);
+    make_tiles(H, kBigTile, H.crange, H.tilesA_b, H.tilesC_b, big_min64);
   } else {
-    make_tiles(H, edge, H.crange, H.tilesA_w, H.tilesC_w);
+    H.tilesA_b.clear();
+    H.tilesC_b.clear();
+    if (edge == kTile) {
+      H.tilesA_w = H.tilesA;
+      H.tilesC_w = H.tilesC;
+    } else {
+      make_tiles(H, edge, H.crange, H.tilesA_w, H.tilesC_w);
+    }
   }
+  // bounds of every FP64 / FP32 tile list
+  const int64_t src_size = H.explicit_u ? H.U_size : H.T_size;
+  for (const auto* list : {&H.tilesC_w, &H.tilesC_b})
+    for (const TileC& t : *list)
+      for (int32_t c = t.chunk_begin; c < t.chunk_end; ++c) {
+        if (c < 0 || c >= (int32_t)H.chunksC.size() || H.chunksC[c].r_off < 0 ||
+            H.chunksC[c].r_off + (int64_t)H.chunksC[c].K * t.N > (int64_t)H.Rdata.size())
+          throw VmpsError(VMPS_ERR_INTERNAL, "stage C: R block out of range");
+        for (int32_t j = H.chunksC[c].term_begin; j < H.chunksC[c].term_end; ++j)
+          if (H.termsC[j].t_off + (int64_t)t.M * H.chunksC[c].K > src_size)
+            throw VmpsError(VMPS_ERR_INTERNAL, "stage C: T block out of range");
+      }
+  for (const auto* list : {&H.tilesA_w, &H.tilesA_b})
+    for (const TileA& t : *list)
+      if (t.a_off < 0 || t.a_off + (int64_t)t.M * t.K > (int64_t)H.Ldata.size() || t.b_off < 0 ||
+          t.b_off + (int64_t)t.K * t.N > H.ysize || t.c_off < 0 || t.c_off + (int64_t)t.M * t.N > H.T_size)
+        throw VmpsError(VMPS_ERR_INTERNAL, "stage A: tile out of range");
 }
 
 std::string tile_signature(const HostPlan& H) {
@@ -667,7 +703,7 @@ void build_tiles(HostPlan& H) {
     crange[p].second = (int32_t)H.chunksC.size();
   }
   make_tiles(H, kTile, crange, H.tilesA, H.tilesC);
-  set_wave_edge(H, choose_wave_edge(H));
+  set_wave_edge(H, choose_wave_edge(H), H.big_min64);
   // bounds of stage C: every T (or U) read stays inside T (or U)
   const int64_t src_size = H.explicit_u ? H.U_size : H.T_size;
   for (const CombTask& t : H.comb_tasks) {

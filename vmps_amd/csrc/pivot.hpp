@@ -320,6 +320,12 @@ struct HostPlan {
   // split-BF16: problems with both dimensions >= big_min take the LDS-staged
   // kBigTile tiles (big_min <= 0: none do)
   int big_min = kBigMin;
+  // FP64 / FP32: problems with both dimensions >= big_min64 take the
+  // LDS-staged kBigTile tiles tilesA_b / tilesC_b and are absent from the wave
+  // lists (0: none do, the default)
+  int big_min64 = 0;
+  std::vector<TileA> tilesA_b;
+  std::vector<TileC> tilesC_b;
   // stage B: false = fused into stage C's operand load (U never stored);
   // true = U formed by its own kernel (comb_tasks / comb_terms) and stage C's
   // chunks read U with one unit-weight term each (termsC t_off = U offsets)
@@ -365,8 +371,9 @@ void build_host_plan(const Pivot& P, HostPlan& H);
 void triples_from_pivot(const Pivot& P, TripleSet& S);
 void build_plan(const TripleSet& S, HostPlan& H);
 void build_tiles(HostPlan& H);
-// rebuild the FP64 / FP32 wave tile lists tilesA_w / tilesC_w for edge 16 or 32
-void set_wave_edge(HostPlan& H, int edge);
+// rebuild the FP64 / FP32 tile lists tilesA_w / tilesC_w (wave tiles of edge
+// 16 or 32) and tilesA_b / tilesC_b (big tiles of the problems >= big_min64)
+void set_wave_edge(HostPlan& H, int edge, int big_min64);
 // key of the plan's block-size histogram: the share of stage A and stage C
 // flops in classes of (rows, columns), quantised, so that pivots of similar
 // shape share one tuned tile configuration

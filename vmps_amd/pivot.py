@@ -76,12 +76,15 @@ class PivotMatrix2:
         _lib.check(self._lib.vmps_plan_get_tiles(self._plan, ctypes.byref(cfg)))
         return cfg.as_dict()
 
-    def set_tiles(self, wave_edge=None, big_min=None, explicit_u=None):
-        cur = self.tiles()
-        _lib.check(self._lib.vmps_plan_set_tiles(
-            self._plan, int(cur["wave_edge"] if wave_edge is None else wave_edge),
-            int(cur["big_min"] if big_min is None else big_min),
-            int(cur["explicit_u"] if explicit_u is None else explicit_u)))
+    def set_tiles(self, wave_edge=None, big_min=None, explicit_u=None, big_min64=None):
+        """Set the tile configuration; parameters left None keep their value."""
+        cfg = _lib.TileConfig()
+        _lib.check(self._lib.vmps_plan_get_tiles(self._plan, ctypes.byref(cfg)))
+        for k, v in (("wave_edge", wave_edge), ("big_min", big_min), ("explicit_u", explicit_u),
+                     ("big_min64", big_min64)):
+            if v is not None:
+                setattr(cfg, k, int(v))
+        _lib.check(self._lib.vmps_plan_set_tiles(self._plan, ctypes.byref(cfg)))
 
     def autotune(self, iters=10):
         """Time the candidate tile configurations of this plan's precision and
