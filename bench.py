@@ -14,6 +14,11 @@ timed in the mixed modes, FP32 MFMA and split-BF16x3 with FP32 accumulation
 (`mixed_hxv`: rate, speedup over FP64 and the relative difference of the
 result from the FP64 one).
 
+The default run also times one large-block pivot (`--large`, default the
+long-range SU(2) Heisenberg pivot with D_W = 40 and M = 4096 multiplets, 4.5
+TFLOP per HxV) in FP64, FP32 and split-BF16x3 (`large_pivot`), where the
+contraction is MFMA- rather than latency-bound.
+
 Beside the HxV line the default run also times full two-site DMRG sweeps of
 the HeisenbergU1 L=32 M=64 chain, FP64 and mixed (split-BF16x3 Lanczos refined
 in FP64), and reports the energy difference (`sweeps` key).
@@ -107,6 +112,34 @@ def sweep_bench(args, model="HeisenbergU1", L=32, M=64, sweeps=4):
     return out
 
 
+def large_pivot_bench(args, name, iters=3):
+    """One large-block pivot (the MFMA-bound regime) in FP64, FP32 and
+    split-BF16x3, each autotuned: rate, share of the dtype's MFMA peak, and the
+    mixed results' relative difference from the FP64 one."""
+    from vmps_amd import synth
+    from vmps_amd.pivot import HxV, PivotMatrix2
+    pd = synth.make_workload(name, seed=args.seed)
+    v = synth.random_vector(pd, args.seed + 1)
+    out = {"workload": name}
+    ref = None
+    for prec in ("fp64", "fp32", "bf16x3"):
+        H = PivotMatrix2(pd, precision=prec)
+        st = H.stats()
+        tiles = H.autotune(iters=iters)
+        ms = H.time_hxv(warmup=1, iters=iters)
+        o = HxV(H, v)
+        if ref is None:
+            ref = o
+            out.update({"flops_alg": st["flops_alg"], "max_block_dim": st["max_block_dim"], "vsize": int(H.vsize)})
+        tf = st["flops_alg"] / (ms * 1e-3) / 1e12
+        out[prec] = {"ms_per_hxv": ms, "tflops": tf, "frac_of_peak": tf / PEAK_TFLOPS[prec],
+                     "rel_err_vs_fp64": float(np.linalg.norm(o - ref) / np.linalg.norm(ref)),
+                     "tiles": {k: tiles[k] for k in ("wave_edge", "big_min", "big_min64", "explicit_u")}}
+        H.close()
+    out["speedup_bf16x3_vs_fp64"] = out["fp64"]["ms_per_hxv"] / out["bf16x3"]["ms_per_hxv"]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -121,6 +154,8 @@ def main():
                     help="fixed tile configuration wave_edge,big_min,big_min64,explicit_u (instead of autotuning)")
     ap.add_argument("--no-mixed", action="store_true", help="skip the FP32 / split-BF16 x3 HxV lines")
     ap.add_argument("--seed", type=int, default=20261202)
+    ap.add_argument("--large", default="heisenberg_su2_lr40_L60_M4096",
+                    help="MFMA-regime pivot timed in FP64 and the mixed modes beside the main line ('' = skip)")
     ap.add_argument("--sweeps", type=int, default=4, help="DMRG sweeps of the HeisenbergU1 L=32 M=64 chain (0: none)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="only run 5 device copies of this many MiB (reference kernel for the HBM byte counters)")
@@ -291,6 +326,8 @@ def main():
         res["parity_pivots_checked"] = len(errs)
     if args.sweeps > 0:
         res["sweeps"] = sweep_bench(args, sweeps=args.sweeps)
+    if args.large and args.precision == "fp64":
+        res["large_pivot"] = large_pivot_bench(args, args.large)
     print(json.dumps(res))
 
 
