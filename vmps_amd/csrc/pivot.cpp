@@ -588,6 +588,80 @@ void make_tiles(const HostPlan& H, int edge, const std::vector<std::pair<int32_t
   }
 }
 
+}  // namespace
+
+std::vector<int64_t> xcd_order(const std::vector<int64_t>& group, const std::vector<int64_t>& cost) {
+  const size_t n = group.size();
+  // supertiles in order of first appearance, with their items in list order
+  std::unordered_map<int64_t, size_t> gid;
+  std::vector<std::vector<int64_t>> items;
+  std::vector<int64_t> gcost;
+  for (size_t i = 0; i < n; ++i) {
+    auto it = gid.find(group[i]);
+    if (it == gid.end()) {
+      it = gid.emplace(group[i], items.size()).first;
+      items.emplace_back();
+      gcost.push_back(0);
+    }
+    items[it->second].push_back((int64_t)i);
+    gcost[it->second] += cost[i];
+  }
+  std::vector<size_t> gorder(items.size());
+  for (size_t g = 0; g < gorder.size(); ++g) gorder[g] = g;
+  std::stable_sort(gorder.begin(), gorder.end(), [&](size_t a, size_t b) { return gcost[a] > gcost[b]; });
+  // largest supertile first onto the least loaded XCD
+  std::vector<std::vector<int64_t>> queue(kXcdCount);
+  std::vector<int64_t> load(kXcdCount, 0);
+  for (size_t g : gorder) {
+    const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+    queue[x].insert(queue[x].end(), items[g].begin(), items[g].end());
+    load[x] += gcost[g];
+  }
+  size_t len = 0;
+  for (auto& q : queue) len = std::max(len, q.size());
+  std::vector<int64_t> perm;
+  perm.reserve(len * kXcdCount);
+  for (size_t k = 0; k < len; ++k)
+    for (int x = 0; x < kXcdCount; ++x) perm.push_back(k < queue[x].size() ? queue[x][k] : -1);
+  return perm;
+}
+
+namespace {
+// supertile key of a tile: (problem, 4 x 4 block of tiles)
+int64_t supertile_key(std::unordered_map<int64_t, int64_t>& ids, int64_t problem, int mt, int nt) {
+  const int64_t key = (problem * 8192 + mt / 4) * 8192 + nt / 4;
+  auto it = ids.find(key);
+  if (it == ids.end()) it = ids.emplace(key, (int64_t)ids.size()).first;
+  return it->second;
+}
+
+void order_for_xcds(std::vector<TileA>& tiles) {
+  std::unordered_map<int64_t, int64_t> ids, prob;
+  std::vector<int64_t> group, cost;
+  for (const TileA& t : tiles) {
+    const int64_t pkey = t.c_off;  // the T block identifies the (L, V) problem
+    group.push_back(supertile_key(ids, pkey, t.mt, t.nt));
+    cost.push_back(t.K);
+  }
+  std::vector<TileA> out;
+  for (int64_t i : xcd_order(group, cost)) out.push_back(i >= 0 ? tiles[i] : TileA{0, 0, 0, 0, 0, 0, 0, 0});
+  tiles.swap(out);
+}
+
+void order_for_xcds(std::vector<TileC>& tiles, const std::vector<ChunkC>& chunks) {
+  std::unordered_map<int64_t, int64_t> ids;
+  std::vector<int64_t> group, cost;
+  for (const TileC& t : tiles) {
+    group.push_back(supertile_key(ids, t.c_off, t.mt, t.nt));
+    int64_t w = 0;
+    for (int32_t c = t.chunk_begin; c < t.chunk_end; ++c) w += (int64_t)chunks[c].K * (1 + chunks[c].term_end - chunks[c].term_begin);
+    cost.push_back(w);
+  }
+  std::vector<TileC> out;
+  for (int64_t i : xcd_order(group, cost)) out.push_back(i >= 0 ? tiles[i] : TileC{0, 0, 0, 0, 0, 0, 0, 0});
+  tiles.swap(out);
+}
+
 int choose_wave_edge(const HostPlan& H) {
   if (const char* e = std::getenv("VMPS_WAVE_TILE")) {
     const int v = std::atoi(e);
@@ -606,6 +680,8 @@ void set_wave_edge(HostPlan& H, int edge, int big_min64) {
   if (big_min64 > 0) {
     make_tiles(H, edge, H.crange, H.tilesA_w, H.tilesC_w, -big_min64);
     make_tiles(H, kBigTile, H.crange, H.tilesA_b, H.tilesC_b, big_min64);
+    order_for_xcds(H.tilesA_b);
+    order_for_xcds(H.tilesC_b, H.chunksC);
   } else {
     H.tilesA_b.clear();
     H.tilesC_b.clear();
@@ -1085,6 +1161,57 @@ void build_bf16_layout(const HostPlan& H, Bf16Layout& B, bool operands) {
       B.redC_nsplit.push_back((int32_t)S);
       B.partial_slots += S;
     }
+  }
+  // launch order for the XCDs (see xcd_order): big stage A tiles and big
+  // stage C work items, grouped by 4 x 4 supertiles of their block
+  {
+    std::unordered_map<int64_t, int64_t> ids;
+    auto key = [&](int64_t problem, int mt, int nt) {
+      const int64_t k = (problem * 8192 + mt / 4) * 8192 + nt / 4;
+      auto it = ids.find(k);
+      if (it == ids.end()) it = ids.emplace(k, (int64_t)ids.size()).first;
+      return it->second;
+    };
+    std::vector<int64_t> group, cost;
+    for (const TileA16& a : B.tilesA_big) {
+      group.push_back(key(a.c_off, a.mt, a.nt));
+      cost.push_back(a.Kp);
+    }
+    std::vector<TileA16> outA;
+    for (int64_t i : xcd_order(group, cost))
+      outA.push_back(i >= 0 ? B.tilesA_big[i] : TileA16{0, 0, 0, 0, 0, 0, 0, 0, 0, 0});
+    B.tilesA_big.swap(outA);
+    group.clear();
+    cost.clear();
+    ids.clear();
+    for (size_t w = 0; w < B.splitC_tile.size(); ++w) {
+      const TileC& t = B.tilesC_big[B.splitC_tile[w]];
+      group.push_back(key(t.c_off, t.mt, t.nt));
+      cost.push_back(B.splitC_end[w] - B.splitC_begin[w]);
+    }
+    const int32_t empty = (int32_t)B.tilesC_big.size();  // an empty tile for the padding items
+    std::vector<int32_t> wt, wb, we;
+    std::vector<int64_t> ws;
+    bool padded = false;
+    for (int64_t i : xcd_order(group, cost)) {
+      if (i >= 0) {
+        wt.push_back(B.splitC_tile[i]);
+        wb.push_back(B.splitC_begin[i]);
+        we.push_back(B.splitC_end[i]);
+        ws.push_back(B.splitC_slot[i]);
+      } else {
+        wt.push_back(empty);
+        wb.push_back(0);
+        we.push_back(0);
+        ws.push_back(-1);
+        padded = true;
+      }
+    }
+    if (padded) B.tilesC_big.push_back(TileC{0, 0, 0, 0, 0, 0, 0, 0});
+    B.splitC_tile.swap(wt);
+    B.splitC_begin.swap(wb);
+    B.splitC_end.swap(we);
+    B.splitC_slot.swap(ws);
   }
   for (const ChunkC& c : H.chunksC) {
     ChunkC16 d;

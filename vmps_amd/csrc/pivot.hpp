@@ -374,6 +374,14 @@ void build_tiles(HostPlan& H);
 // rebuild the FP64 / FP32 tile lists tilesA_w / tilesC_w (wave tiles of edge
 // 16 or 32) and tilesA_b / tilesC_b (big tiles of the problems >= big_min64)
 void set_wave_edge(HostPlan& H, int edge, int big_min64);
+// Launch order of big-tile work items for MI355X, which hands workgroup b of
+// a launch to XCD b % kXcdCount (each XCD has its own L2): items are grouped
+// by `group` (supertiles: 4 x 4 tiles of one block, which share operand
+// panels), the groups dealt largest first to the least loaded XCD, and the XCD
+// queues interleaved.  Returns item indices in launch order; -1 marks padding
+// slots, filled with empty items, that keep later items on their XCD.
+constexpr int kXcdCount = 8;
+std::vector<int64_t> xcd_order(const std::vector<int64_t>& group, const std::vector<int64_t>& cost);
 // key of the plan's block-size histogram: the share of stage A and stage C
 // flops in classes of (rows, columns), quantised, so that pivots of similar
 // shape share one tuned tile configuration
