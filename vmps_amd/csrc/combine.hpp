@@ -29,13 +29,28 @@ __global__ __launch_bounds__(256) void k_combine(const CombTask* __restrict__ ta
       my_off = tm.t_off;
       my_c = (T)tm.c;
     }
-    for (int j = 0; j < nt; ++j) {
-      const T* __restrict__ src = Tbuf + readlane64(my_off, j) + t.e0;
-      const T c = readlane_t<T>(my_c, j);
+    int j = 0;
+    for (; j + 1 < nt; j += 2) {  // two terms per round: their loads are in flight together
+      const T* __restrict__ s0 = Tbuf + readlane64(my_off, j) + t.e0;
+      const T* __restrict__ s1 = Tbuf + readlane64(my_off, j + 1) + t.e0;
+      const T c0 = readlane_t<T>(my_c, j), c1 = readlane_t<T>(my_c, j + 1);
+      T x0[P], x1[P];
 #pragma unroll
       for (int p = 0; p < P; ++p) {
         const int e = (int)threadIdx.x + 256 * p;
-        if (e < t.n) acc[p] += c * src[e];
+        x0[p] = e < t.n ? s0[e] : T(0);
+        x1[p] = e < t.n ? s1[e] : T(0);
+      }
+#pragma unroll
+      for (int p = 0; p < P; ++p) acc[p] += c0 * x0[p] + c1 * x1[p];
+    }
+    if (j < nt) {
+      const T* __restrict__ s0 = Tbuf + readlane64(my_off, j) + t.e0;
+      const T c0 = readlane_t<T>(my_c, j);
+#pragma unroll
+      for (int p = 0; p < P; ++p) {
+        const int e = (int)threadIdx.x + 256 * p;
+        if (e < t.n) acc[p] += c0 * s0[e];
       }
     }
   }
