@@ -370,7 +370,10 @@ __global__ __launch_bounds__(256) void k_stage_a_big(const TileA* __restrict__ t
 // Stage C, large output blocks: Vout = sum over chunks (sum_j c_j T_j) . R_r.
 // The combine U = sum c_j T_j is formed while staging the A operand: the
 // chunk's term list is loaded once per lane and broadcast with readlane.
-template <class T, class OutT>
+// kOneTerm: every chunk has exactly one term (explicit stage B), which the
+// kernel then streams with the next K step's loads in flight; a separate
+// instantiation, so that the general path's registers do not add up with it.
+template <class T, class OutT, bool kOneTerm>
 __global__ __launch_bounds__(256) void k_stage_c_big(const TileC* __restrict__ tiles, int ntiles,
                                                      const ChunkC* __restrict__ chunks,
                                                      const TermC* __restrict__ terms, const T* __restrict__ Tbuf,
@@ -400,6 +403,32 @@ __global__ __launch_bounds__(256) void k_stage_c_big(const TileC* __restrict__ t
       const TermC tm = terms[ch.term_begin + lane];
       my_toff = tm.t_off;
       my_c = (T)tm.c;
+    }
+    if (kOneTerm) {  // one term (explicit stage B): a plain product, next step's loads in flight
+      const T* __restrict__ T0 = Tbuf + readlane64(my_toff, 0);
+      const T c0 = readlane_t<T>(my_c, 0);
+      T ra[8], rb[8];
+      auto gload = [&](int kb) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          const int ka = kb + ak + 2 * q, kbb = kb + bk + q;
+          ra[q] = (arow && ka < K) ? T0[(int64_t)ka * t.M + m0 + ai] : T(0);
+          rb[q] = (bcol && kbb < K) ? B[(int64_t)(n0 + bj) * K + kbb] : T(0);
+        }
+      };
+      gload(0);
+      for (int kb = 0; kb < K; kb += kBKd) {
+        __syncthreads();  // the previous step's LDS reads are done
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          s.A[ak + 2 * q][ai] = c0 * ra[q];
+          s.B[bk + q][bj] = rb[q];
+        }
+        __syncthreads();
+        if (kb + kBKd < K) gload(kb + kBKd);
+        big_tile_mac<T>(s, wm, wn, li, lk, acc);
+      }
+      continue;
     }
     for (int kb = 0; kb < K; kb += kBKd) {
       T ra[8], rb[8];
@@ -626,9 +655,12 @@ void DevPlan::launch_stage(int stage, const double* d_vin, double* d_vout, hipSt
                              combTerms.p, T.p, Ub.p);
         src = Ub.p;
       }
-      if (nCb)
-        hipLaunchKernelGGL((k_stage_c_big<double, double>), dim3(nCb), dim3(256), 0, s, tilesCb.p, nCb, chunksC.p,
-                           termsC.p, src, R.p, d_vout);
+      if (nCb && H.explicit_u)
+        hipLaunchKernelGGL((k_stage_c_big<double, double, true>), dim3(nCb), dim3(256), 0, s, tilesCb.p, nCb,
+                           chunksC.p, termsC.p, src, R.p, d_vout);
+      else if (nCb)
+        hipLaunchKernelGGL((k_stage_c_big<double, double, false>), dim3(nCb), dim3(256), 0, s, tilesCb.p, nCb,
+                           chunksC.p, termsC.p, src, R.p, d_vout);
       if (nC && H.wave_edge == 16)
         hipLaunchKernelGGL((k_stage_c<double, double, 16>), dim3(nC), dim3(256), 0, s, tilesC.p, nC, chunksC.p,
                            termsC.p, src, R.p, d_vout);
@@ -658,9 +690,12 @@ void DevPlan::launch_stage(int stage, const double* d_vin, double* d_vout, hipSt
                              combTerms.p, T32.p, U32.p);
         src = U32.p;
       }
-      if (nCb)
-        hipLaunchKernelGGL((k_stage_c_big<float, double>), dim3(nCb), dim3(256), 0, s, tilesCb.p, nCb, chunksC.p,
-                           termsC.p, src, R32.p, d_vout);
+      if (nCb && H.explicit_u)
+        hipLaunchKernelGGL((k_stage_c_big<float, double, true>), dim3(nCb), dim3(256), 0, s, tilesCb.p, nCb,
+                           chunksC.p, termsC.p, src, R32.p, d_vout);
+      else if (nCb)
+        hipLaunchKernelGGL((k_stage_c_big<float, double, false>), dim3(nCb), dim3(256), 0, s, tilesCb.p, nCb,
+                           chunksC.p, termsC.p, src, R32.p, d_vout);
       if (nC && H.wave_edge == 16)
         hipLaunchKernelGGL((k_stage_c<float, double, 16>), dim3(nC), dim3(256), 0, s, tilesC.p, nC, chunksC.p,
                            termsC.p, src, R32.p, d_vout);

@@ -440,6 +440,9 @@ __global__ __launch_bounds__(256) void k_stage_a_bf16x3_big(const TileA16* __res
 // Work item w covers iterations [it_begin, it_end) of its tile's flattened
 // (chunk, kBK-deep k block) loop; with slot >= 0 it writes an FP32 partial tile
 // (column-major 128 x 128) that k_reduce_partials sums in slot order.
+// kOneTerm: every chunk has one term (explicit stage B): streamed with the
+// next K step's loads in flight (a separate instantiation, see k_stage_c_big)
+template <bool kOneTerm>
 __global__ __launch_bounds__(256) void k_stage_c_bf16x3_big(const TileC* __restrict__ tiles,
                                                             const int32_t* __restrict__ w_tile,
                                                             const int32_t* __restrict__ w_begin,
@@ -483,6 +486,46 @@ __global__ __launch_bounds__(256) void k_stage_c_bf16x3_big(const TileC* __restr
       const TermC16 tm = terms[ch.term_begin + lane];
       my_toff = tm.t_off;
       my_c = tm.c;
+    }
+    if (kOneTerm) {  // one term (explicit stage B): a plain product, next step's loads in flight
+      const float* __restrict__ T0 = Tp + readlane64(my_toff, 0);
+      const float c0 = readlane_f(my_c, 0);
+      const int s_lo = max(0, it0 - it), s_hi = min(nsteps, it1 - it);  // this work item's K steps
+      v4f xr[kPieces][2];
+      v8bf bh[kPieces], bl[kPieces];
+      auto gload = [&](int kb) {
+#pragma unroll
+        for (int h = 0; h < kPieces; ++h) {
+          const int row = piece_row(tid, h), k = kb + piece_k(tid, h);
+          const bool ka = (m0 + row < t.M) && (k < ch.Kp), kbv = (n0 + row < t.N) && (k < ch.Kp);
+          const float* __restrict__ src = T0 + (int64_t)(m0 + row) * ch.Kp + k;
+          xr[h][0] = ka ? *reinterpret_cast<const v4f*>(src) : zf;
+          xr[h][1] = ka ? *reinterpret_cast<const v4f*>(src + 4) : zf;
+          bh[h] = kbv ? ld8(Rhi + ch.r_off + (int64_t)(n0 + row) * ch.Kp + k) : z;
+          bl[h] = kbv ? ld8(Rlo + ch.r_off + (int64_t)(n0 + row) * ch.Kp + k) : z;
+        }
+      };
+      if (s_lo < s_hi) gload(s_lo * kBK);
+      for (int st = s_lo; st < s_hi; ++st) {
+        v8bf ah[kPieces], al[kPieces];
+#pragma unroll
+        for (int h = 0; h < kPieces; ++h) {
+          float u[8];
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            u[e] = c0 * xr[h][0][e];
+            u[e + 4] = c0 * xr[h][1][e];
+          }
+          split8(u, ah[h], al[h]);
+        }
+        __syncthreads();  // the previous step's LDS reads are done
+        big_stage(s, tid, ah, al, bh, bl);
+        __syncthreads();
+        if (st + 1 < s_hi) gload((st + 1) * kBK);
+        big_compute(s, wm, wn, li, lg, acc);
+      }
+      it += nsteps;
+      continue;
     }
     for (int kb = 0; kb < ch.Kp; kb += kBK, ++it) {
       if (it < it0 || it >= it1) continue;  // workgroup-uniform
@@ -652,10 +695,14 @@ void DevPlan::launch_stage_bf16x3(int stage, const double* d_vin, double* d_vout
                            bfCombTasks.p, bfCombTerms.p, Tp.p, Up.p);
       src = Up.p;
     }
-    if (nCb && nW)
-      hipLaunchKernelGGL(k_stage_c_bf16x3_big, dim3((unsigned)nW), dim3(256), 0, s, bfCbig.p, wTile.p, wBegin.p,
-                         wEnd.p, wSlot.p, nW, bfC.p, bfT.p, src, (const __bf16*)Rhi.p, (const __bf16*)Rlo.p, d_vout,
-                         Cpart.p);
+    if (nCb && nW && H.explicit_u)
+      hipLaunchKernelGGL(k_stage_c_bf16x3_big<true>, dim3((unsigned)nW), dim3(256), 0, s, bfCbig.p, wTile.p,
+                         wBegin.p, wEnd.p, wSlot.p, nW, bfC.p, bfT.p, src, (const __bf16*)Rhi.p, (const __bf16*)Rlo.p,
+                         d_vout, Cpart.p);
+    else if (nCb && nW)
+      hipLaunchKernelGGL(k_stage_c_bf16x3_big<false>, dim3((unsigned)nW), dim3(256), 0, s, bfCbig.p, wTile.p,
+                         wBegin.p, wEnd.p, wSlot.p, nW, bfC.p, bfT.p, src, (const __bf16*)Rhi.p, (const __bf16*)Rlo.p,
+                         d_vout, Cpart.p);
     if (nR)
       hipLaunchKernelGGL(k_reduce_partials, dim3((unsigned)nR), dim3(256), 0, s, bfCbig.p, rTile.p, rSlot.p, rNs.p,
                          Cpart.p, d_vout);
