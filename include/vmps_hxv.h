@@ -247,6 +247,19 @@ typedef struct vmps_env vmps_env;
 vmps_status vmps_env_create(int which, int nq, const int* kinds, int nloc, const int* qloc, int nop, const int* qop,
                             vmps_env** out);
 void vmps_env_destroy(vmps_env* e);
+
+/* ---- two-site split -------------------------------------------------------
+ * Batched thin SVD of dense FP64 blocks on the device: the per-sector SVD of
+ * Biped::truncateSVD (tensors/Biped.h:754-775) that split_AA2
+ * (tensors/DmrgContractions.h:2460-2630) applies to every middle sector of a
+ * two-site wavefunction.  Block b is rows[b] x cols[b], column-major, packed
+ * back to back in A.  With r_b = min(rows[b], cols[b]) the outputs are packed
+ * per block: U (rows x r), S (r values, descending), Vt (r x cols), all
+ * column-major.  One-sided Jacobi in FP64 (every singular value to rounding-
+ * level relative accuracy); max_sweeps (may be NULL) receives the largest
+ * number of Jacobi sweeps a block needed.  Host buffers, synchronous. */
+vmps_status vmps_block_svd(int nblocks, const int64_t* rows, const int64_t* cols, const double* A, double* U,
+                           double* S, double* Vt, int* max_sweeps);
 /* one sector {in, out, mid} of Lold / Rold, packed as in vmps_pivot2_add_env */
 vmps_status vmps_env_add_old(vmps_env* e, const int* qin, const int* qout, const int* qmid, int adim,
                              const int64_t* rows, const int64_t* cols, const double* data);

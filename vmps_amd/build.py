@@ -16,7 +16,7 @@ BUILD = os.path.join(HERE, "_build")
 ARCH = os.environ.get("VMPS_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = ["sym.cpp", "pivot.cpp", "env.cpp", "capi.cpp", "capi_env.cpp", "engine.hip", "engine_bf16.hip",
-           "lanczos.hip"]
+           "lanczos.hip", "svd.hip"]
 # C++ units that call the HIP runtime API are compiled in HIP mode
 HIP_HOST_UNITS = {"capi.cpp", "capi_env.cpp"}
 
