@@ -120,6 +120,26 @@ vmps_status vmps_pivot2_add_V_block(vmps_pivot2* p, int s, const int* qin, const
                                     int64_t cols, int64_t* offset);
 vmps_status vmps_pivot2_vsize(const vmps_pivot2* p, int64_t* size);
 
+/* Bulk forms of the three builder calls above, for hosts that hold a whole
+ * Tripod, W slot or PivotVector layout at once (same checks, item by item;
+ * the first failing item's status is returned):
+ *  - add_env_sectors: nsec sectors; qin/qout/qmid nsec*nq ints, adim[nsec],
+ *    rows/cols sum(adim) entries in sector order, data the non-empty blocks
+ *    back to back in the same order;
+ *  - add_W_blocks: nblk blocks of slot `which`; idx = nblk x {s1, s2, k},
+ *    qin/qout nblk*nq, dims = nblk x {rows, cols}, nnz[nblk], and the
+ *    triplets of all blocks back to back in row/col/val;
+ *  - add_V_blocks: nblk blocks, component s[b]; offsets (may be NULL)
+ *    receives each block's element offset. */
+vmps_status vmps_pivot2_add_env_sectors(vmps_pivot2* p, int which, int nsec, const int* qin, const int* qout,
+                                        const int* qmid, const int* adim, const int64_t* rows, const int64_t* cols,
+                                        const double* data);
+vmps_status vmps_pivot2_add_W_blocks(vmps_pivot2* p, int which, int nblk, const int* idx, const int* qin,
+                                     const int* qout, const int* dims, const int64_t* nnz, const int* row,
+                                     const int* col, const double* val);
+vmps_status vmps_pivot2_add_V_blocks(vmps_pivot2* p, int nblk, const int* s, const int* qin, const int* qout,
+                                     const int64_t* rows, const int64_t* cols, int64_t* offsets);
+
 /* precalc_blockStructure(L, Vin, W12, W34, Vin, R, qloc12, qloc34, qOp12,
  * qOp34, qlhs, qrhs, factor_cgcs) (tensors/DmrgIndexGymnastics.h:698-856),
  * or the one-site form (L, Vin, W, Vin, R, qloc, qOp, ...) (:485-590), for
@@ -269,6 +289,15 @@ vmps_status vmps_env_add_A(vmps_env* e, int bra, int s, const int* qin, const in
 /* sparse block of W[s1][s2][k] */
 vmps_status vmps_env_add_W(vmps_env* e, int s1, int s2, int k, const int* qin, const int* qout, int rows, int cols,
                            int64_t nnz, const int* row, const int* col, const double* val);
+/* bulk forms of the three calls above, packed as the vmps_pivot2_add_*
+ * bulk forms (A blocks: s[nblk], rows/cols[nblk], data back to back) */
+vmps_status vmps_env_add_old_sectors(vmps_env* e, int nsec, const int* qin, const int* qout, const int* qmid,
+                                     const int* adim, const int64_t* rows, const int64_t* cols, const double* data);
+vmps_status vmps_env_add_A_blocks(vmps_env* e, int bra, int nblk, const int* s, const int* qin, const int* qout,
+                                  const int64_t* rows, const int64_t* cols, const double* data);
+vmps_status vmps_env_add_W_blocks(vmps_env* e, int nblk, const int* idx, const int* qin, const int* qout,
+                                  const int* dims, const int64_t* nnz, const int* row, const int* col,
+                                  const double* val);
 /* plan and run the contraction on the current HIP device (synchronous) */
 vmps_status vmps_env_contract(vmps_env* e, vmps_precision prec);
 /* result: nsec sectors, nslots = sum of adim, size = total elements */

@@ -60,7 +60,50 @@ def test_layout_errors():
     # block structure read before precalc
     nl, nr = ctypes.c_int64(), ctypes.c_int64()
     assert lib.vmps_pivot2_get_blockstructure(h, ctypes.byref(nl), ctypes.byref(nr), None, None, None, None) == 6
+    # bulk forms: a sector already fed is a duplicate, a bad W index an argument error,
+    # with the status of the failing item
+    from vmps_amd import pack
+    bk = pack.tripod(pd.L[:1], 1)
+    assert lib.vmps_pivot2_add_env_sectors(h, 0, bk.count, *bk.args()) == 2
+    assert b"duplicate" in lib.vmps_last_error()
+    s1, s2, k, qi, qo, rows, cols, r, c, v = W.blocks[0]
+    bw = pack.w_blocks([(s1, s2, 99, qi, qo, rows, cols, r, c, v)], 1)
+    assert lib.vmps_pivot2_add_W_blocks(h, 12, bw.count, *bw.args()) == 1
+    comp = next(i for i, blocks in enumerate(pd.V) if blocks)
+    bv = pack.v_blocks([[] for _ in range(comp)] + [pd.V[comp][:1]], 1)
+    assert lib.vmps_pivot2_add_V_blocks(h, bv.count, *bv.args(), None) == 2
     lib.vmps_pivot2_destroy(h)
+
+
+def test_bulk_builders_equal_single_item_calls():
+    """The engine's bulk feeding and the one-call-per-item feeding give the
+    same vector layout and block structure."""
+    lib = _lib.load()
+    from vmps_amd.pivot import get_blockstructure
+    from vmps_amd import models
+    W = models.heisenberg_u1_mpo()
+    rng = np.random.default_rng(3)
+    pd = synth.make_pivot(W, W, [((1,), 3), ((-1,), 2), ((3,), 1)], [((1,), 2), ((-1,), 3), ((-3,), 1)], rng)
+    hb = synth.feed(lib, "vmps_", pd, _lib.check)
+
+    class Single:  # the same library seen under a prefix that selects the per-item path
+        def __getattr__(self, name):
+            return getattr(lib, name.replace("single_", "vmps_", 1))
+
+    hs = synth.feed(Single(), "single_", pd, _lib.check)
+    for h in (hb, hs):
+        _lib.check(lib.vmps_pivot2_precalc(h))
+    sizes = []
+    for h in (hb, hs):
+        n = ctypes.c_int64()
+        _lib.check(lib.vmps_pivot2_vsize(h, ctypes.byref(n)))
+        sizes.append(n.value)
+    assert sizes[0] == sizes[1] == pd.vsize
+    bb, bs = get_blockstructure(lib, hb), get_blockstructure(lib, hs)
+    for x, y in zip(bb, bs):
+        assert np.array_equal(np.asarray(x), np.asarray(y))
+    for h in (hb, hs):
+        lib.vmps_pivot2_destroy(h)
 
 
 def test_set_blockstructure_rejects_bad_tables():

@@ -304,6 +304,61 @@ vmps_status vmps_pivot2_add_V_block(vmps_pivot2* p, int s, const int* qin, const
   VMPS_API_END
 }
 
+// Whole-environment / whole-slot forms of the builder calls above: one call
+// per Tripod, W slot or vector instead of one per sector or block.  Each
+// item goes through the single-item entry point, so the checks are the same
+// and the first failing item's status is returned.
+vmps_status vmps_pivot2_add_env_sectors(vmps_pivot2* p, int which, int nsec, const int* qin, const int* qout,
+                                        const int* qmid, const int* adim, const int64_t* rows, const int64_t* cols,
+                                        const double* data) {
+  VMPS_API_BEGIN
+  require(p && nsec >= 0 && (nsec == 0 || (qin && qout && qmid && adim)), VMPS_ERR_ARG, "bad argument");
+  const int nq = p->P.sym.nq;
+  int64_t slot = 0, pos = 0;
+  for (int q = 0; q < nsec; ++q) {
+    require(adim[q] >= 0, VMPS_ERR_ARG, "adim < 0");
+    const vmps_status st = vmps_pivot2_add_env(p, which, qin + q * nq, qout + q * nq, qmid + q * nq, adim[q],
+                                               rows ? rows + slot : nullptr, cols ? cols + slot : nullptr,
+                                               data ? data + pos : nullptr);
+    if (st != VMPS_OK) return st;
+    for (int a = 0; a < adim[q]; ++a) pos += rows[slot + a] * cols[slot + a];
+    slot += adim[q];
+  }
+  VMPS_API_END
+}
+
+vmps_status vmps_pivot2_add_W_blocks(vmps_pivot2* p, int which, int nblk, const int* idx, const int* qin,
+                                     const int* qout, const int* dims, const int64_t* nnz, const int* row,
+                                     const int* col, const double* val) {
+  VMPS_API_BEGIN
+  require(p && nblk >= 0 && (nblk == 0 || (idx && qin && qout && dims && nnz)), VMPS_ERR_ARG, "bad argument");
+  const int nq = p->P.sym.nq;
+  int64_t pos = 0;
+  for (int b = 0; b < nblk; ++b) {
+    require(nnz[b] >= 0, VMPS_ERR_ARG, "nnz < 0");
+    const vmps_status st =
+        vmps_pivot2_add_W(p, which, idx[3 * b], idx[3 * b + 1], idx[3 * b + 2], qin + b * nq, qout + b * nq,
+                          dims[2 * b], dims[2 * b + 1], nnz[b], row ? row + pos : nullptr,
+                          col ? col + pos : nullptr, val ? val + pos : nullptr);
+    if (st != VMPS_OK) return st;
+    pos += nnz[b];
+  }
+  VMPS_API_END
+}
+
+vmps_status vmps_pivot2_add_V_blocks(vmps_pivot2* p, int nblk, const int* s, const int* qin, const int* qout,
+                                     const int64_t* rows, const int64_t* cols, int64_t* offsets) {
+  VMPS_API_BEGIN
+  require(p && nblk >= 0 && (nblk == 0 || (s && qin && qout && rows && cols)), VMPS_ERR_ARG, "bad argument");
+  const int nq = p->P.sym.nq;
+  for (int b = 0; b < nblk; ++b) {
+    const vmps_status st = vmps_pivot2_add_V_block(p, s[b], qin + b * nq, qout + b * nq, rows[b], cols[b],
+                                                   offsets ? offsets + b : nullptr);
+    if (st != VMPS_OK) return st;
+  }
+  VMPS_API_END
+}
+
 vmps_status vmps_pivot2_vsize(const vmps_pivot2* p, int64_t* size) {
   VMPS_API_BEGIN
   require(p && size, VMPS_ERR_ARG, "NULL argument");

@@ -161,6 +161,59 @@ vmps_status vmps_env_add_W(vmps_env* e, int s1, int s2, int k, const int* qin, c
   VMPS_API_END
 }
 
+// Whole-Tripod / whole-list forms of the three calls above (one call instead
+// of one per sector or block); each item goes through the single-item call.
+vmps_status vmps_env_add_old_sectors(vmps_env* e, int nsec, const int* qin, const int* qout, const int* qmid,
+                                     const int* adim, const int64_t* rows, const int64_t* cols, const double* data) {
+  VMPS_API_BEGIN
+  require(e && nsec >= 0 && (nsec == 0 || (qin && qout && qmid && adim)), VMPS_ERR_ARG, "bad argument");
+  const int nq = e->P.sym.nq;
+  int64_t slot = 0, pos = 0;
+  for (int q = 0; q < nsec; ++q) {
+    require(adim[q] >= 0, VMPS_ERR_ARG, "adim < 0");
+    const vmps_status st = vmps_env_add_old(e, qin + q * nq, qout + q * nq, qmid + q * nq, adim[q],
+                                            rows ? rows + slot : nullptr, cols ? cols + slot : nullptr,
+                                            data ? data + pos : nullptr);
+    if (st != VMPS_OK) return st;
+    for (int a = 0; a < adim[q]; ++a) pos += rows[slot + a] * cols[slot + a];
+    slot += adim[q];
+  }
+  VMPS_API_END
+}
+
+vmps_status vmps_env_add_A_blocks(vmps_env* e, int bra, int nblk, const int* s, const int* qin, const int* qout,
+                                  const int64_t* rows, const int64_t* cols, const double* data) {
+  VMPS_API_BEGIN
+  require(e && nblk >= 0 && (nblk == 0 || (s && qin && qout && rows && cols)), VMPS_ERR_ARG, "bad argument");
+  const int nq = e->P.sym.nq;
+  int64_t pos = 0;
+  for (int b = 0; b < nblk; ++b) {
+    const vmps_status st = vmps_env_add_A(e, bra, s[b], qin + b * nq, qout + b * nq, rows[b], cols[b],
+                                          data ? data + pos : nullptr);
+    if (st != VMPS_OK) return st;
+    pos += rows[b] * cols[b];
+  }
+  VMPS_API_END
+}
+
+vmps_status vmps_env_add_W_blocks(vmps_env* e, int nblk, const int* idx, const int* qin, const int* qout,
+                                  const int* dims, const int64_t* nnz, const int* row, const int* col,
+                                  const double* val) {
+  VMPS_API_BEGIN
+  require(e && nblk >= 0 && (nblk == 0 || (idx && qin && qout && dims && nnz)), VMPS_ERR_ARG, "bad argument");
+  const int nq = e->P.sym.nq;
+  int64_t pos = 0;
+  for (int b = 0; b < nblk; ++b) {
+    require(nnz[b] >= 0, VMPS_ERR_ARG, "nnz < 0");
+    const vmps_status st = vmps_env_add_W(e, idx[3 * b], idx[3 * b + 1], idx[3 * b + 2], qin + b * nq, qout + b * nq,
+                                          dims[2 * b], dims[2 * b + 1], nnz[b], row ? row + pos : nullptr,
+                                          col ? col + pos : nullptr, val ? val + pos : nullptr);
+    if (st != VMPS_OK) return st;
+    pos += nnz[b];
+  }
+  VMPS_API_END
+}
+
 vmps_status vmps_env_contract(vmps_env* e, vmps_precision prec) {
   VMPS_API_BEGIN
   require(e != nullptr, VMPS_ERR_ARG, "NULL env");

@@ -22,6 +22,7 @@ multiplets.
 """
 import ctypes
 import math
+import os
 import time
 
 import numpy as np
@@ -236,8 +237,35 @@ def block_svd_device(mats):
     return out
 
 
+_HOST_SVD = {}
+
+
+def _host_workers():
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
 def block_svd_host(mats):
-    return [np.linalg.svd(m, full_matrices=False) for m in mats]
+    """Thin SVD of every matrix with host LAPACK (numpy's gesdd).  When the
+    work is large enough, the blocks go to a thread pool, largest first,
+    with one BLAS thread per call (LAPACK releases the GIL; nested BLAS
+    threads would oversubscribe the cores).  The result of each block is the
+    same as a single-threaded call's."""
+    cost = [m.shape[0] * m.shape[1] * min(m.shape) for m in mats]
+    if len(mats) < 2 or sum(cost) < 4_000_000:
+        return [np.linalg.svd(m, full_matrices=False) for m in mats]
+    if not _HOST_SVD:
+        from concurrent.futures import ThreadPoolExecutor
+        from threadpoolctl import ThreadpoolController
+        _HOST_SVD["pool"] = ThreadPoolExecutor(_host_workers())
+        _HOST_SVD["blas"] = ThreadpoolController()
+    pool = _HOST_SVD["pool"]
+    with _HOST_SVD["blas"].limit(limits=1, user_api="blas"):
+        futs = {i: pool.submit(np.linalg.svd, mats[i], False) for i in sorted(range(len(mats)), key=lambda i: -cost[i])}
+        return [futs[i].result() for i in range(len(mats))]
 
 
 def split_psi(pd, v, qloc, M, eps_svd=1e-14, left_to_right=True, svd=None):

@@ -9,7 +9,7 @@ import ctypes
 
 import numpy as np
 
-from . import _lib
+from . import _lib, pack
 
 P_int = ctypes.POINTER(ctypes.c_int)
 P_i64 = ctypes.POINTER(ctypes.c_int64)
@@ -31,33 +31,16 @@ def _contract(which, site, old, Abra, Aket, precision="fp64", stats=None):
     _lib.check(lib.vmps_env_create(which, sym.nq, pk, len(site.qloc), pql, len(site.qop), pqo, ctypes.byref(h)))
     keep = []
     try:
-        for (qin, qout, qmid, blocks) in old:
-            a, pa = _iarr(qin)
-            b, pb = _iarr(qout)
-            c, pc = _iarr(qmid)
-            rows = np.array([x.shape[0] if x is not None else 0 for x in blocks], dtype=np.int64)
-            cols = np.array([x.shape[1] if x is not None else 0 for x in blocks], dtype=np.int64)
-            data = np.ascontiguousarray(np.concatenate([x.ravel(order="F") for x in blocks if x is not None]
-                                                       or [np.zeros(0)]))
-            keep.append((a, b, c, rows, cols, data))
-            _lib.check(lib.vmps_env_add_old(h, pa, pb, pc, len(blocks), rows.ctypes.data_as(P_i64),
-                                            cols.ctypes.data_as(P_i64), data.ctypes.data_as(P_dbl)))
+        bk = pack.tripod(old, sym.nq)
+        keep.append(bk)
+        _lib.check(lib.vmps_env_add_old_sectors(h, bk.count, *bk.args()))
         for bra, A in ((1, Abra), (0, Aket)):
-            for s, blocks in enumerate(A):
-                for (qin, qout, blk) in blocks:
-                    a, pa = _iarr(qin)
-                    b, pb = _iarr(qout)
-                    d = np.ascontiguousarray(np.asarray(blk, dtype=np.float64).ravel(order="F"))
-                    _lib.check(lib.vmps_env_add_A(h, bra, s, pa, pb, blk.shape[0], blk.shape[1],
-                                                  d.ctypes.data_as(P_dbl)))
-        for (s1, s2, k, qi, qo_, rows, cols, r, c, v) in site.blocks:
-            a, pa = _iarr(qi)
-            b, pb = _iarr(qo_)
-            r = np.ascontiguousarray(r, dtype=np.int32)
-            c = np.ascontiguousarray(c, dtype=np.int32)
-            v = np.ascontiguousarray(v, dtype=np.float64)
-            _lib.check(lib.vmps_env_add_W(h, s1, s2, k, pa, pb, rows, cols, len(v), r.ctypes.data_as(P_int),
-                                          c.ctypes.data_as(P_int), v.ctypes.data_as(P_dbl)))
+            bk = pack.a_blocks(A, sym.nq)
+            keep.append(bk)
+            _lib.check(lib.vmps_env_add_A_blocks(h, bra, bk.count, *bk.args()))
+        bk = pack.w_blocks(site.blocks, sym.nq)
+        keep.append(bk)
+        _lib.check(lib.vmps_env_add_W_blocks(h, bk.count, *bk.args()))
         _lib.check(lib.vmps_env_contract(h, _lib.PREC[precision]))
         nsec, nslots, size = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
         _lib.check(lib.vmps_env_result_size(h, ctypes.byref(nsec), ctypes.byref(nslots), ctypes.byref(size)))

@@ -14,7 +14,7 @@ import math
 
 import numpy as np
 
-from . import models
+from . import models, pack
 
 
 # ---------------------------------------------------------------------------
@@ -418,7 +418,13 @@ def _iarr(seq):
     return a, a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
 
 
-def _feed_envs(fn, h, L, R, check, keep):
+def _feed_envs(fn, h, L, R, check, keep, nq=None, bulk=False):
+    if bulk:
+        for which, env in ((0, L), (1, R)):
+            pk = pack.tripod(env, nq)
+            keep.append(pk)
+            check(fn("pivot2_add_env_sectors")(h, which, pk.count, *pk.args()))
+        return
     for which, env in ((0, L), (1, R)):
         for (qin, qout, qmid, blocks) in env:
             a, pa = _iarr(qin)
@@ -435,7 +441,12 @@ def _feed_envs(fn, h, L, R, check, keep):
                                        data.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
 
 
-def _feed_W(fn, h, slots, check):
+def _feed_W(fn, h, slots, check, nq=None, bulk=False):
+    if bulk:
+        for which, W in slots:
+            pk = pack.w_blocks(W.blocks, nq)
+            check(fn("pivot2_add_W_blocks")(h, which, pk.count, *pk.args()))
+        return
     for which, W in slots:
         for (s1, s2, k, qi, qo, rows, cols, r, c, v) in W.blocks:
             a, pa = _iarr(qi)
@@ -463,6 +474,7 @@ def feed(lib, prefix, pd, check, projectors=None, Epenalty=0.0, terms=None):
     which summands to feed (0 = the first, i >= 1 = pd.extra[i-1]); the
     default is all of them.  Returns the handle."""
     fn = lambda name: getattr(lib, prefix + name)
+    bulk = prefix == "vmps_"  # the engine takes whole Tripods / W slots / layouts per call
     sym = pd.sym
     kinds, pk = _iarr(sym.kinds)
     h = ctypes.c_void_p()
@@ -492,10 +504,13 @@ def feed(lib, prefix, pd, check, projectors=None, Epenalty=0.0, terms=None):
             n12 = len(t.W12.qop) if t.W12 is not None else 0
             n34 = len(t.W34.qop) if t.W34 is not None else 0
             check(fn("pivot_add_term")(h, n12, po12, n34, po34, None))
-        _feed_envs(fn, h, t.L, t.R, check, keep)
-        _feed_W(fn, h, _w_slots(pd, t.W12, t.W34), check)
+        _feed_envs(fn, h, t.L, t.R, check, keep, sym.nq, bulk)
+        _feed_W(fn, h, _w_slots(pd, t.W12, t.W34), check, sym.nq, bulk)
+    if bulk:
+        pk = pack.v_blocks(pd.V, sym.nq)
+        check(fn("pivot2_add_V_blocks")(h, pk.count, *pk.args(), None))
     off = ctypes.c_int64()
-    for s, blocks in enumerate(pd.V):
+    for s, blocks in (() if bulk else enumerate(pd.V)):
         for (qi, qo, rr, cc) in blocks:
             a, pa = _iarr(qi)
             b, pb = _iarr(qo)
