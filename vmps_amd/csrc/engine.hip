@@ -95,12 +95,14 @@ struct WaveTile {
 // Stage A: T (M x N, ld M) = L (M x K, ld M) . V (K x N, ld K), one E x E tile
 // per wave.  The loads of the next 16-deep K block are issued before the
 // MFMAs of the current one (two register buffers), so a wave waits for memory
-// once per tile rather than once per K block.
+// once per tile rather than once per K block.  The wave index goes through
+// readfirstlane so the compiler knows it is wave-uniform: the descriptor is
+// then one scalar load and the K loop branches on scalars instead of masks.
 template <class T, int E>
 __global__ __launch_bounds__(256) void k_stage_a(const TileA* __restrict__ tiles, int ntiles, const T* __restrict__ L,
                                                  const T* __restrict__ V, T* __restrict__ Tout) {
   constexpr int S = E / 16;
-  const int w = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int w = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   if (w >= ntiles) return;
   const int lane = threadIdx.x & 63;
   const TileA t = tiles[w];
@@ -153,7 +155,7 @@ __global__ __launch_bounds__(256) void k_stage_c(const TileC* __restrict__ tiles
   typedef typename Mfma<T>::acc_t acc_t;
   __shared__ acc_t red[3][S * S][64];
   if ((int)blockIdx.x >= ntiles) return;  // workgroup-uniform
-  const int wave = threadIdx.x >> 6;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   const TileC t = tiles[blockIdx.x];
   const int m0 = t.mt * E, n0 = t.nt * E;
@@ -329,7 +331,7 @@ __global__ __launch_bounds__(256) void k_stage_a_big(const TileA* __restrict__ t
                                                      T* __restrict__ Tout) {
   __shared__ BigTileSmem<T> s;
   if ((int)blockIdx.x >= ntiles) return;  // workgroup-uniform
-  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int tid = threadIdx.x, wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
   const int li = lane & 15, lk = lane >> 4;
   const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
   const TileA t = tiles[blockIdx.x];
@@ -380,7 +382,7 @@ __global__ __launch_bounds__(256) void k_stage_c_big(const TileC* __restrict__ t
                                                      const T* __restrict__ R, OutT* __restrict__ Vout) {
   __shared__ BigTileSmem<T> s;
   if ((int)blockIdx.x >= ntiles) return;  // workgroup-uniform
-  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int tid = threadIdx.x, wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
   const int li = lane & 15, lk = lane >> 4;
   const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
   const TileC t = tiles[blockIdx.x];

@@ -105,7 +105,7 @@ __global__ __launch_bounds__(256) void k_stage_a_bf16x3(const TileA16* __restric
                                                         const __bf16* __restrict__ Vhi, const __bf16* __restrict__ Vlo,
                                                         float* __restrict__ Tp) {
   const int nwaves = gridDim.x * 4;
-  const int w0 = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int w0 = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   if (w0 >= ntiles) return;
   const int nmine = (ntiles - 1 - w0) / nwaves + 1;  // tiles w0 + j * nwaves, j < nmine
   const int lane = threadIdx.x & 63;
@@ -193,7 +193,7 @@ __global__ __launch_bounds__(256) void k_stage_c_bf16x3(const TileC* __restrict_
                                                         double* __restrict__ Vout) {
   __shared__ v4f red[3][4][64];
   if ((int)blockIdx.x >= ntiles) return;  // workgroup-uniform
-  const int wave = threadIdx.x >> 6;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   const TileC t = tiles[blockIdx.x];
   const int m0 = t.mt * kTile, n0 = t.nt * kTile;
@@ -390,7 +390,7 @@ __global__ __launch_bounds__(256) void k_stage_a_bf16x3_big(const TileA16* __res
                                                             const __bf16* __restrict__ Vlo, float* __restrict__ Tp) {
   __shared__ BigSmem s;
   if ((int)blockIdx.x >= ntiles) return;  // workgroup-uniform
-  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int tid = threadIdx.x, wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
   const int li = lane & 15, lg = lane >> 4;
   const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
   const TileA16 t = tiles[blockIdx.x];
@@ -456,7 +456,7 @@ __global__ __launch_bounds__(256) void k_stage_c_bf16x3_big(const TileC* __restr
                                                             double* __restrict__ Vout, float* __restrict__ partial) {
   __shared__ BigSmem s;
   if ((int)blockIdx.x >= nwork) return;  // workgroup-uniform
-  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int tid = threadIdx.x, wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
   const int li = lane & 15, lg = lane >> 4;
   const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
   const TileC t = tiles[w_tile[blockIdx.x]];

@@ -590,7 +590,7 @@ void make_tiles(const HostPlan& H, int edge, const std::vector<std::pair<int32_t
 
 }  // namespace
 
-std::vector<int64_t> xcd_order(const std::vector<int64_t>& group, const std::vector<int64_t>& cost) {
+std::vector<int64_t> xcd_order(const std::vector<int64_t>& group, const std::vector<int64_t>& cost, int per) {
   const size_t n = group.size();
   // supertiles in order of first appearance, with their items in list order
   std::unordered_map<int64_t, size_t> gid;
@@ -619,10 +619,12 @@ std::vector<int64_t> xcd_order(const std::vector<int64_t>& group, const std::vec
   }
   size_t len = 0;
   for (auto& q : queue) len = std::max(len, q.size());
+  len = (len + per - 1) / per * per;
   std::vector<int64_t> perm;
   perm.reserve(len * kXcdCount);
-  for (size_t k = 0; k < len; ++k)
-    for (int x = 0; x < kXcdCount; ++x) perm.push_back(k < queue[x].size() ? queue[x][k] : -1);
+  for (size_t k = 0; k < len; k += per)
+    for (int x = 0; x < kXcdCount; ++x)
+      for (int u = 0; u < per; ++u) perm.push_back(k + u < queue[x].size() ? queue[x][k + u] : -1);
   return perm;
 }
 
@@ -645,6 +647,21 @@ void order_for_xcds(std::vector<TileA>& tiles) {
   }
   std::vector<TileA> out;
   for (int64_t i : xcd_order(group, cost)) out.push_back(i >= 0 ? tiles[i] : TileA{0, 0, 0, 0, 0, 0, 0, 0});
+  tiles.swap(out);
+}
+
+// FP64 / FP32 wave tiles of stage A: 4 consecutive tiles form one workgroup,
+// so the XCD queues are interleaved 4 tiles at a time; all tiles of one T
+// block go to one XCD, whose L2 then serves the L and V re-reads.
+void order_wave_tiles_for_xcds(std::vector<TileA>& tiles) {
+  std::unordered_map<int64_t, int64_t> ids;
+  std::vector<int64_t> group, cost;
+  for (const TileA& t : tiles) {
+    group.push_back(supertile_key(ids, t.c_off, t.mt, t.nt));
+    cost.push_back(t.K + 8);  // + fixed cost of a tile (descriptor, store)
+  }
+  std::vector<TileA> out;
+  for (int64_t i : xcd_order(group, cost, 4)) out.push_back(i >= 0 ? tiles[i] : TileA{0, 0, 0, 0, 0, 0, 0, 0});
   tiles.swap(out);
 }
 
@@ -692,6 +709,11 @@ void set_wave_edge(HostPlan& H, int edge, int big_min64) {
       make_tiles(H, edge, H.crange, H.tilesA_w, H.tilesC_w);
     }
   }
+  // XCD-local wave tiles (one workgroup per 4 stage A tiles, one per stage C
+  // tile): the tiles of a block share one XCD's L2.  Short lists stay in
+  // cost order, where the padding of the XCD rounds would cost more.
+  if (H.tilesA_w.size() >= kXcdMinTilesA) order_wave_tiles_for_xcds(H.tilesA_w);
+  if (H.tilesC_w.size() >= kXcdMinTilesC) order_for_xcds(H.tilesC_w, H.chunksC);
   // bounds of every FP64 / FP32 tile list
   const int64_t src_size = H.explicit_u ? H.U_size : H.T_size;
   for (const auto* list : {&H.tilesC_w, &H.tilesC_b})

@@ -378,10 +378,13 @@ void set_wave_edge(HostPlan& H, int edge, int big_min64);
 // a launch to XCD b % kXcdCount (each XCD has its own L2): items are grouped
 // by `group` (supertiles: 4 x 4 tiles of one block, which share operand
 // panels), the groups dealt largest first to the least loaded XCD, and the XCD
-// queues interleaved.  Returns item indices in launch order; -1 marks padding
-// slots, filled with empty items, that keep later items on their XCD.
+// queues interleaved `per` items at a time (the items of one workgroup).
+// Returns item indices in launch order; -1 marks padding slots, filled with
+// empty items, that keep later items on their XCD.
 constexpr int kXcdCount = 8;
-std::vector<int64_t> xcd_order(const std::vector<int64_t>& group, const std::vector<int64_t>& cost);
+std::vector<int64_t> xcd_order(const std::vector<int64_t>& group, const std::vector<int64_t>& cost, int per = 1);
+// wave tile lists at least this long are grouped per XCD (set_wave_edge)
+constexpr size_t kXcdMinTilesA = 2048, kXcdMinTilesC = 512;
 // key of the plan's block-size histogram: the share of stage A and stage C
 // flops in classes of (rows, columns), quantised, so that pivots of similar
 // shape share one tuned tile configuration
