@@ -270,6 +270,9 @@ __global__ __launch_bounds__(256) void k_stage_c(const TileC* __restrict__ tiles
 }
 
 // ---- LDS-staged 128 x 128 workgroup tiles for large blocks ------------------
+// The big-tile kernels ask for two waves per SIMD (amdgpu_waves_per_eu): the
+// compiler then keeps accumulators and operands within 256 registers (no
+// spills) instead of using up to 384, and two workgroups share a CU.
 // Problems with both dimensions >= H.big_min64 (a tile configuration choice,
 // off by default).  4 waves in a 2 x 2 grid, each owning 64 x 64 = 4 x 4 MFMA
 // accumulators.  Per 16-deep K step the workgroup
@@ -326,7 +329,7 @@ __device__ inline void big_tile_store(OutT* __restrict__ C, int M, int N, int m0
 //   A (column-major, ld M): row i = tid % 128, k = kb + tid/128 + 2q, q < 8
 //   B (column-major K x N, ld K): column j = tid / 2, k = kb + 8 (tid % 2) + q
 template <class T>
-__global__ __launch_bounds__(256) void k_stage_a_big(const TileA* __restrict__ tiles, int ntiles,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_stage_a_big(const TileA* __restrict__ tiles, int ntiles,
                                                      const T* __restrict__ L, const T* __restrict__ V,
                                                      T* __restrict__ Tout) {
   __shared__ BigTileSmem<T> s;
@@ -376,7 +379,7 @@ __global__ __launch_bounds__(256) void k_stage_a_big(const TileA* __restrict__ t
 // kernel then streams with the next K step's loads in flight; a separate
 // instantiation, so that the general path's registers do not add up with it.
 template <class T, class OutT, bool kOneTerm>
-__global__ __launch_bounds__(256) void k_stage_c_big(const TileC* __restrict__ tiles, int ntiles,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_stage_c_big(const TileC* __restrict__ tiles, int ntiles,
                                                      const ChunkC* __restrict__ chunks,
                                                      const TermC* __restrict__ terms, const T* __restrict__ Tbuf,
                                                      const T* __restrict__ R, OutT* __restrict__ Vout) {

@@ -383,7 +383,7 @@ __device__ inline void big_stage(BigSmem& s, int tid, const v8bf (&ah)[kPieces],
 }
 
 // Stage A, large problems: Tp (row-major M x Np) = Lt (row-major M x Kp) . V (col-major Kp x N)
-__global__ __launch_bounds__(256) void k_stage_a_bf16x3_big(const TileA16* __restrict__ tiles, int ntiles,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_stage_a_bf16x3_big(const TileA16* __restrict__ tiles, int ntiles,
                                                             const __bf16* __restrict__ Lhi,
                                                             const __bf16* __restrict__ Llo,
                                                             const __bf16* __restrict__ Vhi,
@@ -443,7 +443,7 @@ __global__ __launch_bounds__(256) void k_stage_a_bf16x3_big(const TileA16* __res
 // kOneTerm: every chunk has one term (explicit stage B): streamed with the
 // next K step's loads in flight (a separate instantiation, see k_stage_c_big)
 template <bool kOneTerm>
-__global__ __launch_bounds__(256) void k_stage_c_bf16x3_big(const TileC* __restrict__ tiles,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_stage_c_bf16x3_big(const TileC* __restrict__ tiles,
                                                             const int32_t* __restrict__ w_tile,
                                                             const int32_t* __restrict__ w_begin,
                                                             const int32_t* __restrict__ w_end,

@@ -808,9 +808,12 @@ void build_tiles(HostPlan& H) {
     if (t.term_begin < 0 || t.term_end > (int32_t)H.comb_terms.size() || t.term_end < t.term_begin || t.n < 0 ||
         t.u_off + t.e0 + t.n > H.U_size)
       throw VmpsError(VMPS_ERR_INTERNAL, "stage B: task out of range");
+    if ((t.u_off + t.e0) % kElemAlign)
+      throw VmpsError(VMPS_ERR_INTERNAL, "stage B: U task not aligned");
     for (int32_t j = t.term_begin; j < t.term_end; ++j)
-      if (H.comb_terms[j].t_off < 0 || H.comb_terms[j].t_off + t.e0 + t.n > H.T_size)
-        throw VmpsError(VMPS_ERR_INTERNAL, "stage B: T block out of range");
+      if (H.comb_terms[j].t_off < 0 || H.comb_terms[j].t_off + t.e0 + t.n > H.T_size ||
+          H.comb_terms[j].t_off % kElemAlign)
+        throw VmpsError(VMPS_ERR_INTERNAL, "stage B: T block out of range or not aligned");
   }
   for (const ChunkC& c : H.chunksC) {
     if (c.term_end - c.term_begin > kMaxTermsPerChunk || c.term_end < c.term_begin)
@@ -925,7 +928,7 @@ void build_plan(const TripleSet& S, HostPlan& H) {
       H.chunkA.push_back(ch);
       H.probA.push_back(pr);
       H.T_off.push_back(H.T_size);
-      H.T_size += L.rows * vi.cols;
+      H.T_size += align_elems(L.rows * vi.cols);
       st.flops_alg += 2.0 * L.rows * L.cols * vi.cols;
       st.flops_A += 2.0 * L.rows * L.cols * vi.cols;
     }
@@ -964,7 +967,7 @@ void build_plan(const TripleSet& S, HostPlan& H) {
     }
     cl.end = (int32_t)H.comb_t.size();
     uoff[u] = H.U_size;
-    H.U_size += cl.size;
+    H.U_size += align_elems(cl.size);
     H.comb.push_back(cl);
     st.flops_alg += 2.0 * vo.rows * R.rows * R.cols;
     st.flops_C += 2.0 * vo.rows * R.rows * R.cols;
@@ -1252,9 +1255,10 @@ void build_bf16_layout(const HostPlan& H, Bf16Layout& B, bool operands) {
     if (t.term_begin < 0 || t.term_end > (int32_t)B.comb_terms.size() || t.term_end < t.term_begin ||
         t.u_off + t.e0 + t.n > B.U_size)
       throw VmpsError(VMPS_ERR_INTERNAL, "bf16 stage B task out of range");
+    if ((t.u_off + t.e0) % kElemAlign) throw VmpsError(VMPS_ERR_INTERNAL, "bf16 stage B task not aligned");
     for (int32_t j = t.term_begin; j < t.term_end; ++j)
-      if (B.comb_terms[j].t_off + t.e0 + t.n > B.T_size)
-        throw VmpsError(VMPS_ERR_INTERNAL, "bf16 stage B term out of range");
+      if (B.comb_terms[j].t_off + t.e0 + t.n > B.T_size || B.comb_terms[j].t_off % kElemAlign)
+        throw VmpsError(VMPS_ERR_INTERNAL, "bf16 stage B term out of range or not aligned");
   }
   for (const auto* list : {&B.tilesA, &B.tilesA_big})
     for (const TileA16& a : *list)

@@ -217,6 +217,10 @@ struct TermC {             // c_j T_j with T_j an M x K block at t_off
 // multiple of 8 (so an MFMA lane's 8 consecutive k are one aligned 16-byte
 // load and the padding supplies the zeros past K).
 inline int64_t pad8(int64_t x) { return (x + 7) & ~int64_t(7); }
+// T and U blocks start at multiples of kElemAlign elements (16 bytes in FP32,
+// 32 in FP64), so the stage B kernel can move them 16 bytes per lane.
+constexpr int64_t kElemAlign = 4;
+inline int64_t align_elems(int64_t x) { return (x + kElemAlign - 1) / kElemAlign * kElemAlign; }
 struct TileA16 {           // T(M x Np, row-major) tile = Lt(M x Kp, row-major) . V(Kp x N, col-major)
   int64_t a_off, b_off, c_off;
   int32_t M, N, Kp, Np;
