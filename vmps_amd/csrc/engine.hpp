@@ -47,16 +47,20 @@ struct DevBuf {
 
 struct DevPlan;
 
-// Krylov workspace: basis Q ((kmax+1) x n), work vector w, fixed HxV input v,
-// Gram-Schmidt coefficients c (two passes of kmax + 1), the iteration index
-// in device memory, and the HIP graph of one iteration.
+// Krylov workspace: basis Q ((cap+1) x n), work vector w, fixed HxV input v,
+// Gram-Schmidt coefficients c (two passes of cap + 1), the iteration index
+// in device memory, the three scalars of every iteration (out3, 4 per
+// iteration), and the HIP graph of kIterBatch iterations.  cap is the
+// solve's iteration limit plus kIterBatch, room for the iterations a batch
+// runs past the limit or past convergence.
 struct Krylov {
+  static constexpr int kIterBatch = 4;
   DevBuf<double> Q, w, v, c, scal, out3;
   DevBuf<int> kdev;
   DevBuf<double> ew;      // coeff_dot weight of every vector element
   DevBuf<double> gpart;   // partial sums of the basis dot products, (kmax + 1) x ngrp
   DevBuf<double> npart;   // partial sums of ||w||^2
-  double* host3 = nullptr;  // pinned readback of out3
+  double* host3 = nullptr;  // pinned readback of out3 (4 * kIterBatch doubles)
   int ngrp = 0;           // element chunks of the basis dot products
   int64_t chunk = 0;
   int cap = 0;
@@ -68,7 +72,7 @@ struct Krylov {
   int graph_gen[3] = {-1, -1, -1};
   void ensure(DevPlan& D, int kmax);
   void enqueue_iteration(DevPlan& D, hipStream_t s);
-  hipGraphExec_t ensure_graph(DevPlan& D);  // the graph of D.prec
+  hipGraphExec_t ensure_graph(DevPlan& D);  // the graph of D.prec (kIterBatch iterations)
   void release_graph();
   Krylov() = default;
   Krylov(const Krylov&) = delete;

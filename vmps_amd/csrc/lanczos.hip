@@ -83,6 +83,7 @@ __global__ __launch_bounds__(256) void k_multidot_rows(const int* __restrict__ k
   const int b = blockIdx.x;
   const int64_t e0 = (int64_t)b * chunk, e1 = min(n, e0 + chunk);
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
   for (int64_t e = e0 + threadIdx.x; e < e1; e += 256) {
     const double we = w[e] * ew[e];
 #pragma unroll
@@ -120,7 +121,7 @@ __global__ __launch_bounds__(256) void k_multidot_rows_final(const int* __restri
 }
 
 // w -= sum_{i<=k} c[i] Q_i; with norm_part, also the workgroup's share of the
-// weighted ||w||^2 of the result (reduced by k_sum_final)
+// weighted ||w||^2 of the result (summed by k_normalize_next and k_lanczos_scalars_next)
 __global__ __launch_bounds__(256) void k_sub_combination_dk(const double* __restrict__ Q, int64_t n,
                                                             const double* __restrict__ c, const int* __restrict__ kdev,
                                                             double* __restrict__ w, const double* __restrict__ ew,
@@ -147,8 +148,50 @@ __global__ __launch_bounds__(256) void k_sub_combination_dk(const double* __rest
   if (threadIdx.x == 0) norm_part[blockIdx.x] = red[0];
 }
 
-// out[0] = sum of part[0 .. np-1] (one workgroup, fixed order)
-__global__ __launch_bounds__(256) void k_sum_final(const double* __restrict__ part, int np, double* __restrict__ out) {
+// k_multidot_rows_final and k_sub_combination_dk in one kernel, for short
+// partial lists (nb <= kFuseMaxGroups, k < kFuseMaxRows): every workgroup sums
+// the nb partials of each row c[i], i <= k, into LDS (the same sums, in the
+// same order, in every workgroup); workgroup 0 also stores them in c for the
+// host scalars.  Then w -= sum_i c[i] Q_i as in k_sub_combination_dk.
+constexpr int kFuseMaxGroups = 64;
+constexpr int kFuseMaxRows = 512;
+__global__ __launch_bounds__(256) void k_sub_combination_fused(const double* __restrict__ Q, int64_t n,
+                                                               const double* __restrict__ partial, int nb,
+                                                               const int* __restrict__ kdev, double* __restrict__ c,
+                                                               double* __restrict__ w, const double* __restrict__ ew,
+                                                               double* __restrict__ norm_part) {
+  __shared__ double cs[kFuseMaxRows];
+  const int k = *kdev;
+  for (int i = threadIdx.x; i <= k; i += 256) {
+    double acc = 0.0;
+    for (int b = 0; b < nb; ++b) acc += partial[(int64_t)i * nb + b];
+    cs[i] = acc;
+    if (blockIdx.x == 0) c[i] = acc;
+  }
+  __syncthreads();
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  double nrm = 0.0;
+  for (; e < n; e += stride) {
+    double s = 0.0;
+    for (int i = 0; i <= k; ++i) s += cs[i] * Q[(int64_t)i * n + e];
+    const double y = w[e] - s;
+    w[e] = y;
+    nrm += ew[e] * y * y;
+  }
+  if (!norm_part) return;
+  __shared__ double red[256];
+  red[threadIdx.x] = nrm;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) norm_part[blockIdx.x] = red[0];
+}
+
+// sum of part[0 .. np-1] in one workgroup, fixed order; every thread returns it
+__device__ inline double block_sum(const double* __restrict__ part, int np) {
   __shared__ double red[256];
   double acc = 0.0;
   for (int b = threadIdx.x; b < np; b += 256) acc += part[b];
@@ -158,31 +201,37 @@ __global__ __launch_bounds__(256) void k_sum_final(const double* __restrict__ pa
     if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
     __syncthreads();
   }
-  if (threadIdx.x == 0) out[0] = red[0];
+  return red[0];
 }
 
 // the three numbers the host needs after iteration k (<q_k, H q_k> in two
-// Gram-Schmidt parts and ||w||^2), then k += 1.  Runs after every other
-// kernel of the iteration has read k.
-__global__ void k_lanczos_scalars_next(int* __restrict__ kdev, const double* __restrict__ c1,
-                                       const double* __restrict__ c2, const double* __restrict__ nrm2,
-                                       double* __restrict__ out) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    const int k = *kdev;
-    out[0] = c1[k];
-    out[1] = c2[k];
-    out[2] = nrm2[0];
+// Gram-Schmidt parts and ||w||^2) into out[4k ..], then k += 1.  Runs after
+// every other kernel of the iteration has read k.
+// ||w||^2 is the sum of the np norm partials (block_sum, the same order as in
+// k_normalize_next).  One workgroup of 256 threads.
+__global__ __launch_bounds__(256) void k_lanczos_scalars_next(int* __restrict__ kdev, const double* __restrict__ c1,
+                                                              const double* __restrict__ c2,
+                                                              const double* __restrict__ npart, int np,
+                                                              double* __restrict__ out) {
+  const int k = *kdev;
+  const double nrm2 = block_sum(npart, np);
+  __syncthreads();  // every thread has read k before it changes
+  if (threadIdx.x == 0) {
+    out[4 * k] = c1[k];
+    out[4 * k + 1] = c2[k];
+    out[4 * k + 2] = nrm2;
     *kdev = k + 1;
   }
 }
 
 // q_{k+1} = w / ||w|| into the basis and into the fixed HxV input vector v
-// (zero for a zero w: the iteration then ends on the breakdown test)
-__global__ void k_normalize_next(const int* __restrict__ kdev, const double* __restrict__ nrm2,
-                                 const double* __restrict__ w, double* __restrict__ Q, int64_t n,
-                                 double* __restrict__ v) {
+// (zero for a zero w: the iteration then ends on the breakdown test); every
+// workgroup sums the np norm partials itself (block_sum)
+__global__ __launch_bounds__(256) void k_normalize_next(const int* __restrict__ kdev, const double* __restrict__ npart,
+                                                        int np, const double* __restrict__ w, double* __restrict__ Q,
+                                                        int64_t n, double* __restrict__ v) {
   const int k = *kdev;
-  const double nv = nrm2[0];
+  const double nv = block_sum(npart, np);
   const double f = nv > 0.0 ? 1.0 / sqrt(nv) : 0.0;
   double* __restrict__ q = Q + (int64_t)(k + 1) * n;
   int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -300,18 +349,25 @@ void Krylov::enqueue_iteration(DevPlan& D, hipStream_t s) {
   const int64_t n = D.H.vsize;
   const unsigned gy = (unsigned)((cap + 3) / 4);
   const unsigned gs = vgrid(n);
+  const bool fused = ngrp <= kFuseMaxGroups && cap < kFuseMaxRows;
   auto pass = [&](double* c_out, double* norm_part) {
     hipLaunchKernelGGL(k_multidot_rows, dim3(ngrp, gy), dim3(256), 0, s, kdev.p, Q.p, n, w.p, ew.p, chunk, gpart.p,
                        ngrp);
-    hipLaunchKernelGGL(k_multidot_rows_final, dim3(cap), dim3(256), 0, s, kdev.p, gpart.p, ngrp, c_out);
-    hipLaunchKernelGGL(k_sub_combination_dk, dim3(gs), dim3(256), 0, s, Q.p, n, c_out, kdev.p, w.p, ew.p, norm_part);
+    if (fused) {
+      hipLaunchKernelGGL(k_sub_combination_fused, dim3(gs), dim3(256), 0, s, Q.p, n, gpart.p, ngrp, kdev.p, c_out,
+                         w.p, ew.p, norm_part);
+    } else {
+      hipLaunchKernelGGL(k_multidot_rows_final, dim3(cap), dim3(256), 0, s, kdev.p, gpart.p, ngrp, c_out);
+      hipLaunchKernelGGL(k_sub_combination_dk, dim3(gs), dim3(256), 0, s, Q.p, n, c_out, kdev.p, w.p, ew.p,
+                         norm_part);
+    }
   };
   D.hxv(v.p, w.p, s);
   pass(c.p, nullptr);
   pass(c.p + cap, npart.p);
-  hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(256), 0, s, npart.p, (int)gs, scal.p);
-  hipLaunchKernelGGL(k_normalize_next, dim3(gs), dim3(256), 0, s, kdev.p, scal.p, w.p, Q.p, n, v.p);
-  hipLaunchKernelGGL(k_lanczos_scalars_next, dim3(1), dim3(64), 0, s, kdev.p, c.p, c.p + cap, scal.p, out3.p);
+  hipLaunchKernelGGL(k_normalize_next, dim3(gs), dim3(256), 0, s, kdev.p, npart.p, (int)gs, w.p, Q.p, n, v.p);
+  hipLaunchKernelGGL(k_lanczos_scalars_next, dim3(1), dim3(256), 0, s, kdev.p, c.p, c.p + cap, npart.p, (int)gs,
+                     out3.p);
   VMPS_HIP_CHECK(hipGetLastError());
 }
 
@@ -336,7 +392,7 @@ hipGraphExec_t Krylov::ensure_graph(DevPlan& D) {
   hipStream_t s = D.stream;
   VMPS_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
   try {
-    enqueue_iteration(D, s);
+    for (int i = 0; i < kIterBatch; ++i) enqueue_iteration(D, s);
   } catch (...) {
     hipGraph_t g = nullptr;
     (void)hipStreamEndCapture(s, &g);
@@ -373,35 +429,44 @@ LanczosResult lanczos_solve(DevPlan& D, double* d_v, const LanczosOpts& o, vmps_
   double E = 0.0, Eprev = 0.0;
   int k = 0;
   bool converged = false;
-  for (k = 0; k < kmax; ++k) {
-    if (iteration)
+  // The graph runs kIterBatch iterations per launch and the host reads their
+  // scalars with one synchronisation; it then tests them one by one, exactly
+  // as after single iterations.  Iterations a batch runs past convergence (or
+  // past kmax) only extend the basis beyond the part the Ritz vector uses.
+  while (k < kmax && !converged) {
+    int nb = 1;
+    if (iteration) {
       VMPS_HIP_CHECK(hipGraphLaunch(iteration, s));
-    else
+      nb = Krylov::kIterBatch;
+    } else {
       K.enqueue_iteration(D, s);
-    ++res.hxv_calls;
-    VMPS_HIP_CHECK(hipMemcpyAsync(K.host3, K.out3.p, 3 * sizeof(double), hipMemcpyDeviceToHost, s));
+    }
+    res.hxv_calls += nb;
+    VMPS_HIP_CHECK(hipMemcpyAsync(K.host3, K.out3.p + 4 * (size_t)k, 4 * nb * sizeof(double), hipMemcpyDeviceToHost,
+                                  s));
     VMPS_HIP_CHECK(hipStreamSynchronize(s));
-    const double a_k = K.host3[0] + K.host3[1];
-    const double b_k = std::sqrt(std::max(0.0, K.host3[2]));
-    alpha.push_back(a_k);
-    beta.push_back(b_k);
-    Eprev = E;
-    tridiag_lowest(alpha, beta, k + 1, E, y);
-    // residual estimate of the Ritz pair: beta_k |y_k|
-    const double rest = b_k * std::fabs(y[k]);
-    res.iterations = k + 1;
-    // convergence after min_iter iterations; a breakdown (the Krylov space is
-    // invariant, the Ritz pair exact) ends the iteration at any point, since
-    // the next basis vector would be the normalised zero vector
-    const bool breakdown = b_k < 1e-14 * std::max(1.0, std::fabs(a_k));
-    if (breakdown ||
-        (k + 1 >= o.min_iter && k > 0 && std::fabs(E - Eprev) < o.tol_eigval && rest < o.tol_state)) {
-      converged = true;
+    for (int j = 0; j < nb && k < kmax; ++j) {
+      const double a_k = K.host3[4 * j] + K.host3[4 * j + 1];
+      const double b_k = std::sqrt(std::max(0.0, K.host3[4 * j + 2]));
+      alpha.push_back(a_k);
+      beta.push_back(b_k);
+      Eprev = E;
+      tridiag_lowest(alpha, beta, k + 1, E, y);
+      // residual estimate of the Ritz pair: beta_k |y_k|
+      const double rest = b_k * std::fabs(y[k]);
+      res.iterations = k + 1;
       ++k;
-      break;
+      // convergence after min_iter iterations; a breakdown (the Krylov space is
+      // invariant, the Ritz pair exact) ends the iteration at any point, since
+      // the next basis vector would be the normalised zero vector
+      const bool breakdown = b_k < 1e-14 * std::max(1.0, std::fabs(a_k));
+      if (breakdown || (k >= o.min_iter && k > 1 && std::fabs(E - Eprev) < o.tol_eigval && rest < o.tol_state)) {
+        converged = true;
+        break;
+      }
     }
   }
-  if (!converged) k = res.iterations;
+  k = res.iterations;
   // Ritz vector psi = Q y (k basis vectors), normalised, then the true residual
   VMPS_HIP_CHECK(hipMemcpyAsync(K.c.p, y.data(), sizeof(double) * k, hipMemcpyHostToDevice, s));
   hipLaunchKernelGGL(k_combination, dim3(vgrid(n)), dim3(256), 0, s, K.Q.p, n, K.c.p, k, K.w.p);
@@ -430,27 +495,28 @@ LanczosResult lanczos_solve(DevPlan& D, double* d_v, const LanczosOpts& o, vmps_
 
 void Krylov::ensure(DevPlan& D, int kmax) {
   const int64_t n = D.H.vsize;
-  if (cap >= kmax && nvec == n) return;
+  const int need = kmax + kIterBatch;
+  if (cap >= need && nvec == n) return;
   release_graph();
-  Q.alloc((size_t)(kmax + 1) * n);
+  Q.alloc((size_t)(need + 1) * n);
   w.alloc(n);
   v.alloc(n);
-  c.alloc(2 * (size_t)kmax + 2);
+  c.alloc(2 * (size_t)need + 2);
   scal.alloc(8);
-  out3.alloc(4);
+  out3.alloc(4 * (size_t)need);
   kdev.alloc(1);
-  if (!host3) VMPS_HIP_CHECK(hipHostMalloc((void**)&host3, 4 * sizeof(double), hipHostMallocDefault));
+  if (!host3) VMPS_HIP_CHECK(hipHostMalloc((void**)&host3, 4 * kIterBatch * sizeof(double), hipHostMallocDefault));
   // per-element coeff_dot weights and element chunks of the row dot products
   std::vector<double> hw((size_t)n, 1.0);
   for (size_t sg = 0; sg < D.H.seg_off.size(); ++sg)
     for (int32_t e = 0; e < D.H.seg_len[sg]; ++e) hw[(size_t)(D.H.seg_off[sg] + e)] = D.H.seg_w[sg];
   ew.upload(hw, D.stream);
-  ngrp = (int)std::max<int64_t>(1, std::min<int64_t>(1024, (n + 4095) / 4096));
+  ngrp = (int)std::max<int64_t>(1, std::min<int64_t>(1024, (n + 1023) / 1024));  // 4 elements per thread
   chunk = (n + ngrp - 1) / ngrp;
   npart.alloc(vgrid(n));
-  gpart.alloc((size_t)std::max(1, ngrp) * (kmax + 1));
+  gpart.alloc((size_t)std::max(1, ngrp) * (need + 1));
   VMPS_HIP_CHECK(hipStreamSynchronize(D.stream));
-  cap = kmax;
+  cap = need;
   nvec = n;
 }
 
