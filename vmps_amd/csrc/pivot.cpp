@@ -8,6 +8,7 @@
 #include <cstring>
 #include <stdexcept>
 #include <unordered_map>
+#include <unordered_set>
 
 #include "errors.hpp"
 
@@ -294,6 +295,17 @@ void precalc_blockStructure(Pivot& P, PivotTerm& T) {
   using Key = std::array<size_t, 2>;
   std::map<Key, std::pair<std::vector<std::array<size_t, 12>>, std::vector<double>>> lookup;
   std::vector<AawwaaHit> ixs;
+  // Necessary conditions of an AAWWAA match, tested before the call: Vin[s1s3]
+  // holds a block with in = L.in, Vin[s2s4] one with in = L.out, and W12 one
+  // with in = L.mid.  They only skip qL that would match nothing, so the
+  // tables and their order are unchanged.
+  // The L sectors are bucketed by their mid quantum number, so a W12 block
+  // list selects its candidates directly (visited in ascending qL order).
+  std::vector<std::unordered_set<Qn, QnHash>> vin(P.V.in.size());
+  for (size_t s = 0; s < P.V.in.size(); ++s) vin[s].insert(P.V.in[s].begin(), P.V.in[s].end());
+  std::unordered_map<Qn, std::vector<size_t>, QnHash> by_mid;
+  for (size_t qL = 0; qL < T.L.dim(); ++qL) by_mid[T.L.mid(qL)].push_back(qL);
+  std::vector<size_t> cand;
 
   const auto& qop12 = T.qop12;
   const auto& qop34 = T.qop34;
@@ -318,7 +330,16 @@ void precalc_blockStructure(Pivot& P, PivotTerm& T) {
                                                             qop34[k34], qOp, P.qloc12[s1], P.qloc34[s3], qmerge13)
                                      : 1.0;
                     if (std::abs(cgc9) < kCgcEps) continue;
-                    for (size_t qL = 0; qL < T.L.dim(); ++qL) {
+                    const WBiped& w12 = T.W12[s1][s2][k12];
+                    cand.clear();
+                    for (size_t b = 0; b < w12.in.size(); ++b) {
+                      if (std::find(w12.in.begin(), w12.in.begin() + b, w12.in[b]) != w12.in.begin() + b) continue;
+                      auto it = by_mid.find(w12.in[b]);
+                      if (it != by_mid.end()) cand.insert(cand.end(), it->second.begin(), it->second.end());
+                    }
+                    std::sort(cand.begin(), cand.end());
+                    for (size_t qL : cand) {
+                      if (!vin[s1s3].count(T.L.in(qL)) || !vin[s2s4].count(T.L.out(qL))) continue;
                       bool found = AAWWAA(sym, T.L.in(qL), T.L.out(qL), T.L.mid(qL), qop12[k12], qop34[k34],
                                           qmerge13, qmerge24, P.V, (size_t)s1s3, (size_t)s2s4, T.W12[s1][s2][k12],
                                           T.W34[s3][s4][k34], ixs);

@@ -5,38 +5,35 @@
 #include <cmath>
 #include <cstdlib>
 #include <mutex>
+#include <stdexcept>
 
 namespace vmps {
 
-std::vector<Qn> Sym::reduce(const Qn& a, const Qn& b) const {
+QnList Sym::reduce(const Qn& a, const Qn& b) const {
   // Per-component irreps, then the Cartesian product with the first
   // component outermost (S1xS2.h:195-210).
-  std::vector<int> comp[kMaxNq];
+  int lo[kMaxNq] = {0, 0}, cnt[kMaxNq] = {1, 1}, step[kMaxNq] = {0, 0};
   for (int i = 0; i < nq; ++i) {
     if (kind[i] == kU1) {
-      comp[i].push_back(a[i] + b[i]);
+      lo[i] = a[i] + b[i];
+      cnt[i] = 1;
     } else {
-      int qmin = std::abs(a[i] - b[i]) + 1;
-      int qmax = std::abs(a[i] + b[i]) - 1;
-      for (int q = qmin; q <= qmax; q += 2) comp[i].push_back(q);
+      const int qmin = std::abs(a[i] - b[i]) + 1, qmax = std::abs(a[i] + b[i]) - 1;
+      lo[i] = qmin;
+      cnt[i] = qmax >= qmin ? (qmax - qmin) / 2 + 1 : 0;
+      step[i] = 2;
     }
   }
-  std::vector<Qn> out;
-  if (nq == 1) {
-    for (int x : comp[0]) {
+  QnList out;
+  const int n1 = nq > 1 ? cnt[1] : 1;
+  if ((int64_t)cnt[0] * n1 > QnList::kCap) throw std::length_error("too many irreps in one tensor product");
+  for (int x = 0; x < cnt[0]; ++x)
+    for (int y = 0; y < n1; ++y) {
       Qn q;
-      q[0] = x;
-      out.push_back(q);
+      q[0] = lo[0] + step[0] * x;
+      if (nq > 1) q[1] = lo[1] + step[1] * y;
+      out.q[out.n++] = q;
     }
-  } else {
-    for (int x : comp[0])
-      for (int y : comp[1]) {
-        Qn q;
-        q[0] = x;
-        q[1] = y;
-        out.push_back(q);
-      }
-  }
   return out;
 }
 

@@ -39,6 +39,19 @@ struct QnHash {
   }
 };
 
+// Irreps of one tensor product, held inline (reduce is called in the inner
+// loops of the block-structure searches, where heap vectors dominated).
+struct QnList {
+  static constexpr int kCap = 128;
+  Qn q[kCap];
+  int n = 0;
+  const Qn* begin() const { return q; }
+  const Qn* end() const { return q + n; }
+  size_t size() const { return (size_t)n; }
+  bool empty() const { return n == 0; }
+  const Qn& operator[](size_t i) const { return q[i]; }
+};
+
 struct Sym {
   int nq = 1;
   int kind[kMaxNq] = {kU1, kU1};
@@ -60,7 +73,7 @@ struct Sym {
     return r;
   }
   // reduceSilent(ql, qr): all irreps of ql (x) qr, first component outermost.
-  std::vector<Qn> reduce(const Qn& a, const Qn& b) const;
+  QnList reduce(const Qn& a, const Qn& b) const;
   // validate(qarray3{a,b,c}): c in a (x) b.
   bool validate3(const Qn& a, const Qn& b, const Qn& c) const;
   // degeneracy of the irrep (product of the SU(2) dimensions).
