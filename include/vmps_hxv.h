@@ -124,8 +124,9 @@ vmps_status vmps_pivot2_vsize(const vmps_pivot2* p, int64_t* size);
  * Tripod, W slot or PivotVector layout at once (same checks, item by item;
  * the first failing item's status is returned):
  *  - add_env_sectors: nsec sectors; qin/qout/qmid nsec*nq ints, adim[nsec],
- *    rows/cols sum(adim) entries in sector order, data the non-empty blocks
- *    back to back in the same order;
+ *    rows/cols/blocks sum(adim) entries in sector order, blocks[i] the
+ *    column-major data of block i (ignored when rows*cols == 0), so a host
+ *    holding one allocation per block (Eigen MatrixXd) passes them as they are;
  *  - add_W_blocks: nblk blocks of slot `which`; idx = nblk x {s1, s2, k},
  *    qin/qout nblk*nq, dims = nblk x {rows, cols}, nnz[nblk], and the
  *    triplets of all blocks back to back in row/col/val;
@@ -133,7 +134,7 @@ vmps_status vmps_pivot2_vsize(const vmps_pivot2* p, int64_t* size);
  *    receives each block's element offset. */
 vmps_status vmps_pivot2_add_env_sectors(vmps_pivot2* p, int which, int nsec, const int* qin, const int* qout,
                                         const int* qmid, const int* adim, const int64_t* rows, const int64_t* cols,
-                                        const double* data);
+                                        const double* const* blocks);
 vmps_status vmps_pivot2_add_W_blocks(vmps_pivot2* p, int which, int nblk, const int* idx, const int* qin,
                                      const int* qout, const int* dims, const int64_t* nnz, const int* row,
                                      const int* col, const double* val);
@@ -289,12 +290,13 @@ vmps_status vmps_env_add_A(vmps_env* e, int bra, int s, const int* qin, const in
 /* sparse block of W[s1][s2][k] */
 vmps_status vmps_env_add_W(vmps_env* e, int s1, int s2, int k, const int* qin, const int* qout, int rows, int cols,
                            int64_t nnz, const int* row, const int* col, const double* val);
-/* bulk forms of the three calls above, packed as the vmps_pivot2_add_*
- * bulk forms (A blocks: s[nblk], rows/cols[nblk], data back to back) */
+/* bulk forms of the three calls above, laid out as the vmps_pivot2_add_*
+ * bulk forms (A blocks: s[nblk], rows/cols[nblk], blocks[nblk] pointers) */
 vmps_status vmps_env_add_old_sectors(vmps_env* e, int nsec, const int* qin, const int* qout, const int* qmid,
-                                     const int* adim, const int64_t* rows, const int64_t* cols, const double* data);
+                                     const int* adim, const int64_t* rows, const int64_t* cols,
+                                     const double* const* blocks);
 vmps_status vmps_env_add_A_blocks(vmps_env* e, int bra, int nblk, const int* s, const int* qin, const int* qout,
-                                  const int64_t* rows, const int64_t* cols, const double* data);
+                                  const int64_t* rows, const int64_t* cols, const double* const* blocks);
 vmps_status vmps_env_add_W_blocks(vmps_env* e, int nblk, const int* idx, const int* qin, const int* qout,
                                   const int* dims, const int64_t* nnz, const int* row, const int* col,
                                   const double* val);

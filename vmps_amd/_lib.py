@@ -91,7 +91,8 @@ SIGNATURES = {
     "vmps_pivot2_add_W": (c_int, [c_void, c_int, c_int, c_int, c_int, P_int, P_int, c_int, c_int, c_i64,
                                   P_int, P_int, P_dbl]),
     "vmps_pivot2_add_V_block": (c_int, [c_void, c_int, P_int, P_int, c_i64, c_i64, P_i64]),
-    "vmps_pivot2_add_env_sectors": (c_int, [c_void, c_int, c_int, P_int, P_int, P_int, P_int, P_i64, P_i64, P_dbl]),
+    "vmps_pivot2_add_env_sectors": (c_int, [c_void, c_int, c_int, P_int, P_int, P_int, P_int, P_i64, P_i64,
+                                            ctypes.POINTER(c_void)]),
     "vmps_pivot2_add_W_blocks": (c_int, [c_void, c_int, c_int, P_int, P_int, P_int, P_int, P_i64, P_int, P_int,
                                          P_dbl]),
     "vmps_pivot2_add_V_blocks": (c_int, [c_void, c_int, P_int, P_int, P_int, P_i64, P_i64, P_i64]),
@@ -119,8 +120,9 @@ SIGNATURES = {
     "vmps_env_add_old": (c_int, [c_void, P_int, P_int, P_int, c_int, P_i64, P_i64, P_dbl]),
     "vmps_env_add_A": (c_int, [c_void, c_int, c_int, P_int, P_int, c_i64, c_i64, P_dbl]),
     "vmps_env_add_W": (c_int, [c_void, c_int, c_int, c_int, P_int, P_int, c_int, c_int, c_i64, P_int, P_int, P_dbl]),
-    "vmps_env_add_old_sectors": (c_int, [c_void, c_int, P_int, P_int, P_int, P_int, P_i64, P_i64, P_dbl]),
-    "vmps_env_add_A_blocks": (c_int, [c_void, c_int, c_int, P_int, P_int, P_int, P_i64, P_i64, P_dbl]),
+    "vmps_env_add_old_sectors": (c_int, [c_void, c_int, P_int, P_int, P_int, P_int, P_i64, P_i64,
+                                         ctypes.POINTER(c_void)]),
+    "vmps_env_add_A_blocks": (c_int, [c_void, c_int, c_int, P_int, P_int, P_int, P_i64, P_i64, ctypes.POINTER(c_void)]),
     "vmps_env_add_W_blocks": (c_int, [c_void, c_int, P_int, P_int, P_int, P_int, P_i64, P_int, P_int, P_dbl]),
     "vmps_env_contract": (c_int, [c_void, c_int]),
     "vmps_env_result_size": (c_int, [c_void, P_i64, P_i64, P_i64]),

@@ -91,6 +91,36 @@ void init_vector_layout(vmps_pivot2* p, int ns) {
   p->P.V.block.resize(ns);
   p->P.V.dict.resize(ns);
 }
+
+// one Tripod sector from per-block pointers (NULL or ignored for absent
+// blocks, rows * cols == 0): Tripod::push_back, tensors/Multipede.h:77-95
+void add_env_sector(vmps_pivot2* p, int which, const int* qin, const int* qout, const int* qmid, int adim,
+                    const int64_t* rows, const int64_t* cols, const double* const* ptrs) {
+  const int nq = p->P.sym.nq;
+  vmps::Tripod& E = which == 0 ? p->P.term().L : p->P.term().R;
+  vmps::Key3 key{read_qn(qin, nq), read_qn(qout, nq), read_qn(qmid, nq)};
+  require(!E.dict.count(key), VMPS_ERR_LAYOUT, "duplicate environment sector");
+  int64_t total = 0;
+  for (int a = 0; a < adim; ++a) {
+    require(rows[a] >= 0 && cols[a] >= 0, VMPS_ERR_ARG, "negative block dimension");
+    require(rows[a] * cols[a] == 0 || ptrs[a], VMPS_ERR_ARG, "NULL block data");
+    total += rows[a] * cols[a];
+  }
+  std::vector<vmps::DenseBlock> blocks(adim);
+  for (int a = 0; a < adim; ++a) {
+    vmps::DenseBlock b;
+    b.rows = rows[a];
+    b.cols = cols[a];
+    if (b.rows * b.cols > 0) {
+      b.off = (int64_t)E.data.size();
+      E.data.insert(E.data.end(), ptrs[a], ptrs[a] + b.rows * b.cols);
+    }
+    blocks[a] = b;
+  }
+  E.index.push_back({key.a, key.b, key.c});
+  E.block.push_back(std::move(blocks));
+  E.dict[key] = E.index.size() - 1;
+}
 }  // namespace
 
 extern "C" {
@@ -218,32 +248,17 @@ vmps_status vmps_pivot2_add_env(vmps_pivot2* p, int which, const int* qin, const
   require(which == 0 || which == 1, VMPS_ERR_ARG, "which must be 0 (L) or 1 (R)");
   require(adim >= 0, VMPS_ERR_ARG, "adim < 0");
   require(adim == 0 || (rows && cols), VMPS_ERR_ARG, "NULL rows/cols");
-  const int nq = p->P.sym.nq;
-  vmps::Tripod& E = which == 0 ? p->P.term().L : p->P.term().R;
-  vmps::Key3 key{read_qn(qin, nq), read_qn(qout, nq), read_qn(qmid, nq)};
-  require(!E.dict.count(key), VMPS_ERR_LAYOUT, "duplicate environment sector");
-  int64_t total = 0;
-  for (int a = 0; a < adim; ++a) {
-    require(rows[a] >= 0 && cols[a] >= 0, VMPS_ERR_ARG, "negative block dimension");
-    total += rows[a] * cols[a];
-  }
-  require(total == 0 || data, VMPS_ERR_ARG, "NULL data");
-  std::vector<vmps::DenseBlock> blocks(adim);
+  std::vector<const double*> ptrs((size_t)adim, nullptr);
   int64_t src = 0;
   for (int a = 0; a < adim; ++a) {
-    vmps::DenseBlock b;
-    b.rows = rows[a];
-    b.cols = cols[a];
-    if (b.rows * b.cols > 0) {
-      b.off = (int64_t)E.data.size();
-      E.data.insert(E.data.end(), data + src, data + src + b.rows * b.cols);
-      src += b.rows * b.cols;
+    require(rows[a] >= 0 && cols[a] >= 0, VMPS_ERR_ARG, "negative block dimension");
+    if (rows[a] * cols[a] > 0) {
+      require(data != nullptr, VMPS_ERR_ARG, "NULL data");
+      ptrs[(size_t)a] = data + src;
+      src += rows[a] * cols[a];
     }
-    blocks[a] = b;
   }
-  E.index.push_back({key.a, key.b, key.c});
-  E.block.push_back(std::move(blocks));
-  E.dict[key] = E.index.size() - 1;
+  add_env_sector(p, which, qin, qout, qmid, adim, rows, cols, ptrs.data());
   VMPS_API_END
 }
 
@@ -310,18 +325,25 @@ vmps_status vmps_pivot2_add_V_block(vmps_pivot2* p, int s, const int* qin, const
 // and the first failing item's status is returned.
 vmps_status vmps_pivot2_add_env_sectors(vmps_pivot2* p, int which, int nsec, const int* qin, const int* qout,
                                         const int* qmid, const int* adim, const int64_t* rows, const int64_t* cols,
-                                        const double* data) {
+                                        const double* const* blocks) {
   VMPS_API_BEGIN
   require(p && nsec >= 0 && (nsec == 0 || (qin && qout && qmid && adim)), VMPS_ERR_ARG, "bad argument");
+  require(which == 0 || which == 1, VMPS_ERR_ARG, "which must be 0 (L) or 1 (R)");
   const int nq = p->P.sym.nq;
-  int64_t slot = 0, pos = 0;
+  {  // one reservation for all sectors (per-sector growth would copy the data again each time)
+    int64_t total = 0, nslot = 0;
+    for (int q = 0; q < nsec; ++q) nslot += std::max(0, adim[q]);
+    if (rows && cols)
+      for (int64_t i = 0; i < nslot; ++i) total += std::max<int64_t>(0, rows[i]) * std::max<int64_t>(0, cols[i]);
+    vmps::Tripod& E = which == 0 ? p->P.term().L : p->P.term().R;
+    E.data.reserve(E.data.size() + (size_t)total);
+  }
+  int64_t slot = 0;
   for (int q = 0; q < nsec; ++q) {
     require(adim[q] >= 0, VMPS_ERR_ARG, "adim < 0");
-    const vmps_status st = vmps_pivot2_add_env(p, which, qin + q * nq, qout + q * nq, qmid + q * nq, adim[q],
-                                               rows ? rows + slot : nullptr, cols ? cols + slot : nullptr,
-                                               data ? data + pos : nullptr);
-    if (st != VMPS_OK) return st;
-    for (int a = 0; a < adim[q]; ++a) pos += rows[slot + a] * cols[slot + a];
+    require(adim[q] == 0 || (rows && cols && blocks), VMPS_ERR_ARG, "NULL rows/cols/blocks");
+    add_env_sector(p, which, qin + q * nq, qout + q * nq, qmid + q * nq, adim[q], rows + slot, cols + slot,
+                   blocks + slot);
     slot += adim[q];
   }
   VMPS_API_END
@@ -481,7 +503,7 @@ vmps_status vmps_plan_create_batch(const vmps_pivot2* const* pivots, int n, vmps
   }
   if (offsets) offsets[n] = S.ysize;
   auto plan = std::make_unique<vmps_plan>();
-  plan->D.create_triples(S, prec);
+  plan->D.create_triples(std::move(S), prec);
   *out = plan.release();
   VMPS_API_END
 }

@@ -46,6 +46,33 @@ vmps::Qn qn(const int* q, int nq) {
 void require(bool cond, int status, const char* msg) {
   if (!cond) throw VmpsError(status, msg);
 }
+// one sector of Lold / Rold from per-block pointers (absent blocks: rows * cols == 0)
+void add_old_sector(vmps_env* e, const int* qin, const int* qout, const int* qmid, int adim, const int64_t* rows,
+                    const int64_t* cols, const double* const* ptrs) {
+  const int nq = e->P.sym.nq;
+  vmps::Tripod& E = e->P.old;
+  vmps::Key3 key{qn(qin, nq), qn(qout, nq), qn(qmid, nq)};
+  require(!E.dict.count(key), VMPS_ERR_LAYOUT, "duplicate environment sector");
+  int64_t total = 0;
+  for (int a = 0; a < adim; ++a) {
+    require(rows[a] >= 0 && cols[a] >= 0, VMPS_ERR_ARG, "negative block dimension");
+    require(rows[a] * cols[a] == 0 || ptrs[a], VMPS_ERR_ARG, "NULL block data");
+    total += rows[a] * cols[a];
+  }
+  std::vector<vmps::DenseBlock> blocks(adim);
+  for (int a = 0; a < adim; ++a) {
+    blocks[a].rows = rows[a];
+    blocks[a].cols = cols[a];
+    if (rows[a] * cols[a] > 0) {
+      blocks[a].off = (int64_t)E.data.size();
+      E.data.insert(E.data.end(), ptrs[a], ptrs[a] + rows[a] * cols[a]);
+    }
+  }
+  E.index.push_back({key.a, key.b, key.c});
+  E.block.push_back(std::move(blocks));
+  E.dict[key] = E.index.size() - 1;
+  e->done = false;
+}
 }  // namespace
 
 extern "C" {
@@ -80,27 +107,17 @@ vmps_status vmps_env_add_old(vmps_env* e, const int* qin, const int* qout, const
                              const int64_t* rows, const int64_t* cols, const double* data) {
   VMPS_API_BEGIN
   require(e && qin && qout && qmid && adim >= 0 && (adim == 0 || (rows && cols)), VMPS_ERR_ARG, "bad argument");
-  const int nq = e->P.sym.nq;
-  vmps::Tripod& E = e->P.old;
-  vmps::Key3 key{qn(qin, nq), qn(qout, nq), qn(qmid, nq)};
-  require(!E.dict.count(key), VMPS_ERR_LAYOUT, "duplicate environment sector");
-  std::vector<vmps::DenseBlock> blocks(adim);
+  std::vector<const double*> ptrs((size_t)adim, nullptr);
   int64_t src = 0;
   for (int a = 0; a < adim; ++a) {
     require(rows[a] >= 0 && cols[a] >= 0, VMPS_ERR_ARG, "negative block dimension");
-    blocks[a].rows = rows[a];
-    blocks[a].cols = cols[a];
     if (rows[a] * cols[a] > 0) {
       require(data != nullptr, VMPS_ERR_ARG, "NULL data");
-      blocks[a].off = (int64_t)E.data.size();
-      E.data.insert(E.data.end(), data + src, data + src + rows[a] * cols[a]);
+      ptrs[(size_t)a] = data + src;
       src += rows[a] * cols[a];
     }
   }
-  E.index.push_back({key.a, key.b, key.c});
-  E.block.push_back(std::move(blocks));
-  E.dict[key] = E.index.size() - 1;
-  e->done = false;
+  add_old_sector(e, qin, qout, qmid, adim, rows, cols, ptrs.data());
   VMPS_API_END
 }
 
@@ -164,34 +181,37 @@ vmps_status vmps_env_add_W(vmps_env* e, int s1, int s2, int k, const int* qin, c
 // Whole-Tripod / whole-list forms of the three calls above (one call instead
 // of one per sector or block); each item goes through the single-item call.
 vmps_status vmps_env_add_old_sectors(vmps_env* e, int nsec, const int* qin, const int* qout, const int* qmid,
-                                     const int* adim, const int64_t* rows, const int64_t* cols, const double* data) {
+                                     const int* adim, const int64_t* rows, const int64_t* cols,
+                                     const double* const* blocks) {
   VMPS_API_BEGIN
   require(e && nsec >= 0 && (nsec == 0 || (qin && qout && qmid && adim)), VMPS_ERR_ARG, "bad argument");
   const int nq = e->P.sym.nq;
-  int64_t slot = 0, pos = 0;
+  {  // one reservation for all sectors
+    int64_t total = 0, nslot = 0;
+    for (int q = 0; q < nsec; ++q) nslot += std::max(0, adim[q]);
+    if (rows && cols)
+      for (int64_t i = 0; i < nslot; ++i) total += std::max<int64_t>(0, rows[i]) * std::max<int64_t>(0, cols[i]);
+    e->P.old.data.reserve(e->P.old.data.size() + (size_t)total);
+  }
+  int64_t slot = 0;
   for (int q = 0; q < nsec; ++q) {
     require(adim[q] >= 0, VMPS_ERR_ARG, "adim < 0");
-    const vmps_status st = vmps_env_add_old(e, qin + q * nq, qout + q * nq, qmid + q * nq, adim[q],
-                                            rows ? rows + slot : nullptr, cols ? cols + slot : nullptr,
-                                            data ? data + pos : nullptr);
-    if (st != VMPS_OK) return st;
-    for (int a = 0; a < adim[q]; ++a) pos += rows[slot + a] * cols[slot + a];
+    require(adim[q] == 0 || (rows && cols && blocks), VMPS_ERR_ARG, "NULL rows/cols/blocks");
+    add_old_sector(e, qin + q * nq, qout + q * nq, qmid + q * nq, adim[q], rows + slot, cols + slot, blocks + slot);
     slot += adim[q];
   }
   VMPS_API_END
 }
 
 vmps_status vmps_env_add_A_blocks(vmps_env* e, int bra, int nblk, const int* s, const int* qin, const int* qout,
-                                  const int64_t* rows, const int64_t* cols, const double* data) {
+                                  const int64_t* rows, const int64_t* cols, const double* const* blocks) {
   VMPS_API_BEGIN
-  require(e && nblk >= 0 && (nblk == 0 || (s && qin && qout && rows && cols)), VMPS_ERR_ARG, "bad argument");
+  require(e && nblk >= 0 && (nblk == 0 || (s && qin && qout && rows && cols && blocks)), VMPS_ERR_ARG,
+          "bad argument");
   const int nq = e->P.sym.nq;
-  int64_t pos = 0;
   for (int b = 0; b < nblk; ++b) {
-    const vmps_status st = vmps_env_add_A(e, bra, s[b], qin + b * nq, qout + b * nq, rows[b], cols[b],
-                                          data ? data + pos : nullptr);
+    const vmps_status st = vmps_env_add_A(e, bra, s[b], qin + b * nq, qout + b * nq, rows[b], cols[b], blocks[b]);
     if (st != VMPS_OK) return st;
-    pos += rows[b] * cols[b];
   }
   VMPS_API_END
 }
@@ -223,7 +243,7 @@ vmps_status vmps_env_contract(vmps_env* e, vmps_precision prec) {
   e->result.assign(e->S.osize, 0.0);
   if (!e->S.terms.empty() && e->S.osize > 0) {
     vmps::DevPlan D;
-    D.create_triples(e->S, prec);
+    D.create_triples(std::move(e->S), prec);  // the plan takes the packed operands; e->S keeps the layout
     e->stats = D.H.stats;
     const size_t ib = (size_t)D.H.ysize * sizeof(double), ob = (size_t)D.H.osize * sizeof(double);
     if (ib) VMPS_HIP_CHECK(hipMemcpyAsync(D.vin.p, e->P.old.data.data(), ib, hipMemcpyHostToDevice, D.stream));

@@ -574,6 +574,11 @@ void DevPlan::create_triples(const TripleSet& S, vmps_precision precision) {
   upload(precision);
 }
 
+void DevPlan::create_triples(TripleSet&& S, vmps_precision precision) {
+  build_plan(std::move(S), H);
+  upload(precision);
+}
+
 void DevPlan::upload(vmps_precision precision) {
   VMPS_HIP_CHECK(hipGetDevice(&device));
   VMPS_HIP_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device));

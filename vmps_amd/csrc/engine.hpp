@@ -137,6 +137,7 @@ struct DevPlan {
   ~DevPlan();
   void create(const Pivot& P, vmps_precision prec);
   void create_triples(const TripleSet& S, vmps_precision prec);
+  void create_triples(TripleSet&& S, vmps_precision prec);  // takes S's environment data
   void upload(vmps_precision prec);  // device copies of H
   void ensure_precision_buffers();
   void ensure_bf16_buffers();

@@ -3,6 +3,7 @@
 #include "pivot.hpp"
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -11,6 +12,7 @@
 #include <unordered_set>
 
 #include "errors.hpp"
+#include "par.hpp"
 
 namespace vmps {
 
@@ -864,18 +866,27 @@ void triples_from_pivot(const Pivot& P, TripleSet& S) {
   std::vector<std::pair<int, int>> vblocks;
   S = TripleSet();
   expand_terms(P, S.terms, lblocks, rblocks, vblocks);
-  for (const auto& lb : lblocks) {
-    const Tripod& E = P.terms[lb.t].L;
-    const DenseBlock& b = E.block[lb.q][lb.a];
-    S.Xb.push_back({(int64_t)S.Xdata.size(), b.rows, b.cols});
-    S.Xdata.insert(S.Xdata.end(), E.data.begin() + b.off, E.data.begin() + b.off + b.size());
-  }
-  for (const auto& rb : rblocks) {
-    const Tripod& E = P.terms[rb.t].R;
-    const DenseBlock& b = E.block[rb.q][rb.a];
-    S.Zb.push_back({(int64_t)S.Zdata.size(), b.rows, b.cols});
-    S.Zdata.insert(S.Zdata.end(), E.data.begin() + b.off, E.data.begin() + b.off + b.size());
-  }
+  // gather the referenced environment blocks, packed in first-use order
+  // (offsets first, then the copies in parallel)
+  auto gather = [&](const std::vector<EnvRef>& refs, bool left, std::vector<BlockDim>& dims,
+                    std::vector<double>& data) {
+    int64_t o = 0;
+    for (const auto& r : refs) {
+      const Tripod& E = left ? P.terms[r.t].L : P.terms[r.t].R;
+      const DenseBlock& b = E.block[r.q][r.a];
+      dims.push_back({o, b.rows, b.cols});
+      o += b.size();
+    }
+    data.resize((size_t)o);
+    parallel_for((int64_t)refs.size(), o, [&](int64_t i) {
+      const auto& r = refs[(size_t)i];
+      const Tripod& E = left ? P.terms[r.t].L : P.terms[r.t].R;
+      const DenseBlock& b = E.block[r.q][r.a];
+      if (b.size()) std::memcpy(data.data() + dims[(size_t)i].off, E.data.data() + b.off, sizeof(double) * b.size());
+    });
+  };
+  gather(lblocks, true, S.Xb, S.Xdata);
+  gather(rblocks, false, S.Zb, S.Zdata);
   for (const auto& vb : vblocks) {
     const DenseBlock& b = P.V.block[vb.first][vb.second];
     S.Yb.push_back({b.off, b.rows, b.cols});
@@ -888,17 +899,34 @@ void triples_from_pivot(const Pivot& P, TripleSet& S) {
 void build_host_plan(const Pivot& P, HostPlan& H) {
   TripleSet S;
   triples_from_pivot(P, S);
-  build_plan(S, H);
+  build_plan(std::move(S), H);
+}
+
+namespace {
+void build_plan_tables(const TripleSet& S, HostPlan& H);
 }
 
 void build_plan(const TripleSet& S, HostPlan& H) {
   H = HostPlan();
+  H.Ldata = S.Xdata;
+  H.Rdata = S.Zdata;
+  build_plan_tables(S, H);
+}
+
+// the environment data move into the plan; S keeps everything else
+void build_plan(TripleSet&& S, HostPlan& H) {
+  H = HostPlan();
+  H.Ldata = std::move(S.Xdata);
+  H.Rdata = std::move(S.Zdata);
+  build_plan_tables(S, H);
+}
+
+namespace {
+void build_plan_tables(const TripleSet& S, HostPlan& H) {
   Stats& st = H.stats;
   H.ysize = S.ysize;
   H.osize = S.osize;
   H.vsize = S.osize;
-  H.Ldata = S.Xdata;
-  H.Rdata = S.Zdata;
   H.Lblk = S.Xb;
   H.Rblk = S.Zb;
   H.Vblk = S.Yb;
@@ -1048,6 +1076,7 @@ void build_plan(const TripleSet& S, HostPlan& H) {
   st.U_elems = H.U_size;
   st.bytes_alg_fp64 = 8.0 * (double)(st.L_elems + st.R_elems + S.ysize + S.osize);
 }
+}  // namespace
 
 }  // namespace vmps
 
@@ -1089,12 +1118,18 @@ void build_bf16_layout(const HostPlan& H, Bf16Layout& B, bool operands) {
   if (operands) {
     B.Lhi.assign(o, 0);
     B.Llo.assign(o, 0);
-    for (const BlockDim& b : H.Lblk) {
-      const int64_t Kp = pad8(b.cols), base = Lmap[b.off];
-      for (int64_t i = 0; i < b.rows; ++i)
+    // work items: 64-row slabs of the blocks (large blocks spread over threads)
+    std::vector<std::array<int64_t, 3>> items;  // (block, first row, end row)
+    for (size_t x = 0; x < H.Lblk.size(); ++x)
+      for (int64_t i0 = 0; i0 < H.Lblk[x].rows; i0 += 64)
+        items.push_back({(int64_t)x, i0, std::min<int64_t>(H.Lblk[x].rows, i0 + 64)});
+    parallel_for((int64_t)items.size(), o, [&](int64_t w) {
+      const BlockDim& b = H.Lblk[(size_t)items[(size_t)w][0]];
+      const int64_t Kp = pad8(b.cols), base = Lmap.at(b.off);
+      for (int64_t i = items[(size_t)w][1]; i < items[(size_t)w][2]; ++i)
         for (int64_t k = 0; k < b.cols; ++k)
           split(H.Ldata[b.off + k * b.rows + i], B.Lhi[base + i * Kp + k], B.Llo[base + i * Kp + k]);
-    }
+    });
   }
   // R, column-major Kp x N
   o = 0;
@@ -1106,12 +1141,18 @@ void build_bf16_layout(const HostPlan& H, Bf16Layout& B, bool operands) {
   if (operands) {
     B.Rhi.assign(o, 0);
     B.Rlo.assign(o, 0);
-    for (const BlockDim& b : H.Rblk) {
-      const int64_t Kp = pad8(b.rows), base = Rmap[b.off];
-      for (int64_t j = 0; j < b.cols; ++j)
+    // work items: 64-column slabs of the blocks
+    std::vector<std::array<int64_t, 3>> items;  // (block, first column, end column)
+    for (size_t x = 0; x < H.Rblk.size(); ++x)
+      for (int64_t j0 = 0; j0 < H.Rblk[x].cols; j0 += 64)
+        items.push_back({(int64_t)x, j0, std::min<int64_t>(H.Rblk[x].cols, j0 + 64)});
+    parallel_for((int64_t)items.size(), o, [&](int64_t w) {
+      const BlockDim& b = H.Rblk[(size_t)items[(size_t)w][0]];
+      const int64_t Kp = pad8(b.rows), base = Rmap.at(b.off);
+      for (int64_t j = items[(size_t)w][1]; j < items[(size_t)w][2]; ++j)
         for (int64_t k = 0; k < b.rows; ++k)
           split(H.Rdata[b.off + j * b.rows + k], B.Rhi[base + j * Kp + k], B.Rlo[base + j * Kp + k]);
-    }
+    });
   }
   // V, column-major Kp x N (filled per call)
   o = 0;

@@ -374,6 +374,7 @@ void expand_terms(const Pivot& P, std::vector<Term>& terms, std::vector<EnvRef>&
 void build_host_plan(const Pivot& P, HostPlan& H);
 void triples_from_pivot(const Pivot& P, TripleSet& S);
 void build_plan(const TripleSet& S, HostPlan& H);
+void build_plan(TripleSet&& S, HostPlan& H);  // moves S's environment data into H
 void build_tiles(HostPlan& H);
 // rebuild the FP64 / FP32 tile lists tilesA_w / tilesC_w (wave tiles of edge
 // 16 or 32) and tilesA_b / tilesC_b (big tiles of the problems >= big_min64)

@@ -4,9 +4,10 @@ vmps_env_add_old_sectors, vmps_env_add_A_blocks, vmps_env_add_W_blocks): one
 C call per Tripod, W slot, MPS tensor or vector layout instead of one per
 sector or block.
 
-Layouts follow include/vmps_hxv.h: quantum numbers nq ints per item, block
-data column-major and back to back in item order, an absent environment
-block as rows = cols = 0."""
+Layouts follow include/vmps_hxv.h: quantum numbers nq ints per item;
+environment and MPS blocks are passed as one pointer per block (column-major
+float64, no packing copy), W nonzeros back to back; an absent environment
+block is rows = cols = 0."""
 import ctypes
 
 import numpy as np
@@ -14,6 +15,7 @@ import numpy as np
 P_int = ctypes.POINTER(ctypes.c_int)
 P_i64 = ctypes.POINTER(ctypes.c_int64)
 P_dbl = ctypes.POINTER(ctypes.c_double)
+P_ptr = ctypes.POINTER(ctypes.c_void_p)
 
 
 def _qns(seq, nq):
@@ -40,26 +42,34 @@ class Packed:
         return [a.ctypes.data_as(k) for a, k in zip(self.arrays, self.kinds)]
 
 
+def _block_ptrs(blocks):
+    """Pointers to column-major float64 copies-if-needed of `blocks` (None:
+    absent, NULL).  Returns (pointer array, arrays to keep alive)."""
+    keep, ptrs = [], np.zeros(max(1, len(blocks)), dtype=np.uint64)
+    for i, blk in enumerate(blocks):
+        if blk is None or blk.size == 0:
+            continue
+        a = np.asfortranarray(blk, dtype=np.float64)
+        keep.append(a)
+        ptrs[i] = a.ctypes.data
+    return ptrs, keep
+
+
 def tripod(sectors, nq):
     """[(qin, qout, qmid, [block or None, ...]), ...] -> nsec, qin, qout,
-    qmid, adim, rows, cols, data."""
+    qmid, adim, rows, cols, block pointers."""
     qin = _qns((x for s in sectors for x in s[0]), nq)
     qout = _qns((x for s in sectors for x in s[1]), nq)
     qmid = _qns((x for s in sectors for x in s[2]), nq)
     adim = np.array([len(s[3]) for s in sectors] or [0], dtype=np.int32)
-    rows, cols, parts = [], [], []
-    for s in sectors:
-        for blk in s[3]:
-            if blk is None:
-                rows.append(0)
-                cols.append(0)
-            else:
-                rows.append(blk.shape[0])
-                cols.append(blk.shape[1])
-                parts.append(np.asarray(blk, dtype=np.float64).ravel(order="F"))
-    return Packed(len(sectors), [qin, qout, qmid, adim, np.array(rows or [0], dtype=np.int64),
-                                 np.array(cols or [0], dtype=np.int64), _cat(parts, np.float64)],
-                  [P_int, P_int, P_int, P_int, P_i64, P_i64, P_dbl])
+    blocks = [blk for s in sectors for blk in s[3]]
+    rows = np.array([0 if b is None else b.shape[0] for b in blocks] or [0], dtype=np.int64)
+    cols = np.array([0 if b is None else b.shape[1] for b in blocks] or [0], dtype=np.int64)
+    ptrs, keep = _block_ptrs(blocks)
+    pk = Packed(len(sectors), [qin, qout, qmid, adim, rows, cols, ptrs],
+                [P_int, P_int, P_int, P_int, P_i64, P_i64, P_ptr])
+    pk.keep = keep
+    return pk
 
 
 def w_blocks(blocks, nq):
@@ -78,15 +88,18 @@ def w_blocks(blocks, nq):
 
 
 def a_blocks(A, nq):
-    """A[s] = [(qin, qout, block), ...] -> nblk, s, qin, qout, rows, cols, data."""
+    """A[s] = [(qin, qout, block), ...] -> nblk, s, qin, qout, rows, cols,
+    block pointers."""
     items = [(s, qi, qo, blk) for s, blocks in enumerate(A) for (qi, qo, blk) in blocks]
     s = np.array([it[0] for it in items] or [0], dtype=np.int32)
     qin = _qns((x for it in items for x in it[1]), nq)
     qout = _qns((x for it in items for x in it[2]), nq)
     rows = np.array([it[3].shape[0] for it in items] or [0], dtype=np.int64)
     cols = np.array([it[3].shape[1] for it in items] or [0], dtype=np.int64)
-    data = _cat([np.asarray(it[3], dtype=np.float64).ravel(order="F") for it in items], np.float64)
-    return Packed(len(items), [s, qin, qout, rows, cols, data], [P_int, P_int, P_int, P_i64, P_i64, P_dbl])
+    ptrs, keep = _block_ptrs([it[3] for it in items])
+    pk = Packed(len(items), [s, qin, qout, rows, cols, ptrs], [P_int, P_int, P_int, P_i64, P_i64, P_ptr])
+    pk.keep = keep
+    return pk
 
 
 def v_blocks(V, nq):
