@@ -18,6 +18,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <mutex>
 
 namespace vmps {
 
@@ -547,10 +549,48 @@ inline unsigned grid_for(size_t waves) { return (unsigned)((waves + 3) / 4); }
 
 }  // namespace
 
+namespace {
+std::mutex g_stream_mutex;
+std::map<int, std::vector<hipStream_t>> g_free_streams;  // device -> idle streams
+}  // namespace
+
+hipStream_t acquire_stream() {
+  int dev = 0;
+  VMPS_HIP_CHECK(hipGetDevice(&dev));
+  {
+    std::lock_guard<std::mutex> lock(g_stream_mutex);
+    auto& v = g_free_streams[dev];
+    if (!v.empty()) {
+      hipStream_t s = v.back();
+      v.pop_back();
+      return s;
+    }
+  }
+  hipStream_t s = nullptr;
+  VMPS_HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  return s;
+}
+
+void release_stream(hipStream_t s, int device) {
+  if (!s) return;
+  std::lock_guard<std::mutex> lock(g_stream_mutex);
+  g_free_streams[device].push_back(s);
+}
+
+void release_stream(hipStream_t s) {
+  if (!s) return;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) {
+    (void)hipStreamDestroy(s);
+    return;
+  }
+  release_stream(s, dev);
+}
+
 DevPlan::~DevPlan() {
   if (stream) {
     (void)hipStreamSynchronize(stream);
-    (void)hipStreamDestroy(stream);
+    release_stream(stream, device);
   }
   if (ev0) (void)hipEventDestroy(ev0);
   if (ev1) (void)hipEventDestroy(ev1);
@@ -582,7 +622,7 @@ void DevPlan::create_triples(TripleSet&& S, vmps_precision precision) {
 void DevPlan::upload(vmps_precision precision) {
   VMPS_HIP_CHECK(hipGetDevice(&device));
   VMPS_HIP_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device));
-  VMPS_HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  stream = acquire_stream();
   VMPS_HIP_CHECK(hipEventCreate(&ev0));
   VMPS_HIP_CHECK(hipEventCreate(&ev1));
   L.upload(H.Ldata, stream);

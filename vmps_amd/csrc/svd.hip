@@ -158,13 +158,16 @@ extern "C" vmps_status vmps_block_svd(int nblocks, const int64_t* rows, const in
       // empty blocks: nothing to factor
       return VMPS_OK;
     }
-    struct Stream {  // destroyed on every exit path
+    struct Stream {  // synchronised and returned to the pool on every exit path
       hipStream_t s = nullptr;
       ~Stream() {
-        if (s) (void)hipStreamDestroy(s);
+        if (s) {
+          (void)hipStreamSynchronize(s);
+          vmps::release_stream(s);
+        }
       }
     } st;
-    VMPS_HIP_CHECK(hipStreamCreateWithFlags(&st.s, hipStreamNonBlocking));
+    st.s = vmps::acquire_stream();
     hipStream_t s = st.s;
     vmps::DevBuf<vmps::SvdTask> dT;
     vmps::DevBuf<double> dW, dV, dS;
