@@ -359,10 +359,14 @@ class GpuBackend:
     rounding level and is refined in FP64 to the same tolerances as an FP64
     solve.  Pivots whose algorithmic work is below `mixed_min_flops` per HxV
     are latency-bound (the mixed products are no faster there) and are solved
-    in FP64 directly."""
+    in FP64 directly.  The default floor, 4 GFLOP per HxV, is where the
+    split-BF16 HxV runs about twice as fast as FP64 (M = 2048 U(1) chains);
+    below it the FP64 refinement (about a third more HxV per solve) costs more
+    than the mixed iterations save (HeisenbergU1 L=32, M=1024: 3.00 vs 2.60 s
+    per sweep with a 0.5 GFLOP floor)."""
 
     def __init__(self, precision="fp64", refine_fp64=True, tol_eigval=1e-10, tol_state=1e-6, max_iter=100,
-                 mixed_min_flops=5e8, device_svd_min=0):
+                 mixed_min_flops=4e9, device_svd_min=0):
         self.precision = precision
         self.refine = refine_fp64
         self.tol_eigval, self.tol_state, self.max_iter = tol_eigval, tol_state, max_iter
