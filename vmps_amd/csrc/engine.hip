@@ -285,6 +285,7 @@ __global__ __launch_bounds__(256) void k_stage_c(const TileC* __restrict__ tiles
 // issued before the MFMAs of the current one.
 constexpr int kBT = kBigTile;
 constexpr int kBKd = 16;
+constexpr int kBQ = kBKd / 2;  // K elements each thread stages per operand and step
 
 template <class T>
 struct BigTileSmem {
@@ -328,8 +329,8 @@ __device__ inline void big_tile_store(OutT* __restrict__ C, int M, int N, int m0
 }
 
 // Thread roles in the staging loads (256 threads, 2048 elements per operand):
-//   A (column-major, ld M): row i = tid % 128, k = kb + tid/128 + 2q, q < 8
-//   B (column-major K x N, ld K): column j = tid / 2, k = kb + 8 (tid % 2) + q
+//   A (column-major, ld M): row i = tid % 128, k = kb + tid/128 + 2q, q < kBQ
+//   B (column-major K x N, ld K): column j = tid / 2, k = kb + kBQ (tid % 2) + q
 template <class T>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_stage_a_big(const TileA* __restrict__ tiles, int ntiles,
                                                      const T* __restrict__ L, const T* __restrict__ V,
@@ -343,17 +344,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
   const int m0 = t.mt * kBT, n0 = t.nt * kBT;
   const T* __restrict__ A = L + t.a_off;
   const T* __restrict__ B = V + t.b_off;
-  const int ai = tid & 127, ak = tid >> 7, bj = tid >> 1, bk = (tid & 1) * 8;
+  const int ai = tid & 127, ak = tid >> 7, bj = tid >> 1, bk = (tid & 1) * kBQ;
   const bool arow = m0 + ai < t.M, bcol = n0 + bj < t.N;
   typename Mfma<T>::acc_t acc[4][4];
 #pragma unroll
   for (int x = 0; x < 4; ++x)
 #pragma unroll
     for (int y = 0; y < 4; ++y) acc[x][y] = typename Mfma<T>::acc_t{0, 0, 0, 0};
-  T ra[8], rb[8];
+  T ra[kBQ], rb[kBQ];
   auto gload = [&](int kb) {
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
+    for (int q = 0; q < kBQ; ++q) {
       const int ka = kb + ak + 2 * q, kbb = kb + bk + q;
       ra[q] = (arow && ka < t.K) ? A[(int64_t)ka * t.M + m0 + ai] : T(0);
       rb[q] = (bcol && kbb < t.K) ? B[(int64_t)(n0 + bj) * t.K + kbb] : T(0);
@@ -363,7 +364,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
   for (int kb = 0; kb < t.K; kb += kBKd) {
     __syncthreads();  // the previous step's LDS reads are done
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
+    for (int q = 0; q < kBQ; ++q) {
       s.A[ak + 2 * q][ai] = ra[q];
       s.B[bk + q][bj] = rb[q];
     }
@@ -392,7 +393,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
   const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
   const TileC t = tiles[blockIdx.x];
   const int m0 = t.mt * kBT, n0 = t.nt * kBT;
-  const int ai = tid & 127, ak = tid >> 7, bj = tid >> 1, bk = (tid & 1) * 8;
+  const int ai = tid & 127, ak = tid >> 7, bj = tid >> 1, bk = (tid & 1) * kBQ;
   const bool arow = m0 + ai < t.M, bcol = n0 + bj < t.N;
   typename Mfma<T>::acc_t acc[4][4];
 #pragma unroll
@@ -414,10 +415,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     if (kOneTerm) {  // one term (explicit stage B): a plain product, next step's loads in flight
       const T* __restrict__ T0 = Tbuf + readlane64(my_toff, 0);
       const T c0 = readlane_t<T>(my_c, 0);
-      T ra[8], rb[8];
+      T ra[kBQ], rb[kBQ];
       auto gload = [&](int kb) {
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
+        for (int q = 0; q < kBQ; ++q) {
           const int ka = kb + ak + 2 * q, kbb = kb + bk + q;
           ra[q] = (arow && ka < K) ? T0[(int64_t)ka * t.M + m0 + ai] : T(0);
           rb[q] = (bcol && kbb < K) ? B[(int64_t)(n0 + bj) * K + kbb] : T(0);
@@ -427,7 +428,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
       for (int kb = 0; kb < K; kb += kBKd) {
         __syncthreads();  // the previous step's LDS reads are done
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
+        for (int q = 0; q < kBQ; ++q) {
           s.A[ak + 2 * q][ai] = c0 * ra[q];
           s.B[bk + q][bj] = rb[q];
         }
@@ -438,9 +439,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
       continue;
     }
     for (int kb = 0; kb < K; kb += kBKd) {
-      T ra[8], rb[8];
+      T ra[kBQ], rb[kBQ];
 #pragma unroll
-      for (int q = 0; q < 8; ++q) {
+      for (int q = 0; q < kBQ; ++q) {
         const int kbb = kb + bk + q;
         rb[q] = (bcol && kbb < K) ? B[(int64_t)(n0 + bj) * K + kbb] : T(0);
         ra[q] = T(0);
@@ -450,29 +451,29 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
         const T* __restrict__ T0 = Tbuf + readlane64(my_toff, jt);
         const T* __restrict__ T1 = Tbuf + readlane64(my_toff, jt + 1);
         const T c0 = readlane_t<T>(my_c, jt), c1 = readlane_t<T>(my_c, jt + 1);
-        T x0[8], x1[8];
+        T x0[kBQ], x1[kBQ];
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
+        for (int q = 0; q < kBQ; ++q) {
           const int ka = kb + ak + 2 * q;
           const bool ok = arow && ka < K;
           x0[q] = ok ? T0[(int64_t)ka * t.M + m0 + ai] : T(0);
           x1[q] = ok ? T1[(int64_t)ka * t.M + m0 + ai] : T(0);
         }
 #pragma unroll
-        for (int q = 0; q < 8; ++q) ra[q] += c0 * x0[q] + c1 * x1[q];
+        for (int q = 0; q < kBQ; ++q) ra[q] += c0 * x0[q] + c1 * x1[q];
       }
       if (jt < nt) {
         const T* __restrict__ T0 = Tbuf + readlane64(my_toff, jt);
         const T c0 = readlane_t<T>(my_c, jt);
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
+        for (int q = 0; q < kBQ; ++q) {
           const int ka = kb + ak + 2 * q;
           ra[q] += c0 * ((arow && ka < K) ? T0[(int64_t)ka * t.M + m0 + ai] : T(0));
         }
       }
       __syncthreads();  // the previous step's LDS reads are done
 #pragma unroll
-      for (int q = 0; q < 8; ++q) {
+      for (int q = 0; q < kBQ; ++q) {
         s.A[ak + 2 * q][ai] = ra[q];
         s.B[bk + q][bj] = rb[q];
       }
