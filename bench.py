@@ -172,8 +172,8 @@ def main():
         return
 
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU; --gpus %d is not supported "
-              "(the pivot shards over output blocks across GPUs by design, but no multi-GPU path is built)"
+        print("bench.py: the HxV runs on one GPU; --gpus %d is not supported "
+              "(a pivot would shard over its output blocks across GPUs; no multi-GPU path is implemented)"
               % args.gpus, file=sys.stderr)
         sys.exit(2)
 
