@@ -56,6 +56,22 @@ __global__ __launch_bounds__(256) void k_split_v(const VConv* __restrict__ tasks
   }
 }
 
+// FP64 L blocks (column-major M x K, ld M) -> row-major M x Kp split-BF16
+// copies (L^T in the layout of k_split_v's output); one workgroup per task of
+// at most 4096 padded elements, task rows r read src[src_off + k M + r].
+__global__ __launch_bounds__(256) void k_split_lt(const VConv* __restrict__ tasks, const double* __restrict__ L,
+                                                  __bf16* __restrict__ Lhi, __bf16* __restrict__ Llo) {
+  const VConv t = tasks[blockIdx.x];
+  const int n = t.Kp * t.N;
+  for (int e = threadIdx.x; e < n; e += 256) {
+    const int r = e / t.Kp, k = e - r * t.Kp;
+    const double x = (k < t.K) ? L[t.src_off + (int64_t)k * t.pad + r] : 0.0;
+    const __bf16 h = (__bf16)(float)x;
+    Lhi[t.dst_off + e] = h;
+    Llo[t.dst_off + e] = (__bf16)(float)(x - (double)(float)h);
+  }
+}
+
 // The split operands of one 32-deep K block of a 32 x 32 wave tile:
 // A rows i0 = m0 + lane%16 and i0 + 16, B columns j0 = n0 + lane%16 and j0 + 16,
 // k = kb + 8 (lane/16) .. +7.
@@ -615,11 +631,24 @@ __global__ __launch_bounds__(256) void k_reduce_partials(const TileC* __restrict
 void DevPlan::ensure_bf16_buffers() {
   if (bf16_ready || (H.tilesA.empty() && H.tilesC.empty())) return;
   Bf16Layout B;
-  build_bf16_layout(H, B);
-  Lhi.upload(B.Lhi, stream);
-  Llo.upload(B.Llo, stream);
-  Rhi.upload(B.Rhi, stream);
-  Rlo.upload(B.Rlo, stream);
+  build_bf16_layout(H, B, false);  // work lists; the operands are split on the device
+  Lhi.alloc(std::max<int64_t>(1, B.L_size));
+  Llo.alloc(std::max<int64_t>(1, B.L_size));
+  Rhi.alloc(std::max<int64_t>(1, B.R_size));
+  Rlo.alloc(std::max<int64_t>(1, B.R_size));
+  {
+    DevBuf<VConv> lc, rc;  // freed after the synchronisation below
+    lc.upload(B.lconv, stream);
+    rc.upload(B.rconv, stream);
+    if (lc.n)
+      hipLaunchKernelGGL(k_split_lt, dim3((unsigned)lc.n), dim3(256), 0, stream, lc.p, L.p, (__bf16*)Lhi.p,
+                         (__bf16*)Llo.p);
+    if (rc.n)
+      hipLaunchKernelGGL(k_split_v, dim3((unsigned)rc.n), dim3(256), 0, stream, rc.p, R.p, (__bf16*)Rhi.p,
+                         (__bf16*)Rlo.p);
+    VMPS_HIP_CHECK(hipGetLastError());
+    VMPS_HIP_CHECK(hipStreamSynchronize(stream));
+  }
   Vhi.alloc(std::max<int64_t>(1, B.V_size));
   Vlo.alloc(std::max<int64_t>(1, B.V_size));
   Tp.alloc(std::max<int64_t>(1, B.T_size));

@@ -1115,6 +1115,14 @@ void build_bf16_layout(const HostPlan& H, Bf16Layout& B, bool operands) {
     o += b.rows * pad8(b.cols);
   }
   B.L_size = o;
+  for (const BlockDim& b : H.Lblk) {  // conversion tasks of at most 4096 padded elements
+    if (b.rows * b.cols == 0) continue;
+    const int64_t Kp = pad8(b.cols), base = Lmap.at(b.off);
+    const int64_t rows_per = std::max<int64_t>(1, 4096 / Kp);
+    for (int64_t i0 = 0; i0 < b.rows; i0 += rows_per)
+      B.lconv.push_back({b.off + i0, base + i0 * Kp, (int32_t)b.cols, (int32_t)std::min(rows_per, b.rows - i0),
+                         (int32_t)Kp, (int32_t)b.rows});
+  }
   if (operands) {
     B.Lhi.assign(o, 0);
     B.Llo.assign(o, 0);
@@ -1138,6 +1146,14 @@ void build_bf16_layout(const HostPlan& H, Bf16Layout& B, bool operands) {
     o += pad8(b.rows) * b.cols;
   }
   B.R_size = o;
+  for (const BlockDim& b : H.Rblk) {
+    if (b.rows * b.cols == 0) continue;
+    const int64_t Kp = pad8(b.rows), base = Rmap.at(b.off);
+    const int64_t cols_per = std::max<int64_t>(1, 4096 / Kp);
+    for (int64_t j0 = 0; j0 < b.cols; j0 += cols_per)
+      B.rconv.push_back({b.off + j0 * b.rows, base + j0 * Kp, (int32_t)b.rows, (int32_t)std::min(cols_per, b.cols - j0),
+                         (int32_t)Kp, 0});
+  }
   if (operands) {
     B.Rhi.assign(o, 0);
     B.Rlo.assign(o, 0);

@@ -280,6 +280,12 @@ struct Bf16Layout {
   std::vector<ChunkC16> chunksC;
   std::vector<TermC16> termsC;
   std::vector<VConv> vconv;
+  // device conversion of the FP64 environment blocks into the split
+  // operands: R blocks exactly as V (K x N -> padded Kp x N); L blocks
+  // transposed (rconv's format with src_off = block offset + first row,
+  // N = rows of the task, pad = the block's leading dimension M; element
+  // (r, k) of the task reads src[src_off + k M + r], writes dst[dst_off + r Kp + k])
+  std::vector<VConv> lconv, rconv;
 };
 
 struct BlockDim {
