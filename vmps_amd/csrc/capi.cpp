@@ -100,11 +100,9 @@ void add_env_sector(vmps_pivot2* p, int which, const int* qin, const int* qout, 
   vmps::Tripod& E = which == 0 ? p->P.term().L : p->P.term().R;
   vmps::Key3 key{read_qn(qin, nq), read_qn(qout, nq), read_qn(qmid, nq)};
   require(!E.dict.count(key), VMPS_ERR_LAYOUT, "duplicate environment sector");
-  int64_t total = 0;
   for (int a = 0; a < adim; ++a) {
     require(rows[a] >= 0 && cols[a] >= 0, VMPS_ERR_ARG, "negative block dimension");
     require(rows[a] * cols[a] == 0 || ptrs[a], VMPS_ERR_ARG, "NULL block data");
-    total += rows[a] * cols[a];
   }
   std::vector<vmps::DenseBlock> blocks(adim);
   for (int a = 0; a < adim; ++a) {

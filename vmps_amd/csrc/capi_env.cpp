@@ -53,11 +53,9 @@ void add_old_sector(vmps_env* e, const int* qin, const int* qout, const int* qmi
   vmps::Tripod& E = e->P.old;
   vmps::Key3 key{qn(qin, nq), qn(qout, nq), qn(qmid, nq)};
   require(!E.dict.count(key), VMPS_ERR_LAYOUT, "duplicate environment sector");
-  int64_t total = 0;
   for (int a = 0; a < adim; ++a) {
     require(rows[a] >= 0 && cols[a] >= 0, VMPS_ERR_ARG, "negative block dimension");
     require(rows[a] * cols[a] == 0 || ptrs[a], VMPS_ERR_ARG, "NULL block data");
-    total += rows[a] * cols[a];
   }
   std::vector<vmps::DenseBlock> blocks(adim);
   for (int a = 0; a < adim; ++a) {

@@ -631,7 +631,7 @@ __global__ __launch_bounds__(256) void k_reduce_partials(const TileC* __restrict
 void DevPlan::ensure_bf16_buffers() {
   if (bf16_ready || (H.tilesA.empty() && H.tilesC.empty())) return;
   Bf16Layout B;
-  build_bf16_layout(H, B, false);  // work lists; the operands are split on the device
+  build_bf16_layout(H, B);  // work lists; the operands are split on the device
   Lhi.alloc(std::max<int64_t>(1, B.L_size));
   Llo.alloc(std::max<int64_t>(1, B.L_size));
   Rhi.alloc(std::max<int64_t>(1, B.R_size));
@@ -697,7 +697,7 @@ void DevPlan::retile(int wave_edge, int big_min, int big_min64, bool explicit_u)
   if (bf16_lists) {  // split-BF16 work lists; the split operands stay
     VMPS_HIP_CHECK(hipStreamSynchronize(stream));
     Bf16Layout B;
-    build_bf16_layout(H, B, false);
+    build_bf16_layout(H, B);
     upload_bf16_lists(B);
   }
   VMPS_HIP_CHECK(hipStreamSynchronize(stream));

@@ -1082,30 +1082,7 @@ void build_plan_tables(const TripleSet& S, HostPlan& H) {
 
 namespace vmps {
 
-// float -> bfloat16, round to nearest even (NaN kept quiet)
-uint16_t bf16_rne(float f) {
-  uint32_t u;
-  std::memcpy(&u, &f, 4);
-  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
-  const uint32_t lsb = (u >> 16) & 1u;
-  u += 0x7fffu + lsb;
-  return (uint16_t)(u >> 16);
-}
-
-namespace {
-float bf16_to_f32(uint16_t h) {
-  uint32_t u = (uint32_t)h << 16;
-  float f;
-  std::memcpy(&f, &u, 4);
-  return f;
-}
-void split(double x, uint16_t& hi, uint16_t& lo) {
-  hi = bf16_rne((float)x);
-  lo = bf16_rne((float)(x - (double)bf16_to_f32(hi)));
-}
-}  // namespace
-
-void build_bf16_layout(const HostPlan& H, Bf16Layout& B, bool operands) {
+void build_bf16_layout(const HostPlan& H, Bf16Layout& B) {
   B = Bf16Layout();
   // L^T, row-major M x Kp at 8-aligned offsets
   std::unordered_map<int64_t, int64_t> Lmap, Rmap, Vmap, Tmap, Umap;
@@ -1123,22 +1100,6 @@ void build_bf16_layout(const HostPlan& H, Bf16Layout& B, bool operands) {
       B.lconv.push_back({b.off + i0, base + i0 * Kp, (int32_t)b.cols, (int32_t)std::min(rows_per, b.rows - i0),
                          (int32_t)Kp, (int32_t)b.rows});
   }
-  if (operands) {
-    B.Lhi.assign(o, 0);
-    B.Llo.assign(o, 0);
-    // work items: 64-row slabs of the blocks (large blocks spread over threads)
-    std::vector<std::array<int64_t, 3>> items;  // (block, first row, end row)
-    for (size_t x = 0; x < H.Lblk.size(); ++x)
-      for (int64_t i0 = 0; i0 < H.Lblk[x].rows; i0 += 64)
-        items.push_back({(int64_t)x, i0, std::min<int64_t>(H.Lblk[x].rows, i0 + 64)});
-    parallel_for((int64_t)items.size(), o, [&](int64_t w) {
-      const BlockDim& b = H.Lblk[(size_t)items[(size_t)w][0]];
-      const int64_t Kp = pad8(b.cols), base = Lmap.at(b.off);
-      for (int64_t i = items[(size_t)w][1]; i < items[(size_t)w][2]; ++i)
-        for (int64_t k = 0; k < b.cols; ++k)
-          split(H.Ldata[b.off + k * b.rows + i], B.Lhi[base + i * Kp + k], B.Llo[base + i * Kp + k]);
-    });
-  }
   // R, column-major Kp x N
   o = 0;
   for (const BlockDim& b : H.Rblk) {
@@ -1153,22 +1114,6 @@ void build_bf16_layout(const HostPlan& H, Bf16Layout& B, bool operands) {
     for (int64_t j0 = 0; j0 < b.cols; j0 += cols_per)
       B.rconv.push_back({b.off + j0 * b.rows, base + j0 * Kp, (int32_t)b.rows, (int32_t)std::min(cols_per, b.cols - j0),
                          (int32_t)Kp, 0});
-  }
-  if (operands) {
-    B.Rhi.assign(o, 0);
-    B.Rlo.assign(o, 0);
-    // work items: 64-column slabs of the blocks
-    std::vector<std::array<int64_t, 3>> items;  // (block, first column, end column)
-    for (size_t x = 0; x < H.Rblk.size(); ++x)
-      for (int64_t j0 = 0; j0 < H.Rblk[x].cols; j0 += 64)
-        items.push_back({(int64_t)x, j0, std::min<int64_t>(H.Rblk[x].cols, j0 + 64)});
-    parallel_for((int64_t)items.size(), o, [&](int64_t w) {
-      const BlockDim& b = H.Rblk[(size_t)items[(size_t)w][0]];
-      const int64_t Kp = pad8(b.rows), base = Rmap.at(b.off);
-      for (int64_t j = items[(size_t)w][1]; j < items[(size_t)w][2]; ++j)
-        for (int64_t k = 0; k < b.rows; ++k)
-          split(H.Rdata[b.off + j * b.rows + k], B.Rhi[base + j * Kp + k], B.Rlo[base + j * Kp + k]);
-    });
   }
   // V, column-major Kp x N (filled per call)
   o = 0;

@@ -259,8 +259,7 @@ constexpr int kBigK = 64;      // k depth of one LDS stage of those kernels
 // dimensions are at least this (below, the wave-tile kernels waste less).
 constexpr int kBigMin = 96;
 struct Bf16Layout {
-  std::vector<uint16_t> Lhi, Llo, Rhi, Rlo;  // split operands, built once
-  int64_t L_size = 0, R_size = 0;            // their sizes (also when not filled)
+  int64_t L_size = 0, R_size = 0;            // sizes of the split L / R operands
   int64_t V_size = 0, T_size = 0;            // padded sizes of the per-call arrays
   int64_t U_size = 0;                        // padded U (explicit stage B only)
   std::vector<CombTask> comb_tasks;          // explicit stage B on the padded layouts
@@ -400,11 +399,9 @@ constexpr size_t kXcdMinTilesA = 2048, kXcdMinTilesC = 512;
 // flops in classes of (rows, columns), quantised, so that pivots of similar
 // shape share one tuned tile configuration
 std::string tile_signature(const HostPlan& H);
-// Split-BF16 operands and work lists of the BF16x3 path (built on demand).
-// split-BF16 layouts and work lists; with operands = false the split L / R
-// arrays are left empty (their offsets and sizes are still computed)
-void build_bf16_layout(const HostPlan& H, Bf16Layout& B, bool operands = true);
-uint16_t bf16_rne(float f);
+// Split-BF16 layouts and work lists of the BF16x3 path (built on demand; the
+// split operands themselves are formed on the device from lconv / rconv).
+void build_bf16_layout(const HostPlan& H, Bf16Layout& B);
 // Throws VMPS_ERR_INTERNAL if any descriptor would address outside its buffer.
 void validate_host_plan(const HostPlan& H);
 
