@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define VMPS_ABI_VERSION 2
+#define VMPS_ABI_VERSION 3
 
 typedef int vmps_status;
 #define VMPS_OK 0
@@ -310,6 +310,38 @@ vmps_status vmps_env_result_get(const vmps_env* e, int* qin, int* qout, int* qmi
                                 int64_t* cols, double* data);
 /* work counts of the last contraction's plan */
 vmps_status vmps_env_stats(const vmps_env* e, vmps_plan_stats* st);
+
+/* ---- environments resident on the device ----------------------------------
+ * The solvers keep every environment Heff[l].Terms[t].L / R in host memory
+ * and hand the new one from contract_L / contract_R straight to the next
+ * pivot (solvers/DmrgSolver.h:1945-1953).  A vmps_denv keeps such an
+ * environment on the device it was made on: the sector layout on the host,
+ * the blocks in one device allocation, so a sweep need not copy environments
+ * between host and device.  Blocks are read when a plan or contraction is
+ * created from it (the denv may be destroyed afterwards); it is never
+ * written once made. */
+typedef struct vmps_denv vmps_denv;
+
+/* a device environment from host blocks, laid out as vmps_env_add_old_sectors */
+vmps_status vmps_denv_create(int nq, int nsec, const int* qin, const int* qout, const int* qmid, const int* adim,
+                             const int64_t* rows, const int64_t* cols, const double* const* blocks,
+                             vmps_denv** out);
+void vmps_denv_destroy(vmps_denv* d);
+/* layout and (data != NULL) the blocks, as vmps_env_result_size / _get */
+vmps_status vmps_denv_size(const vmps_denv* d, int64_t* nsec, int64_t* nslots, int64_t* size);
+vmps_status vmps_denv_get(const vmps_denv* d, int* qin, int* qout, int* qmid, int* adim, int64_t* rows,
+                          int64_t* cols, double* data);
+/* vmps_env_contract whose result stays on the device as a new vmps_denv
+ * (vmps_env_result_get then reports VMPS_ERR_STATE; result_size and stats
+ * still answer) */
+vmps_status vmps_env_contract_device(vmps_env* e, vmps_precision prec, vmps_denv** out);
+/* the old environment of a contraction: the whole Tripod Lold / Rold at once
+ * (instead of vmps_env_add_old*) */
+vmps_status vmps_env_add_old_device(vmps_env* e, const vmps_denv* E);
+/* the whole environment L (which = 0) or R (which = 1) of the current term
+ * (instead of vmps_pivot2_add_env*); the L of all terms must be all on the
+ * host or all on the device, and likewise the R */
+vmps_status vmps_pivot2_add_env_device(vmps_pivot2* p, int which, const vmps_denv* E);
 
 /* ---- PivotVector algebra on device vectors (DmrgPivotVector.h:121-424) ----
  * All vectors are packed device arrays of the plan's layout; work is enqueued

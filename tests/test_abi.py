@@ -27,7 +27,7 @@ def test_library_exports_all_declared_symbols():
     assert len(names) >= 20
     for n in names:
         assert hasattr(lib, n), n
-    assert lib.vmps_abi_version() == 2
+    assert lib.vmps_abi_version() == 3
 
 
 def test_bad_arguments_give_status_and_message():
@@ -72,7 +72,20 @@ def test_layout_errors():
     comp = next(i for i, blocks in enumerate(pd.V) if blocks)
     bv = pack.v_blocks([[] for _ in range(comp)] + [pd.V[comp][:1]], 1)
     assert lib.vmps_pivot2_add_V_blocks(h, bv.count, *bv.args(), None) == 2
+    # device environments: argument checks come before any device call
+    assert lib.vmps_pivot2_add_env_device(h, 0, None) == 1
     lib.vmps_pivot2_destroy(h)
+    d = ctypes.c_void_p()
+    bk = pack.tripod(pd.L, 1)
+    assert lib.vmps_denv_create(1, bk.count, *bk.args(), None) == 1
+    assert lib.vmps_denv_create(0, bk.count, *bk.args(), ctypes.byref(d)) == 1
+    assert not d.value
+    e = ctypes.c_void_p()
+    kinds = (ctypes.c_int * 1)(1)  # U(1)
+    _lib.check(lib.vmps_env_create(0, 1, kinds, 1, q, 1, q, ctypes.byref(e)))
+    assert lib.vmps_env_contract_device(e, 0, None) == 1
+    assert lib.vmps_env_add_old_device(e, None) == 1
+    lib.vmps_env_destroy(e)
 
 
 def test_bulk_builders_equal_single_item_calls():

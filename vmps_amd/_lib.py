@@ -128,6 +128,14 @@ SIGNATURES = {
     "vmps_env_result_size": (c_int, [c_void, P_i64, P_i64, P_i64]),
     "vmps_env_result_get": (c_int, [c_void, P_int, P_int, P_int, P_int, P_i64, P_i64, P_dbl]),
     "vmps_env_stats": (c_int, [c_void, ctypes.POINTER(PlanStats)]),
+    "vmps_denv_create": (c_int, [c_int, c_int, P_int, P_int, P_int, P_int, P_i64, P_i64, ctypes.POINTER(c_void),
+                                 ctypes.POINTER(c_void)]),
+    "vmps_denv_destroy": (None, [c_void]),
+    "vmps_denv_size": (c_int, [c_void, P_i64, P_i64, P_i64]),
+    "vmps_denv_get": (c_int, [c_void, P_int, P_int, P_int, P_int, P_i64, P_i64, P_dbl]),
+    "vmps_env_contract_device": (c_int, [c_void, c_int, ctypes.POINTER(c_void)]),
+    "vmps_env_add_old_device": (c_int, [c_void, c_void]),
+    "vmps_pivot2_add_env_device": (c_int, [c_void, c_int, c_void]),
     "vmps_vec_alloc": (c_int, [c_void, ctypes.POINTER(c_void)]),
     "vmps_vec_free": (c_int, [c_void, c_void]),
     "vmps_vec_upload": (c_int, [c_void, c_void, P_dbl]),

@@ -11,6 +11,7 @@
 #include <mutex>
 #include <string>
 
+#include "denv.hpp"
 #include "engine.hpp"
 #include "lanczos.hpp"
 #include "errors.hpp"
@@ -98,6 +99,7 @@ void add_env_sector(vmps_pivot2* p, int which, const int* qin, const int* qout, 
                     const int64_t* rows, const int64_t* cols, const double* const* ptrs) {
   const int nq = p->P.sym.nq;
   vmps::Tripod& E = which == 0 ? p->P.term().L : p->P.term().R;
+  require(!E.dev, VMPS_ERR_STATE, "this environment is already given on the device");
   vmps::Key3 key{read_qn(qin, nq), read_qn(qout, nq), read_qn(qmid, nq)};
   require(!E.dict.count(key), VMPS_ERR_LAYOUT, "duplicate environment sector");
   for (int a = 0; a < adim; ++a) {
@@ -257,6 +259,20 @@ vmps_status vmps_pivot2_add_env(vmps_pivot2* p, int which, const int* qin, const
     }
   }
   add_env_sector(p, which, qin, qout, qmid, adim, rows, cols, ptrs.data());
+  VMPS_API_END
+}
+
+vmps_status vmps_pivot2_add_env_device(vmps_pivot2* p, int which, const vmps_denv* E) {
+  VMPS_API_BEGIN
+  require(p && E, VMPS_ERR_ARG, "NULL argument");
+  require(which == 0 || which == 1, VMPS_ERR_ARG, "which must be 0 (L) or 1 (R)");
+  require(E->nq == p->P.sym.nq, VMPS_ERR_ARG, "symmetry mismatch");
+  int dev = -1;
+  VMPS_HIP_CHECK(hipGetDevice(&dev));
+  require(dev == E->device, VMPS_ERR_ARG, "the environment lives on another device");
+  vmps::Tripod& T = which == 0 ? p->P.term().L : p->P.term().R;
+  require(T.dim() == 0 && !T.dev, VMPS_ERR_STATE, "this environment already has sectors");
+  T = E->T;
   VMPS_API_END
 }
 
@@ -485,6 +501,8 @@ vmps_status vmps_plan_create_batch(const vmps_pivot2* const* pivots, int n, vmps
     require(pivots[i]->P.A0proj.empty(), VMPS_ERR_ARG, "projectors are not supported in a batch");
     vmps::TripleSet Si;
     vmps::triples_from_pivot(pivots[i]->P, Si);
+    require(Si.Xgather.empty() && Si.Zgather.empty(), VMPS_ERR_ARG,
+            "a batch takes host environments only (vmps_pivot2_add_env*)");
     const int64_t nx = (int64_t)S.Xb.size(), nz = (int64_t)S.Zb.size(), ny = (int64_t)S.Yb.size();
     const int64_t xo = (int64_t)S.Xdata.size(), zo = (int64_t)S.Zdata.size(), vo = S.ysize;
     if (offsets) offsets[i] = vo;

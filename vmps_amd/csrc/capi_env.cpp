@@ -5,6 +5,7 @@
 #include <cstring>
 #include <memory>
 
+#include "denv.hpp"
 #include "engine.hpp"
 #include "env.hpp"
 #include "errors.hpp"
@@ -16,6 +17,7 @@ struct vmps_env {
   vmps::Stats stats;
   std::vector<double> result;  // output vector in S.Ob order
   bool done = false;
+  bool result_on_device = false;  // the result went to a vmps_denv
 };
 
 using vmps::VmpsError;
@@ -47,10 +49,9 @@ void require(bool cond, int status, const char* msg) {
   if (!cond) throw VmpsError(status, msg);
 }
 // one sector of Lold / Rold from per-block pointers (absent blocks: rows * cols == 0)
-void add_old_sector(vmps_env* e, const int* qin, const int* qout, const int* qmid, int adim, const int64_t* rows,
-                    const int64_t* cols, const double* const* ptrs) {
-  const int nq = e->P.sym.nq;
-  vmps::Tripod& E = e->P.old;
+void add_old_sector(vmps::Tripod& E, int nq, const int* qin, const int* qout, const int* qmid, int adim,
+                    const int64_t* rows, const int64_t* cols, const double* const* ptrs) {
+  require(!E.dev, VMPS_ERR_STATE, "the environment is already given on the device");
   vmps::Key3 key{qn(qin, nq), qn(qout, nq), qn(qmid, nq)};
   require(!E.dict.count(key), VMPS_ERR_LAYOUT, "duplicate environment sector");
   for (int a = 0; a < adim; ++a) {
@@ -69,7 +70,129 @@ void add_old_sector(vmps_env* e, const int* qin, const int* qout, const int* qmi
   E.index.push_back({key.a, key.b, key.c});
   E.block.push_back(std::move(blocks));
   E.dict[key] = E.index.size() - 1;
+}
+
+// sectors given as nsec items (qin, qout, qmid: nq ints each; adim slots
+// each; rows, cols, blocks: one per slot)
+void add_old_sectors(vmps::Tripod& E, int nq, int nsec, const int* qin, const int* qout, const int* qmid,
+                     const int* adim, const int64_t* rows, const int64_t* cols, const double* const* blocks) {
+  require(nsec >= 0 && (nsec == 0 || (qin && qout && qmid && adim)), VMPS_ERR_ARG, "bad argument");
+  {  // one reservation for all sectors
+    int64_t total = 0, nslot = 0;
+    for (int q = 0; q < nsec; ++q) nslot += std::max(0, adim[q]);
+    if (rows && cols)
+      for (int64_t i = 0; i < nslot; ++i) total += std::max<int64_t>(0, rows[i]) * std::max<int64_t>(0, cols[i]);
+    E.data.reserve(E.data.size() + (size_t)total);
+  }
+  int64_t slot = 0;
+  for (int q = 0; q < nsec; ++q) {
+    require(adim[q] >= 0, VMPS_ERR_ARG, "adim < 0");
+    require(adim[q] == 0 || (rows && cols && blocks), VMPS_ERR_ARG, "NULL rows/cols/blocks");
+    add_old_sector(E, nq, qin + q * nq, qout + q * nq, qmid + q * nq, adim[q], rows + slot, cols + slot,
+                   blocks + slot);
+    slot += adim[q];
+  }
+}
+
+// copies the packed blocks of E to a new allocation on the current device
+// and empties E.data
+void move_to_device(vmps_denv* d) {
+  VMPS_HIP_CHECK(hipGetDevice(&d->device));
+  const int64_t n = (int64_t)d->T.data.size();
+  VMPS_HIP_CHECK(hipMalloc((void**)&d->buf, std::max<int64_t>(1, n) * sizeof(double)));
+  if (n) VMPS_HIP_CHECK(hipMemcpy(d->buf, d->T.data.data(), n * sizeof(double), hipMemcpyHostToDevice));
+  d->T.data.clear();
+  d->T.data.shrink_to_fit();
+  d->T.dev = d->buf;
+  d->T.dev_size = n;
+}
+
+void contract(vmps_env* e, vmps_precision prec, vmps_denv* dres) {
+  require(e != nullptr, VMPS_ERR_ARG, "NULL env");
+  require(prec == VMPS_PREC_FP64 || prec == VMPS_PREC_FP32 || prec == VMPS_PREC_BF16X3, VMPS_ERR_ARG,
+          "unknown precision");
+  if (e->P.old.dev) {
+    int dev = -1;
+    VMPS_HIP_CHECK(hipGetDevice(&dev));
+    require(dev == e->P.old.dev_device, VMPS_ERR_ARG, "the old environment lives on another device");
+  }
+  vmps::plan_env(e->P, e->S);
   e->done = false;
+  e->result_on_device = dres != nullptr;
+  e->result.clear();
+  if (!dres) e->result.assign(e->S.osize, 0.0);
+  const int64_t osize = e->S.osize;
+  double* dout = nullptr;  // device result handed to dres
+  if (!e->S.terms.empty() && osize > 0) {
+    vmps::DevPlan D;
+    D.create_triples(std::move(e->S), prec);  // the plan takes the packed operands; e->S keeps the layout
+    e->stats = D.H.stats;
+    const size_t ib = (size_t)D.H.ysize * sizeof(double), ob = (size_t)D.H.osize * sizeof(double);
+    if (ib)
+      VMPS_HIP_CHECK(hipMemcpyAsync(D.vin.p, e->P.old.dev ? e->P.old.dev : e->P.old.data.data(), ib,
+                                    e->P.old.dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, D.stream));
+    D.hxv(D.vin.p, D.vout.p, D.stream);
+    if (dres) {
+      dout = D.vout.p;  // the plan gives up its output buffer
+      D.vout.p = nullptr;
+      D.vout.n = 0;
+    } else {
+      VMPS_HIP_CHECK(hipMemcpyAsync(e->result.data(), D.vout.p, ob, hipMemcpyDeviceToHost, D.stream));
+    }
+    VMPS_HIP_CHECK(hipStreamSynchronize(D.stream));
+  } else {
+    e->stats = vmps::Stats();
+  }
+  if (dres) {
+    dres->nq = e->P.sym.nq;
+    VMPS_HIP_CHECK(hipGetDevice(&dres->device));
+    if (!dout) {
+      VMPS_HIP_CHECK(hipMalloc((void**)&dout, std::max<int64_t>(1, osize) * sizeof(double)));
+      dres->buf = dout;
+      if (osize) VMPS_HIP_CHECK(hipMemset(dout, 0, osize * sizeof(double)));
+    }
+    dres->buf = dout;
+    vmps::Tripod& T = dres->T;
+    T.dev = dout;
+    T.dev_size = osize;
+    T.dev_device = dres->device;
+    for (size_t q = 0; q < e->P.new_index.size(); ++q) {
+      std::vector<vmps::DenseBlock> blocks((size_t)e->P.new_adim[q]);
+      for (int a = 0; a < e->P.new_adim[q]; ++a) {
+        const int64_t id = e->P.new_block[q][a];
+        if (id < 0) continue;
+        const vmps::BlockDim& b = e->S.Ob[id];
+        blocks[a].rows = b.rows;
+        blocks[a].cols = b.cols;
+        if (b.rows * b.cols > 0) blocks[a].off = b.off;
+      }
+      T.index.push_back(e->P.new_index[q]);
+      T.block.push_back(std::move(blocks));
+      T.dict[vmps::Key3{e->P.new_index[q][0], e->P.new_index[q][1], e->P.new_index[q][2]}] = q;
+    }
+  }
+  e->done = true;
+}
+
+// sector layout of T, and (data != NULL) its blocks packed in slot order
+void tripod_get(const vmps::Tripod& T, int nq, const double* src, int* qin, int* qout, int* qmid, int* adim,
+                int64_t* rows, int64_t* cols, double* data) {
+  int64_t slot = 0, pos = 0;
+  for (size_t q = 0; q < T.dim(); ++q) {
+    for (int i = 0; i < nq; ++i) {
+      qin[q * nq + i] = T.index[q][0][i];
+      qout[q * nq + i] = T.index[q][1][i];
+      qmid[q * nq + i] = T.index[q][2][i];
+    }
+    adim[q] = (int)T.block[q].size();
+    for (const vmps::DenseBlock& b : T.block[q]) {
+      rows[slot] = b.present() ? b.rows : 0;
+      cols[slot] = b.present() ? b.cols : 0;
+      if (data && b.present()) std::memcpy(data + pos, src + b.off, sizeof(double) * b.size());
+      if (b.present()) pos += b.size();
+      ++slot;
+    }
+  }
 }
 }  // namespace
 
@@ -115,7 +238,8 @@ vmps_status vmps_env_add_old(vmps_env* e, const int* qin, const int* qout, const
       src += rows[a] * cols[a];
     }
   }
-  add_old_sector(e, qin, qout, qmid, adim, rows, cols, ptrs.data());
+  add_old_sector(e->P.old, e->P.sym.nq, qin, qout, qmid, adim, rows, cols, ptrs.data());
+  e->done = false;
   VMPS_API_END
 }
 
@@ -182,22 +306,9 @@ vmps_status vmps_env_add_old_sectors(vmps_env* e, int nsec, const int* qin, cons
                                      const int* adim, const int64_t* rows, const int64_t* cols,
                                      const double* const* blocks) {
   VMPS_API_BEGIN
-  require(e && nsec >= 0 && (nsec == 0 || (qin && qout && qmid && adim)), VMPS_ERR_ARG, "bad argument");
-  const int nq = e->P.sym.nq;
-  {  // one reservation for all sectors
-    int64_t total = 0, nslot = 0;
-    for (int q = 0; q < nsec; ++q) nslot += std::max(0, adim[q]);
-    if (rows && cols)
-      for (int64_t i = 0; i < nslot; ++i) total += std::max<int64_t>(0, rows[i]) * std::max<int64_t>(0, cols[i]);
-    e->P.old.data.reserve(e->P.old.data.size() + (size_t)total);
-  }
-  int64_t slot = 0;
-  for (int q = 0; q < nsec; ++q) {
-    require(adim[q] >= 0, VMPS_ERR_ARG, "adim < 0");
-    require(adim[q] == 0 || (rows && cols && blocks), VMPS_ERR_ARG, "NULL rows/cols/blocks");
-    add_old_sector(e, qin + q * nq, qout + q * nq, qmid + q * nq, adim[q], rows + slot, cols + slot, blocks + slot);
-    slot += adim[q];
-  }
+  require(e != nullptr, VMPS_ERR_ARG, "NULL env");
+  add_old_sectors(e->P.old, e->P.sym.nq, nsec, qin, qout, qmid, adim, rows, cols, blocks);
+  e->done = false;
   VMPS_API_END
 }
 
@@ -234,22 +345,76 @@ vmps_status vmps_env_add_W_blocks(vmps_env* e, int nblk, const int* idx, const i
 
 vmps_status vmps_env_contract(vmps_env* e, vmps_precision prec) {
   VMPS_API_BEGIN
-  require(e != nullptr, VMPS_ERR_ARG, "NULL env");
-  require(prec == VMPS_PREC_FP64 || prec == VMPS_PREC_FP32 || prec == VMPS_PREC_BF16X3, VMPS_ERR_ARG,
-          "unknown precision");
-  vmps::plan_env(e->P, e->S);
-  e->result.assign(e->S.osize, 0.0);
-  if (!e->S.terms.empty() && e->S.osize > 0) {
-    vmps::DevPlan D;
-    D.create_triples(std::move(e->S), prec);  // the plan takes the packed operands; e->S keeps the layout
-    e->stats = D.H.stats;
-    const size_t ib = (size_t)D.H.ysize * sizeof(double), ob = (size_t)D.H.osize * sizeof(double);
-    if (ib) VMPS_HIP_CHECK(hipMemcpyAsync(D.vin.p, e->P.old.data.data(), ib, hipMemcpyHostToDevice, D.stream));
-    D.hxv(D.vin.p, D.vout.p, D.stream);
-    VMPS_HIP_CHECK(hipMemcpyAsync(e->result.data(), D.vout.p, ob, hipMemcpyDeviceToHost, D.stream));
-    VMPS_HIP_CHECK(hipStreamSynchronize(D.stream));
+  contract(e, prec, nullptr);
+  VMPS_API_END
+}
+
+vmps_status vmps_env_contract_device(vmps_env* e, vmps_precision prec, vmps_denv** out) {
+  VMPS_API_BEGIN
+  require(out != nullptr, VMPS_ERR_ARG, "out is NULL");
+  *out = nullptr;
+  auto d = std::make_unique<vmps_denv>();
+  contract(e, prec, d.get());
+  *out = d.release();
+  VMPS_API_END
+}
+
+vmps_status vmps_env_add_old_device(vmps_env* e, const vmps_denv* E) {
+  VMPS_API_BEGIN
+  require(e && E, VMPS_ERR_ARG, "NULL argument");
+  require(E->nq == e->P.sym.nq, VMPS_ERR_ARG, "symmetry mismatch");
+  require(e->P.old.dim() == 0 && !e->P.old.dev, VMPS_ERR_STATE, "the old environment already has sectors");
+  e->P.old = E->T;
+  e->done = false;
+  VMPS_API_END
+}
+
+vmps_status vmps_denv_create(int nq, int nsec, const int* qin, const int* qout, const int* qmid, const int* adim,
+                             const int64_t* rows, const int64_t* cols, const double* const* blocks,
+                             vmps_denv** out) {
+  VMPS_API_BEGIN
+  require(out != nullptr, VMPS_ERR_ARG, "out is NULL");
+  *out = nullptr;
+  require(nq >= 1 && nq <= vmps::kMaxNq, VMPS_ERR_ARG, "bad nq");
+  auto d = std::make_unique<vmps_denv>();
+  d->nq = nq;
+  add_old_sectors(d->T, nq, nsec, qin, qout, qmid, adim, rows, cols, blocks);
+  move_to_device(d.get());
+  d->T.dev_device = d->device;
+  *out = d.release();
+  VMPS_API_END
+}
+
+void vmps_denv_destroy(vmps_denv* d) { delete d; }
+
+vmps_status vmps_denv_size(const vmps_denv* d, int64_t* nsec, int64_t* nslots, int64_t* size) {
+  VMPS_API_BEGIN
+  require(d && nsec && nslots && size, VMPS_ERR_ARG, "NULL argument");
+  *nsec = (int64_t)d->T.dim();
+  int64_t ns = 0, n = 0;
+  for (const auto& blocks : d->T.block) {
+    ns += (int64_t)blocks.size();
+    for (const auto& b : blocks)
+      if (b.present()) n += b.size();
   }
-  e->done = true;
+  *nslots = ns;
+  *size = n;
+  VMPS_API_END
+}
+
+vmps_status vmps_denv_get(const vmps_denv* d, int* qin, int* qout, int* qmid, int* adim, int64_t* rows,
+                          int64_t* cols, double* data) {
+  VMPS_API_BEGIN
+  require(d && qin && qout && qmid && adim && rows && cols, VMPS_ERR_ARG, "NULL argument");
+  std::vector<double> host;
+  if (data && d->T.dev_size) {
+    int dev = -1;
+    VMPS_HIP_CHECK(hipGetDevice(&dev));
+    require(dev == d->device, VMPS_ERR_ARG, "the environment lives on another device");
+    host.resize((size_t)d->T.dev_size);
+    VMPS_HIP_CHECK(hipMemcpy(host.data(), d->buf, host.size() * sizeof(double), hipMemcpyDeviceToHost));
+  }
+  tripod_get(d->T, d->nq, host.data(), qin, qout, qmid, adim, rows, cols, data);
   VMPS_API_END
 }
 
@@ -270,6 +435,7 @@ vmps_status vmps_env_result_get(const vmps_env* e, int* qin, int* qout, int* qmi
   VMPS_API_BEGIN
   require(e && qin && qout && qmid && adim && rows && cols, VMPS_ERR_ARG, "NULL argument");
   require(e->done, VMPS_ERR_STATE, "vmps_env_contract has not run");
+  require(!e->result_on_device, VMPS_ERR_STATE, "the result was handed to a device environment");
   const int nq = e->P.sym.nq;
   int64_t slot = 0, pos = 0;
   for (size_t q = 0; q < e->P.new_index.size(); ++q) {

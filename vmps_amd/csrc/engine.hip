@@ -620,14 +620,42 @@ void DevPlan::create_triples(TripleSet&& S, vmps_precision precision) {
   upload(precision);
 }
 
+// Environment blocks resident on the device, copied into the packed operand
+// (one workgroup per block, strided over the grid).
+__global__ void k_gather_blocks(const Gather* g, int n, double* dst) {
+  for (int i = blockIdx.x; i < n; i += gridDim.x) {
+    const double* src = g[i].src;
+    double* d = dst + g[i].dst_off;
+    const int64_t len = g[i].n;
+    for (int64_t j = threadIdx.x; j < len; j += blockDim.x) d[j] = src[j];
+  }
+}
+
+namespace {
+void upload_env(DevBuf<double>& dst, const std::vector<double>& host, const std::vector<Gather>& g, int64_t n,
+                hipStream_t s) {
+  if (g.empty()) {
+    dst.upload(host, s);
+    return;
+  }
+  dst.alloc((size_t)n);
+  DevBuf<Gather> list;
+  list.upload(g, s);
+  const int nb = (int)g.size();
+  hipLaunchKernelGGL(k_gather_blocks, dim3(std::min(nb, 8192)), dim3(256), 0, s, list.p, nb, dst.p);
+  VMPS_HIP_CHECK(hipGetLastError());
+  VMPS_HIP_CHECK(hipStreamSynchronize(s));  // the list is freed on return
+}
+}  // namespace
+
 void DevPlan::upload(vmps_precision precision) {
   VMPS_HIP_CHECK(hipGetDevice(&device));
   VMPS_HIP_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device));
   stream = acquire_stream();
   VMPS_HIP_CHECK(hipEventCreate(&ev0));
   VMPS_HIP_CHECK(hipEventCreate(&ev1));
-  L.upload(H.Ldata, stream);
-  R.upload(H.Rdata, stream);
+  upload_env(L, H.Ldata, H.Lgather, H.L_size, stream);
+  upload_env(R, H.Rdata, H.Rgather, H.R_size, stream);
   T.alloc(H.T_size);
   vin.alloc(H.ysize);
   vout.alloc(H.osize);
@@ -672,8 +700,8 @@ void DevPlan::ensure_precision_buffers() {
     ++gen;
   }
   if (prec == VMPS_PREC_FP32 && !T32.p && (H.T_size || H.vsize)) {
-    L32.alloc(H.Ldata.size());
-    R32.alloc(H.Rdata.size());
+    L32.alloc(H.L_size);
+    R32.alloc(H.R_size);
     V32.alloc(H.ysize);
     T32.alloc(H.T_size);
     if (H.explicit_u) U32.alloc(H.U_size);

@@ -89,7 +89,7 @@ void plan_env(EnvProblem& P, TripleSet& S) {
       S.Yb.push_back({b.off, b.rows, b.cols});
     }
   }
-  S.ysize = (int64_t)P.old.data.size();
+  S.ysize = P.old.size();
 
   std::map<std::pair<size_t, size_t>, int64_t> xmap, zmap;  // (s, q) of A -> packed block id
   std::unordered_map<Key3, size_t, Key3Hash> sec;            // new sector -> index

@@ -516,8 +516,8 @@ void check_gemm(const std::vector<GemmProb>& probs, const std::vector<GemmChunk>
 }  // namespace
 
 void validate_host_plan(const HostPlan& H) {
-  check_gemm(H.probA, H.chunkA, (int64_t)H.Ldata.size(), H.ysize, H.T_size, "stage A");
-  check_gemm(H.probC, H.chunkC, H.U_size, (int64_t)H.Rdata.size(), H.osize, "stage C");
+  check_gemm(H.probA, H.chunkA, H.L_size, H.ysize, H.T_size, "stage A");
+  check_gemm(H.probC, H.chunkC, H.U_size, H.R_size, H.osize, "stage C");
   for (const CombineList& cl : H.comb) {
     if (cl.u_off < 0 || cl.size <= 0 || cl.u_off + cl.size > H.U_size || cl.begin < 0 || cl.end < cl.begin ||
         cl.end > (int32_t)H.comb_t.size() || cl.size > INT32_MAX)
@@ -743,7 +743,7 @@ void set_wave_edge(HostPlan& H, int edge, int big_min64) {
     for (const TileC& t : *list)
       for (int32_t c = t.chunk_begin; c < t.chunk_end; ++c) {
         if (c < 0 || c >= (int32_t)H.chunksC.size() || H.chunksC[c].r_off < 0 ||
-            H.chunksC[c].r_off + (int64_t)H.chunksC[c].K * t.N > (int64_t)H.Rdata.size())
+            H.chunksC[c].r_off + (int64_t)H.chunksC[c].K * t.N > H.R_size)
           throw VmpsError(VMPS_ERR_INTERNAL, "stage C: R block out of range");
         for (int32_t j = H.chunksC[c].term_begin; j < H.chunksC[c].term_end; ++j)
           if (H.termsC[j].t_off + (int64_t)t.M * H.chunksC[c].K > src_size)
@@ -751,7 +751,7 @@ void set_wave_edge(HostPlan& H, int edge, int big_min64) {
       }
   for (const auto* list : {&H.tilesA_w, &H.tilesA_b})
     for (const TileA& t : *list)
-      if (t.a_off < 0 || t.a_off + (int64_t)t.M * t.K > (int64_t)H.Ldata.size() || t.b_off < 0 ||
+      if (t.a_off < 0 || t.a_off + (int64_t)t.M * t.K > H.L_size || t.b_off < 0 ||
           t.b_off + (int64_t)t.K * t.N > H.ysize || t.c_off < 0 || t.c_off + (int64_t)t.M * t.N > H.T_size)
         throw VmpsError(VMPS_ERR_INTERNAL, "stage A: tile out of range");
 }
@@ -848,7 +848,7 @@ void build_tiles(HostPlan& H) {
   for (const auto* list : {&H.tilesC, &H.tilesC_w})
     for (const TileC& t : *list)
       for (int32_t c = t.chunk_begin; c < t.chunk_end; ++c) {
-        if (H.chunksC[c].r_off < 0 || H.chunksC[c].r_off + (int64_t)H.chunksC[c].K * t.N > (int64_t)H.Rdata.size())
+        if (H.chunksC[c].r_off < 0 || H.chunksC[c].r_off + (int64_t)H.chunksC[c].K * t.N > H.R_size)
           throw VmpsError(VMPS_ERR_INTERNAL, "stage C: R block out of range");
         for (int32_t j = H.chunksC[c].term_begin; j < H.chunksC[c].term_end; ++j)
           if (H.termsC[j].t_off + (int64_t)t.M * H.chunksC[c].K > src_size)
@@ -856,7 +856,7 @@ void build_tiles(HostPlan& H) {
       }
   for (const auto* list : {&H.tilesA, &H.tilesA_w})
     for (const TileA& t : *list)
-      if (t.a_off < 0 || t.a_off + (int64_t)t.M * t.K > (int64_t)H.Ldata.size() || t.b_off < 0 ||
+      if (t.a_off < 0 || t.a_off + (int64_t)t.M * t.K > H.L_size || t.b_off < 0 ||
           t.b_off + (int64_t)t.K * t.N > H.ysize || t.c_off < 0 || t.c_off + (int64_t)t.M * t.N > H.T_size)
         throw VmpsError(VMPS_ERR_INTERNAL, "stage A: tile operand out of range");
 }
@@ -868,15 +868,26 @@ void triples_from_pivot(const Pivot& P, TripleSet& S) {
   expand_terms(P, S.terms, lblocks, rblocks, vblocks);
   // gather the referenced environment blocks, packed in first-use order
   // (offsets first, then the copies in parallel)
+  // (offsets first, then the copies in parallel; environments resident on
+  // the device become device gathers, all of one side or none)
   auto gather = [&](const std::vector<EnvRef>& refs, bool left, std::vector<BlockDim>& dims,
-                    std::vector<double>& data) {
+                    std::vector<double>& data, std::vector<Gather>& dev, int64_t& size) {
     int64_t o = 0;
+    int ndev = 0;
     for (const auto& r : refs) {
       const Tripod& E = left ? P.terms[r.t].L : P.terms[r.t].R;
       const DenseBlock& b = E.block[r.q][r.a];
       dims.push_back({o, b.rows, b.cols});
+      if (E.dev) {
+        ++ndev;
+        if (b.size()) dev.push_back({E.dev + b.off, o, b.size()});
+      }
       o += b.size();
     }
+    size = o;
+    if (ndev && ndev != (int)refs.size())
+      throw VmpsError(VMPS_ERR_ARG, "the environments of one side must all be on the host or all on the device");
+    if (ndev) return;
     data.resize((size_t)o);
     parallel_for((int64_t)refs.size(), o, [&](int64_t i) {
       const auto& r = refs[(size_t)i];
@@ -885,8 +896,8 @@ void triples_from_pivot(const Pivot& P, TripleSet& S) {
       if (b.size()) std::memcpy(data.data() + dims[(size_t)i].off, E.data.data() + b.off, sizeof(double) * b.size());
     });
   };
-  gather(lblocks, true, S.Xb, S.Xdata);
-  gather(rblocks, false, S.Zb, S.Zdata);
+  gather(lblocks, true, S.Xb, S.Xdata, S.Xgather, S.Xsize);
+  gather(rblocks, false, S.Zb, S.Zdata, S.Zgather, S.Zsize);
   for (const auto& vb : vblocks) {
     const DenseBlock& b = P.V.block[vb.first][vb.second];
     S.Yb.push_back({b.off, b.rows, b.cols});
@@ -910,14 +921,22 @@ void build_plan(const TripleSet& S, HostPlan& H) {
   H = HostPlan();
   H.Ldata = S.Xdata;
   H.Rdata = S.Zdata;
+  H.Lgather = S.Xgather;
+  H.Rgather = S.Zgather;
+  H.L_size = S.Xgather.empty() ? (int64_t)S.Xdata.size() : S.Xsize;
+  H.R_size = S.Zgather.empty() ? (int64_t)S.Zdata.size() : S.Zsize;
   build_plan_tables(S, H);
 }
 
 // the environment data move into the plan; S keeps everything else
 void build_plan(TripleSet&& S, HostPlan& H) {
   H = HostPlan();
+  H.L_size = S.Xgather.empty() ? (int64_t)S.Xdata.size() : S.Xsize;
+  H.R_size = S.Zgather.empty() ? (int64_t)S.Zdata.size() : S.Zsize;
   H.Ldata = std::move(S.Xdata);
   H.Rdata = std::move(S.Zdata);
+  H.Lgather = std::move(S.Xgather);
+  H.Rgather = std::move(S.Zgather);
   build_plan_tables(S, H);
 }
 
@@ -1069,8 +1088,8 @@ void build_plan_tables(const TripleSet& S, HostPlan& H) {
   st.n_L = (int64_t)S.Xb.size();
   st.n_R = (int64_t)S.Zb.size();
   st.n_V = nO;
-  st.L_elems = (int64_t)H.Ldata.size();
-  st.R_elems = (int64_t)H.Rdata.size();
+  st.L_elems = H.L_size;
+  st.R_elems = H.R_size;
   st.V_elems = S.osize;
   st.T_elems = H.T_size;
   st.U_elems = H.U_size;

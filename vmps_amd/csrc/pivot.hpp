@@ -50,8 +50,14 @@ struct Tripod {
   std::vector<std::array<Qn, 3>> index;          // {in, out, mid}
   std::vector<std::vector<DenseBlock>> block;    // [q][a]
   std::vector<double> data;                      // packed FP64 blocks
+  // device-resident environment: the blocks' offsets refer to `dev` (FP64
+  // on the current device, owned by the caller) and `data` stays empty
+  const double* dev = nullptr;
+  int64_t dev_size = 0;
+  int dev_device = -1;
   std::unordered_map<Key3, size_t, Key3Hash> dict;
   size_t dim() const { return index.size(); }
+  int64_t size() const { return dev ? dev_size : (int64_t)data.size(); }  // packed elements
   const Qn& in(size_t q) const { return index[q][0]; }
   const Qn& out(size_t q) const { return index[q][1]; }
   const Qn& mid(size_t q) const { return index[q][2]; }
@@ -302,8 +308,18 @@ struct Term {
 // Z = R) and of the environment updates (X = A_bra^T, Y = L, Z = A_ket for
 // contract_L; X = A_ket, Y = R, Z = A_bra^T for contract_R).  X and Z are
 // packed column-major in product orientation (transposes already applied).
+// one device-to-device copy of a gather: n elements from src to dst_off
+struct Gather {
+  const double* src;
+  int64_t dst_off, n;
+};
+
 struct TripleSet {
   std::vector<double> Xdata, Zdata;
+  // environments resident on the device: the X / Z blocks are gathered there
+  // (Xdata / Zdata stay empty); Xsize / Zsize count the packed elements
+  std::vector<Gather> Xgather, Zgather;
+  int64_t Xsize = 0, Zsize = 0;
   std::vector<BlockDim> Xb, Zb;
   std::vector<BlockDim> Yb, Ob;  // blocks of the input and output vectors
   int64_t ysize = 0, osize = 0;
@@ -313,7 +329,9 @@ struct TripleSet {
 
 struct HostPlan {
   // packed operands actually referenced by terms
-  std::vector<double> Ldata, Rdata;  // X and Z blocks
+  std::vector<double> Ldata, Rdata;  // X and Z blocks (empty when gathered on the device)
+  std::vector<Gather> Lgather, Rgather;  // device gathers of device-resident environments
+  int64_t L_size = 0, R_size = 0;    // packed L / R elements
   std::vector<BlockDim> Lblk, Rblk, Vblk;  // blocks of Ldata, Rdata and the input vector
   int64_t vsize = 0;                 // output vector size (== input size for HxV plans)
   int64_t ysize = 0, osize = 0;      // input / output vector sizes

@@ -421,6 +421,9 @@ def _iarr(seq):
 def _feed_envs(fn, h, L, R, check, keep, nq=None, bulk=False):
     if bulk:
         for which, env in ((0, L), (1, R)):
+            if hasattr(env, "handle"):  # env.DeviceEnv: blocks already on the device
+                check(fn("pivot2_add_env_device")(h, which, env.handle))
+                continue
             pk = pack.tripod(env, nq)
             keep.append(pk)
             check(fn("pivot2_add_env_sectors")(h, which, pk.count, *pk.args()))
