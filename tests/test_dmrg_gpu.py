@@ -63,3 +63,16 @@ def test_gpu_dmrg_hubbard_su2xu1_matches_ed():
     e_ed = ed.hubbard_ground(6, U=8.0)
     r = dmrg.run_dmrg("HubbardSU2xU1", 6, 40, 2, backend=dmrg.GpuBackend(), params=dict(t=1.0, U=8.0))
     assert abs(r["energy"] - e_ed) <= 1e-10 * abs(e_ed)
+
+
+def test_device_environments_give_identical_sweeps():
+    """Environments kept on the device (the default) and environments copied
+    back to the host after every contraction give the same sweep, bit for bit."""
+    kw = dict(tol_eigval=1e-12, tol_state=1e-7)
+    print("case HubbardU1xU1 L=8 M=64, device vs host environments", flush=True)
+    a = dmrg.run_dmrg("HubbardU1xU1", 8, 64, 2, backend=dmrg.GpuBackend(device_envs=True, **kw),
+                      params=dict(t=1.0, U=8.0))
+    b = dmrg.run_dmrg("HubbardU1xU1", 8, 64, 2, backend=dmrg.GpuBackend(device_envs=False, **kw),
+                      params=dict(t=1.0, U=8.0))
+    assert a["energies"] == b["energies"]
+    assert a["truncation"] == b["truncation"]

@@ -363,11 +363,16 @@ class GpuBackend:
     split-BF16 HxV runs about twice as fast as FP64 (M = 2048 U(1) chains);
     below it the FP64 refinement (about a third more HxV per solve) costs more
     than the mixed iterations save (HeisenbergU1 L=32, M=1024: 3.00 vs 2.60 s
-    per sweep with a 0.5 GFLOP floor)."""
+    per sweep with a 0.5 GFLOP floor).
+
+    With `device_envs` (default) every environment the sweep contracts stays
+    on the device (env.DeviceEnv) and feeds the next contraction and pivot
+    without a host copy; the results are bit-identical to host environments."""
 
     def __init__(self, precision="fp64", refine_fp64=True, tol_eigval=1e-10, tol_state=1e-6, max_iter=100,
-                 mixed_min_flops=4e9, device_svd_min=0):
+                 mixed_min_flops=4e9, device_svd_min=0, device_envs=True):
         self.precision = precision
+        self.device_envs = device_envs
         self.refine = refine_fp64
         self.tol_eigval, self.tol_state, self.max_iter = tol_eigval, tol_state, max_iter
         self.mixed_min_flops = mixed_min_flops
@@ -406,12 +411,12 @@ class GpuBackend:
     def contract_L(self, site, Lold, A):
         from . import env
         blocks = _mps_to_blocks(A)
-        return env.contract_L(site, Lold, blocks, blocks)
+        return env.contract_L(site, Lold, blocks, blocks, device=self.device_envs)
 
     def contract_R(self, site, Rold, A):
         from . import env
         blocks = _mps_to_blocks(A)
-        return env.contract_R(site, Rold, blocks, blocks)
+        return env.contract_R(site, Rold, blocks, blocks, device=self.device_envs)
 
 
 def run_dmrg(model, L, M, sweeps, Q=None, seed=20261202, backend=None, params=None, verbose=False):
