@@ -16,6 +16,13 @@ from vmps_amd.pivot import HxV, PivotBatch, PivotMatrix0, PivotMatrix1, PivotMat
 sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
 import pivots  # noqa: E402
 from envs import random_env, random_site_tensor  # noqa: E402
+from test_su2_cpu import su2_pivot, su2_pivot1  # noqa: E402
+
+KINDS = dict(pivots.ALL_KINDS)
+KINDS["terms"] = lambda: pivots.with_terms(pivots.small_hubbard())
+KINDS["su2_2"] = su2_pivot
+KINDS["su2_1"] = su2_pivot1
+KINDS["su2_lr40"] = lambda: synth.make_workload("heisenberg_su2_lr40_L60_M1024", M=96)
 
 pytestmark = pytest.mark.gpu
 
@@ -62,9 +69,9 @@ def test_round_trip():
 
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32", "bf16x3"])
-@pytest.mark.parametrize("kind", ["heis2", "hub2", "hub1", "heis0", "terms"])
+@pytest.mark.parametrize("kind", ["heis2", "hub2", "hub1", "heis0", "terms", "su2_2", "su2_1", "su2_lr40"])
 def test_pivot_device_envs_match_host(kind, precision):
-    pd = pivots.with_terms(pivots.small_hubbard()) if kind == "terms" else pivots.ALL_KINDS[kind]()
+    pd = KINDS[kind]()
     print("case", kind, precision, "vsize", pd.vsize, flush=True)
     cls = MATRIX[pd.nsite]
     v = synth.random_vector(pd, 3)
