@@ -81,7 +81,7 @@ struct Krylov {
   hipGraphExec_t exec[3] = {nullptr, nullptr, nullptr};
   int graph_gen[3] = {-1, -1, -1};
   void ensure(DevPlan& D, int kmax);
-  void enqueue_iteration(DevPlan& D, hipStream_t s);
+  void enqueue_iteration(DevPlan& D, hipStream_t s, int kofs);  // iteration *kdev + kofs; *kdev is not advanced
   hipGraphExec_t ensure_graph(DevPlan& D);  // the graph of D.prec (kIterBatch iterations)
   void release_graph();
   Krylov() = default;

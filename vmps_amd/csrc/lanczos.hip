@@ -64,18 +64,21 @@ __global__ void k_fill_uniform(double* __restrict__ v, int64_t n, uint64_t seed)
 inline unsigned vgrid(int64_t n) { return (unsigned)std::max<int64_t>(1, std::min<int64_t>(2048, (n + 255) / 256)); }
 
 // ---- kernels of the graph-captured iteration ----------------------------------
-// The iteration index k lives in device memory (*kdev), so one captured graph
-// serves every iteration: the kernels read k instead of taking it as an argument.
+// The iteration index k lives in device memory: an iteration's kernels read
+// k = *kdev + kofs, where kofs is the iteration's place in a captured batch
+// (0 .. kIterBatch - 1), and one k_add_int after the batch advances *kdev.  One
+// captured graph thus serves every batch.
 
 // partial[i * nb + b] = sum over elements e of chunk b of ew[e] Q_i[e] w[e]
 // (ew: the coeff_dot weight of e's block), for the rows i = 4 y .. 4 y + 3
 // that are <= k; workgroup (b, y).  Groups past k exit at once, so the grid is
 // sized for the largest k and stays the same in every iteration.
-__global__ __launch_bounds__(256) void k_multidot_rows(const int* __restrict__ kdev, const double* __restrict__ Q,
-                                                       int64_t n, const double* __restrict__ w,
+__global__ __launch_bounds__(256) void k_multidot_rows(const int* __restrict__ kdev, int kofs,
+                                                       const double* __restrict__ Q, int64_t n,
+                                                       const double* __restrict__ w,
                                                        const double* __restrict__ ew, int64_t chunk,
                                                        double* __restrict__ partial, int nb) {
-  const int k = *kdev;
+  const int k = *kdev + kofs;
   const int i0 = 4 * blockIdx.y;
   if (i0 > k) return;  // workgroup-uniform
   __shared__ double red[4][256];
@@ -103,11 +106,11 @@ __global__ __launch_bounds__(256) void k_multidot_rows(const int* __restrict__ k
 }
 
 // c[i] = sum_b partial[i * nb + b] for i <= k; one workgroup per possible row.
-__global__ __launch_bounds__(256) void k_multidot_rows_final(const int* __restrict__ kdev,
+__global__ __launch_bounds__(256) void k_multidot_rows_final(const int* __restrict__ kdev, int kofs,
                                                              const double* __restrict__ partial, int nb,
                                                              double* __restrict__ c) {
   const int i = blockIdx.x;
-  if (i > *kdev) return;  // workgroup-uniform
+  if (i > *kdev + kofs) return;  // workgroup-uniform
   __shared__ double red[256];
   double acc = 0.0;
   for (int b = threadIdx.x; b < nb; b += 256) acc += partial[(int64_t)i * nb + b];
@@ -121,12 +124,13 @@ __global__ __launch_bounds__(256) void k_multidot_rows_final(const int* __restri
 }
 
 // w -= sum_{i<=k} c[i] Q_i; with norm_part, also the workgroup's share of the
-// weighted ||w||^2 of the result (summed by k_normalize_next and k_lanczos_scalars_next)
+// weighted ||w||^2 of the result (summed by k_normalize_next)
 __global__ __launch_bounds__(256) void k_sub_combination_dk(const double* __restrict__ Q, int64_t n,
                                                             const double* __restrict__ c, const int* __restrict__ kdev,
-                                                            double* __restrict__ w, const double* __restrict__ ew,
+                                                            int kofs, double* __restrict__ w,
+                                                            const double* __restrict__ ew,
                                                             double* __restrict__ norm_part) {
-  const int k = *kdev;
+  const int k = *kdev + kofs;
   int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   double nrm = 0.0;
@@ -157,11 +161,12 @@ constexpr int kFuseMaxGroups = 64;
 constexpr int kFuseMaxRows = 512;
 __global__ __launch_bounds__(256) void k_sub_combination_fused(const double* __restrict__ Q, int64_t n,
                                                                const double* __restrict__ partial, int nb,
-                                                               const int* __restrict__ kdev, double* __restrict__ c,
-                                                               double* __restrict__ w, const double* __restrict__ ew,
+                                                               const int* __restrict__ kdev, int kofs,
+                                                               double* __restrict__ c, double* __restrict__ w,
+                                                               const double* __restrict__ ew,
                                                                double* __restrict__ norm_part) {
   __shared__ double cs[kFuseMaxRows];
-  const int k = *kdev;
+  const int k = *kdev + kofs;
   for (int i = threadIdx.x; i <= k; i += 256) {
     double acc = 0.0;
     for (int b = 0; b < nb; ++b) acc += partial[(int64_t)i * nb + b];
@@ -204,34 +209,25 @@ __device__ inline double block_sum(const double* __restrict__ part, int np) {
   return red[0];
 }
 
-// the three numbers the host needs after iteration k (<q_k, H q_k> in two
-// Gram-Schmidt parts and ||w||^2) into out[4k ..], then k += 1.  Runs after
-// every other kernel of the iteration has read k.
-// ||w||^2 is the sum of the np norm partials (block_sum, the same order as in
-// k_normalize_next).  One workgroup of 256 threads.
-__global__ __launch_bounds__(256) void k_lanczos_scalars_next(int* __restrict__ kdev, const double* __restrict__ c1,
-                                                              const double* __restrict__ c2,
-                                                              const double* __restrict__ npart, int np,
-                                                              double* __restrict__ out) {
-  const int k = *kdev;
-  const double nrm2 = block_sum(npart, np);
-  __syncthreads();  // every thread has read k before it changes
-  if (threadIdx.x == 0) {
-    out[4 * k] = c1[k];
-    out[4 * k + 1] = c2[k];
-    out[4 * k + 2] = nrm2;
-    *kdev = k + 1;
-  }
-}
-
 // q_{k+1} = w / ||w|| into the basis and into the fixed HxV input vector v
 // (zero for a zero w: the iteration then ends on the breakdown test); every
-// workgroup sums the np norm partials itself (block_sum)
-__global__ __launch_bounds__(256) void k_normalize_next(const int* __restrict__ kdev, const double* __restrict__ npart,
-                                                        int np, const double* __restrict__ w, double* __restrict__ Q,
-                                                        int64_t n, double* __restrict__ v) {
-  const int k = *kdev;
+// workgroup sums the np norm partials itself (block_sum, fixed order).
+// Workgroup 0 also stores the three numbers the host needs after iteration k
+// (<q_k, H q_k> in two Gram-Schmidt parts c1[k] + c2[k], and ||w||^2) in
+// out[4k ..].
+__global__ __launch_bounds__(256) void k_normalize_next(const int* __restrict__ kdev, int kofs,
+                                                        const double* __restrict__ npart, int np,
+                                                        const double* __restrict__ w, double* __restrict__ Q,
+                                                        int64_t n, double* __restrict__ v,
+                                                        const double* __restrict__ c1,
+                                                        const double* __restrict__ c2, double* __restrict__ out) {
+  const int k = *kdev + kofs;
   const double nv = block_sum(npart, np);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    out[4 * k] = c1[k];
+    out[4 * k + 1] = c2[k];
+    out[4 * k + 2] = nv;
+  }
   const double f = nv > 0.0 ? 1.0 / sqrt(nv) : 0.0;
   double* __restrict__ q = Q + (int64_t)(k + 1) * n;
   int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -241,6 +237,11 @@ __global__ __launch_bounds__(256) void k_normalize_next(const int* __restrict__ 
     q[e] = y;
     v[e] = y;
   }
+}
+
+// *p += d, after every kernel of a batch has read *p
+__global__ void k_add_int(int* __restrict__ p, int d) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *p += d;
 }
 
 __global__ void k_set_int(int* __restrict__ p, int value) {
@@ -345,29 +346,28 @@ void dev_fill_uniform(double* v, int64_t n, uint64_t seed, hipStream_t s) {
 // is captured once per plan configuration as a HIP graph and replayed each
 // iteration; without it the same kernels are launched one by one.  Either way
 // the host waits once per iteration, for three numbers.
-void Krylov::enqueue_iteration(DevPlan& D, hipStream_t s) {
+void Krylov::enqueue_iteration(DevPlan& D, hipStream_t s, int kofs) {
   const int64_t n = D.H.vsize;
   const unsigned gy = (unsigned)((cap + 3) / 4);
   const unsigned gs = vgrid(n);
   const bool fused = ngrp <= kFuseMaxGroups && cap < kFuseMaxRows;
   auto pass = [&](double* c_out, double* norm_part) {
-    hipLaunchKernelGGL(k_multidot_rows, dim3(ngrp, gy), dim3(256), 0, s, kdev.p, Q.p, n, w.p, ew.p, chunk, gpart.p,
-                       ngrp);
+    hipLaunchKernelGGL(k_multidot_rows, dim3(ngrp, gy), dim3(256), 0, s, kdev.p, kofs, Q.p, n, w.p, ew.p, chunk,
+                       gpart.p, ngrp);
     if (fused) {
-      hipLaunchKernelGGL(k_sub_combination_fused, dim3(gs), dim3(256), 0, s, Q.p, n, gpart.p, ngrp, kdev.p, c_out,
-                         w.p, ew.p, norm_part);
+      hipLaunchKernelGGL(k_sub_combination_fused, dim3(gs), dim3(256), 0, s, Q.p, n, gpart.p, ngrp, kdev.p, kofs,
+                         c_out, w.p, ew.p, norm_part);
     } else {
-      hipLaunchKernelGGL(k_multidot_rows_final, dim3(cap), dim3(256), 0, s, kdev.p, gpart.p, ngrp, c_out);
-      hipLaunchKernelGGL(k_sub_combination_dk, dim3(gs), dim3(256), 0, s, Q.p, n, c_out, kdev.p, w.p, ew.p,
+      hipLaunchKernelGGL(k_multidot_rows_final, dim3(cap), dim3(256), 0, s, kdev.p, kofs, gpart.p, ngrp, c_out);
+      hipLaunchKernelGGL(k_sub_combination_dk, dim3(gs), dim3(256), 0, s, Q.p, n, c_out, kdev.p, kofs, w.p, ew.p,
                          norm_part);
     }
   };
   D.hxv(v.p, w.p, s);
   pass(c.p, nullptr);
   pass(c.p + cap, npart.p);
-  hipLaunchKernelGGL(k_normalize_next, dim3(gs), dim3(256), 0, s, kdev.p, npart.p, (int)gs, w.p, Q.p, n, v.p);
-  hipLaunchKernelGGL(k_lanczos_scalars_next, dim3(1), dim3(256), 0, s, kdev.p, c.p, c.p + cap, npart.p, (int)gs,
-                     out3.p);
+  hipLaunchKernelGGL(k_normalize_next, dim3(gs), dim3(256), 0, s, kdev.p, kofs, npart.p, (int)gs, w.p, Q.p, n, v.p,
+                     c.p, c.p + cap, out3.p);
   VMPS_HIP_CHECK(hipGetLastError());
 }
 
@@ -392,7 +392,8 @@ hipGraphExec_t Krylov::ensure_graph(DevPlan& D) {
   hipStream_t s = D.stream;
   VMPS_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
   try {
-    for (int i = 0; i < kIterBatch; ++i) enqueue_iteration(D, s);
+    for (int i = 0; i < kIterBatch; ++i) enqueue_iteration(D, s, i);
+    hipLaunchKernelGGL(k_add_int, dim3(1), dim3(64), 0, s, kdev.p, kIterBatch);
   } catch (...) {
     hipGraph_t g = nullptr;
     (void)hipStreamEndCapture(s, &g);
@@ -439,7 +440,8 @@ LanczosResult lanczos_solve(DevPlan& D, double* d_v, const LanczosOpts& o, vmps_
       VMPS_HIP_CHECK(hipGraphLaunch(iteration, s));
       nb = Krylov::kIterBatch;
     } else {
-      K.enqueue_iteration(D, s);
+      K.enqueue_iteration(D, s, 0);
+      hipLaunchKernelGGL(k_add_int, dim3(1), dim3(64), 0, s, K.kdev.p, 1);
     }
     res.hxv_calls += nb;
     VMPS_HIP_CHECK(hipMemcpyAsync(K.host3, K.out3.p + 4 * (size_t)k, 4 * nb * sizeof(double), hipMemcpyDeviceToHost,
