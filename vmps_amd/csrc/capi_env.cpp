@@ -133,7 +133,8 @@ void contract(vmps_env* e, vmps_precision prec, vmps_denv* dres) {
                                     e->P.old.dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, D.stream));
     D.hxv(D.vin.p, D.vout.p, D.stream);
     if (dres) {
-      dout = D.vout.p;  // the plan gives up its output buffer
+      dout = D.vout.p;  // the plan gives up its output buffer to dres
+      dres->buf = dout;
       D.vout.p = nullptr;
       D.vout.n = 0;
     } else {
@@ -146,12 +147,11 @@ void contract(vmps_env* e, vmps_precision prec, vmps_denv* dres) {
   if (dres) {
     dres->nq = e->P.sym.nq;
     VMPS_HIP_CHECK(hipGetDevice(&dres->device));
-    if (!dout) {
+    if (!dout) {  // nothing to contract: a zero environment of the result's layout
       VMPS_HIP_CHECK(hipMalloc((void**)&dout, std::max<int64_t>(1, osize) * sizeof(double)));
       dres->buf = dout;
       if (osize) VMPS_HIP_CHECK(hipMemset(dout, 0, osize * sizeof(double)));
     }
-    dres->buf = dout;
     vmps::Tripod& T = dres->T;
     T.dev = dout;
     T.dev_size = osize;
