@@ -85,15 +85,20 @@ def test_pivot_device_envs_match_host(kind, precision):
 
 
 @pytest.mark.parametrize("precision", ["fp64", "bf16x3"])
-def test_contract_device_matches_host(precision):
+@pytest.mark.parametrize("model", ["hub", "su2"])
+def test_contract_device_matches_host(model, precision):
     rng = np.random.default_rng(11)
-    site = models.hubbard_u1xu1_mpo()
-    left = synth.hubbard_bond(3, 20, alpha=1.0, lam=1.2)
-    right = synth.hubbard_bond(4, 20, alpha=1.0, lam=1.2)
+    if model == "hub":
+        site = models.hubbard_u1xu1_mpo()
+        left = synth.hubbard_bond(3, 20, alpha=1.0, lam=1.2)
+        right = synth.hubbard_bond(4, 20, alpha=1.0, lam=1.2)
+    else:  # SU(2): the contraction carries the recoupling factors
+        site = models.heisenberg_su2_mpo()
+        left, right = synth.heisenberg_su2_bond(5, 16), synth.heisenberg_su2_bond(6, 16)
     Abra = random_site_tensor(site, left, right, rng)
     Aket = random_site_tensor(site, left, right, rng)
     nq = site.sym.nq
-    print("case contract", precision, flush=True)
+    print("case contract", model, precision, flush=True)
     Lold = random_env(site, left, rng, "L")
     host = env.contract_L(site, Lold, Abra, Aket, precision=precision)
     for old in (Lold, env.DeviceEnv.from_host(Lold, nq)):
