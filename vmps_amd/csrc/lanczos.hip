@@ -63,6 +63,18 @@ __global__ void k_fill_uniform(double* __restrict__ v, int64_t n, uint64_t seed)
 
 inline unsigned vgrid(int64_t n) { return (unsigned)std::max<int64_t>(1, std::min<int64_t>(2048, (n + 255) / 256)); }
 
+// Sum over the 256 threads of a workgroup in a fixed order (shuffles within
+// each wave, then the four wave sums); every thread returns it.
+__device__ inline double wg_sum(double x) {
+  __shared__ double ws[4];
+  for (int m = 32; m > 0; m >>= 1) x += __shfl_xor(x, m);
+  if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = x;
+  __syncthreads();
+  const double r = (ws[0] + ws[1]) + (ws[2] + ws[3]);
+  __syncthreads();  // ws is free for the next call
+  return r;
+}
+
 // ---- kernels of the graph-captured iteration ----------------------------------
 // The iteration index k lives in device memory: an iteration's kernels read
 // k = *kdev + kofs, where kofs is the iteration's place in a captured batch
@@ -81,7 +93,7 @@ __global__ __launch_bounds__(256) void k_multidot_rows(const int* __restrict__ k
   const int k = *kdev + kofs;
   const int i0 = 4 * blockIdx.y;
   if (i0 > k) return;  // workgroup-uniform
-  __shared__ double red[4][256];
+  __shared__ double ws[4][4];  // [wave][row]
   const int nr = min(4, k + 1 - i0);
   const int b = blockIdx.x;
   const int64_t e0 = (int64_t)b * chunk, e1 = min(n, e0 + chunk);
@@ -94,15 +106,14 @@ __global__ __launch_bounds__(256) void k_multidot_rows(const int* __restrict__ k
       if (r < nr) acc[r] += Q[(int64_t)(i0 + r) * n + e] * we;
   }
 #pragma unroll
-  for (int r = 0; r < 4; ++r) red[r][threadIdx.x] = acc[r];
-  __syncthreads();
-  for (int h = 128; h > 0; h >>= 1) {
-    if ((int)threadIdx.x < h)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) red[r][threadIdx.x] += red[r][threadIdx.x + h];
-    __syncthreads();
+  for (int r = 0; r < 4; ++r) {
+    double x = acc[r];
+    for (int m = 32; m > 0; m >>= 1) x += __shfl_xor(x, m);
+    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6][r] = x;
   }
-  if ((int)threadIdx.x < nr) partial[(int64_t)(i0 + threadIdx.x) * nb + b] = red[threadIdx.x][0];
+  __syncthreads();
+  const int t = threadIdx.x;
+  if (t < nr) partial[(int64_t)(i0 + t) * nb + b] = (ws[0][t] + ws[1][t]) + (ws[2][t] + ws[3][t]);
 }
 
 // c[i] = sum_b partial[i * nb + b] for i <= k; one workgroup per possible row.
@@ -141,15 +152,9 @@ __global__ __launch_bounds__(256) void k_sub_combination_dk(const double* __rest
     w[e] = y;
     nrm += ew[e] * y * y;
   }
-  if (!norm_part) return;
-  __shared__ double red[256];
-  red[threadIdx.x] = nrm;
-  __syncthreads();
-  for (int h = 128; h > 0; h >>= 1) {
-    if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) norm_part[blockIdx.x] = red[0];
+  if (!norm_part) return;  // workgroup-uniform
+  const double sum = wg_sum(nrm);
+  if (threadIdx.x == 0) norm_part[blockIdx.x] = sum;
 }
 
 // k_multidot_rows_final and k_sub_combination_dk in one kernel, for short
@@ -184,29 +189,16 @@ __global__ __launch_bounds__(256) void k_sub_combination_fused(const double* __r
     w[e] = y;
     nrm += ew[e] * y * y;
   }
-  if (!norm_part) return;
-  __shared__ double red[256];
-  red[threadIdx.x] = nrm;
-  __syncthreads();
-  for (int h = 128; h > 0; h >>= 1) {
-    if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) norm_part[blockIdx.x] = red[0];
+  if (!norm_part) return;  // workgroup-uniform
+  const double sum = wg_sum(nrm);
+  if (threadIdx.x == 0) norm_part[blockIdx.x] = sum;
 }
 
 // sum of part[0 .. np-1] in one workgroup, fixed order; every thread returns it
 __device__ inline double block_sum(const double* __restrict__ part, int np) {
-  __shared__ double red[256];
   double acc = 0.0;
   for (int b = threadIdx.x; b < np; b += 256) acc += part[b];
-  red[threadIdx.x] = acc;
-  __syncthreads();
-  for (int h = 128; h > 0; h >>= 1) {
-    if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
-    __syncthreads();
-  }
-  return red[0];
+  return wg_sum(acc);
 }
 
 // q_{k+1} = w / ||w|| into the basis and into the fixed HxV input vector v
