@@ -67,10 +67,11 @@ def hubbard_ground(L, t=1.0, U=8.0, N=None, two_sz=0):
     return float(eigsh(Hs, k=1, which="SA", tol=1e-13)[0][0])
 
 
-def heisenberg_dense(L, J=1.0, S=0.5, Js=None):
+def heisenberg_dense(L, J=1.0, S=0.5, Js=None, Jmat=None):
     """Dense spin-S Heisenberg chain and total Sz, S^2 (basis: site 0 most
     significant).  Js = [J_1, J_2, ...] couples sites at distance d with J_d
-    (default: J at distance 1 only)."""
+    (default: J at distance 1 only); Jmat, an L x L matrix, couples every
+    pair (i, j) with Jmat[i, j] instead."""
     d = int(round(2 * S)) + 1
     m = np.arange(S, -S - 1, -1)[::-1]  # local Sz values, index 0 = lowest
     Sz = np.diag(m)
@@ -80,11 +81,13 @@ def heisenberg_dense(L, J=1.0, S=0.5, Js=None):
     Sm = Sp.T
     ops = lambda A, i: _op(A, i, L, d).toarray()
     H = np.zeros((d ** L, d ** L))
-    Js = [J] if Js is None else list(Js)
-    for dist, Jd in enumerate(Js, start=1):
-        for i in range(L - dist):
-            j = i + dist
-            H += Jd * (ops(Sz, i) @ ops(Sz, j) + 0.5 * (ops(Sp, i) @ ops(Sm, j) + ops(Sm, i) @ ops(Sp, j)))
+    if Jmat is not None:
+        pairs = [(i, j, Jmat[i][j]) for i in range(L) for j in range(i + 1, L) if Jmat[i][j] != 0.0]
+    else:
+        Js = [J] if Js is None else list(Js)
+        pairs = [(i, i + dist, Jd) for dist, Jd in enumerate(Js, start=1) for i in range(L - dist)]
+    for i, j, Jd in pairs:
+        H += Jd * (ops(Sz, i) @ ops(Sz, j) + 0.5 * (ops(Sp, i) @ ops(Sm, j) + ops(Sm, i) @ ops(Sp, j)))
     Szt = sum(ops(Sz, i) for i in range(L))
     Spt = sum(ops(Sp, i) for i in range(L))
     S2 = Spt.T @ Spt + Szt @ Szt + Szt  # S^2 = S-S+ + Sz^2 + Sz
@@ -98,10 +101,10 @@ def heisenberg_sector_spectrum(L, two_sz, J=1.0):
     return np.sort(np.linalg.eigvalsh(H[np.ix_(idx, idx)]))
 
 
-def heisenberg_multiplet_spectrum(L, S_tot, J=1.0, Js=None):
+def heisenberg_multiplet_spectrum(L, S_tot, J=1.0, Js=None, Jmat=None):
     """One eigenvalue per spin-S_tot multiplet: H restricted to the highest-weight
     states (Sz = S_tot and S^2 = S_tot(S_tot+1))."""
-    H, sz, S2 = heisenberg_dense(L, J, Js=Js)
+    H, sz, S2 = heisenberg_dense(L, J, Js=Js, Jmat=Jmat)
     idx = np.where(np.isclose(sz, S_tot))[0]
     w, U = np.linalg.eigh(S2[np.ix_(idx, idx)])
     P = U[:, np.isclose(w, S_tot * (S_tot + 1))]

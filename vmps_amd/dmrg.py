@@ -146,14 +146,94 @@ def random_mps(sites, Q, M, rng, spread=2, max_spin2=6):
     return A, bonds
 
 
-def right_normalize(A, bonds, qloc, sym=None):
+def su2_gaussian_bonds(sym, qloc, L, Q, M, width=0.5):
+    """Bond sectors of an SU(2) chain with at most M multiplets per bond: every
+    multiplet q = 2S+1 reachable from both ends, with counts ~ exp(-S^2 /
+    (2 sigma^2)), sigma = width * sqrt(min(b, L - b)), each capped by the
+    exact number of multiplets on either side of the bond (so the bonds
+    near the ends keep the full space)."""
+    assert sym.kinds == (models.SU2_KIND,)
+    bonds = []
+    for b in range(L + 1):
+        cl = left_counts(sym, qloc, b)
+        cr = right_counts(sym, qloc, L - b, Q)
+        caps = {q: min(c, cr[q]) for q, c in cl.items() if q in cr}
+        if sum(caps.values()) <= M:
+            bonds.append(dict(sorted(caps.items())))
+            continue
+        sigma = max(1.0, width * math.sqrt(min(b, L - b)))
+        qs = sorted(caps)
+        w = np.array([math.exp(-((q[0] - 1) / 2.0) ** 2 / (2 * sigma ** 2)) for q in qs])
+        cap = np.array([caps[q] for q in qs], dtype=np.float64)
+        lo, hi = 0.0, float(M) / w.min()
+        for _ in range(100):  # largest scale whose rounded, capped counts fit in M
+            mid = 0.5 * (lo + hi)
+            if np.minimum(cap, np.floor(mid * w + 0.5)).sum() <= M:
+                lo = mid
+            else:
+                hi = mid
+        dims = np.minimum(cap, np.floor(lo * w + 0.5)).astype(int)
+        bonds.append({q: int(d) for q, d in zip(qs, dims) if d > 0})
+    return bonds
+
+
+def random_blocks(sym, qloc, bonds, rng):
+    """N(0,1) site tensors for every block the bond sectors allow."""
+    A = []
+    for l in range(len(bonds) - 1):
+        site = []
+        for s in qloc:
+            blocks = {}
+            for qin, din in bonds[l].items():
+                for qout in sym.reduce(qin, s):
+                    if qout in bonds[l + 1]:
+                        blocks[(qin, qout)] = rng.standard_normal((din, bonds[l + 1][qout]))
+            site.append(blocks)
+        A.append(site)
+    return A
+
+
+def left_normalize(A, bonds, qloc, sym, stop):
+    """Left-to-right sweep of QR decompositions (Mps::leftSweep) over the
+    sites l < stop: each becomes left-normalized,
+    sum_{s, qin} A[s](qin,qout)^T A[s](qin,qout) = 1 (the normalisation the
+    left factor U of split_psi has), and its R factor moves into site l+1."""
+    for l in range(stop):
+        newA = [dict() for _ in qloc]
+        carry = {}
+        for qout, dout in bonds[l + 1].items():
+            parts = [(s, qin, A[l][s][(qin, qout)]) for s in range(len(qloc)) for qin in bonds[l]
+                     if (qin, qout) in A[l][s]]
+            if not parts:
+                continue
+            Mm = np.concatenate([p[2] for p in parts], axis=0)
+            q, r = np.linalg.qr(Mm)
+            r0 = 0
+            for (s, qin, blk) in parts:
+                newA[s][(qin, qout)] = q[r0:r0 + blk.shape[0], :]
+                r0 += blk.shape[0]
+            carry[qout] = r
+        A[l] = newA
+        bonds[l + 1] = {q: carry[q].shape[0] for q in carry}
+        for s in range(len(qloc)):
+            for key in list(A[l + 1][s].keys()):
+                qin, qout = key
+                if qin in carry:
+                    A[l + 1][s][key] = carry[qin] @ A[l + 1][s][key]
+                else:
+                    del A[l + 1][s][key]
+    return A, bonds
+
+
+def right_normalize(A, bonds, qloc, sym=None, stop=1):
     """Right-to-left sweep of QR decompositions (Mps::rightSweep): every site
-    but the first becomes right-normalized,
+    l >= stop becomes right-normalized,
     sum_{s, qout} (deg qout / deg qin) A[s](qin,qout) A[s](qin,qout)^T = 1
-    (deg = coeff_dot; the weight is 1 for U(1))."""
+    (deg = coeff_dot; the weight is 1 for U(1)), and its R factor moves into
+    site l-1."""
     L = len(A)
     sym = sym or models.Sym([models.U1_KIND] * len(qloc[0]))
-    for l in range(L - 1, 0, -1):
+    for l in range(L - 1, stop - 1, -1):
         newA = [dict() for _ in qloc]
         carry = {}
         for qin, din in bonds[l].items():
@@ -419,6 +499,108 @@ class GpuBackend:
         return env.contract_R(site, Rold, blocks, blocks, device=self.device_envs)
 
 
+class Chain:
+    """An MPS with its MPO and environments: A[l] per site (list over the
+    local index of {(qin, qout): block}), bonds[b] = {q: dim} for b = 0..L,
+    Lenv[b] / Renv[b] the environments on bond b (None where not built)."""
+
+    def __init__(self, sites, Q, M, A, bonds):
+        self.sites = sites
+        self.sym = sites[0].sym
+        self.qloc = sites[0].qloc
+        self.L = len(sites)
+        self.Q = tuple(Q)
+        self.M = M
+        self.A = A
+        self.bonds = bonds
+        vac = self.sym.vacuum()
+        self.Lenv = [None] * (self.L + 1)
+        self.Renv = [None] * (self.L + 1)
+        self.Lenv[0] = [(vac, vac, tuple(sites[0].aux_in[0][0]), [np.ones((1, 1))])]
+        self.Renv[self.L] = [(self.Q, self.Q, tuple(sites[-1].aux_out[0][0]), [np.ones((1, 1))])]
+
+    def copy(self):
+        """A second handle on the same state: steps replace A[l], bonds[b] and
+        environments rather than change them in place, so the lists are copied
+        and their entries shared."""
+        c = Chain.__new__(Chain)
+        c.__dict__.update(self.__dict__)
+        c.A = list(self.A)
+        c.bonds = [dict(b) for b in self.bonds]
+        c.Lenv = list(self.Lenv)
+        c.Renv = list(self.Renv)
+        return c
+
+    def pivot(self, l):
+        """Two-site pivot (l, l+1): PivotData with the environments Lenv[l],
+        Renv[l+2] and every wavefunction block the bonds allow."""
+        sym, sites, bonds = self.sym, self.sites, self.bonds
+        pd = synth.PivotData(sym, sites[l], sites[l + 1], sorted(bonds[l].items()), sorted(bonds[l + 2].items()))
+        pd.L = self.Lenv[l]
+        pd.R = self.Renv[l + 2]
+        pd.basis = synth.two_site_basis(sym, sites[l].qloc, sites[l + 1].qloc)
+        rdim = bonds[l + 2]
+        pd.V = []
+        for (qm, s1, s3) in pd.basis:
+            bl = []
+            for qin, din in sorted(bonds[l].items()):
+                for qout in sym.reduce(qin, qm):
+                    if qout in rdim:
+                        bl.append((qin, qout, din, rdim[qout]))
+            pd.V.append(bl)
+        return pd
+
+
+def center_chain(sites, Q, M, p, backend, rng, bonds=None):
+    """A random state prepared for two-site steps at the pivot (p, p+1): sites
+    left of p left-normalized, sites right of p+1 right-normalized, and the
+    environments Lenv[0..p] and Renv[p+2..L] contracted through the backend.
+    Bond sectors default to su2_gaussian_bonds (SU(2) chains)."""
+    sym, qloc, L = sites[0].sym, sites[0].qloc, len(sites)
+    bonds = bonds or su2_gaussian_bonds(sym, qloc, L, Q, M)
+    A = random_blocks(sym, qloc, bonds, rng)
+    A, bonds = left_normalize(A, bonds, qloc, sym, p)
+    A, bonds = right_normalize(A, bonds, qloc, sym, stop=p + 2)
+    chain = Chain(sites, Q, M, A, bonds)
+    for b in range(1, p + 1):
+        chain.Lenv[b] = backend.contract_L(sites[b - 1], chain.Lenv[b - 1], chain.A[b - 1])
+    for b in range(L - 1, p + 1, -1):
+        chain.Renv[b] = backend.contract_R(sites[b], chain.Renv[b + 1], chain.A[b])
+    return chain
+
+
+def dmrg_step(chain, backend, l, direction, rng=None, timings=None):
+    """One two-site DMRG step on the pivot (l, l+1) (DmrgSolver::iteration_two,
+    solvers/DmrgSolver.h:1398-1523): contract_AA2, lowest eigenpair of H_eff,
+    split_AA2 with truncation to chain.M, and the environment update in the
+    sweep direction ("right": Lenv[l+1], "left": Renv[l+1]).  Returns
+    (energy, truncation weight, solver info); `timings`, a dict, collects
+    the seconds of each phase."""
+    t = time.perf_counter
+    t0 = t()
+    pd = chain.pivot(l)
+    v0 = two_site_psi(pd, chain.A[l], chain.A[l + 1], chain.qloc)
+    if not np.any(v0):
+        v0 = (rng or np.random.default_rng(l)).uniform(-1, 1, pd.vsize)
+    t1 = t()
+    E, v, info = backend.solve(pd, v0)
+    t2 = t()
+    Al, Ar, tw, ent, kept = split_psi(pd, v, chain.qloc, chain.M, left_to_right=(direction == "right"),
+                                      svd=getattr(backend, "svd", None))
+    chain.A[l], chain.A[l + 1] = Al, Ar
+    chain.bonds[l + 1] = {qm: k for qm, k in kept.items()}
+    t3 = t()
+    if direction == "right":
+        chain.Lenv[l + 1] = backend.contract_L(chain.sites[l], chain.Lenv[l], chain.A[l])
+    else:
+        chain.Renv[l + 1] = backend.contract_R(chain.sites[l + 1], chain.Renv[l + 2], chain.A[l + 1])
+    t4 = t()
+    if timings is not None:
+        for key, dt in (("psi", t1 - t0), ("solve", t2 - t1), ("split", t3 - t2), ("env", t4 - t3)):
+            timings[key] = timings.get(key, 0.0) + dt
+    return E, tw, info
+
+
 def run_dmrg(model, L, M, sweeps, Q=None, seed=20261202, backend=None, params=None, verbose=False):
     """Ground state by two-site DMRG.  Returns a dict with energies per half
     sweep, the final energy, the truncation weights and timings."""
@@ -429,19 +611,14 @@ def run_dmrg(model, L, M, sweeps, Q=None, seed=20261202, backend=None, params=No
     sym = sites[0].sym
     if Q is None:
         Q = {"HeisenbergU1": (0,), "HubbardU1xU1": (L, 0), "HeisenbergSU2": (1,), "HubbardSU2xU1": (1, L),
-             "HeisenbergSU2LR": (1,), "HubbardSU2xU1Ladder": (1, L)}[model]
+             "HeisenbergSU2LR": (1,), "HeisenbergSU2Full": (1,), "HubbardSU2xU1Ladder": (1, L)}[model]
     Q = tuple(Q)
     rng = np.random.default_rng(seed)
     A, bonds = random_mps(sites, Q, M, rng)
     A, bonds = right_normalize(A, bonds, qloc, sym)
-    vac = sym.vacuum()
-    # environments: L[l] sits on bond l, R[l] on bond l
-    Lenv = [None] * (L + 1)
-    Renv = [None] * (L + 1)
-    Lenv[0] = [(vac, vac, tuple(sites[0].aux_in[0][0]), [np.ones((1, 1))])]
-    Renv[L] = [(Q, Q, tuple(sites[-1].aux_out[0][0]), [np.ones((1, 1))])]
+    chain = Chain(sites, Q, M, A, bonds)
     for l in range(L - 1, 1, -1):
-        Renv[l] = backend.contract_R(sites[l], Renv[l + 1], A[l])
+        chain.Renv[l] = backend.contract_R(sites[l], chain.Renv[l + 1], chain.A[l])
     energies, truncs = [], []
     t_sweeps = []
     E = None
@@ -450,38 +627,11 @@ def run_dmrg(model, L, M, sweeps, Q=None, seed=20261202, backend=None, params=No
         for direction in ("right", "left"):
             ls = range(0, L - 1) if direction == "right" else range(L - 2, -1, -1)
             for l in ls:
-                pd = synth.PivotData(sites[0].sym, sites[l], sites[l + 1],
-                                     sorted(bonds[l].items()), sorted(bonds[l + 2].items()))
-                pd.L = Lenv[l]
-                pd.R = Renv[l + 2]
-                pd.basis = synth.two_site_basis(pd.sym, sites[l].qloc, sites[l + 1].qloc)
-                rdim = bonds[l + 2]
-                pd.V = []
-                for (qm, s1, s3) in pd.basis:
-                    bl = []
-                    for qin, din in sorted(bonds[l].items()):
-                        for qout in sym.reduce(qin, qm):
-                            if qout in rdim:
-                                bl.append((qin, qout, din, rdim[qout]))
-                    pd.V.append(bl)
-                v0 = two_site_psi(pd, A[l], A[l + 1], qloc)
-                if not np.any(v0):
-                    v0 = rng.uniform(-1, 1, pd.vsize)
-                E, v, _ = backend.solve(pd, v0)
-                Al, Ar, tw, ent, kept = split_psi(pd, v, qloc, M, left_to_right=(direction == "right"),
-                                                  svd=getattr(backend, "svd", None))
-                A[l], A[l + 1] = Al, Ar
-                bonds[l + 1] = {}
-                for qm, k in kept.items():
-                    bonds[l + 1][qm] = k
+                E, tw, _ = dmrg_step(chain, backend, l, direction, rng=rng)
                 truncs.append(tw)
-                if direction == "right":
-                    Lenv[l + 1] = backend.contract_L(sites[l], Lenv[l], A[l])
-                else:
-                    Renv[l + 1] = backend.contract_R(sites[l + 1], Renv[l + 2], A[l + 1])
             energies.append(E)
             if verbose:
                 print("sweep %d %s  E = %.12f  max trunc = %.2e" % (sweep, direction, E, max(truncs[-(L - 1):])))
         t_sweeps.append(time.perf_counter() - t0)
     return {"energy": E, "energies": energies, "truncation": truncs, "sweep_seconds": t_sweeps,
-            "bond_dims": [sum(b.values()) for b in bonds]}
+            "bond_dims": [sum(b.values()) for b in chain.bonds]}

@@ -14,6 +14,8 @@ Models:
   HubbardU1xU1   H = -t sum_sigma (c+ c + h.c.) + U sum n_up n_dn (models/HubbardU1xU1.h)
   HeisenbergSU2  S=1/2, H = J sum S.S in SU(2)-reduced form          (models/HeisenbergSU2.h)
   HeisenbergSU2LR  H = sum_d J_d sum_i S_i.S_{i+d}, range R, D_W = R + 2
+  HeisenbergSU2Full  H = sum_{i<j} J_ij S_i.S_j for any coupling matrix,
+                   D_W = 2 + rank of the couplings across each bond
   HubbardSU2xU1 / HubbardSU2xU1Ladder  chain / snake-ordered two-leg ladder
 """
 from dataclasses import dataclass, field
@@ -362,6 +364,91 @@ def heisenberg_su2_lr_mpo(Js, position="bulk", S=0.5):
     return _place(SU2, qloc, q, ent, position)
 
 
+def coupling_channels(J, tol=1e-12):
+    """Operator channels of a two-body Hamiltonian sum_{i<j} J_ij O_i . O_j
+    on an open chain, at the minimal count per bond.
+
+    At bond b (between sites b-1 and b) the terms that cross it are
+    sum_{i<b<=j} J_ij O_i O_j = sum_k Ohat_k P_k with Ohat = Q_b^T O_{0:b},
+    where the orthonormal columns of Q_b span the column space of
+    C_b = J[0:b, b:L]; their number r_b = rank(C_b) is the least any MPO can
+    carry (the reference reaches it by deleting linearly dependent rows and
+    columns, MpoTerms.h:1438-1560).  Since the first b-1 rows of C_{b+1} are
+    columns of C_b, Q_{b+1}[0:b] = Q_b T_b with T_b = Q_b^T Q_{b+1}[0:b].
+
+    Returns per site s: (Q_{s+1}[s], T_s, close_s) with
+      start -> channel k at bond s+1 : Q_{s+1}[s, k]   (the operator placed at s)
+      channel k' -> channel k        : T_s[k', k]       (passed on with the identity)
+      channel k' -> done             : close_s[k'] = (Q_s^T J[0:s, s])[k']."""
+    J = np.asarray(J, dtype=np.float64)
+    L = J.shape[0]
+    assert J.shape == (L, L) and np.allclose(J, J.T), "J must be a symmetric L x L matrix"
+    Q = [np.zeros((0, 0))]
+    for b in range(1, L):
+        C = J[:b, b:]
+        U, s, _ = np.linalg.svd(C, full_matrices=False)
+        r = int(np.sum(s > tol * max(1.0, s[0] if len(s) else 0.0)))
+        Q.append(U[:, :r])
+    Q.append(np.zeros((L, 0)))
+    out = []
+    for s in range(L):
+        Qs, Qn = Q[s], Q[s + 1]
+        start = Qn[s, :] if Qn.shape[1] else np.zeros(0)
+        T = Qs.T @ Qn[:s, :] if s > 0 else np.zeros((0, Qn.shape[1]))
+        close = Qs.T @ J[:s, s] if s > 0 else np.zeros(0)
+        out.append((start, T, close))
+    return out
+
+
+def heisenberg_su2_full_mpo(J, S=0.5, tol=1e-14):
+    """H = sum_{i<j} J_ij S_i . S_j for an arbitrary coupling matrix (the
+    HeisenbergSU2 'Jfull' Hamiltonian, models/HeisenbergSU2.h:223-229) in
+    SU(2)-reduced form, one MpoSite per site.  Auxiliary order at bond b:
+    0 done | 1..r_b spin-1 channels | r_b + 1 start, with r_b from
+    coupling_channels (D_W = r_b + 2 multiplets).  The first site keeps the
+    start row only, the last the done column only.  The channel couplings
+    mirror heisenberg_su2_lr_mpo: S opens a channel with sqrt(3) (-red),
+    closes it with J red."""
+    L = J.shape[0]
+    q2 = int(round(2 * S)) + 1
+    red = np.array([[np.sqrt(S * (S + 1.0))]])
+    Id = np.eye(1)
+    qloc = [(q2,)]
+    ch = coupling_channels(J)
+    r = [0] + [len(ch[s][0]) for s in range(L - 1)] + [0]  # channels at bond b
+
+    def aux(b):
+        if b == 0:
+            return [(1,)]                       # start only
+        if b == L:
+            return [(1,)]                       # done only
+        return [(1,)] + [(3,)] * r[b] + [(1,)]
+
+    sites = []
+    for s in range(L):
+        start, T, close = ch[s]
+        qin, qout = aux(s), aux(s + 1)
+        i_start = 0 if s == 0 else r[s] + 1
+        o_start = None if s == L - 1 else r[s + 1] + 1
+        ent = []
+        if s > 0:
+            ent.append((0, 0, Id, (1,)))                                   # done -> done
+            for k in range(r[s]):
+                if abs(close[k]) > tol:
+                    ent.append((1 + k, 0, close[k] * red, (3,)))           # close a channel
+        if o_start is not None:
+            ent.append((i_start, o_start, Id, (1,)))                        # start -> start
+            for k in range(r[s + 1]):
+                if abs(start[k]) > tol:
+                    ent.append((i_start, 1 + k, start[k] * np.sqrt(3.0) * (-red), (3,)))  # open
+            for kp in range(r[s]):
+                for k in range(r[s + 1]):
+                    if abs(T[kp, k]) > tol:
+                        ent.append((1 + kp, 1 + k, T[kp, k] * Id, (1,)))   # pass on
+        sites.append(build_mpo_site(SU2, qloc, qin, qout, ent))
+    return sites
+
+
 def hubbard_su2xu1_ladder_mpo(t=1.0, U=6.0, rung=True, position="bulk"):
     """Two-leg Hubbard ladder in snake order (site 2x: leg 0, site 2x+1: leg 1
     of column x), spin SU(2) x charge U(1), reduced form (HubbardSU2xU1 with
@@ -431,6 +518,10 @@ def _place(sym, qloc, q, ent, position, conj=None):
 
 def chain_mpo(model, L, **params):
     """Full open-boundary MPO as a list of MpoSite."""
+    if model == "HeisenbergSU2Full":
+        J = np.asarray(params["J"], dtype=np.float64)
+        assert J.shape == (L, L)
+        return heisenberg_su2_full_mpo(J, **{k: v for k, v in params.items() if k != "J"})
     if model in ("HubbardSU2xU1Ladder", "HeisenbergSU2LR"):
         def f(i, position):
             if model == "HubbardSU2xU1Ladder":
