@@ -25,11 +25,16 @@
 // device HxV and copies the result into Vout's blocks.
 //
 // Plan reuse: a plan is cached per calling thread, keyed by the pivot's
-// address and a fingerprint of its contents (block shapes, data pointers and
-// first/last element of every environment block, layout of Vin).  A caller
-// that rebuilds the pivot in place therefore gets a new plan; a caller that
-// changes environment values in place without touching their first or last
-// element must call vmps_adapter::invalidate().
+// address and a fingerprint of its contents: block shapes, the layout of Vin,
+// the MPO nonzeros and block-structure tables, and every value of every
+// environment block (a 64-bit hash over all of them, read once per call at
+// ~10 GB/s on one core).  A pivot whose values change, in place or by being
+// rebuilt at the same address, therefore gets a new plan.  A caller that
+// guarantees it never changes environment values behind a pivot's back (the
+// reference's solvers build a new PivotMatrix per step) may call
+// vmps_adapter::trust_pointers(true): environment blocks then enter the
+// fingerprint by address, shape and first/last element only, which costs
+// nothing per call; vmps_adapter::invalidate() drops cached plans explicitly.
 //
 // Errors from the library are thrown as std::runtime_error with the
 // library's message (the reference's HxV is void and has no status).
@@ -103,6 +108,14 @@ inline std::unordered_map<const void*, std::unique_ptr<PlanEntry>>& plan_cache()
   return cache;
 }
 
+// Fingerprint environments by address (true) or by content (false, default);
+// per calling thread.
+inline bool& trust_pointers_flag() {
+  thread_local bool flag = false;
+  return flag;
+}
+inline void trust_pointers(bool on) { trust_pointers_flag() = on; }
+
 // Drop the cached plans of the calling thread (all, or the one of `pivot`).
 inline void invalidate(const void* pivot = nullptr) {
   if (pivot)
@@ -121,6 +134,27 @@ struct Fnv {
   void add(const T& x) {
     add(&x, sizeof(T));
   }
+  // every value of n doubles: four independent multiply-xorshift lanes over
+  // the 64-bit patterns, folded into h (order- and position-sensitive)
+  void add_values(const double* d, size_t n) {
+    uint64_t a[4] = {0x9E3779B97F4A7C15ull, 0xC2B2AE3D27D4EB4Full, 0x165667B19E3779F9ull, 0x27D4EB2F165667C5ull};
+    size_t i = 0;
+    for (; i + 4 <= n; i += 4)
+      for (int j = 0; j < 4; ++j) {
+        uint64_t w;
+        std::memcpy(&w, d + i + j, 8);
+        a[j] = (a[j] ^ w) * 0xFF51AFD7ED558CCDull;
+        a[j] ^= a[j] >> 29;
+      }
+    for (; i < n; ++i) {
+      uint64_t w;
+      std::memcpy(&w, d + i, 8);
+      a[0] = (a[0] ^ w) * 0xFF51AFD7ED558CCDull;
+      a[0] ^= a[0] >> 29;
+    }
+    add(a);
+    add(n);
+  }
 };
 
 template <typename TripodT>
@@ -136,10 +170,69 @@ void fingerprint_env(Fnv& f, const TripodT& E) {
       f.add(c);
       if (M.size() > 0) {
         const double* d = M.data();
-        f.add(d);
-        f.add(d[0]);
-        f.add(d[M.size() - 1]);
+        if (trust_pointers_flag()) {
+          f.add(d);
+          f.add(d[0]);
+          f.add(d[M.size() - 1]);
+        } else {
+          f.add_values(d, static_cast<size_t>(M.size()));
+        }
       }
+    }
+  }
+}
+
+// W[s1][s2][k]: shapes and every nonzero (by address only when pointers are trusted)
+template <typename WT>
+void fingerprint_W(Fnv& f, const WT& W) {
+  if (trust_pointers_flag()) {
+    f.add(&W);
+    return;
+  }
+  f.add(W.size());
+  for (const auto& W1 : W)
+    for (const auto& W2 : W1)
+      for (const auto& B : W2) {
+        f.add(B.dim);
+        for (size_t q = 0; q < B.dim; ++q) {
+          const auto& M = B.block[q];
+          using SM = typename std::decay<decltype(M)>::type;
+          f.add(static_cast<int64_t>(M.rows()));
+          f.add(static_cast<int64_t>(M.cols()));
+          for (int o = 0; o < M.outerSize(); ++o)
+            for (typename SM::InnerIterator it(M, o); it; ++it) {
+              f.add(static_cast<int64_t>(it.row()));
+              f.add(static_cast<int64_t>(it.col()));
+              const double v = it.value();
+              f.add(v);
+            }
+        }
+      }
+}
+
+// qlhs / qrhs / factor_cgcs, A0proj and Epenalty of a pivot
+template <typename HT>
+void fingerprint_tables(Fnv& f, const HT& H) {
+  f.add(H.qlhs.size());
+  f.add(H.A0proj.size());
+  const double ep = H.Epenalty;
+  f.add(ep);
+  if (trust_pointers_flag()) return;
+  for (size_t i = 0; i < H.qlhs.size(); ++i) {
+    f.add(static_cast<uint64_t>(H.qlhs[i][0]));
+    f.add(static_cast<uint64_t>(H.qlhs[i][1]));
+    f.add(H.qrhs[i].size());
+    for (size_t j = 0; j < H.qrhs[i].size(); ++j) {
+      for (size_t c = 0; c < H.qrhs[i][j].size(); ++c) f.add(static_cast<uint64_t>(H.qrhs[i][j][c]));
+      const double x = H.factor_cgcs[i][j];
+      f.add(x);
+    }
+  }
+  for (const auto& P : H.A0proj) {
+    f.add(P.size());
+    for (const auto& B : P) {
+      f.add(B.dim);
+      for (size_t r = 0; r < B.dim; ++r) f.add_values(B.block[r].data(), static_cast<size_t>(B.block[r].size()));
     }
   }
 }
@@ -342,12 +435,11 @@ void OxV(const PivotMatrix2<Symmetry, double, double>& H, const PivotVector<Symm
   for (const auto& t : H.Terms) {
     A::fingerprint_env(f, t.L);
     A::fingerprint_env(f, t.R);
-    f.add(&t.W12);
-    f.add(&t.W34);
+    A::fingerprint_W(f, t.W12);
+    A::fingerprint_W(f, t.W34);
   }
   A::fingerprint_layout(f, Vin);
-  f.add(H.qlhs.size());
-  f.add(H.A0proj.size());
+  A::fingerprint_tables(f, H);
   auto& e = A::plan_for(&H, f.h, Vin, [&](A::PivotHandle& h) {
     const auto& t0 = H.Terms.at(0);
     const auto k = A::kinds<Symmetry>();
@@ -394,11 +486,10 @@ void OxV(const PivotMatrix1<Symmetry, double, double>& H, const PivotVector<Symm
   for (const auto& t : H.Terms) {
     A::fingerprint_env(f, t.L);
     A::fingerprint_env(f, t.R);
-    f.add(&t.W);
+    A::fingerprint_W(f, t.W);
   }
   A::fingerprint_layout(f, Vin);
-  f.add(H.qlhs.size());
-  f.add(H.A0proj.size());
+  A::fingerprint_tables(f, H);
   auto& e = A::plan_for(&H, f.h, Vin, [&](A::PivotHandle& h) {
     const auto& t0 = H.Terms.at(0);
     const auto k = A::kinds<Symmetry>();
@@ -567,22 +658,29 @@ void contract_env(int which, const TripodT& Eold, const AV& Abra, const WT& W, c
 
 }  // namespace vmps_adapter
 
-template <typename Symmetry, typename MatrixType, typename MpoMatrixType>
+// The seven-argument contract_L / contract_R of the reference
+// (tensors/DmrgContractions.h:34-44, 179-189) for real double blocks.  The
+// environment and the MPS blocks share one matrix type here, which makes
+// these templates more specialised than the reference's (where the Tripod's
+// MatrixType2 is free); the enable_if removes them for any other scalar, so
+// the complex calls of TDVP (solvers/TDVPPropagator.h:876-897) and the
+// reference's extra-argument forms still bind the generic templates.
+template <typename Symmetry, typename MatrixType, typename MpoMatrixType,
+          typename std::enable_if<std::is_same<typename MatrixType::Scalar, double>::value, int>::type = 0>
 void contract_L(const Tripod<Symmetry, MatrixType>& Lold, const std::vector<Biped<Symmetry, MatrixType>>& Abra,
                 const std::vector<std::vector<std::vector<Biped<Symmetry, MpoMatrixType>>>>& W,
                 const std::vector<Biped<Symmetry, MatrixType>>& Aket,
                 const std::vector<qarray<Symmetry::Nq>>& qloc, const std::vector<qarray<Symmetry::Nq>>& qOp,
                 Tripod<Symmetry, MatrixType>& Lnew) {
-  static_assert(std::is_same<typename MatrixType::Scalar, double>::value, "the device contraction is FP64");
   vmps_adapter::contract_env<Symmetry>(0, Lold, Abra, W, Aket, qloc, qOp, Lnew);
 }
 
-template <typename Symmetry, typename MatrixType, typename MpoMatrixType>
+template <typename Symmetry, typename MatrixType, typename MpoMatrixType,
+          typename std::enable_if<std::is_same<typename MatrixType::Scalar, double>::value, int>::type = 0>
 void contract_R(const Tripod<Symmetry, MatrixType>& Rold, const std::vector<Biped<Symmetry, MatrixType>>& Abra,
                 const std::vector<std::vector<std::vector<Biped<Symmetry, MpoMatrixType>>>>& W,
                 const std::vector<Biped<Symmetry, MatrixType>>& Aket,
                 const std::vector<qarray<Symmetry::Nq>>& qloc, const std::vector<qarray<Symmetry::Nq>>& qOp,
                 Tripod<Symmetry, MatrixType>& Rnew) {
-  static_assert(std::is_same<typename MatrixType::Scalar, double>::value, "the device contraction is FP64");
   vmps_adapter::contract_env<Symmetry>(1, Rold, Abra, W, Aket, qloc, qOp, Rnew);
 }

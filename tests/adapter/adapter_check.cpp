@@ -9,6 +9,7 @@
 // calls contract_L / contract_R and writes the new environment.  Exit status: 0 on
 // success, 2 when the generic (non-adapter) template was selected, 3 when the
 // adapter reported a library error (message on stderr), 1 on bad input.
+#include <cstdlib>
 #include <cstdio>
 #include <fstream>
 #include <iostream>
@@ -223,6 +224,40 @@ void run(Reader& r, int nsite, const char* outpath) {
     read_bs<decltype(H), 12>(r, H);
     HxV(H, Vin, Vout);
     HxV(H, Vin, Vout);  // second call reuses the cached plan
+    if (std::getenv("VMPS_CHECK_INPLACE")) {
+      // change one interior value of an environment block in place: the
+      // cached plan must not be served, the result must equal a fresh plan's
+      bool changed = false;
+      for (auto& T : H.Terms) {
+        for (size_t q = 0; q < T.L.dim && !changed; ++q)
+          for (size_t a = 0; a < T.L.block[q].shape()[0] && !changed; ++a) {
+            auto& M = T.L.block[q][a][0];
+            if (M.size() >= 3) {
+              M.v[M.size() / 2] += 0.5;
+              changed = true;
+            }
+          }
+        if (changed) break;
+      }
+      if (!changed) throw std::runtime_error("no environment block with three entries");
+      auto before = Vout;
+      HxV(H, Vin, Vout);
+      vmps_adapter::invalidate();
+      PivotVector<Sym, double> fresh;
+      HxV(H, Vin, fresh);
+      bool same_as_fresh = true, same_as_before = true;
+      for (size_t s = 0; s < Vout.data.size(); ++s)
+        for (size_t q = 0; q < Vout.data[s].dim; ++q)
+          for (int64_t i = 0; i < Vout.data[s].block[q].size(); ++i) {
+            const double x = Vout.data[s].block[q].v[i];
+            if (x != fresh.data[s].block[q].v[i]) same_as_fresh = false;
+            if (x != before.data[s].block[q].v[i]) same_as_before = false;
+          }
+      if (!same_as_fresh || same_as_before) {
+        std::cerr << "stale plan after an in-place environment change\n";
+        std::exit(4);
+      }
+    }
   } else if (nsite == 1) {
     PivotMatrix1<Sym, double, double> H;
     auto qloc = rqs<Sym>(r);
@@ -252,7 +287,30 @@ void run(Reader& r, int nsite, const char* outpath) {
 }
 }  // namespace
 
+// The complex calls of TDVP (solvers/TDVPPropagator.h:876-897) must bind the
+// reference's generic contract_L / contract_R, not the FP64 adapter.
+int complex_dispatch() {
+  using Sym = MockSym<1, S>;
+  Tripod<Sym, MatrixXcd> Eold, Enew;
+  std::vector<Biped<Sym, MatrixXcd>> A;
+  WTensor<Sym> W;
+  std::vector<qarray<1>> qloc, qOp;
+  int generic = 0;
+  try {
+    contract_L(Eold, A, W, A, qloc, qOp, Enew);
+  } catch (const GenericSelected&) {
+    ++generic;
+  }
+  try {
+    contract_R(Eold, A, W, A, qloc, qOp, Enew);
+  } catch (const GenericSelected&) {
+    ++generic;
+  }
+  return generic == 2 ? 0 : 5;
+}
+
 int main(int argc, char** argv) {
+  if (argc == 2 && std::string(argv[1]) == "--complex-dispatch") return complex_dispatch();
   if (argc != 3) {
     std::cerr << "usage: adapter_check <pivot.txt> <vout.txt>\n";
     return 1;

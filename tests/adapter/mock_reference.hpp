@@ -13,6 +13,7 @@
 // was selected: the adapter's overloads must win overload resolution.
 #pragma once
 #include <array>
+#include <complex>
 #include <cstddef>
 #include <cstdint>
 #include <stdexcept>
@@ -77,6 +78,17 @@ struct MatrixXd {
     c = cols;
     v.assign(rows * cols, 0.0);
   }
+};
+
+// complex blocks (the TDVP instantiation, Scalar = complex<double>); only
+// overload resolution looks at it
+struct MatrixXcd {
+  typedef std::complex<double> Scalar;
+  int64_t r = 0, c = 0;
+  std::vector<Scalar> v;
+  int64_t rows() const { return r; }
+  int64_t cols() const { return c; }
+  int64_t size() const { return r * c; }
 };
 
 // column-major sparse matrix with Eigen's InnerIterator interface

@@ -144,6 +144,28 @@ def test_adapter_hxv_matches_oracle(tmp_path, kind, terms, pass_bs):
     assert np.linalg.norm(out - ref) <= 1e-12 * np.linalg.norm(ref)
 
 
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind,pass_bs", [("hub2", True), ("heis2", False)])
+def test_adapter_inplace_env_change_gets_new_plan(tmp_path, kind, pass_bs):
+    """An environment value changed in place (an interior element, so address,
+    shape and first/last element stay the same) must not be served from the
+    cached plan: the next HxV equals a freshly planned one and differs from
+    the one before the change."""
+    pd, v, src, dst = _prepare(tmp_path, kind, False, pass_bs)
+    print("case inplace", kind, flush=True)
+    env = dict(os.environ, VMPS_CHECK_INPLACE="1")
+    r = subprocess.run([_exe(), str(src), str(dst)], capture_output=True, text=True, timeout=120, env=env)
+    assert r.returncode == 0, (r.returncode, r.stderr)
+
+
+def test_adapter_complex_calls_keep_generic_templates():
+    """Complex seven-argument contract_L / contract_R calls (the TDVP
+    instantiation, solvers/TDVPPropagator.h:876-897) compile and bind the
+    reference's generic templates, not the FP64 device overloads."""
+    r = subprocess.run([_exe(), "--complex-dispatch"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, (r.returncode, r.stderr)
+
+
 # ---- contract_L / contract_R through the seven-argument overloads ----------
 def _A(A):
     out = ["%d" % len(A)]
