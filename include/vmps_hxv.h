@@ -43,6 +43,7 @@ typedef int vmps_status;
 #define VMPS_ERR_NONFINITE 5  /* NaN/Inf found in an input vector */
 #define VMPS_ERR_STATE 6      /* call out of order (e.g. no block structure yet) */
 #define VMPS_ERR_INTERNAL 7
+#define VMPS_ERR_NOCONV 8     /* an iterative method stopped at its iteration limit */
 
 #define VMPS_SYM_U1 1
 #define VMPS_SYM_SU2 2
@@ -278,7 +279,12 @@ void vmps_env_destroy(vmps_env* e);
  * per block: U (rows x r), S (r values, descending), Vt (r x cols), all
  * column-major.  One-sided Jacobi in FP64 (every singular value to rounding-
  * level relative accuracy); max_sweeps (may be NULL) receives the largest
- * number of Jacobi sweeps a block needed.  Host buffers, synchronous. */
+ * number of Jacobi sweeps a block needed.  A block still rotating after 60
+ * sweeps makes the call return VMPS_ERR_NOCONV (outputs written as they
+ * are).  The singular vector of a zero singular value (U column when
+ * rows >= cols, else Vt row) is zero, not completed to an orthonormal basis;
+ * the split discards such values.  Dimensions up to 2^20.  Host buffers,
+ * synchronous. */
 vmps_status vmps_block_svd(int nblocks, const int64_t* rows, const int64_t* cols, const double* A, double* U,
                            double* S, double* Vt, int* max_sweeps);
 /* one sector {in, out, mid} of Lold / Rold, packed as in vmps_pivot2_add_env */
@@ -317,9 +323,11 @@ vmps_status vmps_env_stats(const vmps_env* e, vmps_plan_stats* st);
  * pivot (solvers/DmrgSolver.h:1945-1953).  A vmps_denv keeps such an
  * environment on the device it was made on: the sector layout on the host,
  * the blocks in one device allocation, so a sweep need not copy environments
- * between host and device.  Blocks are read when a plan or contraction is
- * created from it (the denv may be destroyed afterwards); it is never
- * written once made. */
+ * between host and device.  It is never written once made.  A pivot or
+ * contraction that takes it (vmps_pivot2_add_env_device,
+ * vmps_env_add_old_device) shares the ownership of its device blocks: the
+ * denv may be destroyed at any point afterwards, the blocks stay allocated
+ * until the last of the denv, the pivot and the contraction is destroyed. */
 typedef struct vmps_denv vmps_denv;
 
 /* a device environment from host blocks, laid out as vmps_env_add_old_sectors */

@@ -20,7 +20,7 @@ c_void = ctypes.c_void_p
 VMPS_OK = 0
 STATUS_NAMES = {
     0: "VMPS_OK", 1: "VMPS_ERR_ARG", 2: "VMPS_ERR_LAYOUT", 3: "VMPS_ERR_OOM", 4: "VMPS_ERR_HIP",
-    5: "VMPS_ERR_NONFINITE", 6: "VMPS_ERR_STATE", 7: "VMPS_ERR_INTERNAL",
+    5: "VMPS_ERR_NONFINITE", 6: "VMPS_ERR_STATE", 7: "VMPS_ERR_INTERNAL", 8: "VMPS_ERR_NOCONV",
 }
 PREC = {"fp64": 0, "fp32": 1, "bf16x3": 2}
 

@@ -24,6 +24,7 @@ struct vmps_pivot2 {
 };
 struct vmps_plan {
   vmps::DevPlan D;
+  int npivots = 1;  // > 1: a batch of independent pivots (vmps_plan_create_batch)
 };
 
 namespace vmps {
@@ -520,6 +521,7 @@ vmps_status vmps_plan_create_batch(const vmps_pivot2* const* pivots, int n, vmps
   if (offsets) offsets[n] = S.ysize;
   auto plan = std::make_unique<vmps_plan>();
   plan->D.create_triples(std::move(S), prec);
+  plan->npivots = n;
   *out = plan.release();
   VMPS_API_END
 }
@@ -808,6 +810,8 @@ static vmps_status lanczos_impl(vmps_plan* plan, double* d_v, bool random_start,
                                 vmps_lanczos_result* res) {
   VMPS_API_BEGIN
   require(plan && opts && res, VMPS_ERR_ARG, "NULL argument");
+  require(plan->npivots == 1, VMPS_ERR_STATE,
+          "a batch plan holds independent pivots; use vmps_lanczos_batch for one eigenpair per pivot");
   require(opts->max_iter >= 1 && opts->tol_eigval >= 0 && opts->tol_state >= 0, VMPS_ERR_ARG, "bad options");
   require(opts->precision >= VMPS_PREC_FP64 && opts->precision <= VMPS_PREC_BF16X3, VMPS_ERR_ARG, "bad precision");
   auto& D = plan->D;

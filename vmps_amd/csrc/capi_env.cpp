@@ -99,11 +99,13 @@ void add_old_sectors(vmps::Tripod& E, int nq, int nsec, const int* qin, const in
 void move_to_device(vmps_denv* d) {
   VMPS_HIP_CHECK(hipGetDevice(&d->device));
   const int64_t n = (int64_t)d->T.data.size();
-  VMPS_HIP_CHECK(hipMalloc((void**)&d->buf, std::max<int64_t>(1, n) * sizeof(double)));
-  if (n) VMPS_HIP_CHECK(hipMemcpy(d->buf, d->T.data.data(), n * sizeof(double), hipMemcpyHostToDevice));
+  double* buf = nullptr;
+  VMPS_HIP_CHECK(hipMalloc((void**)&buf, std::max<int64_t>(1, n) * sizeof(double)));
+  d->T.dev_hold = vmps::own_device(buf, d->device);
+  if (n) VMPS_HIP_CHECK(hipMemcpy(buf, d->T.data.data(), n * sizeof(double), hipMemcpyHostToDevice));
   d->T.data.clear();
   d->T.data.shrink_to_fit();
-  d->T.dev = d->buf;
+  d->T.dev = buf;
   d->T.dev_size = n;
 }
 
@@ -134,9 +136,10 @@ void contract(vmps_env* e, vmps_precision prec, vmps_denv* dres) {
     D.hxv(D.vin.p, D.vout.p, D.stream);
     if (dres) {
       dout = D.vout.p;  // the plan gives up its output buffer to dres
-      dres->buf = dout;
       D.vout.p = nullptr;
       D.vout.n = 0;
+      VMPS_HIP_CHECK(hipGetDevice(&dres->device));
+      dres->T.dev_hold = vmps::own_device(dout, dres->device);
     } else {
       VMPS_HIP_CHECK(hipMemcpyAsync(e->result.data(), D.vout.p, ob, hipMemcpyDeviceToHost, D.stream));
     }
@@ -149,7 +152,7 @@ void contract(vmps_env* e, vmps_precision prec, vmps_denv* dres) {
     VMPS_HIP_CHECK(hipGetDevice(&dres->device));
     if (!dout) {  // nothing to contract: a zero environment of the result's layout
       VMPS_HIP_CHECK(hipMalloc((void**)&dout, std::max<int64_t>(1, osize) * sizeof(double)));
-      dres->buf = dout;
+      dres->T.dev_hold = vmps::own_device(dout, dres->device);
       if (osize) VMPS_HIP_CHECK(hipMemset(dout, 0, osize * sizeof(double)));
     }
     vmps::Tripod& T = dres->T;
@@ -412,7 +415,7 @@ vmps_status vmps_denv_get(const vmps_denv* d, int* qin, int* qout, int* qmid, in
     VMPS_HIP_CHECK(hipGetDevice(&dev));
     require(dev == d->device, VMPS_ERR_ARG, "the environment lives on another device");
     host.resize((size_t)d->T.dev_size);
-    VMPS_HIP_CHECK(hipMemcpy(host.data(), d->buf, host.size() * sizeof(double), hipMemcpyDeviceToHost));
+    VMPS_HIP_CHECK(hipMemcpy(host.data(), d->T.dev, host.size() * sizeof(double), hipMemcpyDeviceToHost));
   }
   tripod_get(d->T, d->nq, host.data(), qin, qout, qmid, adim, rows, cols, data);
   VMPS_API_END

@@ -4,14 +4,27 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <memory>
+
 #include "pivot.hpp"
+
+namespace vmps {
+// Takes ownership of a device allocation made on `device`: the last owner
+// (the vmps_denv, or a pivot / contraction that took its Tripod) frees it.
+inline std::shared_ptr<const double> own_device(double* p, int device) {
+  return std::shared_ptr<const double>(p, [device](const double* q) {
+    if (!q) return;
+    int cur = -1;
+    const bool restore = hipGetDevice(&cur) == hipSuccess && cur != device;
+    if (restore) (void)hipSetDevice(device);
+    (void)hipFree(const_cast<double*>(q));
+    if (restore) (void)hipSetDevice(cur);
+  });
+}
+}  // namespace vmps
 
 struct vmps_denv {
   int nq = 0;
   int device = -1;
-  vmps::Tripod T;  // T.dev = buf, T.dev_size = size, T.data empty
-  double* buf = nullptr;
-  ~vmps_denv() {
-    if (buf) (void)hipFree(buf);
-  }
+  vmps::Tripod T;  // T.dev / T.dev_hold = the device blocks, T.data empty
 };

@@ -411,8 +411,16 @@ LanczosResult lanczos_solve(DevPlan& D, double* d_v, const LanczosOpts& o, vmps_
   K.ensure(D, kmax);
   hipGraphExec_t iteration = o.graph ? K.ensure_graph(D) : nullptr;
 
-  // q0 = v / ||v|| into the basis and into the HxV input; k = 0
+  // q0 = v / ||v|| into the basis and into the HxV input; k = 0.  A zero
+  // (or non-finite) start vector has no direction to start from.
   D.dot_async(d_v, d_v, K.scal.p, s);
+  VMPS_HIP_CHECK(hipMemcpyAsync(K.host3, K.scal.p, sizeof(double), hipMemcpyDeviceToHost, s));
+  VMPS_HIP_CHECK(hipStreamSynchronize(s));
+  if (!(K.host3[0] > 0.0) || !std::isfinite(K.host3[0])) {
+    D.prec = saved;
+    throw VmpsError(K.host3[0] == 0.0 ? VMPS_ERR_ARG : VMPS_ERR_NONFINITE,
+                    K.host3[0] == 0.0 ? "the start vector is zero" : "the start vector is not finite");
+  }
   hipLaunchKernelGGL(k_normalize_into, dim3(vgrid(n)), dim3(256), 0, s, K.scal.p, d_v, K.Q.p, n);
   hipLaunchKernelGGL(k_copy, dim3(vgrid(n)), dim3(256), 0, s, K.Q.p, K.v.p, n);
   hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, s, K.kdev.p, 0);

@@ -15,6 +15,7 @@
 #include <map>
 #include <string>
 #include <unordered_map>
+#include <memory>
 #include <vector>
 
 #include "sym.hpp"
@@ -51,8 +52,11 @@ struct Tripod {
   std::vector<std::vector<DenseBlock>> block;    // [q][a]
   std::vector<double> data;                      // packed FP64 blocks
   // device-resident environment: the blocks' offsets refer to `dev` (FP64
-  // on the current device, owned by the caller) and `data` stays empty
+  // on device dev_device) and `data` stays empty.  dev_hold shares the
+  // ownership of that allocation: it stays valid while any copy of the
+  // Tripod (in a vmps_denv, a pivot or a contraction) exists.
   const double* dev = nullptr;
+  std::shared_ptr<const double> dev_hold;
   int64_t dev_size = 0;
   int dev_device = -1;
   std::unordered_map<Key3, size_t, Key3Hash> dict;

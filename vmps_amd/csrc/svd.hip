@@ -54,7 +54,8 @@ __global__ __launch_bounds__(256) void k_jacobi_svd(const SvdTask* __restrict__ 
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   double* __restrict__ Wb = W + t.w_off;
   double* __restrict__ Vb = V + t.v_off;
-  for (int e = threadIdx.x; e < n * n; e += 256) Vb[e] = (e % (n + 1) == 0) ? 1.0 : 0.0;
+  const int64_t nn = (int64_t)n * n;
+  for (int64_t e = threadIdx.x; e < nn; e += 256) Vb[e] = (e % (n + 1) == 0) ? 1.0 : 0.0;
   const int np = n + (n & 1);
   int sweep = 0;
   for (; sweep < kMaxSweeps && n > 1; ++sweep) {
@@ -220,6 +221,10 @@ extern "C" vmps_status vmps_block_svd(int nblocks, const int64_t* rows, const in
       vto += n * c;
     }
     if (max_sweeps) *max_sweeps = worst;
+    if (worst >= vmps::kMaxSweeps) {
+      vmps::set_last_error("block SVD: a block was still rotating after the Jacobi sweep limit");
+      return VMPS_ERR_NOCONV;
+    }
     return VMPS_OK;
   } catch (const VmpsError& e) {
     vmps::set_last_error(e.what());
