@@ -242,6 +242,18 @@ def chain_pivot(sym, mpo, Qt, p, nsite, contract_L, contract_R):
     left = [(q, m) for (q, m) in B[p]]
     right = [(q, m) for (q, m) in C[p + nsite]]
     rdim = dict(right)
+    if nsite == 0:
+        # zero-site pivot on the bond left of site p (PivotMatrix0): the bond
+        # matrix C has one block (q, q) per multiplet q both sides carry
+        ld = dict(left)
+        common = [q for q, _ in left if q in rdim]
+        pd = synth.PivotData(sym, None, None, [(q, ld[q]) for q in common], [(q, rdim[q]) for q in common], nsite=0)
+        cd = set(common)
+        pd.L = [e for e in Lenv if tuple(e[0]) in cd and tuple(e[1]) in cd]
+        pd.R = [e for e in Renv if tuple(e[0]) in cd and tuple(e[1]) in cd]
+        pd.basis = [(None, 0)]
+        pd.V = [[(q, q, ld[q], rdim[q]) for q in common]]
+        return pd
     if nsite == 2:
         pd = synth.PivotData(sym, mpo[p], mpo[p + 1], left, right)
         pd.basis = synth.two_site_basis(sym, mpo[p].qloc, mpo[p + 1].qloc)

@@ -29,7 +29,7 @@ def heis_u1_chain(L, J=1.0):
     return models.chain_mpo("HeisenbergU1", L, J=J)
 
 
-@pytest.mark.parametrize("nsite,p", [(2, 2), (2, 0), (1, 3)])
+@pytest.mark.parametrize("nsite,p", [(2, 2), (2, 0), (1, 3), (0, 3)])
 @pytest.mark.parametrize("two_sz", [0, 2, 4])
 def test_fullbasis_u1_spectrum_equals_ed(nsite, p, two_sz):
     """Harness check with U(1): the exact pivot's generalized spectrum is the
@@ -42,7 +42,7 @@ def test_fullbasis_u1_spectrum_equals_ed(nsite, p, two_sz):
     assert np.allclose(ev, ref, atol=1e-10)
 
 
-@pytest.mark.parametrize("nsite,p", [(2, 2), (2, 0), (2, 4), (1, 2)])
+@pytest.mark.parametrize("nsite,p", [(2, 2), (2, 0), (2, 4), (1, 2), (0, 3), (0, 2)])
 @pytest.mark.parametrize("S", [0, 1, 2])
 def test_fullbasis_su2_spectrum_equals_ed(nsite, p, S):
     """SU(2)-reduced HeisenbergSU2 pivot (oracle HxV + restated contract_L/R
@@ -103,7 +103,7 @@ def test_engine_su2_blockstructure_equals_oracle(make):
     assert np.any(np.abs(np.abs(bo[3]) - 1.0) > 1e-3)
 
 
-@pytest.mark.parametrize("nsite,p", [(2, 1), (1, 2)])
+@pytest.mark.parametrize("nsite,p", [(2, 1), (1, 2), (0, 2)])
 @pytest.mark.parametrize("S2,N", [(1, 4), (3, 4), (2, 3), (2, 5), (1, 2)])
 def test_fullbasis_hubbard_su2xu1_spectrum_equals_ed(nsite, p, S2, N):
     """HubbardSU2xU1 (reduced c, c^dagger with the fermionic sign, U d):
