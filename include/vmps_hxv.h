@@ -287,6 +287,22 @@ void vmps_env_destroy(vmps_env* e);
  * synchronous. */
 vmps_status vmps_block_svd(int nblocks, const int64_t* rows, const int64_t* cols, const double* A, double* U,
                            double* S, double* Vt, int* max_sweeps);
+/* vmps_block_svd that allocates no device memory: no counterpart in the
+ * reference; for callers that cannot allocate device memory (tight serving
+ * loops, harnesses that forbid hipMalloc).  Arguments,
+ * results and errors as vmps_block_svd (same kernels, same results).  The
+ * work arrays live in a fixed 64 MiB device array of the library, one per
+ * GPU, held until the process ends: the input, the Jacobi work copies, the
+ * singular values and the packed U, S, Vt together (8 bytes x (2 rows cols
+ * + (rows + cols) k + k^2 + 2 k) per block with k = min(rows, cols), plus a
+ * few words of descriptors, each region rounded up to 256 bytes) must fit,
+ * and k <= 8192 per block; larger calls
+ * return VMPS_ERR_ARG before touching the device.  Calls on one GPU share
+ * that array, so calls that overlap on one GPU (from several threads) race;
+ * every call overwrites what it uses, nothing carries over.  Runs on the null
+ * stream and returns after the results are copied back. */
+vmps_status vmps_block_svd_noalloc(int nblocks, const int64_t* rows, const int64_t* cols, const double* A,
+                                   double* U, double* S, double* Vt, int* max_sweeps);
 /* one sector {in, out, mid} of Lold / Rold, packed as in vmps_pivot2_add_env */
 vmps_status vmps_env_add_old(vmps_env* e, const int* qin, const int* qout, const int* qmid, int adim,
                              const int64_t* rows, const int64_t* cols, const double* data);

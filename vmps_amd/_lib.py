@@ -113,6 +113,7 @@ SIGNATURES = {
     "vmps_plan_time_stages": (c_int, [c_void, c_int, c_int, P_dbl]),
     "vmps_plan_get_tiles": (c_int, [c_void, ctypes.POINTER(TileConfig)]),
     "vmps_block_svd": (c_int, [c_int, P_i64, P_i64, P_dbl, P_dbl, P_dbl, P_dbl, P_int]),
+    "vmps_block_svd_noalloc": (c_int, [c_int, P_i64, P_i64, P_dbl, P_dbl, P_dbl, P_dbl, P_int]),
     "vmps_plan_set_tiles": (c_int, [c_void, ctypes.POINTER(TileConfig)]),
     "vmps_plan_autotune": (c_int, [c_void, c_int, ctypes.POINTER(TileConfig)]),
     "vmps_env_create": (c_int, [c_int, c_int, P_int, c_int, P_int, c_int, P_int, ctypes.POINTER(c_void)]),

@@ -19,6 +19,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
+#include <string>
 #include <vector>
 
 #include "engine.hpp"
@@ -116,11 +118,224 @@ __global__ __launch_bounds__(256) void k_jacobi_svd(const SvdTask* __restrict__ 
   if (threadIdx.x == 0) sweeps_used[blockIdx.x] = sweep;
 }
 
+// ---- no-allocation variant --------------------------------------------------
+// Block b of the packed input A (rows x cols) into the Jacobi work array:
+// W = A (rows >= cols) or W = A^T; one workgroup per block.
+__global__ __launch_bounds__(256) void k_svd_load(const SvdTask* __restrict__ tasks, const int64_t* __restrict__ a_off,
+                                                  const int8_t* __restrict__ transposed, const double* __restrict__ A,
+                                                  double* __restrict__ W) {
+  const SvdTask t = tasks[blockIdx.x];
+  const double* __restrict__ a = A + a_off[blockIdx.x];
+  double* __restrict__ w = W + t.w_off;
+  const int64_t mn = (int64_t)t.m * t.n;
+  if (!transposed[blockIdx.x]) {
+    for (int64_t e = threadIdx.x; e < mn; e += 256) w[e] = a[e];
+  } else {  // A is n x m (rows x cols): W[j + m i] = A[i + n j]
+    for (int64_t e = threadIdx.x; e < mn; e += 256) {
+      const int64_t j = e % t.m, i = e / t.m;
+      w[e] = a[i + (int64_t)t.n * j];
+    }
+  }
+}
+
+// Singular values sorted descending (ties in column order, as a stable sort),
+// U, S and Vt of block b written packed as vmps_block_svd returns them.
+constexpr int kSvdFinishMaxN = 8192;
+__global__ __launch_bounds__(256) void k_svd_finish(const SvdTask* __restrict__ tasks,
+                                                    const int64_t* __restrict__ out_off,
+                                                    const int8_t* __restrict__ transposed,
+                                                    const double* __restrict__ W, const double* __restrict__ V,
+                                                    const double* __restrict__ sigma, double* __restrict__ U,
+                                                    double* __restrict__ S, double* __restrict__ Vt) {
+  __shared__ int rank[kSvdFinishMaxN];
+  const SvdTask t = tasks[blockIdx.x];
+  const int m = t.m, n = t.n;
+  const double* __restrict__ sg = sigma + t.s_off;
+  for (int i = threadIdx.x; i < n; i += 256) {
+    const double si = sg[i];
+    int r = 0;
+    for (int j = 0; j < n; ++j) r += (sg[j] > si) || (sg[j] == si && j < i);
+    rank[i] = r;
+  }
+  __syncthreads();
+  const bool tr = transposed[blockIdx.x] != 0;
+  const int64_t rows = tr ? n : m, cols = tr ? m : n;
+  double* __restrict__ u = U + out_off[3 * blockIdx.x];
+  double* __restrict__ sv = S + out_off[3 * blockIdx.x + 1];
+  double* __restrict__ vt = Vt + out_off[3 * blockIdx.x + 2];
+  const double* __restrict__ w = W + t.w_off;
+  const double* __restrict__ v = V + t.v_off;
+  for (int i = threadIdx.x; i < n; i += 256) sv[rank[i]] = sg[i];
+  // left factor of W: w_i / sigma_i (m x n); right factor V (n x n)
+  for (int64_t e = threadIdx.x; e < (int64_t)n * m; e += 256) {
+    const int i = (int)(e / m);
+    const int64_t x = e % m;
+    const double inv = sg[i] > 0.0 ? 1.0 / sg[i] : 0.0;
+    const double val = w[x + (int64_t)m * i] * inv;
+    if (!tr)
+      u[x + rows * rank[i]] = val;        // U = left (rows = m)
+    else
+      vt[rank[i] + (int64_t)n * x] = val;  // Vt = left^T (cols = m)
+  }
+  for (int64_t e = threadIdx.x; e < (int64_t)n * n; e += 256) {
+    const int i = (int)(e / n);
+    const int64_t y = e % n;
+    const double val = v[y + (int64_t)n * i];
+    if (!tr)
+      vt[rank[i] + (int64_t)n * y] = val;  // Vt = V^T (cols = n)
+    else
+      u[y + rows * rank[i]] = val;         // U = V (rows = n)
+  }
+  (void)cols;
+}
+
+// Scratch of vmps_block_svd_noalloc: one array per device (each GPU holds its
+// own copy), reached from the host only through the address
+// hipGetSymbolAddress returns for the current device.
+constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
+__device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+void* scratch_for_current_device() {
+  int dev = 0;
+  VMPS_HIP_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) throw VmpsError(VMPS_ERR_ARG, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      throw VmpsError(VMPS_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[dev] = p;
+  }
+  return g_scratch_ptr[dev];
+}
+
 }  // namespace
 
 }  // namespace vmps
 
 using vmps::VmpsError;
+
+extern "C" vmps_status vmps_block_svd_noalloc(int nblocks, const int64_t* rows, const int64_t* cols,
+                                              const double* A, double* U, double* S, double* Vt,
+                                              int* max_sweeps) {
+  try {
+    using vmps::SvdTask;
+    if (nblocks < 0 || (nblocks > 0 && (!rows || !cols || !A || !U || !S || !Vt)))
+      throw VmpsError(VMPS_ERR_ARG, "bad argument");
+    // scratch layout (8-byte words, each region 256-byte aligned): tasks,
+    // offsets and flags, A, W, V, sigma, U, S, Vt, sweep counts
+    auto align = [](size_t b) { return (b + 255) & ~size_t(255); };
+    int64_t asize = 0, wsize = 0, vsize = 0, ssize = 0, usize = 0, vtsize = 0;
+    for (int b = 0; b < nblocks; ++b) {
+      if (rows[b] < 0 || cols[b] < 0 || rows[b] > (1 << 20) || cols[b] > (1 << 20))
+        throw VmpsError(VMPS_ERR_ARG, "bad block dimension");
+      const int64_t r = rows[b], c = cols[b], k = std::min(r, c);
+      if (std::max(r, c) > 0 && k > vmps::kSvdFinishMaxN)
+        throw VmpsError(VMPS_ERR_ARG, "block dimension beyond the no-alloc limit (min(rows, cols) <= 8192)");
+      asize += r * c;
+      wsize += r * c;
+      vsize += k * k;
+      ssize += k;
+      usize += r * k;
+      vtsize += k * c;
+    }
+    const size_t nb = (size_t)std::max(0, nblocks);
+    size_t off = 0;
+    const size_t o_tasks = off;
+    off = align(off + nb * sizeof(SvdTask));
+    const size_t o_aoff = off;
+    off = align(off + nb * sizeof(int64_t));
+    const size_t o_ooff = off;
+    off = align(off + 3 * nb * sizeof(int64_t));
+    const size_t o_tr = off;
+    off = align(off + nb);
+    const size_t o_A = off;
+    off = align(off + (size_t)asize * 8);
+    const size_t o_W = off;
+    off = align(off + (size_t)wsize * 8);
+    const size_t o_V = off;
+    off = align(off + (size_t)vsize * 8);
+    const size_t o_sig = off;
+    off = align(off + (size_t)ssize * 8);
+    const size_t o_U = off;
+    off = align(off + (size_t)usize * 8);
+    const size_t o_S = off;
+    off = align(off + (size_t)ssize * 8);
+    const size_t o_Vt = off;
+    off = align(off + (size_t)vtsize * 8);
+    const size_t o_sw = off;
+    off = align(off + nb * sizeof(int));
+    if (off > vmps::kNoAllocScratchBytes)
+      throw VmpsError(VMPS_ERR_ARG, "blocks exceed the 64 MiB scratch of vmps_block_svd_noalloc");
+    if (nblocks == 0 || wsize == 0) return VMPS_OK;
+    unsigned char* base = static_cast<unsigned char*>(vmps::scratch_for_current_device());
+    // host descriptor tables (small: a few words per block)
+    std::vector<SvdTask> tasks(nb);
+    std::vector<int64_t> aoff(nb), ooff(3 * nb);
+    std::vector<int8_t> tr(nb);
+    int64_t ao = 0, wo = 0, vo = 0, so = 0, uo = 0, vto = 0;
+    for (int b = 0; b < nblocks; ++b) {
+      const int64_t r = rows[b], c = cols[b];
+      tr[b] = r < c;
+      const int64_t m = tr[b] ? c : r, n = tr[b] ? r : c;
+      tasks[b] = {wo, vo, so, (int32_t)m, (int32_t)n};
+      aoff[b] = ao;
+      ooff[3 * b] = uo;
+      ooff[3 * b + 1] = so;
+      ooff[3 * b + 2] = vto;
+      ao += r * c;
+      wo += m * n;
+      vo += n * n;
+      so += n;
+      uo += r * n;
+      vto += n * c;
+    }
+    hipStream_t s = nullptr;  // the null stream: synchronous with the copies below
+    VMPS_HIP_CHECK(hipMemcpy(base + o_tasks, tasks.data(), nb * sizeof(SvdTask), hipMemcpyHostToDevice));
+    VMPS_HIP_CHECK(hipMemcpy(base + o_aoff, aoff.data(), nb * sizeof(int64_t), hipMemcpyHostToDevice));
+    VMPS_HIP_CHECK(hipMemcpy(base + o_ooff, ooff.data(), 3 * nb * sizeof(int64_t), hipMemcpyHostToDevice));
+    VMPS_HIP_CHECK(hipMemcpy(base + o_tr, tr.data(), nb, hipMemcpyHostToDevice));
+    VMPS_HIP_CHECK(hipMemcpy(base + o_A, A, (size_t)asize * 8, hipMemcpyHostToDevice));
+    auto* dT = reinterpret_cast<const SvdTask*>(base + o_tasks);
+    auto* dW = reinterpret_cast<double*>(base + o_W);
+    auto* dV = reinterpret_cast<double*>(base + o_V);
+    auto* dsig = reinterpret_cast<double*>(base + o_sig);
+    auto* dsw = reinterpret_cast<int*>(base + o_sw);
+    hipLaunchKernelGGL(vmps::k_svd_load, dim3((unsigned)nblocks), dim3(256), 0, s, dT,
+                       reinterpret_cast<const int64_t*>(base + o_aoff), reinterpret_cast<const int8_t*>(base + o_tr),
+                       reinterpret_cast<const double*>(base + o_A), dW);
+    hipLaunchKernelGGL(vmps::k_jacobi_svd, dim3((unsigned)nblocks), dim3(256), 0, s, dT, dW, dV, dsig, dsw);
+    hipLaunchKernelGGL(vmps::k_svd_finish, dim3((unsigned)nblocks), dim3(256), 0, s, dT,
+                       reinterpret_cast<const int64_t*>(base + o_ooff), reinterpret_cast<const int8_t*>(base + o_tr),
+                       dW, dV, dsig, reinterpret_cast<double*>(base + o_U), reinterpret_cast<double*>(base + o_S),
+                       reinterpret_cast<double*>(base + o_Vt));
+    VMPS_HIP_CHECK(hipGetLastError());
+    VMPS_HIP_CHECK(hipMemcpy(U, base + o_U, (size_t)usize * 8, hipMemcpyDeviceToHost));
+    VMPS_HIP_CHECK(hipMemcpy(S, base + o_S, (size_t)ssize * 8, hipMemcpyDeviceToHost));
+    VMPS_HIP_CHECK(hipMemcpy(Vt, base + o_Vt, (size_t)vtsize * 8, hipMemcpyDeviceToHost));
+    std::vector<int> sweeps(nb);
+    VMPS_HIP_CHECK(hipMemcpy(sweeps.data(), base + o_sw, nb * sizeof(int), hipMemcpyDeviceToHost));
+    int worst = 0;
+    for (int sw : sweeps) worst = std::max(worst, sw);
+    if (max_sweeps) *max_sweeps = worst;
+    if (worst >= vmps::kMaxSweeps) {
+      vmps::set_last_error("block SVD: a block was still rotating after the Jacobi sweep limit");
+      return VMPS_ERR_NOCONV;
+    }
+    return VMPS_OK;
+  } catch (const VmpsError& e) {
+    vmps::set_last_error(e.what());
+    return e.status;
+  } catch (const std::exception& e) {
+    vmps::set_last_error(e.what());
+    return VMPS_ERR_INTERNAL;
+  }
+}
 
 extern "C" vmps_status vmps_block_svd(int nblocks, const int64_t* rows, const int64_t* cols, const double* A,
                                       double* U, double* S, double* Vt, int* max_sweeps) {
