@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define VMPS_ABI_VERSION 3
+#define VMPS_ABI_VERSION 4
 
 typedef int vmps_status;
 #define VMPS_OK 0
@@ -287,6 +287,21 @@ void vmps_env_destroy(vmps_env* e);
  * synchronous. */
 vmps_status vmps_block_svd(int nblocks, const int64_t* rows, const int64_t* cols, const double* A, double* U,
                            double* S, double* Vt, int* max_sweeps);
+/* The split's factorisation through the reduced density matrix of one
+ * side, on the device (stands in for the dense SVD of Biped::truncateSVD,
+ * tensors/Biped.h:754-775, inside split_AA2, tensors/DmrgContractions.h:
+ * 2460-2630).  For every block Psi (rows x cols, column-major, packed as in
+ * vmps_block_svd):
+ *   side 0: rho = Psi Psi^T (k = rows), C = W^T Psi   (k x cols)
+ *   side 1: rho = Psi^T Psi (k = cols), C = Psi W     (rows x k)
+ * with rho = W diag(lambda) W^T: lambda (k values, ascending; lambda =
+ * sigma^2 of Psi, absolute accuracy eps ||Psi||^2), W (k x k, orthonormal
+ * columns, the new site tensor of the sweep direction) and C (the centre
+ * that moves on).  Outputs packed per block like the inputs.  Dense GEMMs by
+ * rocBLAS, the eigensolver rocSOLVER's dsyevd; host buffers, synchronous;
+ * VMPS_ERR_NOCONV if the eigensolver fails. */
+vmps_status vmps_block_eigh_split(int nblocks, const int64_t* rows, const int64_t* cols, const double* A, int side,
+                                  double* lambda, double* W, double* C);
 /* vmps_block_svd that allocates no device memory: no counterpart in the
  * reference; for callers that cannot allocate device memory (tight serving
  * loops, harnesses that forbid hipMalloc).  Arguments,
@@ -403,7 +418,18 @@ typedef struct {
   uint64_t seed;            /* start vector when none is given */
   int graph;                /* 1: each iteration is one replay of a captured HIP graph (default);
                                0: the same kernels launched one by one */
+  int refine_method;        /* mixed solves with refine_fp64: VMPS_REFINE_CORRECTION (default) or
+                               VMPS_REFINE_RESTART */
 } vmps_lanczos_opts;
+
+/* Refinement of a mixed-precision eigenpair to the FP64 tolerances:
+ * CORRECTION: FP64 residual, correction from conjugate gradients on the
+ *   projected equation (1 - psi psi^T)(H' - E)(1 - psi psi^T) t = -r with the
+ *   mixed operator H', FP64 Rayleigh-Ritz on {psi, t}; one FP64 HxV per step
+ *   (at most 4 steps, then RESTART takes over).
+ * RESTART: an FP64 Lanczos solve started from the mixed eigenvector. */
+#define VMPS_REFINE_CORRECTION 0
+#define VMPS_REFINE_RESTART 1
 
 typedef struct {
   double energy;            /* <psi|H|psi> with the FP64 operator when refined / FP64 */
@@ -415,6 +441,9 @@ typedef struct {
   double energy_mixed;      /* Rayleigh quotient of the mixed solve with its own operator */
   double residual_mixed;
   double ms;                /* wall time of the call */
+  int hxv_fp64;             /* HxV calls with the FP64 operator (all of hxv_calls for FP64 solves) */
+  int refine_steps;         /* correction steps of VMPS_REFINE_CORRECTION */
+  int refine_cg;            /* mixed HxV calls inside those corrections */
 } vmps_lanczos_result;
 
 void vmps_lanczos_opts_default(vmps_lanczos_opts* o);
@@ -424,6 +453,19 @@ vmps_status vmps_lanczos(vmps_plan* plan, double* v, const vmps_lanczos_opts* op
 /* Same with a device-resident start / result vector. */
 vmps_status vmps_lanczos_device(vmps_plan* plan, double* d_v, const vmps_lanczos_opts* opts,
                                 vmps_lanczos_result* res);
+
+/* One lowest eigenpair per pivot of a batch plan (vmps_plan_create_batch):
+ * the independent solves of VumpsSolver's loop over the unit cell
+ * (VUMPS/VumpsSolver.h:1168-1211) run together.  Each pivot keeps its own
+ * Gram-Schmidt coefficients, tridiagonal matrix and convergence test (opts
+ * as for vmps_lanczos); every iteration of the whole batch is one replay of a
+ * captured HIP graph (opts->graph).  v: host vector of the batch layout
+ * (pivot i at offsets[i] of vmps_plan_create_batch), start vectors on entry
+ * (NULL: uniform random, seed opts->seed), eigenvectors on exit.  res: one
+ * result per pivot.  A mixed precision with refine_fp64 is refined by an FP64
+ * batch solve started from the mixed eigenvectors (the restart form).
+ * VMPS_ERR_STATE for a plan that is not a batch. */
+vmps_status vmps_lanczos_batch(vmps_plan* plan, double* v, const vmps_lanczos_opts* opts, vmps_lanczos_result* res);
 
 /* Per-stage device time (ms per call, HIP events on the plan's stream, mean
  * over `iters` launches of each stage alone on resident data):

@@ -27,7 +27,7 @@ def test_library_exports_all_declared_symbols():
     assert len(names) >= 20
     for n in names:
         assert hasattr(lib, n), n
-    assert lib.vmps_abi_version() == 3
+    assert lib.vmps_abi_version() == 4
 
 
 def test_bad_arguments_give_status_and_message():

@@ -2,9 +2,10 @@
 backend (oracle HxV + ARPACK + Python restatement of contract_L / contract_R)."""
 import sys
 
+import numpy as np
 import pytest
 
-from vmps_amd import dmrg
+from vmps_amd import dmrg, models, synth
 
 sys.path.insert(0, __file__.rsplit("/", 1)[0])
 import dmrg_oracle  # noqa: E402
@@ -120,3 +121,30 @@ def test_hubbard_su2xu1_ladder_matches_ed(N, S):
     r = dmrg.run_dmrg("HubbardSU2xU1Ladder", L, 48, 3, Q=(int(round(2 * S)) + 1, N),
                       backend=dmrg_oracle.OracleBackend(), params=dict(t=1.0, U=6.0))
     assert abs(r["energy"] - e_ed) <= 1e-9 * abs(e_ed)
+
+
+def test_split_svd_falls_back_when_gesdd_fails(monkeypatch):
+    """A block on which LAPACK's gesdd reports no convergence is factored by
+    gesvd instead, giving the same split; non-finite wavefunctions are refused."""
+    W = models.heisenberg_u1_mpo()
+    pd = synth.make_pivot(W, W, synth.heisenberg_bond(5, 12), synth.heisenberg_bond(7, 12), np.random.default_rng(3))
+    v = synth.random_vector(pd, 4)
+    ref = dmrg.split_psi(pd, v, W.qloc, 8)
+    real_svd = np.linalg.svd
+    calls = {"n": 0}
+
+    def failing_svd(m, full_matrices=True, *a, **k):
+        calls["n"] += 1
+        raise np.linalg.LinAlgError("SVD did not converge")
+    monkeypatch.setattr(np.linalg, "svd", failing_svd)
+    got = dmrg.split_psi(pd, v, W.qloc, 8)
+    monkeypatch.setattr(np.linalg, "svd", real_svd)
+    assert calls["n"] > 0
+    assert got[4] == ref[4] and abs(got[2] - ref[2]) <= 1e-12
+    back_ref = dmrg.two_site_psi(pd, ref[0], ref[1], W.qloc)
+    back = dmrg.two_site_psi(pd, got[0], got[1], W.qloc)
+    assert np.linalg.norm(back - back_ref) <= 1e-12 * np.linalg.norm(back_ref)
+    bad = v.copy()
+    bad[3] = np.nan
+    with pytest.raises(FloatingPointError):
+        dmrg.split_psi(pd, bad, W.qloc, 8)

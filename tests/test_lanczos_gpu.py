@@ -133,3 +133,34 @@ def test_graph_iteration_matches_eager(kind):
         assert rg["iterations"] == re_["iterations"] and rg["iterations_refine"] == re_["iterations_refine"]
         assert rg["energy"] == re_["energy"]
         assert np.array_equal(vg, ve)
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
+def test_mixed_refinement_correction_vs_restart(prec):
+    """The correction refinement (FP64 residual, mixed CG correction, FP64
+    Rayleigh-Ritz) and the FP64 restart reach the FP64 eigenvalue of the
+    M=512 Hubbard pivot; the correction needs at most four FP64 HxV."""
+    pd = synth.make_workload("hubbard_u1xu1_L48_M512", hermitian=True)
+    H = PivotMatrix2(pd)
+    v0 = np.random.default_rng(3).uniform(-1, 1, pd.vsize)
+    _, r64 = lanczos(H, v0=v0, max_iter=300, tol_eigval=1e-12, tol_state=1e-7)
+    H.set_precision(prec)
+    out = {}
+    for meth in ("correction", "restart"):
+        v, r = lanczos(H, v0=v0, precision=prec, max_iter=300, tol_eigval=1e-12, tol_state=1e-7,
+                       refine_tol_eigval=1e-12, refine_tol_state=1e-7, refine_method=meth)
+        print(prec, meth, r, flush=True)
+        assert r["converged"]
+        assert abs(r["energy"] - r64["energy"]) <= 1e-10 * abs(r64["energy"])
+        assert r["residual"] <= 1e-6
+        out[meth] = r
+    assert out["correction"]["hxv_fp64"] <= 4 < out["restart"]["hxv_fp64"]
+
+
+def test_zero_start_vector_refused():
+    from vmps_amd import _lib
+    pd = _herm_small("heis")
+    H = PivotMatrix2(pd)
+    with pytest.raises(_lib.VmpsStatusError) as e:
+        lanczos(H, v0=np.zeros(pd.vsize))
+    assert "zero" in str(e.value)

@@ -55,3 +55,42 @@ def test_dmrg_device_split_matches_host_split(model, L, M):
     print(model, "device", r_dev["energy"], "host", r_host["energy"], "device splits", be_dev.device_svds, flush=True)
     assert be_dev.device_svds > 0 and be_host.device_svds == 0
     assert abs(r_dev["energy"] - r_host["energy"]) <= 1e-9 * abs(r_host["energy"])
+
+
+def test_block_eigh_split_against_lapack():
+    """Density-matrix factorisation on the device: lambda = sigma^2 (LAPACK),
+    W orthonormal, W C (side 0) or C W^T (side 1) reproduces the matrix."""
+    rng = np.random.default_rng(2)
+    shapes = [(1, 1), (5, 1), (1, 7), (40, 17), (17, 40), (300, 260), (0, 4), (4, 0)]
+    mats = [rng.standard_normal(s) for s in shapes]
+    mats.append(_known(rng, 60, 50, np.logspace(0, -6, 50)))
+    for side in (0, 1):
+        out = dmrg.block_eigh_device(mats, side)
+        for A, (lam, W, C) in zip(mats, out):
+            print("side", side, "shape", A.shape, flush=True)
+            k = A.shape[0] if side == 0 else A.shape[1]
+            assert lam.shape == (k,) and W.shape == (k, k) and C.shape == A.shape
+            if k == 0 or A.size == 0:
+                continue
+            sv = np.linalg.svd(A, compute_uv=False)
+            ref = np.zeros(k)
+            ref[:len(sv)] = sv ** 2
+            assert np.all(np.diff(lam) <= 1e-14 * lam[0])
+            assert np.max(np.abs(lam - ref)) <= 1e-13 * ref[0]
+            assert np.allclose(W.T @ W, np.eye(k), atol=1e-12)
+            back = W @ C if side == 0 else C @ W.T
+            assert np.linalg.norm(back - A) <= 1e-12 * np.linalg.norm(A)
+
+
+@pytest.mark.parametrize("model,L,M", [("HeisenbergU1", 20, 400), ("HeisenbergSU2", 24, 300)])
+def test_dmrg_density_matrix_split_matches_svd_split(model, L, M):
+    """Sweeps whose split factors on the device through the reduced density
+    matrix reach the energy of the same sweeps with the host SVD."""
+    be_dev = dmrg.GpuBackend(device_split_min=64)
+    r_dev = dmrg.run_dmrg(model, L, M, 2, backend=be_dev, params={"J": 1.0})
+    be_host = dmrg.GpuBackend(device_split_min=0)
+    r_host = dmrg.run_dmrg(model, L, M, 2, backend=be_host, params={"J": 1.0})
+    print(model, "device", r_dev["energy"], "host", r_host["energy"], "device splits", be_dev.device_splits,
+          flush=True)
+    assert be_dev.device_splits > 0 and be_host.device_splits == 0
+    assert abs(r_dev["energy"] - r_host["energy"]) <= 1e-9 * abs(r_host["energy"])

@@ -42,7 +42,7 @@ class LanczosOpts(ctypes.Structure):
     _fields_ = [
         ("max_iter", c_int), ("min_iter", c_int), ("tol_eigval", c_dbl), ("tol_state", c_dbl),
         ("precision", c_int), ("refine_fp64", c_int), ("refine_tol_eigval", c_dbl), ("refine_tol_state", c_dbl),
-        ("seed", ctypes.c_uint64), ("graph", c_int),
+        ("seed", ctypes.c_uint64), ("graph", c_int), ("refine_method", c_int),
     ]
 
 
@@ -50,7 +50,7 @@ class LanczosResult(ctypes.Structure):
     _fields_ = [
         ("energy", c_dbl), ("residual", c_dbl), ("iterations", c_int), ("iterations_refine", c_int),
         ("hxv_calls", c_int), ("converged", c_int), ("energy_mixed", c_dbl), ("residual_mixed", c_dbl),
-        ("ms", c_dbl),
+        ("ms", c_dbl), ("hxv_fp64", c_int), ("refine_steps", c_int), ("refine_cg", c_int),
     ]
 
     def as_dict(self):
@@ -114,6 +114,7 @@ SIGNATURES = {
     "vmps_plan_get_tiles": (c_int, [c_void, ctypes.POINTER(TileConfig)]),
     "vmps_block_svd": (c_int, [c_int, P_i64, P_i64, P_dbl, P_dbl, P_dbl, P_dbl, P_int]),
     "vmps_block_svd_noalloc": (c_int, [c_int, P_i64, P_i64, P_dbl, P_dbl, P_dbl, P_dbl, P_int]),
+    "vmps_block_eigh_split": (c_int, [c_int, P_i64, P_i64, P_dbl, c_int, P_dbl, P_dbl, P_dbl]),
     "vmps_plan_set_tiles": (c_int, [c_void, ctypes.POINTER(TileConfig)]),
     "vmps_plan_autotune": (c_int, [c_void, c_int, ctypes.POINTER(TileConfig)]),
     "vmps_env_create": (c_int, [c_int, c_int, P_int, c_int, P_int, c_int, P_int, ctypes.POINTER(c_void)]),
@@ -148,6 +149,7 @@ SIGNATURES = {
     "vmps_lanczos_opts_default": (None, [ctypes.POINTER(LanczosOpts)]),
     "vmps_lanczos": (c_int, [c_void, P_dbl, ctypes.POINTER(LanczosOpts), ctypes.POINTER(LanczosResult)]),
     "vmps_lanczos_device": (c_int, [c_void, c_void, ctypes.POINTER(LanczosOpts), ctypes.POINTER(LanczosResult)]),
+    "vmps_lanczos_batch": (c_int, [c_void, P_dbl, ctypes.POINTER(LanczosOpts), ctypes.POINTER(LanczosResult)]),
 }
 
 _lib = None

@@ -15,10 +15,10 @@ OUT = os.path.join(HERE, "libvmps_hxv.so")
 BUILD = os.path.join(HERE, "_build")
 ARCH = os.environ.get("VMPS_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["sym.cpp", "pivot.cpp", "env.cpp", "capi.cpp", "capi_env.cpp", "engine.hip", "engine_bf16.hip",
-           "lanczos.hip", "svd.hip"]
+SOURCES = ["sym.cpp", "pivot.cpp", "env.cpp", "capi.cpp", "capi_env.cpp", "split.cpp", "engine.hip",
+           "engine_bf16.hip", "lanczos.hip", "lanczos_batch.hip", "svd.hip"]
 # C++ units that call the HIP runtime API are compiled in HIP mode
-HIP_HOST_UNITS = {"capi.cpp", "capi_env.cpp"}
+HIP_HOST_UNITS = {"capi.cpp", "capi_env.cpp", "split.cpp"}
 
 COMMON = [
     "-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function",
@@ -60,7 +60,9 @@ def build(verbose=True):
     with ThreadPoolExecutor(max_workers=min(8, len(SOURCES))) as ex:
         objs = list(ex.map(_compile, SOURCES))
     tmp = OUT + ".tmp"
-    cmd = [_hipcc(), "-shared", "-o", tmp] + objs + ["--offload-arch=" + ARCH, "-lamdhip64"]
+    # rocBLAS / rocSOLVER: the dense GEMM and eigensolver of the device split (split.cpp)
+    cmd = [_hipcc(), "-shared", "-o", tmp] + objs + ["--offload-arch=" + ARCH, "-lamdhip64", "-lrocsolver",
+                                                     "-lrocblas"]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError("link failed: %s\n%s\n%s" % (" ".join(cmd), res.stdout, res.stderr))

@@ -25,6 +25,9 @@ struct vmps_pivot2 {
 struct vmps_plan {
   vmps::DevPlan D;
   int npivots = 1;  // > 1: a batch of independent pivots (vmps_plan_create_batch)
+  bool batch = false;
+  std::vector<int64_t> offsets;  // batch: pivot i owns [offsets[i], offsets[i+1])
+  std::unique_ptr<vmps::BatchKrylov> bk;
 };
 
 namespace vmps {
@@ -497,6 +500,7 @@ vmps_status vmps_plan_create_batch(const vmps_pivot2* const* pivots, int n, vmps
           "unknown precision");
   // one triple set over the concatenated vectors: pivot i owns [off_i, off_i + vsize_i)
   vmps::TripleSet S;
+  std::vector<int64_t> starts;
   for (int i = 0; i < n; ++i) {
     require(pivots[i] != nullptr, VMPS_ERR_ARG, "NULL pivot in batch");
     require(pivots[i]->P.A0proj.empty(), VMPS_ERR_ARG, "projectors are not supported in a batch");
@@ -507,6 +511,7 @@ vmps_status vmps_plan_create_batch(const vmps_pivot2* const* pivots, int n, vmps
     const int64_t nx = (int64_t)S.Xb.size(), nz = (int64_t)S.Zb.size(), ny = (int64_t)S.Yb.size();
     const int64_t xo = (int64_t)S.Xdata.size(), zo = (int64_t)S.Zdata.size(), vo = S.ysize;
     if (offsets) offsets[i] = vo;
+    starts.push_back(vo);
     for (auto b : Si.Xb) S.Xb.push_back({b.off + xo, b.rows, b.cols});
     for (auto b : Si.Zb) S.Zb.push_back({b.off + zo, b.rows, b.cols});
     for (auto b : Si.Yb) S.Yb.push_back({b.off + vo, b.rows, b.cols});
@@ -519,9 +524,13 @@ vmps_status vmps_plan_create_batch(const vmps_pivot2* const* pivots, int n, vmps
     S.osize += Si.osize;
   }
   if (offsets) offsets[n] = S.ysize;
+  const int64_t S_ysize = S.ysize;
   auto plan = std::make_unique<vmps_plan>();
   plan->D.create_triples(std::move(S), prec);
   plan->npivots = n;
+  plan->batch = true;
+  for (int i = 0; i < n; ++i) plan->offsets.push_back(starts[i]);
+  plan->offsets.push_back(S_ysize);
   *out = plan.release();
   VMPS_API_END
 }
@@ -804,16 +813,19 @@ void vmps_lanczos_opts_default(vmps_lanczos_opts* o) {
   o->refine_tol_state = 1e-7;
   o->seed = 1;
   o->graph = 1;
+  o->refine_method = VMPS_REFINE_CORRECTION;
 }
 
 static vmps_status lanczos_impl(vmps_plan* plan, double* d_v, bool random_start, const vmps_lanczos_opts* opts,
                                 vmps_lanczos_result* res) {
   VMPS_API_BEGIN
   require(plan && opts && res, VMPS_ERR_ARG, "NULL argument");
-  require(plan->npivots == 1, VMPS_ERR_STATE,
+  require(!plan->batch || plan->npivots == 1, VMPS_ERR_STATE,
           "a batch plan holds independent pivots; use vmps_lanczos_batch for one eigenpair per pivot");
   require(opts->max_iter >= 1 && opts->tol_eigval >= 0 && opts->tol_state >= 0, VMPS_ERR_ARG, "bad options");
   require(opts->precision >= VMPS_PREC_FP64 && opts->precision <= VMPS_PREC_BF16X3, VMPS_ERR_ARG, "bad precision");
+  require(opts->refine_method == VMPS_REFINE_CORRECTION || opts->refine_method == VMPS_REFINE_RESTART, VMPS_ERR_ARG,
+          "bad refine_method");
   auto& D = plan->D;
   const auto t0 = std::chrono::steady_clock::now();
   std::memset(res, 0, sizeof(*res));
@@ -844,17 +856,36 @@ static vmps_status lanczos_impl(vmps_plan* plan, double* d_v, bool random_start,
   res->energy = r.energy;
   res->residual = r.residual;
   res->converged = r.converged;
+  res->hxv_fp64 = prec == VMPS_PREC_FP64 ? r.hxv_calls : 0;
   if (prec != VMPS_PREC_FP64 && opts->refine_fp64) {
-    vmps::LanczosOpts ro = lo;
-    ro.tol_eigval = opts->refine_tol_eigval;
-    ro.tol_state = opts->refine_tol_state;
-    ro.min_iter = 2;
-    vmps::LanczosResult f = vmps::lanczos_solve(D, d_v, ro, VMPS_PREC_FP64);
-    res->iterations_refine = f.iterations;
-    res->hxv_calls += f.hxv_calls;
-    res->energy = f.energy;
-    res->residual = f.residual;
-    res->converged = f.converged;
+    bool done = false;
+    if (opts->refine_method == VMPS_REFINE_CORRECTION) {
+      vmps::RefineOpts ro;
+      ro.tol_eigval = opts->refine_tol_eigval;
+      ro.tol_state = opts->refine_tol_state;
+      vmps::RefineResult f = vmps::refine_eigenpair(D, d_v, ro, prec);
+      res->refine_steps = f.outer;
+      res->refine_cg = f.cg_iterations;
+      res->hxv_calls += f.cg_iterations + f.hxv_fp64;
+      res->hxv_fp64 += f.hxv_fp64;
+      res->energy = f.energy;
+      res->residual = f.residual;
+      res->converged = f.converged;
+      done = f.converged;
+    }
+    if (!done) {  // RESTART, or corrections that did not reach the tolerances
+      vmps::LanczosOpts ro = lo;
+      ro.tol_eigval = opts->refine_tol_eigval;
+      ro.tol_state = opts->refine_tol_state;
+      ro.min_iter = 2;
+      vmps::LanczosResult f = vmps::lanczos_solve(D, d_v, ro, VMPS_PREC_FP64);
+      res->iterations_refine = f.iterations;
+      res->hxv_calls += f.hxv_calls;
+      res->hxv_fp64 += f.hxv_calls;
+      res->energy = f.energy;
+      res->residual = f.residual;
+      res->converged = f.converged;
+    }
   }
   res->ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   VMPS_API_END
@@ -885,6 +916,74 @@ vmps_status vmps_lanczos(vmps_plan* plan, double* v, const vmps_lanczos_opts* op
     VMPS_HIP_CHECK(hipMemcpyAsync(v, D.vin.p, bytes, hipMemcpyDeviceToHost, D.stream));
     VMPS_HIP_CHECK(hipStreamSynchronize(D.stream));
   }
+  VMPS_API_END
+}
+
+vmps_status vmps_lanczos_batch(vmps_plan* plan, double* v, const vmps_lanczos_opts* opts, vmps_lanczos_result* res) {
+  VMPS_API_BEGIN
+  require(plan && opts && res, VMPS_ERR_ARG, "NULL argument");
+  require(plan->batch, VMPS_ERR_STATE, "not a batch plan (vmps_plan_create_batch)");
+  require(opts->max_iter >= 1 && opts->tol_eigval >= 0 && opts->tol_state >= 0, VMPS_ERR_ARG, "bad options");
+  require(opts->precision >= VMPS_PREC_FP64 && opts->precision <= VMPS_PREC_BF16X3, VMPS_ERR_ARG, "bad precision");
+  auto& D = plan->D;
+  const int P = plan->npivots;
+  const auto t0 = std::chrono::steady_clock::now();
+  std::memset(res, 0, sizeof(*res) * (size_t)P);
+  const size_t bytes = (size_t)D.H.vsize * sizeof(double);
+  if (v && bytes) {
+    for (int64_t i = 0; i < D.H.vsize; ++i)
+      if (!std::isfinite(v[i])) throw VmpsError(VMPS_ERR_NONFINITE, "non-finite entry in the start vector");
+    VMPS_HIP_CHECK(hipMemcpyAsync(D.vin.p, v, bytes, hipMemcpyHostToDevice, D.stream));
+  } else if (bytes) {
+    vmps::dev_fill_uniform(D.vin.p, D.H.vsize, opts->seed, D.stream);
+  }
+  if (!plan->bk) {
+    plan->bk = std::make_unique<vmps::BatchKrylov>();
+    plan->bk->setup(D, plan->offsets);
+  }
+  vmps::LanczosOpts lo;
+  lo.max_iter = opts->max_iter;
+  lo.min_iter = opts->min_iter;
+  lo.tol_eigval = opts->tol_eigval;
+  lo.tol_state = opts->tol_state;
+  lo.graph = opts->graph != 0;
+  const auto prec = (vmps_precision)opts->precision;
+  const bool refine = prec != VMPS_PREC_FP64 && opts->refine_fp64;
+  if (refine) {  // the mixed phase stops at its rounding level (as vmps_lanczos)
+    const double state_floor = prec == VMPS_PREC_FP32 ? 1e-6 : 2e-4;
+    lo.tol_state = std::max(lo.tol_state, state_floor);
+    lo.tol_eigval = std::max(lo.tol_eigval, state_floor * state_floor);
+  }
+  std::vector<vmps::LanczosResult> r = vmps::lanczos_solve_batch(D, *plan->bk, D.vin.p, lo, prec);
+  for (int p = 0; p < P; ++p) {
+    res[p].iterations = r[p].iterations;
+    res[p].hxv_calls = r[p].hxv_calls;
+    res[p].energy = res[p].energy_mixed = r[p].energy;
+    res[p].residual = res[p].residual_mixed = r[p].residual;
+    res[p].converged = r[p].converged;
+    res[p].hxv_fp64 = prec == VMPS_PREC_FP64 ? r[p].hxv_calls : 0;
+  }
+  if (refine) {
+    vmps::LanczosOpts ro = lo;
+    ro.tol_eigval = opts->refine_tol_eigval;
+    ro.tol_state = opts->refine_tol_state;
+    ro.min_iter = 2;
+    std::vector<vmps::LanczosResult> f = vmps::lanczos_solve_batch(D, *plan->bk, D.vin.p, ro, VMPS_PREC_FP64);
+    for (int p = 0; p < P; ++p) {
+      res[p].iterations_refine = f[p].iterations;
+      res[p].hxv_calls += f[p].hxv_calls;
+      res[p].hxv_fp64 += f[p].hxv_calls;
+      res[p].energy = f[p].energy;
+      res[p].residual = f[p].residual;
+      res[p].converged = f[p].converged;
+    }
+  }
+  if (v && bytes) {
+    VMPS_HIP_CHECK(hipMemcpyAsync(v, D.vin.p, bytes, hipMemcpyDeviceToHost, D.stream));
+    VMPS_HIP_CHECK(hipStreamSynchronize(D.stream));
+  }
+  const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  for (int p = 0; p < P; ++p) res[p].ms = ms;
   VMPS_API_END
 }
 

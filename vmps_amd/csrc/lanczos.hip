@@ -495,6 +495,124 @@ LanczosResult lanczos_solve(DevPlan& D, double* d_v, const LanczosOpts& o, vmps_
   return res;
 }
 
+// ---- mixed-precision refinement ---------------------------------------------
+// Iterative refinement of an eigenpair in the manner of Jacobi-Davidson: the
+// residual r = H psi - E psi is formed with the FP64 operator H; the
+// correction t, orthogonal to psi, solves the projected equation
+//   (1 - psi psi^T)(H' - E)(1 - psi psi^T) t = -r
+// by conjugate gradients with the mixed-precision operator H' (the projected
+// operator is positive definite on the complement of psi while E is below
+// the second eigenvalue); then the FP64 operator's Rayleigh-Ritz pair on
+// span{psi, t} replaces psi.  An outer step costs one FP64 HxV (H psi is
+// carried along as w), so the FP64 work no longer grows with the Krylov
+// dimension: the error shrinks per step by about max(CG tolerance,
+// eps(H') ||H|| / gap).  Inner products carry the coeff_dot weights.
+RefineResult refine_eigenpair(DevPlan& D, double* d_v, const RefineOpts& o, vmps_precision inner) {
+  hipStream_t s = D.stream;
+  const int64_t n = D.H.vsize;
+  RefineResult res;
+  if (n == 0) return res;
+  Krylov& K = D.krylov;
+  K.ensure(D, std::max(2, K.cap - Krylov::kIterBatch));  // at least six vectors of workspace in Q
+  double* psi = d_v;
+  double* w = K.Q.p;   // H psi (FP64)
+  double* t = w + n;   // correction
+  double* g = t + n;   // CG residual; -r before the CG
+  double* pv = g + n;  // CG direction
+  double* q = pv + n;  // projected operator on pv; H t afterwards
+  const vmps_precision saved = D.prec;
+  auto apply = [&](vmps_precision p, const double* x, double* y) {
+    D.prec = p;
+    D.ensure_precision_buffers();
+    D.hxv(x, y, s);
+  };
+  auto dot = [&](const double* x, const double* y) { return D.dot(x, y, s); };
+  auto axpby = [&](double a, const double* x, double b, double* y) { dev_axpby(a, x, b, y, n, s); };
+  auto copy = [&](const double* x, double* y) {
+    hipLaunchKernelGGL(k_copy, dim3(vgrid(n)), dim3(256), 0, s, x, y, n);
+    VMPS_HIP_CHECK(hipGetLastError());
+  };
+  try {
+    apply(VMPS_PREC_FP64, psi, w);
+    ++res.hxv_fp64;
+    double E = dot(psi, w);
+    copy(w, g);
+    axpby(E, psi, -1.0, g);  // g = E psi - H psi = -r
+    double rho = std::sqrt(std::max(0.0, dot(g, g)));
+    for (;;) {
+      if (rho <= o.tol_state && rho * rho <= o.tol_eigval) {
+        res.converged = true;
+        break;
+      }
+      if (res.outer >= o.max_outer || !std::isfinite(rho)) break;
+      ++res.outer;
+      // CG on the projected correction equation, from t = 0 (residual g = -r);
+      // it stops once the correction is a few times finer than the step needs
+      const double cg_tol = std::min(0.1, std::max(1e-3, 0.3 * o.tol_state / rho));
+      VMPS_HIP_CHECK(hipMemsetAsync(t, 0, sizeof(double) * n, s));
+      copy(g, pv);
+      double gg = rho * rho;
+      for (int it = 0; it < o.cg_max; ++it) {
+        apply(inner, pv, q);
+        ++res.cg_iterations;
+        axpby(-E, pv, 1.0, q);
+        const double c = dot(psi, q);
+        axpby(-c, psi, 1.0, q);
+        const double pq = dot(pv, q);
+        if (!(pq > 0.0)) break;  // not positive definite here: keep the correction so far
+        const double alpha = gg / pq;
+        axpby(alpha, pv, 1.0, t);
+        axpby(-alpha, q, 1.0, g);
+        const double gg_new = dot(g, g);
+        if (std::sqrt(gg_new) <= cg_tol * rho) break;
+        axpby(1.0, g, gg_new / gg, pv);  // pv = g + beta pv
+        gg = gg_new;
+      }
+      // t -> unit vector orthogonal to psi
+      const double ct = dot(psi, t);
+      axpby(-ct, psi, 1.0, t);
+      const double nt = std::sqrt(std::max(0.0, dot(t, t)));
+      if (!(nt > 1e-300) || !std::isfinite(nt)) break;
+      axpby(0.0, t, 1.0 / nt, t);
+      // Rayleigh-Ritz of the FP64 operator on span{psi, t}
+      apply(VMPS_PREC_FP64, t, q);
+      ++res.hxv_fp64;
+      const double h11 = E, h12 = dot(psi, q), h22 = dot(t, q);
+      const double mean = 0.5 * (h11 + h22), half = 0.5 * (h11 - h22);
+      const double lam = mean - std::hypot(half, h12);
+      double c1 = h12, c2 = lam - h11;  // (H - lam) c = 0, two equivalent rows: take the longer
+      if (std::hypot(lam - h22, h12) > std::hypot(c1, c2)) {
+        c1 = lam - h22;
+        c2 = h12;
+      }
+      const double cn = std::hypot(c1, c2);
+      if (!(cn > 0.0)) break;
+      c1 /= cn;
+      c2 /= cn;
+      if (c1 < 0.0) {
+        c1 = -c1;
+        c2 = -c2;
+      }
+      axpby(c2, t, c1, psi);
+      axpby(c2, q, c1, w);
+      const double nrm = std::sqrt(dot(psi, psi));
+      axpby(0.0, psi, 1.0 / nrm, psi);
+      axpby(0.0, w, 1.0 / nrm, w);
+      E = dot(psi, w);
+      copy(w, g);
+      axpby(E, psi, -1.0, g);
+      rho = std::sqrt(std::max(0.0, dot(g, g)));
+    }
+    res.energy = E;
+    res.residual = rho;
+  } catch (...) {
+    D.prec = saved;
+    throw;
+  }
+  D.prec = saved;
+  return res;
+}
+
 void Krylov::ensure(DevPlan& D, int kmax) {
   const int64_t n = D.H.vsize;
   const int need = kmax + kIterBatch;
@@ -513,7 +631,12 @@ void Krylov::ensure(DevPlan& D, int kmax) {
   for (size_t sg = 0; sg < D.H.seg_off.size(); ++sg)
     for (int32_t e = 0; e < D.H.seg_len[sg]; ++e) hw[(size_t)(D.H.seg_off[sg] + e)] = D.H.seg_w[sg];
   ew.upload(hw, D.stream);
-  ngrp = (int)std::max<int64_t>(1, std::min<int64_t>(1024, (n + 1023) / 1024));  // 4 elements per thread
+  // chunks of 1024 elements (4 per thread) for long vectors; vectors of up to
+  // 64 x 8192 elements are cut into at most kFuseMaxGroups chunks instead, so
+  // an iteration takes the fused path (two kernels fewer, the launch-bound
+  // regime of small pivots)
+  ngrp = (int)std::max<int64_t>(1, std::min<int64_t>(1024, (n + 1023) / 1024));
+  if (n <= (int64_t)kFuseMaxGroups * 8192) ngrp = std::min<int>(ngrp, kFuseMaxGroups);
   chunk = (n + ngrp - 1) / ngrp;
   npart.alloc(vgrid(n));
   gpart.alloc((size_t)std::max(1, ngrp) * (need + 1));

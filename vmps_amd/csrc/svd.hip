@@ -59,6 +59,11 @@ __global__ __launch_bounds__(256) void k_jacobi_svd(const SvdTask* __restrict__ 
   const int64_t nn = (int64_t)n * n;
   for (int64_t e = threadIdx.x; e < nn; e += 256) Vb[e] = (e % (n + 1) == 0) ? 1.0 : 0.0;
   const int np = n + (n & 1);
+  // rotation threshold on |w_p . w_q| / (|w_p| |w_q|): a computed dot of m
+  // terms carries rounding of order sqrt(m) eps, so a threshold of eps alone
+  // can stay unreachable and the sweeps run to the limit (the one-sided
+  // Jacobi of LAPACK's dgesvj uses sqrt(m) eps the same way)
+  const double tol = kJacobiEps * fmax(1.0, sqrt((double)m));
   int sweep = 0;
   for (; sweep < kMaxSweeps && n > 1; ++sweep) {
     if (threadIdx.x == 0) rotated_any = 0;
@@ -81,7 +86,7 @@ __global__ __launch_bounds__(256) void k_jacobi_svd(const SvdTask* __restrict__ 
         a = wave_sum(a);
         b = wave_sum(b);
         g = wave_sum(g);
-        if (!(fabs(g) > kJacobiEps * sqrt(a * b)) || g == 0.0) continue;
+        if (!(fabs(g) > tol * sqrt(a * b)) || g == 0.0) continue;
         // rotation that zeroes the (p, q) entry of W^T W
         const double zeta = (b - a) / (2.0 * g);
         const double tt = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));

@@ -328,15 +328,30 @@ def _host_workers():
     return max(1, min(16, n))
 
 
+def svd_robust(m, full_matrices=False):
+    """Thin SVD by LAPACK's divide and conquer (gesdd, numpy), falling back to
+    the QR-iteration driver (gesvd, scipy) when gesdd reports that it did not
+    converge, which it can for some matrices that gesvd handles (the
+    reference picks Eigen's BDCSVD, or JacobiSVD with DONT_USE_BDCSVD, at
+    compile time, tensors/Biped.h:754-759).  Non-finite input is refused."""
+    if not np.all(np.isfinite(m)):
+        raise FloatingPointError("split: non-finite entries in the two-site wavefunction")
+    try:
+        return np.linalg.svd(m, full_matrices=full_matrices)
+    except np.linalg.LinAlgError:
+        import scipy.linalg
+        return scipy.linalg.svd(m, full_matrices=full_matrices, lapack_driver="gesvd")
+
+
 def block_svd_host(mats):
-    """Thin SVD of every matrix with host LAPACK (numpy's gesdd).  When the
+    """Thin SVD of every matrix with host LAPACK (svd_robust).  When the
     work is large enough, the blocks go to a thread pool, largest first,
     with one BLAS thread per call (LAPACK releases the GIL; nested BLAS
     threads would oversubscribe the cores).  The result of each block is the
     same as a single-threaded call's."""
     cost = [m.shape[0] * m.shape[1] * min(m.shape) for m in mats]
     if len(mats) < 2 or sum(cost) < 4_000_000:
-        return [np.linalg.svd(m, full_matrices=False) for m in mats]
+        return [svd_robust(m) for m in mats]
     if not _HOST_SVD:
         from concurrent.futures import ThreadPoolExecutor
         from threadpoolctl import ThreadpoolController
@@ -344,17 +359,56 @@ def block_svd_host(mats):
         _HOST_SVD["blas"] = ThreadpoolController()
     pool = _HOST_SVD["pool"]
     with _HOST_SVD["blas"].limit(limits=1, user_api="blas"):
-        futs = {i: pool.submit(np.linalg.svd, mats[i], False) for i in sorted(range(len(mats)), key=lambda i: -cost[i])}
+        futs = {i: pool.submit(svd_robust, mats[i], False) for i in sorted(range(len(mats)), key=lambda i: -cost[i])}
         return [futs[i].result() for i in range(len(mats))]
 
 
-def split_psi(pd, v, qloc, M, eps_svd=1e-14, left_to_right=True, svd=None):
+def block_eigh_device(mats, side):
+    """Density-matrix factorisation of every matrix of `mats` on the device
+    (vmps_block_eigh_split): side 0 gives (lambda, W, C) with
+    Psi Psi^T = W diag(lambda) W^T and C = W^T Psi, side 1 the same with
+    Psi^T Psi and C = Psi W; lambda descending (= sigma^2)."""
+    lib = _lib.load()
+    if not mats:
+        return []
+    rows = np.array([m.shape[0] for m in mats], dtype=np.int64)
+    cols = np.array([m.shape[1] for m in mats], dtype=np.int64)
+    k = rows if side == 0 else cols
+    A = np.concatenate([np.asarray(m, dtype=np.float64).ravel(order="F") for m in mats] + [np.zeros(1)])
+    lam = np.empty(int(k.sum()) + 1)
+    W = np.empty(int((k * k).sum()) + 1)
+    C = np.empty(A.size)
+    P = lambda a, t=ctypes.c_double: a.ctypes.data_as(ctypes.POINTER(t))
+    _lib.check(lib.vmps_block_eigh_split(len(mats), P(rows, ctypes.c_int64), P(cols, ctypes.c_int64), P(A), side,
+                                         P(lam), P(W), P(C)))
+    out, lo, wo, co = [], 0, 0, 0
+    for r, c, kk in zip(rows, cols, k):
+        r, c, kk = int(r), int(c), int(kk)
+        l_ = lam[lo:lo + kk][::-1].copy()
+        w_ = W[wo:wo + kk * kk].reshape((kk, kk), order="F")[:, ::-1]
+        c_ = C[co:co + r * c].reshape((r, c), order="F")
+        c_ = c_[::-1, :] if side == 0 else c_[:, ::-1]
+        out.append((l_, np.ascontiguousarray(w_), np.ascontiguousarray(c_)))
+        lo, wo, co = lo + kk, wo + kk * kk, co + r * c
+    return out
+
+
+def split_psi(pd, v, qloc, M, eps_svd=1e-14, left_to_right=True, svd=None, eigh=None, eps_rho=1e-15):
     """split_AA2 + truncateSVD: recouple the two-site blocks to the middle
-    sector Qm (coeff_splitAA), SVD per Qm, keep the M largest singular values
-    (multiplets) above eps_svd.  Returns (A_l, A_r, truncation weight =
-    sum of discarded deg(Qm) sigma^2, entropy, kept dims per Qm).  `svd` maps a
-    list of matrices to their thin SVDs (default: host LAPACK)."""
+    sector Qm (coeff_splitAA), factor each Qm, keep the M largest singular
+    values (multiplets) above eps_svd.  Returns (A_l, A_r, truncation weight =
+    sum of discarded deg(Qm) sigma^2, entropy, kept dims per Qm).
+
+    `svd` maps a list of matrices to their thin SVDs (default: host LAPACK).
+    With `eigh` (block_eigh_device, or a function that returns None to leave
+    the matrices to `svd`) the factorisation is that of the reduced
+    density matrix of the side that becomes orthonormal: sigma^2 are its
+    eigenvalues, the new site tensor its eigenvectors and the centre W^T Psi
+    (or Psi W), so no singular value is divided by; sigma^2 below eps_rho
+    (the eigenvalues' resolution relative to ||Psi||^2) count as discarded."""
     sym = pd.sym
+    if not np.all(np.isfinite(v)):
+        raise FloatingPointError("split: the two-site wavefunction has non-finite entries")
     clump = {}  # Qm -> {(row key, col key): block}
     rdims, cdims = {}, {}
     off = 0
@@ -391,14 +445,25 @@ def split_psi(pd, v, qloc, M, eps_svd=1e-14, left_to_right=True, svd=None):
             Mm[ro[rk]:ro[rk] + blk.shape[0], co[ck]:co[ck] + blk.shape[1]] = blk
         mats.append(Mm)
         meta.append((Qm, ro, co))
-    factors = (svd or block_svd_host)(mats)
-    svd = {}
+    fac = {}
     allsv = []
-    for (Qm, ro, co), (U, S, Vt) in zip(meta, factors):
-        svd[Qm] = (U, S, Vt, ro, co)
-        allsv.extend((sv, Qm, i) for i, sv in enumerate(S))
+    efac = eigh(mats, 0 if left_to_right else 1) if eigh is not None else None  # None: declined
+    if efac is not None:
+        norm2 = sum(float(np.sum(m * m)) for m in mats)
+        floor = max(eps_svd, math.sqrt(eps_rho * norm2))
+        for (Qm, ro, co), (lam, W, C) in zip(meta, efac):
+            S = np.sqrt(np.maximum(lam, 0.0))
+            X, Y = (W, C) if left_to_right else (C, W.T)  # X Y = Psi, X or Y orthonormal
+            fac[Qm] = (X, S, Y, ro, co)
+            allsv.extend((sv, Qm, i) for i, sv in enumerate(S))
+    else:
+        floor = eps_svd
+        for (Qm, ro, co), (U, S, Vt) in zip(meta, (svd or block_svd_host)(mats)):
+            X, Y = (U, S[:, None] * Vt) if left_to_right else (U * S[None, :], Vt)
+            fac[Qm] = (X, S, Y, ro, co)
+            allsv.extend((sv, Qm, i) for i, sv in enumerate(S))
     allsv.sort(key=lambda x: -x[0])
-    keep = [x for x in allsv[:M] if x[0] > eps_svd]
+    keep = [x for x in allsv[:M] if x[0] > floor]
     kept = {}
     for (sv, Qm, i) in keep:
         kept[Qm] = max(kept.get(Qm, 0), i + 1)
@@ -414,12 +479,8 @@ def split_psi(pd, v, qloc, M, eps_svd=1e-14, left_to_right=True, svd=None):
     Al = [dict() for _ in qloc]
     Ar = [dict() for _ in qloc]
     for Qm, k in kept.items():
-        U, S, Vt, ro, co = svd[Qm]
-        U, S, Vt = U[:, :k], S[:k], Vt[:k, :]
-        if left_to_right:
-            X, Y = U, S[:, None] * Vt
-        else:
-            X, Y = U * S[None, :], Vt
+        X, S, Y, ro, co = fac[Qm]
+        X, Y = X[:, :k], Y[:k, :]
         for (sl, qin), o in ro.items():
             Al[sl][(qin, Qm)] = X[o:o + rdims[Qm][(sl, qin)], :].copy()
         for (sr, qout), o in co.items():
@@ -447,19 +508,38 @@ class GpuBackend:
 
     With `device_envs` (default) every environment the sweep contracts stays
     on the device (env.DeviceEnv) and feeds the next contraction and pivot
-    without a host copy; the results are bit-identical to host environments."""
+    without a host copy; the results are bit-identical to host environments.
+
+    The split factors on the device through the reduced density matrix
+    (block_eigh_device) when a middle sector has both dimensions >=
+    device_split_min (default 256), on host LAPACK otherwise.  Pivots with at
+    least tune_min_flops of work per HxV get autotuned tiles (cached per
+    block-size histogram, so a sweep tunes once per shape)."""
 
     def __init__(self, precision="fp64", refine_fp64=True, tol_eigval=1e-10, tol_state=1e-6, max_iter=100,
-                 mixed_min_flops=4e9, device_svd_min=0, device_envs=True):
+                 mixed_min_flops=4e9, device_svd_min=0, device_envs=True, refine_method="correction",
+                 device_split_min=256, tune_min_flops=1e9):
         self.precision = precision
         self.device_envs = device_envs
         self.refine = refine_fp64
+        self.refine_method = refine_method
         self.tol_eigval, self.tol_state, self.max_iter = tol_eigval, tol_state, max_iter
         self.mixed_min_flops = mixed_min_flops
         self.device_svd_min = device_svd_min
+        self.device_split_min = device_split_min
+        self.tune_min_flops = tune_min_flops
         self.hxv_calls = 0
         self.mixed_solves = 0
         self.device_svds = 0
+        self.device_splits = 0
+
+    def eigh(self, mats, side):
+        """Density-matrix factorisation of the split on the device, or None
+        (host SVD) when every middle sector is small."""
+        if not self.device_split_min or not mats or max(min(m.shape) for m in mats) < self.device_split_min:
+            return None
+        self.device_splits += 1
+        return block_eigh_device(mats, side)
 
     def svd(self, mats):
         """Two-site split SVDs: on the device (vmps_block_svd) when
@@ -475,15 +555,24 @@ class GpuBackend:
     def solve(self, pd, v0):
         from .pivot import PivotMatrix2, lanczos
         H = PivotMatrix2(pd, precision="fp64")
+        flops = H.stats()["flops_alg"]
+        tune = self.tune_min_flops is not None and flops >= self.tune_min_flops
+        if tune:
+            H.autotune(iters=2)
         prec = self.precision
-        if prec != "fp64" and H.stats()["flops_alg"] < self.mixed_min_flops:
+        if prec != "fp64" and flops < self.mixed_min_flops:
             prec = "fp64"
         if prec != "fp64":
             H.set_precision(prec)
+            if tune:
+                H.autotune(iters=2)
             self.mixed_solves += 1
         v, r = lanczos(H, v0=v0, precision=prec, refine_fp64=self.refine, max_iter=self.max_iter,
                        tol_eigval=self.tol_eigval, tol_state=self.tol_state,
-                       refine_tol_eigval=self.tol_eigval, refine_tol_state=self.tol_state)
+                       refine_tol_eigval=self.tol_eigval, refine_tol_state=self.tol_state,
+                       refine_method=self.refine_method)
+        r["flops_alg"] = H.stats()["flops_alg"]
+        r["precision"] = prec
         H.close()
         self.hxv_calls += r["hxv_calls"]
         return r["energy"], v, r
@@ -551,6 +640,16 @@ class Chain:
         return pd
 
 
+def build_envs(chain, backend, p):
+    """Environments Lenv[1..p] and Renv[p+2..L-1] of the pivot (p, p+1),
+    contracted from the chain's ends through the backend."""
+    for b in range(1, p + 1):
+        chain.Lenv[b] = backend.contract_L(chain.sites[b - 1], chain.Lenv[b - 1], chain.A[b - 1])
+    for b in range(chain.L - 1, p + 1, -1):
+        chain.Renv[b] = backend.contract_R(chain.sites[b], chain.Renv[b + 1], chain.A[b])
+    return chain
+
+
 def center_chain(sites, Q, M, p, backend, rng, bonds=None):
     """A random state prepared for two-site steps at the pivot (p, p+1): sites
     left of p left-normalized, sites right of p+1 right-normalized, and the
@@ -561,12 +660,131 @@ def center_chain(sites, Q, M, p, backend, rng, bonds=None):
     A = random_blocks(sym, qloc, bonds, rng)
     A, bonds = left_normalize(A, bonds, qloc, sym, p)
     A, bonds = right_normalize(A, bonds, qloc, sym, stop=p + 2)
-    chain = Chain(sites, Q, M, A, bonds)
-    for b in range(1, p + 1):
-        chain.Lenv[b] = backend.contract_L(sites[b - 1], chain.Lenv[b - 1], chain.A[b - 1])
-    for b in range(L - 1, p + 1, -1):
+    return build_envs(Chain(sites, Q, M, A, bonds), backend, p)
+
+
+def warm_chain(sites, Q, schedule, p, backend, rng, log=None):
+    """A DMRG state converged at small bond dimensions, ready for steps at the
+    pivot (p, p+1): from a random state, one full sweep (right, then left) at
+    each M of `schedule`, then a right half sweep up to the pivot (p-1, p), so
+    that sites < p are left-normalized, site p carries the state and sites >
+    p are right-normalized."""
+    L = len(sites)
+    sym, qloc = sites[0].sym, sites[0].qloc
+    A, bonds = random_mps(sites, Q, schedule[0], rng)
+    A, bonds = right_normalize(A, bonds, qloc, sym)
+    chain = Chain(sites, Q, schedule[0], A, bonds)
+    for b in range(L - 1, 1, -1):
         chain.Renv[b] = backend.contract_R(sites[b], chain.Renv[b + 1], chain.A[b])
+    for M in schedule:
+        chain.M = M
+        t0 = time.perf_counter()
+        for l in range(0, L - 1):
+            E, _, _ = dmrg_step(chain, backend, l, "right", rng=rng)
+        for l in range(L - 2, -1, -1):
+            E, _, _ = dmrg_step(chain, backend, l, "left", rng=rng)
+        if log:
+            log("warm-up sweep M=%d  E = %.10f  (%.1f s)" % (M, E, time.perf_counter() - t0))
+    for l in range(0, p):
+        E, _, _ = dmrg_step(chain, backend, l, "right", rng=rng)
+    chain.energy = E
     return chain
+
+
+def _orth_complement(Old, ncols, rng):
+    """ncols orthonormal columns orthogonal to the orthonormal columns of Old
+    (random, two rounds of projection, then QR)."""
+    N = rng.standard_normal((Old.shape[0], ncols))
+    for _ in range(2):
+        if Old.shape[1]:
+            N -= Old @ (Old.T @ N)
+    q, _ = np.linalg.qr(N)
+    return q
+
+
+def expand_chain(chain, M, p, backend, rng):
+    """The same state with larger bonds: every bond grows towards the sectors
+    of su2_gaussian_bonds(M) (never below its current dimensions), the new
+    basis states orthonormal completions of the old ones, so sites < p stay
+    left-normalized, sites > p right-normalized, and site p holds the old
+    state in its old-old block (zeros elsewhere).  The environments of the
+    pivot (p, p+1) are rebuilt.  This is a DMRG state right after its bond
+    dimension was raised: the next two-site steps start from an excellent
+    guess in a much larger space."""
+    sym, qloc, L = chain.sym, chain.qloc, chain.L
+    old, A = chain.bonds, chain.A
+    tgt = su2_gaussian_bonds(sym, qloc, L, chain.Q, M)
+    new = [dict(b) for b in old]
+    newA = [None] * L
+    # left part: site l defines bond l + 1
+    for l in range(p):
+        site = [dict() for _ in qloc]
+        nb = {}
+        for qout in sorted(set(old[l + 1]) | set(tgt[l + 1])):
+            parts = [(s, qin) for s in range(len(qloc)) for qin in sorted(new[l]) if qout in sym.reduce(qin, qloc[s])]
+            rows = sum(new[l][qin] for _, qin in parts)
+            d_old = old[l + 1].get(qout, 0)
+            D = min(max(d_old, tgt[l + 1].get(qout, 0)), rows)
+            if D == 0:
+                continue
+            X = np.zeros((rows, D))
+            r0, offs = 0, []
+            for (s, qin) in parts:
+                blk = A[l][s].get((qin, qout))
+                if blk is not None and d_old:
+                    X[r0:r0 + blk.shape[0], :d_old] = blk
+                offs.append(r0)
+                r0 += new[l][qin]
+            if D > d_old:
+                X[:, d_old:] = _orth_complement(X[:, :d_old], D - d_old, rng)
+            for (s, qin), o in zip(parts, offs):
+                site[s][(qin, qout)] = np.ascontiguousarray(X[o:o + new[l][qin], :])
+            nb[qout] = D
+        new[l + 1] = nb
+        newA[l] = site
+    # right part: site l defines bond l
+    for l in range(L - 1, p, -1):
+        site = [dict() for _ in qloc]
+        nb = {}
+        for qin in sorted(set(old[l]) | set(tgt[l])):
+            parts = [(s, qout) for s in range(len(qloc)) for qout in sorted(new[l + 1])
+                     if qout in sym.reduce(qin, qloc[s])]
+            cols = sum(new[l + 1][qo] for _, qo in parts)
+            d_old = old[l].get(qin, 0)
+            D = min(max(d_old, tgt[l].get(qin, 0)), cols)
+            if D == 0:
+                continue
+            w = [math.sqrt(sym.coeff_dot(qo) / sym.coeff_dot(qin)) for _, qo in parts]
+            Y = np.zeros((D, cols))
+            c0, offs = 0, []
+            for (s, qo), wi in zip(parts, w):
+                blk = A[l][s].get((qin, qo))
+                if blk is not None and d_old:
+                    Y[:d_old, c0:c0 + blk.shape[1]] = wi * blk
+                offs.append(c0)
+                c0 += new[l + 1][qo]
+            if D > d_old:
+                Y[d_old:, :] = _orth_complement(Y[:d_old, :].T, D - d_old, rng).T
+            for (s, qo), wi, o in zip(parts, w, offs):
+                site[s][(qin, qo)] = np.ascontiguousarray(Y[:, o:o + new[l + 1][qo]] / wi)
+            nb[qin] = D
+        new[l] = nb
+        newA[l] = site
+    # centre: the old state in the old-old block
+    site = [dict() for _ in qloc]
+    for s in range(len(qloc)):
+        for qin, din in new[p].items():
+            for qout in sym.reduce(qin, qloc[s]):
+                if qout not in new[p + 1]:
+                    continue
+                blk = np.zeros((din, new[p + 1][qout]))
+                b = A[p][s].get((qin, qout))
+                if b is not None:
+                    blk[:b.shape[0], :b.shape[1]] = b
+                site[s][(qin, qout)] = blk
+    newA[p] = site
+    out = Chain(chain.sites, chain.Q, M, newA, new)
+    return build_envs(out, backend, p)
 
 
 def dmrg_step(chain, backend, l, direction, rng=None, timings=None):
@@ -586,7 +804,7 @@ def dmrg_step(chain, backend, l, direction, rng=None, timings=None):
     E, v, info = backend.solve(pd, v0)
     t2 = t()
     Al, Ar, tw, ent, kept = split_psi(pd, v, chain.qloc, chain.M, left_to_right=(direction == "right"),
-                                      svd=getattr(backend, "svd", None))
+                                      svd=getattr(backend, "svd", None), eigh=getattr(backend, "eigh", None))
     chain.A[l], chain.A[l + 1] = Al, Ar
     chain.bonds[l + 1] = {qm: k for qm, k in kept.items()}
     t3 = t()

@@ -196,10 +196,15 @@ def dot(H, x, y):
     return r.value
 
 
+REFINE = {"correction": 0, "restart": 1}
+
+
 def lanczos(H, v0=None, precision="fp64", refine_fp64=True, max_iter=100, tol_eigval=1e-8, tol_state=1e-5,
-            refine_tol_eigval=1e-12, refine_tol_state=1e-7, seed=1, graph=True):
+            refine_tol_eigval=1e-12, refine_tol_state=1e-7, seed=1, graph=True, refine_method="correction"):
     """Lowest eigenpair of the pivot (the LanczosSolver::edgeState call of
-    DmrgSolver::iteration_two).  Returns (eigenvector, result dict)."""
+    DmrgSolver::iteration_two).  A mixed-precision solve is refined to the
+    FP64 tolerances by corrections ("correction") or an FP64 restart
+    ("restart").  Returns (eigenvector, result dict)."""
     o = _lib.LanczosOpts()
     H._lib.vmps_lanczos_opts_default(ctypes.byref(o))
     o.max_iter, o.tol_eigval, o.tol_state = max_iter, tol_eigval, tol_state
@@ -208,12 +213,36 @@ def lanczos(H, v0=None, precision="fp64", refine_fp64=True, max_iter=100, tol_ei
     o.refine_tol_eigval, o.refine_tol_state = refine_tol_eigval, refine_tol_state
     o.seed = seed
     o.graph = 1 if graph else 0
+    o.refine_method = REFINE[refine_method]
     r = _lib.LanczosResult()
     if v0 is None:  # GaussianRandomVector::fill: uniform(-1, 1) entries
         v0 = np.random.default_rng(seed).uniform(-1.0, 1.0, H.vsize)
     v = np.ascontiguousarray(v0, dtype=np.float64).copy()
     _lib.check(H._lib.vmps_lanczos(H._plan, _ptr(v), ctypes.byref(o), ctypes.byref(r)))
     return v, r.as_dict()
+
+
+def lanczos_batch(H, v0=None, precision="fp64", refine_fp64=True, max_iter=100, tol_eigval=1e-8, tol_state=1e-5,
+                  refine_tol_eigval=1e-12, refine_tol_state=1e-7, seed=1, graph=True):
+    """One lowest eigenpair per pivot of a PivotBatch, all solved together
+    (the per-site solves of VumpsSolver, VUMPS/VumpsSolver.h:1168-1211).
+    Returns (batch eigenvector, list of per-pivot result dicts)."""
+    if not isinstance(H, PivotBatch):
+        raise TypeError("lanczos_batch needs a PivotBatch")
+    o = _lib.LanczosOpts()
+    H._lib.vmps_lanczos_opts_default(ctypes.byref(o))
+    o.max_iter, o.tol_eigval, o.tol_state = max_iter, tol_eigval, tol_state
+    o.precision = _lib.PREC[precision]
+    o.refine_fp64 = 1 if refine_fp64 else 0
+    o.refine_tol_eigval, o.refine_tol_state = refine_tol_eigval, refine_tol_state
+    o.seed = seed
+    o.graph = 1 if graph else 0
+    res = (_lib.LanczosResult * len(H.pds))()
+    if v0 is None:
+        v0 = np.random.default_rng(seed).uniform(-1.0, 1.0, H.vsize)
+    v = np.ascontiguousarray(v0, dtype=np.float64).copy()
+    _lib.check(H._lib.vmps_lanczos_batch(H._plan, _ptr(v), ctypes.byref(o), res))
+    return v, [r.as_dict() for r in res]
 
 
 def get_blockstructure(lib, h, prefix="vmps_"):

@@ -335,9 +335,10 @@ WORKLOADS = {
     # two-leg Hubbard ladder 2 x 32 in snake order, spin SU(2) x charge U(1),
     # t=1, U=6, filling n = 0.875, M = 2048 multiplets, D_W = 6 (rung + leg hopping)
     "hubbard_su2xu1_ladder_L64_M2048": dict(model="HubbardSU2xU1Ladder", L=64, M=2048, filling=0.875),
-    # long-range SU(2) Heisenberg, 60 sites, couplings up to distance 38
-    # (D_W = 40, the bandwidth of the snake-ordered C60 graph's Jfull MPO),
-    # M = 4096 multiplets; smaller variants for parity tests
+    # long-range SU(2) Heisenberg, 60 sites, random couplings up to distance
+    # 38 (D_W = 40), M = 4096 multiplets: a synthetic large-block pivot with a
+    # wider MPO than the C60 Jfull model's (at most 12 multiplet channels,
+    # lattice.cut_ranks); smaller variants for parity tests
     "heisenberg_su2_lr40_L60_M4096": dict(model="HeisenbergSU2LR", L=60, M=4096, R=38),
     "heisenberg_su2_lr40_L60_M1024": dict(model="HeisenbergSU2LR", L=60, M=1024, R=38),
 }
