@@ -1,27 +1,41 @@
-"""Benchmark: effective-Hamiltonian application (HxV) on one MI355X.
+"""Benchmark on one MI355X: two-site DMRG steps of the C60 Heisenberg model
+(HeisenbergSU2 with the full coupling matrix of the truncated icosahedron,
+M = 4096 SU(2) multiplets), mixed precision against plain FP64, at the same
+energies.
 
-One step = one HxV of the workload's two-site pivot (or of every pivot of a
-batched workload, in one launch per stage) on device-resident vectors.
-Default workload: HubbardU1xU1 chain, L=48, half filling, U=8, t=1, M=512,
-D_W=6, pivot in the middle of the chain with random N(0,1) environment and
-wavefunction blocks (synthetic data).  The metric value is the algorithmic
-rate F_alg / t in GFLOP/s, where F_alg counts the staged contraction
-(distinct L.V products plus distinct U.R products).
+Workload (default, --workload c60): the 90 nearest-neighbour bonds of C60
+(J = 1) in Cuthill-McKee chain order, the SU(2)-reduced MPO at its minimal
+bond dimension (2 + rank of the couplings across each bond, at most 12
+multiplets), total spin 0.  The state: DMRG from a random start, one full
+sweep at each M of --warm (32, 64, 128, 256 multiplets), then every bond
+expanded towards M = 4096 with orthonormal completions of the old basis (the
+state unchanged; see vmps_amd.dmrg.expand_chain), environments rebuilt on
+the device.  Synthetic data: no checkpoint is read.
 
-Each plan is autotuned first (its tile configuration is timed over the
-candidates and the fastest kept; `tiles` key).  The same workload is also
-timed in the mixed modes, FP32 MFMA and split-BF16x3 with FP32 accumulation
-(`mixed_hxv`: rate, speedup over FP64 and the relative difference of the
-result from the FP64 one).
+One step = one two-site DMRG step sweeping right through the middle of the
+chain (DmrgSolver::iteration_two): contract_AA2, the lowest eigenpair of
+H_eff to tol_eigval 1e-10 / residual 1e-6, the split with truncation to M,
+and the environment update.  --warmup W steps run first (untimed), then
+--steps K steps are timed.  Both legs start from the same state and walk the
+same pivots:
+  mixed: split-BF16x3 Lanczos (FP64 vectors, dots and reorthogonalisation)
+         refined to the FP64 tolerances by FP64-residual corrections;
+  fp64:  plain FP64 Lanczos.
+`value` is the mixed leg's throughput: algorithmic HxV flops (F_alg per HxV
+call, every call of the step) per second of step wall time, split, host work
+and environment update included.  `dmrg` holds both legs: per-step energies,
+the largest |E_mixed - E_fp64| over the timed steps, HxV calls, and the step
+time of each phase.
 
-The default run also times one large-block pivot (`--large`, default the
-long-range SU(2) Heisenberg pivot with D_W = 40 and M = 4096 multiplets, 4.5
-TFLOP per HxV) in FP64, FP32 and split-BF16x3 (`large_pivot`), where the
-contraction is MFMA- rather than latency-bound.
+`roofline`: the dominant kernel of the mixed HxV on the first timed pivot,
+its share of the staged work over its average launch time (HIP events on the
+plan's stream), against 2.5 PFLOP/s / 3 (dense BF16 MFMA, three products per
+FP32-equivalent one).  `cpu_baseline`: the oracle (the reference's literal
+per-term (L.V).R loop in C, OpenMP over output blocks, all host cores) on
+that pivot, one HxV, plus a single-core sample and the host's dgemm rate.
 
-Beside the HxV line the default run also times full two-site DMRG sweeps of
-the HeisenbergU1 L=32 M=64 chain, FP64 and mixed (split-BF16x3 Lanczos refined
-in FP64), and reports the energy difference (`sweeps` key).
+Other workloads (--workload NAME of vmps_amd.synth.WORKLOADS / BATCHES) time
+the HxV alone on a random-environment pivot (one step = one HxV).
 
 Prints one JSON line.  --gpus N with N > 1 is not supported (one GPU).
 """
@@ -112,71 +126,277 @@ def sweep_bench(args, model="HeisenbergU1", L=32, M=64, sweeps=4):
     return out
 
 
-def large_pivot_bench(args, name, iters=3):
-    """One large-block pivot (the MFMA-bound regime) in FP64, FP32 and
-    split-BF16x3, each autotuned: rate, share of the dtype's MFMA peak, and the
-    mixed results' relative difference from the FP64 one."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline_pivot(pd, falg, vin, threads, single_seconds=4.0):
+    """The oracle on one pivot: one HxV with OpenMP over output blocks on
+    `threads` cores (the rate in F_alg units, so it compares with the GPU
+    figures), a bounded single-core sample (the literal loop over the first
+    output entries until single_seconds have passed, its rate and the
+    projected single-core time of a whole HxV) and the host's single-core
+    dgemm rate on 512^3.  Returns (record, the oracle's HxV output)."""
+    import oracle
     from vmps_amd import synth
-    from vmps_amd.pivot import HxV, PivotMatrix2
-    pd = synth.make_workload(name, seed=args.seed)
-    v = synth.random_vector(pd, args.seed + 1)
-    out = {"workload": name}
-    ref = None
+    import copy
+    lib = oracle.load()
+    P = ctypes.POINTER(ctypes.c_double)
+    pd = copy.copy(pd)  # the oracle takes host environments
+    pd.L = pd.L.to_host() if hasattr(pd.L, "handle") else pd.L
+    pd.R = pd.R.to_host() if hasattr(pd.R, "handle") else pd.R
+    h = synth.feed(lib, "oracle_", pd, oracle.check)
+    oracle.check(lib.oracle_pivot2_precalc(h))
+    fref = ctypes.c_double()
+    oracle.check(lib.oracle_flops_ref(h, ctypes.byref(fref)))
+    out = np.zeros_like(vin)
+    t0 = time.perf_counter()
+    oracle.check(lib.oracle_hxv_omp(h, vin.ctypes.data_as(P), out.ctypes.data_as(P), threads))
+    t_omp = time.perf_counter() - t0
+    # single core: output entries in order until the budget is spent
+    nl = lib.oracle_nlhs(h)
+    scratch = np.zeros_like(vin)
+    q, t_single, f_single = 0, 0.0, 0.0
+    while q < nl and t_single < single_seconds:
+        q1 = min(nl, q + max(1, nl // 64))
+        fr = ctypes.c_double()
+        oracle.check(lib.oracle_flops_ref_range(h, q, q1, ctypes.byref(fr)))
+        t0 = time.perf_counter()
+        oracle.check(lib.oracle_hxv_range(h, vin.ctypes.data_as(P), scratch.ctypes.data_as(P), q, q1))
+        t_single += time.perf_counter() - t0
+        f_single += fr.value
+        q = q1
+    lib.oracle_pivot2_destroy(h)
+    single_rate = f_single / t_single if t_single > 0 else 0.0
+    dgemm = lib.oracle_dgemm_gflops(512, 3)
+    return {
+        "value": falg / t_omp / 1e9,
+        "unit": "GFLOP/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": "one HxV of the oracle (literal per-term (L.V).R loop of DmrgPivotMatrix2.h:161-231 in C, "
+                  "OpenMP over output blocks, %s) on the first timed C60 pivot: %.2f s; rate in F_alg units "
+                  "(the literal loop does %.2fx more flops: %.1f GFLOP/s of its own work)"
+                  % (lib.variant, t_omp, fref.value / falg, fref.value / t_omp / 1e9),
+        "seconds_per_hxv": t_omp,
+        "cpu_model": cpu_model(),
+        "single_core": {"literal_loop_gflops": single_rate / 1e9,
+                        "sample": "%d of %d output entries, %.2f s" % (q, nl, t_single),
+                        "projected_seconds_per_hxv": fref.value / single_rate if single_rate else None},
+        "dgemm_512_gflops_single_core": dgemm,
+    }, out
+
+
+def _c2_hxv_lines(args):
+    """HubbardU1xU1 L=48 M=512 random-environment pivot: HxV time and rate in
+    FP64, FP32 and split-BF16x3 (autotuned), a few seconds in all."""
+    from vmps_amd import synth
+    from vmps_amd.pivot import PivotMatrix2
+    pd = synth.make_workload("hubbard_u1xu1_L48_M512", seed=args.seed)
+    out = {"workload": "hubbard_u1xu1_L48_M512"}
     for prec in ("fp64", "fp32", "bf16x3"):
         H = PivotMatrix2(pd, precision=prec)
         st = H.stats()
-        tiles = H.autotune(iters=iters)
-        ms = H.time_hxv(warmup=1, iters=iters)
-        o = HxV(H, v)
-        if ref is None:
-            ref = o
-            out.update({"flops_alg": st["flops_alg"], "max_block_dim": st["max_block_dim"], "vsize": int(H.vsize)})
-        tf = st["flops_alg"] / (ms * 1e-3) / 1e12
-        out[prec] = {"ms_per_hxv": ms, "tflops": tf, "frac_of_peak": tf / PEAK_TFLOPS[prec],
-                     "rel_err_vs_fp64": float(np.linalg.norm(o - ref) / np.linalg.norm(ref)),
-                     "tiles": {k: tiles[k] for k in ("wave_edge", "big_min", "big_min64", "explicit_u")}}
+        H.autotune(iters=10)
+        ms = H.time_hxv(warmup=20, iters=400)
+        out[prec] = {"ms_per_hxv": ms, "gflops": st["flops_alg"] / (ms * 1e-3) / 1e9}
         H.close()
-    out["speedup_bf16x3_vs_fp64"] = out["fp64"]["ms_per_hxv"] / out["bf16x3"]["ms_per_hxv"]
     return out
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2000)
-    ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="hubbard_u1xu1_L48_M512")
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32", "bf16x3"])
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-tune", action="store_true", help="keep the default tile configuration")
-    ap.add_argument("--tiles", default=None,
-                    help="fixed tile configuration wave_edge,big_min,big_min64,explicit_u (instead of autotuning)")
-    ap.add_argument("--no-mixed", action="store_true", help="skip the FP32 / split-BF16 x3 HxV lines")
-    ap.add_argument("--seed", type=int, default=20261202)
-    ap.add_argument("--large", default="heisenberg_su2_lr40_L60_M4096",
-                    help="MFMA-regime pivot timed in FP64 and the mixed modes beside the main line ('' = skip)")
-    ap.add_argument("--sweeps", type=int, default=4, help="DMRG sweeps of the HeisenbergU1 L=32 M=64 chain (0: none)")
-    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
-                    help="only run 5 device copies of this many MiB (reference kernel for the HBM byte counters)")
-    args = ap.parse_args()
-    if args.calibrate_copy_mib:
-        import torch
-        n = args.calibrate_copy_mib * (1 << 20) // 8
-        src = torch.ones(n, dtype=torch.float64, device="cuda")
-        dst = torch.empty_like(src)
-        for _ in range(5):
-            dst.copy_(src)
+def dmrg_c60_bench(args):
+    import torch
+    from vmps_amd import dmrg, lattice, models, synth
+    from vmps_amd.pivot import HxV, PivotMatrix2
+
+    assert torch.cuda.is_available(), "no HIP device"
+
+    def log(*a):
+        print(*a, file=sys.stderr, flush=True)
+
+    L = 60
+    J = lattice.c60_couplings()
+    sites = models.chain_mpo("HeisenbergSU2Full", L, J=J)
+    p0 = args.pivot
+    nsteps = args.warmup + args.steps
+    if p0 + nsteps > L - 1:
+        raise SystemExit("bench.py: --pivot + --warmup + --steps must stay within the chain (<= %d)" % (L - 1))
+    tol = dict(tol_eigval=1e-10, tol_state=1e-6, max_iter=300)
+    prep = dmrg.GpuBackend(precision="fp64", **tol)
+    t0 = time.perf_counter()
+    warm = dmrg.warm_chain(sites, (1,), [int(x) for x in args.warm.split(",") if x], p0, prep,
+                           np.random.default_rng(args.seed), log=log)
+    t_warm = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    big = dmrg.expand_chain(warm, args.M, p0, prep, np.random.default_rng(args.seed + 1))
+    t_expand = time.perf_counter() - t0
+    log("state ready: warm-up %.1f s (E = %.10f), expansion to M=%d %.1f s" % (t_warm, warm.energy, args.M, t_expand))
+
+    if args.hxv_reps:
+        # profiling target: only HxV of the first timed pivot, mixed and FP64
+        c = big.copy()
+        for i in range(args.warmup):
+            dmrg.dmrg_step(c, prep, p0 + i, "right")
+        pd = c.pivot(p0 + args.warmup)
+        v = synth.random_vector(pd, args.seed + 7)
+        rec = {"workload": "c60_heisenberg_su2_M%d" % args.M, "pivot": p0 + args.warmup, "vsize": int(pd.vsize)}
+        for prec in (args.mixed, "fp64"):
+            H = PivotMatrix2(pd, precision=prec)
+            rec["tiles_" + prec] = H.autotune(iters=2)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(args.hxv_reps):
+                HxV(H, v)
+            torch.cuda.synchronize()
+            rec[prec + "_ms_per_hxv_incl_copies"] = (time.perf_counter() - t0) / args.hxv_reps * 1e3
+            rec["flops_alg"] = H.stats()["flops_alg"]
+            H.close()
+        return rec
+
+    legs = {}
+    first_pd = None
+    for leg, prec in [("mixed", args.mixed)] + ([] if args.no_fp64_leg else [("fp64", "fp64")]):
+        c = big.copy()
+        be = dmrg.GpuBackend(precision=prec, mixed_min_flops=0, **tol)
+        steps = []
+        t_timed0 = None
+        for i in range(nsteps):
+            l = p0 + i
+            if i == args.warmup:
+                if leg == "mixed":
+                    first_pd = c.pivot(l)
+                torch.cuda.synchronize()
+                t_timed0 = time.perf_counter()
+            tm = {}
+            ts = time.perf_counter()
+            E, tw, info = dmrg.dmrg_step(c, be, l, "right", timings=tm)
+            dt = time.perf_counter() - ts
+            steps.append({"pivot": l, "timed": i >= args.warmup, "energy": E, "truncation": tw, "seconds": dt,
+                          "phases": {k: round(v, 4) for k, v in tm.items()},
+                          "hxv_calls": info["hxv_calls"], "hxv_fp64": info["hxv_fp64"],
+                          "iterations": info["iterations"], "iterations_refine": info["iterations_refine"],
+                          "refine_steps": info["refine_steps"], "refine_cg": info["refine_cg"],
+                          "residual": info["residual"], "flops_alg": info["flops_alg"]})
+            log("%s step %d (pivot %d%s): E = %.12f  %.2f s  HxV %d (FP64 %d)  residual %.1e"
+                % (leg, i, l, "" if i >= args.warmup else ", warm-up", E, dt, info["hxv_calls"], info["hxv_fp64"],
+                   info["residual"]))
         torch.cuda.synchronize()
-        print(json.dumps({"calibration_copy_bytes": n * 8, "copies": 5}))
-        return
+        t_timed = time.perf_counter() - t_timed0
+        timed = [st for st in steps if st["timed"]]
+        work = sum(st["flops_alg"] * st["hxv_calls"] for st in timed)
+        phases = {}
+        for st in timed:
+            for k, v in st["phases"].items():
+                phases[k] = phases.get(k, 0.0) + v
+        legs[leg] = {"precision": prec, "seconds_timed": t_timed, "ms_per_step": t_timed / args.steps * 1e3,
+                     "gflops": work / t_timed / 1e9, "hxv_calls": sum(st["hxv_calls"] for st in timed),
+                     "hxv_fp64": sum(st["hxv_fp64"] for st in timed),
+                     "phase_seconds": {k: round(v, 3) for k, v in phases.items()}, "steps": steps}
+    mx = legs["mixed"]
+    dmrg_rec = {"legs": legs}
+    if "fp64" in legs:
+        f64 = legs["fp64"]
+        dE = [abs(a["energy"] - b["energy"]) for a, b in zip(mx["steps"], f64["steps"]) if a["timed"]]
+        dmrg_rec.update({"max_abs_dE_mixed_vs_fp64": max(dE), "dE_per_step": dE,
+                         "step_speedup_mixed_vs_fp64": f64["seconds_timed"] / mx["seconds_timed"],
+                         "fp64_ms_per_step": f64["ms_per_step"], "fp64_gflops": f64["gflops"]})
 
-    if args.gpus != 1:
-        print("bench.py: the HxV runs on one GPU; --gpus %d is not supported "
-              "(a pivot would shard over its output blocks across GPUs; no multi-GPU path is implemented)"
-              % args.gpus, file=sys.stderr)
-        sys.exit(2)
+    # dominant kernel of the mixed HxV on the first timed pivot (HIP events on the plan's stream)
+    H = PivotMatrix2(first_pd, precision=args.mixed)
+    st = H.stats()
+    tiles = H.autotune(iters=3)
+    ms = H.time_stages(warmup=1, iters=5)
+    stage_flops = {"A": st["flops_A"], "C": st["flops_C"]}
+    dom = "A" if ms[0] >= ms[2] else "C"
+    t_dom = ms[0] if dom == "A" else ms[2]
+    achieved = stage_flops[dom] / (t_dom * 1e-3) / 1e12
+    peak = PEAK_TFLOPS[args.mixed]
+    big_tiles = tiles["big_min"] if args.mixed == "bf16x3" else tiles["big_min64"]
+    kname = {"A": "k_stage_a", "C": "k_stage_c"}[dom] + ("_bf16x3" if args.mixed == "bf16x3" else "") + \
+        ("_big (128x128 LDS tiles) + small-tile kernel" if big_tiles else "")
+    hxv_mixed_ms = ms[3]
+    H.close()
+    H64 = PivotMatrix2(first_pd, precision="fp64")
+    H64.autotune(iters=3)
+    hxv_fp64_ms = H64.time_hxv(warmup=1, iters=5)
+    vin = synth.random_vector(first_pd, args.seed + 7)
+    out_gpu = HxV(H64, vin)
+    H64.close()
+    traffic = None
+    try:
+        with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
+            tr = json.load(f).get("c60_M%d/%s" % (args.M, args.mixed), {})
+        traffic = tr.get("k_stage_a" if dom == "A" else "k_stage_c")
+    except (OSError, ValueError):
+        pass
 
+    res = {
+        "metric": "DMRG HxV GFLOP/s + sweep time, mixed vs plain FP64 GPU, at |dE|<=1e-9: two-site DMRG step "
+                  "throughput of the mixed path (F_alg of every HxV call per second of step wall time)",
+        "value": round(mx["gflops"], 3),
+        "unit": "GFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": mx["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": {"bf16x3": "bf16x3 (FP32 accumulation; FP64 vectors, dots and refinement)",
+                  "fp32": "f32 (FP64 vectors, dots and refinement)"}[args.mixed],
+        "data": "synthetic (C60 couplings; state from a random start, warm-up sweeps at small M, bonds expanded)",
+        "config": {
+            "workload": "c60_heisenberg_su2_M%d" % args.M,
+            "model": "HeisenbergSU2, full coupling matrix of the C60 truncated icosahedron (90 bonds, J=1), "
+                     "Cuthill-McKee chain order, S_total = 0",
+            "L": L, "M": args.M, "D_W_max": max(W.dw_out for W in sites),
+            "warm_up_M": args.warm, "first_pivot": p0, "timed_pivots": [p0 + args.warmup, p0 + nsteps - 1],
+            "lanczos": "tol_eigval 1e-10, tol_state 1e-6, full reorthogonalisation",
+            "first_timed_pivot": {"vsize": int(first_pd.vsize), "flops_alg": st["flops_alg"],
+                                  "flops_ref": st["flops_ref"], "max_block_dim": st["max_block_dim"],
+                                  "hxv_ms": {args.mixed: hxv_mixed_ms, "fp64": hxv_fp64_ms}},
+            "parallelism": "single GPU",
+        },
+        "dmrg": dmrg_rec,
+        "prep_seconds": {"warm_up": t_warm, "expansion": t_expand},
+        "tiles": {k: tiles[k] for k in ("wave_edge", "big_min", "big_min64", "explicit_u")},
+        "roofline": {
+            "bound": "mfma",
+            "kernel": kname,
+            "stage": dom,
+            "achieved": achieved,
+            "peak": peak,
+            "unit": "TFLOP/s",
+            "frac": achieved / peak,
+            "traffic": traffic,
+            "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
+            "stage_ms": {"A": ms[0], "B": ms[1], "C": ms[2], "hxv": ms[3]},
+            "peak_note": "dense BF16 MFMA 2.5 PFLOP/s / 3 products" if args.mixed == "bf16x3" else "FP32 MFMA",
+        },
+    }
+    if not args.no_cpu:
+        cb, out_cpu = cpu_baseline_pivot(first_pd, st["flops_alg"], vin, host_cores())
+        res["cpu_baseline"] = cb
+        res["parity_rel_err_vs_oracle"] = float(np.linalg.norm(out_gpu - out_cpu) / np.linalg.norm(out_cpu))
+    extras = [x for x in args.extras.split(",") if x]
+    if "c2" in extras:
+        res["hxv_c2"] = _c2_hxv_lines(args)
+    if "c1" in extras:
+        res["sweeps_c1"] = sweep_bench(args, sweeps=4)
+    return res
+
+
+def hxv_bench(args):
+    """One step = one HxV of the workload's random-environment pivot (or of
+    every pivot of a batched workload, one launch per stage) on
+    device-resident vectors; the metric is F_alg / t."""
     import torch
     from vmps_amd import _lib, synth
     from vmps_amd.pivot import PivotBatch, PivotMatrix2
@@ -324,10 +544,66 @@ def main():
                 for i, ref in sorted(outs.items())]
         res["parity_rel_err_vs_oracle"] = max(errs)
         res["parity_pivots_checked"] = len(errs)
-    if args.sweeps > 0:
-        res["sweeps"] = sweep_bench(args, sweeps=args.sweeps)
-    if args.large and args.precision == "fp64":
-        res["large_pivot"] = large_pivot_bench(args, args.large)
+    return res
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (C60: DMRG steps, default 20; "
+                                                            "HxV workloads: HxV calls, default 2000)")
+    ap.add_argument("--warmup", type=int, default=None, help="untimed steps first (C60 default 5, HxV 20)")
+    ap.add_argument("--workload", default="c60", help="c60 (DMRG steps) or an HxV workload of vmps_amd.synth")
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32", "bf16x3"],
+                    help="HxV workloads: precision of the main line")
+    ap.add_argument("--M", type=int, default=4096, help="C60: bond dimension (SU(2) multiplets)")
+    ap.add_argument("--warm", default="32,64,128,256", help="C60: bond dimensions of the warm-up sweeps")
+    ap.add_argument("--pivot", type=int, default=14, help="C60: site where the expanded state's steps start")
+    ap.add_argument("--mixed", default="bf16x3", choices=["bf16x3", "fp32"], help="C60: mixed precision")
+    ap.add_argument("--no-fp64-leg", action="store_true", help="C60: skip the FP64 comparison leg")
+    ap.add_argument("--hxv-reps", type=int, default=0,
+                    help="C60 profiling target: prepare the state, then only this many HxV of the first timed pivot "
+                         "in the mixed precision and in FP64 (no DMRG legs)")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default tile configuration")
+    ap.add_argument("--tiles", default=None,
+                    help="fixed tile configuration wave_edge,big_min,big_min64,explicit_u (instead of autotuning)")
+    ap.add_argument("--no-mixed", action="store_true", help="HxV workloads: skip the FP32 / split-BF16 x3 lines")
+    ap.add_argument("--seed", type=int, default=20261202)
+    ap.add_argument("--extras", default="c2,c1",
+                    help="C60: secondary lines in the default run: c2 (HubbardU1xU1 M=512 HxV), "
+                         "c1 (HeisenbergU1 L=32 M=64 sweeps); '' for none")
+    ap.add_argument("--sweeps", type=int, default=4, help="HxV workloads: DMRG sweeps of the C1 chain (0: none)")
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="only run 5 device copies of this many MiB (reference kernel for the HBM byte counters)")
+    args = ap.parse_args()
+    if args.calibrate_copy_mib:
+        import torch
+        n = args.calibrate_copy_mib * (1 << 20) // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibration_copy_bytes": n * 8, "copies": 5}))
+        return
+
+    if args.gpus != 1:
+        print("bench.py: the HxV runs on one GPU; --gpus %d is not supported "
+              "(a pivot would shard over its output blocks across GPUs; no multi-GPU path is implemented)"
+              % args.gpus, file=sys.stderr)
+        sys.exit(2)
+    if args.workload == "c60":
+        args.steps = 20 if args.steps is None else args.steps
+        args.warmup = 5 if args.warmup is None else args.warmup
+        res = dmrg_c60_bench(args)
+    else:
+        args.steps = 2000 if args.steps is None else args.steps
+        args.warmup = 20 if args.warmup is None else args.warmup
+        res = hxv_bench(args)
+        if args.sweeps > 0:
+            res["sweeps"] = sweep_bench(args, sweeps=args.sweeps)
     print(json.dumps(res))
 
 

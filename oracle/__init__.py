@@ -40,6 +40,9 @@ SIGNATURES = {
     "oracle_dot": (ctypes.c_int, [c_void, P_dbl, P_dbl, P_dbl]),
     "oracle_flops_ref": (ctypes.c_int, [c_void, P_dbl]),
     "oracle_dgemm_gflops": (ctypes.c_double, [ctypes.c_int, ctypes.c_int]),
+    "oracle_hxv_range": (ctypes.c_int, [c_void, P_dbl, P_dbl, ctypes.c_int64, ctypes.c_int64]),
+    "oracle_nlhs": (ctypes.c_int64, [c_void]),
+    "oracle_flops_ref_range": (ctypes.c_int, [c_void, ctypes.c_int64, ctypes.c_int64, P_dbl]),
     "oracle_wigner6j": (ctypes.c_double, [ctypes.c_int] * 6),
     "oracle_wigner9j": (ctypes.c_double, [ctypes.c_int] * 9),
     "oracle_coeff_HPsi": (ctypes.c_double, [ctypes.c_int, P_int, P_int]),

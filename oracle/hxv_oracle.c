@@ -1200,12 +1200,36 @@ int oracle_hxv_omp(const void* h, const double* vin, double* vout, int nthreads)
   return finish(P, vin, vout);
 }
 
+/* Single-threaded literal loop over the output entries q0 <= q < q1 only
+ * (a bounded sample of one HxV for the single-core CPU rate); vout must be
+ * zeroed by the caller and holds only those entries' blocks afterwards. */
+int oracle_hxv_range(const void* h, const double* vin, double* vout, int64_t q0, int64_t q1) {
+  const opivot_t* P = (const opivot_t*)h;
+  if (!P->has_bs) return fail(ERR_STATE, "no block structure");
+  if (P->nsite != 2) return fail(ERR_STATE, "range form implemented for two-site pivots");
+  if (q0 < 0 || q1 > P->nlhs || q0 > q1) return fail(ERR_ARG, "bad output range");
+  if (oxv_any(P, vin, vout, q0, q1)) return fail(ERR_LAYOUT, "layout mismatch in OxV");
+  return OK;
+}
+
+int64_t oracle_nlhs(const void* h) { return ((const opivot_t*)h)->nlhs; }
+
+static int flops_ref_range(const void* h, int64_t q0, int64_t q1, double* flops);
+
 /* work counts of the literal loop (for reporting the CPU baseline rate) */
 int oracle_flops_ref(const void* h, double* flops) {
+  return flops_ref_range(h, 0, ((const opivot_t*)h)->nlhs, flops);
+}
+
+int oracle_flops_ref_range(const void* h, int64_t q0, int64_t q1, double* flops) {
+  return flops_ref_range(h, q0, q1, flops);
+}
+
+static int flops_ref_range(const void* h, int64_t q0, int64_t q1, double* flops) {
   const opivot_t* P = (const opivot_t*)h;
   double f = 0.0;
   if (P->nsite != 2) return fail(ERR_STATE, "flop count implemented for two-site pivots");
-  for (int64_t q = 0; q < P->nlhs; ++q)
+  for (int64_t q = q0; q < q1; ++q)
     for (int64_t p = P->rhs_begin[q]; p < P->rhs_begin[q + 1]; ++p) {
       const int64_t* e = &P->qrhs[12 * p];
       const wblk_t* w12 = &W12AT(P, e[4], e[5], e[6])->b[e[7]];
