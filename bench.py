@@ -14,7 +14,7 @@ the device.  Synthetic data: no checkpoint is read.
 
 One step = one two-site DMRG step sweeping right through the middle of the
 chain (DmrgSolver::iteration_two): contract_AA2, the lowest eigenpair of
-H_eff to tol_eigval 1e-10 / residual 1e-6, the split with truncation to M,
+H_eff to tol_eigval 1e-11 / residual 1e-7, the split with truncation to M,
 and the environment update.  --warmup W steps run first (untimed), then
 --steps K steps are timed.  Both legs start from the same state and walk the
 same pivots:
@@ -211,6 +211,57 @@ def _c2_hxv_lines(args):
     return out
 
 
+def precision_report_c3(args, M=2048, steps=3):
+    """HubbardSU2xU1 two-leg ladder 2 x 32 (snake order, t = 1, U = 6, N = 56,
+    S = 0) at M = 2048 multiplets: the same two-site DMRG steps (from a state
+    warmed up at small M and expanded to M) in FP64, FP32 and split-BF16x3,
+    each mixed solve refined in FP64.  Per precision and step: the energy and
+    its difference from the FP64 leg, the residual ||H psi - E psi|| (FP64
+    operator), the mixed solve's own Rayleigh quotient before refinement and
+    its error, the truncation weight and its difference from FP64."""
+    from vmps_amd import dmrg, models
+    L, Q, p0 = 64, (1, 56), 30
+    sites = models.chain_mpo("HubbardSU2xU1Ladder", L, t=1.0, U=6.0)
+    tol = dict(tol_eigval=1e-11, tol_state=1e-7, max_iter=300)
+    prep = dmrg.GpuBackend(precision="fp64", **tol)
+    t0 = time.perf_counter()
+    warm = dmrg.warm_chain(sites, Q, [32, 64, 128, 256], p0, prep, np.random.default_rng(args.seed + 3))
+    big = dmrg.expand_chain(warm, M, p0, prep, np.random.default_rng(args.seed + 4))
+    out = {"model": "HubbardSU2xU1 ladder 2x32 snake, t=1, U=6, N=56, S=0", "M": M, "pivots": [p0, p0 + steps - 1],
+           "prep_seconds": time.perf_counter() - t0}
+    legs = {}
+    for prec in ("fp64", "fp32", "bf16x3"):
+        c = big.copy()
+        be = dmrg.GpuBackend(precision=prec, mixed_min_flops=0, **tol)
+        rec = []
+        for i in range(steps):
+            ts = time.perf_counter()
+            E, tw, info = dmrg.dmrg_step(c, be, p0 + i, "right")
+            rec.append({"energy": E, "truncation": tw, "residual": info["residual"], "seconds": time.perf_counter() - ts,
+                        "energy_unrefined": info["energy_mixed"], "residual_unrefined": info["residual_mixed"],
+                        "hxv_calls": info["hxv_calls"], "hxv_fp64": info["hxv_fp64"]})
+        legs[prec] = rec
+    for prec in ("fp32", "bf16x3"):
+        for a, b in zip(legs[prec], legs["fp64"]):
+            a["abs_dE_vs_fp64"] = abs(a["energy"] - b["energy"])
+            a["abs_dE_unrefined_vs_fp64"] = abs(a["energy_unrefined"] - b["energy"])
+            a["d_truncation_vs_fp64"] = a["truncation"] - b["truncation"]
+    out["legs"] = legs
+    return out
+
+
+def c2_sweeps(args, sweeps=3):
+    """HubbardU1xU1 L=48 (t=1, U=8, half filling, Sz=0), M=512: full two-site
+    DMRG sweeps in FP64 from a random MPS, energy per half sweep and time per
+    sweep."""
+    from vmps_amd import dmrg
+    be = dmrg.GpuBackend(precision="fp64", tol_eigval=1e-10, tol_state=1e-6)
+    r = dmrg.run_dmrg("HubbardU1xU1", 48, 512, sweeps, seed=args.seed, backend=be, params={"t": 1.0, "U": 8.0})
+    return {"model": "HubbardU1xU1 L=48 t=1 U=8 N=48 Sz=0", "M": 512, "energy": r["energy"],
+            "energies_per_half_sweep": r["energies"], "sweep_seconds": r["sweep_seconds"],
+            "max_truncation_last_sweep": max(r["truncation"][-47:]), "hxv_calls": be.hxv_calls}
+
+
 def dmrg_c60_bench(args):
     import torch
     from vmps_amd import dmrg, lattice, models, synth
@@ -228,7 +279,7 @@ def dmrg_c60_bench(args):
     nsteps = args.warmup + args.steps
     if p0 + nsteps > L - 1:
         raise SystemExit("bench.py: --pivot + --warmup + --steps must stay within the chain (<= %d)" % (L - 1))
-    tol = dict(tol_eigval=1e-10, tol_state=1e-6, max_iter=300)
+    tol = dict(tol_eigval=1e-11, tol_state=1e-7, max_iter=300)
     prep = dmrg.GpuBackend(precision="fp64", **tol)
     t0 = time.perf_counter()
     warm = dmrg.warm_chain(sites, (1,), [int(x) for x in args.warm.split(",") if x], p0, prep,
@@ -358,7 +409,8 @@ def dmrg_c60_bench(args):
                      "Cuthill-McKee chain order, S_total = 0",
             "L": L, "M": args.M, "D_W_max": max(W.dw_out for W in sites),
             "warm_up_M": args.warm, "first_pivot": p0, "timed_pivots": [p0 + args.warmup, p0 + nsteps - 1],
-            "lanczos": "tol_eigval 1e-10, tol_state 1e-6, full reorthogonalisation",
+            "lanczos": "tol_eigval 1e-11, residual 1e-7 (both legs; the mixed leg refined to them in FP64), "
+                       "full reorthogonalisation",
             "first_timed_pivot": {"vsize": int(first_pd.vsize), "flops_alg": st["flops_alg"],
                                   "flops_ref": st["flops_ref"], "max_block_dim": st["max_block_dim"],
                                   "hxv_ms": {args.mixed: hxv_mixed_ms, "fp64": hxv_fp64_ms}},
@@ -390,6 +442,10 @@ def dmrg_c60_bench(args):
         res["hxv_c2"] = _c2_hxv_lines(args)
     if "c1" in extras:
         res["sweeps_c1"] = sweep_bench(args, sweeps=4)
+    if "c3" in extras:
+        res["precision_c3"] = precision_report_c3(args)
+    if "c2sweep" in extras:
+        res["sweeps_c2"] = c2_sweeps(args)
     return res
 
 
@@ -571,9 +627,10 @@ def main():
                     help="fixed tile configuration wave_edge,big_min,big_min64,explicit_u (instead of autotuning)")
     ap.add_argument("--no-mixed", action="store_true", help="HxV workloads: skip the FP32 / split-BF16 x3 lines")
     ap.add_argument("--seed", type=int, default=20261202)
-    ap.add_argument("--extras", default="c2,c1",
+    ap.add_argument("--extras", default="c2,c1,c3,c2sweep",
                     help="C60: secondary lines in the default run: c2 (HubbardU1xU1 M=512 HxV), "
-                         "c1 (HeisenbergU1 L=32 M=64 sweeps); '' for none")
+                         "c1 (HeisenbergU1 L=32 M=64 sweeps), c3 (precision report, SU(2)xU(1) ladder M=2048), "
+                         "c2sweep (HubbardU1xU1 L=48 M=512 sweeps); '' for none")
     ap.add_argument("--sweeps", type=int, default=4, help="HxV workloads: DMRG sweeps of the C1 chain (0: none)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="only run 5 device copies of this many MiB (reference kernel for the HBM byte counters)")

@@ -147,27 +147,47 @@ def random_mps(sites, Q, M, rng, spread=2, max_spin2=6):
 
 
 def su2_gaussian_bonds(sym, qloc, L, Q, M, width=0.5):
-    """Bond sectors of an SU(2) chain with at most M multiplets per bond: every
-    multiplet q = 2S+1 reachable from both ends, with counts ~ exp(-S^2 /
-    (2 sigma^2)), sigma = width * sqrt(min(b, L - b)), each capped by the
-    exact number of multiplets on either side of the bond (so the bonds
-    near the ends keep the full space)."""
-    assert sym.kinds == (models.SU2_KIND,)
+    """Bond sectors of a chain with at most M states (multiplets for SU(2))
+    per bond: every sector reachable from both ends, with counts ~
+    exp(-sum_i (q_i - c_i)^2 / (2 sigma^2)), sigma = width * sqrt(min(b, L - b)),
+    around c_i = 0 in the spin S of SU(2) components and b Q_i / L in U(1)
+    charges, each capped by the exact count on either side of the bond (so
+    the bonds near the ends keep the full space)."""
     bonds = []
+    lefts, rights = [{sym.vacuum(): 1}], [{tuple(Q): 1}]
+    for _ in range(L):  # exact counts after 1..L sites from either end
+        new = {}
+        for q, c in lefts[-1].items():
+            for sq in qloc:
+                for k in sym.reduce(q, sq):
+                    new[k] = new.get(k, 0) + c
+        lefts.append(new)
+        new = {}
+        for q, c in rights[-1].items():
+            for sq in qloc:
+                for k in sym.preimages(q, sq):
+                    new[k] = new.get(k, 0) + c
+        rights.append(new)
     for b in range(L + 1):
-        cl = left_counts(sym, qloc, b)
-        cr = right_counts(sym, qloc, L - b, Q)
+        cl, cr = lefts[b], rights[L - b]
         caps = {q: min(c, cr[q]) for q, c in cl.items() if q in cr}
         if sum(caps.values()) <= M:
             bonds.append(dict(sorted(caps.items())))
             continue
         sigma = max(1.0, width * math.sqrt(min(b, L - b)))
         qs = sorted(caps)
-        w = np.array([math.exp(-((q[0] - 1) / 2.0) ** 2 / (2 * sigma ** 2)) for q in qs])
+
+        def dist2(q):
+            d = 0.0
+            for i, k in enumerate(sym.kinds):
+                x = (q[i] - 1) / 2.0 if k == models.SU2_KIND else q[i] - b * Q[i] / L
+                d += x * x
+            return d
+        w = np.array([math.exp(-dist2(q) / (2 * sigma ** 2)) for q in qs])
         cap = np.array([caps[q] for q in qs], dtype=np.float64)
-        lo, hi = 0.0, float(M) / w.min()
-        for _ in range(100):  # largest scale whose rounded, capped counts fit in M
-            mid = 0.5 * (lo + hi)
+        lo, hi = 1e-300, 1e300
+        for _ in range(200):  # largest scale whose rounded, capped counts fit in M (geometric bisection)
+            mid = math.sqrt(lo * hi)
             if np.minimum(cap, np.floor(mid * w + 0.5)).sum() <= M:
                 lo = mid
             else:
