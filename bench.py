@@ -14,10 +14,13 @@ the device.  Synthetic data: no checkpoint is read.
 
 One step = one two-site DMRG step sweeping right through the middle of the
 chain (DmrgSolver::iteration_two): contract_AA2, the lowest eigenpair of
-H_eff to tol_eigval 1e-11 / residual 1e-7, the split with truncation to M,
+H_eff to tol_eigval 1e-11 / residual 1e-7 (--tol-eigval, --tol-state),
+the split with truncation to M,
 and the environment update.  --warmup W steps run first (untimed), then
---steps K steps are timed.  Both legs start from the same state and walk the
-same pivots:
+--steps K steps are timed.  The sweep follows the mixed steps; before each,
+the same step runs in plain FP64 from the same state on a copy of the chain
+(timed on its own and dropped), so both legs solve the same effective
+Hamiltonians from the same start vectors:
   mixed: split-BF16x3 Lanczos (FP64 vectors, dots and reorthogonalisation)
          refined to the FP64 tolerances by FP64-residual corrections;
   fp64:  plain FP64 Lanczos.
@@ -194,6 +197,20 @@ def cpu_baseline_pivot(pd, falg, vin, threads, single_seconds=4.0):
     }, out
 
 
+def lanczos_iteration_ms(H, vsize, iters):
+    """Wall time of one Lanczos iteration (HxV, two Gram-Schmidt passes over the
+    whole basis, normalisation; replayed from a HIP graph, the scalars read by
+    the host every few iterations) averaged over `iters` iterations with the
+    tolerances at 0, from the same start vector as a first untimed solve that
+    captures the graph."""
+    from vmps_amd.pivot import lanczos
+    v0 = np.random.default_rng(1).uniform(-1, 1, vsize)
+    lanczos(H, v0=v0, max_iter=8, tol_eigval=0.0, tol_state=0.0)
+    t0 = time.perf_counter()
+    _, r = lanczos(H, v0=v0, max_iter=iters, tol_eigval=0.0, tol_state=0.0)
+    return (time.perf_counter() - t0) / r["iterations"] * 1e3
+
+
 def _c2_hxv_lines(args):
     """HubbardU1xU1 L=48 M=512 random-environment pivot: HxV time and rate in
     FP64, FP32 and split-BF16x3 (autotuned), a few seconds in all."""
@@ -207,6 +224,9 @@ def _c2_hxv_lines(args):
         H.autotune(iters=10)
         ms = H.time_hxv(warmup=20, iters=400)
         out[prec] = {"ms_per_hxv": ms, "gflops": st["flops_alg"] / (ms * 1e-3) / 1e9}
+        if prec == "fp64":
+            out["lanczos_ms_per_iteration"] = lanczos_iteration_ms(H, pd.vsize, 60)
+            out["lanczos_iterations"] = 60
         H.close()
     return out
 
@@ -279,7 +299,7 @@ def dmrg_c60_bench(args):
     nsteps = args.warmup + args.steps
     if p0 + nsteps > L - 1:
         raise SystemExit("bench.py: --pivot + --warmup + --steps must stay within the chain (<= %d)" % (L - 1))
-    tol = dict(tol_eigval=1e-11, tol_state=1e-7, max_iter=300)
+    tol = dict(tol_eigval=args.tol_eigval, tol_state=args.tol_state, max_iter=300)
     prep = dmrg.GpuBackend(precision="fp64", **tol)
     t0 = time.perf_counter()
     warm = dmrg.warm_chain(sites, (1,), [int(x) for x in args.warm.split(",") if x], p0, prep,
@@ -298,9 +318,16 @@ def dmrg_c60_bench(args):
         pd = c.pivot(p0 + args.warmup)
         v = synth.random_vector(pd, args.seed + 7)
         rec = {"workload": "c60_heisenberg_su2_M%d" % args.M, "pivot": p0 + args.warmup, "vsize": int(pd.vsize)}
-        for prec in (args.mixed, "fp64"):
+        precs = {"both": (args.mixed, "fp64"), "mixed": (args.mixed,), "fp64": ("fp64",)}[args.hxv_precision]
+        for prec in precs:
             H = PivotMatrix2(pd, precision=prec)
-            rec["tiles_" + prec] = H.autotune(iters=2)
+            if args.tiles:
+                # a pinned configuration keeps the tuning candidates out of a profile
+                we, bm, b64, xu = (int(x) for x in args.tiles.split(","))
+                H.set_tiles(wave_edge=we, big_min=bm, big_min64=b64, explicit_u=xu)
+                rec["tiles_" + prec] = H.tiles()
+            else:
+                rec["tiles_" + prec] = H.autotune(iters=2)
             torch.cuda.synchronize()
             t0 = time.perf_counter()
             for _ in range(args.hxv_reps):
@@ -311,47 +338,67 @@ def dmrg_c60_bench(args):
             H.close()
         return rec
 
-    legs = {}
-    first_pd = None
-    for leg, prec in [("mixed", args.mixed)] + ([] if args.no_fp64_leg else [("fp64", "fp64")]):
-        c = big.copy()
-        be = dmrg.GpuBackend(precision=prec, mixed_min_flops=0, **tol)
-        steps = []
-        t_timed0 = None
-        for i in range(nsteps):
-            l = p0 + i
-            if i == args.warmup:
-                if leg == "mixed":
-                    first_pd = c.pivot(l)
-                torch.cuda.synchronize()
-                t_timed0 = time.perf_counter()
-            tm = {}
-            ts = time.perf_counter()
-            E, tw, info = dmrg.dmrg_step(c, be, l, "right", timings=tm)
-            dt = time.perf_counter() - ts
-            steps.append({"pivot": l, "timed": i >= args.warmup, "energy": E, "truncation": tw, "seconds": dt,
-                          "phases": {k: round(v, 4) for k, v in tm.items()},
-                          "hxv_calls": info["hxv_calls"], "hxv_fp64": info["hxv_fp64"],
-                          "iterations": info["iterations"], "iterations_refine": info["iterations_refine"],
-                          "refine_steps": info["refine_steps"], "refine_cg": info["refine_cg"],
-                          "residual": info["residual"], "flops_alg": info["flops_alg"]})
-            log("%s step %d (pivot %d%s): E = %.12f  %.2f s  HxV %d (FP64 %d)  residual %.1e"
-                % (leg, i, l, "" if i >= args.warmup else ", warm-up", E, dt, info["hxv_calls"], info["hxv_fp64"],
-                   info["residual"]))
+    # One trajectory, the mixed one.  Before each mixed step an FP64 step runs
+    # from the same state on a copy of the chain (timed on its own, then
+    # dropped): the two solve the same effective Hamiltonian from the same
+    # start vector, so |E_mixed - E_fp64| measures the precision alone.  Two
+    # independent trajectories would not: at M = 4096 the kept set at each
+    # split is sensitive to perturbations far below the truncation weight, and
+    # two FP64 runs that differ only in the Lanczos tolerance drift apart by
+    # some 1e-8 over 20 steps.
+    def timed_step(chain, be, l):
+        tm = {}
         torch.cuda.synchronize()
-        t_timed = time.perf_counter() - t_timed0
-        timed = [st for st in steps if st["timed"]]
+        ts = time.perf_counter()
+        E, tw, info = dmrg.dmrg_step(chain, be, l, "right", timings=tm)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - ts
+        return {"pivot": l, "timed": False, "energy": E, "truncation": tw, "seconds": dt,
+                "phases": {k: round(v, 4) for k, v in tm.items()},
+                "hxv_calls": info["hxv_calls"], "hxv_fp64": info["hxv_fp64"],
+                "iterations": info["iterations"], "iterations_refine": info["iterations_refine"],
+                "refine_steps": info["refine_steps"], "refine_cg": info["refine_cg"],
+                "residual": info["residual"], "flops_alg": info["flops_alg"]}
+
+    be = {"mixed": dmrg.GpuBackend(precision=args.mixed, mixed_min_flops=0, **tol)}
+    if not args.no_fp64_leg:
+        be["fp64"] = dmrg.GpuBackend(precision="fp64", **tol)
+    steps = {leg: [] for leg in be}
+    c = big.copy()
+    first_pd = None
+    for i in range(nsteps):
+        l = p0 + i
+        if i == args.warmup:
+            first_pd = c.pivot(l)
+        if "fp64" in be:
+            ctrl = c.copy()
+            steps["fp64"].append(timed_step(ctrl, be["fp64"], l))
+            del ctrl
+        steps["mixed"].append(timed_step(c, be["mixed"], l))
+        for leg in be:
+            st = steps[leg][-1]
+            st["timed"] = i >= args.warmup
+            log("%s step %d (pivot %d%s): E = %.12f  %.2f s  HxV %d (FP64 %d)  residual %.1e"
+                % (leg, i, l, "" if i >= args.warmup else ", warm-up", st["energy"], st["seconds"],
+                   st["hxv_calls"], st["hxv_fp64"], st["residual"]))
+    legs = {}
+    for leg in be:
+        timed = [st for st in steps[leg] if st["timed"]]
+        t_timed = sum(st["seconds"] for st in timed)
         work = sum(st["flops_alg"] * st["hxv_calls"] for st in timed)
         phases = {}
         for st in timed:
             for k, v in st["phases"].items():
                 phases[k] = phases.get(k, 0.0) + v
-        legs[leg] = {"precision": prec, "seconds_timed": t_timed, "ms_per_step": t_timed / args.steps * 1e3,
+        legs[leg] = {"precision": be[leg].precision, "seconds_timed": t_timed,
+                     "ms_per_step": t_timed / args.steps * 1e3,
                      "gflops": work / t_timed / 1e9, "hxv_calls": sum(st["hxv_calls"] for st in timed),
                      "hxv_fp64": sum(st["hxv_fp64"] for st in timed),
-                     "phase_seconds": {k: round(v, 3) for k, v in phases.items()}, "steps": steps}
+                     "phase_seconds": {k: round(v, 3) for k, v in phases.items()}, "steps": steps[leg]}
     mx = legs["mixed"]
-    dmrg_rec = {"legs": legs}
+    dmrg_rec = {"trajectory": "the mixed leg's; every FP64 step starts from the same state (a copy of the chain) "
+                              "and is dropped after timing",
+                "legs": legs}
     if "fp64" in legs:
         f64 = legs["fp64"]
         dE = [abs(a["energy"] - b["energy"]) for a, b in zip(mx["steps"], f64["steps"]) if a["timed"]]
@@ -409,8 +456,8 @@ def dmrg_c60_bench(args):
                      "Cuthill-McKee chain order, S_total = 0",
             "L": L, "M": args.M, "D_W_max": max(W.dw_out for W in sites),
             "warm_up_M": args.warm, "first_pivot": p0, "timed_pivots": [p0 + args.warmup, p0 + nsteps - 1],
-            "lanczos": "tol_eigval 1e-11, residual 1e-7 (both legs; the mixed leg refined to them in FP64), "
-                       "full reorthogonalisation",
+            "lanczos": "tol_eigval %g, residual %g (both legs; the mixed leg refined to them in FP64), "
+                       "full reorthogonalisation" % (args.tol_eigval, args.tol_state),
             "first_timed_pivot": {"vsize": int(first_pd.vsize), "flops_alg": st["flops_alg"],
                                   "flops_ref": st["flops_ref"], "max_block_dim": st["max_block_dim"],
                                   "hxv_ms": {args.mixed: hxv_mixed_ms, "fp64": hxv_fp64_ms}},
@@ -493,6 +540,9 @@ def hxv_bench(args):
     sec = timed(H._plan, vout)
     gflops = st["flops_alg"] / sec / 1e9
     out_gpu = vout.cpu().numpy()
+    lanczos_ms = None
+    if args.lanczos_iters and not batched:
+        lanczos_ms = lanczos_iteration_ms(H, pds[0].vsize, args.lanczos_iters)
 
     # the same workload in the mixed modes (split-BF16 x3 and FP32 MFMA with
     # FP32 accumulation), each autotuned, against the FP64 result
@@ -561,6 +611,7 @@ def hxv_bench(args):
         "tiles": {k: tiles[k] for k in ("wave_edge", "big_min", "big_min64", "explicit_u")} | {
                   "autotuned": not args.no_tune},
         "mixed_hxv": mixed,
+        "lanczos_ms_per_iteration": lanczos_ms,
         "roofline": {
             "bound": "mfma",
             "kernel": kname,
@@ -617,9 +668,14 @@ def main():
     ap.add_argument("--pivot", type=int, default=14, help="C60: site where the expanded state's steps start")
     ap.add_argument("--mixed", default="bf16x3", choices=["bf16x3", "fp32"], help="C60: mixed precision")
     ap.add_argument("--no-fp64-leg", action="store_true", help="C60: skip the FP64 comparison leg")
+    ap.add_argument("--tol-eigval", type=float, default=1e-11, help="C60: Lanczos eigenvalue tolerance (both legs)")
+    ap.add_argument("--tol-state", type=float, default=1e-7,
+                    help="C60: residual tolerance ||Hv - Ev|| (both legs; the mixed leg refined to it in FP64)")
     ap.add_argument("--hxv-reps", type=int, default=0,
                     help="C60 profiling target: prepare the state, then only this many HxV of the first timed pivot "
                          "in the mixed precision and in FP64 (no DMRG legs)")
+    ap.add_argument("--hxv-precision", default="both", choices=["both", "mixed", "fp64"],
+                    help="C60 profiling target: which operator(s) --hxv-reps runs")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the default tile configuration")
@@ -632,6 +688,8 @@ def main():
                          "c1 (HeisenbergU1 L=32 M=64 sweeps), c3 (precision report, SU(2)xU(1) ladder M=2048), "
                          "c2sweep (HubbardU1xU1 L=48 M=512 sweeps); '' for none")
     ap.add_argument("--sweeps", type=int, default=4, help="HxV workloads: DMRG sweeps of the C1 chain (0: none)")
+    ap.add_argument("--lanczos-iters", type=int, default=0,
+                    help="HxV workloads: also time this many Lanczos iterations on the pivot (0: none)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="only run 5 device copies of this many MiB (reference kernel for the HBM byte counters)")
     args = ap.parse_args()

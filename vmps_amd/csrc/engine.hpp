@@ -70,7 +70,8 @@ struct Krylov {
   DevBuf<double> ew;      // coeff_dot weight of every vector element
   DevBuf<double> gpart;   // partial sums of the basis dot products, (kmax + 1) x ngrp
   DevBuf<double> npart;   // partial sums of ||w||^2
-  double* host3 = nullptr;  // pinned readback of out3 (4 * kIterBatch doubles)
+  double* host3 = nullptr;  // pinned readback of out3: two slots of 4 * kIterBatch doubles
+  hipEvent_t done[2] = {nullptr, nullptr};  // a slot's readback has landed
   int ngrp = 0;           // element chunks of the basis dot products
   int64_t chunk = 0;
   int cap = 0;
@@ -79,10 +80,12 @@ struct Krylov {
   // refinement each keep theirs), valid for the plan generation it was made at
   hipGraph_t graph[3] = {nullptr, nullptr, nullptr};
   hipGraphExec_t exec[3] = {nullptr, nullptr, nullptr};
+  hipGraphExec_t exec2[3] = {nullptr, nullptr, nullptr};  // a second instance of each graph
   int graph_gen[3] = {-1, -1, -1};
   void ensure(DevPlan& D, int kmax);
   void enqueue_iteration(DevPlan& D, hipStream_t s, int kofs);  // iteration *kdev + kofs; *kdev is not advanced
-  hipGraphExec_t ensure_graph(DevPlan& D);  // the graph of D.prec (kIterBatch iterations)
+  // an executable instance (copy 0 or 1) of the graph of D.prec (kIterBatch iterations)
+  hipGraphExec_t ensure_graph(DevPlan& D, int copy);
   void release_graph();
   Krylov() = default;
   Krylov(const Krylov&) = delete;

@@ -366,20 +366,28 @@ void Krylov::enqueue_iteration(DevPlan& D, hipStream_t s, int kofs) {
 void Krylov::release_graph() {
   for (int i = 0; i < 3; ++i) {
     if (exec[i]) (void)hipGraphExecDestroy(exec[i]);
+    if (exec2[i]) (void)hipGraphExecDestroy(exec2[i]);
     if (graph[i]) (void)hipGraphDestroy(graph[i]);
     exec[i] = nullptr;
+    exec2[i] = nullptr;
     graph[i] = nullptr;
     graph_gen[i] = -1;
   }
 }
 
-hipGraphExec_t Krylov::ensure_graph(DevPlan& D) {
+hipGraphExec_t Krylov::ensure_graph(DevPlan& D, int copy) {
   const int p = (int)D.prec;
   if (p < 0 || p > 2) throw VmpsError(VMPS_ERR_ARG, "unsupported precision");
-  if (exec[p] && graph_gen[p] == D.gen) return exec[p];
+  if (exec[p] && graph_gen[p] == D.gen) {
+    if (copy == 0) return exec[p];
+    if (!exec2[p]) VMPS_HIP_CHECK(hipGraphInstantiate(&exec2[p], graph[p], nullptr, nullptr, 0));
+    return exec2[p];
+  }
   if (exec[p]) (void)hipGraphExecDestroy(exec[p]);
+  if (exec2[p]) (void)hipGraphExecDestroy(exec2[p]);
   if (graph[p]) (void)hipGraphDestroy(graph[p]);
   exec[p] = nullptr;
+  exec2[p] = nullptr;
   graph[p] = nullptr;
   hipStream_t s = D.stream;
   VMPS_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
@@ -395,7 +403,9 @@ hipGraphExec_t Krylov::ensure_graph(DevPlan& D) {
   VMPS_HIP_CHECK(hipStreamEndCapture(s, &graph[p]));
   VMPS_HIP_CHECK(hipGraphInstantiate(&exec[p], graph[p], nullptr, nullptr, 0));
   graph_gen[p] = D.gen;
-  return exec[p];
+  if (copy == 0) return exec[p];
+  VMPS_HIP_CHECK(hipGraphInstantiate(&exec2[p], graph[p], nullptr, nullptr, 0));
+  return exec2[p];
 }
 
 LanczosResult lanczos_solve(DevPlan& D, double* d_v, const LanczosOpts& o, vmps_precision prec) {
@@ -409,7 +419,7 @@ LanczosResult lanczos_solve(DevPlan& D, double* d_v, const LanczosOpts& o, vmps_
   D.prec = prec;
   D.ensure_precision_buffers();
   K.ensure(D, kmax);
-  hipGraphExec_t iteration = o.graph ? K.ensure_graph(D) : nullptr;
+  hipGraphExec_t iteration = o.graph ? K.ensure_graph(D, 0) : nullptr;
 
   // q0 = v / ||v|| into the basis and into the HxV input; k = 0.  A zero
   // (or non-finite) start vector has no direction to start from.
@@ -434,22 +444,54 @@ LanczosResult lanczos_solve(DevPlan& D, double* d_v, const LanczosOpts& o, vmps_
   // scalars with one synchronisation; it then tests them one by one, exactly
   // as after single iterations.  Iterations a batch runs past convergence (or
   // past kmax) only extend the basis beyond the part the Ritz vector uses.
+  // On cheap operators, while the last batch read left the residual estimate
+  // or the eigenvalue change at least 100x above its tolerance (so that
+  // convergence within the next batch is unlikely), one more batch is
+  // queued before the host reads the oldest, and the device does not wait for
+  // the readback and the tridiagonal solve.  A batch queued past convergence
+  // is the price (kIterBatch HxV, at most ~0.2 ms under kAheadMaxFlops); short
+  // warm-started solves, which converge within a batch or two, never pay it.
+  // Consecutive batches alternate between two executable instances of the
+  // captured graph, so an instance is launched again only after its previous
+  // launch has completed (the host waited for it).
+  constexpr int B = Krylov::kIterBatch;
+  constexpr double kAheadMaxFlops = 1e9;
+  const bool ahead = iteration && D.H.stats.flops_alg <= kAheadMaxFlops;
+  hipGraphExec_t instance[2] = {iteration, ahead ? K.ensure_graph(D, 1) : iteration};
+  bool far = false;  // the last batch read left both tests far from their tolerances
+  int kq = 0;        // first iteration of the next batch to queue
+  int inflight = 0;  // batches queued whose scalars the host has not read
+  int head = 0;      // readback slot of the oldest of them
+  auto queue_batch = [&]() {
+    const int slot = (head + inflight) & 1;
+    VMPS_HIP_CHECK(hipGraphLaunch(instance[(kq / B) & 1], s));
+    VMPS_HIP_CHECK(hipMemcpyAsync(K.host3 + 4 * B * slot, K.out3.p + 4 * (size_t)kq, 4 * B * sizeof(double),
+                                  hipMemcpyDeviceToHost, s));
+    VMPS_HIP_CHECK(hipEventRecord(K.done[slot], s));
+    kq += B;
+    ++inflight;
+  };
   while (k < kmax && !converged) {
     int nb = 1;
+    const double* hs = K.host3;
     if (iteration) {
-      VMPS_HIP_CHECK(hipGraphLaunch(iteration, s));
-      nb = Krylov::kIterBatch;
+      if (inflight == 0) queue_batch();
+      if (ahead && far && inflight == 1 && kq < kmax) queue_batch();
+      VMPS_HIP_CHECK(hipEventSynchronize(K.done[head]));
+      hs = K.host3 + 4 * B * head;
+      head ^= 1;
+      --inflight;
+      nb = B;
     } else {
       K.enqueue_iteration(D, s, 0);
       hipLaunchKernelGGL(k_add_int, dim3(1), dim3(64), 0, s, K.kdev.p, 1);
+      VMPS_HIP_CHECK(hipMemcpyAsync(K.host3, K.out3.p + 4 * (size_t)k, 4 * sizeof(double), hipMemcpyDeviceToHost, s));
+      VMPS_HIP_CHECK(hipStreamSynchronize(s));
     }
     res.hxv_calls += nb;
-    VMPS_HIP_CHECK(hipMemcpyAsync(K.host3, K.out3.p + 4 * (size_t)k, 4 * nb * sizeof(double), hipMemcpyDeviceToHost,
-                                  s));
-    VMPS_HIP_CHECK(hipStreamSynchronize(s));
     for (int j = 0; j < nb && k < kmax; ++j) {
-      const double a_k = K.host3[4 * j] + K.host3[4 * j + 1];
-      const double b_k = std::sqrt(std::max(0.0, K.host3[4 * j + 2]));
+      const double a_k = hs[4 * j] + hs[4 * j + 1];
+      const double b_k = std::sqrt(std::max(0.0, hs[4 * j + 2]));
       alpha.push_back(a_k);
       beta.push_back(b_k);
       Eprev = E;
@@ -466,8 +508,12 @@ LanczosResult lanczos_solve(DevPlan& D, double* d_v, const LanczosOpts& o, vmps_
         converged = true;
         break;
       }
+      far = k > 1 && (std::fabs(E - Eprev) > 100.0 * o.tol_eigval || rest > 100.0 * o.tol_state);
     }
   }
+  // a batch queued past convergence runs before the work below (stream order)
+  // and only extends the basis; its HxV applications are counted as done
+  res.hxv_calls += B * inflight;
   k = res.iterations;
   // Ritz vector psi = Q y (k basis vectors), normalised, then the true residual
   VMPS_HIP_CHECK(hipMemcpyAsync(K.c.p, y.data(), sizeof(double) * k, hipMemcpyHostToDevice, s));
@@ -625,7 +671,9 @@ void Krylov::ensure(DevPlan& D, int kmax) {
   scal.alloc(8);
   out3.alloc(4 * (size_t)need);
   kdev.alloc(1);
-  if (!host3) VMPS_HIP_CHECK(hipHostMalloc((void**)&host3, 4 * kIterBatch * sizeof(double), hipHostMallocDefault));
+  if (!host3) VMPS_HIP_CHECK(hipHostMalloc((void**)&host3, 8 * kIterBatch * sizeof(double), hipHostMallocDefault));
+  for (auto& e : done)
+    if (!e) VMPS_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   // per-element coeff_dot weights and element chunks of the row dot products
   std::vector<double> hw((size_t)n, 1.0);
   for (size_t sg = 0; sg < D.H.seg_off.size(); ++sg)
@@ -649,6 +697,10 @@ Krylov::~Krylov() {
   release_graph();
   if (host3) (void)hipHostFree(host3);
   host3 = nullptr;
+  for (auto& e : done) {
+    if (e) (void)hipEventDestroy(e);
+    e = nullptr;
+  }
 }
 
 }  // namespace vmps

@@ -42,23 +42,27 @@ void rb_check(rocblas_status st, const char* what) {
     throw VmpsError(VMPS_ERR_HIP, std::string(what) + ": " + rocblas_status_to_string(st));
 }
 
-// one rocBLAS handle per device, made on first use (rocBLAS handles keep their
-// workspace; the stream is set per call, calls on one handle are serialised)
+// rocBLAS handles per device, made on first use and kept until the process
+// ends, like the device's stream pool: kLanes of them, so that the sectors of
+// one split factor concurrently on as many streams (each handle owns its
+// workspace, so concurrent work needs one handle per stream).  Calls on one
+// device are serialised by the slot's mutex.
+constexpr int kLanes = 4;
 struct HandleSlot {
-  rocblas_handle h = nullptr;
+  rocblas_handle h[kLanes] = {};
   std::mutex m;
 };
 std::mutex g_handles_mutex;
 std::map<int, HandleSlot*> g_handles;
 
-HandleSlot& handle_for_current_device() {
+HandleSlot& handles_for_current_device() {
   int dev = 0;
   VMPS_HIP_CHECK(hipGetDevice(&dev));
   std::lock_guard<std::mutex> lock(g_handles_mutex);
   HandleSlot*& s = g_handles[dev];
   if (!s) {
-    auto* slot = new HandleSlot();  // kept until the process ends, like the device's stream pool
-    rb_check(rocblas_create_handle(&slot->h), "rocblas_create_handle");
+    auto* slot = new HandleSlot();
+    for (auto& h : slot->h) rb_check(rocblas_create_handle(&h), "rocblas_create_handle");
     s = slot;
   }
   return *s;
@@ -75,14 +79,13 @@ extern "C" vmps_status vmps_block_eigh_split(int nblocks, const int64_t* rows, c
     if (nblocks < 0 || (nblocks > 0 && (!rows || !cols || !A || !lambda || !W || !C)))
       throw VmpsError(VMPS_ERR_ARG, "bad argument");
     if (side != 0 && side != 1) throw VmpsError(VMPS_ERR_ARG, "side must be 0 (left) or 1 (right)");
-    int64_t maxrc = 0, maxk = 0;
+    int64_t maxrc = 0;
     for (int b = 0; b < nblocks; ++b) {
       if (rows[b] < 0 || cols[b] < 0 || rows[b] > (1 << 20) || cols[b] > (1 << 20))
         throw VmpsError(VMPS_ERR_ARG, "bad block dimension");
       const int64_t k = side == 0 ? rows[b] : cols[b];
       if (k > (1 << 16)) throw VmpsError(VMPS_ERR_ARG, "block too large for the eigensolver (side > 65536)");
       maxrc = std::max(maxrc, rows[b] * cols[b]);
-      maxk = std::max(maxk, k);
     }
     if (nblocks == 0 || maxrc == 0) {
       // nothing to factor: empty blocks have empty outputs, but a k x k
@@ -97,80 +100,122 @@ extern "C" vmps_status vmps_block_eigh_split(int nblocks, const int64_t* rows, c
       }
       return VMPS_OK;
     }
-    vmps::HandleSlot& slot = vmps::handle_for_current_device();
+    // Sectors go to kLanes streams, largest first, each to the lane with the
+    // least work so far (work ~ k^3); every lane runs its sectors in order
+    // through its own input buffer, while each sector keeps its own outputs
+    // on the device until all lanes are done.
+    vmps::HandleSlot& slot = vmps::handles_for_current_device();
     std::lock_guard<std::mutex> hold(slot.m);
-    struct Stream {  // synchronised and returned to the pool on every exit path
-      hipStream_t s = nullptr;
-      ~Stream() {
-        if (s) {
-          (void)hipStreamSynchronize(s);
-          vmps::release_stream(s);
-        }
+    constexpr int NL = vmps::kLanes;
+    struct Lanes {  // synchronised and returned to the pool on every exit path
+      hipStream_t s[NL] = {};
+      ~Lanes() {
+        for (auto& x : s)
+          if (x) {
+            (void)hipStreamSynchronize(x);
+            vmps::release_stream(x);
+          }
       }
-    } st;
-    st.s = vmps::acquire_stream();
-    hipStream_t s = st.s;
-    vmps::rb_check(rocblas_set_stream(slot.h, s), "rocblas_set_stream");
-    vmps::DevBuf<double> dA, dG, dC, dL, dE;
-    vmps::DevBuf<int> dInfo;
-    dA.alloc((size_t)maxrc);
-    dC.alloc((size_t)maxrc);
-    dG.alloc((size_t)std::max<int64_t>(1, maxk * maxk));
-    dL.alloc((size_t)std::max<int64_t>(1, maxk));
-    dE.alloc((size_t)std::max<int64_t>(1, maxk));
-    dInfo.alloc(1);
-    const double one = 1.0, zero = 0.0;
-    int64_t ao = 0, lo = 0, wo = 0;
+    } lanes;
+    for (auto& x : lanes.s) x = vmps::acquire_stream();
+    std::vector<int64_t> ao(nblocks), lo(nblocks), wo(nblocks);
+    std::vector<int> order;
+    int64_t a_acc = 0, l_acc = 0, w_acc = 0;
     for (int b = 0; b < nblocks; ++b) {
-      const int64_t r = rows[b], c = cols[b], k = side == 0 ? r : c;
-      if (k == 0) {
-        ao += r * c;
+      const int64_t k = side == 0 ? rows[b] : cols[b];
+      ao[b] = a_acc;
+      lo[b] = l_acc;
+      wo[b] = w_acc;
+      a_acc += rows[b] * cols[b];
+      l_acc += k;
+      w_acc += k * k;
+      if (k == 0) continue;
+      if (rows[b] * cols[b] == 0) {  // a k x k zero density matrix
+        for (int64_t i = 0; i < k; ++i) lambda[lo[b] + i] = 0.0;
+        for (int64_t i = 0; i < k * k; ++i) W[wo[b] + i] = (i % (k + 1) == 0) ? 1.0 : 0.0;
         continue;
       }
-      if (r * c == 0) {  // a k x k zero density matrix
-        for (int64_t i = 0; i < k; ++i) lambda[lo + i] = 0.0;
-        for (int64_t i = 0; i < k * k; ++i) W[wo + i] = (i % (k + 1) == 0) ? 1.0 : 0.0;
-        lo += k;
-        wo += k * k;
-        continue;
-      }
-      VMPS_HIP_CHECK(hipMemcpyAsync(dA.p, A + ao, sizeof(double) * r * c, hipMemcpyHostToDevice, s));
+      order.push_back(b);
+    }
+    std::sort(order.begin(), order.end(), [&](int x, int y) {
+      const int64_t kx = side == 0 ? rows[x] : cols[x], ky = side == 0 ? rows[y] : cols[y];
+      return kx != ky ? kx > ky : x < y;
+    });
+    std::vector<int> lane(nblocks, -1);
+    double work[NL] = {};
+    int64_t lane_rc[NL] = {}, lane_k[NL] = {};
+    int64_t out_g = 0, out_c = 0;  // device outputs of all sectors
+    std::vector<int64_t> g_off(nblocks, 0), c_off(nblocks, 0);
+    for (int b : order) {
+      const int64_t k = side == 0 ? rows[b] : cols[b];
+      int best = 0;
+      for (int l = 1; l < NL; ++l)
+        if (work[l] < work[best]) best = l;
+      lane[b] = best;
+      work[best] += (double)k * k * k;
+      lane_rc[best] = std::max(lane_rc[best], rows[b] * cols[b]);
+      lane_k[best] = std::max(lane_k[best], k);
+      g_off[b] = out_g;
+      c_off[b] = out_c;
+      out_g += k * k;
+      out_c += rows[b] * cols[b];
+    }
+    vmps::DevBuf<double> dA[NL], dE[NL], dG, dC, dL;
+    vmps::DevBuf<int> dInfo;
+    for (int l = 0; l < NL; ++l) {
+      if (lane_rc[l] == 0) continue;
+      dA[l].alloc((size_t)lane_rc[l]);
+      dE[l].alloc((size_t)std::max<int64_t>(1, lane_k[l]));
+    }
+    dG.alloc((size_t)std::max<int64_t>(1, out_g));
+    dC.alloc((size_t)std::max<int64_t>(1, out_c));
+    dL.alloc((size_t)std::max<int64_t>(1, l_acc));
+    dInfo.alloc((size_t)std::max(1, nblocks));
+    const double one = 1.0, zero = 0.0;
+    for (int l = 0; l < NL; ++l) vmps::rb_check(rocblas_set_stream(slot.h[l], lanes.s[l]), "rocblas_set_stream");
+    for (int b : order) {
+      const int l = lane[b];
+      rocblas_handle h = slot.h[l];
+      hipStream_t st = lanes.s[l];
+      const rocblas_int r = (rocblas_int)rows[b], c = (rocblas_int)cols[b], k = side == 0 ? r : c;
+      double* G = dG.p + g_off[b];
+      double* Cb = dC.p + c_off[b];
+      VMPS_HIP_CHECK(hipMemcpyAsync(dA[l].p, A + ao[b], sizeof(double) * r * c, hipMemcpyHostToDevice, st));
       // rho = A A^T (r x r) or A^T A (c x c), the full matrix
       if (side == 0)
-        vmps::rb_check(rocblas_dgemm(slot.h, rocblas_operation_none, rocblas_operation_transpose, (rocblas_int)r,
-                                     (rocblas_int)r, (rocblas_int)c, &one, dA.p, (rocblas_int)r, dA.p,
-                                     (rocblas_int)r, &zero, dG.p, (rocblas_int)r),
+        vmps::rb_check(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_transpose, r, r, c, &one, dA[l].p, r,
+                                     dA[l].p, r, &zero, G, r),
                        "rocblas_dgemm");
       else
-        vmps::rb_check(rocblas_dgemm(slot.h, rocblas_operation_transpose, rocblas_operation_none, (rocblas_int)c,
-                                     (rocblas_int)c, (rocblas_int)r, &one, dA.p, (rocblas_int)r, dA.p,
-                                     (rocblas_int)r, &zero, dG.p, (rocblas_int)c),
+        vmps::rb_check(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, c, c, r, &one, dA[l].p, r,
+                                     dA[l].p, r, &zero, G, c),
                        "rocblas_dgemm");
-      vmps::rb_check(rocsolver_dsyevd(slot.h, rocblas_evect_original, rocblas_fill_upper, (rocblas_int)k, dG.p,
-                                      (rocblas_int)k, dL.p, dE.p, dInfo.p),
+      vmps::rb_check(rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_upper, k, G, k, dL.p + lo[b], dE[l].p,
+                                      dInfo.p + b),
                      "rocsolver_dsyevd");
       // centre: W^T A (k x c) or A W (r x k)
       if (side == 0)
-        vmps::rb_check(rocblas_dgemm(slot.h, rocblas_operation_transpose, rocblas_operation_none, (rocblas_int)r,
-                                     (rocblas_int)c, (rocblas_int)r, &one, dG.p, (rocblas_int)r, dA.p,
-                                     (rocblas_int)r, &zero, dC.p, (rocblas_int)r),
+        vmps::rb_check(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, r, c, r, &one, G, r,
+                                     dA[l].p, r, &zero, Cb, r),
                        "rocblas_dgemm");
       else
-        vmps::rb_check(rocblas_dgemm(slot.h, rocblas_operation_none, rocblas_operation_none, (rocblas_int)r,
-                                     (rocblas_int)c, (rocblas_int)c, &one, dA.p, (rocblas_int)r, dG.p,
-                                     (rocblas_int)c, &zero, dC.p, (rocblas_int)r),
+        vmps::rb_check(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, r, c, c, &one, dA[l].p, r, G,
+                                     c, &zero, Cb, r),
                        "rocblas_dgemm");
-      int info = 0;
-      VMPS_HIP_CHECK(hipMemcpyAsync(lambda + lo, dL.p, sizeof(double) * k, hipMemcpyDeviceToHost, s));
-      VMPS_HIP_CHECK(hipMemcpyAsync(W + wo, dG.p, sizeof(double) * k * k, hipMemcpyDeviceToHost, s));
-      VMPS_HIP_CHECK(hipMemcpyAsync(C + ao, dC.p, sizeof(double) * r * c, hipMemcpyDeviceToHost, s));
-      VMPS_HIP_CHECK(hipMemcpyAsync(&info, dInfo.p, sizeof(int), hipMemcpyDeviceToHost, s));
-      VMPS_HIP_CHECK(hipStreamSynchronize(s));
-      if (info != 0) throw VmpsError(VMPS_ERR_NOCONV, "rocsolver_dsyevd did not converge (info " +
-                                                          std::to_string(info) + ")");
-      ao += r * c;
-      lo += k;
-      wo += k * k;
+    }
+    for (auto x : lanes.s) VMPS_HIP_CHECK(hipStreamSynchronize(x));
+    std::vector<int> info(std::max(1, nblocks), 0);
+    if (!order.empty()) {
+      VMPS_HIP_CHECK(hipMemcpy(info.data(), dInfo.p, sizeof(int) * nblocks, hipMemcpyDeviceToHost));
+    }
+    for (int b : order) {
+      if (info[b] != 0)
+        throw VmpsError(VMPS_ERR_NOCONV, "rocsolver_dsyevd did not converge (info " + std::to_string(info[b]) + ")");
+      const int64_t k = side == 0 ? rows[b] : cols[b];
+      VMPS_HIP_CHECK(hipMemcpy(lambda + lo[b], dL.p + lo[b], sizeof(double) * k, hipMemcpyDeviceToHost));
+      VMPS_HIP_CHECK(hipMemcpy(W + wo[b], dG.p + g_off[b], sizeof(double) * k * k, hipMemcpyDeviceToHost));
+      VMPS_HIP_CHECK(hipMemcpy(C + ao[b], dC.p + c_off[b], sizeof(double) * rows[b] * cols[b],
+                               hipMemcpyDeviceToHost));
     }
     return VMPS_OK;
   } catch (const VmpsError& e) {

@@ -574,6 +574,7 @@ class GpuBackend:
 
     def solve(self, pd, v0):
         from .pivot import PivotMatrix2, lanczos
+        t0 = time.perf_counter()
         H = PivotMatrix2(pd, precision="fp64")
         flops = H.stats()["flops_alg"]
         tune = self.tune_min_flops is not None and flops >= self.tune_min_flops
@@ -587,12 +588,14 @@ class GpuBackend:
             if tune:
                 H.autotune(iters=2)
             self.mixed_solves += 1
+        t_plan = time.perf_counter() - t0
         v, r = lanczos(H, v0=v0, precision=prec, refine_fp64=self.refine, max_iter=self.max_iter,
                        tol_eigval=self.tol_eigval, tol_state=self.tol_state,
                        refine_tol_eigval=self.tol_eigval, refine_tol_state=self.tol_state,
                        refine_method=self.refine_method)
         r["flops_alg"] = H.stats()["flops_alg"]
         r["precision"] = prec
+        r["plan_seconds"] = t_plan  # plan set-up and tile choice, part of the solve
         H.close()
         self.hxv_calls += r["hxv_calls"]
         return r["energy"], v, r
@@ -836,6 +839,8 @@ def dmrg_step(chain, backend, l, direction, rng=None, timings=None):
     if timings is not None:
         for key, dt in (("psi", t1 - t0), ("solve", t2 - t1), ("split", t3 - t2), ("env", t4 - t3)):
             timings[key] = timings.get(key, 0.0) + dt
+        if "plan_seconds" in info:  # the part of "solve" spent building the plan
+            timings["plan"] = timings.get("plan", 0.0) + info["plan_seconds"]
     return E, tw, info
 
 
