@@ -406,6 +406,9 @@ def dmrg_c60_bench(args):
                          "step_speedup_mixed_vs_fp64": f64["seconds_timed"] / mx["seconds_timed"],
                          "fp64_ms_per_step": f64["ms_per_step"], "fp64_gflops": f64["gflops"]})
 
+    if args.legs_only:  # profiling target: the DMRG steps and nothing after them
+        return {"workload": "c60_heisenberg_su2_M%d" % args.M, "dmrg": dmrg_rec}
+
     # dominant kernel of the mixed HxV on the first timed pivot (HIP events on the plan's stream)
     H = PivotMatrix2(first_pd, precision=args.mixed)
     st = H.stats()
@@ -674,6 +677,8 @@ def main():
     ap.add_argument("--hxv-reps", type=int, default=0,
                     help="C60 profiling target: prepare the state, then only this many HxV of the first timed pivot "
                          "in the mixed precision and in FP64 (no DMRG legs)")
+    ap.add_argument("--legs-only", action="store_true",
+                    help="C60 profiling target: only the DMRG steps (no roofline timing, CPU baseline or extras)")
     ap.add_argument("--hxv-precision", default="both", choices=["both", "mixed", "fp64"],
                     help="C60 profiling target: which operator(s) --hxv-reps runs")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)

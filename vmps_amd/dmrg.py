@@ -295,7 +295,10 @@ def two_site_psi(pd, Al, Ar, qloc):
     for s, blocks in enumerate(pd.V):
         qmerge, s1, s3 = pd.basis[s]
         for (qin, qout, r, c) in blocks:
-            acc = None
+            # the block's column-major slot, written through its transpose
+            # (row-major c x r): (A_l A_r)^T = A_r^T A_l^T lands in place
+            dst_t = v[off:off + r * c].reshape((c, r))
+            first = True
             for qm in sym.reduce(qin, qloc[s1]):
                 a = Al[s1].get((qin, qm))
                 b = Ar[s3].get((qm, qout))
@@ -304,10 +307,14 @@ def two_site_psi(pd, Al, Ar, qloc):
                 f = coeff_two_site_gate(sym, qin, qloc[s1], qm, qloc[s3], qout, qmerge)
                 if abs(f) < 1e-14:
                     continue
-                t = f * (a @ b)
-                acc = t if acc is None else acc + t
-            if acc is not None:
-                v[off:off + r * c] = acc.ravel(order="F")
+                if first:
+                    np.matmul(b.T, a.T, out=dst_t)
+                    dst_t *= f
+                    first = False
+                else:
+                    t = np.matmul(b.T, a.T)
+                    t *= f
+                    dst_t += t
             off += r * c
     return v
 
@@ -383,6 +390,21 @@ def block_svd_host(mats):
         return [futs[i].result() for i in range(len(mats))]
 
 
+def _packed_base(mats):
+    """The buffer that holds `mats` back to back in column-major order (as
+    split_psi lays them out, with one spare element at the end), or None."""
+    base = mats[0].base
+    if base is None or base.ndim != 1 or base.dtype != np.float64 or not base.flags.c_contiguous:
+        return None
+    ptr = base.ctypes.data
+    off = 0
+    for m in mats:
+        if m.base is not base or not m.flags.f_contiguous or m.ctypes.data != ptr + 8 * off:
+            return None
+        off += m.size
+    return base if base.size == off + 1 else None
+
+
 def block_eigh_device(mats, side):
     """Density-matrix factorisation of every matrix of `mats` on the device
     (vmps_block_eigh_split): side 0 gives (lambda, W, C) with
@@ -394,7 +416,9 @@ def block_eigh_device(mats, side):
     rows = np.array([m.shape[0] for m in mats], dtype=np.int64)
     cols = np.array([m.shape[1] for m in mats], dtype=np.int64)
     k = rows if side == 0 else cols
-    A = np.concatenate([np.asarray(m, dtype=np.float64).ravel(order="F") for m in mats] + [np.zeros(1)])
+    A = _packed_base(mats)
+    if A is None:
+        A = np.concatenate([np.asarray(m, dtype=np.float64).ravel(order="F") for m in mats] + [np.zeros(1)])
     lam = np.empty(int(k.sum()) + 1)
     W = np.empty(int((k * k).sum()) + 1)
     C = np.empty(A.size)
@@ -408,12 +432,13 @@ def block_eigh_device(mats, side):
         w_ = W[wo:wo + kk * kk].reshape((kk, kk), order="F")[:, ::-1]
         c_ = C[co:co + r * c].reshape((r, c), order="F")
         c_ = c_[::-1, :] if side == 0 else c_[:, ::-1]
-        out.append((l_, np.ascontiguousarray(w_), np.ascontiguousarray(c_)))
+        out.append((l_, w_, c_))  # views in descending order of lambda
         lo, wo, co = lo + kk, wo + kk * kk, co + r * c
     return out
 
 
-def split_psi(pd, v, qloc, M, eps_svd=1e-14, left_to_right=True, svd=None, eigh=None, eps_rho=1e-15):
+def split_psi(pd, v, qloc, M, eps_svd=1e-14, left_to_right=True, svd=None, eigh=None, eps_rho=1e-15,
+              timings=None):
     """split_AA2 + truncateSVD: recouple the two-site blocks to the middle
     sector Qm (coeff_splitAA), factor each Qm, keep the M largest singular
     values (multiplets) above eps_svd.  Returns (A_l, A_r, truncation weight =
@@ -427,29 +452,30 @@ def split_psi(pd, v, qloc, M, eps_svd=1e-14, left_to_right=True, svd=None, eigh=
     (or Psi W), so no singular value is divided by; sigma^2 below eps_rho
     (the eigenvalues' resolution relative to ||Psi||^2) count as discarded."""
     sym = pd.sym
+    t0 = time.perf_counter()
     if not np.all(np.isfinite(v)):
         raise FloatingPointError("split: the two-site wavefunction has non-finite entries")
-    clump = {}  # Qm -> {(row key, col key): block}
+    # first pass: which blocks go to which middle sector Qm, with their factors
+    parts = {}  # Qm -> [(row key, col key, offset of the block in v, r, c, factor)]
     rdims, cdims = {}, {}
     off = 0
     for s, bl in enumerate(pd.V):
         q12, sl, sr = pd.basis[s]
         for (qin, qout, r, c) in bl:
-            blk = v[off:off + r * c].reshape((r, c), order="F")
-            off += r * c
             for Qm in sym.reduce(qin, qloc[sl]):
                 if qout not in sym.reduce(Qm, qloc[sr]):
                     continue
                 f = coeff_split6(sym, qin, qout, q12, qloc[sr], qloc[sl], Qm)
                 if abs(f) < 1e-14:
                     continue
-                key = ((sl, qin), (sr, qout))
-                d = clump.setdefault(Qm, {})
-                d[key] = f * blk if key not in d else d[key] + f * blk
+                parts.setdefault(Qm, []).append(((sl, qin), (sr, qout), off, r, c, f))
                 rdims.setdefault(Qm, {})[(sl, qin)] = r
                 cdims.setdefault(Qm, {})[(sr, qout)] = c
-    mats, meta = [], []
-    for Qm, parts in clump.items():
+            off += r * c
+    # second pass: every sector matrix written in place, column-major, into one
+    # packed buffer (the layout the device factorisation reads), no temporaries
+    meta, shapes, total = [], [], 0
+    for Qm in parts:
         ro, o = {}, 0
         for k in sorted(rdims[Qm]):
             ro[k] = o
@@ -459,17 +485,27 @@ def split_psi(pd, v, qloc, M, eps_svd=1e-14, left_to_right=True, svd=None, eigh=
         for k in sorted(cdims[Qm]):
             co[k] = o
             o += cdims[Qm][k]
-        nc = o
-        Mm = np.zeros((nr, nc))
-        for (rk, ck), blk in parts.items():
-            Mm[ro[rk]:ro[rk] + blk.shape[0], co[ck]:co[ck] + blk.shape[1]] = blk
-        mats.append(Mm)
         meta.append((Qm, ro, co))
+        shapes.append((nr, o, total))
+        total += nr * o
+    packed = np.zeros(total + 1)
+    mats = [packed[o0:o0 + nr * nc].reshape((nr, nc), order="F") for (nr, nc, o0) in shapes]
+    for (Qm, ro, co), Mm in zip(meta, mats):
+        seen = set()
+        for (rk, ck, o0, r, c, f) in parts[Qm]:
+            blk = v[o0:o0 + r * c].reshape((r, c), order="F")
+            dst = Mm[ro[rk]:ro[rk] + r, co[ck]:co[ck] + c]
+            if (rk, ck) in seen:
+                dst += f * blk
+            else:
+                np.multiply(blk, f, out=dst)
+                seen.add((rk, ck))
     fac = {}
     allsv = []
+    t1 = time.perf_counter()
     efac = eigh(mats, 0 if left_to_right else 1) if eigh is not None else None  # None: declined
     if efac is not None:
-        norm2 = sum(float(np.sum(m * m)) for m in mats)
+        norm2 = float(np.dot(packed, packed))
         floor = max(eps_svd, math.sqrt(eps_rho * norm2))
         for (Qm, ro, co), (lam, W, C) in zip(meta, efac):
             S = np.sqrt(np.maximum(lam, 0.0))
@@ -482,6 +518,7 @@ def split_psi(pd, v, qloc, M, eps_svd=1e-14, left_to_right=True, svd=None, eigh=
             X, Y = (U, S[:, None] * Vt) if left_to_right else (U * S[None, :], Vt)
             fac[Qm] = (X, S, Y, ro, co)
             allsv.extend((sv, Qm, i) for i, sv in enumerate(S))
+    t2 = time.perf_counter()
     allsv.sort(key=lambda x: -x[0])
     keep = [x for x in allsv[:M] if x[0] > floor]
     kept = {}
@@ -501,10 +538,18 @@ def split_psi(pd, v, qloc, M, eps_svd=1e-14, left_to_right=True, svd=None, eigh=
     for Qm, k in kept.items():
         X, S, Y, ro, co = fac[Qm]
         X, Y = X[:, :k], Y[:k, :]
+        # site tensors in column-major order, the layout the environment
+        # update and the next pivot read without another copy
         for (sl, qin), o in ro.items():
-            Al[sl][(qin, Qm)] = X[o:o + rdims[Qm][(sl, qin)], :].copy()
+            Al[sl][(qin, Qm)] = X[o:o + rdims[Qm][(sl, qin)], :].copy(order="F")
         for (sr, qout), o in co.items():
-            Ar[sr][(Qm, qout)] = coeff_split3(sym, Qm, qout, qloc[sr]) * Y[:, o:o + cdims[Qm][(sr, qout)]]
+            src = Y[:, o:o + cdims[Qm][(sr, qout)]]
+            dst = np.empty(src.shape, order="F")
+            np.multiply(src, coeff_split3(sym, Qm, qout, qloc[sr]), out=dst)
+            Ar[sr][(Qm, qout)] = dst
+    if timings is not None:
+        for key, dt in (("split_assemble", t1 - t0), ("split_factor", t2 - t1), ("split_finish", time.perf_counter() - t2)):
+            timings[key] = timings.get(key, 0.0) + dt
     return Al, Ar, trunc, entropy, kept
 
 
@@ -600,15 +645,15 @@ class GpuBackend:
         self.hxv_calls += r["hxv_calls"]
         return r["energy"], v, r
 
-    def contract_L(self, site, Lold, A):
+    def contract_L(self, site, Lold, A, timings=None):
         from . import env
         blocks = _mps_to_blocks(A)
-        return env.contract_L(site, Lold, blocks, blocks, device=self.device_envs)
+        return env.contract_L(site, Lold, blocks, blocks, device=self.device_envs, timings=timings)
 
-    def contract_R(self, site, Rold, A):
+    def contract_R(self, site, Rold, A, timings=None):
         from . import env
         blocks = _mps_to_blocks(A)
-        return env.contract_R(site, Rold, blocks, blocks, device=self.device_envs)
+        return env.contract_R(site, Rold, blocks, blocks, device=self.device_envs, timings=timings)
 
 
 class Chain:
@@ -827,14 +872,16 @@ def dmrg_step(chain, backend, l, direction, rng=None, timings=None):
     E, v, info = backend.solve(pd, v0)
     t2 = t()
     Al, Ar, tw, ent, kept = split_psi(pd, v, chain.qloc, chain.M, left_to_right=(direction == "right"),
-                                      svd=getattr(backend, "svd", None), eigh=getattr(backend, "eigh", None))
+                                      svd=getattr(backend, "svd", None), eigh=getattr(backend, "eigh", None),
+                                      timings=timings)
     chain.A[l], chain.A[l + 1] = Al, Ar
     chain.bonds[l + 1] = {qm: k for qm, k in kept.items()}
     t3 = t()
+    kw = {"timings": timings} if timings is not None else {}  # backends without sub-phase timings take none
     if direction == "right":
-        chain.Lenv[l + 1] = backend.contract_L(chain.sites[l], chain.Lenv[l], chain.A[l])
+        chain.Lenv[l + 1] = backend.contract_L(chain.sites[l], chain.Lenv[l], chain.A[l], **kw)
     else:
-        chain.Renv[l + 1] = backend.contract_R(chain.sites[l + 1], chain.Renv[l + 2], chain.A[l + 1])
+        chain.Renv[l + 1] = backend.contract_R(chain.sites[l + 1], chain.Renv[l + 2], chain.A[l + 1], **kw)
     t4 = t()
     if timings is not None:
         for key, dt in (("psi", t1 - t0), ("solve", t2 - t1), ("split", t3 - t2), ("env", t4 - t3)):

@@ -6,6 +6,7 @@ MPS tensors are lists over the local index s of [(qin, qout, block)];
 the MPO site tensor is a models.MpoSite.  Blocks are 2-D float64 arrays.
 """
 import ctypes
+import time
 
 import numpy as np
 
@@ -86,7 +87,8 @@ class DeviceEnv:
             pass
 
 
-def _contract(which, site, old, Abra, Aket, precision="fp64", stats=None, device=False):
+def _contract(which, site, old, Abra, Aket, precision="fp64", stats=None, device=False, timings=None):
+    t0 = time.perf_counter()
     lib = _lib.load()
     sym = site.sym
     kinds, pk = _iarr(sym.kinds)
@@ -109,6 +111,7 @@ def _contract(which, site, old, Abra, Aket, precision="fp64", stats=None, device
         bk = pack.w_blocks(site.blocks, sym.nq)
         keep.append(bk)
         _lib.check(lib.vmps_env_add_W_blocks(h, bk.count, *bk.args()))
+        t1 = time.perf_counter()
         if device:
             d = ctypes.c_void_p()
             _lib.check(lib.vmps_env_contract_device(h, _lib.PREC[precision], ctypes.byref(d)))
@@ -120,18 +123,21 @@ def _contract(which, site, old, Abra, Aket, precision="fp64", stats=None, device
             st = _lib.PlanStats()
             _lib.check(lib.vmps_env_stats(h, ctypes.byref(st)))
             stats.update(st.as_dict())
+        if timings is not None:  # host-side hand-over of the blocks, then the contraction itself
+            for key, dt in (("env_prepare", t1 - t0), ("env_contract", time.perf_counter() - t1)):
+                timings[key] = timings.get(key, 0.0) + dt
     finally:
         lib.vmps_env_destroy(h)
     return out
 
 
-def contract_L(site, Lold, Abra, Aket, precision="fp64", stats=None, device=False):
+def contract_L(site, Lold, Abra, Aket, precision="fp64", stats=None, device=False, timings=None):
     """contract_L(Lold, Abra, W, Aket, qloc, qOp, Lnew) on the device; Lold
     may be a DeviceEnv, and device=True returns Lnew as one."""
-    return _contract(0, site, Lold, Abra, Aket, precision, stats, device)
+    return _contract(0, site, Lold, Abra, Aket, precision, stats, device, timings)
 
 
-def contract_R(site, Rold, Abra, Aket, precision="fp64", stats=None, device=False):
+def contract_R(site, Rold, Abra, Aket, precision="fp64", stats=None, device=False, timings=None):
     """contract_R(Rold, Abra, W, Aket, qloc, qOp, Rnew) on the device; Rold
     may be a DeviceEnv, and device=True returns Rnew as one."""
-    return _contract(1, site, Rold, Abra, Aket, precision, stats, device)
+    return _contract(1, site, Rold, Abra, Aket, precision, stats, device, timings)
