@@ -351,20 +351,26 @@ __global__ __launch_bounds__(256) void k_stage_c_bf16x3(const TileC* __restrict_
 // ---- LDS-staged 128 x 128 workgroup tiles for large blocks -------------------
 // 4 waves in a 2 x 2 grid, each owning 64 x 64 = 4 x 4 MFMA blocks.  Per
 // 64-deep K step the workgroup stages the split operand tiles A[128 x 64] and
-// B[64 x 128] (hi and lo) in LDS, k-contiguous with rows padded to 72
-// elements (conflict-free 16-byte reads), and each wave then issues
+// B[64 x 128] (hi and lo) in LDS, k-contiguous with rows padded to 80
+// elements (16-byte aligned reads), and each wave then issues
 // 2 x 4 x 4 x 3 MFMAs from LDS reads.  The global loads of the next step are
 // in flight during those MFMAs.
 constexpr int kBT = kBigTile;
 constexpr int kBK = kBigK;
-constexpr int kLS = kBK + 8;   // LDS row stride in bf16 elements
+// LDS row stride in bf16 elements: 160-byte rows (two workgroups of 80 KB
+// fill the CU's 160 KB); 3% faster than 144-byte rows on the C60 pivots
+constexpr int kLS = kBK + 16;
 constexpr int kPieces = kBT * kBK / 8 / 256;  // 16-byte pieces per operand tile and thread
 
 struct BigSmem {
   __bf16 Ah[kBT * kLS], Al[kBT * kLS], Bh[kBT * kLS], Bl[kBT * kLS];
 };
 
+// The wave raises its issue priority for its MFMA block (s_setprio), so that
+// a wave with operands in registers is served before the other workgroup's
+// waves staging their loads.
 __device__ inline void big_compute(const BigSmem& s, int wm, int wn, int li, int lg, v4f (&acc)[4][4]) {
+  __builtin_amdgcn_s_setprio(1);
 #pragma unroll
   for (int ks = 0; ks < kBK; ks += 32) {
     v8bf ah[4], al[4], bh[4], bl[4];
@@ -380,6 +386,7 @@ __device__ inline void big_compute(const BigSmem& s, int wm, int wn, int li, int
 #pragma unroll
       for (int ni = 0; ni < 4; ++ni) acc[mi][ni] = mac3(bh[ni], bl[ni], ah[mi], al[mi], acc[mi][ni]);
   }
+  __builtin_amdgcn_s_setprio(0);
 }
 
 // thread tid's piece h of a staged operand tile: row (of A) / column (of B), k offset

@@ -172,11 +172,19 @@ __global__ __launch_bounds__(256) void k_sub_combination_fused(const double* __r
                                                                double* __restrict__ norm_part) {
   __shared__ double cs[kFuseMaxRows];
   const int k = *kdev + kofs;
-  for (int i = threadIdx.x; i <= k; i += 256) {
+  // four threads per row, each summing every fourth partial, then the four
+  // sums combined by shuffles within the quad: a fixed order in every
+  // workgroup, and a quarter of the dependent load chain per thread
+  const int quad = threadIdx.x & 3;
+  for (int i = threadIdx.x >> 2; i <= k; i += 64) {  // quad-uniform
     double acc = 0.0;
-    for (int b = 0; b < nb; ++b) acc += partial[(int64_t)i * nb + b];
-    cs[i] = acc;
-    if (blockIdx.x == 0) c[i] = acc;
+    for (int b = quad; b < nb; b += 4) acc += partial[(int64_t)i * nb + b];
+    acc += __shfl_xor(acc, 1);
+    acc += __shfl_xor(acc, 2);
+    if (quad == 0) {
+      cs[i] = acc;
+      if (blockIdx.x == 0) c[i] = acc;
+    }
   }
   __syncthreads();
   int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;

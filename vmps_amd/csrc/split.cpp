@@ -47,7 +47,7 @@ void rb_check(rocblas_status st, const char* what) {
 // one split factor concurrently on as many streams (each handle owns its
 // workspace, so concurrent work needs one handle per stream).  Calls on one
 // device are serialised by the slot's mutex.
-constexpr int kLanes = 4;
+constexpr int kLanes = 8;
 struct HandleSlot {
   rocblas_handle h[kLanes] = {};
   std::mutex m;
