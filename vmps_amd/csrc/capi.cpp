@@ -704,6 +704,11 @@ vmps_status vmps_plan_autotune(vmps_plan* plan, int iters, vmps_tile_config* out
     return (double)ms / iters;
   };
   TileChoice pick{D.H.wave_edge, D.H.big_min, D.H.big_min64, D.H.explicit_u};
+  // Pivots with blocks of 256 and more always take the FP64 / FP32 big tiles:
+  // without them such a pivot's HxV is 2-3x slower (C60 at M = 4096: 124 ms
+  // against 45 ms), so the search neither starts from nor times that choice.
+  const bool large = D.H.stats.max_block_dim >= 2 * vmps::kBigTile && D.prec != VMPS_PREC_BF16X3;
+  if (large && pick.big_min64 <= 0) pick.big_min64 = vmps::kBigTile;
   double best = measure(pick);
   int tried = 1;
   auto consider = [&](TileChoice c) {
@@ -731,6 +736,7 @@ vmps_status vmps_plan_autotune(vmps_plan* plan, int iters, vmps_tile_config* out
       consider(c);
     }
     for (int b64 : {0, vmps::kBigTile}) {
+      if (large && b64 == 0) continue;
       TileChoice c = pick;
       c.big_min64 = b64;
       consider(c);

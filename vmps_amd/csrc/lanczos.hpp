@@ -30,6 +30,7 @@ struct RefineOpts {
   double tol_state = 1e-7;    // (rho = ||H psi - E psi||, FP64 operator)
   int max_outer = 4;
   int cg_max = 200;           // CG iterations per correction
+  double cg_tol_min = 1e-4;   // finest CG tolerance relative to the residual
 };
 
 struct RefineResult {

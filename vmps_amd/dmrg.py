@@ -624,14 +624,14 @@ class GpuBackend:
         flops = H.stats()["flops_alg"]
         tune = self.tune_min_flops is not None and flops >= self.tune_min_flops
         if tune:
-            H.autotune(iters=2)
+            H.autotune(iters=1)
         prec = self.precision
         if prec != "fp64" and flops < self.mixed_min_flops:
             prec = "fp64"
         if prec != "fp64":
             H.set_precision(prec)
             if tune:
-                H.autotune(iters=2)
+                H.autotune(iters=1)
             self.mixed_solves += 1
         t_plan = time.perf_counter() - t0
         v, r = lanczos(H, v0=v0, precision=prec, refine_fp64=self.refine, max_iter=self.max_iter,
