@@ -76,6 +76,7 @@ struct Krylov {
   int64_t chunk = 0;
   int cap = 0;
   int64_t nvec = -1;
+  int device = 0;  // where Q lives
   // one captured iteration per precision (a mixed solve and its FP64
   // refinement each keep theirs), valid for the plan generation it was made at
   hipGraph_t graph[3] = {nullptr, nullptr, nullptr};

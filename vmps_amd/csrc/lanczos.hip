@@ -10,6 +10,8 @@
 // tridiagonal matrix by Sturm bisection plus inverse iteration, and the
 // residual ||H psi - E psi|| evaluated with one extra application.
 #include <cmath>
+#include <map>
+#include <mutex>
 #include <cstring>
 #include <vector>
 
@@ -669,12 +671,61 @@ RefineResult refine_eigenpair(DevPlan& D, double* d_v, const RefineOpts& o, vmps
   return res;
 }
 
+// The Krylov basis is the largest buffer of a solve (kmax + 5 vectors: 37 GB
+// for 300 iterations on the C60 M = 4096 pivots), and a DMRG sweep builds a
+// new plan, hence a new basis, at every step; allocating it took ~25 ms.  One
+// released basis per device is kept (until the process ends) for the next
+// plan that fits in it.  A basis is released only after its plan's stream has
+// finished (the plan synchronises its stream before its members go).
+namespace {
+std::mutex g_basis_mutex;
+std::map<int, std::pair<double*, size_t>> g_basis;  // device -> (pointer, doubles)
+
+void basis_acquire(DevBuf<double>& Q, size_t count, int device) {
+  Q.release();
+  {
+    std::lock_guard<std::mutex> lock(g_basis_mutex);
+    auto it = g_basis.find(device);
+    if (it != g_basis.end() && it->second.second >= count) {
+      Q.p = it->second.first;
+      Q.n = it->second.second;
+      g_basis.erase(it);
+      return;
+    }
+  }
+  Q.alloc(count);
+}
+
+void basis_release(DevBuf<double>& Q, int device) {
+  if (!Q.p) return;
+  double* give = Q.p;
+  size_t n = Q.n;
+  Q.p = nullptr;
+  Q.n = 0;
+  double* drop = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(g_basis_mutex);
+    auto& slot = g_basis[device];
+    if (slot.first && slot.second >= n) {
+      drop = give;  // the kept one is at least as large
+    } else {
+      drop = slot.first;
+      slot = {give, n};
+    }
+  }
+  if (drop) (void)hipFree(drop);
+}
+}  // namespace
+
 void Krylov::ensure(DevPlan& D, int kmax) {
   const int64_t n = D.H.vsize;
   const int need = kmax + kIterBatch;
   if (cap >= need && nvec == n) return;
   release_graph();
-  Q.alloc((size_t)(need + 1) * n);
+  VMPS_HIP_CHECK(hipStreamSynchronize(D.stream));  // a basis given back has no work pending
+  basis_release(Q, device);
+  device = D.device;
+  basis_acquire(Q, (size_t)(need + 1) * n, device);
   w.alloc(n);
   v.alloc(n);
   c.alloc(2 * (size_t)need + 2);
@@ -705,6 +756,7 @@ void Krylov::ensure(DevPlan& D, int kmax) {
 
 Krylov::~Krylov() {
   release_graph();
+  basis_release(Q, device);
   if (host3) (void)hipHostFree(host3);
   host3 = nullptr;
   for (auto& e : done) {

@@ -617,8 +617,10 @@ class GpuBackend:
             return block_svd_device(mats)
         return block_svd_host(mats)
 
-    def solve(self, pd, v0):
-        from .pivot import PivotMatrix2, lanczos
+    def prepare(self, pd):
+        """The pivot's plan and tile choice (they do not need the start
+        vector).  Returns (plan, precision, seconds)."""
+        from .pivot import PivotMatrix2
         t0 = time.perf_counter()
         H = PivotMatrix2(pd, precision="fp64")
         flops = H.stats()["flops_alg"]
@@ -633,15 +635,21 @@ class GpuBackend:
             if tune:
                 H.autotune(iters=1)
             self.mixed_solves += 1
-        t_plan = time.perf_counter() - t0
-        v, r = lanczos(H, v0=v0, precision=prec, refine_fp64=self.refine, max_iter=self.max_iter,
-                       tol_eigval=self.tol_eigval, tol_state=self.tol_state,
-                       refine_tol_eigval=self.tol_eigval, refine_tol_state=self.tol_state,
-                       refine_method=self.refine_method)
-        r["flops_alg"] = H.stats()["flops_alg"]
+        return H, prec, time.perf_counter() - t0
+
+    def solve(self, pd, v0, prepared=None):
+        from .pivot import lanczos
+        H, prec, t_plan = prepared if prepared is not None else self.prepare(pd)
+        try:
+            v, r = lanczos(H, v0=v0, precision=prec, refine_fp64=self.refine, max_iter=self.max_iter,
+                           tol_eigval=self.tol_eigval, tol_state=self.tol_state,
+                           refine_tol_eigval=self.tol_eigval, refine_tol_state=self.tol_state,
+                           refine_method=self.refine_method)
+            r["flops_alg"] = H.stats()["flops_alg"]
+        finally:
+            H.close()
         r["precision"] = prec
         r["plan_seconds"] = t_plan  # plan set-up and tile choice, part of the solve
-        H.close()
         self.hxv_calls += r["hxv_calls"]
         return r["energy"], v, r
 
