@@ -233,12 +233,15 @@ def _c2_hxv_lines(args):
 
 def precision_report_c3(args, M=2048, steps=3):
     """HubbardSU2xU1 two-leg ladder 2 x 32 (snake order, t = 1, U = 6, N = 56,
-    S = 0) at M = 2048 multiplets: the same two-site DMRG steps (from a state
-    warmed up at small M and expanded to M) in FP64, FP32 and split-BF16x3,
-    each mixed solve refined in FP64.  Per precision and step: the energy and
-    its difference from the FP64 leg, the residual ||H psi - E psi|| (FP64
-    operator), the mixed solve's own Rayleigh quotient before refinement and
-    its error, the truncation weight and its difference from FP64."""
+    S = 0) at M = 2048 multiplets: two-site DMRG steps (from a state warmed up
+    at small M and expanded to M) in FP64, FP32 and split-BF16x3, each mixed
+    solve refined in FP64.  The sweep follows the FP64 steps; at every pivot
+    the FP32 and split-BF16x3 steps start from the same state (copies of the
+    chain), so that their differences are the precision's alone.  Per
+    precision and step: the energy and its difference from FP64, the residual
+    ||H psi - E psi|| (FP64 operator), the mixed solve's own Rayleigh quotient
+    before refinement and its error, the truncation weight and its difference
+    from FP64."""
     from vmps_amd import dmrg, models
     L, Q, p0 = 64, (1, 56), 30
     sites = models.chain_mpo("HubbardSU2xU1Ladder", L, t=1.0, U=6.0)
@@ -249,18 +252,18 @@ def precision_report_c3(args, M=2048, steps=3):
     big = dmrg.expand_chain(warm, M, p0, prep, np.random.default_rng(args.seed + 4))
     out = {"model": "HubbardSU2xU1 ladder 2x32 snake, t=1, U=6, N=56, S=0", "M": M, "pivots": [p0, p0 + steps - 1],
            "prep_seconds": time.perf_counter() - t0}
-    legs = {}
-    for prec in ("fp64", "fp32", "bf16x3"):
-        c = big.copy()
-        be = dmrg.GpuBackend(precision=prec, mixed_min_flops=0, **tol)
-        rec = []
-        for i in range(steps):
+    legs = {prec: [] for prec in ("fp64", "fp32", "bf16x3")}
+    be = {prec: dmrg.GpuBackend(precision=prec, mixed_min_flops=0, **tol) for prec in legs}
+    c = big.copy()
+    for i in range(steps):
+        for prec in ("fp32", "bf16x3", "fp64"):  # the FP64 step last: the sweep continues from it
+            chain = c if prec == "fp64" else c.copy()
             ts = time.perf_counter()
-            E, tw, info = dmrg.dmrg_step(c, be, p0 + i, "right")
-            rec.append({"energy": E, "truncation": tw, "residual": info["residual"], "seconds": time.perf_counter() - ts,
-                        "energy_unrefined": info["energy_mixed"], "residual_unrefined": info["residual_mixed"],
-                        "hxv_calls": info["hxv_calls"], "hxv_fp64": info["hxv_fp64"]})
-        legs[prec] = rec
+            E, tw, info = dmrg.dmrg_step(chain, be[prec], p0 + i, "right")
+            legs[prec].append({"energy": E, "truncation": tw, "residual": info["residual"],
+                               "seconds": time.perf_counter() - ts,
+                               "energy_unrefined": info["energy_mixed"], "residual_unrefined": info["residual_mixed"],
+                               "hxv_calls": info["hxv_calls"], "hxv_fp64": info["hxv_fp64"]})
     for prec in ("fp32", "bf16x3"):
         for a, b in zip(legs[prec], legs["fp64"]):
             a["abs_dE_vs_fp64"] = abs(a["energy"] - b["energy"])
