@@ -404,6 +404,20 @@ vmps_status vmps_vec_axpby(vmps_plan* plan, double a, const double* d_x, double 
  * normalised (DmrgPivotVector.h:410-424); counter-based, reproducible per seed */
 vmps_status vmps_vec_fill_random(vmps_plan* plan, double* d_v, uint64_t seed);
 
+/* contract_AA2 (tensors/DmrgContractions.h:1869-1927) into a device vector
+ * of the plan's layout: d_out is zeroed, then for every term t
+ *   d_out[out_off[t] ..] (r x c, column-major) += factor[t] * A1[i1[t]] . A2[i2[t]]
+ * with A1[b] (rows1[b] x cols1[b]) and A2[b] (rows2[b] x cols2[b]) host
+ * blocks, column-major float64, copied to the device (A1 blocks: the
+ * Biped blocks of A[l][s1], A2 blocks those of A[l+1][s2]; factor: the
+ * coeff_twoSiteGate factor of the term; out_off: the offset of the Apair
+ * block (in1, q2) of s1s2 in the pivot vector).  The products are rocBLAS
+ * GEMMs on the plan's stream; returns when d_out is complete. */
+vmps_status vmps_contract_aa2_device(vmps_plan* plan, int n1, const int64_t* rows1, const int64_t* cols1,
+                                     const double* const* A1, int n2, const int64_t* rows2, const int64_t* cols2,
+                                     const double* const* A2, int nterms, const int64_t* out_off, const int* i1,
+                                     const int* i2, const double* factor, double* d_out);
+
 /* ---- Krylov eigensolver (stands in for the external LanczosSolver that
  * DmrgSolver::iteration_two calls, solvers/DmrgSolver.h:1466-1471) ---- */
 typedef struct {

@@ -1,10 +1,13 @@
 // extern "C" boundary (include/vmps_hxv.h).  Every entry point catches all
 // exceptions and turns them into a status code plus a thread-local message.
 #include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
 
 #include <algorithm>
 #include <chrono>
+#include <climits>
 #include <cmath>
+#include <cstdint>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -829,6 +832,101 @@ vmps_status vmps_vec_fill_random(vmps_plan* plan, double* d_v, uint64_t seed) {
     vmps::dev_axpby(0.0, d_v, 1.0 / nrm, d_v, n, D.stream);
     VMPS_HIP_CHECK(hipStreamSynchronize(D.stream));
   }
+  VMPS_API_END
+}
+
+namespace {
+// rocBLAS handle per device for the two-site vector build, made on first use
+// and kept until the process ends; calls on one device are serialised.
+struct BlasSlot {
+  rocblas_handle h = nullptr;
+  std::mutex m;
+};
+std::mutex g_blas_mutex;
+std::map<int, BlasSlot*> g_blas;
+
+BlasSlot& blas_for_device(int dev) {
+  std::lock_guard<std::mutex> lock(g_blas_mutex);
+  BlasSlot*& s = g_blas[dev];
+  if (!s) {
+    auto* slot = new BlasSlot();
+    if (rocblas_create_handle(&slot->h) != rocblas_status_success) {
+      delete slot;
+      throw VmpsError(VMPS_ERR_HIP, "rocblas_create_handle failed");
+    }
+    s = slot;
+  }
+  return *s;
+}
+}  // namespace
+
+vmps_status vmps_contract_aa2_device(vmps_plan* plan, int n1, const int64_t* rows1, const int64_t* cols1,
+                                     const double* const* A1, int n2, const int64_t* rows2, const int64_t* cols2,
+                                     const double* const* A2, int nterms, const int64_t* out_off, const int* i1,
+                                     const int* i2, const double* factor, double* d_out) {
+  VMPS_API_BEGIN
+  require(plan && !plan->batch, VMPS_ERR_ARG, "NULL or batch plan");
+  require(n1 >= 0 && n2 >= 0 && nterms >= 0, VMPS_ERR_ARG, "negative count");
+  require(n1 == 0 || (rows1 && cols1 && A1), VMPS_ERR_ARG, "NULL A1 table");
+  require(n2 == 0 || (rows2 && cols2 && A2), VMPS_ERR_ARG, "NULL A2 table");
+  require(nterms == 0 || (out_off && i1 && i2 && factor), VMPS_ERR_ARG, "NULL term table");
+  auto& D = plan->D;
+  const int64_t vsize = D.H.vsize;
+  require(vsize == 0 || d_out, VMPS_ERR_ARG, "NULL output vector");
+  // every term's operands chain and its block lies inside the vector
+  std::vector<char> used1((size_t)n1, 0), used2((size_t)n2, 0);
+  for (int t = 0; t < nterms; ++t) {
+    require(i1[t] >= 0 && i1[t] < n1 && i2[t] >= 0 && i2[t] < n2, VMPS_ERR_ARG, "term block index out of range");
+    const int64_t r = rows1[i1[t]], k = cols1[i1[t]], k2 = rows2[i2[t]], c = cols2[i2[t]];
+    require(r >= 0 && k >= 0 && k2 >= 0 && c >= 0, VMPS_ERR_ARG, "negative block dimension");
+    require(k == k2, VMPS_ERR_LAYOUT, "A1 and A2 blocks of a term do not chain");
+    require(out_off[t] >= 0 && out_off[t] + r * c <= vsize, VMPS_ERR_LAYOUT, "term block outside the vector");
+    require(r <= INT32_MAX && c <= INT32_MAX && k <= INT32_MAX, VMPS_ERR_ARG, "block too large");
+    if (r * c == 0 || k == 0) continue;
+    require(A1[i1[t]] && A2[i2[t]], VMPS_ERR_ARG, "NULL block data");
+    used1[(size_t)i1[t]] = used2[(size_t)i2[t]] = 1;
+  }
+  // the blocks the terms use, packed on the device
+  std::vector<int64_t> off1((size_t)n1, 0), off2((size_t)n2, 0);
+  int64_t tot1 = 0, tot2 = 0;
+  for (int b = 0; b < n1; ++b)
+    if (used1[(size_t)b]) off1[(size_t)b] = tot1, tot1 += rows1[b] * cols1[b];
+  for (int b = 0; b < n2; ++b)
+    if (used2[(size_t)b]) off2[(size_t)b] = tot2, tot2 += rows2[b] * cols2[b];
+  vmps::DevBuf<double> d1, d2;
+  d1.alloc((size_t)std::max<int64_t>(1, tot1));
+  d2.alloc((size_t)std::max<int64_t>(1, tot2));
+  const hipStream_t s = D.stream;
+  for (int b = 0; b < n1; ++b)
+    if (used1[(size_t)b])
+      VMPS_HIP_CHECK(hipMemcpyAsync(d1.p + off1[(size_t)b], A1[b], sizeof(double) * rows1[b] * cols1[b],
+                                    hipMemcpyHostToDevice, s));
+  for (int b = 0; b < n2; ++b)
+    if (used2[(size_t)b])
+      VMPS_HIP_CHECK(hipMemcpyAsync(d2.p + off2[(size_t)b], A2[b], sizeof(double) * rows2[b] * cols2[b],
+                                    hipMemcpyHostToDevice, s));
+  if (vsize) VMPS_HIP_CHECK(hipMemsetAsync(d_out, 0, sizeof(double) * vsize, s));
+  int dev = 0;
+  VMPS_HIP_CHECK(hipGetDevice(&dev));
+  BlasSlot& slot = blas_for_device(dev);
+  {
+    std::lock_guard<std::mutex> hold(slot.m);
+    if (rocblas_set_stream(slot.h, s) != rocblas_status_success)
+      throw VmpsError(VMPS_ERR_HIP, "rocblas_set_stream failed");
+    const double one = 1.0;
+    // out block (r x c, column-major) += f A1 (r x k) . A2 (k x c)
+    for (int t = 0; t < nterms; ++t) {
+      const int64_t r = rows1[i1[t]], k = cols1[i1[t]], c = cols2[i2[t]];
+      if (r * c == 0 || k == 0) continue;
+      const rocblas_status st =
+          rocblas_dgemm(slot.h, rocblas_operation_none, rocblas_operation_none, (rocblas_int)r, (rocblas_int)c,
+                        (rocblas_int)k, &factor[t], d1.p + off1[(size_t)i1[t]], (rocblas_int)r,
+                        d2.p + off2[(size_t)i2[t]], (rocblas_int)k, &one, d_out + out_off[t], (rocblas_int)r);
+      if (st != rocblas_status_success)
+        throw VmpsError(VMPS_ERR_HIP, std::string("rocblas_dgemm: ") + rocblas_status_to_string(st));
+    }
+  }
+  VMPS_HIP_CHECK(hipStreamSynchronize(s));  // the packed blocks are freed on return
   VMPS_API_END
 }
 

@@ -321,6 +321,13 @@ vmps_status vmps_env_add_A_blocks(vmps_env* e, int bra, int nblk, const int* s, 
   require(e && nblk >= 0 && (nblk == 0 || (s && qin && qout && rows && cols && blocks)), VMPS_ERR_ARG,
           "bad argument");
   const int nq = e->P.sym.nq;
+  {  // one reservation per local index
+    std::vector<vmps::DBiped>& As = bra ? e->P.Abra : e->P.Aket;
+    std::vector<size_t> add(As.size(), 0);
+    for (int b = 0; b < nblk; ++b)
+      if (s[b] >= 0 && s[b] < (int)As.size() && rows[b] > 0 && cols[b] > 0) add[s[b]] += (size_t)(rows[b] * cols[b]);
+    for (size_t i = 0; i < As.size(); ++i) As[i].data.reserve(As[i].data.size() + add[i]);
+  }
   for (int b = 0; b < nblk; ++b) {
     const vmps_status st = vmps_env_add_A(e, bra, s[b], qin + b * nq, qout + b * nq, rows[b], cols[b], blocks[b]);
     if (st != VMPS_OK) return st;

@@ -692,6 +692,8 @@ void DevPlan::upload_bf16_lists(const Bf16Layout& B) {
 }
 
 void DevPlan::retile(int wave_edge, int big_min, int big_min64, bool explicit_u) {
+  if (wave_edge == H.wave_edge && big_min == H.big_min && big_min64 == H.big_min64 && explicit_u == H.explicit_u)
+    return;  // the lists on the device are already these
   ++gen;
   const bool bf16_lists = bf16_ready && (big_min != H.big_min || explicit_u != H.explicit_u);
   if (explicit_u != H.explicit_u) {

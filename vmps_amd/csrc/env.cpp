@@ -1,6 +1,7 @@
 // contract_L / contract_R planners; see env.hpp.
 #include "env.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <map>
 
@@ -61,8 +62,16 @@ void pack(const double* src, int64_t rows, int64_t cols, bool transpose, std::ve
     blocks.push_back({off, rows, cols});
   } else {
     dst.resize(dst.size() + rows * cols);
-    for (int64_t j = 0; j < cols; ++j)
-      for (int64_t i = 0; i < rows; ++i) dst[off + i * cols + j] = src[j * rows + i];  // (cols x rows) col-major
+    // (cols x rows) col-major, in 32 x 32 tiles so that reads and writes
+    // both stay within a few cache lines per row
+    double* d = dst.data() + off;
+    constexpr int64_t kT = 32;
+    for (int64_t j0 = 0; j0 < cols; j0 += kT)
+      for (int64_t i0 = 0; i0 < rows; i0 += kT) {
+        const int64_t j1 = std::min(cols, j0 + kT), i1 = std::min(rows, i0 + kT);
+        for (int64_t j = j0; j < j1; ++j)
+          for (int64_t i = i0; i < i1; ++i) d[i * cols + j] = src[j * rows + i];
+      }
     blocks.push_back({off, cols, rows});
   }
 }
@@ -91,6 +100,13 @@ void plan_env(EnvProblem& P, TripleSet& S) {
   }
   S.ysize = P.old.size();
 
+  {  // each A block is packed at most once per operand: reserve that much
+    size_t nbra = 0, nket = 0;
+    for (const DBiped& A : P.Abra) nbra += A.data.size();
+    for (const DBiped& A : P.Aket) nket += A.data.size();
+    S.Xdata.reserve(L ? nbra : nket);
+    S.Zdata.reserve(L ? nket : nbra);
+  }
   std::map<std::pair<size_t, size_t>, int64_t> xmap, zmap;  // (s, q) of A -> packed block id
   std::unordered_map<Key3, size_t, Key3Hash> sec;            // new sector -> index
   std::vector<Hit> hits;

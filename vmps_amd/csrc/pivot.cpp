@@ -1080,6 +1080,14 @@ void build_plan_tables(const TripleSet& S, HostPlan& H) {
   }
 
   validate_host_plan(H);
+  // Pivots with blocks of 256 and more start from the explicit stage B and
+  // the FP64 / FP32 big tiles, which the autotuner picks on every such pivot
+  // measured; the tuner then finds the lists already built for its first
+  // candidate instead of building them a second time.
+  if (st.max_block_dim >= 2 * kBigTile) {
+    H.explicit_u = true;
+    if (H.big_min64 <= 0) H.big_min64 = kBigTile;
+  }
   build_tiles(H);
   st.n_terms = (int64_t)terms.size();
   st.n_T = (int64_t)H.probA.size();

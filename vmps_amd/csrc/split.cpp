@@ -26,6 +26,7 @@
 #include <map>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "engine.hpp"
@@ -171,52 +172,84 @@ extern "C" vmps_status vmps_block_eigh_split(int nblocks, const int64_t* rows, c
     dC.alloc((size_t)std::max<int64_t>(1, out_c));
     dL.alloc((size_t)std::max<int64_t>(1, l_acc));
     dInfo.alloc((size_t)std::max(1, nblocks));
-    const double one = 1.0, zero = 0.0;
     for (int l = 0; l < NL; ++l) vmps::rb_check(rocblas_set_stream(slot.h[l], lanes.s[l]), "rocblas_set_stream");
-    for (int b : order) {
-      const int l = lane[b];
-      rocblas_handle h = slot.h[l];
-      hipStream_t st = lanes.s[l];
-      const rocblas_int r = (rocblas_int)rows[b], c = (rocblas_int)cols[b], k = side == 0 ? r : c;
-      double* G = dG.p + g_off[b];
-      double* Cb = dC.p + c_off[b];
-      VMPS_HIP_CHECK(hipMemcpyAsync(dA[l].p, A + ao[b], sizeof(double) * r * c, hipMemcpyHostToDevice, st));
-      // rho = A A^T (r x r) or A^T A (c x c), the full matrix
-      if (side == 0)
-        vmps::rb_check(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_transpose, r, r, c, &one, dA[l].p, r,
-                                     dA[l].p, r, &zero, G, r),
-                       "rocblas_dgemm");
-      else
-        vmps::rb_check(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, c, c, r, &one, dA[l].p, r,
-                                     dA[l].p, r, &zero, G, c),
-                       "rocblas_dgemm");
-      vmps::rb_check(rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_upper, k, G, k, dL.p + lo[b], dE[l].p,
-                                      dInfo.p + b),
-                     "rocsolver_dsyevd");
-      // centre: W^T A (k x c) or A W (r x k)
-      if (side == 0)
-        vmps::rb_check(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, r, c, r, &one, G, r,
-                                     dA[l].p, r, &zero, Cb, r),
-                       "rocblas_dgemm");
-      else
-        vmps::rb_check(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, r, c, c, &one, dA[l].p, r, G,
-                                     c, &zero, Cb, r),
-                       "rocblas_dgemm");
+    // One host thread per lane: rocSOLVER's dsyevd issues thousands of small
+    // kernels per sector, so one thread launching for all lanes leaves the
+    // device waiting on launches.  Each thread works through its lane's
+    // sectors with its own handle and stream and copies their results back.
+    int dev = 0;
+    VMPS_HIP_CHECK(hipGetDevice(&dev));
+    std::string err[NL];
+    int err_status[NL] = {};
+    auto run_lane = [&](int l) {
+      try {
+        VMPS_HIP_CHECK(hipSetDevice(dev));
+        rocblas_handle h = slot.h[l];
+        hipStream_t st = lanes.s[l];
+        const double one = 1.0, zero = 0.0;
+        for (int b : order) {
+          if (lane[b] != l) continue;
+          const rocblas_int r = (rocblas_int)rows[b], c = (rocblas_int)cols[b], k = side == 0 ? r : c;
+          double* G = dG.p + g_off[b];
+          double* Cb = dC.p + c_off[b];
+          VMPS_HIP_CHECK(hipMemcpyAsync(dA[l].p, A + ao[b], sizeof(double) * r * c, hipMemcpyHostToDevice, st));
+          // rho = A A^T (r x r) or A^T A (c x c), the full matrix
+          if (side == 0)
+            vmps::rb_check(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_transpose, r, r, c, &one,
+                                         dA[l].p, r, dA[l].p, r, &zero, G, r),
+                           "rocblas_dgemm");
+          else
+            vmps::rb_check(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, c, c, r, &one,
+                                         dA[l].p, r, dA[l].p, r, &zero, G, c),
+                           "rocblas_dgemm");
+          vmps::rb_check(rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_upper, k, G, k, dL.p + lo[b],
+                                          dE[l].p, dInfo.p + b),
+                         "rocsolver_dsyevd");
+          // centre: W^T A (k x c) or A W (r x k)
+          if (side == 0)
+            vmps::rb_check(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, r, c, r, &one, G, r,
+                                         dA[l].p, r, &zero, Cb, r),
+                           "rocblas_dgemm");
+          else
+            vmps::rb_check(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, r, c, c, &one, dA[l].p,
+                                         r, G, c, &zero, Cb, r),
+                           "rocblas_dgemm");
+        }
+        for (int b : order) {
+          if (lane[b] != l) continue;
+          const int64_t k = side == 0 ? rows[b] : cols[b];
+          VMPS_HIP_CHECK(
+              hipMemcpyAsync(lambda + lo[b], dL.p + lo[b], sizeof(double) * k, hipMemcpyDeviceToHost, st));
+          VMPS_HIP_CHECK(
+              hipMemcpyAsync(W + wo[b], dG.p + g_off[b], sizeof(double) * k * k, hipMemcpyDeviceToHost, st));
+          VMPS_HIP_CHECK(hipMemcpyAsync(C + ao[b], dC.p + c_off[b], sizeof(double) * rows[b] * cols[b],
+                                        hipMemcpyDeviceToHost, st));
+        }
+        VMPS_HIP_CHECK(hipStreamSynchronize(st));
+      } catch (const VmpsError& e) {
+        err[l] = e.what();
+        err_status[l] = e.status;
+      } catch (const std::exception& e) {
+        err[l] = e.what();
+        err_status[l] = VMPS_ERR_INTERNAL;
+      }
+    };
+    {
+      std::vector<std::thread> workers;
+      for (int l = 1; l < NL; ++l)
+        if (lane_rc[l]) workers.emplace_back(run_lane, l);
+      if (lane_rc[0]) run_lane(0);
+      for (auto& t : workers) t.join();
     }
-    for (auto x : lanes.s) VMPS_HIP_CHECK(hipStreamSynchronize(x));
+    for (int l = 0; l < NL; ++l)
+      if (err_status[l]) throw VmpsError(err_status[l], err[l]);
     std::vector<int> info(std::max(1, nblocks), 0);
     if (!order.empty()) {
       VMPS_HIP_CHECK(hipMemcpy(info.data(), dInfo.p, sizeof(int) * nblocks, hipMemcpyDeviceToHost));
     }
-    for (int b : order) {
+    for (int b : order)
       if (info[b] != 0)
         throw VmpsError(VMPS_ERR_NOCONV, "rocsolver_dsyevd did not converge (info " + std::to_string(info[b]) + ")");
-      const int64_t k = side == 0 ? rows[b] : cols[b];
-      VMPS_HIP_CHECK(hipMemcpy(lambda + lo[b], dL.p + lo[b], sizeof(double) * k, hipMemcpyDeviceToHost));
-      VMPS_HIP_CHECK(hipMemcpy(W + wo[b], dG.p + g_off[b], sizeof(double) * k * k, hipMemcpyDeviceToHost));
-      VMPS_HIP_CHECK(hipMemcpy(C + ao[b], dC.p + c_off[b], sizeof(double) * rows[b] * cols[b],
-                               hipMemcpyDeviceToHost));
-    }
     return VMPS_OK;
   } catch (const VmpsError& e) {
     vmps::set_last_error(e.what());

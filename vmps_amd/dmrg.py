@@ -319,6 +319,41 @@ def two_site_psi(pd, Al, Ar, qloc):
     return v
 
 
+def two_site_psi_terms(pd, Al, Ar, qloc):
+    """The products of contract_AA2 (two_site_psi) in the pivot's layout,
+    for the device build (pivot.contract_aa2): the A_l and A_r blocks they
+    use, and per product the offset of its output block, the indices of its
+    two blocks and its coeff_twoSiteGate factor."""
+    sym = pd.sym
+    idx1, idx2, blocks1, blocks2 = {}, {}, [], []
+    out_off, i1, i2, factor = [], [], [], []
+    off = 0
+    for s, blocks in enumerate(pd.V):
+        qmerge, s1, s3 = pd.basis[s]
+        for (qin, qout, r, c) in blocks:
+            for qm in sym.reduce(qin, qloc[s1]):
+                a = Al[s1].get((qin, qm))
+                b = Ar[s3].get((qm, qout))
+                if a is None or b is None:
+                    continue
+                f = coeff_two_site_gate(sym, qin, qloc[s1], qm, qloc[s3], qout, qmerge)
+                if abs(f) < 1e-14:
+                    continue
+                k1, k2 = (s1, qin, qm), (s3, qm, qout)
+                if k1 not in idx1:
+                    idx1[k1] = len(blocks1)
+                    blocks1.append(a)
+                if k2 not in idx2:
+                    idx2[k2] = len(blocks2)
+                    blocks2.append(b)
+                out_off.append(off)
+                i1.append(idx1[k1])
+                i2.append(idx2[k2])
+                factor.append(f)
+            off += r * c
+    return blocks1, blocks2, out_off, i1, i2, factor
+
+
 def block_svd_device(mats):
     """Thin SVD of every matrix of `mats` in one device call (vmps_block_svd,
     one-sided Jacobi in FP64): list of (U, S, Vt) with S descending."""
@@ -558,6 +593,31 @@ def _mps_to_blocks(Asite):
 
 
 # ---------------------------------------------------------------------------
+_HOST_MEMORY_RETAINED = False
+
+
+def retain_freed_host_memory():
+    """Serve large host arrays from the heap and keep freed memory mapped
+    (glibc mallopt: no mmap'd chunks, trim threshold at its maximum).  A DMRG
+    step at M = 4096 allocates and frees a few hundred MB of host arrays (the
+    two-site vector, the packed sectors of the split, the site tensors, the
+    operand packing of the environment update); served from fresh mappings,
+    every 4 KB page of them faults in again at each step (30 ms of the
+    environment hand-over alone on the C60 steps).  Process-wide, applied
+    once; returns whether the allocator took the settings."""
+    global _HOST_MEMORY_RETAINED
+    if _HOST_MEMORY_RETAINED:
+        return True
+    try:
+        libc = ctypes.CDLL("libc.so.6")
+        M_TRIM_THRESHOLD, M_MMAP_MAX = -1, -4
+        ok = libc.mallopt(M_MMAP_MAX, 0) == 1 and libc.mallopt(M_TRIM_THRESHOLD, 2**31 - 1) == 1
+    except (OSError, AttributeError):
+        ok = False
+    _HOST_MEMORY_RETAINED = ok
+    return ok
+
+
 class GpuBackend:
     """Pivot eigensolver and environment updates on the device.
 
@@ -579,11 +639,17 @@ class GpuBackend:
     (block_eigh_device) when a middle sector has both dimensions >=
     device_split_min (default 256), on host LAPACK otherwise.  Pivots with at
     least tune_min_flops of work per HxV get autotuned tiles (cached per
-    block-size histogram, so a sweep tunes once per shape)."""
+    block-size histogram, so a sweep tunes once per shape).
+
+    With `retain_host_memory` (default) the backend asks the C library's
+    allocator to keep freed host memory mapped
+    (retain_freed_host_memory)."""
 
     def __init__(self, precision="fp64", refine_fp64=True, tol_eigval=1e-10, tol_state=1e-6, max_iter=100,
                  mixed_min_flops=4e9, device_svd_min=0, device_envs=True, refine_method="correction",
-                 device_split_min=256, tune_min_flops=1e9):
+                 device_split_min=256, tune_min_flops=1e9, retain_host_memory=True):
+        if retain_host_memory:
+            retain_freed_host_memory()
         self.precision = precision
         self.device_envs = device_envs
         self.refine = refine_fp64
@@ -637,6 +703,37 @@ class GpuBackend:
             self.mixed_solves += 1
         return H, prec, time.perf_counter() - t0
 
+    def solve_two_site(self, pd, Al, Ar, qloc, seed=0):
+        """contract_AA2 on the device into the start vector (a zero vector is
+        replaced by a random one), then solve() on it without a host copy of
+        the start vector.  Returns (energy, eigenvector on the host, result
+        dict with "psi_seconds", the time of the start-vector build)."""
+        from .pivot import DeviceVector, contract_aa2, lanczos_device, vec_fill_random, vec_norm
+        H, prec, t_plan = self.prepare(pd)
+        dv = None
+        try:
+            t0 = time.perf_counter()
+            dv = DeviceVector(H)
+            contract_aa2(H, dv, *two_site_psi_terms(pd, Al, Ar, qloc))
+            if vec_norm(H, dv) == 0.0:
+                vec_fill_random(H, dv, seed)
+            t_psi = time.perf_counter() - t0
+            r = lanczos_device(H, dv, precision=prec, refine_fp64=self.refine, max_iter=self.max_iter,
+                               tol_eigval=self.tol_eigval, tol_state=self.tol_state,
+                               refine_tol_eigval=self.tol_eigval, refine_tol_state=self.tol_state,
+                               refine_method=self.refine_method)
+            v = dv.download()
+            r["flops_alg"] = H.stats()["flops_alg"]
+        finally:
+            if dv is not None:
+                dv.free()
+            H.close()
+        r["precision"] = prec
+        r["plan_seconds"] = t_plan
+        r["psi_seconds"] = t_psi
+        self.hxv_calls += r["hxv_calls"]
+        return r["energy"], v, r
+
     def solve(self, pd, v0, prepared=None):
         from .pivot import lanczos
         H, prec, t_plan = prepared if prepared is not None else self.prepare(pd)
@@ -644,7 +741,7 @@ class GpuBackend:
             v, r = lanczos(H, v0=v0, precision=prec, refine_fp64=self.refine, max_iter=self.max_iter,
                            tol_eigval=self.tol_eigval, tol_state=self.tol_state,
                            refine_tol_eigval=self.tol_eigval, refine_tol_state=self.tol_state,
-                           refine_method=self.refine_method)
+                           refine_method=self.refine_method, overwrite_v0=True)
             r["flops_alg"] = H.stats()["flops_alg"]
         finally:
             H.close()
@@ -873,12 +970,18 @@ def dmrg_step(chain, backend, l, direction, rng=None, timings=None):
     t = time.perf_counter
     t0 = t()
     pd = chain.pivot(l)
-    v0 = two_site_psi(pd, chain.A[l], chain.A[l + 1], chain.qloc)
-    if not np.any(v0):
-        v0 = (rng or np.random.default_rng(l)).uniform(-1, 1, pd.vsize)
-    t1 = t()
-    E, v, info = backend.solve(pd, v0)
-    t2 = t()
+    if hasattr(backend, "solve_two_site"):  # contract_AA2 on the device, inside the solve
+        t1 = t()
+        E, v, info = backend.solve_two_site(pd, chain.A[l], chain.A[l + 1], chain.qloc, seed=l + 1)
+        t2 = t()
+        t1 += info["psi_seconds"]
+    else:
+        v0 = two_site_psi(pd, chain.A[l], chain.A[l + 1], chain.qloc)
+        if not np.any(v0):
+            v0 = (rng or np.random.default_rng(l)).uniform(-1, 1, pd.vsize)
+        t1 = t()
+        E, v, info = backend.solve(pd, v0)
+        t2 = t()
     Al, Ar, tw, ent, kept = split_psi(pd, v, chain.qloc, chain.M, left_to_right=(direction == "right"),
                                       svd=getattr(backend, "svd", None), eigh=getattr(backend, "eigh", None),
                                       timings=timings)

@@ -199,12 +199,8 @@ def dot(H, x, y):
 REFINE = {"correction": 0, "restart": 1}
 
 
-def lanczos(H, v0=None, precision="fp64", refine_fp64=True, max_iter=100, tol_eigval=1e-8, tol_state=1e-5,
-            refine_tol_eigval=1e-12, refine_tol_state=1e-7, seed=1, graph=True, refine_method="correction"):
-    """Lowest eigenpair of the pivot (the LanczosSolver::edgeState call of
-    DmrgSolver::iteration_two).  A mixed-precision solve is refined to the
-    FP64 tolerances by corrections ("correction") or an FP64 restart
-    ("restart").  Returns (eigenvector, result dict)."""
+def _lanczos_opts(H, precision, refine_fp64, max_iter, tol_eigval, tol_state, refine_tol_eigval, refine_tol_state,
+                  seed, graph, refine_method):
     o = _lib.LanczosOpts()
     H._lib.vmps_lanczos_opts_default(ctypes.byref(o))
     o.max_iter, o.tol_eigval, o.tol_state = max_iter, tol_eigval, tol_state
@@ -214,10 +210,72 @@ def lanczos(H, v0=None, precision="fp64", refine_fp64=True, max_iter=100, tol_ei
     o.seed = seed
     o.graph = 1 if graph else 0
     o.refine_method = REFINE[refine_method]
+    return o
+
+
+def lanczos_device(H, dv, precision="fp64", refine_fp64=True, max_iter=100, tol_eigval=1e-8, tol_state=1e-5,
+                   refine_tol_eigval=1e-12, refine_tol_state=1e-7, seed=1, graph=True, refine_method="correction"):
+    """lanczos() on a DeviceVector: dv holds the start vector and receives
+    the eigenvector.  Returns the result dict."""
+    o = _lanczos_opts(H, precision, refine_fp64, max_iter, tol_eigval, tol_state, refine_tol_eigval,
+                      refine_tol_state, seed, graph, refine_method)
+    r = _lib.LanczosResult()
+    _lib.check(H._lib.vmps_lanczos_device(H._plan, dv.ptr, ctypes.byref(o), ctypes.byref(r)))
+    return r.as_dict()
+
+
+def contract_aa2(H, dv, blocks1, blocks2, out_off, i1, i2, factor):
+    """contract_AA2 on the device into the DeviceVector dv
+    (vmps_contract_aa2_device): dv = sum over terms t of factor[t] *
+    blocks1[i1[t]] . blocks2[i2[t]] at out_off[t], zero elsewhere."""
+    from . import pack
+    p1, keep1 = pack._block_ptrs(blocks1)
+    p2, keep2 = pack._block_ptrs(blocks2)
+    r1 = np.array([b.shape[0] for b in blocks1] or [0], dtype=np.int64)
+    c1 = np.array([b.shape[1] for b in blocks1] or [0], dtype=np.int64)
+    r2 = np.array([b.shape[0] for b in blocks2] or [0], dtype=np.int64)
+    c2 = np.array([b.shape[1] for b in blocks2] or [0], dtype=np.int64)
+    oo = np.array(out_off or [0], dtype=np.int64)
+    t1 = np.array(i1 or [0], dtype=np.int32)
+    t2 = np.array(i2 or [0], dtype=np.int32)
+    f = np.array(factor or [0.0], dtype=np.float64)
+    PP = ctypes.POINTER(ctypes.c_void_p)
+    P64 = ctypes.POINTER(ctypes.c_int64)
+    P32 = ctypes.POINTER(ctypes.c_int)
+    _lib.check(H._lib.vmps_contract_aa2_device(
+        H._plan, len(blocks1), r1.ctypes.data_as(P64), c1.ctypes.data_as(P64), p1.ctypes.data_as(PP),
+        len(blocks2), r2.ctypes.data_as(P64), c2.ctypes.data_as(P64), p2.ctypes.data_as(PP), len(out_off),
+        oo.ctypes.data_as(P64), t1.ctypes.data_as(P32), t2.ctypes.data_as(P32), _ptr(f), dv.ptr))
+    del keep1, keep2
+
+
+def vec_norm(H, dv):
+    """norm(V) of a DeviceVector (coeff_dot weights)."""
+    r = ctypes.c_double()
+    _lib.check(H._lib.vmps_vec_norm(H._plan, dv.ptr, ctypes.byref(r)))
+    return r.value
+
+
+def vec_fill_random(H, dv, seed):
+    _lib.check(H._lib.vmps_vec_fill_random(H._plan, dv.ptr, seed))
+
+
+def lanczos(H, v0=None, precision="fp64", refine_fp64=True, max_iter=100, tol_eigval=1e-8, tol_state=1e-5,
+            refine_tol_eigval=1e-12, refine_tol_state=1e-7, seed=1, graph=True, refine_method="correction",
+            overwrite_v0=False):
+    """Lowest eigenpair of the pivot (the LanczosSolver::edgeState call of
+    DmrgSolver::iteration_two).  A mixed-precision solve is refined to the
+    FP64 tolerances by corrections ("correction") or an FP64 restart
+    ("restart").  Returns (eigenvector, result dict); with `overwrite_v0` a
+    contiguous float64 v0 receives the eigenvector in place (no copy)."""
+    o = _lanczos_opts(H, precision, refine_fp64, max_iter, tol_eigval, tol_state, refine_tol_eigval,
+                      refine_tol_state, seed, graph, refine_method)
     r = _lib.LanczosResult()
     if v0 is None:  # GaussianRandomVector::fill: uniform(-1, 1) entries
         v0 = np.random.default_rng(seed).uniform(-1.0, 1.0, H.vsize)
-    v = np.ascontiguousarray(v0, dtype=np.float64).copy()
+    v = np.ascontiguousarray(v0, dtype=np.float64)
+    if v is v0 and not overwrite_v0:
+        v = v.copy()
     _lib.check(H._lib.vmps_lanczos(H._plan, _ptr(v), ctypes.byref(o), ctypes.byref(r)))
     return v, r.as_dict()
 
