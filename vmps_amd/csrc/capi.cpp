@@ -130,6 +130,31 @@ void add_env_sector(vmps_pivot2* p, int which, const int* qin, const int* qout, 
 }
 }  // namespace
 
+namespace {
+// rocBLAS handle per device for the two-site vector build, made on first use
+// and kept until the process ends; calls on one device are serialised.
+struct BlasSlot {
+  rocblas_handle h = nullptr;
+  std::mutex m;
+};
+std::mutex g_blas_mutex;
+std::map<int, BlasSlot*> g_blas;
+
+BlasSlot& blas_for_device(int dev) {
+  std::lock_guard<std::mutex> lock(g_blas_mutex);
+  BlasSlot*& s = g_blas[dev];
+  if (!s) {
+    auto* slot = new BlasSlot();
+    if (rocblas_create_handle(&slot->h) != rocblas_status_success) {
+      delete slot;
+      throw VmpsError(VMPS_ERR_HIP, "rocblas_create_handle failed");
+    }
+    s = slot;
+  }
+  return *s;
+}
+}  // namespace
+
 extern "C" {
 
 int vmps_abi_version(void) { return VMPS_ABI_VERSION; }
@@ -834,31 +859,6 @@ vmps_status vmps_vec_fill_random(vmps_plan* plan, double* d_v, uint64_t seed) {
   }
   VMPS_API_END
 }
-
-namespace {
-// rocBLAS handle per device for the two-site vector build, made on first use
-// and kept until the process ends; calls on one device are serialised.
-struct BlasSlot {
-  rocblas_handle h = nullptr;
-  std::mutex m;
-};
-std::mutex g_blas_mutex;
-std::map<int, BlasSlot*> g_blas;
-
-BlasSlot& blas_for_device(int dev) {
-  std::lock_guard<std::mutex> lock(g_blas_mutex);
-  BlasSlot*& s = g_blas[dev];
-  if (!s) {
-    auto* slot = new BlasSlot();
-    if (rocblas_create_handle(&slot->h) != rocblas_status_success) {
-      delete slot;
-      throw VmpsError(VMPS_ERR_HIP, "rocblas_create_handle failed");
-    }
-    s = slot;
-  }
-  return *s;
-}
-}  // namespace
 
 vmps_status vmps_contract_aa2_device(vmps_plan* plan, int n1, const int64_t* rows1, const int64_t* cols1,
                                      const double* const* A1, int n2, const int64_t* rows2, const int64_t* cols2,

@@ -429,7 +429,14 @@ LanczosResult lanczos_solve(DevPlan& D, double* d_v, const LanczosOpts& o, vmps_
   D.prec = prec;
   D.ensure_precision_buffers();
   K.ensure(D, kmax);
-  hipGraphExec_t iteration = o.graph ? K.ensure_graph(D, 0) : nullptr;
+  // A captured batch runs kIterBatch iterations, so a solve runs (kIterBatch
+  // - 1) / 2 HxV past convergence on average; where one HxV costs more than
+  // the launches and the per-iteration readback the graph saves (~0.1 ms per
+  // iteration against ~1.5 ms for an HxV of 5e10 FLOP), the iterations are
+  // launched one at a time instead.
+  constexpr double kGraphMaxFlops = 5e10;
+  hipGraphExec_t iteration =
+      o.graph && D.H.stats.flops_alg < kGraphMaxFlops ? K.ensure_graph(D, 0) : nullptr;
 
   // q0 = v / ||v|| into the basis and into the HxV input; k = 0.  A zero
   // (or non-finite) start vector has no direction to start from.
