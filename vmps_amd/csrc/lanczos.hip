@@ -99,13 +99,20 @@ __global__ __launch_bounds__(256) void k_multidot_rows(const int* __restrict__ k
   const int nr = min(4, k + 1 - i0);
   const int b = blockIdx.x;
   const int64_t e0 = (int64_t)b * chunk, e1 = min(n, e0 + chunk);
+  // rows past k read row k again (their sums are not stored), so that the
+  // loop has no branch and the loads of several elements are in flight together
+  const double* __restrict__ q0 = Q + (int64_t)i0 * n;
+  const double* __restrict__ q1 = Q + (int64_t)min(i0 + 1, k) * n;
+  const double* __restrict__ q2 = Q + (int64_t)min(i0 + 2, k) * n;
+  const double* __restrict__ q3 = Q + (int64_t)min(i0 + 3, k) * n;
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll 4
   for (int64_t e = e0 + threadIdx.x; e < e1; e += 256) {
     const double we = w[e] * ew[e];
-#pragma unroll
-    for (int r = 0; r < 4; ++r)
-      if (r < nr) acc[r] += Q[(int64_t)(i0 + r) * n + e] * we;
+    acc[0] += q0[e] * we;
+    acc[1] += q1[e] * we;
+    acc[2] += q2[e] * we;
+    acc[3] += q3[e] * we;
   }
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
@@ -136,6 +143,24 @@ __global__ __launch_bounds__(256) void k_multidot_rows_final(const int* __restri
   if (threadIdx.x == 0) c[i] = red[0];
 }
 
+// sum_{i<=k} c[i] Q_i[e], added in order of i, with the basis loads issued
+// eight rows at a time (eight memory round trips in flight per thread
+// instead of the compiler's four)
+__device__ inline double combination_at(const double* __restrict__ Q, int64_t n, const double* c, int k,
+                                        int64_t e) {
+  double s = 0.0;
+  int i = 0;
+  for (; i + 8 <= k + 1; i += 8) {
+    double q[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) q[u] = Q[(int64_t)(i + u) * n + e];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) s += c[i + u] * q[u];
+  }
+  for (; i <= k; ++i) s += c[i] * Q[(int64_t)i * n + e];
+  return s;
+}
+
 // w -= sum_{i<=k} c[i] Q_i; with norm_part, also the workgroup's share of the
 // weighted ||w||^2 of the result (summed by k_normalize_next)
 __global__ __launch_bounds__(256) void k_sub_combination_dk(const double* __restrict__ Q, int64_t n,
@@ -148,8 +173,7 @@ __global__ __launch_bounds__(256) void k_sub_combination_dk(const double* __rest
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   double nrm = 0.0;
   for (; e < n; e += stride) {
-    double s = 0.0;
-    for (int i = 0; i <= k; ++i) s += c[i] * Q[(int64_t)i * n + e];
+    const double s = combination_at(Q, n, c, k, e);
     const double y = w[e] - s;
     w[e] = y;
     nrm += ew[e] * y * y;
@@ -177,10 +201,19 @@ __global__ __launch_bounds__(256) void k_sub_combination_fused(const double* __r
   // four threads per row, each summing every fourth partial, then the four
   // sums combined by shuffles within the quad: a fixed order in every
   // workgroup, and a quarter of the dependent load chain per thread
+  // (all of a thread's partials are loaded before the first add, one memory
+  // round trip instead of one per partial; absent ones add as 0.0)
   const int quad = threadIdx.x & 3;
   for (int i = threadIdx.x >> 2; i <= k; i += 64) {  // quad-uniform
+    double p[kFuseMaxGroups / 4];
+#pragma unroll
+    for (int j = 0; j < kFuseMaxGroups / 4; ++j) {
+      const int b = quad + 4 * j;
+      p[j] = b < nb ? partial[(int64_t)i * nb + b] : 0.0;
+    }
     double acc = 0.0;
-    for (int b = quad; b < nb; b += 4) acc += partial[(int64_t)i * nb + b];
+#pragma unroll
+    for (int j = 0; j < kFuseMaxGroups / 4; ++j) acc += p[j];
     acc += __shfl_xor(acc, 1);
     acc += __shfl_xor(acc, 2);
     if (quad == 0) {
@@ -193,8 +226,7 @@ __global__ __launch_bounds__(256) void k_sub_combination_fused(const double* __r
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   double nrm = 0.0;
   for (; e < n; e += stride) {
-    double s = 0.0;
-    for (int i = 0; i <= k; ++i) s += cs[i] * Q[(int64_t)i * n + e];
+    const double s = combination_at(Q, n, cs, k, e);
     const double y = w[e] - s;
     w[e] = y;
     nrm += ew[e] * y * y;
@@ -204,10 +236,19 @@ __global__ __launch_bounds__(256) void k_sub_combination_fused(const double* __r
   if (threadIdx.x == 0) norm_part[blockIdx.x] = sum;
 }
 
-// sum of part[0 .. np-1] in one workgroup, fixed order; every thread returns it
+// sum of part[0 .. np-1] (np <= 2048, the largest vgrid) in one workgroup,
+// fixed order; every thread returns it.  A thread's eight loads are issued
+// together; absent ones add as 0.0.
 __device__ inline double block_sum(const double* __restrict__ part, int np) {
+  double p[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const int b = (int)threadIdx.x + 256 * j;
+    p[j] = b < np ? part[b] : 0.0;
+  }
   double acc = 0.0;
-  for (int b = threadIdx.x; b < np; b += 256) acc += part[b];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) acc += p[j];
   return wg_sum(acc);
 }
 
