@@ -70,6 +70,8 @@ struct Krylov {
   DevBuf<double> ew;      // coeff_dot weight of every vector element
   DevBuf<double> gpart;   // partial sums of the basis dot products, (kmax + 1) x ngrp
   DevBuf<double> npart;   // partial sums of ||w||^2
+  DevBuf<double> om;      // partial re-orthogonalisation: rows k-1, k, k+1 of the omega estimate
+  DevBuf<int> gate;       // gate[k]: iteration k ran the full Gram-Schmidt pass
   double* host3 = nullptr;  // pinned readback of out3: two slots of 4 * kIterBatch doubles
   hipEvent_t done[2] = {nullptr, nullptr};  // a slot's readback has landed
   int ngrp = 0;           // element chunks of the basis dot products
@@ -85,6 +87,9 @@ struct Krylov {
   int graph_gen[3] = {-1, -1, -1};
   void ensure(DevPlan& D, int kmax);
   void enqueue_iteration(DevPlan& D, hipStream_t s, int kofs);  // iteration *kdev + kofs; *kdev is not advanced
+  // FP64 solves on vectors of the fused Gram-Schmidt path re-orthogonalise
+  // partially (local passes, the whole basis when the omega estimate asks)
+  bool partial_reortho(const DevPlan& D) const;
   // an executable instance (copy 0 or 1) of the graph of D.prec (kIterBatch iterations)
   hipGraphExec_t ensure_graph(DevPlan& D, int copy);
   void release_graph();

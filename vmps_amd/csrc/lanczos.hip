@@ -87,13 +87,15 @@ __device__ inline double wg_sum(double x) {
 // (ew: the coeff_dot weight of e's block), for the rows i = 4 y .. 4 y + 3
 // that are <= k; workgroup (b, y).  Groups past k exit at once, so the grid is
 // sized for the largest k and stays the same in every iteration.
+// With `local`, only the rows k-1 and k (one row group): the local
+// orthogonalisation of the partially re-orthogonalised iteration.
 __global__ __launch_bounds__(256) void k_multidot_rows(const int* __restrict__ kdev, int kofs,
                                                        const double* __restrict__ Q, int64_t n,
                                                        const double* __restrict__ w,
                                                        const double* __restrict__ ew, int64_t chunk,
-                                                       double* __restrict__ partial, int nb) {
+                                                       double* __restrict__ partial, int nb, int local) {
   const int k = *kdev + kofs;
-  const int i0 = 4 * blockIdx.y;
+  const int i0 = (local ? max(0, k - 1) : 0) + 4 * blockIdx.y;
   if (i0 > k) return;  // workgroup-uniform
   __shared__ double ws[4][4];  // [wave][row]
   const int nr = min(4, k + 1 - i0);
@@ -143,13 +145,13 @@ __global__ __launch_bounds__(256) void k_multidot_rows_final(const int* __restri
   if (threadIdx.x == 0) c[i] = red[0];
 }
 
-// sum_{i<=k} c[i] Q_i[e], added in order of i, with the basis loads issued
+// sum_{r0<=i<=k} c[i] Q_i[e], added in order of i, with the basis loads issued
 // eight rows at a time (eight memory round trips in flight per thread
 // instead of the compiler's four)
 __device__ inline double combination_at(const double* __restrict__ Q, int64_t n, const double* c, int k,
-                                        int64_t e) {
+                                        int64_t e, int r0 = 0) {
   double s = 0.0;
-  int i = 0;
+  int i = r0;
   for (; i + 8 <= k + 1; i += 8) {
     double q[8];
 #pragma unroll
@@ -190,21 +192,27 @@ __global__ __launch_bounds__(256) void k_sub_combination_dk(const double* __rest
 // host scalars.  Then w -= sum_i c[i] Q_i as in k_sub_combination_dk.
 constexpr int kFuseMaxGroups = 64;
 constexpr int kFuseMaxRows = 512;
+// With `local`, the rows k-1 and k only; with `gate`, the kernel does
+// nothing unless gate[k] is set (the partially re-orthogonalised iteration's
+// full pass).
 __global__ __launch_bounds__(256) void k_sub_combination_fused(const double* __restrict__ Q, int64_t n,
                                                                const double* __restrict__ partial, int nb,
                                                                const int* __restrict__ kdev, int kofs,
                                                                double* __restrict__ c, double* __restrict__ w,
                                                                const double* __restrict__ ew,
-                                                               double* __restrict__ norm_part) {
+                                                               double* __restrict__ norm_part, int local,
+                                                               const int* __restrict__ gate) {
   __shared__ double cs[kFuseMaxRows];
   const int k = *kdev + kofs;
+  if (gate && !gate[k]) return;  // workgroup-uniform
+  const int r0 = local ? max(0, k - 1) : 0;
   // four threads per row, each summing every fourth partial, then the four
   // sums combined by shuffles within the quad: a fixed order in every
   // workgroup, and a quarter of the dependent load chain per thread
   // (all of a thread's partials are loaded before the first add, one memory
   // round trip instead of one per partial; absent ones add as 0.0)
   const int quad = threadIdx.x & 3;
-  for (int i = threadIdx.x >> 2; i <= k; i += 64) {  // quad-uniform
+  for (int i = r0 + (threadIdx.x >> 2); i <= k; i += 64) {  // quad-uniform
     double p[kFuseMaxGroups / 4];
 #pragma unroll
     for (int j = 0; j < kFuseMaxGroups / 4; ++j) {
@@ -226,7 +234,7 @@ __global__ __launch_bounds__(256) void k_sub_combination_fused(const double* __r
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   double nrm = 0.0;
   for (; e < n; e += stride) {
-    const double s = combination_at(Q, n, cs, k, e);
+    const double s = combination_at(Q, n, cs, k, e, r0);
     const double y = w[e] - s;
     w[e] = y;
     nrm += ew[e] * y * y;
@@ -255,21 +263,24 @@ __device__ inline double block_sum(const double* __restrict__ part, int np) {
 // q_{k+1} = w / ||w|| into the basis and into the fixed HxV input vector v
 // (zero for a zero w: the iteration then ends on the breakdown test); every
 // workgroup sums the np norm partials itself (block_sum, fixed order).
-// Workgroup 0 also stores the three numbers the host needs after iteration k
-// (<q_k, H q_k> in two Gram-Schmidt parts c1[k] + c2[k], and ||w||^2) in
-// out[4k ..].
+// Workgroup 0 also stores the numbers the host needs after iteration k
+// (<q_k, H q_k> in its Gram-Schmidt parts c1[k] + c2[k] + c3[k], and ||w||^2)
+// in out[4k ..].
 __global__ __launch_bounds__(256) void k_normalize_next(const int* __restrict__ kdev, int kofs,
                                                         const double* __restrict__ npart, int np,
                                                         const double* __restrict__ w, double* __restrict__ Q,
                                                         int64_t n, double* __restrict__ v,
                                                         const double* __restrict__ c1,
-                                                        const double* __restrict__ c2, double* __restrict__ out) {
+                                                        const double* __restrict__ c2,
+                                                        const double* __restrict__ c3, const int* __restrict__ gate,
+                                                        double* __restrict__ out) {
   const int k = *kdev + kofs;
   const double nv = block_sum(npart, np);
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     out[4 * k] = c1[k];
     out[4 * k + 1] = c2[k];
     out[4 * k + 2] = nv;
+    out[4 * k + 3] = (c3 && gate[k]) ? c3[k] : 0.0;  // the full pass of a partially re-orthogonalised iteration
   }
   const double f = nv > 0.0 ? 1.0 / sqrt(nv) : 0.0;
   double* __restrict__ q = Q + (int64_t)(k + 1) * n;
@@ -280,6 +291,111 @@ __global__ __launch_bounds__(256) void k_normalize_next(const int* __restrict__ 
     q[e] = y;
     v[e] = y;
   }
+}
+
+// ---- partial re-orthogonalisation (FP64 operators) --------------------------
+// Simon's recurrence (H. D. Simon, Math. Comp. 42 (1984) 115-142) estimates
+// omega_{k+1,j} = <q_{k+1}, q_j> from the Lanczos coefficients alone: with
+// H q_j = beta_{j-1} q_{j-1} + alpha_j q_j + beta_j q_{j+1} + f_j,
+//   beta_k omega_{k+1,j} = beta_j omega_{k,j+1} + (alpha_j - alpha_k) omega_{k,j}
+//                          + beta_{j-1} omega_{k,j-1} - beta_{k-1} omega_{k-1,j} + noise,
+// the noise bounding the rounding terms <f_j, q_k> - <q_j, f_k> by
+// eps1 (beta_k + beta_j) with the sign that makes |omega| larger.  While
+// every |omega_{k+1,j}| stays below sqrt(eps) the basis is semi-orthogonal,
+// which keeps the Ritz values accurate to working precision; an iteration
+// whose estimate crosses it orthogonalises w against the whole basis (and the
+// estimate is reset to eps1).  Rows k-1, k, k+1 of omega live in three slots.
+constexpr double kProThreshold = 1.4901161193847656e-08;  // sqrt(2^-52)
+
+__device__ inline double* omega_row(double* om, int row, int ld) { return om + (int64_t)(row % 3) * ld; }
+
+// Every workgroup: beta_k from the norm partials of the local passes, the new
+// omega row and the decision (the same in every workgroup); workgroup (0,0)
+// stores the row (reset to eps1 when the full pass runs) and gate[k].  With
+// the full pass due, workgroup (b, y) then forms the basis dot partials of
+// rows 4y .. 4y+3 <= k as k_multidot_rows does.
+__global__ __launch_bounds__(256) void k_pro_decide_dots(const int* __restrict__ kdev, int kofs,
+                                                         const double* __restrict__ Q, int64_t n,
+                                                         const double* __restrict__ w,
+                                                         const double* __restrict__ ew, int64_t chunk,
+                                                         double* __restrict__ partial, int nb,
+                                                         const double* __restrict__ npart, int np,
+                                                         const double* __restrict__ c1,
+                                                         const double* __restrict__ c2,
+                                                         const double* __restrict__ out, double* __restrict__ om,
+                                                         int ld, double eps1, int* __restrict__ gate) {
+  __shared__ double red[4];
+  const int k = *kdev + kofs;
+  const double bk = sqrt(block_sum(npart, np));
+  const double ak = c1[k] + c2[k];
+  const double bkm1 = k > 0 ? sqrt(out[4 * (k - 1) + 2]) : 0.0;
+  const double* __restrict__ wk = omega_row(om, k, ld);
+  const double* __restrict__ wkm1 = omega_row(om, k + 2, ld);  // row k-1
+  double* __restrict__ wnew = omega_row(om, k + 1, ld);
+  // omega_{k+1,j}, j < k: thread t takes j = t, t + 256 (k < kFuseMaxRows)
+  double mx = 0.0, val[2] = {0.0, 0.0};
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int j = (int)threadIdx.x + 256 * h;
+    if (j >= k || !(bk > 0.0)) continue;
+    const double aj = out[4 * j] + out[4 * j + 1] + out[4 * j + 3];
+    const double bj = sqrt(out[4 * j + 2]);
+    const double bjm1 = j > 0 ? sqrt(out[4 * (j - 1) + 2]) : 0.0;
+    double num = bj * wk[j + 1] + (aj - ak) * wk[j] + (j > 0 ? bjm1 * wk[j - 1] : 0.0) - bkm1 * wkm1[j];
+    num += (num >= 0.0 ? 1.0 : -1.0) * eps1 * (bk + bj);
+    val[h] = num / bk;
+    mx = fmax(mx, fabs(val[h]));
+  }
+  for (int m = 32; m > 0; m >>= 1) mx = fmax(mx, __shfl_xor(mx, m));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = mx;
+  __syncthreads();
+  const bool need = fmax(fmax(red[0], red[1]), fmax(red[2], red[3])) > kProThreshold;
+  if (blockIdx.x == 0 && blockIdx.y == 0) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int j = (int)threadIdx.x + 256 * h;
+      if (j < k) wnew[j] = need ? eps1 : val[h];
+    }
+    if (threadIdx.x == 0) {
+      wnew[k] = eps1;  // locally orthogonal
+      wnew[k + 1] = 1.0;
+      gate[k] = need ? 1 : 0;
+    }
+  }
+  if (!need) return;  // workgroup-uniform
+  const int i0 = 4 * blockIdx.y;
+  if (i0 > k) return;
+  __shared__ double ws[4][4];
+  const int nr = min(4, k + 1 - i0);
+  const int b = blockIdx.x;
+  const int64_t e0 = (int64_t)b * chunk, e1 = min(n, e0 + chunk);
+  const double* __restrict__ q0 = Q + (int64_t)i0 * n;
+  const double* __restrict__ q1 = Q + (int64_t)min(i0 + 1, k) * n;
+  const double* __restrict__ q2 = Q + (int64_t)min(i0 + 2, k) * n;
+  const double* __restrict__ q3 = Q + (int64_t)min(i0 + 3, k) * n;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
+  for (int64_t e = e0 + threadIdx.x; e < e1; e += 256) {
+    const double we = w[e] * ew[e];
+    acc[0] += q0[e] * we;
+    acc[1] += q1[e] * we;
+    acc[2] += q2[e] * we;
+    acc[3] += q3[e] * we;
+  }
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    double x = acc[r];
+    for (int m = 32; m > 0; m >>= 1) x += __shfl_xor(x, m);
+    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6][r] = x;
+  }
+  __syncthreads();
+  const int t = threadIdx.x;
+  if (t < nr) partial[(int64_t)(i0 + t) * nb + b] = (ws[0][t] + ws[1][t]) + (ws[2][t] + ws[3][t]);
+}
+
+// omega_{0,0} = 1 before the first iteration of a solve
+__global__ void k_pro_init(double* __restrict__ om) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) om[0] = 1.0;
 }
 
 // *p += d, after every kernel of a batch has read *p
@@ -389,17 +505,24 @@ void dev_fill_uniform(double* v, int64_t n, uint64_t seed, hipStream_t s) {
 // is captured once per plan configuration as a HIP graph and replayed each
 // iteration; without it the same kernels are launched one by one.  Either way
 // the host waits once per iteration, for three numbers.
+bool Krylov::partial_reortho(const DevPlan& D) const {
+  return ngrp <= kFuseMaxGroups && cap < kFuseMaxRows && D.prec == VMPS_PREC_FP64;
+}
+
 void Krylov::enqueue_iteration(DevPlan& D, hipStream_t s, int kofs) {
   const int64_t n = D.H.vsize;
   const unsigned gy = (unsigned)((cap + 3) / 4);
   const unsigned gs = vgrid(n);
   const bool fused = ngrp <= kFuseMaxGroups && cap < kFuseMaxRows;
-  auto pass = [&](double* c_out, double* norm_part) {
-    hipLaunchKernelGGL(k_multidot_rows, dim3(ngrp, gy), dim3(256), 0, s, kdev.p, kofs, Q.p, n, w.p, ew.p, chunk,
-                       gpart.p, ngrp);
+  double* c1 = c.p;
+  double* c2 = c.p + cap;
+  double* c3 = c.p + 2 * cap;
+  auto pass = [&](double* c_out, double* norm_part, int local) {
+    hipLaunchKernelGGL(k_multidot_rows, dim3(ngrp, local ? 1u : gy), dim3(256), 0, s, kdev.p, kofs, Q.p, n, w.p,
+                       ew.p, chunk, gpart.p, ngrp, local);
     if (fused) {
       hipLaunchKernelGGL(k_sub_combination_fused, dim3(gs), dim3(256), 0, s, Q.p, n, gpart.p, ngrp, kdev.p, kofs,
-                         c_out, w.p, ew.p, norm_part);
+                         c_out, w.p, ew.p, norm_part, local, (const int*)nullptr);
     } else {
       hipLaunchKernelGGL(k_multidot_rows_final, dim3(cap), dim3(256), 0, s, kdev.p, kofs, gpart.p, ngrp, c_out);
       hipLaunchKernelGGL(k_sub_combination_dk, dim3(gs), dim3(256), 0, s, Q.p, n, c_out, kdev.p, kofs, w.p, ew.p,
@@ -407,10 +530,24 @@ void Krylov::enqueue_iteration(DevPlan& D, hipStream_t s, int kofs) {
     }
   };
   D.hxv(v.p, w.p, s);
-  pass(c.p, nullptr);
-  pass(c.p + cap, npart.p);
-  hipLaunchKernelGGL(k_normalize_next, dim3(gs), dim3(256), 0, s, kdev.p, kofs, npart.p, (int)gs, w.p, Q.p, n, v.p,
-                     c.p, c.p + cap, out3.p);
+  if (partial_reortho(D)) {
+    // FP64 operator: classical Gram-Schmidt twice against q_{k-1} and q_k,
+    // then the whole basis only where the omega estimate asks for it
+    pass(c1, nullptr, 1);
+    pass(c2, npart.p, 1);
+    const double eps1 = 1.1102230246251565e-16 * std::sqrt((double)n);
+    hipLaunchKernelGGL(k_pro_decide_dots, dim3(ngrp, gy), dim3(256), 0, s, kdev.p, kofs, Q.p, n, w.p, ew.p, chunk,
+                       gpart.p, ngrp, npart.p, (int)gs, c1, c2, out3.p, om.p, cap + 2, eps1, gate.p);
+    hipLaunchKernelGGL(k_sub_combination_fused, dim3(gs), dim3(256), 0, s, Q.p, n, gpart.p, ngrp, kdev.p, kofs, c3,
+                       w.p, ew.p, npart.p, 0, (const int*)gate.p);
+    hipLaunchKernelGGL(k_normalize_next, dim3(gs), dim3(256), 0, s, kdev.p, kofs, npart.p, (int)gs, w.p, Q.p, n,
+                       v.p, c1, c2, c3, (const int*)gate.p, out3.p);
+  } else {
+    pass(c1, nullptr, 0);
+    pass(c2, npart.p, 0);
+    hipLaunchKernelGGL(k_normalize_next, dim3(gs), dim3(256), 0, s, kdev.p, kofs, npart.p, (int)gs, w.p, Q.p, n,
+                       v.p, c1, c2, (const double*)nullptr, (const int*)nullptr, out3.p);
+  }
   VMPS_HIP_CHECK(hipGetLastError());
 }
 
@@ -492,6 +629,7 @@ LanczosResult lanczos_solve(DevPlan& D, double* d_v, const LanczosOpts& o, vmps_
   hipLaunchKernelGGL(k_normalize_into, dim3(vgrid(n)), dim3(256), 0, s, K.scal.p, d_v, K.Q.p, n);
   hipLaunchKernelGGL(k_copy, dim3(vgrid(n)), dim3(256), 0, s, K.Q.p, K.v.p, n);
   hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, s, K.kdev.p, 0);
+  if (K.partial_reortho(D)) hipLaunchKernelGGL(k_pro_init, dim3(1), dim3(64), 0, s, K.om.p);
   VMPS_HIP_CHECK(hipGetLastError());
 
   std::vector<double> alpha, beta, y;
@@ -548,7 +686,7 @@ LanczosResult lanczos_solve(DevPlan& D, double* d_v, const LanczosOpts& o, vmps_
     }
     res.hxv_calls += nb;
     for (int j = 0; j < nb && k < kmax; ++j) {
-      const double a_k = hs[4 * j] + hs[4 * j + 1];
+      const double a_k = hs[4 * j] + hs[4 * j + 1] + hs[4 * j + 3];
       const double b_k = std::sqrt(std::max(0.0, hs[4 * j + 2]));
       alpha.push_back(a_k);
       beta.push_back(b_k);
@@ -776,7 +914,9 @@ void Krylov::ensure(DevPlan& D, int kmax) {
   basis_acquire(Q, (size_t)(need + 1) * n, device);
   w.alloc(n);
   v.alloc(n);
-  c.alloc(2 * (size_t)need + 2);
+  c.alloc(3 * (size_t)need + 3);
+  om.alloc(3 * ((size_t)need + 2));
+  gate.alloc((size_t)need + 1);
   scal.alloc(8);
   out3.alloc(4 * (size_t)need);
   kdev.alloc(1);
