@@ -197,18 +197,29 @@ def cpu_baseline_pivot(pd, falg, vin, threads, single_seconds=4.0):
     }, out
 
 
-def lanczos_iteration_ms(H, vsize, iters):
-    """Wall time of one Lanczos iteration (HxV, two Gram-Schmidt passes over the
-    whole basis, normalisation; replayed from a HIP graph, the scalars read by
-    the host every few iterations) averaged over `iters` iterations with the
+def lanczos_iteration_ms(workload, seed, iters):
+    """Wall time of one FP64 Lanczos iteration (HxV, re-orthogonalisation,
+    normalisation; replayed from a HIP graph, the scalars read by the host
+    every few iterations) averaged over `iters` iterations with the
     tolerances at 0, from the same start vector as a first untimed solve that
-    captures the graph."""
-    from vmps_amd.pivot import lanczos
-    v0 = np.random.default_rng(1).uniform(-1, 1, vsize)
-    lanczos(H, v0=v0, max_iter=8, tol_eigval=0.0, tol_state=0.0)
-    t0 = time.perf_counter()
-    _, r = lanczos(H, v0=v0, max_iter=iters, tol_eigval=0.0, tol_state=0.0)
-    return (time.perf_counter() - t0) / r["iterations"] * 1e3
+    captures the graph.  It runs on the Hermitian form of the workload's
+    pivot (environments symmetrised through the MPO, as an effective
+    Hamiltonian is): the partial re-orthogonalisation reads the loss of
+    orthogonality off the Lanczos coefficients, which holds for a symmetric
+    operator only."""
+    from vmps_amd import synth
+    from vmps_amd.pivot import PivotMatrix2, lanczos
+    pd = synth.make_workload(workload, seed=seed, hermitian=True)
+    H = PivotMatrix2(pd, precision="fp64")
+    try:
+        H.autotune(iters=10)
+        v0 = np.random.default_rng(1).uniform(-1, 1, pd.vsize)
+        lanczos(H, v0=v0, max_iter=8, tol_eigval=0.0, tol_state=0.0)
+        t0 = time.perf_counter()
+        _, r = lanczos(H, v0=v0, max_iter=iters, tol_eigval=0.0, tol_state=0.0)
+        return (time.perf_counter() - t0) / r["iterations"] * 1e3
+    finally:
+        H.close()
 
 
 def _c2_hxv_lines(args):
@@ -225,7 +236,7 @@ def _c2_hxv_lines(args):
         ms = H.time_hxv(warmup=20, iters=400)
         out[prec] = {"ms_per_hxv": ms, "gflops": st["flops_alg"] / (ms * 1e-3) / 1e9}
         if prec == "fp64":
-            out["lanczos_ms_per_iteration"] = lanczos_iteration_ms(H, pd.vsize, 60)
+            out["lanczos_ms_per_iteration"] = lanczos_iteration_ms("hubbard_u1xu1_L48_M512", args.seed, 60)
             out["lanczos_iterations"] = 60
         H.close()
     return out
@@ -548,7 +559,7 @@ def hxv_bench(args):
     out_gpu = vout.cpu().numpy()
     lanczos_ms = None
     if args.lanczos_iters and not batched:
-        lanczos_ms = lanczos_iteration_ms(H, pds[0].vsize, args.lanczos_iters)
+        lanczos_ms = lanczos_iteration_ms(args.workload, args.seed, args.lanczos_iters)
 
     # the same workload in the mixed modes (split-BF16 x3 and FP32 MFMA with
     # FP32 accumulation), each autotuned, against the FP64 result

@@ -302,6 +302,20 @@ vmps_status vmps_block_svd(int nblocks, const int64_t* rows, const int64_t* cols
  * VMPS_ERR_NOCONV if the eigensolver fails. */
 vmps_status vmps_block_eigh_split(int nblocks, const int64_t* rows, const int64_t* cols, const double* A, int side,
                                   double* lambda, double* W, double* C);
+/* The same factorisation with the sector matrices built on the device from
+ * the blocks of a device vector d_v (vsize doubles; the two-site
+ * wavefunction, e.g. the eigenvector vmps_lanczos_device left in a plan
+ * vector): the recoupling of split_AA2 (tensors/DmrgContractions.h:2460-2630)
+ * as a gather.  Destination d is the r x c block at (dst_row[d], dst_col[d])
+ * of sector dst_block[d]; it receives the sum, in order, of
+ * part_factor[p] * v[part_src[p] .. part_src[p] + r c) (column-major r x c)
+ * over p in [part_begin[d], part_begin[d+1]); entries no destination covers
+ * are zero.  Outputs as vmps_block_eigh_split. */
+vmps_status vmps_block_eigh_split_gather(const double* d_v, int64_t vsize, int nblocks, const int64_t* rows,
+                                         const int64_t* cols, int ndst, const int* dst_block, const int64_t* dst_row,
+                                         const int64_t* dst_col, const int64_t* dst_r, const int64_t* dst_c,
+                                         const int64_t* part_begin, const int64_t* part_src,
+                                         const double* part_factor, int side, double* lambda, double* W, double* C);
 /* vmps_block_svd that allocates no device memory: no counterpart in the
  * reference; for callers that cannot allocate device memory (tight serving
  * loops, harnesses that forbid hipMalloc).  Arguments,

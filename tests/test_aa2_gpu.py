@@ -103,3 +103,52 @@ def test_dmrg_step_device_psi_matches_host_psi(state):
     e_host = dmrg.dmrg_step(chain.copy(), HostPsi(be), p, "right")[0]
     print("step energy device psi %.14f host psi %.14f" % (e_dev, e_host), flush=True)
     assert abs(e_dev - e_host) <= 1e-10 * abs(e_host)
+
+
+def test_device_gather_split_matches_host_split(state):
+    """The split of the eigenvector left on the device (sector matrices built
+    by vmps_block_eigh_split_gather) against the split of its host copy: the
+    same kept dimensions and truncation weight, and the same two-site
+    wavefunction rebuilt from the new site tensors."""
+    chain, p = state
+    pd = chain.pivot(p)
+    be = dmrg.GpuBackend(tol_eigval=1e-12, tol_state=1e-8, device_split_min=1)
+    E, dev, info = be.solve_two_site(pd, chain.A[p], chain.A[p + 1], chain.qloc, seed=1, keep_device=True)
+    try:
+        v = dev.download()
+        M = max(8, sum(chain.bonds[p + 1].values()) // 2)  # truncating
+        host = dmrg.split_psi(pd, v, chain.qloc, M, eigh=be.eigh)
+        devs = dmrg.split_psi(pd, dev, chain.qloc, M, eigh=be.eigh, eigh_gather=be.eigh_gather)
+    finally:
+        dev.close()
+    assert host[4] == devs[4]
+    assert abs(host[2] - devs[2]) <= 1e-12 * max(1.0, abs(host[2])) + 1e-15
+    psi_h = dmrg.two_site_psi(pd, host[0], host[1], chain.qloc)
+    psi_d = dmrg.two_site_psi(pd, devs[0], devs[1], chain.qloc)
+    err = np.linalg.norm(psi_d - psi_h) / np.linalg.norm(psi_h)
+    print("gather split: kept", sum(devs[4].values()), "truncation %.3e" % devs[2], "psi difference %.2e" % err, flush=True)
+    # the two density matrices differ by rounding (fused multiply-adds on the
+    # device), which turns the kept states near the cut by rounding / gap; the
+    # rebuilt wavefunctions agree to that level, and each misses the full one
+    # by exactly its truncation weight (weighted norm)
+    assert err <= 1e-9
+    w = np.concatenate([np.full(r * c, pd.sym.coeff_dot(qo)) for (_, _, qo, _, r, c) in pd.block_offsets()])
+    nrm = np.sum(w * v * v)
+    for psi, res in ((psi_h, host), (psi_d, devs)):
+        assert abs(np.sum(w * (psi - v) ** 2) - res[2]) <= 1e-12 * nrm
+
+
+def test_device_gather_split_rejects_bad_tables(state):
+    chain, p = state
+    pd = chain.pivot(p)
+    be = dmrg.GpuBackend(tol_eigval=1e-10, tol_state=1e-6)
+    E, dev, info = be.solve_two_site(pd, chain.A[p], chain.A[p + 1], chain.qloc, seed=1, keep_device=True)
+    try:
+        with pytest.raises(_lib.VmpsStatusError):  # block outside its sector
+            dmrg.block_eigh_gather_device(dev, [(2, 2)], [(0, 1, 0, 2, 2, [(0, 1.0)])], 0)
+        with pytest.raises(_lib.VmpsStatusError):  # part outside the vector
+            dmrg.block_eigh_gather_device(dev, [(2, 2)], [(0, 0, 0, 2, 2, [(pd.vsize - 1, 1.0)])], 0)
+        with pytest.raises(_lib.VmpsStatusError):  # sector index out of range
+            dmrg.block_eigh_gather_device(dev, [(2, 2)], [(1, 0, 0, 2, 2, [(0, 1.0)])], 0)
+    finally:
+        dev.close()

@@ -164,3 +164,59 @@ def test_zero_start_vector_refused():
     with pytest.raises(_lib.VmpsStatusError) as e:
         lanczos(H, v0=np.zeros(pd.vsize))
     assert "zero" in str(e.value)
+
+
+def test_partial_reorthogonalisation_keeps_full_accuracy():
+    """FP64 solves re-orthogonalise partially (omega recurrence): on the
+    Hermitian M=512 Hubbard pivot, 60 iterations from a random start give the
+    Ritz value of the fully re-orthogonalised basis (the lowest eigenvalue of
+    the tridiagonal matrix of a numpy run with full re-orthogonalisation on
+    the oracle's HxV) as the Rayleigh quotient of the returned vector, and
+    no fallback to full re-orthogonalisation (one HxV per iteration plus the
+    residual's)."""
+    pd = synth.make_workload("hubbard_u1xu1_L48_M512", hermitian=True)
+    H = PivotMatrix2(pd)
+    v0 = np.random.default_rng(1).uniform(-1, 1, pd.vsize)
+    _, r = lanczos(H, v0=v0, max_iter=60, tol_eigval=0.0, tol_state=0.0)
+    H.close()
+    # reference: the same 60 iterations with full re-orthogonalisation on the oracle's HxV
+    lib = oracle.load()
+    h = pivots.build_oracle(lib, pd)
+    w = np.concatenate([np.full(rr * c, pd.sym.coeff_dot(qo)) for (_, _, qo, _, rr, c) in pd.block_offsets()])
+    sw = np.sqrt(w)
+    try:
+        Q = [v0 * sw / np.linalg.norm(v0 * sw)]
+        al, be = [], []
+        for k in range(60):
+            x = pivots.oracle_hxv(lib, h, Q[-1] / sw, 16) * sw
+            for p in range(2):
+                c = np.array(Q) @ x
+                x = x - np.array(Q).T @ c
+                al.append(c[-1]) if p == 0 else al.__setitem__(-1, al[-1] + c[-1])
+            be.append(np.linalg.norm(x))
+            Q.append(x / be[-1])
+    finally:
+        lib.oracle_pivot2_destroy(h)
+    T = np.diag(al) + np.diag(be[:-1], 1) + np.diag(be[:-1], -1)
+    e_ref = np.linalg.eigvalsh(T)[0]
+    print("partial re-orthogonalisation: energy %.14f reference %.14f hxv %d" % (
+        r["energy"], e_ref, r["hxv_calls"]), flush=True)
+    assert abs(r["energy"] - e_ref) <= 1e-11 * abs(e_ref)
+    assert r["hxv_calls"] <= r["iterations"] + 4  # no second, fully re-orthogonalised solve
+
+
+def test_partial_reorthogonalisation_falls_back_on_a_nonsymmetric_operator():
+    """A random-environment pivot that is not symmetric breaks the omega
+    recurrence's assumption; the Ritz value then disagrees with the Rayleigh
+    quotient and the solve runs again with full re-orthogonalisation (about
+    twice the HxV); graph and eager runs agree bit for bit."""
+    pd = synth.make_workload("hubbard_u1xu1_L48_M512", hermitian=False)
+    H = PivotMatrix2(pd)
+    v0 = np.random.default_rng(1).uniform(-1, 1, pd.vsize)
+    _, r1 = lanczos(H, v0=v0, max_iter=60, tol_eigval=0.0, tol_state=0.0)
+    _, r2 = lanczos(H, v0=v0, max_iter=60, tol_eigval=0.0, tol_state=0.0, graph=False)
+    H.close()
+    print("non-symmetric pivot: hxv %d / %d iterations, energy %.10f / %.10f" % (
+        r1["hxv_calls"], r1["iterations"], r1["energy"], r2["energy"]), flush=True)
+    assert r1["hxv_calls"] >= 2 * r1["iterations"]
+    assert r1["energy"] == r2["energy"] and r1["residual"] == r2["residual"]

@@ -146,6 +146,8 @@ SIGNATURES = {
     "vmps_vec_norm": (c_int, [c_void, c_void, P_dbl]),
     "vmps_vec_axpby": (c_int, [c_void, c_dbl, c_void, c_dbl, c_void]),
     "vmps_vec_fill_random": (c_int, [c_void, c_void, ctypes.c_uint64]),
+    "vmps_block_eigh_split_gather": (c_int, [c_void, c_i64, c_int, P_i64, P_i64, c_int, P_int, P_i64, P_i64, P_i64,
+                                             P_i64, P_i64, P_i64, P_dbl, c_int, P_dbl, P_dbl, P_dbl]),
     "vmps_contract_aa2_device": (c_int, [c_void, c_int, P_i64, P_i64, ctypes.POINTER(c_void), c_int, P_i64, P_i64,
                                          ctypes.POINTER(c_void), c_int, P_i64, P_int, P_int, P_dbl, c_void]),
     "vmps_lanczos_opts_default": (None, [ctypes.POINTER(LanczosOpts)]),

@@ -90,6 +90,7 @@ struct Krylov {
   // FP64 solves on vectors of the fused Gram-Schmidt path re-orthogonalise
   // partially (local passes, the whole basis when the omega estimate asks)
   bool partial_reortho(const DevPlan& D) const;
+  bool full_reortho = false;  // set for a solve that found the partial scheme unreliable
   // an executable instance (copy 0 or 1) of the graph of D.prec (kIterBatch iterations)
   hipGraphExec_t ensure_graph(DevPlan& D, int copy);
   void release_graph();

@@ -506,7 +506,7 @@ void dev_fill_uniform(double* v, int64_t n, uint64_t seed, hipStream_t s) {
 // iteration; without it the same kernels are launched one by one.  Either way
 // the host waits once per iteration, for three numbers.
 bool Krylov::partial_reortho(const DevPlan& D) const {
-  return ngrp <= kFuseMaxGroups && cap < kFuseMaxRows && D.prec == VMPS_PREC_FP64;
+  return !full_reortho && ngrp <= kFuseMaxGroups && cap < kFuseMaxRows && D.prec == VMPS_PREC_FP64;
 }
 
 void Krylov::enqueue_iteration(DevPlan& D, hipStream_t s, int kofs) {
@@ -629,7 +629,8 @@ LanczosResult lanczos_solve(DevPlan& D, double* d_v, const LanczosOpts& o, vmps_
   hipLaunchKernelGGL(k_normalize_into, dim3(vgrid(n)), dim3(256), 0, s, K.scal.p, d_v, K.Q.p, n);
   hipLaunchKernelGGL(k_copy, dim3(vgrid(n)), dim3(256), 0, s, K.Q.p, K.v.p, n);
   hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, s, K.kdev.p, 0);
-  if (K.partial_reortho(D)) hipLaunchKernelGGL(k_pro_init, dim3(1), dim3(64), 0, s, K.om.p);
+  const bool pro = K.partial_reortho(D);
+  if (pro) hipLaunchKernelGGL(k_pro_init, dim3(1), dim3(64), 0, s, K.om.p);
   VMPS_HIP_CHECK(hipGetLastError());
 
   std::vector<double> alpha, beta, y;
@@ -734,6 +735,34 @@ LanczosResult lanczos_solve(DevPlan& D, double* d_v, const LanczosOpts& o, vmps_
   res.residual = std::sqrt(std::max(0.0, r2));
   res.converged = converged;
   D.prec = saved;
+  if (pro) {
+    // With a semi-orthogonal basis the Ritz value equals the Rayleigh
+    // quotient of the Ritz vector to O(eps ||T||).  A larger gap means the
+    // basis lost orthogonality that the omega estimate did not see (it
+    // assumes a symmetric operator): solve again from the same start vector
+    // with the whole basis re-orthogonalised at every iteration.
+    double tnorm = 0.0;
+    for (int j = 0; j < k; ++j)
+      tnorm = std::max(tnorm, std::fabs(alpha[j]) + beta[j] + (j > 0 ? beta[j - 1] : 0.0));
+    if (!(std::fabs(E - rq) <= 1e-8 * std::max(tnorm, 1e-300))) {
+      hipLaunchKernelGGL(k_copy, dim3(vgrid(n)), dim3(256), 0, s, K.Q.p, d_v, n);  // q_0
+      VMPS_HIP_CHECK(hipGetLastError());
+      K.full_reortho = true;
+      K.release_graph();
+      LanczosResult again;
+      try {
+        again = lanczos_solve(D, d_v, o, prec);
+      } catch (...) {
+        K.full_reortho = false;
+        K.release_graph();
+        throw;
+      }
+      K.full_reortho = false;
+      K.release_graph();
+      again.hxv_calls += res.hxv_calls;
+      return again;
+    }
+  }
   return res;
 }
 

@@ -74,12 +74,14 @@ HandleSlot& handles_for_current_device() {
 
 using vmps::VmpsError;
 
-extern "C" vmps_status vmps_block_eigh_split(int nblocks, const int64_t* rows, const int64_t* cols, const double* A,
-                                             int side, double* lambda, double* W, double* C) {
-  try {
-    if (nblocks < 0 || (nblocks > 0 && (!rows || !cols || !A || !lambda || !W || !C)))
-      throw VmpsError(VMPS_ERR_ARG, "bad argument");
-    if (side != 0 && side != 1) throw VmpsError(VMPS_ERR_ARG, "side must be 0 (left) or 1 (right)");
+namespace {
+
+// The factorisation of every sector: A is given either on the host (packed
+// column-major sectors, copied lane by lane) or already on the device
+// (dA_packed, the same layout).  Arguments are validated by the caller.
+void factor_sectors(int nblocks, const int64_t* rows, const int64_t* cols, const double* A, const double* dA_packed,
+                    int side, double* lambda, double* W, double* C) {
+  {
     int64_t maxrc = 0;
     for (int b = 0; b < nblocks; ++b) {
       if (rows[b] < 0 || cols[b] < 0 || rows[b] > (1 << 20) || cols[b] > (1 << 20))
@@ -99,7 +101,7 @@ extern "C" vmps_status vmps_block_eigh_split(int nblocks, const int64_t* rows, c
         lo += k;
         wo += k * k;
       }
-      return VMPS_OK;
+      return;
     }
     // Sectors go to kLanes streams, largest first, each to the lane with the
     // least work so far (work ~ k^3); every lane runs its sectors in order
@@ -165,7 +167,7 @@ extern "C" vmps_status vmps_block_eigh_split(int nblocks, const int64_t* rows, c
     vmps::DevBuf<int> dInfo;
     for (int l = 0; l < NL; ++l) {
       if (lane_rc[l] == 0) continue;
-      dA[l].alloc((size_t)lane_rc[l]);
+      if (!dA_packed) dA[l].alloc((size_t)lane_rc[l]);
       dE[l].alloc((size_t)std::max<int64_t>(1, lane_k[l]));
     }
     dG.alloc((size_t)std::max<int64_t>(1, out_g));
@@ -192,15 +194,17 @@ extern "C" vmps_status vmps_block_eigh_split(int nblocks, const int64_t* rows, c
           const rocblas_int r = (rocblas_int)rows[b], c = (rocblas_int)cols[b], k = side == 0 ? r : c;
           double* G = dG.p + g_off[b];
           double* Cb = dC.p + c_off[b];
-          VMPS_HIP_CHECK(hipMemcpyAsync(dA[l].p, A + ao[b], sizeof(double) * r * c, hipMemcpyHostToDevice, st));
+          const double* Ab = dA_packed ? dA_packed + ao[b] : dA[l].p;
+          if (!dA_packed)
+            VMPS_HIP_CHECK(hipMemcpyAsync(dA[l].p, A + ao[b], sizeof(double) * r * c, hipMemcpyHostToDevice, st));
           // rho = A A^T (r x r) or A^T A (c x c), the full matrix
           if (side == 0)
-            vmps::rb_check(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_transpose, r, r, c, &one,
-                                         dA[l].p, r, dA[l].p, r, &zero, G, r),
+            vmps::rb_check(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_transpose, r, r, c, &one, Ab, r,
+                                         Ab, r, &zero, G, r),
                            "rocblas_dgemm");
           else
-            vmps::rb_check(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, c, c, r, &one,
-                                         dA[l].p, r, dA[l].p, r, &zero, G, c),
+            vmps::rb_check(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, c, c, r, &one, Ab, r,
+                                         Ab, r, &zero, G, c),
                            "rocblas_dgemm");
           vmps::rb_check(rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_upper, k, G, k, dL.p + lo[b],
                                           dE[l].p, dInfo.p + b),
@@ -208,11 +212,11 @@ extern "C" vmps_status vmps_block_eigh_split(int nblocks, const int64_t* rows, c
           // centre: W^T A (k x c) or A W (r x k)
           if (side == 0)
             vmps::rb_check(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, r, c, r, &one, G, r,
-                                         dA[l].p, r, &zero, Cb, r),
+                                         Ab, r, &zero, Cb, r),
                            "rocblas_dgemm");
           else
-            vmps::rb_check(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, r, c, c, &one, dA[l].p,
-                                         r, G, c, &zero, Cb, r),
+            vmps::rb_check(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, r, c, c, &one, Ab, r, G,
+                                         c, &zero, Cb, r),
                            "rocblas_dgemm");
         }
         for (int b : order) {
@@ -250,6 +254,124 @@ extern "C" vmps_status vmps_block_eigh_split(int nblocks, const int64_t* rows, c
     for (int b : order)
       if (info[b] != 0)
         throw VmpsError(VMPS_ERR_NOCONV, "rocsolver_dsyevd did not converge (info " + std::to_string(info[b]) + ")");
+  }
+}
+
+void check_sectors(int nblocks, const int64_t* rows, const int64_t* cols, int side) {
+  if (side != 0 && side != 1) throw VmpsError(VMPS_ERR_ARG, "side must be 0 (left) or 1 (right)");
+  for (int b = 0; b < nblocks; ++b) {
+    if (rows[b] < 0 || cols[b] < 0 || rows[b] > (1 << 20) || cols[b] > (1 << 20))
+      throw VmpsError(VMPS_ERR_ARG, "bad block dimension");
+    const int64_t k = side == 0 ? rows[b] : cols[b];
+    if (k > (1 << 16)) throw VmpsError(VMPS_ERR_ARG, "block too large for the eigensolver (side > 65536)");
+  }
+}
+
+// Sector matrices from the blocks of a device vector: destination block d is
+// sum over its parts p of factor[p] * v[src[p] .. src[p] + r c) (r x c,
+// column-major), written at (row0, col0) of sector dst_block[d]; one
+// workgroup per destination block (grid-strided), parts added in order.
+struct GatherDst {
+  int64_t a_off;  // element offset of (row0, col0) in the packed sectors
+  int64_t ld;     // rows of the sector
+  int64_t r, c;
+  int64_t p0, p1;  // parts [p0, p1)
+};
+
+__global__ __launch_bounds__(256) void k_gather_sectors(const GatherDst* __restrict__ dst, int ndst,
+                                                        const int64_t* __restrict__ src,
+                                                        const double* __restrict__ fac, const double* __restrict__ v,
+                                                        double* __restrict__ A) {
+  for (int d = blockIdx.x; d < ndst; d += gridDim.x) {
+    const GatherDst g = dst[d];
+    const int64_t n = g.r * g.c;
+    for (int64_t e = threadIdx.x; e < n; e += blockDim.x) {
+      double x = 0.0;
+      for (int64_t p = g.p0; p < g.p1; ++p) x += fac[p] * v[src[p] + e];
+      const int64_t i = e % g.r, j = e / g.r;
+      A[g.a_off + i + j * g.ld] = x;
+    }
+  }
+}
+
+}  // namespace
+
+extern "C" vmps_status vmps_block_eigh_split(int nblocks, const int64_t* rows, const int64_t* cols, const double* A,
+                                             int side, double* lambda, double* W, double* C) {
+  try {
+    if (nblocks < 0 || (nblocks > 0 && (!rows || !cols || !A || !lambda || !W || !C)))
+      throw VmpsError(VMPS_ERR_ARG, "bad argument");
+    check_sectors(nblocks, rows, cols, side);
+    factor_sectors(nblocks, rows, cols, A, nullptr, side, lambda, W, C);
+    return VMPS_OK;
+  } catch (const VmpsError& e) {
+    vmps::set_last_error(e.what());
+    return e.status;
+  } catch (const std::exception& e) {
+    vmps::set_last_error(e.what());
+    return VMPS_ERR_INTERNAL;
+  }
+}
+
+extern "C" vmps_status vmps_block_eigh_split_gather(const double* d_v, int64_t vsize, int nblocks,
+                                                    const int64_t* rows, const int64_t* cols, int ndst,
+                                                    const int* dst_block, const int64_t* dst_row,
+                                                    const int64_t* dst_col, const int64_t* dst_r,
+                                                    const int64_t* dst_c, const int64_t* part_begin,
+                                                    const int64_t* part_src, const double* part_factor, int side,
+                                                    double* lambda, double* W, double* C) {
+  try {
+    if (nblocks < 0 || ndst < 0 || vsize < 0 || (nblocks > 0 && (!rows || !cols || !lambda || !W || !C)))
+      throw VmpsError(VMPS_ERR_ARG, "bad argument");
+    if (ndst > 0 && (!d_v || !dst_block || !dst_row || !dst_col || !dst_r || !dst_c || !part_begin))
+      throw VmpsError(VMPS_ERR_ARG, "NULL gather table");
+    check_sectors(nblocks, rows, cols, side);
+    std::vector<int64_t> ao(nblocks + 1, 0);
+    for (int b = 0; b < nblocks; ++b) ao[b + 1] = ao[b] + rows[b] * cols[b];
+    const int64_t nparts = ndst > 0 ? part_begin[ndst] : 0;
+    if (nparts > 0 && (!part_src || !part_factor)) throw VmpsError(VMPS_ERR_ARG, "NULL gather parts");
+    // every destination inside its sector, every part inside the vector
+    std::vector<GatherDst> g((size_t)ndst);
+    for (int d = 0; d < ndst; ++d) {
+      const int b = dst_block[d];
+      if (b < 0 || b >= nblocks) throw VmpsError(VMPS_ERR_ARG, "gather: sector index out of range");
+      if (dst_r[d] < 0 || dst_c[d] < 0 || dst_row[d] < 0 || dst_col[d] < 0 || dst_row[d] + dst_r[d] > rows[b] ||
+          dst_col[d] + dst_c[d] > cols[b])
+        throw VmpsError(VMPS_ERR_LAYOUT, "gather: block outside its sector");
+      if (part_begin[d] < 0 || part_begin[d + 1] < part_begin[d] || part_begin[d + 1] > nparts)
+        throw VmpsError(VMPS_ERR_ARG, "gather: bad part range");
+      for (int64_t p = part_begin[d]; p < part_begin[d + 1]; ++p)
+        if (part_src[p] < 0 || part_src[p] + dst_r[d] * dst_c[d] > vsize)
+          throw VmpsError(VMPS_ERR_LAYOUT, "gather: part outside the vector");
+      g[(size_t)d] = {ao[b] + dst_row[d] + dst_col[d] * rows[b], rows[b], dst_r[d], dst_c[d], part_begin[d],
+                      part_begin[d + 1]};
+    }
+    const int64_t total = ao[nblocks];
+    vmps::DevBuf<double> dA;
+    dA.alloc((size_t)std::max<int64_t>(1, total));
+    hipStream_t s = vmps::acquire_stream();
+    struct Release {
+      hipStream_t s;
+      ~Release() {
+        (void)hipStreamSynchronize(s);
+        vmps::release_stream(s);
+      }
+    } rel{s};
+    if (total) VMPS_HIP_CHECK(hipMemsetAsync(dA.p, 0, sizeof(double) * total, s));
+    if (ndst > 0) {
+      vmps::DevBuf<GatherDst> dg;
+      vmps::DevBuf<int64_t> dsrc;
+      vmps::DevBuf<double> dfac;
+      dg.upload(g, s);
+      dsrc.upload(std::vector<int64_t>(part_src, part_src + std::max<int64_t>(1, nparts)), s);
+      dfac.upload(std::vector<double>(part_factor, part_factor + std::max<int64_t>(1, nparts)), s);
+      hipLaunchKernelGGL(k_gather_sectors, dim3((unsigned)std::min(ndst, 65535)), dim3(256), 0, s, dg.p, ndst,
+                         dsrc.p, dfac.p, d_v, dA.p);
+      VMPS_HIP_CHECK(hipGetLastError());
+      VMPS_HIP_CHECK(hipStreamSynchronize(s));  // the tables are freed on leaving this block
+    }
+    VMPS_HIP_CHECK(hipStreamSynchronize(s));
+    factor_sectors(nblocks, rows, cols, nullptr, dA.p, side, lambda, W, C);
     return VMPS_OK;
   } catch (const VmpsError& e) {
     vmps::set_last_error(e.what());

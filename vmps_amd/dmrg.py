@@ -472,8 +472,69 @@ def block_eigh_device(mats, side):
     return out
 
 
+class DevicePsi:
+    """A two-site wavefunction left on the device by the eigensolver (a plan
+    vector): the split reads it there (vmps_block_eigh_split_gather) and
+    copies it to the host only when the factorisation runs on the host."""
+
+    def __init__(self, H, dv):
+        self.H, self.dv = H, dv
+        self.size = H.vsize
+
+    def download(self):
+        return self.dv.download()
+
+    def close(self):
+        if self.dv is not None:
+            self.dv.free()
+            self.dv = None
+        if self.H is not None:
+            self.H.close()
+            self.H = None
+
+
+def block_eigh_gather_device(dev, shapes, dst, side):
+    """block_eigh_device on sector matrices built on the device from the
+    blocks of a DevicePsi: shapes = [(rows, cols)] per sector, dst = list of
+    (sector, row0, col0, r, c, [(src offset, factor), ...]); same return
+    format as block_eigh_device."""
+    lib = _lib.load()
+    rows = np.array([r for r, _ in shapes] or [0], dtype=np.int64)
+    cols = np.array([c for _, c in shapes] or [0], dtype=np.int64)
+    db = np.array([d[0] for d in dst] or [0], dtype=np.int32)
+    dr0 = np.array([d[1] for d in dst] or [0], dtype=np.int64)
+    dc0 = np.array([d[2] for d in dst] or [0], dtype=np.int64)
+    dr = np.array([d[3] for d in dst] or [0], dtype=np.int64)
+    dc = np.array([d[4] for d in dst] or [0], dtype=np.int64)
+    pb = np.zeros(len(dst) + 1, dtype=np.int64)
+    pb[1:] = np.cumsum([len(d[5]) for d in dst])
+    src = np.array([o for d in dst for (o, _) in d[5]] or [0], dtype=np.int64)
+    fac = np.array([f for d in dst for (_, f) in d[5]] or [0.0], dtype=np.float64)
+    k = rows if side == 0 else cols
+    k = k[:len(shapes)]
+    lam = np.empty(int(k.sum()) + 1)
+    W = np.empty(int((k * k).sum()) + 1)
+    C = np.empty(int((rows[:len(shapes)] * cols[:len(shapes)]).sum()) + 1)
+    P = lambda a, t=ctypes.c_double: a.ctypes.data_as(ctypes.POINTER(t))
+    Pi = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+    _lib.check(lib.vmps_block_eigh_split_gather(
+        dev.dv.ptr, dev.size, len(shapes), P(rows, ctypes.c_int64), P(cols, ctypes.c_int64), len(dst), Pi(db),
+        P(dr0, ctypes.c_int64), P(dc0, ctypes.c_int64), P(dr, ctypes.c_int64), P(dc, ctypes.c_int64),
+        P(pb, ctypes.c_int64), P(src, ctypes.c_int64), P(fac), side, P(lam), P(W), P(C)))
+    out, lo, wo, co = [], 0, 0, 0
+    for (r, c), kk in zip(shapes, k):
+        kk = int(kk)
+        l_ = lam[lo:lo + kk][::-1].copy()
+        w_ = W[wo:wo + kk * kk].reshape((kk, kk), order="F")[:, ::-1]
+        c_ = C[co:co + r * c].reshape((r, c), order="F")
+        c_ = c_[::-1, :] if side == 0 else c_[:, ::-1]
+        out.append((l_, w_, c_))
+        lo, wo, co = lo + kk, wo + kk * kk, co + r * c
+    return out
+
+
 def split_psi(pd, v, qloc, M, eps_svd=1e-14, left_to_right=True, svd=None, eigh=None, eps_rho=1e-15,
-              timings=None):
+              timings=None, eigh_gather=None):
     """split_AA2 + truncateSVD: recouple the two-site blocks to the middle
     sector Qm (coeff_splitAA), factor each Qm, keep the M largest singular
     values (multiplets) above eps_svd.  Returns (A_l, A_r, truncation weight =
@@ -488,7 +549,8 @@ def split_psi(pd, v, qloc, M, eps_svd=1e-14, left_to_right=True, svd=None, eigh=
     (the eigenvalues' resolution relative to ||Psi||^2) count as discarded."""
     sym = pd.sym
     t0 = time.perf_counter()
-    if not np.all(np.isfinite(v)):
+    dev = v if isinstance(v, DevicePsi) else None
+    if dev is None and not np.all(np.isfinite(v)):
         raise FloatingPointError("split: the two-site wavefunction has non-finite entries")
     # first pass: which blocks go to which middle sector Qm, with their factors
     parts = {}  # Qm -> [(row key, col key, offset of the block in v, r, c, factor)]
@@ -523,24 +585,51 @@ def split_psi(pd, v, qloc, M, eps_svd=1e-14, left_to_right=True, svd=None, eigh=
         meta.append((Qm, ro, co))
         shapes.append((nr, o, total))
         total += nr * o
-    packed = np.zeros(total + 1)
-    mats = [packed[o0:o0 + nr * nc].reshape((nr, nc), order="F") for (nr, nc, o0) in shapes]
-    for (Qm, ro, co), Mm in zip(meta, mats):
-        seen = set()
-        for (rk, ck, o0, r, c, f) in parts[Qm]:
-            blk = v[o0:o0 + r * c].reshape((r, c), order="F")
-            dst = Mm[ro[rk]:ro[rk] + r, co[ck]:co[ck] + c]
-            if (rk, ck) in seen:
-                dst += f * blk
-            else:
-                np.multiply(blk, f, out=dst)
-                seen.add((rk, ck))
+    efac = None
+    if dev is not None:
+        # with the wavefunction on the device, the sector matrices are built
+        # there (or, when the factorisation declines the device, on the host
+        # from a copy)
+        if eigh_gather is not None:
+            dst = []
+            for b, (Qm, ro, co) in enumerate(meta):
+                groups = {}
+                for (rk, ck, o0, r, c, f) in parts[Qm]:
+                    groups.setdefault((rk, ck), (r, c, []))[2].append((o0, f))
+                for (rk, ck), (r, c, lst) in groups.items():
+                    dst.append((b, ro[rk], co[ck], r, c, lst))
+            t1 = time.perf_counter()
+            efac = eigh_gather(dev, [(nr, nc) for (nr, nc, _) in shapes], dst, 0 if left_to_right else 1)
+        if efac is None:
+            v = dev.download()
+            if not np.all(np.isfinite(v)):
+                raise FloatingPointError("split: the two-site wavefunction has non-finite entries")
+    if efac is not None:
+        norm2 = float(sum(np.sum(lam) for (lam, _, _) in efac))  # trace of rho = ||Psi||^2
+        if not np.isfinite(norm2):
+            raise FloatingPointError("split: the two-site wavefunction has non-finite entries")
+        packed = None
+    else:
+        packed = np.zeros(total + 1)
+    if packed is not None:
+        mats = [packed[o0:o0 + nr * nc].reshape((nr, nc), order="F") for (nr, nc, o0) in shapes]
+        for (Qm, ro, co), Mm in zip(meta, mats):
+            seen = set()
+            for (rk, ck, o0, r, c, f) in parts[Qm]:
+                blk = v[o0:o0 + r * c].reshape((r, c), order="F")
+                dst = Mm[ro[rk]:ro[rk] + r, co[ck]:co[ck] + c]
+                if (rk, ck) in seen:
+                    dst += f * blk
+                else:
+                    np.multiply(blk, f, out=dst)
+                    seen.add((rk, ck))
+        t1 = time.perf_counter()
+        efac = eigh(mats, 0 if left_to_right else 1) if eigh is not None else None  # None: declined
+        if efac is not None:
+            norm2 = float(np.dot(packed, packed))
     fac = {}
     allsv = []
-    t1 = time.perf_counter()
-    efac = eigh(mats, 0 if left_to_right else 1) if eigh is not None else None  # None: declined
     if efac is not None:
-        norm2 = float(np.dot(packed, packed))
         floor = max(eps_svd, math.sqrt(eps_rho * norm2))
         for (Qm, ro, co), (lam, W, C) in zip(meta, efac):
             S = np.sqrt(np.maximum(lam, 0.0))
@@ -549,7 +638,7 @@ def split_psi(pd, v, qloc, M, eps_svd=1e-14, left_to_right=True, svd=None, eigh=
             allsv.extend((sv, Qm, i) for i, sv in enumerate(S))
     else:
         floor = eps_svd
-        for (Qm, ro, co), (U, S, Vt) in zip(meta, (svd or block_svd_host)(mats)):
+        for (Qm, ro, co), (U, S, Vt) in zip(meta, (svd or block_svd_host)(mats)):  # host packing above
             X, Y = (U, S[:, None] * Vt) if left_to_right else (U * S[None, :], Vt)
             fac[Qm] = (X, S, Y, ro, co)
             allsv.extend((sv, Qm, i) for i, sv in enumerate(S))
@@ -672,6 +761,14 @@ class GpuBackend:
         self.device_splits += 1
         return block_eigh_device(mats, side)
 
+    def eigh_gather(self, dev, shapes, dst, side):
+        """eigh() with the sector matrices built on the device from a
+        DevicePsi (block_eigh_gather_device), or None as eigh() declines."""
+        if not self.device_split_min or not shapes or max(min(sh) for sh in shapes) < self.device_split_min:
+            return None
+        self.device_splits += 1
+        return block_eigh_gather_device(dev, shapes, dst, side)
+
     def svd(self, mats):
         """Two-site split SVDs: on the device (vmps_block_svd) when
         device_svd_min > 0 and the largest block has both dimensions >=
@@ -703,14 +800,17 @@ class GpuBackend:
             self.mixed_solves += 1
         return H, prec, time.perf_counter() - t0
 
-    def solve_two_site(self, pd, Al, Ar, qloc, seed=0):
+    def solve_two_site(self, pd, Al, Ar, qloc, seed=0, keep_device=False):
         """contract_AA2 on the device into the start vector (a zero vector is
         replaced by a random one), then solve() on it without a host copy of
-        the start vector.  Returns (energy, eigenvector on the host, result
-        dict with "psi_seconds", the time of the start-vector build)."""
+        the start vector.  Returns (energy, eigenvector, result dict with
+        "psi_seconds", the time of the start-vector build); the eigenvector is
+        a host array, or with `keep_device` a DevicePsi (the caller closes it,
+        which also releases the plan)."""
         from .pivot import DeviceVector, contract_aa2, lanczos_device, vec_fill_random, vec_norm
         H, prec, t_plan = self.prepare(pd)
         dv = None
+        kept = False
         try:
             t0 = time.perf_counter()
             dv = DeviceVector(H)
@@ -722,12 +822,17 @@ class GpuBackend:
                                tol_eigval=self.tol_eigval, tol_state=self.tol_state,
                                refine_tol_eigval=self.tol_eigval, refine_tol_state=self.tol_state,
                                refine_method=self.refine_method)
-            v = dv.download()
             r["flops_alg"] = H.stats()["flops_alg"]
+            if keep_device:
+                v = DevicePsi(H, dv)
+                kept = True
+            else:
+                v = dv.download()
         finally:
-            if dv is not None:
-                dv.free()
-            H.close()
+            if not kept:
+                if dv is not None:
+                    dv.free()
+                H.close()
         r["precision"] = prec
         r["plan_seconds"] = t_plan
         r["psi_seconds"] = t_psi
@@ -972,7 +1077,8 @@ def dmrg_step(chain, backend, l, direction, rng=None, timings=None):
     pd = chain.pivot(l)
     if hasattr(backend, "solve_two_site"):  # contract_AA2 on the device, inside the solve
         t1 = t()
-        E, v, info = backend.solve_two_site(pd, chain.A[l], chain.A[l + 1], chain.qloc, seed=l + 1)
+        E, v, info = backend.solve_two_site(pd, chain.A[l], chain.A[l + 1], chain.qloc, seed=l + 1,
+                                            keep_device=hasattr(backend, "eigh_gather"))
         t2 = t()
         t1 += info["psi_seconds"]
     else:
@@ -982,9 +1088,13 @@ def dmrg_step(chain, backend, l, direction, rng=None, timings=None):
         t1 = t()
         E, v, info = backend.solve(pd, v0)
         t2 = t()
-    Al, Ar, tw, ent, kept = split_psi(pd, v, chain.qloc, chain.M, left_to_right=(direction == "right"),
-                                      svd=getattr(backend, "svd", None), eigh=getattr(backend, "eigh", None),
-                                      timings=timings)
+    try:
+        Al, Ar, tw, ent, kept = split_psi(pd, v, chain.qloc, chain.M, left_to_right=(direction == "right"),
+                                          svd=getattr(backend, "svd", None), eigh=getattr(backend, "eigh", None),
+                                          timings=timings, eigh_gather=getattr(backend, "eigh_gather", None))
+    finally:
+        if isinstance(v, DevicePsi):
+            v.close()
     chain.A[l], chain.A[l + 1] = Al, Ar
     chain.bonds[l + 1] = {qm: k for qm, k in kept.items()}
     t3 = t()
