@@ -88,7 +88,7 @@ def test_dmrg_step_device_psi_matches_host_psi(state):
     """One DMRG step through the device start vector and one through the host
     start vector, from copies of the same state: the same energy."""
     chain, p = state
-    be = dmrg.GpuBackend(tol_eigval=1e-12, tol_state=1e-8)
+    be = dmrg.GpuBackend(tol_eigval=1e-12, tol_state=1e-8, device_psi_min=0)
 
     class HostPsi:  # the same backend without the device start-vector path
         def __init__(self, b):

@@ -505,8 +505,14 @@ void dev_fill_uniform(double* v, int64_t n, uint64_t seed, hipStream_t s) {
 // is captured once per plan configuration as a HIP graph and replayed each
 // iteration; without it the same kernels are launched one by one.  Either way
 // the host waits once per iteration, for three numbers.
+// Below 64k elements a pass over the whole basis costs about what the local
+// passes and the decision cost (every kernel is at its launch floor), so
+// small vectors keep full re-orthogonalisation.
+constexpr int64_t kProMinElements = 65536;
+
 bool Krylov::partial_reortho(const DevPlan& D) const {
-  return !full_reortho && ngrp <= kFuseMaxGroups && cap < kFuseMaxRows && D.prec == VMPS_PREC_FP64;
+  return !full_reortho && nvec >= kProMinElements && ngrp <= kFuseMaxGroups && cap < kFuseMaxRows &&
+         D.prec == VMPS_PREC_FP64;
 }
 
 void Krylov::enqueue_iteration(DevPlan& D, hipStream_t s, int kofs) {

@@ -22,6 +22,7 @@
 #include <rocsolver/rocsolver.h>
 
 #include <algorithm>
+#include <climits>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -52,6 +53,7 @@ constexpr int kLanes = 8;
 struct HandleSlot {
   rocblas_handle h[kLanes] = {};
   std::mutex m;
+  bool warm[2] = {false, false};  // a factorisation of this side has run here with its lanes on one thread
 };
 std::mutex g_handles_mutex;
 std::map<int, HandleSlot*> g_handles;
@@ -238,7 +240,16 @@ void factor_sectors(int nblocks, const int64_t* rows, const int64_t* cols, const
         err_status[l] = VMPS_ERR_INTERNAL;
       }
     };
-    {
+    if (!slot.warm[side]) {
+      // The first factorisation on a device runs its lanes one after another
+      // on this thread: the libraries load their GPU code on first use (the
+      // GEMM and eigensolver kernels this side calls), and that loading
+      // should not run on eight threads at once.  (A run under the kernel
+      // tracer faulted inside such a first launch from a lane thread.)
+      for (int l = 0; l < NL; ++l)
+        if (lane_rc[l]) run_lane(l);
+      slot.warm[side] = true;
+    } else {
       std::vector<std::thread> workers;
       for (int l = 1; l < NL; ++l)
         if (lane_rc[l]) workers.emplace_back(run_lane, l);
@@ -269,27 +280,31 @@ void check_sectors(int nblocks, const int64_t* rows, const int64_t* cols, int si
 
 // Sector matrices from the blocks of a device vector: destination block d is
 // sum over its parts p of factor[p] * v[src[p] .. src[p] + r c) (r x c,
-// column-major), written at (row0, col0) of sector dst_block[d]; one
-// workgroup per destination block (grid-strided), parts added in order.
-struct GatherDst {
-  int64_t a_off;  // element offset of (row0, col0) in the packed sectors
+// column-major), written at (row0, col0) of its sector.  The work is cut into
+// tasks of whole columns of one destination, about 4096 elements each, one
+// workgroup per task (grid-strided); parts are added in order.
+struct GatherTask {
+  int64_t a_off;  // element offset of (row0, col0 + j0) in the packed sectors
   int64_t ld;     // rows of the sector
-  int64_t r, c;
+  int64_t src_off;  // column j0 of the block within a part: j0 * r
+  int32_t r, nj;  // block rows, columns of this task
   int64_t p0, p1;  // parts [p0, p1)
 };
 
-__global__ __launch_bounds__(256) void k_gather_sectors(const GatherDst* __restrict__ dst, int ndst,
+__global__ __launch_bounds__(256) void k_gather_sectors(const GatherTask* __restrict__ tasks, int ntasks,
                                                         const int64_t* __restrict__ src,
                                                         const double* __restrict__ fac, const double* __restrict__ v,
                                                         double* __restrict__ A) {
-  for (int d = blockIdx.x; d < ndst; d += gridDim.x) {
-    const GatherDst g = dst[d];
-    const int64_t n = g.r * g.c;
-    for (int64_t e = threadIdx.x; e < n; e += blockDim.x) {
-      double x = 0.0;
-      for (int64_t p = g.p0; p < g.p1; ++p) x += fac[p] * v[src[p] + e];
-      const int64_t i = e % g.r, j = e / g.r;
-      A[g.a_off + i + j * g.ld] = x;
+  for (int t = blockIdx.x; t < ntasks; t += gridDim.x) {
+    const GatherTask g = tasks[t];
+    for (int j = 0; j < g.nj; ++j) {
+      const int64_t sc = g.src_off + (int64_t)j * g.r;  // column j of the task within a part
+      double* __restrict__ dst = A + g.a_off + (int64_t)j * g.ld;
+      for (int i = threadIdx.x; i < g.r; i += blockDim.x) {
+        double x = 0.0;
+        for (int64_t p = g.p0; p < g.p1; ++p) x += fac[p] * v[src[p] + sc + i];
+        dst[i] = x;
+      }
     }
   }
 }
@@ -331,7 +346,7 @@ extern "C" vmps_status vmps_block_eigh_split_gather(const double* d_v, int64_t v
     const int64_t nparts = ndst > 0 ? part_begin[ndst] : 0;
     if (nparts > 0 && (!part_src || !part_factor)) throw VmpsError(VMPS_ERR_ARG, "NULL gather parts");
     // every destination inside its sector, every part inside the vector
-    std::vector<GatherDst> g((size_t)ndst);
+    std::vector<GatherTask> g;
     for (int d = 0; d < ndst; ++d) {
       const int b = dst_block[d];
       if (b < 0 || b >= nblocks) throw VmpsError(VMPS_ERR_ARG, "gather: sector index out of range");
@@ -343,9 +358,14 @@ extern "C" vmps_status vmps_block_eigh_split_gather(const double* d_v, int64_t v
       for (int64_t p = part_begin[d]; p < part_begin[d + 1]; ++p)
         if (part_src[p] < 0 || part_src[p] + dst_r[d] * dst_c[d] > vsize)
           throw VmpsError(VMPS_ERR_LAYOUT, "gather: part outside the vector");
-      g[(size_t)d] = {ao[b] + dst_row[d] + dst_col[d] * rows[b], rows[b], dst_r[d], dst_c[d], part_begin[d],
-                      part_begin[d + 1]};
+      if (dst_r[d] > INT32_MAX) throw VmpsError(VMPS_ERR_ARG, "gather: block too large");
+      if (dst_r[d] * dst_c[d] == 0) continue;
+      const int64_t per = std::max<int64_t>(1, 4096 / dst_r[d]);  // columns per task
+      for (int64_t j0 = 0; j0 < dst_c[d]; j0 += per)
+        g.push_back({ao[b] + dst_row[d] + (dst_col[d] + j0) * rows[b], rows[b], j0 * dst_r[d], (int32_t)dst_r[d],
+                     (int32_t)std::min(per, dst_c[d] - j0), part_begin[d], part_begin[d + 1]});
     }
+    const int ntasks = (int)g.size();
     const int64_t total = ao[nblocks];
     vmps::DevBuf<double> dA;
     dA.alloc((size_t)std::max<int64_t>(1, total));
@@ -358,14 +378,14 @@ extern "C" vmps_status vmps_block_eigh_split_gather(const double* d_v, int64_t v
       }
     } rel{s};
     if (total) VMPS_HIP_CHECK(hipMemsetAsync(dA.p, 0, sizeof(double) * total, s));
-    if (ndst > 0) {
-      vmps::DevBuf<GatherDst> dg;
+    if (ntasks > 0) {
+      vmps::DevBuf<GatherTask> dg;
       vmps::DevBuf<int64_t> dsrc;
       vmps::DevBuf<double> dfac;
       dg.upload(g, s);
       dsrc.upload(std::vector<int64_t>(part_src, part_src + std::max<int64_t>(1, nparts)), s);
       dfac.upload(std::vector<double>(part_factor, part_factor + std::max<int64_t>(1, nparts)), s);
-      hipLaunchKernelGGL(k_gather_sectors, dim3((unsigned)std::min(ndst, 65535)), dim3(256), 0, s, dg.p, ndst,
+      hipLaunchKernelGGL(k_gather_sectors, dim3((unsigned)std::min(ntasks, 65535)), dim3(256), 0, s, dg.p, ntasks,
                          dsrc.p, dfac.p, d_v, dA.p);
       VMPS_HIP_CHECK(hipGetLastError());
       VMPS_HIP_CHECK(hipStreamSynchronize(s));  // the tables are freed on leaving this block

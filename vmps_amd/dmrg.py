@@ -730,13 +730,17 @@ class GpuBackend:
     least tune_min_flops of work per HxV get autotuned tiles (cached per
     block-size histogram, so a sweep tunes once per shape).
 
+    Pivots of at least `device_psi_min` elements build their Lanczos start
+    vector on the device (contract_AA2) and leave the eigenvector there for
+    the split (DevicePsi).
+
     With `retain_host_memory` (default) the backend asks the C library's
     allocator to keep freed host memory mapped
     (retain_freed_host_memory)."""
 
     def __init__(self, precision="fp64", refine_fp64=True, tol_eigval=1e-10, tol_state=1e-6, max_iter=100,
                  mixed_min_flops=4e9, device_svd_min=0, device_envs=True, refine_method="correction",
-                 device_split_min=256, tune_min_flops=1e9, retain_host_memory=True):
+                 device_split_min=256, tune_min_flops=1e9, retain_host_memory=True, device_psi_min=65536):
         if retain_host_memory:
             retain_freed_host_memory()
         self.precision = precision
@@ -748,6 +752,7 @@ class GpuBackend:
         self.device_svd_min = device_svd_min
         self.device_split_min = device_split_min
         self.tune_min_flops = tune_min_flops
+        self.device_psi_min = device_psi_min
         self.hxv_calls = 0
         self.mixed_solves = 0
         self.device_svds = 0
@@ -799,6 +804,13 @@ class GpuBackend:
                 H.autotune(iters=1)
             self.mixed_solves += 1
         return H, prec, time.perf_counter() - t0
+
+    def device_psi(self, pd):
+        """Whether the step builds its start vector on the device and leaves
+        the eigenvector there for the split: for pivots of at least
+        device_psi_min elements (on smaller ones the block transfers and
+        GEMM launches cost more than the host products)."""
+        return pd.vsize >= self.device_psi_min
 
     def solve_two_site(self, pd, Al, Ar, qloc, seed=0, keep_device=False):
         """contract_AA2 on the device into the start vector (a zero vector is
@@ -1075,7 +1087,7 @@ def dmrg_step(chain, backend, l, direction, rng=None, timings=None):
     t = time.perf_counter
     t0 = t()
     pd = chain.pivot(l)
-    if hasattr(backend, "solve_two_site"):  # contract_AA2 on the device, inside the solve
+    if hasattr(backend, "solve_two_site") and backend.device_psi(pd):  # contract_AA2 on the device, inside the solve
         t1 = t()
         E, v, info = backend.solve_two_site(pd, chain.A[l], chain.A[l + 1], chain.qloc, seed=l + 1,
                                             keep_device=hasattr(backend, "eigh_gather"))
