@@ -68,7 +68,7 @@ struct Krylov {
   DevBuf<double> Q, w, v, c, scal, out3;
   DevBuf<int> kdev;
   DevBuf<double> ew;      // coeff_dot weight of every vector element
-  DevBuf<double> gpart;   // partial sums of the basis dot products, (kmax + 1) x ngrp
+  DevBuf<double> gpart;   // partial sums of the basis dot products, (kmax + 3) x ngrp
   DevBuf<double> npart;   // partial sums of ||w||^2
   DevBuf<double> om;      // partial re-orthogonalisation: rows k-1, k, k+1 of the omega estimate
   DevBuf<int> gate;       // gate[k]: iteration k ran the full Gram-Schmidt pass

@@ -201,11 +201,12 @@ __global__ __launch_bounds__(256) void k_sub_combination_fused(const double* __r
                                                                double* __restrict__ c, double* __restrict__ w,
                                                                const double* __restrict__ ew,
                                                                double* __restrict__ norm_part, int local,
-                                                               const int* __restrict__ gate) {
+                                                               const int* __restrict__ gate, int poff) {
   __shared__ double cs[kFuseMaxRows];
   const int k = *kdev + kofs;
   if (gate && !gate[k]) return;  // workgroup-uniform
   const int r0 = local ? max(0, k - 1) : 0;
+  partial += (int64_t)poff * nb;  // row i's partials at row i + poff
   // four threads per row, each summing every fourth partial, then the four
   // sums combined by shuffles within the quad: a fixed order in every
   // workgroup, and a quarter of the dependent load chain per thread
@@ -242,6 +243,77 @@ __global__ __launch_bounds__(256) void k_sub_combination_fused(const double* __r
   if (!norm_part) return;  // workgroup-uniform
   const double sum = wg_sum(nrm);
   if (threadIdx.x == 0) norm_part[blockIdx.x] = sum;
+}
+
+// The first local pass's subtraction and the second local pass's dots in one
+// kernel (partially re-orthogonalised iterations, fused path): workgroup b
+// sums the partials of rows r0 = max(0, k-1) .. k as k_sub_combination_fused
+// does (workgroup 0 stores them in c), subtracts sum_i c[i] Q_i from w over
+// chunk b, and forms the chunk's dot partials of the result with the same
+// rows, into rows i + 2 of `partial` (the rows the subtraction reads stay
+// intact for the other workgroups).  Elements, their order per thread and the
+// reductions are those of k_multidot_rows, so the partials are bit-identical
+// to a k_sub_combination_fused + k_multidot_rows pair.
+__global__ __launch_bounds__(256) void k_sub_local_dots(const double* __restrict__ Q, int64_t n,
+                                                        double* __restrict__ partial, int nb,
+                                                        const int* __restrict__ kdev, int kofs,
+                                                        double* __restrict__ c, double* __restrict__ w,
+                                                        const double* __restrict__ ew, int64_t chunk) {
+  __shared__ double cs[2];
+  __shared__ double ws[4][2];
+  const int k = *kdev + kofs;
+  const int r0 = max(0, k - 1);
+  const int quad = threadIdx.x & 3;
+  const int ir = threadIdx.x >> 2;
+  if (r0 + ir <= k) {  // quad-uniform: threads 0..7 (rows r0, r0 + 1)
+    const int i = r0 + ir;
+    double p[kFuseMaxGroups / 4];
+#pragma unroll
+    for (int j = 0; j < kFuseMaxGroups / 4; ++j) {
+      const int b = quad + 4 * j;
+      p[j] = b < nb ? partial[(int64_t)i * nb + b] : 0.0;
+    }
+    double acc = 0.0;
+#pragma unroll
+    for (int j = 0; j < kFuseMaxGroups / 4; ++j) acc += p[j];
+    acc += __shfl_xor(acc, 1);
+    acc += __shfl_xor(acc, 2);
+    if (quad == 0) {
+      cs[ir] = acc;
+      if (blockIdx.x == 0) c[i] = acc;
+    }
+  }
+  __syncthreads();
+  const int nr = k + 1 - r0;  // 1 or 2
+  const double c0 = cs[0], c1 = nr > 1 ? cs[1] : 0.0;
+  const double* __restrict__ q0 = Q + (int64_t)r0 * n;
+  const double* __restrict__ q1 = Q + (int64_t)k * n;  // row r0 again when k = 0
+  const int b = blockIdx.x;
+  const int64_t e0 = (int64_t)b * chunk, e1 = min(n, e0 + chunk);
+  double a0 = 0.0, a1 = 0.0;
+#pragma unroll 4
+  for (int64_t e = e0 + threadIdx.x; e < e1; e += 256) {
+    const double x0 = q0[e], x1 = q1[e];
+    double s = 0.0;
+    s += c0 * x0;
+    if (nr > 1) s += c1 * x1;
+    const double y = w[e] - s;
+    w[e] = y;
+    const double we = y * ew[e];
+    a0 += x0 * we;
+    a1 += x1 * we;
+  }
+  for (int m = 32; m > 0; m >>= 1) {
+    a0 += __shfl_xor(a0, m);
+    a1 += __shfl_xor(a1, m);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    ws[threadIdx.x >> 6][0] = a0;
+    ws[threadIdx.x >> 6][1] = a1;
+  }
+  __syncthreads();
+  const int t = threadIdx.x;
+  if (t < nr) partial[(int64_t)(r0 + t + 2) * nb + b] = (ws[0][t] + ws[1][t]) + (ws[2][t] + ws[3][t]);
 }
 
 // sum of part[0 .. np-1] (np <= 2048, the largest vgrid) in one workgroup,
@@ -528,7 +600,7 @@ void Krylov::enqueue_iteration(DevPlan& D, hipStream_t s, int kofs) {
                        ew.p, chunk, gpart.p, ngrp, local);
     if (fused) {
       hipLaunchKernelGGL(k_sub_combination_fused, dim3(gs), dim3(256), 0, s, Q.p, n, gpart.p, ngrp, kdev.p, kofs,
-                         c_out, w.p, ew.p, norm_part, local, (const int*)nullptr);
+                         c_out, w.p, ew.p, norm_part, local, (const int*)nullptr, 0);
     } else {
       hipLaunchKernelGGL(k_multidot_rows_final, dim3(cap), dim3(256), 0, s, kdev.p, kofs, gpart.p, ngrp, c_out);
       hipLaunchKernelGGL(k_sub_combination_dk, dim3(gs), dim3(256), 0, s, Q.p, n, c_out, kdev.p, kofs, w.p, ew.p,
@@ -537,15 +609,20 @@ void Krylov::enqueue_iteration(DevPlan& D, hipStream_t s, int kofs) {
   };
   D.hxv(v.p, w.p, s);
   if (partial_reortho(D)) {
-    // FP64 operator: classical Gram-Schmidt twice against q_{k-1} and q_k,
-    // then the whole basis only where the omega estimate asks for it
-    pass(c1, nullptr, 1);
-    pass(c2, npart.p, 1);
+    // FP64 operator: classical Gram-Schmidt twice against q_{k-1} and q_k
+    // (the first pass's subtraction and the second pass's dots in one
+    // kernel), then the whole basis only where the omega estimate asks for it
+    hipLaunchKernelGGL(k_multidot_rows, dim3(ngrp, 1u), dim3(256), 0, s, kdev.p, kofs, Q.p, n, w.p, ew.p, chunk,
+                       gpart.p, ngrp, 1);
+    hipLaunchKernelGGL(k_sub_local_dots, dim3(ngrp), dim3(256), 0, s, Q.p, n, gpart.p, ngrp, kdev.p, kofs, c1, w.p,
+                       ew.p, chunk);
+    hipLaunchKernelGGL(k_sub_combination_fused, dim3(gs), dim3(256), 0, s, Q.p, n, gpart.p, ngrp, kdev.p, kofs, c2,
+                       w.p, ew.p, npart.p, 1, (const int*)nullptr, 2);
     const double eps1 = 1.1102230246251565e-16 * std::sqrt((double)n);
     hipLaunchKernelGGL(k_pro_decide_dots, dim3(ngrp, gy), dim3(256), 0, s, kdev.p, kofs, Q.p, n, w.p, ew.p, chunk,
                        gpart.p, ngrp, npart.p, (int)gs, c1, c2, out3.p, om.p, cap + 2, eps1, gate.p);
     hipLaunchKernelGGL(k_sub_combination_fused, dim3(gs), dim3(256), 0, s, Q.p, n, gpart.p, ngrp, kdev.p, kofs, c3,
-                       w.p, ew.p, npart.p, 0, (const int*)gate.p);
+                       w.p, ew.p, npart.p, 0, (const int*)gate.p, 0);
     hipLaunchKernelGGL(k_normalize_next, dim3(gs), dim3(256), 0, s, kdev.p, kofs, npart.p, (int)gs, w.p, Q.p, n,
                        v.p, c1, c2, c3, (const int*)gate.p, out3.p);
   } else {
@@ -971,7 +1048,7 @@ void Krylov::ensure(DevPlan& D, int kmax) {
   if (n <= (int64_t)kFuseMaxGroups * 8192) ngrp = std::min<int>(ngrp, kFuseMaxGroups);
   chunk = (n + ngrp - 1) / ngrp;
   npart.alloc(vgrid(n));
-  gpart.alloc((size_t)std::max(1, ngrp) * (need + 1));
+  gpart.alloc((size_t)std::max(1, ngrp) * (need + 3));  // rows 0 .. k and k + 1, k + 2 (k_sub_local_dots)
   VMPS_HIP_CHECK(hipStreamSynchronize(D.stream));
   cap = need;
   nvec = n;
