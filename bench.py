@@ -299,7 +299,7 @@ def c2_sweeps(args, sweeps=3):
 def dmrg_c60_bench(args):
     import torch
     from vmps_amd import dmrg, lattice, models, synth
-    from vmps_amd.pivot import HxV, PivotMatrix2
+    from vmps_amd.pivot import HxV, PivotMatrix2, clear_tile_cache
 
     assert torch.cuda.is_available(), "no HIP device"
 
@@ -352,49 +352,69 @@ def dmrg_c60_bench(args):
             H.close()
         return rec
 
-    # One trajectory, the mixed one.  Before each mixed step an FP64 step runs
-    # from the same state on a copy of the chain (timed on its own, then
-    # dropped): the two solve the same effective Hamiltonian from the same
-    # start vector, so |E_mixed - E_fp64| measures the precision alone.  Two
-    # independent trajectories would not: at M = 4096 the kept set at each
-    # split is sensitive to perturbations far below the truncation weight, and
-    # two FP64 runs that differ only in the Lanczos tolerance drift apart by
-    # some 1e-8 over 20 steps.
-    def timed_step(chain, be, l):
-        tm = {}
-        torch.cuda.synchronize()
-        ts = time.perf_counter()
-        E, tw, info = dmrg.dmrg_step(chain, be, l, "right", timings=tm)
-        torch.cuda.synchronize()
-        dt = time.perf_counter() - ts
-        return {"pivot": l, "timed": False, "energy": E, "truncation": tw, "seconds": dt,
+    # Each leg is one trajectory of consecutive steps from the same expanded
+    # state, timed as a block: one device synchronisation before the first
+    # timed step and one after the last, as in a sweep, where a step's
+    # environment update still runs on the device while the host builds the
+    # next pivot's plan.  A step's seconds run from its start to the next
+    # step's start.  The precision check is separate: from the mixed
+    # trajectory's state before each step, one FP64 control step (untimed)
+    # solves the same effective Hamiltonian from the same start vector, so
+    # |E_mixed - E_fp64| measures the precision alone.  Comparing the two
+    # trajectories would not: at M = 4096 the kept set at each split is
+    # sensitive to perturbations far below the truncation weight, and two
+    # FP64 runs that differ only in the Lanczos tolerance drift apart by some
+    # 1e-8 over 20 steps.
+    def step_record(l, E, tw, info, tm, ts):
+        return {"pivot": l, "timed": False, "energy": E, "truncation": tw, "start": ts,
                 "phases": {k: round(v, 4) for k, v in tm.items()},
                 "hxv_calls": info["hxv_calls"], "hxv_fp64": info["hxv_fp64"],
                 "iterations": info["iterations"], "iterations_refine": info["iterations_refine"],
                 "refine_steps": info["refine_steps"], "refine_cg": info["refine_cg"],
                 "residual": info["residual"], "flops_alg": info["flops_alg"]}
 
+    def run_leg(leg, chain, snapshots=None):
+        clear_tile_cache()  # each leg pays for its own tile tuning, as a sweep of its own would
+        recs = []
+        for i in range(nsteps):
+            l = p0 + i
+            if snapshots is not None:
+                snapshots.append(chain.copy())
+            if i == args.warmup:
+                torch.cuda.synchronize()
+            tm = {}
+            ts = time.perf_counter()
+            E, tw, info = dmrg.dmrg_step(chain, be[leg], l, "right", timings=tm)
+            recs.append(step_record(l, E, tw, info, tm, ts))
+            recs[-1]["timed"] = i >= args.warmup
+            log("%s step %d (pivot %d%s): E = %.12f  HxV %d (FP64 %d)  residual %.1e"
+                % (leg, i, l, "" if i >= args.warmup else ", warm-up", E, info["hxv_calls"], info["hxv_fp64"],
+                   info["residual"]))
+        torch.cuda.synchronize()
+        t_end = time.perf_counter()
+        for a, b in zip(recs, recs[1:] + [None]):
+            a["seconds"] = (b["start"] if b else t_end) - a["start"]
+            del a["start"]
+        return recs
+
     be = {"mixed": dmrg.GpuBackend(precision=args.mixed, mixed_min_flops=0, **tol)}
     if not args.no_fp64_leg:
         be["fp64"] = dmrg.GpuBackend(precision="fp64", **tol)
-    steps = {leg: [] for leg in be}
-    c = big.copy()
-    first_pd = None
-    for i in range(nsteps):
-        l = p0 + i
-        if i == args.warmup:
-            first_pd = c.pivot(l)
-        if "fp64" in be:
-            ctrl = c.copy()
-            steps["fp64"].append(timed_step(ctrl, be["fp64"], l))
+    snapshots = []
+    steps = {"mixed": run_leg("mixed", big.copy(), snapshots)}
+    first_pd = snapshots[args.warmup].pivot(p0 + args.warmup)
+    control = []
+    if "fp64" in be:
+        steps["fp64"] = run_leg("fp64", big.copy())
+        for i in range(args.warmup, nsteps):
+            ctrl = snapshots[i]
+            E, tw, info = dmrg.dmrg_step(ctrl, be["fp64"], p0 + i, "right")
+            control.append({"pivot": p0 + i, "energy": E, "truncation": tw, "hxv_calls": info["hxv_calls"],
+                            "residual": info["residual"]})
+            snapshots[i] = None
             del ctrl
-        steps["mixed"].append(timed_step(c, be["mixed"], l))
-        for leg in be:
-            st = steps[leg][-1]
-            st["timed"] = i >= args.warmup
-            log("%s step %d (pivot %d%s): E = %.12f  %.2f s  HxV %d (FP64 %d)  residual %.1e"
-                % (leg, i, l, "" if i >= args.warmup else ", warm-up", st["energy"], st["seconds"],
-                   st["hxv_calls"], st["hxv_fp64"], st["residual"]))
+        torch.cuda.synchronize()
+    del snapshots
     legs = {}
     for leg in be:
         timed = [st for st in steps[leg] if st["timed"]]
@@ -410,13 +430,15 @@ def dmrg_c60_bench(args):
                      "hxv_fp64": sum(st["hxv_fp64"] for st in timed),
                      "phase_seconds": {k: round(v, 3) for k, v in phases.items()}, "steps": steps[leg]}
     mx = legs["mixed"]
-    dmrg_rec = {"trajectory": "the mixed leg's; every FP64 step starts from the same state (a copy of the chain) "
-                              "and is dropped after timing",
+    dmrg_rec = {"trajectory": "each leg one trajectory of consecutive steps from the same state, timed as a block; "
+                              "the precision check from FP64 control steps (untimed) at the mixed trajectory's "
+                              "state before each timed step",
                 "legs": legs}
     if "fp64" in legs:
         f64 = legs["fp64"]
-        dE = [abs(a["energy"] - b["energy"]) for a, b in zip(mx["steps"], f64["steps"]) if a["timed"]]
-        dmrg_rec.update({"max_abs_dE_mixed_vs_fp64": max(dE), "dE_per_step": dE,
+        mixed_timed = [st for st in mx["steps"] if st["timed"]]
+        dE = [abs(a["energy"] - b["energy"]) for a, b in zip(mixed_timed, control)]
+        dmrg_rec.update({"max_abs_dE_mixed_vs_fp64": max(dE), "dE_per_step": dE, "fp64_control_steps": control,
                          "step_speedup_mixed_vs_fp64": f64["seconds_timed"] / mx["seconds_timed"],
                          "fp64_ms_per_step": f64["ms_per_step"], "fp64_gflops": f64["gflops"]})
 

@@ -257,6 +257,11 @@ vmps_status vmps_plan_set_tiles(vmps_plan* plan, const vmps_tile_config* cfg);  
  * classes of block rows x columns) and per precision, so later plans of the
  * same shape take it without timing.  Thread-safe. */
 vmps_status vmps_plan_autotune(vmps_plan* plan, int iters, vmps_tile_config* out);
+/* Forgets every cached choice of vmps_plan_autotune in this process, so that
+ * the next plan of each shape is timed again (for instance, to time two
+ * solver configurations that should each pay their own tuning).  No
+ * counterpart in the reference, which has no tile configuration.  Thread-safe. */
+vmps_status vmps_autotune_clear_cache(void);
 
 /* ---- environment updates ---------------------------------------------------
  * contract_L(Lold, Abra, W, Aket, qloc, qOp, Lnew)  (tensors/DmrgContractions.h:34-158)
@@ -386,7 +391,12 @@ vmps_status vmps_denv_get(const vmps_denv* d, int* qin, int* qout, int* qmid, in
                           int64_t* cols, double* data);
 /* vmps_env_contract whose result stays on the device as a new vmps_denv
  * (vmps_env_result_get then reports VMPS_ERR_STATE; result_size and stats
- * still answer) */
+ * still answer).  With the old environment on the device (or none), it
+ * returns once the contraction's kernels are queued: every reader of the new
+ * vmps_denv (a plan that gathers it, a contraction that takes it as its old
+ * environment, vmps_denv_get, vmps_denv_destroy) waits for them first, so the
+ * caller can build the next plan on the host meanwhile.  A contraction from
+ * a host environment returns when it has finished. */
 vmps_status vmps_env_contract_device(vmps_env* e, vmps_precision prec, vmps_denv** out);
 /* the old environment of a contraction: the whole Tripod Lold / Rold at once
  * (instead of vmps_env_add_old*) */

@@ -141,6 +141,40 @@ def test_sweep_chain_stays_on_device():
     Hd.close()
 
 
+def test_queued_updates_feed_the_next_update_and_a_pivot():
+    """A device update returns with its kernels queued; the next update that
+    reads it and a pivot plan that gathers it, made at once (no download in
+    between), wait for it: the same results as the host-fed chain."""
+    rng = np.random.default_rng(13)
+    site = models.heisenberg_u1_mpo()
+    b0, b1, b2 = synth.heisenberg_bond(4, 12), synth.heisenberg_bond(5, 12), synth.heisenberg_bond(6, 12)
+    A1 = random_site_tensor(site, b0, b1, rng)
+    A2 = random_site_tensor(site, b1, b2, rng)
+    L0 = random_env(site, b0, rng, "L")
+    print("case queued chain", flush=True)
+    L1_host = env.contract_L(site, L0, A1, A1)
+    L2_host = env.contract_L(site, L1_host, A2, A2)
+    L1_dev = env.contract_L(site, env.DeviceEnv.from_host(L0, site.sym.nq), A1, A1, device=True)
+    L2_dev = env.contract_L(site, L1_dev, A2, A2, device=True)
+    _same_env(L2_dev.to_host(), L2_host)
+    if not any(blk is not None for s in L1_host for blk in s[3]):
+        pytest.skip("empty environment")
+    pd = synth.make_pivot(site, site, b1, synth.heisenberg_bond(7, 12), rng)
+    pd.L = L1_host
+    v = synth.random_vector(pd, 5)
+    try:
+        H = PivotMatrix2(pd)
+    except _lib.VmpsStatusError:
+        pytest.skip("contracted environment does not match the synthetic pivot's mid sectors")
+    ref = HxV(H, v)
+    H.close()
+    dd = copy.copy(pd)
+    dd.L = env.contract_L(site, env.DeviceEnv.from_host(L0, site.sym.nq), A1, A1, device=True)  # still queued
+    Hd = PivotMatrix2(dd)
+    assert np.array_equal(HxV(Hd, v), ref)
+    Hd.close()
+
+
 def test_errors():
     lib = _lib.load()
     pd = pivots.small_heisenberg()

@@ -117,6 +117,7 @@ SIGNATURES = {
     "vmps_block_eigh_split": (c_int, [c_int, P_i64, P_i64, P_dbl, c_int, P_dbl, P_dbl, P_dbl]),
     "vmps_plan_set_tiles": (c_int, [c_void, ctypes.POINTER(TileConfig)]),
     "vmps_plan_autotune": (c_int, [c_void, c_int, ctypes.POINTER(TileConfig)]),
+    "vmps_autotune_clear_cache": (c_int, []),
     "vmps_env_create": (c_int, [c_int, c_int, P_int, c_int, P_int, c_int, P_int, ctypes.POINTER(c_void)]),
     "vmps_env_destroy": (None, [c_void]),
     "vmps_env_add_old": (c_int, [c_void, P_int, P_int, P_int, c_int, P_i64, P_i64, P_dbl]),

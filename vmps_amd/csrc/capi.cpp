@@ -784,6 +784,13 @@ vmps_status vmps_plan_autotune(vmps_plan* plan, int iters, vmps_tile_config* out
   VMPS_API_END
 }
 
+vmps_status vmps_autotune_clear_cache(void) {
+  VMPS_API_BEGIN
+  std::lock_guard<std::mutex> lock(g_tune_mutex);
+  g_tune_cache.clear();
+  VMPS_API_END
+}
+
 vmps_status vmps_vec_alloc(vmps_plan* plan, double** d_v) {
   VMPS_API_BEGIN
   require(plan && d_v, VMPS_ERR_ARG, "NULL argument");

@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cstring>
 #include <memory>
+#include <mutex>
 
 #include "denv.hpp"
 #include "engine.hpp"
@@ -109,6 +110,46 @@ void move_to_device(vmps_denv* d) {
   d->T.dev_size = n;
 }
 
+namespace {
+// An environment update whose kernels were queued and not waited for: the
+// plan that runs them and the device blocks they read (the old environment)
+// and write (the result) stay alive until wait() has seen them finish.
+class PendingUpdate final : public vmps::DevReady {
+ public:
+  PendingUpdate(std::unique_ptr<vmps::DevPlan> plan, hipEvent_t done, int device,
+                std::shared_ptr<const double> reads, std::shared_ptr<const double> writes)
+      : plan_(std::move(plan)), done_(done), device_(device), reads_(std::move(reads)), writes_(std::move(writes)) {}
+  PendingUpdate(const PendingUpdate&) = delete;
+  PendingUpdate& operator=(const PendingUpdate&) = delete;
+  ~PendingUpdate() override {
+    finish(false);
+    if (done_) (void)hipEventDestroy(done_);
+  }
+  void wait() override { finish(true); }
+
+ private:
+  void finish(bool check) {
+    std::lock_guard<std::mutex> lock(m_);
+    if (!plan_) return;
+    int cur = -1;
+    const bool other = hipGetDevice(&cur) == hipSuccess && cur != device_;
+    if (other) (void)hipSetDevice(device_);
+    const hipError_t st = hipEventSynchronize(done_);
+    plan_.reset();  // its stream is idle now
+    reads_.reset();
+    writes_.reset();
+    if (other) (void)hipSetDevice(cur);
+    if (check && st != hipSuccess)
+      throw VmpsError(VMPS_ERR_HIP, std::string("environment update: ") + hipGetErrorString(st));
+  }
+  std::mutex m_;
+  std::unique_ptr<vmps::DevPlan> plan_;
+  hipEvent_t done_ = nullptr;
+  int device_ = -1;
+  std::shared_ptr<const double> reads_, writes_;
+};
+}  // namespace
+
 void contract(vmps_env* e, vmps_precision prec, vmps_denv* dres) {
   require(e != nullptr, VMPS_ERR_ARG, "NULL env");
   require(prec == VMPS_PREC_FP64 || prec == VMPS_PREC_FP32 || prec == VMPS_PREC_BF16X3, VMPS_ERR_ARG,
@@ -126,14 +167,23 @@ void contract(vmps_env* e, vmps_precision prec, vmps_denv* dres) {
   const int64_t osize = e->S.osize;
   double* dout = nullptr;  // device result handed to dres
   if (!e->S.terms.empty() && osize > 0) {
-    vmps::DevPlan D;
+    auto plan = std::make_unique<vmps::DevPlan>();
+    vmps::DevPlan& D = *plan;
     D.create_triples(std::move(e->S), prec);  // the plan takes the packed operands; e->S keeps the layout
     e->stats = D.H.stats;
     const size_t ib = (size_t)D.H.ysize * sizeof(double), ob = (size_t)D.H.osize * sizeof(double);
-    if (ib)
+    if (ib) {
+      e->P.old.wait_device();  // an old environment still being written by its own update
       VMPS_HIP_CHECK(hipMemcpyAsync(D.vin.p, e->P.old.dev ? e->P.old.dev : e->P.old.data.data(), ib,
                                     e->P.old.dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, D.stream));
+    }
     D.hxv(D.vin.p, D.vout.p, D.stream);
+    // A device result from device (or no) input returns with its kernels
+    // queued: the host goes on (the next pivot's plan) while they run, and
+    // every reader of the result waits on their event first.  Host input
+    // (a copy from pageable memory that the caller may free) and host
+    // results wait here.
+    const bool queued = dres && (!ib || e->P.old.dev);
     if (dres) {
       dout = D.vout.p;  // the plan gives up its output buffer to dres
       D.vout.p = nullptr;
@@ -143,7 +193,19 @@ void contract(vmps_env* e, vmps_precision prec, vmps_denv* dres) {
     } else {
       VMPS_HIP_CHECK(hipMemcpyAsync(e->result.data(), D.vout.p, ob, hipMemcpyDeviceToHost, D.stream));
     }
-    VMPS_HIP_CHECK(hipStreamSynchronize(D.stream));
+    if (queued) {
+      hipEvent_t done = nullptr;
+      VMPS_HIP_CHECK(hipEventCreateWithFlags(&done, hipEventDisableTiming));
+      const hipError_t st = hipEventRecord(done, D.stream);
+      if (st != hipSuccess) {
+        (void)hipEventDestroy(done);
+        VMPS_HIP_CHECK(st);
+      }
+      dres->T.dev_ready = std::make_shared<PendingUpdate>(std::move(plan), done, dres->device, e->P.old.dev_hold,
+                                                          dres->T.dev_hold);
+    } else {
+      VMPS_HIP_CHECK(hipStreamSynchronize(D.stream));
+    }
   } else {
     e->stats = vmps::Stats();
   }
@@ -422,6 +484,7 @@ vmps_status vmps_denv_get(const vmps_denv* d, int* qin, int* qout, int* qmid, in
     VMPS_HIP_CHECK(hipGetDevice(&dev));
     require(dev == d->device, VMPS_ERR_ARG, "the environment lives on another device");
     host.resize((size_t)d->T.dev_size);
+    d->T.wait_device();
     VMPS_HIP_CHECK(hipMemcpy(host.data(), d->T.dev, host.size() * sizeof(double), hipMemcpyDeviceToHost));
   }
   tripod_get(d->T, d->nq, host.data(), qin, qout, qmid, adim, rows, cols, data);

@@ -654,6 +654,9 @@ void DevPlan::upload(vmps_precision precision) {
   stream = acquire_stream();
   VMPS_HIP_CHECK(hipEventCreate(&ev0));
   VMPS_HIP_CHECK(hipEventCreate(&ev1));
+  // device-resident environments whose update may still be running
+  for (auto& w : H.env_waits) w->wait();
+  H.env_waits.clear();
   upload_env(L, H.Ldata, H.Lgather, H.L_size, stream);
   upload_env(R, H.Rdata, H.Rgather, H.R_size, stream);
   T.alloc(H.T_size);

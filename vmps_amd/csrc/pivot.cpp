@@ -881,6 +881,8 @@ void triples_from_pivot(const Pivot& P, TripleSet& S) {
       if (E.dev) {
         ++ndev;
         if (b.size()) dev.push_back({E.dev + b.off, o, b.size()});
+        if (E.dev_ready && std::find(S.env_waits.begin(), S.env_waits.end(), E.dev_ready) == S.env_waits.end())
+          S.env_waits.push_back(E.dev_ready);
       }
       o += b.size();
     }
@@ -923,6 +925,7 @@ void build_plan(const TripleSet& S, HostPlan& H) {
   H.Rdata = S.Zdata;
   H.Lgather = S.Xgather;
   H.Rgather = S.Zgather;
+  H.env_waits = S.env_waits;
   H.L_size = S.Xgather.empty() ? (int64_t)S.Xdata.size() : S.Xsize;
   H.R_size = S.Zgather.empty() ? (int64_t)S.Zdata.size() : S.Zsize;
   build_plan_tables(S, H);
@@ -937,6 +940,7 @@ void build_plan(TripleSet&& S, HostPlan& H) {
   H.Rdata = std::move(S.Zdata);
   H.Lgather = std::move(S.Xgather);
   H.Rgather = std::move(S.Zgather);
+  H.env_waits = std::move(S.env_waits);
   build_plan_tables(S, H);
 }
 

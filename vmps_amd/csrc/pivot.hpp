@@ -47,6 +47,15 @@ struct Key3Hash {
 };
 
 // Environment tensor (Multipede<3,...> with block[q][a][0]).
+// Completion of queued device work that writes a device-resident
+// environment (an environment update returned before its kernels finished):
+// wait() returns once that work is done and releases what it held.  Safe to
+// call from several threads and more than once.
+struct DevReady {
+  virtual ~DevReady() = default;
+  virtual void wait() = 0;
+};
+
 struct Tripod {
   std::vector<std::array<Qn, 3>> index;          // {in, out, mid}
   std::vector<std::vector<DenseBlock>> block;    // [q][a]
@@ -57,8 +66,15 @@ struct Tripod {
   // Tripod (in a vmps_denv, a pivot or a contraction) exists.
   const double* dev = nullptr;
   std::shared_ptr<const double> dev_hold;
+  // set while the blocks at `dev` may still be being written: every reader of
+  // `dev` calls wait_device() first (declared after dev_hold, so that a last
+  // owner waits before it frees the blocks)
+  std::shared_ptr<DevReady> dev_ready;
   int64_t dev_size = 0;
   int dev_device = -1;
+  void wait_device() const {
+    if (dev_ready) dev_ready->wait();
+  }
   std::unordered_map<Key3, size_t, Key3Hash> dict;
   size_t dim() const { return index.size(); }
   int64_t size() const { return dev ? dev_size : (int64_t)data.size(); }  // packed elements
@@ -323,6 +339,7 @@ struct TripleSet {
   // environments resident on the device: the X / Z blocks are gathered there
   // (Xdata / Zdata stay empty); Xsize / Zsize count the packed elements
   std::vector<Gather> Xgather, Zgather;
+  std::vector<std::shared_ptr<DevReady>> env_waits;  // pending writes of those environments
   int64_t Xsize = 0, Zsize = 0;
   std::vector<BlockDim> Xb, Zb;
   std::vector<BlockDim> Yb, Ob;  // blocks of the input and output vectors
@@ -335,6 +352,7 @@ struct HostPlan {
   // packed operands actually referenced by terms
   std::vector<double> Ldata, Rdata;  // X and Z blocks (empty when gathered on the device)
   std::vector<Gather> Lgather, Rgather;  // device gathers of device-resident environments
+  std::vector<std::shared_ptr<DevReady>> env_waits;  // waited on before the gathers run
   int64_t L_size = 0, R_size = 0;    // packed L / R elements
   std::vector<BlockDim> Lblk, Rblk, Vblk;  // blocks of Ldata, Rdata and the input vector
   int64_t vsize = 0;                 // output vector size (== input size for HxV plans)

@@ -791,13 +791,19 @@ class GpuBackend:
         from .pivot import PivotMatrix2
         t0 = time.perf_counter()
         H = PivotMatrix2(pd, precision="fp64")
-        flops = H.stats()["flops_alg"]
+        st = H.stats()
+        flops = st["flops_alg"]
         tune = self.tune_min_flops is not None and flops >= self.tune_min_flops
-        if tune:
-            H.autotune(iters=1)
         prec = self.precision
         if prec != "fp64" and flops < self.mixed_min_flops:
             prec = "fp64"
+        # A mixed solve applies the FP64 operator twice (the refinement's
+        # residuals): on pivots with blocks of 256 and more, whose FP64 HxV
+        # takes the big tiles in any case, its tuning (six FP64 HxV) would
+        # cost more than it could save, and the split-BF16 tuning below
+        # chooses the shared explicit-U setting again anyway.
+        if tune and (prec == "fp64" or st["max_block_dim"] < 256):
+            H.autotune(iters=1)
         if prec != "fp64":
             H.set_precision(prec)
             if tune:

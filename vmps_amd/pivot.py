@@ -179,6 +179,12 @@ class DeviceVector:
             pass
 
 
+def clear_tile_cache():
+    """Forget the autotuner's cached tile choices (process-wide), so that the
+    next plan of each shape is timed again."""
+    _lib.check(_lib.load().vmps_autotune_clear_cache())
+
+
 def HxV(H, vin):
     vin = np.ascontiguousarray(vin, dtype=np.float64)
     if vin.size != H.vsize:
