@@ -348,6 +348,8 @@ def dmrg_c60_bench(args):
                 HxV(H, v)
             torch.cuda.synchronize()
             rec[prec + "_ms_per_hxv_incl_copies"] = (time.perf_counter() - t0) / args.hxv_reps * 1e3
+            # device time of the stages (HIP events on the plan's stream): A, B, C, whole HxV
+            rec[prec + "_stage_ms"] = [round(x, 4) for x in H.time_stages(warmup=2, iters=args.hxv_reps)]
             rec["flops_alg"] = H.stats()["flops_alg"]
             H.close()
         return rec
