@@ -393,8 +393,8 @@ vmps_status vmps_denv_get(const vmps_denv* d, int* qin, int* qout, int* qmid, in
  * (vmps_env_result_get then reports VMPS_ERR_STATE; result_size and stats
  * still answer).  With the old environment on the device (or none), it
  * returns once the contraction's kernels are queued: every reader of the new
- * vmps_denv (a plan that gathers it, a contraction that takes it as its old
- * environment, vmps_denv_get, vmps_denv_destroy) waits for them first, so the
+ * environment -- a plan that gathers it, a contraction that takes it as its
+ * old environment, vmps_denv_get, vmps_denv_destroy -- waits for them first, so the
  * caller can build the next plan on the host meanwhile.  A contraction from
  * a host environment returns when it has finished. */
 vmps_status vmps_env_contract_device(vmps_env* e, vmps_precision prec, vmps_denv** out);
@@ -476,7 +476,9 @@ typedef struct {
   int iterations_refine;    /* iterations of the FP64 refinement */
   int hxv_calls;
   int converged;
-  double energy_mixed;      /* Rayleigh quotient of the mixed solve with its own operator */
+  double energy_mixed;      /* Rayleigh quotient of the mixed solve's vector: with the FP64 operator when
+                               refine_method is VMPS_REFINE_CORRECTION (the refinement forms it anyway),
+                               else with the mixed operator; residual_mixed likewise */
   double residual_mixed;
   double ms;                /* wall time of the call */
   int hxv_fp64;             /* HxV calls with the FP64 operator (all of hxv_calls for FP64 solves) */
