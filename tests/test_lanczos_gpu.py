@@ -51,7 +51,7 @@ def test_lanczos_mixed_refined_to_fp64(prec):
     H = PivotMatrix2(pd, precision=prec)
     v, r = lanczos(H, precision=prec, max_iter=min(n, 120), tol_eigval=1e-6, tol_state=1e-4)
     print(r, e0)
-    assert abs(r["energy_mixed"] - e0) <= 1e-4 * abs(e0)   # FP32 operator
+    assert abs(r["energy_mixed"] - e0) <= 1e-4 * abs(e0)   # the mixed solve's vector, before the corrections
     assert abs(r["energy"] - e0) <= 1e-9 * max(1.0, abs(e0))  # after FP64 refinement
 
 

@@ -984,6 +984,10 @@ static vmps_status lanczos_impl(vmps_plan* plan, double* d_v, bool random_start,
     lo.tol_state = std::max(lo.tol_state, state_floor);
     lo.tol_eigval = std::max(lo.tol_eigval, state_floor * state_floor);
   }
+  // a correction refinement forms the mixed vector's Rayleigh quotient and
+  // residual with the FP64 operator anyway: no extra mixed HxV for them
+  const bool corrections = prec != VMPS_PREC_FP64 && opts->refine_fp64 && opts->refine_method == VMPS_REFINE_CORRECTION;
+  lo.final_rq = !corrections;
   vmps::LanczosResult r = vmps::lanczos_solve(D, d_v, lo, prec);
   res->iterations = r.iterations;
   res->hxv_calls = r.hxv_calls;
@@ -1000,6 +1004,8 @@ static vmps_status lanczos_impl(vmps_plan* plan, double* d_v, bool random_start,
       ro.tol_eigval = opts->refine_tol_eigval;
       ro.tol_state = opts->refine_tol_state;
       vmps::RefineResult f = vmps::refine_eigenpair(D, d_v, ro, prec);
+      res->energy_mixed = f.energy0;
+      res->residual_mixed = f.residual0;
       res->refine_steps = f.outer;
       res->refine_cg = f.cg_iterations;
       res->hxv_calls += f.cg_iterations + f.hxv_fp64;

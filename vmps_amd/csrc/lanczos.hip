@@ -717,7 +717,7 @@ LanczosResult lanczos_solve(DevPlan& D, double* d_v, const LanczosOpts& o, vmps_
   VMPS_HIP_CHECK(hipGetLastError());
 
   std::vector<double> alpha, beta, y;
-  double E = 0.0, Eprev = 0.0;
+  double E = 0.0, Eprev = 0.0, rest_last = 0.0;
   int k = 0;
   bool converged = false;
   // The graph runs kIterBatch iterations per launch and the host reads their
@@ -778,6 +778,7 @@ LanczosResult lanczos_solve(DevPlan& D, double* d_v, const LanczosOpts& o, vmps_
       tridiag_lowest(alpha, beta, k + 1, E, y);
       // residual estimate of the Ritz pair: beta_k |y_k|
       const double rest = b_k * std::fabs(y[k]);
+      rest_last = rest;
       res.iterations = k + 1;
       ++k;
       // convergence after min_iter iterations; a breakdown (the Krylov space is
@@ -801,22 +802,27 @@ LanczosResult lanczos_solve(DevPlan& D, double* d_v, const LanczosOpts& o, vmps_
   D.dot_async(K.w.p, K.w.p, K.scal.p, s);
   hipLaunchKernelGGL(k_normalize_into, dim3(vgrid(n)), dim3(256), 0, s, K.scal.p, K.w.p, d_v, n);
   VMPS_HIP_CHECK(hipGetLastError());
-  // Rayleigh quotient and residual with the requested precision's operator
-  D.hxv(d_v, K.w.p, s);
-  ++res.hxv_calls;
-  D.dot_async(d_v, K.w.p, K.scal.p, s);
-  VMPS_HIP_CHECK(hipMemcpyAsync(K.host3, K.scal.p, sizeof(double), hipMemcpyDeviceToHost, s));
-  VMPS_HIP_CHECK(hipStreamSynchronize(s));
-  const double rq = K.host3[0];
-  dev_axpby(-rq, d_v, 1.0, K.w.p, n, s);
-  D.dot_async(K.w.p, K.w.p, K.scal.p, s);
-  VMPS_HIP_CHECK(hipMemcpyAsync(K.host3, K.scal.p, sizeof(double), hipMemcpyDeviceToHost, s));
-  VMPS_HIP_CHECK(hipStreamSynchronize(s));
-  const double r2 = K.host3[0];
-  res.energy = rq;
   res.ritz = E;
-  res.residual = std::sqrt(std::max(0.0, r2));
   res.converged = converged;
+  double rq = E;
+  if (o.final_rq || pro) {
+    // Rayleigh quotient and residual with the requested precision's operator
+    D.hxv(d_v, K.w.p, s);
+    ++res.hxv_calls;
+    D.dot_async(d_v, K.w.p, K.scal.p, s);
+    VMPS_HIP_CHECK(hipMemcpyAsync(K.host3, K.scal.p, sizeof(double), hipMemcpyDeviceToHost, s));
+    VMPS_HIP_CHECK(hipStreamSynchronize(s));
+    rq = K.host3[0];
+    dev_axpby(-rq, d_v, 1.0, K.w.p, n, s);
+    D.dot_async(K.w.p, K.w.p, K.scal.p, s);
+    VMPS_HIP_CHECK(hipMemcpyAsync(K.host3, K.scal.p, sizeof(double), hipMemcpyDeviceToHost, s));
+    VMPS_HIP_CHECK(hipStreamSynchronize(s));
+    res.energy = rq;
+    res.residual = std::sqrt(std::max(0.0, K.host3[0]));
+  } else {
+    res.energy = E;
+    res.residual = rest_last;
+  }
   D.prec = saved;
   if (pro) {
     // With a semi-orthogonal basis the Ritz value equals the Rayleigh
@@ -893,6 +899,8 @@ RefineResult refine_eigenpair(DevPlan& D, double* d_v, const RefineOpts& o, vmps
     copy(w, g);
     axpby(E, psi, -1.0, g);  // g = E psi - H psi = -r
     double rho = std::sqrt(std::max(0.0, dot(g, g)));
+    res.energy0 = E;
+    res.residual0 = rho;
     for (;;) {
       if (rho <= o.tol_state && rho * rho <= o.tol_eigval) {
         res.converged = true;

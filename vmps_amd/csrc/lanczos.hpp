@@ -12,6 +12,10 @@ struct LanczosOpts {
   double tol_eigval = 1e-8;  // DmrgTypedefs.h:371 default
   double tol_state = 1e-5;   // DmrgTypedefs.h:372 default
   bool graph = true;         // replay one captured HIP graph per iteration
+  // one more HxV for the returned vector's Rayleigh quotient and residual;
+  // without it (a refinement that forms both with the FP64 operator
+  // follows) energy / residual are the Ritz value and the Lanczos estimate
+  bool final_rq = true;
 };
 
 struct LanczosResult {
@@ -36,6 +40,8 @@ struct RefineOpts {
 struct RefineResult {
   double energy = 0.0;    // <psi|H|psi>, FP64 operator
   double residual = 0.0;  // ||H psi - E psi||, FP64 operator
+  double energy0 = 0.0;   // the same two for the start vector
+  double residual0 = 0.0;
   int outer = 0;          // correction steps taken
   int cg_iterations = 0;  // mixed-precision HxV of the corrections
   int hxv_fp64 = 0;
