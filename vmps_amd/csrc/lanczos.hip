@@ -909,10 +909,10 @@ RefineResult refine_eigenpair(DevPlan& D, double* d_v, const RefineOpts& o, vmps
       if (res.outer >= o.max_outer || !std::isfinite(rho)) break;
       ++res.outer;
       // CG on the projected correction equation, from t = 0 (residual g = -r);
-      // it stops once the correction is a few times finer than the step needs,
+      // it stops once the correction reaches half the residual the step needs,
       // down to 1e-4 of the residual (split-BF16x3 Lanczos vectors, residual
       // ~3e-4, then reach a 1e-7 residual in one step instead of two)
-      const double cg_tol = std::min(0.1, std::max(o.cg_tol_min, 0.3 * o.tol_state / rho));
+      const double cg_tol = std::min(0.1, std::max(o.cg_tol_min, 0.5 * o.tol_state / rho));
       VMPS_HIP_CHECK(hipMemsetAsync(t, 0, sizeof(double) * n, s));
       copy(g, pv);
       double gg = rho * rho;
