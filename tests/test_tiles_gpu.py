@@ -8,7 +8,7 @@ import pytest
 
 import oracle
 from vmps_amd import synth
-from vmps_amd.pivot import HxV, PivotMatrix2
+from vmps_amd.pivot import HxV, PivotMatrix2, clear_tile_cache
 
 sys.path.insert(0, __file__.rsplit("/", 1)[0])
 import pivots  # noqa: E402
@@ -106,6 +106,12 @@ def test_autotune_picks_candidate_and_caches(case, precision, tol):
     assert (cfg2["wave_edge"], cfg2["big_min"], cfg2["explicit_u"]) == (cfg["wave_edge"], cfg["big_min"],
                                                                       cfg["explicit_u"])
     assert _rel(HxV(H2, v), ref) <= tol
+    # after the cache is cleared, the next plan of that shape is timed again
+    clear_tile_cache()
+    H3 = PivotMatrix2(pd, precision=precision)
+    cfg3 = H3.autotune(iters=3)
+    assert cfg3["from_cache"] == 0 and cfg3["n_tried"] >= 2
+    assert _rel(HxV(H3, v), ref) <= tol
 
 
 def test_set_tiles_rejects_bad_edge(case):
